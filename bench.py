@@ -1,0 +1,145 @@
+"""Benchmark of the KMM channel time step on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): turbulent
+channel at Re_tau = 180, 128 x 129 x 128, Lx = 4 pi, Lz = 4 pi/3, re = 2800
+(constant pressure gradient), passive scalar at Pr = 0.71 with a uniform
+source, Goldstein feedback on one spanwise rib (imatrix=1, y indices 1..19 of
+the y=+1 wall, 3 x cells at x = 0.4 Lx, all z; gain = 20, ugain = 100, i.e.
+alpha = -1e4, beta = -1e2 at dt = 0.002), seeded perturbed laminar start,
+plain FP64.
+
+A step is one time step of the whole channel.  value = grid points
+(nx * (ny+1) * nz) x steps / wall seconds / 1e6 over exactly --steps steps
+with the state resident in HBM, bracketed by device synchronisation.
+
+roofline: the dominant kernel's algorithmic bytes per launch divided by its
+average launch time, measured with HIP events on the library's stream over
+the timed steps.  cpu_baseline: the numpy restatement of the reference step
+timed on this host for a bounded number of steps of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mgridpt-steps/s at Re_tau 180 (128x129x128): FP64 vs mixed+learned; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+NX, NY, NZ = 128, 128, 128
+XL, ZL = 4.0 * math.pi, 4.0 * math.pi / 3.0
+RE, RTAU, DT = 2800.0, 180.0, 0.002
+PR, SRC = 0.71, 1.0
+GAIN, UGAIN = 20.0, 100.0
+SEED = 20261202 + 1
+
+
+def rib(nx, ny, nz):
+    from polymer_dns_amd.restart import rib_geometry
+    nyp = ny + 1
+    y = np.cos(np.arange(nyp) * math.pi / ny)
+    ycells = int(np.sum(y >= 0.9))             # height 0.1 from the y=+1 wall
+    mx = 3 * nx // 2
+    width = max(1, int(round(0.2 / (XL / mx))))  # streamwise width 0.2
+    return rib_geometry(nx, ny, nz, y_cells=ycells, x_center=int(round(0.4 * mx)), x_width=width)
+
+
+def workload_state():
+    from polymer_dns_amd import ic
+    st = ic.seeded_state(NX, NY, NZ, XL, ZL, RE, RTAU, SEED, amp=0.05, kmax=8)
+    st["scalar"] = np.zeros((NX // 2, NZ, NY + 1), dtype=np.complex128)
+    return st
+
+
+def kernel_bytes(G):
+    """Algorithmic bytes per launch (FP64, NS + scalar): field transfers x 8 G
+    bytes (one spectral field = nyp*nz*nxh complex128 = 8 G bytes); the
+    z-padded intermediates are 1.5 fields each."""
+    S = 8.0 * G
+    return {
+        "k_pencil": 25 * S,      # read v, omy, fnm1, gnm1, th, scnm1 + 4 NL; write 6 state + 9 transfer
+        "k_zinv": (9 + 13.5) * S,
+        "k_xplane": (13.5 + 6) * S,
+        "k_zfwd": (6 + 4) * S,
+        "k_ibdrag": 0.0,
+    }
+
+
+def cpu_baseline(nsteps=2):
+    from oracle import kmm_oracle as O
+    p = O.Params(nx=NX, ny=NY, nz=NZ, xl=XL, zl=ZL, re=RE, dt=DT, R_tau=RTAU, gain=GAIN, ugain=UGAIN,
+                 scalar=True, diff=PR, scl_flag=6, deltaT=SRC)
+    st = workload_state()
+    sol = O.Solver(p, imatrix=rib(NX, NY, NZ))
+    sol.startup(st)
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        sol.step(st)
+    t = time.perf_counter() - t0
+    G = NX * (NY + 1) * NZ
+    return dict(value=G * nsteps / t / 1e6, unit="Mgridpt-steps/s", cores=1, kind="port",
+                sample=f"{nsteps} FP64 steps of the same workload (numpy restatement, single thread)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs one channel on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    from polymer_dns_amd import kmm
+    cfg = kmm.make_config(NX, NY, NZ, xl=XL, zl=ZL, re=RE, dt=DT, r_tau=RTAU, gain=GAIN, ugain=UGAIN,
+                          scalar=1, diff=PR, scl_flag=6, deltaT=SRC)
+    G = NX * (NY + 1) * NZ
+    ch = kmm.Channel(cfg)
+    ch.set_geometry(rib(NX, NY, NZ))
+    st = workload_state()
+    for k in ("v", "omy", "u0", "w0", "scalar"):
+        ch.put(k, st[k])
+    ch.startup()
+    ch.step(args.warmup)
+    ch.set_timing(True)
+    t0 = time.perf_counter()
+    ch.step(args.steps)            # returns after a device synchronisation
+    t = time.perf_counter() - t0
+    ktimes, nsamp = ch.kernel_times()
+    cfl, it = ch.last_cfl()
+    ms = 1000.0 * t / args.steps
+    value = G * args.steps / t / 1e6
+    kb = kernel_bytes(G)
+    dom = max(ktimes, key=lambda k: ktimes[k])
+    achieved = kb[dom] / (ktimes[dom] * 1e-3) / 1e9
+    step_bytes = 320.0 * G   # 40 FP64 field transfers per grid point (NS + scalar)
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mgridpt-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "Re_tau 180 channel 128x129x128, FP64, scalar Pr 0.71 + uniform source, "
+                               "Goldstein rib IB", "grid": [NX, NY + 1, NZ], "dt": DT, "mode": "fp64"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel_ms": {k: round(v, 4) for k, v in ktimes.items()}},
+        "step_roofline": {"bytes_per_gridpt": 320, "achieved_GBs": round(step_bytes / (ms * 1e-3) / 1e9, 1),
+                          "frac": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "cfl_last": cfl, "it": it,
+    }
+    ch.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,145 @@
+/*
+ * kmm.h -- C ABI of the MI355X KMM channel-step library (libkmm.so).
+ *
+ * The library replaces the time-step path of the Polymer_DNS reference
+ * (FFTW_DNS/code/src/dns): the body of the time loop of ``program dns``
+ * (main_dns.f90:198-811) together with the solver/derivative/helper routines
+ * it calls and ``vcw3dp`` (solvers.f90:1726-3538).  Fortran hosts bind these
+ * entry points through iso_c_binding; every argument is a plain pointer or
+ * scalar.  Arrays passed in "reference layout" are Fortran column-major:
+ *   spectral field  complex(8)  (nyp, nz, nxh)   -- y fastest
+ *   mean profile    real(8)     (nyp)
+ *   IB integrals    real(8)     (nyp, mz, mx)
+ *   geometry        integer(4)  (nyp, mz, mx)
+ * Complex numbers are interleaved (re, im) doubles.
+ *
+ * Every function returns a kmm_status (0 on success).  The library keeps
+ * the state on the GPU between calls; one context per host thread.
+ */
+#ifndef KMM_H
+#define KMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum kmm_status {
+  KMM_OK = 0,
+  KMM_E_CFL = 1,      /* CFL > 1 (solvers.f90:3503-3514 "CFL failure") */
+  KMM_E_PARTICLE = 2, /* particle left the domain (part_track.f90:498-504) */
+  KMM_E_HIP = 3,      /* HIP runtime error */
+  KMM_E_CONFIG = 4,   /* unsupported or inconsistent configuration */
+  KMM_E_IO = 5,       /* file missing, short or malformed */
+  KMM_E_ARG = 6       /* bad argument */
+} kmm_status;
+
+/* Runtime equivalent of dns.config (main_dns.f90:934-1068) plus the
+ * compile-time sizes of grid_size.f90:2. */
+typedef struct kmm_config {
+  int nx, ny, nz;         /* NX1, NY1, NZ1 */
+  int nsteps, iprnfrq;    /* NSTEPS, IPRNFRQ */
+  double dt;              /* DT */
+  double gain, ugain;     /* Goldstein feedback gains */
+  double theta;           /* THETA (CN implicitness, <= 0.99) */
+  double yl, zl, xl;      /* YL, ZL, XL */
+  double re;              /* RE = 1/nu */
+  double uinf;            /* Uinf */
+  int kwall;              /* KWALL */
+  int perturbtime;        /* perturbation staying time (imatrix type 4) */
+  double r_tau;           /* R_tau (sets dPdx, init_flow.f90:117) */
+  int irstrt, crstrt;     /* restart flags */
+  int flow_select;        /* 1 channel, 2 2-D Couette, 3 3-D Couette */
+  int scalar;             /* 1 = SCALAR build (passive scalar) */
+  int scl_flag;           /* 0,1,3: no source; 6: uniform source deltaT */
+  double diff;            /* DIFF (Prandtl/Schmidt number) */
+  double deltaT;          /* scalar source rate for scl_flag 6 */
+  int src_start, src_stop;
+  int cadence;
+} kmm_config;
+
+typedef struct kmm_ctx kmm_ctx;
+
+/* field identifiers for kmm_put_field / kmm_get_field */
+enum {
+  KMM_V = 0, KMM_FN, KMM_FNM1, KMM_OMY, KMM_GN, KMM_GNM1,    /* restart record 1 */
+  KMM_U0, KMM_H1N, KMM_H1NM1, KMM_W0, KMM_H3N, KMM_H3NM1,   /* mean profiles */
+  KMM_FXINTG, KMM_FYINTG, KMM_FZINTG,                       /* restart record 2 */
+  KMM_SCALAR, KMM_SCN, KMM_SCNM1,                           /* SCALAR build */
+  KMM_NFIELDS
+};
+
+const char* kmm_status_string(int status);
+
+/* Positional dns.config reader with the reference's line semantics
+ * (main_dns.f90:934-1068; replaces setstuff's read block). */
+int kmm_config_read(const char* path, kmm_config* cfg);
+
+/* Allocate device state for cfg on HIP device `device` (setstuff, gfcn,
+ * denom and forcepre: main_dns.f90:148-163). */
+int kmm_create(const kmm_config* cfg, int device, kmm_ctx** ctx);
+int kmm_destroy(kmm_ctx* ctx);
+
+/* imatrix cell types (main_dns.f90:1120-1122, read as (nyp,mz,mx));
+ * types 0, 1 and 4 are supported. */
+int kmm_set_geometry(kmm_ctx* ctx, const int32_t* imatrix);
+
+int kmm_put_field(kmm_ctx* ctx, int field, const double* host);
+int kmm_get_field(kmm_ctx* ctx, int field, double* host);
+
+/* New-run start-up of ``initial`` (main_dns.f90:1923-2229) from the current
+ * v, omega_y, u0, w0 (and scalar): nonlinear terms at it=0 and the AB2
+ * history set equal to them. */
+int kmm_startup(kmm_ctx* ctx);
+
+/* Advance nsteps time steps (main_dns.f90:198-811).  Returns KMM_E_CFL with
+ * the failing step in kmm_last_cfl if the CFL number exceeds 1. */
+int kmm_step(kmm_ctx* ctx, int nsteps);
+
+/* Largest CFL number of the last step run and the step counter it. */
+int kmm_last_cfl(kmm_ctx* ctx, double* cflmax, int* it);
+
+/* outputs/last-restart (main_dns.f90:776-781) / setup/restart (1744-1747) */
+int kmm_write_restart(kmm_ctx* ctx, const char* path);
+int kmm_read_restart(kmm_ctx* ctx, const char* path);
+/* scalar state as the first three records of c-restart
+ * (scalar / scn / scnm1; main_dns.f90:782-793, 1910-1916) */
+int kmm_write_crestart(kmm_ctx* ctx, const char* path);
+int kmm_read_crestart(kmm_ctx* ctx, const char* path);
+
+/* --- per-routine entry points, reference argument semantics ----------- */
+
+/* cderiv(f,df), derivatives.f90:258: f, df complex (nyp,nz,nxh) */
+int kmm_cderiv(kmm_ctx* ctx, const double* f, double* df);
+
+/* solve(u,x,g,dyde,bctop,bcbot,ib,a,wrkc), solvers.f90:174: one kx slab
+ * u complex (nyp,nz) in/out, bctop/bcbot complex (nz); a/wrkc are internal */
+int kmm_solve(kmm_ctx* ctx, double* u, double x, double g, double dyde, const double* bctop,
+              const double* bcbot, int ib);
+
+/* penta(u,x,g,dyde,ib,at,bt,gtop,ab,bb,gbot,a,wrkc), solvers.f90:680 */
+int kmm_penta(kmm_ctx* ctx, double* u, double x, double g, double dyde, int ib, double at, double bt,
+              const double* gtop, double ab, double bb, const double* gbot);
+
+/* vcw3dp (solvers.f90:1726), NS (+SCALAR) argument subset: inputs
+ * u,v,w,omx,omy,omz (+ sclx,scly,sclz) spectral, outputs gn=(u x om)_x,
+ * fn=(u x om)_y, omz_out=(u x om)_z (+ scn).  Updates the IB integrals. */
+int kmm_vcw3dp(kmm_ctx* ctx, const double* u, const double* v, const double* w, const double* omx,
+               const double* omy, const double* omz, const double* sclx, const double* scly,
+               const double* sclz, double* gn, double* fn, double* omz_out, double* scn, int it);
+
+/* time spent in the first step of the last kmm_step call (ms): pencil
+ * stage, plane stage */
+int kmm_stage_times(kmm_ctx* ctx, double* pencil_ms, double* plane_ms);
+
+/* Per-kernel HIP-event timing of every step of later kmm_step calls:
+ * average ms per launch of k_pencil, k_zinv, k_ibdrag, k_xplane, k_zfwd
+ * (ms[5]) over the last kmm_step call. */
+int kmm_set_timing(kmm_ctx* ctx, int on);
+int kmm_kernel_times(kmm_ctx* ctx, double* ms, int* nsamples);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KMM_H */

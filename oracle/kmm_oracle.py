@@ -1,0 +1,868 @@
+"""CPU restatement (numpy, FP64) of the Kim-Moin-Moser channel time step of the
+Polymer_DNS reference (FFTW_DNS/code/src/dns/*.f90).
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker: only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg may import it.  The
+product path (polymer_dns_amd) never imports, links or falls back to it.
+
+Array layout: every spectral field is a numpy array ``a[kx, kz, n]`` of
+complex128 with shape (nxh, nz, nyp).  In C order this is byte-identical to
+the reference's Fortran column-major ``complex(nyp,nz,nxh)``: y (Chebyshev
+index n, n=0 is y=+1) fastest, then kz in FFT order, then kx >= 0 (no x
+Nyquist).  Physical 3/2 fields are ``p[k, j, i]`` with shape (mx, mz, nyp),
+byte-identical to Fortran ``real(nyp,mz,mx)``.
+
+Every function cites the reference routine it restates.  The restatement
+follows the reference's equations and elimination order; floating-point
+operation order inside an expression is not reproduced (parity is to 1e-10
+relative, far above FP64 rounding).
+
+Pinning (no oracle/_ref build is possible: the reference needs FFTW3, which is
+absent from the image, and a stand-in library is not allowed):
+  * cderiv / c1derbw / c1dertw -- closed forms T_n' = n U_{n-1},
+    T_n'(+-1) = (+-1)^(n+1) n^2 (tests/test_oracle_pins.py)
+  * solve / penta / psolve1d -- dense tau-matrix LU built independently from
+    the Chebyshev recurrences, plus manufactured polynomial solutions and BC
+    residuals
+  * inverse/forward transforms -- scipy.fft.dct(type=1) == FFTW REDFT00 and
+    numpy.fft == FFTW unnormalised DFTs, single-mode cos/sin fields, round trip
+  * the full step -- laminar Poiseuille fixed point under constant pressure
+    gradient, divergence-free velocity, Hermitian symmetry, and the linear
+    Orr-Sommerfeld / Squire decay of a small perturbation
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+import scipy.fft as sfft
+
+
+# --------------------------------------------------------------------------- #
+#  Parameters (dns.config semantics, FFTW_DNS/code/src/dns/main_dns.f90:829-1171)
+# --------------------------------------------------------------------------- #
+@dataclasses.dataclass
+class Params:
+    nx: int
+    ny: int
+    nz: int
+    xl: float = 4.0 * math.pi
+    yl: float = 2.0
+    zl: float = 2.0 * math.pi
+    re: float = 2800.0
+    dt: float = 0.005
+    theta: float = 0.5
+    gain: float = 0.0
+    ugain: float = 0.0
+    R_tau: float = math.sqrt(3.0 * 2800.0)
+    flow_select: int = 1
+    perturbtime: int = 0
+    # scalar (SCALAR build); scl_flag 6 = uniform volumetric source of rate
+    # deltaT (extension, not a reference mode)
+    scalar: bool = False
+    diff: float = 1.0
+    scl_flag: int = 0
+    deltaT: float = 0.0
+
+    def __post_init__(self):
+        assert self.ny % 2 == 0 and self.nz % 2 == 0 and self.nx % 2 == 0
+        self.nyp = self.ny + 1
+        self.nyh = self.ny // 2
+        self.nyhp = self.nyh + 1
+        self.nyhm = self.nyh - 1
+        self.nxh = self.nx // 2
+        self.nzh = self.nz // 2
+        self.mx = 3 * self.nx // 2
+        self.mz = 3 * self.nz // 2
+        pi = 2.0 * math.acos(0.0)
+        self.alpha = 2.0 * pi / self.xl
+        self.beta = 2.0 * pi / self.zl
+        # main_dns.f90:1137-1148
+        self.wavx = np.arange(self.nxh, dtype=np.float64) * self.alpha
+        kz = np.arange(self.nz, dtype=np.float64)
+        kz[self.nzh:] -= self.nz
+        self.wavz = kz * self.beta
+        # main_dns.f90:1151-1155
+        self.c = np.ones(self.nyp)
+        self.c[0] = 2.0
+        self.c[-1] = 2.0
+        self.dyde = 2.0 / self.yl
+        # main_dns.f90:1112-1117 (index 0 is y=+yl/2)
+        n = np.arange(self.nyp, dtype=np.float64)
+        self.ycoord = np.cos(n * pi / self.ny) * (self.yl / 2.0)
+        self.ycoord[0] = self.yl / 2.0
+        seg = np.zeros(self.nyp)
+        seg[1:] = np.abs(self.ycoord[1:] - self.ycoord[:-1])
+        seg[0] = seg[-1]
+        self.seght = seg
+        # init_flow.f90:117 (constant pressure gradient)
+        self.dPdx = -2.0 / self.yl * (2.0 * self.R_tau / (self.yl * self.re)) ** 2
+        # setbcs main_dns.f90:1240-1265 (channel) and 1406-1409
+        self.atu, self.btu, self.gtu = 1.0, 0.0, 0.0
+        self.atw, self.btw, self.gtw = 1.0, 0.0, 0.0
+        self.abu, self.bbu, self.gbu = 1.0, 0.0, 0.0
+        self.abw, self.bbw, self.gbw = 1.0, 0.0, 0.0
+        self.nt = 1
+        self.nb = 1
+        self.p1b, self.p1t, self.p2b, self.p2t = 1.0, -1.0, -1.0, 0.5
+
+    @property
+    def kx2(self):
+        return (self.wavx ** 2)[:, None]
+
+    @property
+    def k2(self):
+        return self.wavx[:, None] ** 2 + self.wavz[None, :] ** 2
+
+
+# --------------------------------------------------------------------------- #
+#  Chebyshev derivatives (FFTW_DNS/code/src/dns/derivatives.f90)
+# --------------------------------------------------------------------------- #
+def cderiv(f, dyde):
+    """derivatives.f90:258-314 -- backward recurrence along the last axis."""
+    ny = f.shape[-1] - 1
+    df = np.zeros_like(f)
+    df[..., ny - 1] = 2.0 * ny * f[..., ny] * dyde
+    for n in range(ny - 2, -1, -1):
+        df[..., n] = df[..., n + 2] + 2.0 * (n + 1) * f[..., n + 1] * dyde
+    df[..., 0] = 0.5 * df[..., 0]
+    return df
+
+
+def c1derbw(f, dyde):
+    """derivatives.f90:370-427 -- d/dy at y=-1: sum (-1)^(n+1) n^2 a_n dyde."""
+    n = np.arange(f.shape[-1], dtype=np.float64)
+    w = n * n * np.where(n % 2 == 1, 1.0, -1.0) * dyde
+    return (f * w).sum(axis=-1)
+
+
+def c1dertw(f, dyde):
+    """derivatives.f90:541-595 -- d/dy at y=+1: sum n^2 a_n dyde."""
+    n = np.arange(f.shape[-1], dtype=np.float64)
+    return (f * (n * n * dyde)).sum(axis=-1)
+
+
+# --------------------------------------------------------------------------- #
+#  Tau systems (FFTW_DNS/code/src/dns/solvers.f90:174-1061)
+#  All arrays below are 1-based along the first axis (index 0 unused) so the
+#  elimination reads like the recurrences; trailing axes batch over (kx, kz).
+# --------------------------------------------------------------------------- #
+def _evnc(wn, dyde, nyh):
+    """solvers.f90:324-351 -- even-mode tridiagonal coefficients a(i,1..3), i=1..nyh."""
+    shp = (nyh + 2,) + wn.shape
+    a1 = np.zeros(shp)
+    a2 = np.zeros(shp)
+    a3 = np.zeros(shp)
+    for i in range(1, nyh + 1):
+        j = 2 * i
+        rj1 = 1.0 / (4 * j * (j - 1))
+        rj2 = 1.0 / (2 * (j * j - 1))
+        rj3 = 1.0 / (4 * j * (j + 1))
+        a1[i] = wn * rj1
+        a2[i] = -dyde * dyde - wn * rj2
+        a3[i] = wn * rj3
+    a1[1] = 2.0 * a1[1]
+    a3[nyh - 1] = 0.0
+    a2[nyh] = -dyde * dyde
+    a3[nyh] = 0.0
+    return a1, a2, a3
+
+
+def _oddc(wn, dyde, nyh):
+    """solvers.f90:355-383 -- odd-mode tridiagonal coefficients, i=1..nyh-1."""
+    nyhm = nyh - 1
+    shp = (nyh + 2,) + wn.shape
+    a1 = np.zeros(shp)
+    a2 = np.zeros(shp)
+    a3 = np.zeros(shp)
+    for i in range(1, nyhm + 1):
+        j = 2 * i + 1
+        rj1 = 1.0 / (4 * j * (j - 1))
+        rj2 = 1.0 / (2 * (j * j - 1))
+        rj3 = 1.0 / (4 * j * (j + 1))
+        a1[i] = wn * rj1
+        a2[i] = -dyde * dyde - wn * rj2
+        a3[i] = wn * rj3
+    a3[nyhm - 1] = 0.0
+    a2[nyhm] = -dyde * dyde
+    a3[nyhm] = 0.0
+    return a1, a2, a3
+
+
+def _evns(s, ny, c):
+    """solvers.f90:387-440 (rows 2..nyhp; row 1 is the BC row set by the caller).
+    s: Chebyshev coefficients with n on axis 0."""
+    nyh = ny // 2
+    nyhm = nyh - 1
+    w = np.zeros((nyh + 2,) + s.shape[1:], dtype=s.dtype)
+    for i in range(2, nyhm + 1):
+        j = 2 * i - 2
+        rj1 = 1.0 / (4 * j * (j - 1))
+        rj2 = 1.0 / (2 * (j * j - 1))
+        rj3 = 1.0 / (4 * j * (j + 1))
+        w[i] = -c[j - 2] * s[j - 2] * rj1 + s[j] * rj2 - s[j + 2] * rj3
+    j = ny - 2
+    w[nyh] = -s[j - 2] / (4 * j * (j - 1)) + s[j] / (2 * (j * j - 1))
+    j = ny
+    w[nyh + 1] = -s[j - 2] / (4 * j * (j - 1))
+    return w
+
+
+def _odds(s, ny):
+    """solvers.f90:444-496 (rows 2..nyh)."""
+    nyh = ny // 2
+    nyhm = nyh - 1
+    w = np.zeros((nyh + 2,) + s.shape[1:], dtype=s.dtype)
+    for i in range(2, nyhm):
+        j = 2 * i - 1
+        rj1 = 1.0 / (4 * j * (j - 1))
+        rj2 = 1.0 / (2 * (j * j - 1))
+        rj3 = 1.0 / (4 * j * (j + 1))
+        w[i] = -s[j - 2] * rj1 + s[j] * rj2 - s[j + 2] * rj3
+    j = ny - 3
+    w[nyhm] = -s[j - 2] / (4 * j * (j - 1)) + s[j] / (2 * (j * j - 1))
+    j = ny - 1
+    w[nyh] = -s[j - 2] / (4 * j * (j - 1))
+    return w
+
+
+def _eqsolve(a1, a2, a3, t, w, nrank):
+    """solvers.f90:500-553 -- quasi-tridiagonal elimination with a full BC row.
+    t: BC row weights t[1..nrank] (1-based, shape (nrank+1,) + batch)."""
+    m = nrank - 1
+    d = np.zeros_like(a2)
+    d[m] = a2[m]
+    w = w.copy()
+    for i in range(m - 1, 0, -1):
+        d[i] = a2[i] - a3[i] * a1[i + 1] / d[i + 1]
+        w[i + 1] = w[i + 1] - a3[i] * w[i + 2] / d[i + 1]
+    f = t[m] - t[nrank] * a1[m] / d[m]
+    w[1] = w[1] - t[nrank] * w[nrank] / d[m]
+    for i in range(m - 1, 0, -1):
+        w[1] = w[1] - f * w[i + 1] / d[i]
+        f = t[i] - f * a1[i] / d[i]
+    w[1] = w[1] / f
+    for i in range(2, nrank + 1):
+        w[i] = (w[i] - a1[i - 1] * w[i - 1]) / d[i - 1]
+    return w
+
+
+def solve(f, wn, dyde, bctop, bcbot, ib):
+    """solvers.f90:174-283 -- (dyde^2 D^2 - wn) u = f, Dirichlet (ib=0) or
+    Neumann (ib!=0) rows.  f: (..., nyp) Chebyshev RHS; wn, bctop, bcbot
+    broadcast over the leading axes.  Returns u (..., nyp)."""
+    nyp = f.shape[-1]
+    ny = nyp - 1
+    nyh = ny // 2
+    nyhp = nyh + 1
+    c = np.ones(nyp)
+    c[0] = c[-1] = 2.0
+    s = np.moveaxis(f, -1, 0)
+    wn = np.broadcast_to(np.asarray(wn, dtype=np.float64), s.shape[1:]).copy()
+    kzero = None
+    if ib != 0:
+        kzero = wn <= 1.0e-20
+        wn[kzero] = 1.0
+    bshape = (1,) * len(s.shape[1:])
+    # even
+    a1, a2, a3 = _evnc(wn, dyde, nyh)
+    w = _evns(s, ny, c)
+    if ib == 0:
+        w[1] = 0.5 * (bctop + bcbot)
+        te = np.ones(nyhp + 1)
+    else:
+        w[1] = 0.5 * (bctop - bcbot)
+        te = np.array([0.0] + [dyde * (2 * (j - 1)) ** 2 for j in range(1, nyhp + 1)])
+    we = _eqsolve(a1, a2, a3, te.reshape((-1,) + bshape), w, nyhp)
+    # odd
+    a1, a2, a3 = _oddc(wn, dyde, nyh)
+    w = _odds(s, ny)
+    if ib == 0:
+        w[1] = 0.5 * (bctop - bcbot)
+        to = np.ones(nyh + 1)
+    else:
+        w[1] = 0.5 * (bctop + bcbot)
+        to = np.array([0.0] + [dyde * (2 * (j - 1) + 1) ** 2 for j in range(1, nyh + 1)])
+    wo = _eqsolve(a1, a2, a3, to.reshape((-1,) + bshape), w, nyh)
+    u = np.zeros_like(s)
+    u[0::2] = we[1:nyhp + 1]
+    u[1::2] = wo[1:nyh + 1]
+    if kzero is not None:
+        u[:, kzero] = 0.0
+    return np.moveaxis(u, 0, -1)
+
+
+def _mixed_rows(at, ab, bt, bb, dyde, nyp, bottom_sign):
+    """mixedbc (solvers.f90:777-803, bottom_sign=+1, quirk: bottom derivative
+    sign flipped) and mixedbc1d (solvers.f90:1249-1269, bottom_sign=-1)."""
+    j1 = np.arange(nyp, dtype=np.float64)
+    tt = at + dyde * bt * j1 * j1
+    tb = ab + bottom_sign * dyde * bb * j1 * j1
+    tb[1::2] = -tb[1::2]
+    return tt, tb
+
+
+def _pntsolve_pair(ae, ao, we, wo, tt, tb, gtop, gbot, ny):
+    """pntsolve (solvers.f90:906-1017) for the even (ip=1) then odd (ip=2)
+    systems sharing the two Robin rows; returns u (n on axis 0)."""
+    nyh = ny // 2
+    res = {}
+    s = [None, np.array(gtop, dtype=np.complex128), np.array(gbot, dtype=np.complex128)]
+    for ip, (a, w, nrank) in ((1, (ae, we, nyh + 1)), (2, (ao, wo, nyh))):
+        a1, a2, a3 = a
+        n = nrank
+        m = n - 1
+        cs = a1.copy()
+        gs = w.copy()
+        ds = np.zeros_like(a2)
+        ds[m] = a2[m]
+        for i in range(m - 1, 0, -1):
+            ds[i] = a2[i] - a3[i] * cs[i + 1] / ds[i + 1]
+            gs[i + 1] = gs[i + 1] - a3[i] * gs[i + 2] / ds[i + 1]
+        f = {}
+        for js, tv in ((2, tb), (1, tt)):
+            # fillt solvers.f90:1021-1061
+            if ip == 1:
+                t = np.concatenate([[0.0], tv[0::2][:nrank]])
+            else:
+                t = np.concatenate([[0.0], tv[1::2][:nrank]])
+            fj = t[m] - t[n] * cs[m] / ds[m]
+            s[js] = s[js] - t[n] * gs[n] / ds[m]
+            for i in range(m - 1, 0, -1):
+                s[js] = s[js] - fj * gs[i + 1] / ds[i]
+                fj = t[i] - fj * cs[i] / ds[i]
+            f[js] = fj
+        res[ip] = (cs, ds, gs, f)
+    (cse, dse, gse, fe), (cso, dso, gso, fo) = res[1], res[2]
+    det = fe[1] * fo[2] - fe[2] * fo[1]
+    u0 = (s[1] * fo[2] - s[2] * fo[1]) / det
+    u1 = (s[2] * fe[1] - s[1] * fe[2]) / det
+    nyp = ny + 1
+    u = np.zeros((nyp,) + np.shape(u0), dtype=np.complex128)
+    u[0] = u0
+    u[1] = u1
+    for i in range(2, nyh + 2):          # even, ii = 2i-1 (1-based) -> n = 2i-2
+        u[2 * i - 2] = (gse[i] - cse[i - 1] * u[2 * i - 4]) / dse[i - 1]
+    for i in range(2, nyh + 1):          # odd, ii = 2i -> n = 2i-1
+        u[2 * i - 1] = (gso[i] - cso[i - 1] * u[2 * i - 3]) / dso[i - 1]
+    return u
+
+
+def penta(f, wn, dyde, ib, at, bt, gtop, ab, bb, gbot):
+    """solvers.f90:680-773 -- (dyde^2 D^2 - wn) u = f with Robin rows
+    at*u + bt*u' = gtop (y=+1), ab*u +/- bb*u' = gbot (y=-1, quirk Q9)."""
+    nyp = f.shape[-1]
+    ny = nyp - 1
+    nyh = ny // 2
+    c = np.ones(nyp)
+    c[0] = c[-1] = 2.0
+    s = np.moveaxis(f, -1, 0)
+    wn = np.broadcast_to(np.asarray(wn, dtype=np.float64), s.shape[1:]).copy()
+    kzero = None
+    if ib != 0:
+        kzero = wn <= 1.0e-20
+        wn[kzero] = 1.0
+    tt, tb = _mixed_rows(at, ab, bt, bb, dyde, nyp, +1.0)
+    ae = _evnc(wn, dyde, nyh)
+    we = _evns(s, ny, c)
+    we[1] = 0.0
+    ao = _oddc(wn, dyde, nyh)
+    wo = _odds(s, ny)
+    wo[1] = 0.0
+    gt = np.broadcast_to(gtop, s.shape[1:])
+    gb = np.broadcast_to(gbot, s.shape[1:])
+    u = _pntsolve_pair(ae, ao, we, wo, tt, tb, gt, gb, ny)
+    if kzero is not None:
+        u[:, kzero] = 0.0
+    return np.moveaxis(u, 0, -1)
+
+
+def psolve1d(f, g, dyde, at, bt, gtop, ab, bb, gbot):
+    """solvers.f90:1180-1245 (+ p1d* 1249-1493): 1-D Robin Helmholtz for the
+    mean mode, correct bottom-derivative sign (mixedbc1d)."""
+    nyp = f.shape[-1]
+    ny = nyp - 1
+    nyh = ny // 2
+    c = np.ones(nyp)
+    c[0] = c[-1] = 2.0
+    s = f.astype(np.complex128)
+    wn = np.asarray(g, dtype=np.float64)
+    tt, tb = _mixed_rows(at, ab, bt, bb, dyde, nyp, -1.0)
+    ae = _evnc(wn, dyde, nyh)
+    we = _evns(s, ny, c)
+    we[1] = 0.0
+    ao = _oddc(wn, dyde, nyh)
+    wo = _odds(s, ny)
+    wo[1] = 0.0
+    u = _pntsolve_pair(ae, ao, we, wo, tt, tb, gtop, gbot, ny)
+    return u.real.copy()
+
+
+# --------------------------------------------------------------------------- #
+#  Spectral-space operators (solvers.f90, helpers.f90)
+# --------------------------------------------------------------------------- #
+def norm(a):
+    """helpers.f90:91-158 -- reality/symmetry projection (in place)."""
+    nz = a.shape[1]
+    a[0, 0, :] = a[0, 0, :].real
+    j = np.arange(1, nz // 2)
+    jp = nz - j
+    a[0, j, :] = 0.5 * (a[0, j, :] + np.conj(a[0, jp, :]))
+    a[0, jp, :] = np.conj(a[0, j, :])
+    a[:, nz // 2, :] = 0.0
+    return a
+
+
+def phirhs(v, fn, fnm1, var, p: Params):
+    """solvers.f90:555-676 -- CN/AB2 right-hand side.  Returns the RHS that
+    the reference leaves in ``v`` (the zeroed bctop/bcbot are implicit)."""
+    re, dt, th = p.re, p.dt, p.theta
+    wrkc = -re * var / (dt * th) * v - re * var / (2.0 * th) * (3.0 * fn - fnm1)
+    if th > 0.99:
+        v = wrkc.copy()
+    wrkc = wrkc + (1.0 - th) * p.k2[:, :, None] * v / th
+    d2 = cderiv(cderiv(v, p.dyde), p.dyde)
+    return wrkc - (1.0 - th) * d2 / th
+
+
+def uzero(u0, h1, h1l, uflag, p: Params):
+    """solvers.f90:1065-1176 -- mean (kx=kz=0) streamwise/spanwise velocity."""
+    re, dt, th = p.re, p.dt, p.theta
+    wrk1 = -re * u0 / (dt * th) - re * (3.0 * h1 - h1l) / (2.0 * th)
+    if th > 0.99:
+        rhs = wrk1
+    else:
+        d2 = cderiv(cderiv(u0, p.dyde), p.dyde)
+        rhs = wrk1 - (1.0 - th) * d2 / th
+    g = re / (th * dt)
+    if uflag == 1:
+        return psolve1d(rhs, g, p.dyde, p.atu, p.btu, p.gtu, p.abu, p.bbu, p.gbu)
+    # quirk Q10: btu passed for w (solvers.f90:1173)
+    return psolve1d(rhs, g, p.dyde, p.atw, p.btu, p.gtw, p.abw, p.bbw, p.gbw)
+
+
+def veloc(v, omy, u0, w0, p: Params):
+    """solvers.f90:1576-1659 -- u, w from continuity and omega_y."""
+    dv = cderiv(v, p.dyde)
+    kx = p.wavx[:, None, None]
+    kz = p.wavz[None, :, None]
+    w2 = (kx ** 2 + kz ** 2).copy()
+    w2[0, 0, 0] = 1.0
+    u = -(-1j * kx * dv + 1j * kz * omy) / w2
+    w = -(-1j * kz * dv - 1j * kx * omy) / w2
+    u[0, 0, :] = u0
+    w[0, 0, :] = w0
+    v = v.copy()
+    v[0, 0, :] = 0.0
+    return u, v, w
+
+
+def vort(u, v, w, p: Params):
+    """solvers.f90:1663-1722 -- omega_x, omega_z."""
+    kx = p.wavx[:, None, None]
+    kz = p.wavz[None, :, None]
+    omz = 1j * kx * v - cderiv(u, p.dyde)
+    omx = cderiv(w, p.dyde) - 1j * kz * v
+    return omx, omz
+
+
+def nonlin(fn, omz, gn, p: Params):
+    """solvers.f90:1497-1572 -- fn (4th-order v system) and gn (omega_y)."""
+    kx = p.wavx[:, None, None]
+    kz = p.wavz[None, :, None]
+    wrkc = 1j * (kz * omz + kx * gn)
+    wrk1 = cderiv(wrkc, p.dyde)
+    fn_new = -(kx ** 2 + kz ** 2) * fn - wrk1
+    gn_new = 1j * (kz * gn - kx * omz)
+    return fn_new, gn_new
+
+
+def gradscl(sc, p: Params):
+    """derivatives.f90:90-144."""
+    kx = p.wavx[:, None, None]
+    kz = p.wavz[None, :, None]
+    return 1j * kx * sc, cderiv(sc, p.dyde), 1j * kz * sc
+
+
+def gradv(u, v, w, p: Params):
+    """derivatives.f90:6-86 -- 9 gradients (u_ij = d u_i / d x_j) + 3 Laplacians."""
+    kx = p.wavx[:, None, None]
+    kz = p.wavz[None, :, None]
+    k2 = kx ** 2 + kz ** 2
+    d = lambda a: cderiv(a, p.dyde)
+    u12, u22, u32 = d(u), d(v), d(w)
+    return dict(u11=1j * kx * u, u12=u12, u13=1j * kz * u,
+                u21=1j * kx * v, u22=u22, u23=1j * kz * v,
+                u31=1j * kx * w, u32=u32, u33=1j * kz * w,
+                Lu=d(u12) - k2 * u, Lv=d(u22) - k2 * v, Lw=d(u32) - k2 * w)
+
+
+def smoothing_factors(p: Params):
+    """main_dns.f90:710-745 -- exp(-3 (|kz|/nzh)^10) * exp(-3 (n/ny)^10)."""
+    j = np.arange(p.nz)
+    jj = np.where(j < p.nzh, j, p.nz - j)
+    zs = np.exp(-3.0 * (jj / p.nzh) ** 10)
+    ys = np.exp(-3.0 * (np.arange(p.nyp) / p.ny) ** 10)
+    return zs[None, :, None] * ys[None, None, :]
+
+
+# --------------------------------------------------------------------------- #
+#  Transforms (solvers.f90:2241-2879 inverse, 3147-3416 forward)
+# --------------------------------------------------------------------------- #
+def to_physical(a, p: Params):
+    """Spectral (nxh, nz, nyp) -> physical 3/2 grid (mx, mz, nyp).
+    Chebyshev scale c_n/2 then DCT-I (REDFT00) along y; zero-pad kz to mz
+    (negative kz on top) and kx to mx/2+1; BACKWARD c2c along z then c2r along
+    x, all unnormalised."""
+    fac = p.c / 2.0
+    ay = a * fac
+    ayr = sfft.dct(ay.real, type=1, axis=-1)
+    ayi = sfft.dct(ay.imag, type=1, axis=-1)
+    ay = ayr + 1j * ayi                                  # (nxh, nz, nyp), physical y
+    pad = np.zeros((p.nyp, p.mz, p.mx // 2 + 1), dtype=np.complex128)
+    t = np.transpose(ay, (2, 1, 0))                      # (nyp, nz, nxh)
+    pad[:, :p.nzh, :p.nxh] = t[:, :p.nzh, :]
+    pad[:, p.mz - p.nzh:, :p.nxh] = t[:, p.nzh:, :]
+    zt = np.fft.ifft(pad, axis=1) * p.mz                 # BACKWARD along z
+    phys = np.fft.irfft(zt, n=p.mx, axis=2) * p.mx       # c2r along x
+    return np.ascontiguousarray(np.transpose(phys, (2, 1, 0)))   # (mx, mz, nyp)
+
+
+def to_spectral(ph, p: Params):
+    """Physical 3/2 (mx, mz, nyp) -> spectral (nxh, nz, nyp): r2c along x,
+    FORWARD c2c along z, keep nxh x nz modes, / (mx mz); DCT-I, / ny,
+    halve n=0 and n=ny."""
+    t = np.transpose(ph, (2, 1, 0))                      # (nyp, mz, mx)
+    xt = np.fft.rfft(t, axis=2)
+    zt = np.fft.fft(xt, axis=1)
+    sp = np.empty((p.nyp, p.nz, p.nxh), dtype=np.complex128)
+    sp[:, :p.nzh, :] = zt[:, :p.nzh, :p.nxh]
+    sp[:, p.nzh:, :] = zt[:, p.mz - p.nzh:, :p.nxh]
+    sp /= float(p.mx * p.mz)
+    sp = np.transpose(sp, (2, 1, 0))                     # (nxh, nz, nyp)
+    r = sfft.dct(sp.real, type=1, axis=-1)
+    i = sfft.dct(sp.imag, type=1, axis=-1)
+    out = (r + 1j * i) / float(p.ny)
+    out[..., 0] /= 2.0
+    out[..., -1] /= 2.0
+    return np.ascontiguousarray(out)
+
+
+def xyzfft(b, p: Params):
+    """helpers.f90:6-87 (is=-1) -- physical (nx, nz, nyp) unpadded -> spectral."""
+    t = sfft.dct(b, type=1, axis=-1) / float(p.ny)
+    t[..., 0] /= 2.0
+    t[..., -1] /= 2.0
+    t = np.transpose(t, (2, 1, 0))                       # (nyp, nz, nx)
+    xt = np.fft.rfft(t, axis=2)[:, :, :p.nxh] / float(p.nx)
+    zt = np.fft.fft(xt, axis=1) / float(p.nz)
+    return np.ascontiguousarray(np.transpose(zt, (2, 1, 0)))
+
+
+# --------------------------------------------------------------------------- #
+#  Immersed boundary (solvers.f90:2996-3108, main_dns.f90:1526-1558)
+# --------------------------------------------------------------------------- #
+def fspread_matrix():
+    """main_dns.f90:1550-1558 -- fspread(j,i) = exp(-2 dsq) if dsq < 13."""
+    fs = np.zeros((10, 10))   # [jdif-1, idif-1]
+    for i in range(10):
+        for j in range(10):
+            dsq = float(i * i + j * j)
+            if dsq < 13:
+                fs[j, i] = math.exp(-2.0 * dsq)
+    return fs
+
+
+def kmaxsurf(imatrix):
+    """main_dns.f90:1526-1535 -- largest 1-based x index of a solid cell (quirk Q7)."""
+    solid = (imatrix == 1) | (imatrix == 3) | (imatrix == 4)
+    ks = np.nonzero(solid.any(axis=(1, 2)))[0]
+    return int(ks.max()) + 1 if ks.size else 0
+
+
+def apply_ib_plane(i, up, vp, wp, vw, drag_src, fxi, fyi, fzi, imat, it, p: Params,
+                   gain, ugain, kmax):
+    """Goldstein feedback at y-plane i (0-based) with the reference's
+    order-dependent 5x5 spreading (quirks Q5, Q6, Q7).
+
+    up/vp/wp/vw*: (mx, mz) arrays for this plane, x index first; the
+    reference's (mz, mx) plane is column-major so the linear address of
+    (j,k) (1-based) is (k-1)*mz + j, which is also the loop visit order.
+    vw = [vwx, vwy, vwz] already hold the assigned products (incl. -dPdx).
+    fxi/fyi/fzi: (mx, mz) integral slices, updated in place."""
+    mx, mz = up.shape
+    src = np.argwhere((imat == 1) | ((imat == 4) & (it <= p.perturbtime)))
+    dragx = np.zeros((mx, mz))
+    dragy = np.zeros((mx, mz))
+    dragz = np.zeros((mx, mz))
+    for k, j in src:
+        fxi[k, j] += up[k, j]
+        fyi[k, j] += vp[k, j]
+        fzi[k, j] += wp[k, j]
+        dragx[k, j] = -ugain * up[k, j] - gain * fxi[k, j]
+        dragy[k, j] = -ugain * vp[k, j] - gain * fyi[k, j]
+        dragz[k, j] = -ugain * wp[k, j] - gain * fzi[k, j]
+    other = np.argwhere((imat != 0) & (imat != 1) & (imat != 4))
+    if other.size:
+        raise NotImplementedError("imatrix types other than 0/1/4 are not supported")
+    if (i + 1) > kmax or src.size == 0:
+        return
+    fs = fspread_matrix()
+    flat = [a.reshape(-1) for a in vw]          # linear index = k*mz + j (0-based)
+    for k, j in src:
+        ls = k * mz + j
+        for ii in range(-2, 3):
+            ipii = k + ii
+            if ipii < 0:
+                ipii += mx
+            if ipii >= mx:
+                ipii -= mx
+            for jj in range(-2, 3):
+                w = fs[abs(jj), abs(ii)]
+                lt = ipii * mz + (j + jj)
+                if lt < 0 or lt >= mx * mz:
+                    raise ValueError("IB spreading leaves the plane (quirk Q5)")
+                if lt <= ls:       # target visited at or before the source
+                    flat[0][lt] += w * dragx[k, j]
+                    flat[1][lt] += w * dragy[k, j]
+                    flat[2][lt] += w * dragz[k, j]
+
+
+# --------------------------------------------------------------------------- #
+#  Nonlinear term (solvers.f90:1726-3538, NS + scalar + IB subset)
+# --------------------------------------------------------------------------- #
+class CFLError(RuntimeError):
+    pass
+
+
+def vcw3dp(st, p: Params, it, fields):
+    """Returns (gn, fn, omz, scn) = spectral (u x omega)_x, _y, _z and the
+    scalar nonlinear term, and updates the IB integrals in st."""
+    u, v, w, omx, omy, omz = (fields[k] for k in ("u", "v", "w", "omx", "omy", "omz"))
+    up, vp, wp = to_physical(u, p), to_physical(v, p), to_physical(w, p)
+    wxp, wyp, wzp = to_physical(omx, p), to_physical(omy, p), to_physical(omz, p)
+    vwx = vp * wzp - wp * wyp
+    vwy = -up * wzp + wp * wxp
+    vwz = up * wyp - vp * wxp
+    # CFL (solvers.f90:2985-2986, 3503-3514)
+    delxm = p.xl / p.mx
+    delzm = p.zl / p.mz
+    cfl = (np.abs(up) / delxm + np.abs(vp) / p.seght[None, None, :]
+           + np.abs(wp) / delzm) * p.dt
+    cflmax = float(cfl.max())
+    st["cflmax"] = cflmax
+    vc = None
+    if p.scalar:
+        cxp = to_physical(fields["sclx"], p)
+        cyp = to_physical(fields["scly"], p)
+        czp = to_physical(fields["sclz"], p)
+        vc = -(up * cxp + vp * cyp + wp * czp)
+        if p.scl_flag == 6:
+            vc = vc + p.deltaT
+    imat = st.get("imatrix")
+    if p.flow_select == 1:
+        if imat is None:
+            vwx = vwx - p.dPdx
+        else:
+            vwx = vwx - p.dPdx * (imat == 0)
+    if imat is not None and np.any(imat != 0):
+        kmax = kmaxsurf(imat)
+        for i in range(p.nyp):
+            plane = imat[:, :, i]
+            if not np.any(plane):
+                continue
+            vwp = [vwx[:, :, i].copy(), vwy[:, :, i].copy(), vwz[:, :, i].copy()]
+            fxi = st["fxintg"][:, :, i].copy()
+            fyi = st["fyintg"][:, :, i].copy()
+            fzi = st["fzintg"][:, :, i].copy()
+            apply_ib_plane(i, up[:, :, i], vp[:, :, i], wp[:, :, i], vwp, None,
+                           fxi, fyi, fzi, plane, it, p, p.gain, p.ugain, kmax)
+            vwx[:, :, i], vwy[:, :, i], vwz[:, :, i] = vwp
+            st["fxintg"][:, :, i] = fxi
+            st["fyintg"][:, :, i] = fyi
+            st["fzintg"][:, :, i] = fzi
+    gn = to_spectral(vwx, p)
+    fn = to_spectral(vwy, p)
+    omz_o = to_spectral(vwz, p)
+    scn = to_spectral(vc, p) if vc is not None else None
+    phys = dict(up=up, vp=vp, wp=wp)
+    return gn, fn, omz_o, scn, phys
+
+
+# --------------------------------------------------------------------------- #
+#  Influence matrix (solvers.f90:5-172, main_dns.f90:151-161, 433-466)
+# --------------------------------------------------------------------------- #
+def gfcn(pb, pt, p: Params):
+    """Green's function for phi(+1)=pt, phi(-1)=pb, v(+-1)=0; returns
+    (gf, dnvb, dnvt)."""
+    g = p.re / (p.theta * p.dt)
+    bct = np.full((p.nxh, p.nz), complex(pt, pt))
+    bcb = np.full((p.nxh, p.nz), complex(pb, pb))
+    bct[0, p.nz // 2:] = np.conj(bct[0, p.nz // 2:])
+    bcb[0, p.nz // 2:] = np.conj(bcb[0, p.nz // 2:])
+    bct[0, 0] = complex(pt, 0.0)
+    bcb[0, 0] = complex(pb, 0.0)
+    zero = np.zeros((p.nxh, p.nz, p.nyp), dtype=np.complex128)
+    phi = solve(zero, p.k2 + g, p.dyde, bct, bcb, 0)
+    gf = solve(phi, p.k2 + 0.0, p.dyde, 0.0, 0.0, 0)
+    gf[0, 0, :] = gf[0, 0, :].real
+    j = np.arange(1, p.nz // 2)
+    jp = p.nz - j
+    gf[0, j, :] = 0.5 * (gf[0, j, :] + np.conj(gf[0, jp, :]))
+    gf[0, jp, :] = np.conj(gf[0, j, :])
+    if p.nb == 1:
+        dnvb = c1derbw(gf, p.dyde)
+    else:
+        dnvb = c1derbw(cderiv(gf, p.dyde), p.dyde)
+    if p.nt == 1:
+        dnvt = c1dertw(gf, p.dyde)
+    else:
+        dnvt = c1dertw(cderiv(gf, p.dyde), p.dyde)
+    return gf, dnvb, dnvt
+
+
+class Solver:
+    """State + one reference time step.  State keys mirror the restart file:
+    v, fn, fnm1, omy, gn, gnm1 (spectral), u0, h1n, h1nm1, w0, h3n, h3nm1
+    (nyp), fxintg/fyintg/fzintg (mx, mz, nyp), optional scalar/scn/scnm1,
+    optional imatrix (mx, mz, nyp) int32."""
+
+    def __init__(self, p: Params, imatrix=None):
+        self.p = p
+        self.gf1, self.dnv1b, self.dnv1t = gfcn(p.p1b, p.p1t, p)
+        self.gf2, self.dnv2b, self.dnv2t = gfcn(p.p2b, p.p2t, p)
+        self.denom = self.dnv1t * self.dnv2b - self.dnv2t * self.dnv1b
+        self.smooth = smoothing_factors(p)
+        self.imatrix = imatrix
+        self.it = 0
+
+    # main_dns.f90:1602-2231 (new-run branch, from spectral u/v/w)
+    def startup(self, st):
+        """Given v, omy, u0, w0 (and scalar), fill the derived fields and the
+        AB2 history exactly as ``initial`` does at it=0."""
+        p = self.p
+        for k in ("v", "omy"):
+            norm(st[k])
+        u, v, w = veloc(st["v"], st["omy"], st["u0"], st["w0"], p)
+        norm(u); norm(v); norm(w)
+        st["v"] = v
+        st["omy"][0, 0, :] = 0.0
+        st.setdefault("fxintg", np.zeros((p.mx, p.mz, p.nyp)))
+        st.setdefault("fyintg", np.zeros((p.mx, p.mz, p.nyp)))
+        st.setdefault("fzintg", np.zeros((p.mx, p.mz, p.nyp)))
+        self._nonlinear(st, u, v, w, it=0, smooth=False)
+        st["fnm1"] = st["fn"].copy()
+        st["gnm1"] = st["gn"].copy()
+        st["h1nm1"] = st["h1n"].copy()
+        st["h3nm1"] = st["h3n"].copy()
+        if p.scalar:
+            st["scnm1"] = st["scn"].copy()
+        return st
+
+    def _nonlinear(self, st, u, v, w, it, smooth):
+        p = self.p
+        omx, omz = vort(u, v, w, p)
+        norm(omx); norm(omz)
+        fields = dict(u=u, v=v, w=w, omx=omx, omy=st["omy"], omz=omz)
+        if p.scalar:
+            norm(st["scalar"])
+            sx, sy, sz = gradscl(st["scalar"], p)
+            norm(sx); norm(sy); norm(sz)
+            fields.update(sclx=sx, scly=sy, sclz=sz)
+        if self.imatrix is not None:
+            st["imatrix"] = self.imatrix
+        gn, fn, omz_o, scn, phys = vcw3dp(st, p, it, fields)
+        st.pop("imatrix", None)
+        self.last_phys = phys
+        self.last_fields = fields
+        if st["cflmax"] > 1.0:
+            raise CFLError(f"CFL failure at time step {it}: {st['cflmax']}")
+        for a in (fn, gn, omz_o):
+            norm(a)
+        if scn is not None:
+            norm(scn)
+        if smooth:
+            omz_o *= self.smooth
+            gn *= self.smooth
+            fn *= self.smooth
+            if scn is not None:
+                scn *= self.smooth
+        st["h1n"] = gn[0, 0, :].real.copy()
+        st["h3n"] = omz_o[0, 0, :].real.copy()
+        for a in (fn, gn, omz_o):
+            norm(a)
+        fn2, gn2 = nonlin(fn, omz_o, gn, p)
+        norm(fn2); norm(gn2)
+        st["fn"] = fn2
+        st["gn"] = gn2
+        if scn is not None:
+            norm(scn)
+            st["scn"] = scn
+
+    # main_dns.f90:198-811 (NS + scalar)
+    def step(self, st):
+        p = self.p
+        self.it += 1
+        it = self.it
+        g = p.re / (p.dt * p.theta)
+        if p.scalar:
+            rhs = phirhs(st["scalar"], st["scn"], st["scnm1"], p.diff, p)
+            st["scnm1"] = st["scn"].copy()
+            gs = p.re * p.diff / (p.dt * p.theta)
+            st["scalar"] = penta(rhs, p.k2 + gs, p.dyde, 0, 0.0, 1.0, 0.0, 0.0, 1.0, 0.0)
+        v = st["v"]
+        phi = cderiv(cderiv(v, p.dyde), p.dyde) - p.k2[:, :, None] * v
+        rhs = phirhs(phi, st["fn"], st["fnm1"], 1.0, p)
+        st["fnm1"] = st["fn"].copy()
+        phi = solve(rhs, p.k2 + g, p.dyde, 0.0, 0.0, 0)
+        vp = solve(phi, p.k2 + 0.0, p.dyde, 0.0, 0.0, 0)
+        orhs = phirhs(st["omy"], st["gn"], st["gnm1"], 1.0, p)
+        st["gnm1"] = st["gn"].copy()
+        vb = c1derbw(vp, p.dyde)
+        vt = c1dertw(vp, p.dyde)
+        c1 = (vb * self.dnv2t - vt * self.dnv2b) / self.denom
+        c2 = (vt * self.dnv1b - vb * self.dnv1t) / self.denom
+        v = vp + c1[:, :, None] * self.gf1 + c2[:, :, None] * self.gf2
+        omy = penta(orhs, p.k2 + g, p.dyde, 0, p.atu, p.btu, 0.0, p.abu, p.bbu, 0.0)
+        st["u0"] = uzero(st["u0"], st["h1n"], st["h1nm1"], 1, p)
+        st["w0"] = uzero(st["w0"], st["h3n"], st["h3nm1"], 3, p)
+        omy[0, 0, :] = 0.0
+        st["h1nm1"] = st["h1n"].copy()
+        st["h3nm1"] = st["h3n"].copy()
+        norm(v); norm(omy)
+        st["omy"] = omy
+        u, v, w = veloc(v, omy, st["u0"], st["w0"], p)
+        norm(u); norm(w)
+        st["v"] = v
+        self._nonlinear(st, u, v, w, it=it, smooth=True)
+        self.last_u = u
+        self.last_w = w
+        return st
+
+
+# --------------------------------------------------------------------------- #
+#  Seeded initial condition (new; no reference counterpart)
+# --------------------------------------------------------------------------- #
+class SplitMix64:
+    """SplitMix64 (Steele, Lea, Flood 2014) + Box-Muller, FP64."""
+
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed):
+        self.s = seed & self.MASK
+
+    def next_u64(self):
+        self.s = (self.s + 0x9E3779B97F4A7C15) & self.MASK
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & self.MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & self.MASK
+        return z ^ (z >> 31)
+
+    def uniform(self):
+        return ((self.next_u64() >> 11) + 0.5) * (1.0 / 9007199254740992.0)
+
+    def normal(self):
+        u1 = self.uniform()
+        u2 = self.uniform()
+        return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)

@@ -1,0 +1,963 @@
+// Host runtime and C ABI of libkmm (see include/kmm.h).
+//
+// Owns the device-resident state of one channel, the per-pencil tau and
+// Green's-function tables, the immersed-boundary preprocessing, restart I/O
+// and the launch sequence of one time step:
+//   k_pencil (y)  ->  k_zinv  ->  k_ibdrag  ->  k_xplane  ->  k_zfwd
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/kmm.h"
+#include "kmm_kernels.h"
+
+using kmm::PencilArgs;
+using kmm::PlaneArgs;
+
+static thread_local std::string g_last_error;
+
+#define KMM_HIP(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      g_last_error = std::string(#call) + ": " + hipGetErrorString(e_);                 \
+      return KMM_E_HIP;                                                                 \
+    }                                                                                   \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+};
+
+struct kmm_ctx {
+  kmm_config cfg;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nx, ny, nz, nyp, nxh, nzh, mx, mz, P;
+  double alpha, beta, dyde, dPdx, gH, gS;
+  std::vector<double> ycoord, seght;
+  size_t S;  // complex elements of one spectral field
+  // state
+  double2 *V, *OMY, *FN, *FNM1, *GN, *GNM1, *TH, *SCN, *SCNM1;
+  double *u0, *w0, *h1n, *h1nm1, *h3n, *h3nm1;
+  double2* NL[4];
+  double2* T[9];
+  double2 *Z, *ZF;
+  // tables
+  double2 *tabH, *tabP, *tabS, *G1, *G2;
+  double4* gk;
+  double2 *twM, *twZ, *twX;
+  double *smz, *smy, *seght_d;
+  unsigned long long* cfl_d;
+  int cfl_cap = 0;
+  // immersed boundary
+  int nsrc = 0;
+  int4* src = nullptr;
+  double *fint = nullptr, *drag = nullptr;
+  int *trow = nullptr, *tx = nullptr, *tptr = nullptr, *ent_src = nullptr, *fixrow = nullptr, *fixx = nullptr;
+  double* ent_w = nullptr;
+  std::vector<int4> src_h;
+  std::vector<double> fint_full[3];  // host copies (nyp,mz,mx) Fortran order
+  // bookkeeping
+  int it = 0;
+  bool nl_valid = false;  // NL buffers hold the nonlinear terms of the state
+  double last_cfl = 0.0;
+  int last_cfl_it = 0;
+  double t_pencil = 0.0, t_plane = 0.0;
+  hipEvent_t ev[3];
+  // per-kernel timing (pencil, zinv, ibdrag, xplane, zfwd) over a kmm_step call
+  int timing = 0;
+  std::vector<hipEvent_t> tev;
+  double ktime[5] = {0, 0, 0, 0, 0};
+  int ksamples = 0;
+  std::vector<void*> allocs;
+};
+
+static int dalloc(kmm_ctx* c, void** p, size_t bytes) {
+  KMM_HIP(hipMalloc(p, bytes > 0 ? bytes : 16));
+  KMM_HIP(hipMemsetAsync(*p, 0, bytes > 0 ? bytes : 16, c->stream));
+  c->allocs.push_back(*p);
+  return KMM_OK;
+}
+#define DALLOC(ptr, bytes)                                                  \
+  do {                                                                      \
+    int r_ = dalloc(c, (void**)&(ptr), (bytes));                            \
+    if (r_) return r_;                                                      \
+  } while (0)
+
+extern "C" const char* kmm_status_string(int s) {
+  switch (s) {
+    case KMM_OK: return "ok";
+    case KMM_E_CFL: return "CFL failure";
+    case KMM_E_PARTICLE: return "particle out of domain";
+    case KMM_E_HIP: return g_last_error.empty() ? "HIP error" : g_last_error.c_str();
+    case KMM_E_CONFIG: return g_last_error.empty() ? "configuration error" : g_last_error.c_str();
+    case KMM_E_IO: return g_last_error.empty() ? "I/O error" : g_last_error.c_str();
+    case KMM_E_ARG: return g_last_error.empty() ? "bad argument" : g_last_error.c_str();
+    default: return "unknown";
+  }
+}
+
+static int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+// ----------------------------------------------------------- dns.config --
+static bool parse_num(const std::string& line, int nvals, double* out) {
+  std::string s = line;
+  for (auto& ch : s)
+    if (ch == 'd' || ch == 'D') ch = 'e';
+  std::istringstream is(s);
+  for (int i = 0; i < nvals; ++i) {
+    std::string tok;
+    if (!(is >> tok)) return false;
+    // list-directed input also accepts a trailing comma separator
+    if (!tok.empty() && tok.back() == ',') tok.pop_back();
+    char* end = nullptr;
+    out[i] = std::strtod(tok.c_str(), &end);
+    if (end == tok.c_str()) return false;
+  }
+  return true;
+}
+
+extern "C" int kmm_config_read(const char* path, kmm_config* cfg) {
+  if (!path || !cfg) return fail(KMM_E_ARG, "null argument");
+  std::ifstream f(path);
+  if (!f) return fail(KMM_E_IO, std::string("cannot open ") + path);
+  std::vector<std::string> lines;
+  std::string ln;
+  while (std::getline(f, ln)) lines.push_back(ln);
+  if (lines.size() < 75) return fail(KMM_E_IO, "dns.config has fewer than 75 lines");
+  auto get = [&](int lineno, double* v, int n = 1) -> bool { return parse_num(lines[lineno - 1], n, v); };
+  double v[3];
+  std::memset(cfg, 0, sizeof(*cfg));
+#define RD(line, dst, T)                                                        \
+  do {                                                                          \
+    if (!get(line, v)) return fail(KMM_E_IO, "dns.config line " #line);          \
+    cfg->dst = (T)v[0];                                                         \
+  } while (0)
+  RD(4, nsteps, int); RD(5, iprnfrq, int); RD(6, dt, double); RD(7, gain, double); RD(8, ugain, double);
+  RD(9, theta, double); RD(10, ny, int); RD(11, nz, int); RD(12, nx, int); RD(14, yl, double);
+  RD(15, zl, double); RD(16, xl, double); RD(17, re, double); RD(19, uinf, double); RD(20, kwall, int);
+  RD(21, perturbtime, int); RD(22, r_tau, double); RD(29, irstrt, int); RD(30, crstrt, int);
+  RD(34, flow_select, int); RD(38, scl_flag, int); RD(63, diff, double); RD(64, deltaT, double);
+  RD(66, src_start, int); RD(67, src_stop, int); RD(74, cadence, int);
+#undef RD
+  if (cfg->src_stop < cfg->src_start) cfg->src_stop = cfg->nsteps;  // main_dns.f90:1071-1075
+  cfg->scalar = 0;
+  return KMM_OK;
+}
+
+// ----------------------------------------------------------- layout ----
+// reference (nyp,nz,nxh) column-major <-> device [y][kx][kz]
+static void ref_to_dev(const kmm_ctx* c, const double* ref, std::vector<double2>& dev) {
+  dev.resize(c->S);
+  for (int k = 0; k < c->nxh; ++k)
+    for (int j = 0; j < c->nz; ++j)
+      for (int i = 0; i < c->nyp; ++i) {
+        const size_t r = (size_t)i + (size_t)c->nyp * ((size_t)j + (size_t)c->nz * k);
+        dev[((size_t)i * c->nxh + k) * c->nz + j] = make_double2(ref[2 * r], ref[2 * r + 1]);
+      }
+}
+static void dev_to_ref(const kmm_ctx* c, const std::vector<double2>& dev, double* ref) {
+  for (int k = 0; k < c->nxh; ++k)
+    for (int j = 0; j < c->nz; ++j)
+      for (int i = 0; i < c->nyp; ++i) {
+        const size_t r = (size_t)i + (size_t)c->nyp * ((size_t)j + (size_t)c->nz * k);
+        const double2 v = dev[((size_t)i * c->nxh + k) * c->nz + j];
+        ref[2 * r] = v.x;
+        ref[2 * r + 1] = v.y;
+      }
+}
+static int upload_field(kmm_ctx* c, double2* d, const double* ref) {
+  std::vector<double2> h;
+  ref_to_dev(c, ref, h);
+  KMM_HIP(hipMemcpyAsync(d, h.data(), c->S * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  KMM_HIP(hipStreamSynchronize(c->stream));
+  return KMM_OK;
+}
+static int download_field(kmm_ctx* c, const double2* d, double* ref) {
+  std::vector<double2> h(c->S);
+  KMM_HIP(hipMemcpyAsync(h.data(), d, c->S * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+  KMM_HIP(hipStreamSynchronize(c->stream));
+  dev_to_ref(c, h, ref);
+  return KMM_OK;
+}
+
+static std::vector<double2> twiddles(int n) {
+  std::vector<double2> t(n);
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int k = 0; k < n; ++k) {
+    const long double a = -2.0L * pi * (long double)k / (long double)n;
+    t[k] = make_double2((double)cosl(a), (double)sinl(a));
+  }
+  return t;
+}
+
+static bool fft_len_ok(int n) {
+  int m = n;
+  while (m % 2 == 0) m /= 2;
+  while (m % 3 == 0) m /= 3;
+  return m == 1;
+}
+
+// -------------------------------------------------------------- create --
+static PencilArgs pencil_args(kmm_ctx* c) {
+  PencilArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.nxh = c->nxh; a.nz = c->nz; a.nzh = c->nzh; a.P = c->P; a.ny = c->ny;
+  a.alpha = c->alpha; a.beta = c->beta; a.dyde = c->dyde; a.re = c->cfg.re; a.dt = c->cfg.dt;
+  a.theta = c->cfg.theta; a.diff = c->cfg.diff; a.gH = c->gH; a.gS = c->gS;
+  // setbcs (main_dns.f90:1240-1265): no-slip for channel / Couette
+  a.atu = 1.0; a.btu = 0.0; a.gtu = 0.0; a.abu = 1.0; a.bbu = 0.0; a.gbu = 0.0;
+  a.atw = 1.0; a.btw = 0.0; a.gtw = 0.0; a.abw = 1.0; a.bbw = 0.0; a.gbw = 0.0;
+  if (c->cfg.flow_select == 2 || c->cfg.flow_select == 12 || c->cfg.flow_select == 3) a.gtu = c->cfg.uinf;
+  if (c->cfg.flow_select == 3) a.gtw = c->cfg.uinf;
+  a.scalar = c->cfg.scalar;
+  a.smooth = 1;
+  a.first_nl = 1;
+  a.smz = c->smz; a.smy = c->smy; a.twM = c->twM;
+  a.tabH = c->tabH; a.tabP = c->tabP; a.tabS = c->tabS; a.G1 = c->G1; a.G2 = c->G2; a.gk = c->gk;
+  a.NL1 = c->NL[0]; a.NL2 = c->NL[1]; a.NL3 = c->NL[2]; a.NLS = c->NL[3];
+  a.V = c->V; a.OMY = c->OMY; a.FN = c->FN; a.FNM1 = c->FNM1; a.GN = c->GN; a.GNM1 = c->GNM1;
+  a.TH = c->TH; a.SCN = c->SCN; a.SCNM1 = c->SCNM1;
+  a.u0 = c->u0; a.w0 = c->w0; a.h1n = c->h1n; a.h1nm1 = c->h1nm1; a.h3n = c->h3n; a.h3nm1 = c->h3nm1;
+  for (int t = 0; t < 9; ++t) a.T[t] = c->T[t];
+  return a;
+}
+
+static PlaneArgs plane_args(kmm_ctx* c, int it, unsigned long long* cfl) {
+  PlaneArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.nxh = c->nxh; a.nz = c->nz; a.nzh = c->nzh; a.nyp = c->nyp; a.mx = c->mx; a.mz = c->mz; a.P = c->P;
+  a.nfin = c->cfg.scalar ? 9 : 6;
+  a.nfout = c->cfg.scalar ? 4 : 3;
+  a.dt = c->cfg.dt; a.dPdx = c->dPdx;
+  a.delxm = c->cfg.xl / c->mx; a.delzm = c->cfg.zl / c->mz;
+  a.src_uniform = (c->cfg.scalar && c->cfg.scl_flag == 6) ? c->cfg.deltaT : 0.0;
+  a.flow_select = c->cfg.flow_select;
+  a.seght = c->seght_d; a.twZ = c->twZ; a.twX = c->twX;
+  for (int f = 0; f < 9; ++f) a.in[f] = c->T[f];
+  a.Z = c->Z; a.ZF = c->ZF;
+  for (int o = 0; o < 4; ++o) a.out[o] = c->NL[o];
+  a.cfl = cfl;
+  a.nsrc = c->nsrc; a.src = c->src; a.fint = c->fint; a.drag = c->drag;
+  a.gain = c->cfg.gain; a.ugain = c->cfg.ugain; a.it = it; a.perturbtime = c->cfg.perturbtime;
+  a.trow = c->trow; a.tx = c->tx; a.tptr = c->tptr; a.ent_src = c->ent_src; a.ent_w = c->ent_w;
+  a.fixrow = c->fixrow; a.fixx = c->fixx;
+  return a;
+}
+
+extern "C" int kmm_create(const kmm_config* cfg, int device, kmm_ctx** out) {
+  if (!cfg || !out) return fail(KMM_E_ARG, "null argument");
+  *out = nullptr;
+  const int nx = cfg->nx, ny = cfg->ny, nz = cfg->nz;
+  if (nx <= 0 || ny <= 0 || nz <= 0 || nx % 2 || ny % 2 || nz % 2)
+    return fail(KMM_E_CONFIG, "grid sizes must be positive and even");
+  if (!(ny == 32 || ny == 48 || ny == 64 || ny == 96 || ny == 128))
+    return fail(KMM_E_CONFIG, "ny must be one of 32, 48, 64, 96, 128");
+  if ((nx / 2) % 8 != 0) return fail(KMM_E_CONFIG, "nx/2 must be a multiple of 8");
+  if (!fft_len_ok(3 * nx / 2) || !fft_len_ok(3 * nz / 2)) return fail(KMM_E_CONFIG, "3/2 grid not 2^a 3^b");
+  if (cfg->theta > 0.99) return fail(KMM_E_CONFIG, "THETA > 0.99 is not supported");
+  if (!(cfg->flow_select == 1 || cfg->flow_select == 2 || cfg->flow_select == 3 || cfg->flow_select == 11 ||
+        cfg->flow_select == 12))
+    return fail(KMM_E_CONFIG, "flow_select must be a wall-bounded case with nt=nb=1 (1,2,3,11,12)");
+  if (cfg->scalar && !(cfg->scl_flag == 0 || cfg->scl_flag == 1 || cfg->scl_flag == 3 || cfg->scl_flag == 6))
+    return fail(KMM_E_CONFIG, "scl_flag must be 0, 1, 3 or 6 (uniform source)");
+  kmm_ctx* c = new kmm_ctx();
+  c->cfg = *cfg;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) { delete c; return fail(KMM_E_HIP, "hipSetDevice failed"); }
+  if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return fail(KMM_E_HIP, "hipStreamCreate failed"); }
+  for (auto& e : c->ev) hipEventCreate(&e);
+  c->nx = nx; c->ny = ny; c->nz = nz; c->nyp = ny + 1; c->nxh = nx / 2; c->nzh = nz / 2;
+  c->mx = 3 * nx / 2; c->mz = 3 * nz / 2; c->P = c->nxh * nz;
+  c->S = (size_t)c->nyp * c->P;
+  const double pi = 2.0 * std::acos(0.0);
+  c->alpha = 2.0 * pi / cfg->xl;
+  c->beta = 2.0 * pi / cfg->zl;
+  c->dyde = 2.0 / cfg->yl;
+  c->dPdx = -2.0 / cfg->yl * std::pow(2.0 * cfg->r_tau / (cfg->yl * cfg->re), 2);
+  c->gH = cfg->re / (cfg->dt * cfg->theta);
+  c->gS = cfg->re * cfg->diff / (cfg->dt * cfg->theta);
+  c->ycoord.resize(c->nyp);
+  c->seght.resize(c->nyp);
+  c->ycoord[0] = cfg->yl / 2.0;
+  for (int k = 1; k < c->nyp; ++k) {
+    c->ycoord[k] = std::cos((double)k * pi / (double)ny) * (cfg->yl / 2.0);
+    c->seght[k] = std::fabs(c->ycoord[k] - c->ycoord[k - 1]);
+  }
+  c->seght[0] = c->seght[c->nyp - 1];
+  int r = 0;
+  {
+    const size_t fb = c->S * sizeof(double2);
+    double2** st[] = {&c->V, &c->OMY, &c->FN, &c->FNM1, &c->GN, &c->GNM1, &c->TH, &c->SCN, &c->SCNM1};
+    for (auto p : st) if ((r = dalloc(c, (void**)p, fb))) goto fail_;
+    for (auto p : {&c->u0, &c->w0, &c->h1n, &c->h1nm1, &c->h3n, &c->h3nm1})
+      if ((r = dalloc(c, (void**)p, c->nyp * sizeof(double)))) goto fail_;
+    for (int i = 0; i < 4; ++i) if ((r = dalloc(c, (void**)&c->NL[i], fb))) goto fail_;
+    for (int i = 0; i < 9; ++i) if ((r = dalloc(c, (void**)&c->T[i], fb))) goto fail_;
+    const size_t zb = (size_t)c->nyp * c->mz * c->nxh * sizeof(double2);
+    if ((r = dalloc(c, (void**)&c->Z, 9 * zb))) goto fail_;
+    if ((r = dalloc(c, (void**)&c->ZF, 4 * zb))) goto fail_;
+    const size_t tb = (size_t)c->P * 65 * sizeof(double2);
+    for (auto p : {&c->tabH, &c->tabP, &c->tabS}) if ((r = dalloc(c, (void**)p, tb))) goto fail_;
+    const size_t gb = (size_t)c->P * (ny / 2 + 1) * sizeof(double2);
+    for (auto p : {&c->G1, &c->G2}) if ((r = dalloc(c, (void**)p, gb))) goto fail_;
+    if ((r = dalloc(c, (void**)&c->gk, (size_t)c->P * sizeof(double4)))) goto fail_;
+    if ((r = dalloc(c, (void**)&c->cfl_d, 4096 * sizeof(unsigned long long)))) goto fail_;
+    c->cfl_cap = 4096;
+    // twiddles and smoothing
+    auto up = [&](void** d, const void* h, size_t b) -> int {
+      int rr = dalloc(c, d, b);
+      if (rr) return rr;
+      KMM_HIP(hipMemcpyAsync(*d, h, b, hipMemcpyHostToDevice, c->stream));
+      return KMM_OK;
+    };
+    auto tM = twiddles(2 * ny), tZ = twiddles(c->mz), tX = twiddles(c->mx);
+    if ((r = up((void**)&c->twM, tM.data(), tM.size() * sizeof(double2)))) goto fail_;
+    if ((r = up((void**)&c->twZ, tZ.data(), tZ.size() * sizeof(double2)))) goto fail_;
+    if ((r = up((void**)&c->twX, tX.data(), tX.size() * sizeof(double2)))) goto fail_;
+    std::vector<double> smz(nz), smy(c->nyp);
+    for (int j = 0; j < nz; ++j) {  // main_dns.f90:712-717
+      const int jj = (j < c->nzh) ? j : nz - j;
+      smz[j] = std::exp(-3.0 * std::pow((double)jj / (double)c->nzh, 10));
+    }
+    for (int i = 0; i < c->nyp; ++i) smy[i] = std::exp(-3.0 * std::pow((double)i / (double)ny, 10));
+    if ((r = up((void**)&c->smz, smz.data(), nz * sizeof(double)))) goto fail_;
+    if ((r = up((void**)&c->smy, smy.data(), c->nyp * sizeof(double)))) goto fail_;
+    if ((r = up((void**)&c->seght_d, c->seght.data(), c->nyp * sizeof(double)))) goto fail_;
+    for (int d = 0; d < 3; ++d) c->fint_full[d].assign((size_t)c->nyp * c->mz * c->mx, 0.0);
+    // tau tables and Green's functions (gfcn, main_dns.f90:151-161)
+    PencilArgs a = pencil_args(c);
+    hipError_t e;
+    if ((e = kmm::launch_tau_tables(ny, a, c->gH, c->tabH, c->stream)) != hipSuccess) { r = fail(KMM_E_HIP, hipGetErrorString(e)); goto fail_; }
+    if ((e = kmm::launch_tau_tables(ny, a, 0.0, c->tabP, c->stream)) != hipSuccess) { r = fail(KMM_E_HIP, hipGetErrorString(e)); goto fail_; }
+    if ((e = kmm::launch_tau_tables(ny, a, c->gS, c->tabS, c->stream)) != hipSuccess) { r = fail(KMM_E_HIP, hipGetErrorString(e)); goto fail_; }
+    // p1b=1, p1t=-1, p2b=-1, p2t=0.5 (setbcs, main_dns.f90:1406-1409)
+    if ((e = kmm::launch_green(ny, a, 1.0, -1.0, -1.0, 0.5, c->G1, c->G2, c->gk, c->stream)) != hipSuccess) { r = fail(KMM_E_HIP, hipGetErrorString(e)); goto fail_; }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) { r = fail(KMM_E_HIP, "setup kernels failed"); goto fail_; }
+  }
+  *out = c;
+  return KMM_OK;
+fail_:
+  kmm_destroy(c);
+  return r;
+}
+
+extern "C" int kmm_destroy(kmm_ctx* c) {
+  if (!c) return KMM_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) hipFree(p);
+  for (auto& e : c->ev) if (e) hipEventDestroy(e);
+  for (auto& e : c->tev) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return KMM_OK;
+}
+
+// ---------------------------------------------------------- geometry ----
+// Solid cells (imatrix 1 or 4) carry Goldstein feedback; their force is
+// spread with exp(-2 dsq) over a 5x5 stencil (main_dns.f90:1550-1558) on the
+// planes i <= kmaxsurf (quirk: kmaxsurf is the largest solid x index,
+// main_dns.f90:1526-1535).  The reference walks the (mz,mx) plane
+// column-major and assigns the product at each point before spreading, so a
+// contribution survives only at linear addresses not after its source
+// (quirk Q6); z offsets are not wrapped (quirk Q5).
+extern "C" int kmm_set_geometry(kmm_ctx* c, const int32_t* im) {
+  if (!c || !im) return fail(KMM_E_ARG, "null argument");
+  const int nyp = c->nyp, mz = c->mz, mx = c->mx;
+  auto at = [&](int i, int j, int k) { return im[(size_t)i + (size_t)nyp * ((size_t)j + (size_t)mz * k)]; };
+  int kmax = 0;
+  std::vector<int4> src;
+  std::vector<int> srcid((size_t)nyp * mz * mx, -1);
+  for (int k = 0; k < mx; ++k)
+    for (int j = 0; j < mz; ++j)
+      for (int i = 0; i < nyp; ++i) {
+        const int t = at(i, j, k);
+        if (t != 0 && t != 1 && t != 4)
+          return fail(KMM_E_CONFIG, "imatrix types other than 0, 1, 4 are not supported");
+        if (t == 1 || t == 3 || t == 4) kmax = std::max(kmax, k + 1);
+      }
+  // sources ordered by (y, linear in-plane address)
+  for (int i = 0; i < nyp; ++i)
+    for (int k = 0; k < mx; ++k)
+      for (int j = 0; j < mz; ++j) {
+        const int t = at(i, j, k);
+        if (t == 1 || t == 4) {
+          srcid[(size_t)i * mz * mx + (size_t)k * mz + j] = (int)src.size();
+          src.push_back(make_int4(i, j, k, t));
+        }
+      }
+  struct Ent { int y, z, x, s; double w; };
+  std::vector<Ent> ents;
+  for (size_t s = 0; s < src.size(); ++s) {
+    const int i = src[s].x, j = src[s].y, k = src[s].z;
+    if (i + 1 > kmax) continue;
+    const long ls = (long)k * mz + j;
+    for (int ii = -2; ii <= 2; ++ii) {
+      int ip = k + ii;
+      if (ip < 0) ip += mx;
+      if (ip >= mx) ip -= mx;
+      for (int jj = -2; jj <= 2; ++jj) {
+        const long lt = (long)ip * mz + (j + jj);
+        if (lt < 0 || lt >= (long)mz * mx)
+          return fail(KMM_E_CONFIG, "IB spreading leaves the (mz,mx) plane (solid cell at the x=1/mx edge)");
+        if (lt > ls) continue;
+        const double w = std::exp(-2.0 * (double)(ii * ii + jj * jj));
+        ents.push_back({i, (int)(lt % mz), (int)(lt / mz), (int)s, w});
+      }
+    }
+  }
+  std::stable_sort(ents.begin(), ents.end(), [](const Ent& a, const Ent& b) {
+    if (a.y != b.y) return a.y < b.y;
+    if (a.z != b.z) return a.z < b.z;
+    return a.x < b.x;
+  });
+  std::vector<int> trow((size_t)nyp * mz + 1, 0), tx, tptr, esrc;
+  std::vector<double> ew;
+  {
+    size_t q = 0;
+    for (int row = 0; row < nyp * mz; ++row) {
+      trow[row] = (int)tx.size();
+      const int y = row / mz, z = row % mz;
+      while (q < ents.size() && ents[q].y == y && ents[q].z == z) {
+        const int x = ents[q].x;
+        tx.push_back(x);
+        tptr.push_back((int)esrc.size());
+        while (q < ents.size() && ents[q].y == y && ents[q].z == z && ents[q].x == x) {
+          esrc.push_back(ents[q].s);
+          ew.push_back(ents[q].w);
+          ++q;
+        }
+      }
+    }
+    trow[(size_t)nyp * mz] = (int)tx.size();
+    tptr.push_back((int)esrc.size());
+  }
+  std::vector<int> fixrow((size_t)nyp * mz + 1, 0), fixx;
+  for (int row = 0; row < nyp * mz; ++row) {
+    fixrow[row] = (int)fixx.size();
+    const int y = row / mz, z = row % mz;
+    for (int k = 0; k < mx; ++k)
+      if (at(y, z, k) != 0) fixx.push_back(k);
+  }
+  fixrow[(size_t)nyp * mz] = (int)fixx.size();
+  auto up = [&](void** d, const void* h, size_t b) -> int {
+    int rr = dalloc(c, d, b);
+    if (rr) return rr;
+    if (b) KMM_HIP(hipMemcpyAsync(*d, h, b, hipMemcpyHostToDevice, c->stream));
+    return KMM_OK;
+  };
+  int r;
+  c->nsrc = (int)src.size();
+  c->src_h = src;
+  if ((r = up((void**)&c->src, src.data(), src.size() * sizeof(int4)))) return r;
+  if ((r = dalloc(c, (void**)&c->fint, 3 * src.size() * sizeof(double)))) return r;
+  if ((r = dalloc(c, (void**)&c->drag, 3 * src.size() * sizeof(double)))) return r;
+  if ((r = up((void**)&c->trow, trow.data(), trow.size() * sizeof(int)))) return r;
+  if ((r = up((void**)&c->tx, tx.data(), tx.size() * sizeof(int)))) return r;
+  if ((r = up((void**)&c->tptr, tptr.data(), tptr.size() * sizeof(int)))) return r;
+  if ((r = up((void**)&c->ent_src, esrc.data(), esrc.size() * sizeof(int)))) return r;
+  if ((r = up((void**)&c->ent_w, ew.data(), ew.size() * sizeof(double)))) return r;
+  if ((r = up((void**)&c->fixrow, fixrow.data(), fixrow.size() * sizeof(int)))) return r;
+  if ((r = up((void**)&c->fixx, fixx.data(), fixx.size() * sizeof(int)))) return r;
+  // load integrals of the solid cells from the host copies
+  std::vector<double> fi(3 * src.size());
+  for (int d = 0; d < 3; ++d)
+    for (size_t s = 0; s < src.size(); ++s) {
+      const int4 q = src[s];
+      fi[d * src.size() + s] = c->fint_full[d][(size_t)q.x + (size_t)nyp * ((size_t)q.y + (size_t)mz * q.z)];
+    }
+  if (!fi.empty()) KMM_HIP(hipMemcpyAsync(c->fint, fi.data(), fi.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  KMM_HIP(hipStreamSynchronize(c->stream));
+  return KMM_OK;
+}
+
+static int sync_fint_to_host(kmm_ctx* c) {
+  if (c->nsrc == 0) return KMM_OK;
+  std::vector<double> fi(3 * (size_t)c->nsrc);
+  KMM_HIP(hipMemcpyAsync(fi.data(), c->fint, fi.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  KMM_HIP(hipStreamSynchronize(c->stream));
+  for (int d = 0; d < 3; ++d)
+    for (int s = 0; s < c->nsrc; ++s) {
+      const int4 q = c->src_h[s];
+      c->fint_full[d][(size_t)q.x + (size_t)c->nyp * ((size_t)q.y + (size_t)c->mz * q.z)] = fi[(size_t)d * c->nsrc + s];
+    }
+  return KMM_OK;
+}
+static int sync_fint_to_dev(kmm_ctx* c) {
+  if (c->nsrc == 0) return KMM_OK;
+  std::vector<double> fi(3 * (size_t)c->nsrc);
+  for (int d = 0; d < 3; ++d)
+    for (int s = 0; s < c->nsrc; ++s) {
+      const int4 q = c->src_h[s];
+      fi[(size_t)d * c->nsrc + s] = c->fint_full[d][(size_t)q.x + (size_t)c->nyp * ((size_t)q.y + (size_t)c->mz * q.z)];
+    }
+  KMM_HIP(hipMemcpyAsync(c->fint, fi.data(), fi.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  KMM_HIP(hipStreamSynchronize(c->stream));
+  return KMM_OK;
+}
+
+// ------------------------------------------------------------ fields ----
+static double2* spectral_ptr(kmm_ctx* c, int f) {
+  switch (f) {
+    case KMM_V: return c->V;
+    case KMM_FN: return c->FN;
+    case KMM_FNM1: return c->FNM1;
+    case KMM_OMY: return c->OMY;
+    case KMM_GN: return c->GN;
+    case KMM_GNM1: return c->GNM1;
+    case KMM_SCALAR: return c->TH;
+    case KMM_SCN: return c->SCN;
+    case KMM_SCNM1: return c->SCNM1;
+    default: return nullptr;
+  }
+}
+static double* mean_ptr(kmm_ctx* c, int f) {
+  switch (f) {
+    case KMM_U0: return c->u0;
+    case KMM_H1N: return c->h1n;
+    case KMM_H1NM1: return c->h1nm1;
+    case KMM_W0: return c->w0;
+    case KMM_H3N: return c->h3n;
+    case KMM_H3NM1: return c->h3nm1;
+    default: return nullptr;
+  }
+}
+
+static int materialize(kmm_ctx* c);
+
+extern "C" int kmm_put_field(kmm_ctx* c, int f, const double* h) {
+  if (!c || !h) return fail(KMM_E_ARG, "null argument");
+  KMM_HIP(hipSetDevice(c->device));
+  if (c->nl_valid && (f == KMM_FN || f == KMM_GN || f == KMM_SCN || f == KMM_H1N || f == KMM_H3N)) {
+    int r = materialize(c);
+    if (r) return r;
+    c->nl_valid = false;
+  }
+  if (double2* d = spectral_ptr(c, f)) return upload_field(c, d, h);
+  if (double* m = mean_ptr(c, f)) {
+    KMM_HIP(hipMemcpyAsync(m, h, c->nyp * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    KMM_HIP(hipStreamSynchronize(c->stream));
+    return KMM_OK;
+  }
+  if (f >= KMM_FXINTG && f <= KMM_FZINTG) {
+    std::memcpy(c->fint_full[f - KMM_FXINTG].data(), h, c->fint_full[0].size() * sizeof(double));
+    return sync_fint_to_dev(c);
+  }
+  return fail(KMM_E_ARG, "unknown field id");
+}
+
+extern "C" int kmm_get_field(kmm_ctx* c, int f, double* h) {
+  if (!c || !h) return fail(KMM_E_ARG, "null argument");
+  KMM_HIP(hipSetDevice(c->device));
+  if (c->nl_valid && (f == KMM_FN || f == KMM_GN || f == KMM_SCN || f == KMM_H1N || f == KMM_H3N)) {
+    int r = materialize(c);
+    if (r) return r;
+  }
+  if (double2* d = spectral_ptr(c, f)) return download_field(c, d, h);
+  if (double* m = mean_ptr(c, f)) {
+    KMM_HIP(hipMemcpyAsync(h, m, c->nyp * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    KMM_HIP(hipStreamSynchronize(c->stream));
+    return KMM_OK;
+  }
+  if (f >= KMM_FXINTG && f <= KMM_FZINTG) {
+    int r = sync_fint_to_host(c);
+    if (r) return r;
+    std::memcpy(h, c->fint_full[f - KMM_FXINTG].data(), c->fint_full[0].size() * sizeof(double));
+    return KMM_OK;
+  }
+  return fail(KMM_E_ARG, "unknown field id");
+}
+
+// -------------------------------------------------------------- step ----
+static int run_plane(kmm_ctx* c, int it, unsigned long long* cfl, hipEvent_t* ev = nullptr) {
+  PlaneArgs pa = plane_args(c, it, cfl);
+  hipError_t e = ev ? kmm::launch_plane_timed(pa, c->stream, ev) : kmm::launch_plane(pa, c->stream);
+  if (e != hipSuccess) return fail(KMM_E_HIP, std::string("plane stage: ") + hipGetErrorString(e));
+  return KMM_OK;
+}
+
+// NL buffers -> FN, GN, SCN, h1n, h3n (smoothed), without touching the
+// rest of the state.
+static int materialize(kmm_ctx* c) {
+  PencilArgs a = pencil_args(c);
+  a.smooth = 1;
+  hipError_t e = kmm::launch_pencil_finish(c->ny, a, c->stream);
+  if (e != hipSuccess) return fail(KMM_E_HIP, std::string("pencil finish: ") + hipGetErrorString(e));
+  KMM_HIP(hipStreamSynchronize(c->stream));
+  return KMM_OK;
+}
+
+static int check_cfl(kmm_ctx* c, int n, int it0) {
+  std::vector<unsigned long long> h(n);
+  KMM_HIP(hipMemcpyAsync(h.data(), c->cfl_d, n * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+  KMM_HIP(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < n; ++i) {
+    double v;
+    std::memcpy(&v, &h[i], sizeof(double));
+    c->last_cfl = v;
+    c->last_cfl_it = it0 + i;
+    if (v > 1.0) {
+      char msg[128];
+      std::snprintf(msg, sizeof msg, "CFL failure at time step %d: %g", it0 + i, v);
+      g_last_error = msg;
+      return KMM_E_CFL;
+    }
+  }
+  return KMM_OK;
+}
+
+extern "C" int kmm_startup(kmm_ctx* c) {
+  if (!c) return fail(KMM_E_ARG, "null argument");
+  KMM_HIP(hipSetDevice(c->device));
+  PencilArgs a = pencil_args(c);
+  hipError_t e = kmm::launch_pencil_startup(c->ny, a, c->stream);
+  if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
+  KMM_HIP(hipMemsetAsync(c->cfl_d, 0, sizeof(unsigned long long), c->stream));
+  c->it = 0;
+  int r = run_plane(c, 0, c->cfl_d);
+  if (r) return r;
+  a.smooth = 0;  // initial() applies no smoothing (main_dns.f90:2155-2206)
+  e = kmm::launch_pencil_finish(c->ny, a, c->stream);
+  if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
+  const size_t fb = c->S * sizeof(double2);
+  KMM_HIP(hipMemcpyAsync(c->FNM1, c->FN, fb, hipMemcpyDeviceToDevice, c->stream));
+  KMM_HIP(hipMemcpyAsync(c->GNM1, c->GN, fb, hipMemcpyDeviceToDevice, c->stream));
+  if (c->cfg.scalar) KMM_HIP(hipMemcpyAsync(c->SCNM1, c->SCN, fb, hipMemcpyDeviceToDevice, c->stream));
+  KMM_HIP(hipMemcpyAsync(c->h1nm1, c->h1n, c->nyp * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  KMM_HIP(hipMemcpyAsync(c->h3nm1, c->h3n, c->nyp * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  c->nl_valid = false;
+  return check_cfl(c, 1, 0);
+}
+
+extern "C" int kmm_step(kmm_ctx* c, int nsteps) {
+  if (!c || nsteps < 0) return fail(KMM_E_ARG, "bad argument");
+  KMM_HIP(hipSetDevice(c->device));
+  int done = 0;
+  c->t_pencil = c->t_plane = 0.0;
+  for (double& t : c->ktime) t = 0.0;
+  c->ksamples = 0;
+  while (done < nsteps) {
+    const int n = std::min(nsteps - done, c->cfl_cap);
+    const int it0 = c->it + 1;
+    if (c->timing) {
+      const size_t need = (size_t)n * 6;
+      while (c->tev.size() < need) {
+        hipEvent_t e;
+        KMM_HIP(hipEventCreate(&e));
+        c->tev.push_back(e);
+      }
+    }
+    KMM_HIP(hipMemsetAsync(c->cfl_d, 0, n * sizeof(unsigned long long), c->stream));
+    for (int s = 0; s < n; ++s) {
+      c->it += 1;
+      PencilArgs a = pencil_args(c);
+      a.first_nl = c->nl_valid ? 1 : 0;
+      hipEvent_t* ev = c->timing ? &c->tev[(size_t)s * 6] : nullptr;
+      if (ev) hipEventRecord(ev[0], c->stream);
+      else if (s == 0) hipEventRecord(c->ev[0], c->stream);
+      hipError_t e = kmm::launch_pencil_step(c->ny, a, c->stream);
+      if (e != hipSuccess) return fail(KMM_E_HIP, std::string("pencil step: ") + hipGetErrorString(e));
+      if (ev) hipEventRecord(ev[1], c->stream);
+      else if (s == 0) hipEventRecord(c->ev[1], c->stream);
+      int r = run_plane(c, c->it, c->cfl_d + s, ev ? ev + 2 : nullptr);
+      if (r) return r;
+      if (!ev && s == 0) hipEventRecord(c->ev[2], c->stream);
+      c->nl_valid = true;
+    }
+    int r = check_cfl(c, n, it0);
+    if (c->timing) {
+      for (int s = 0; s < n; ++s) {
+        hipEvent_t* ev = &c->tev[(size_t)s * 6];
+        for (int k = 0; k < 5; ++k) {
+          float t = 0.0f;
+          hipEventElapsedTime(&t, ev[k], ev[k + 1]);
+          c->ktime[k] += t;
+        }
+      }
+      c->ksamples += n;
+    } else {
+      float t1 = 0, t2 = 0;
+      hipEventElapsedTime(&t1, c->ev[0], c->ev[1]);
+      hipEventElapsedTime(&t2, c->ev[1], c->ev[2]);
+      c->t_pencil = t1;
+      c->t_plane = t2;
+    }
+    if (r) return r;
+    done += n;
+  }
+  return KMM_OK;
+}
+
+extern "C" int kmm_set_timing(kmm_ctx* c, int on) {
+  if (!c) return fail(KMM_E_ARG, "null argument");
+  c->timing = on ? 1 : 0;
+  return KMM_OK;
+}
+
+extern "C" int kmm_kernel_times(kmm_ctx* c, double* ms, int* nsamples) {
+  if (!c || !ms) return fail(KMM_E_ARG, "null argument");
+  for (int k = 0; k < 5; ++k) ms[k] = c->ksamples ? c->ktime[k] / c->ksamples : 0.0;
+  if (nsamples) *nsamples = c->ksamples;
+  return KMM_OK;
+}
+
+extern "C" int kmm_last_cfl(kmm_ctx* c, double* cflmax, int* it) {
+  if (!c) return fail(KMM_E_ARG, "null argument");
+  if (cflmax) *cflmax = c->last_cfl;
+  if (it) *it = c->last_cfl_it;
+  return KMM_OK;
+}
+
+extern "C" int kmm_stage_times(kmm_ctx* c, double* p, double* q) {
+  if (!c) return fail(KMM_E_ARG, "null argument");
+  if (p) *p = c->t_pencil;
+  if (q) *q = c->t_plane;
+  return KMM_OK;
+}
+
+// ----------------------------------------------------------- restart ----
+// Fortran unformatted sequential: 4-byte little-endian length before and
+// after each record.
+static bool write_record(FILE* f, const std::vector<std::pair<const void*, size_t>>& parts) {
+  size_t tot = 0;
+  for (auto& p : parts) tot += p.second;
+  if (tot > 0x7fffffffu) return false;
+  const uint32_t m = (uint32_t)tot;
+  if (fwrite(&m, 4, 1, f) != 1) return false;
+  for (auto& p : parts)
+    if (p.second && fwrite(p.first, 1, p.second, f) != p.second) return false;
+  return fwrite(&m, 4, 1, f) == 1;
+}
+static bool read_record(FILE* f, const std::vector<std::pair<void*, size_t>>& parts) {
+  uint32_t m = 0, m2 = 0;
+  if (fread(&m, 4, 1, f) != 1) return false;
+  size_t tot = 0;
+  for (auto& p : parts) tot += p.second;
+  if (m != tot) return false;
+  for (auto& p : parts)
+    if (p.second && fread(p.first, 1, p.second, f) != p.second) return false;
+  if (fread(&m2, 4, 1, f) != 1) return false;
+  return m2 == m;
+}
+
+extern "C" int kmm_write_restart(kmm_ctx* c, const char* path) {
+  if (!c || !path) return fail(KMM_E_ARG, "null argument");
+  const size_t fs = c->S * 2, ms = c->nyp;
+  std::vector<double> buf(6 * fs + 6 * ms);
+  const int ids[6] = {KMM_V, KMM_FN, KMM_FNM1, KMM_OMY, KMM_GN, KMM_GNM1};
+  for (int q = 0; q < 6; ++q) {
+    int r = kmm_get_field(c, ids[q], buf.data() + q * fs);
+    if (r) return r;
+  }
+  const int mids[6] = {KMM_U0, KMM_H1N, KMM_H1NM1, KMM_W0, KMM_H3N, KMM_H3NM1};
+  for (int q = 0; q < 6; ++q) {
+    int r = kmm_get_field(c, mids[q], buf.data() + 6 * fs + q * ms);
+    if (r) return r;
+  }
+  int r = sync_fint_to_host(c);
+  if (r) return r;
+  FILE* f = fopen(path, "wb");
+  if (!f) return fail(KMM_E_IO, std::string("cannot write ") + path);
+  const size_t pb = c->fint_full[0].size() * sizeof(double);
+  bool ok = write_record(f, {{buf.data(), buf.size() * sizeof(double)}}) &&
+            write_record(f, {{c->fint_full[0].data(), pb}, {c->fint_full[1].data(), pb}, {c->fint_full[2].data(), pb}});
+  fclose(f);
+  return ok ? KMM_OK : fail(KMM_E_IO, "restart write failed");
+}
+
+extern "C" int kmm_read_restart(kmm_ctx* c, const char* path) {
+  if (!c || !path) return fail(KMM_E_ARG, "null argument");
+  const size_t fs = c->S * 2, ms = c->nyp;
+  std::vector<double> buf(6 * fs + 6 * ms);
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(KMM_E_IO, std::string("cannot open ") + path);
+  std::vector<double> p0(c->fint_full[0].size()), p1(p0.size()), p2(p0.size());
+  const size_t pb = p0.size() * sizeof(double);
+  bool ok = read_record(f, {{buf.data(), buf.size() * sizeof(double)}}) &&
+            read_record(f, {{p0.data(), pb}, {p1.data(), pb}, {p2.data(), pb}});
+  fclose(f);
+  if (!ok) return fail(KMM_E_IO, std::string("restart file ") + path + " has the wrong size or record markers");
+  c->nl_valid = false;
+  const int ids[6] = {KMM_V, KMM_FN, KMM_FNM1, KMM_OMY, KMM_GN, KMM_GNM1};
+  for (int q = 0; q < 6; ++q) {
+    int r = kmm_put_field(c, ids[q], buf.data() + q * fs);
+    if (r) return r;
+  }
+  const int mids[6] = {KMM_U0, KMM_H1N, KMM_H1NM1, KMM_W0, KMM_H3N, KMM_H3NM1};
+  for (int q = 0; q < 6; ++q) {
+    int r = kmm_put_field(c, mids[q], buf.data() + 6 * fs + q * ms);
+    if (r) return r;
+  }
+  c->fint_full[0] = p0;
+  c->fint_full[1] = p1;
+  c->fint_full[2] = p2;
+  c->it = 0;
+  return sync_fint_to_dev(c);
+}
+
+extern "C" int kmm_write_crestart(kmm_ctx* c, const char* path) {
+  if (!c || !path) return fail(KMM_E_ARG, "null argument");
+  if (!c->cfg.scalar) return fail(KMM_E_CONFIG, "no scalar in this configuration");
+  const size_t fs = c->S * 2;
+  std::vector<double> a(fs), b(fs), d(fs);
+  int r;
+  if ((r = kmm_get_field(c, KMM_SCALAR, a.data())) || (r = kmm_get_field(c, KMM_SCN, b.data())) ||
+      (r = kmm_get_field(c, KMM_SCNM1, d.data())))
+    return r;
+  FILE* f = fopen(path, "wb");
+  if (!f) return fail(KMM_E_IO, std::string("cannot write ") + path);
+  const size_t nb = fs * sizeof(double);
+  bool ok = write_record(f, {{a.data(), nb}}) && write_record(f, {{b.data(), nb}}) && write_record(f, {{d.data(), nb}});
+  fclose(f);
+  return ok ? KMM_OK : fail(KMM_E_IO, "c-restart write failed");
+}
+
+extern "C" int kmm_read_crestart(kmm_ctx* c, const char* path) {
+  if (!c || !path) return fail(KMM_E_ARG, "null argument");
+  if (!c->cfg.scalar) return fail(KMM_E_CONFIG, "no scalar in this configuration");
+  const size_t fs = c->S * 2;
+  std::vector<double> a(fs), b(fs), d(fs);
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(KMM_E_IO, std::string("cannot open ") + path);
+  const size_t nb = fs * sizeof(double);
+  bool ok = read_record(f, {{a.data(), nb}}) && read_record(f, {{b.data(), nb}}) && read_record(f, {{d.data(), nb}});
+  fclose(f);
+  if (!ok) return fail(KMM_E_IO, std::string("c-restart file ") + path + " is malformed");
+  int r;
+  if ((r = kmm_put_field(c, KMM_SCALAR, a.data())) || (r = kmm_put_field(c, KMM_SCN, b.data())) ||
+      (r = kmm_put_field(c, KMM_SCNM1, d.data())))
+    return r;
+  return KMM_OK;
+}
+
+// ------------------------------------------------- per-routine entries --
+extern "C" int kmm_cderiv(kmm_ctx* c, const double* f, double* df) {
+  if (!c || !f || !df) return fail(KMM_E_ARG, "null argument");
+  KMM_HIP(hipSetDevice(c->device));
+  int r = upload_field(c, c->T[0], f);
+  if (r) return r;
+  hipError_t e = kmm::launch_cderiv(c->ny, c->P, c->dyde, c->T[0], c->T[1], c->stream);
+  if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
+  return download_field(c, c->T[1], df);
+}
+
+// one kx slab: u complex (nyp,nz) -> device [y][kz] with P = nz pencils
+static int slab_solve(kmm_ctx* c, double* u, double x, double g, double dyde, const double* bt, const double* bb,
+                      int ib, bool penta, double at_, double btt, double ab_, double bbb) {
+  KMM_HIP(hipSetDevice(c->device));
+  const int nz = c->nz, nyp = c->nyp;
+  std::vector<double2> h((size_t)nyp * nz);
+  for (int j = 0; j < nz; ++j)
+    for (int i = 0; i < nyp; ++i) {
+      const size_t r = (size_t)i + (size_t)nyp * j;
+      h[(size_t)i * nz + j] = make_double2(u[2 * r], u[2 * r + 1]);
+    }
+  std::vector<double> wn(nz);
+  std::vector<int> kz(nz, 0);
+  bool anyk = false;
+  for (int j = 0; j < nz; ++j) {
+    const int js = (j < c->nzh) ? j : j - nz;
+    wn[j] = (js * c->beta) * (js * c->beta) + x + g;
+    if (ib != 0 && wn[j] <= 1.0e-20) { wn[j] = 1.0; kz[j] = 1; anyk = true; }
+  }
+  std::vector<double2> bth(nz), bbh(nz);
+  for (int j = 0; j < nz; ++j) {
+    bth[j] = make_double2(bt[2 * j], bt[2 * j + 1]);
+    bbh[j] = make_double2(bb[2 * j], bb[2 * j + 1]);
+  }
+  double2 *du, *dtab, *dbt, *dbb;
+  double* dwn;
+  int* dkz;
+  KMM_HIP(hipMalloc(&du, h.size() * sizeof(double2)));
+  KMM_HIP(hipMalloc(&dtab, (size_t)nz * 65 * sizeof(double2)));
+  KMM_HIP(hipMalloc(&dbt, nz * sizeof(double2)));
+  KMM_HIP(hipMalloc(&dbb, nz * sizeof(double2)));
+  KMM_HIP(hipMalloc(&dwn, nz * sizeof(double)));
+  KMM_HIP(hipMalloc(&dkz, nz * sizeof(int)));
+  KMM_HIP(hipMemcpyAsync(du, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  KMM_HIP(hipMemcpyAsync(dbt, bth.data(), nz * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  KMM_HIP(hipMemcpyAsync(dbb, bbh.data(), nz * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  KMM_HIP(hipMemcpyAsync(dwn, wn.data(), nz * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  KMM_HIP(hipMemcpyAsync(dkz, kz.data(), nz * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  hipError_t e = kmm::launch_tau_tables_wn(c->ny, nz, dyde, dwn, dtab, c->stream);
+  PencilArgs a = pencil_args(c);
+  a.P = nz;
+  a.dyde = dyde;
+  if (e == hipSuccess) {
+    if (!penta)
+      e = kmm::launch_solve(c->ny, a, dtab, dwn, du, dbt, dbb, ib, anyk ? dkz : nullptr, c->stream);
+    else
+      e = kmm::launch_penta(c->ny, a, dtab, dwn, du, at_, btt, dbt, ab_, bbb, dbb, +1, anyk ? dkz : nullptr,
+                            c->stream);
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h.data(), du, h.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(du); hipFree(dtab); hipFree(dbt); hipFree(dbb); hipFree(dwn); hipFree(dkz);
+  if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
+  for (int j = 0; j < nz; ++j)
+    for (int i = 0; i < nyp; ++i) {
+      const size_t r = (size_t)i + (size_t)nyp * j;
+      u[2 * r] = h[(size_t)i * nz + j].x;
+      u[2 * r + 1] = h[(size_t)i * nz + j].y;
+    }
+  return KMM_OK;
+}
+
+extern "C" int kmm_solve(kmm_ctx* c, double* u, double x, double g, double dyde, const double* bctop,
+                         const double* bcbot, int ib) {
+  if (!c || !u || !bctop || !bcbot) return fail(KMM_E_ARG, "null argument");
+  return slab_solve(c, u, x, g, dyde, bctop, bcbot, ib, false, 0, 0, 0, 0);
+}
+
+extern "C" int kmm_penta(kmm_ctx* c, double* u, double x, double g, double dyde, int ib, double at, double bt,
+                         const double* gtop, double ab, double bb, const double* gbot) {
+  if (!c || !u || !gtop || !gbot) return fail(KMM_E_ARG, "null argument");
+  return slab_solve(c, u, x, g, dyde, gtop, gbot, ib, true, at, bt, ab, bb);
+}
+
+extern "C" int kmm_vcw3dp(kmm_ctx* c, const double* u, const double* v, const double* w, const double* omx,
+                          const double* omy, const double* omz, const double* sclx, const double* scly,
+                          const double* sclz, double* gn, double* fn, double* omz_out, double* scn, int it) {
+  if (!c || !u || !v || !w || !omx || !omy || !omz || !gn || !fn || !omz_out) return fail(KMM_E_ARG, "null argument");
+  const bool sc = c->cfg.scalar != 0;
+  if (sc && (!sclx || !scly || !sclz || !scn)) return fail(KMM_E_ARG, "scalar arrays required");
+  KMM_HIP(hipSetDevice(c->device));
+  const double* ins[9] = {u, v, w, omx, omy, omz, sclx, scly, sclz};
+  const int nin = sc ? 9 : 6;
+  // spectral inputs -> NL buffers (scratch) -> Chebyshev-to-physical -> T
+  for (int f = 0; f < nin; ++f) {
+    int r = upload_field(c, c->ZF, ins[f]);
+    if (r) return r;
+    hipError_t e = kmm::launch_cheb(c->ny, c->P, c->twM, c->ZF, c->T[f], 1, c->stream);
+    if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
+  }
+  KMM_HIP(hipMemsetAsync(c->cfl_d, 0, sizeof(unsigned long long), c->stream));
+  int r = run_plane(c, it, c->cfl_d);
+  if (r) return r;
+  double* outs[4] = {gn, fn, omz_out, scn};
+  const int nout = sc ? 4 : 3;
+  for (int o = 0; o < nout; ++o) {
+    hipError_t e = kmm::launch_cheb(c->ny, c->P, c->twM, c->NL[o], c->T[o], 0, c->stream);
+    if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
+    r = download_field(c, c->T[o], outs[o]);
+    if (r) return r;
+  }
+  c->nl_valid = false;
+  return check_cfl(c, 1, it);
+}

@@ -1,0 +1,88 @@
+// Kernel argument blocks and launch entry points shared by kmm_api.cpp and
+// the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+
+namespace kmm {
+
+// Pencil-stage arguments.  Field buffers are [y][kx][kz] complex128; a
+// "pencil" p = kx*nz + kz; P = nxh*nz pencils per y-plane.
+struct PencilArgs {
+  int nxh, nz, nzh, P, ny;
+  double alpha, beta, dyde, re, dt, theta, diff;
+  double gH, gS;  // re/(dt theta), re diff/(dt theta)
+  // mean-mode Robin data (setbcs, main_dns.f90:1175-1418)
+  double atu, btu, gtu, abu, bbu, gbu, atw, btw, gtw, abw, bbw, gbw;
+  int scalar;
+  int smooth;       // apply spectral smoothing to the NL terms
+  int first_nl;     // NL terms come from the NL buffers (else from FN/GN/SCN)
+  double src_uniform;  // unused here (kept for layout stability)
+  const double* smz;   // [nz] z smoothing factors
+  const double* smy;   // [nyp] y smoothing factors
+  const double2* twM;  // FFT twiddles for length 2*ny
+  const double2* tabH; // [P][65] 1/d tables, wn = k2 + gH
+  const double2* tabP; // wn = k2
+  const double2* tabS; // wn = k2 + gS
+  const double2* G1;   // [P][ny/2+1] Green's functions (real, lane-pair packed)
+  const double2* G2;
+  const double4* gk;   // [P] influence coefficients (k1b,k1t,k2b,k2t)
+  const double2 *NL1, *NL2, *NL3, *NLS;   // physical-y NL: (u x w)_x,_y,_z, scalar
+  double2 *V, *OMY, *FN, *FNM1, *GN, *GNM1, *TH, *SCN, *SCNM1;
+  double *u0, *w0, *h1n, *h1nm1, *h3n, *h3nm1;
+  double2* T[9];       // transfer: u v w omx omy omz thx thy thz (physical y)
+};
+
+// Plane-stage arguments.
+struct PlaneArgs {
+  int nxh, nz, nzh, nyp, mx, mz, P;
+  int nfin, nfout;     // input / output field counts (6/3 NS, 9/4 with scalar)
+  double dt, dPdx, delxm, delzm, src_uniform;
+  int flow_select;
+  const double* seght;
+  const double2* twZ;  // length mz twiddles
+  const double2* twX;  // length mx twiddles
+  const double2* in[9];   // spectral-xz, physical-y inputs [y][kx][kz]
+  double2* Z;             // [nfin][nyp][mz][nxh]
+  double2* ZF;            // [nfout][nyp][mz][nxh]
+  double2* out[4];        // NL [y][kx][kz]
+  unsigned long long* cfl;  // bit pattern of the max CFL number for this call
+  // immersed boundary (Goldstein feedback, solvers.f90:2996-3108)
+  int nsrc;
+  const int4* src;        // (y, z, x, imatrix type) per solid cell
+  double* fint;           // [3][nsrc] feedback integrals
+  double* drag;           // [3][nsrc]
+  double gain, ugain;
+  int it, perturbtime;
+  const int* trow;        // [nyp*mz+1] first target of each (y,z) row
+  const int* tx;          // x index per target
+  const int* tptr;        // [ntgt+1] first entry per target
+  const int* ent_src;     // source index per entry
+  const double* ent_w;    // fspread weight per entry
+  const int* fixrow;      // [nyp*mz+1] solid cells per (y,z) row (no -dPdx)
+  const int* fixx;        // x index per solid cell
+};
+
+hipError_t launch_tau_tables(int ny, const PencilArgs& a, double g, double2* tab, hipStream_t s);
+hipError_t launch_green(int ny, const PencilArgs& a, double pb1, double pt1, double pb2, double pt2,
+                        double2* G1, double2* G2, double4* gk, hipStream_t s);
+hipError_t launch_pencil_step(int ny, const PencilArgs& a, hipStream_t s);
+hipError_t launch_pencil_startup(int ny, const PencilArgs& a, hipStream_t s);
+hipError_t launch_pencil_finish(int ny, const PencilArgs& a, hipStream_t s);
+hipError_t launch_plane(const PlaneArgs& a, hipStream_t s);
+hipError_t launch_plane_fwd_only(const PlaneArgs& a, hipStream_t s);
+// ev[0..3] recorded after k_zinv, k_ibdrag, k_xplane, k_zfwd
+hipError_t launch_plane_timed(const PlaneArgs& a, hipStream_t s, hipEvent_t* ev);
+
+// per-routine (parity) kernels
+hipError_t launch_cderiv(int ny, int P, double dyde, const double2* in, double2* out, hipStream_t s);
+hipError_t launch_solve(int ny, const PencilArgs& a, const double2* tab, const double* wn, double2* u,
+                        const double2* bctop, const double2* bcbot, int ib, const int* kzero, hipStream_t s);
+hipError_t launch_penta(int ny, const PencilArgs& a, const double2* tab, const double* wn, double2* u,
+                        double at, double bt, const double2* gtop, double ab, double bb, const double2* gbot,
+                        int bottom_sign, const int* kzero, hipStream_t s);
+hipError_t launch_tau_tables_wn(int ny, int P, double dyde, const double* wn, double2* tab, hipStream_t s);
+hipError_t launch_cheb(int ny, int P, const double2* tw, const double2* in, double2* out, int inverse,
+                       hipStream_t s);
+
+}  // namespace kmm

@@ -1,0 +1,494 @@
+// Pencil-stage kernels of the KMM channel step (wall-normal direction).
+//
+// k_pencil_step fuses, per (kx,kz) pencil, the work the reference spreads
+// over FFTW_DNS/code/src/dns/main_dns.f90:241-811 and the spectral half of
+// vcw3dp: forward Chebyshev transform of the previous step's nonlinear terms
+// (solvers.f90:3279-3416), smoothing (main_dns.f90:710-745), nonlin
+// (solvers.f90:1497-1572), the scalar and phi/v/omega_y CN/AB2 solves
+// (phirhs/solve/penta, main_dns.f90:324-484), the influence-matrix
+// correction (main_dns.f90:433-466), the mean-flow solves (uzero,
+// solvers.f90:1065-1176), veloc/vort/gradscl and the inverse Chebyshev
+// transforms (solvers.f90:2241-2546).  Projections of ``norm`` that couple
+// +kz and -kz are applied once per step in the plane stage (they commute
+// with every pencil operation); the pencil-local ones are applied here.
+#include "kmm_kernels.h"
+#include "kmm_pencil.h"
+
+namespace kmm {
+
+constexpr int PW = 8;  // wavefronts (pencils) per workgroup
+
+template <int NY>
+__device__ __forceinline__ double2* wave_lds(double2* smem, int wv) {
+  return smem + (size_t)wv * 4 * NY;  // 2 buffers of 2*NY
+}
+
+// ---------------------------------------------------------- 1/d tables --
+// d recurrence of eqsolve/pntsolve (solvers.f90:500-553, 931-935) per pencil
+// and parity; thread per pencil.
+template <int NY>
+__global__ void k_tau_tables(int P, int nz, double alpha, double beta, double dyde, double g,
+                             const double* __restrict__ wnv, double2* __restrict__ tab) {
+  constexpr int L = NY / 2;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  double wn;
+  if (wnv) {
+    wn = wnv[p];
+  } else {
+    const int kx = p / nz, kzi = p % nz;
+    const int kzs = (kzi < nz / 2) ? kzi : kzi - nz;
+    const double kxw = kx * alpha, kzw = kzs * beta;
+    wn = kxw * kxw + kzw * kzw + g;
+  }
+  double2* t = tab + (size_t)p * 65;
+  for (int i = 0; i < 65; ++i) t[i] = make_double2(0.0, 0.0);
+  // even: m = L
+  double a1, a2, a3, b1, b2, b3;
+  coef_even<NY>(L, wn, dyde, a1, a2, a3);
+  double d = a2;
+  t[64].x = 1.0 / d;
+  double dn = d;  // d_{i+1}
+  for (int i = L - 1; i >= 1; --i) {
+    coef_even<NY>(i, wn, dyde, a1, a2, a3);
+    coef_even<NY>(i + 1, wn, dyde, b1, b2, b3);
+    d = a2 - a3 * b1 / dn;
+    t[i].x = 1.0 / d;
+    dn = d;
+  }
+  // odd: m = L-1
+  coef_odd<NY>(L - 1, wn, dyde, a1, a2, a3);
+  d = a2;
+  t[L - 1].y = 1.0 / d;
+  dn = d;
+  for (int i = L - 2; i >= 1; --i) {
+    coef_odd<NY>(i, wn, dyde, a1, a2, a3);
+    coef_odd<NY>(i + 1, wn, dyde, b1, b2, b3);
+    d = a2 - a3 * b1 / dn;
+    t[i].y = 1.0 / d;
+    dn = d;
+  }
+}
+
+__device__ __forceinline__ void pencil_k(const PencilArgs& A, int p, double& kx, double& kz, int& kxi, int& kzi) {
+  kxi = p / A.nz;
+  kzi = p % A.nz;
+  const int kzs = (kzi < A.nzh) ? kzi : kzi - A.nz;
+  kx = kxi * A.alpha;
+  kz = kzs * A.beta;
+}
+
+// ------------------------------------------------- Green's functions --
+// gfcn (solvers.f90:5-172) with real boundary data: the reference's complex
+// data (pt,pt) only scales each pencil's solution by a constant that
+// cancels in c1*gf1 + c2*gf2, so the real solution and real influence
+// coefficients give the same composite v.
+template <int NY>
+__global__ __launch_bounds__(PW * 64) void k_green(PencilArgs A, double pb1, double pt1, double pb2, double pt2,
+                                                   double2* __restrict__ G1, double2* __restrict__ G2,
+                                                   double4* __restrict__ gk) {
+  constexpr int L = NY / 2;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int p = blockIdx.x * PW + wv;
+  if (p >= A.P) return;
+  double kx, kz;
+  int kxi, kzi;
+  pencil_k(A, p, kx, kz, kxi, kzi);
+  const double k2 = kx * kx + kz * kz;
+  const double2* tH = A.tabH + (size_t)p * 65;
+  const double2* tP = A.tabP + (size_t)p * 65;
+  double2 gt[2], gb[2];
+  for (int w = 0; w < 2; ++w) {
+    const double pb = w ? pb2 : pb1, pt = w ? pt2 : pt1;
+    PF z = pf_zero();
+    PF phi = tau_solve<NY>(z, k2 + A.gH, A.dyde, tH, make_double2(pt, 0.0), make_double2(pb, 0.0), 0, lane);
+    PF g = tau_solve<NY>(phi, k2, A.dyde, tP, czero(), czero(), 0, lane);
+    g = pf_real(g);
+    double2* G = (w ? G2 : G1) + (size_t)p * (L + 1);
+    if (lane < L) G[lane] = make_double2(g.a.x, g.b.x);
+    if (lane == 0) G[L] = make_double2(g.e.x, 0.0);
+    wall_derivs<NY>(g, A.dyde, lane, gt[w], gb[w]);
+  }
+  if (lane == 0) {
+    const double g1b = gb[0].x, g1t = gt[0].x, g2b = gb[1].x, g2t = gt[1].x;
+    const double den = g1t * g2b - g2t * g1b;
+    // c1 = (vb g2t - vt g2b)/den, c2 = (vt g1b - vb g1t)/den
+    gk[p] = make_double4(g2t / den, -g2b / den, -g1t / den, g1b / den);
+  }
+}
+
+// ----------------------------------------------------------- the step --
+template <int NY>
+__device__ __forceinline__ PF green_load(const double2* __restrict__ G, int p, int lane) {
+  constexpr int L = NY / 2;
+  const double2* g = G + (size_t)p * (L + 1);
+  PF f = pf_zero();
+  if (lane < L) { double2 v = g[lane]; f.a.x = v.x; f.b.x = v.y; }
+  f.e.x = g[L].x;
+  return f;
+}
+
+// complex pencil times a real pencil coefficientwise
+__device__ __forceinline__ PF pf_cmul_real(double2 c, const PF& g) {
+  PF r;
+  r.a = cscale(c, g.a.x); r.b = cscale(c, g.b.x); r.e = cscale(c, g.e.x);
+  return r;
+}
+
+template <int NY>
+__device__ __forceinline__ PF smooth_apply(const PF& f, const double* __restrict__ smy, double sz, int lane) {
+  constexpr int L = NY / 2;
+  PF r = f;
+  if (lane < L) {
+    r.a = cscale(f.a, sz * smy[2 * lane]);
+    r.b = cscale(f.b, sz * smy[2 * lane + 1]);
+  }
+  r.e = cscale(f.e, sz * smy[NY]);
+  return r;
+}
+
+// MODE 0: full step (NL from buffers or FN/GN/SCN), MODE 1: startup
+// (derived fields of the current state only), MODE 2: finish (NL -> FN, GN,
+// SCN, h1n, h3n).
+template <int NY, bool SCALAR, int MODE>
+__global__ __launch_bounds__(PW * 64) void k_pencil(PencilArgs A) {
+  constexpr int L = NY / 2;
+  extern __shared__ double2 smem[];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int p = blockIdx.x * PW + wv;
+  if (p >= A.P) return;
+  double2* buf = wave_lds<NY>(smem, wv);
+  double2* scr = buf + 2 * NY;
+  const size_t P = (size_t)A.P;
+  double kx, kz;
+  int kxi, kzi;
+  pencil_k(A, p, kx, kz, kxi, kzi);
+  const double k2 = kx * kx + kz * kz;
+  const bool is00 = (kxi == 0 && kzi == 0);
+  const bool nyq = (kzi == A.nzh);
+  const double dy = A.dyde;
+  const int nT = SCALAR ? 9 : 6;
+
+  if (nyq) {  // norm zeroes the z-Nyquist row (helpers.f90:149-156)
+    PF z = pf_zero();
+    if (MODE == 0) {
+      pf_store<NY>(A.V, P, p, lane, z); pf_store<NY>(A.OMY, P, p, lane, z);
+      pf_store<NY>(A.FNM1, P, p, lane, z); pf_store<NY>(A.GNM1, P, p, lane, z);
+      if (SCALAR) { pf_store<NY>(A.TH, P, p, lane, z); pf_store<NY>(A.SCNM1, P, p, lane, z); }
+    }
+    if (MODE == 2) {
+      pf_store<NY>(A.FN, P, p, lane, z); pf_store<NY>(A.GN, P, p, lane, z);
+      if (SCALAR) pf_store<NY>(A.SCN, P, p, lane, z);
+    }
+    if (MODE != 2)
+      for (int t = 0; t < nT; ++t) pf_store<NY>(A.T[t], P, p, lane, z);
+    return;
+  }
+
+  // ---------------- part B: nonlinear terms -> fn, gn, scn (spectral) --
+  PF fn, gn, sn;
+  PF h1m, h3m;  // mean-mode profiles (only used by the (0,0) pencil)
+  if (MODE == 0 || MODE == 2) {
+    if (MODE == 2 || A.first_nl) {
+      const double sz = A.smooth ? A.smz[kzi] : 1.0;
+      PF h1 = phys_to_cheb<NY>(pf_load<NY>(A.NL1, P, p, lane), buf, scr, A.twM, lane);
+      PF h3 = phys_to_cheb<NY>(pf_load<NY>(A.NL3, P, p, lane), buf, scr, A.twM, lane);
+      PF h2 = phys_to_cheb<NY>(pf_load<NY>(A.NL2, P, p, lane), buf, scr, A.twM, lane);
+      if (A.smooth) {
+        h1 = smooth_apply<NY>(h1, A.smy, sz, lane);
+        h2 = smooth_apply<NY>(h2, A.smy, sz, lane);
+        h3 = smooth_apply<NY>(h3, A.smy, sz, lane);
+      }
+      if (is00) { h1 = pf_real(h1); h2 = pf_real(h2); h3 = pf_real(h3); }
+      h1m = h1;
+      h3m = h3;
+      // nonlin (solvers.f90:1497-1572)
+      PF x = pf_add(pf_imul(h3, kz), pf_imul(h1, kx));
+      PF dx = cderiv<NY>(x, dy, lane);
+      fn = pf_sub(pf_scale(h2, -k2), dx);
+      gn = pf_sub(pf_imul(h1, kz), pf_imul(h3, kx));
+      if (SCALAR) {
+        sn = phys_to_cheb<NY>(pf_load<NY>(A.NLS, P, p, lane), buf, scr, A.twM, lane);
+        if (A.smooth) sn = smooth_apply<NY>(sn, A.smy, sz, lane);
+        if (is00) sn = pf_real(sn);
+      }
+      if (is00) {
+        // main_dns.f90:748-751 (h1n = Re gn(:,1,1), h3n = Re omz(:,1,1))
+        if (MODE == 2) {
+          pf_store_real<NY>(A.h1n, lane, h1m);
+          pf_store_real<NY>(A.h3n, lane, h3m);
+        }
+      }
+      if (MODE == 2) {
+        pf_store<NY>(A.FN, P, p, lane, fn);
+        pf_store<NY>(A.GN, P, p, lane, gn);
+        if (SCALAR) pf_store<NY>(A.SCN, P, p, lane, sn);
+        return;
+      }
+    } else {
+      fn = pf_load<NY>(A.FN, P, p, lane);
+      gn = pf_load<NY>(A.GN, P, p, lane);
+      if (SCALAR) sn = pf_load<NY>(A.SCN, P, p, lane);
+      if (is00) { h1m = pf_load_real<NY>(A.h1n, lane); h3m = pf_load_real<NY>(A.h3n, lane); }
+    }
+  }
+
+  const double th = A.theta;
+  // ---------------- scalar (main_dns.f90:324-358) ----------------------
+  PF sc;
+  if (SCALAR) {
+    sc = pf_load<NY>(A.TH, P, p, lane);
+    if (MODE == 0) {
+      PF scnm1 = pf_load<NY>(A.SCNM1, P, p, lane);
+      PF d1 = cderiv<NY>(sc, dy, lane);
+      PF d2 = cderiv<NY>(d1, dy, lane);
+      PF rhs = phirhs<NY>(sc, d2, sn, scnm1, k2, A.re, A.diff, A.dt, th);
+      pf_store<NY>(A.SCNM1, P, p, lane, sn);
+      auto tt = [dy](int n) -> double { return dy * (double)n * (double)n; };
+      auto tb = [dy](int n) -> double { double v = dy * (double)n * (double)n; return (n & 1) ? -v : v; };
+      sc = tau_penta<NY>(rhs, k2 + A.gS, dy, A.tabS + (size_t)p * 65, tt, tb, czero(), czero(), lane);
+      if (is00) sc = pf_real(sc);
+      pf_store<NY>(A.TH, P, p, lane, sc);
+    }
+  }
+
+  // ---------------- v and omega_y (main_dns.f90:360-502) ---------------
+  PF v, omy;
+  PF u0p, w0p;
+  if (MODE == 0) {
+    if (!is00) {
+      v = pf_load<NY>(A.V, P, p, lane);
+      PF fnm1 = pf_load<NY>(A.FNM1, P, p, lane);
+      PF d1 = cderiv<NY>(v, dy, lane);
+      PF d2 = cderiv<NY>(d1, dy, lane);
+      PF phi = pf_sub(d2, pf_scale(v, k2));
+      d1 = cderiv<NY>(phi, dy, lane);
+      d2 = cderiv<NY>(d1, dy, lane);
+      PF rhs = phirhs<NY>(phi, d2, fn, fnm1, k2, A.re, 1.0, A.dt, th);
+      pf_store<NY>(A.FNM1, P, p, lane, fn);
+      phi = tau_solve<NY>(rhs, k2 + A.gH, dy, A.tabH + (size_t)p * 65, czero(), czero(), 0, lane);
+      PF vp = tau_solve<NY>(phi, k2, dy, A.tabP + (size_t)p * 65, czero(), czero(), 0, lane);
+      double2 vt, vb;
+      wall_derivs<NY>(vp, dy, lane, vt, vb);
+      const double4 g = A.gk[p];
+      double2 c1 = cadd(cscale(vb, g.x), cscale(vt, g.y));
+      double2 c2 = cadd(cscale(vb, g.z), cscale(vt, g.w));
+      v = pf_add(vp, pf_add(pf_cmul_real(c1, green_load<NY>(A.G1, p, lane)),
+                            pf_cmul_real(c2, green_load<NY>(A.G2, p, lane))));
+      omy = pf_load<NY>(A.OMY, P, p, lane);
+      PF gnm1 = pf_load<NY>(A.GNM1, P, p, lane);
+      PF e1 = cderiv<NY>(omy, dy, lane);
+      PF e2 = cderiv<NY>(e1, dy, lane);
+      PF orhs = phirhs<NY>(omy, e2, gn, gnm1, k2, A.re, 1.0, A.dt, th);
+      pf_store<NY>(A.GNM1, P, p, lane, gn);
+      const double atu = A.atu, btu = A.btu, abu = A.abu, bbu = A.bbu;
+      auto tt = [=](int n) -> double { return atu + dy * btu * (double)n * (double)n; };
+      auto tb = [=](int n) -> double { double t = abu + dy * bbu * (double)n * (double)n; return (n & 1) ? -t : t; };
+      omy = tau_penta<NY>(orhs, k2 + A.gH, dy, A.tabH + (size_t)p * 65, tt, tb, czero(), czero(), lane);
+      pf_store<NY>(A.V, P, p, lane, v);
+      pf_store<NY>(A.OMY, P, p, lane, omy);
+    } else {
+      // (0,0): v = 0 (veloc), omega_y = 0, fnm1 = fn = 0, gnm1 = gn = 0,
+      // and the mean-flow solves (uzero, solvers.f90:1065-1176).
+      v = pf_zero();
+      omy = pf_zero();
+      pf_store<NY>(A.V, P, p, lane, v);
+      pf_store<NY>(A.OMY, P, p, lane, omy);
+      pf_store<NY>(A.FNM1, P, p, lane, pf_zero());
+      pf_store<NY>(A.GNM1, P, p, lane, pf_zero());
+      for (int w = 0; w < 2; ++w) {
+        double* um = w ? A.w0 : A.u0;
+        double* hl = w ? A.h3nm1 : A.h1nm1;
+        double* hn = w ? A.h3n : A.h1n;
+        const PF& hcur = w ? h3m : h1m;
+        PF u = pf_load_real<NY>(um, lane);
+        PF hlp = pf_load_real<NY>(hl, lane);
+        PF d1 = cderiv<NY>(u, dy, lane);
+        PF d2 = cderiv<NY>(d1, dy, lane);
+        // wrk1 = -re u0/(dt th) - re (3 h1 - h1l)/(2 th) - (1-th)/th u0''
+        PF r = pf_scale(u, -A.re / (A.dt * th));
+        r = pf_axpy(-3.0 * A.re / (2.0 * th), hcur, r);
+        r = pf_axpy(A.re / (2.0 * th), hlp, r);
+        r = pf_axpy(-(1.0 - th) / th, d2, r);
+        // psolve1d with mixedbc1d rows (correct bottom sign); w0 uses btu
+        // (quirk of solvers.f90:1173).
+        const double at = w ? A.atw : A.atu, bt = A.btu, ab = w ? A.abw : A.abu, bb = w ? A.bbw : A.bbu;
+        const double gt = w ? A.gtw : A.gtu, gb = w ? A.gbw : A.gbu;
+        auto tt = [=](int n) -> double { return at + dy * bt * (double)n * (double)n; };
+        auto tb = [=](int n) -> double { double t = ab - dy * bb * (double)n * (double)n; return (n & 1) ? -t : t; };
+        PF un = tau_penta<NY>(r, A.gH, dy, A.tabH + (size_t)p * 65, tt, tb, make_double2(gt, 0.0),
+                              make_double2(gb, 0.0), lane);
+        un = pf_real(un);
+        pf_store_real<NY>(um, lane, un);
+        pf_store_real<NY>(hl, lane, hcur);
+        pf_store_real<NY>(hn, lane, hcur);
+        if (w) w0p = un; else u0p = un;
+      }
+    }
+  } else {  // MODE 1 (startup): current state only
+    v = pf_load<NY>(A.V, P, p, lane);
+    omy = pf_load<NY>(A.OMY, P, p, lane);
+    if (is00) {
+      v = pf_zero(); omy = pf_zero();
+      pf_store<NY>(A.V, P, p, lane, v);
+      pf_store<NY>(A.OMY, P, p, lane, omy);
+      u0p = pf_load_real<NY>(A.u0, lane);
+      w0p = pf_load_real<NY>(A.w0, lane);
+    }
+  }
+
+  // ---------------- veloc / vort / gradscl (solvers.f90:1576-1722) ------
+  PF u, w;
+  if (is00) {
+    u = u0p;
+    w = w0p;
+  } else {
+    PF dv = cderiv<NY>(v, dy, lane);
+    const double ik2 = 1.0 / k2;
+    // u = -(-i kx Dv + i kz omy)/k2 ; w = -(-i kz Dv - i kx omy)/k2
+    u = pf_scale(pf_sub(pf_imul(dv, kx), pf_imul(omy, kz)), ik2);
+    w = pf_scale(pf_add(pf_imul(dv, kz), pf_imul(omy, kx)), ik2);
+  }
+  PF ox = pf_sub(cderiv<NY>(w, dy, lane), pf_imul(v, kz));
+  PF oz = pf_sub(pf_imul(v, kx), cderiv<NY>(u, dy, lane));
+  pf_store<NY>(A.T[0], P, p, lane, cheb_to_phys<NY>(u, buf, scr, A.twM, lane));
+  pf_store<NY>(A.T[1], P, p, lane, cheb_to_phys<NY>(v, buf, scr, A.twM, lane));
+  pf_store<NY>(A.T[2], P, p, lane, cheb_to_phys<NY>(w, buf, scr, A.twM, lane));
+  pf_store<NY>(A.T[3], P, p, lane, cheb_to_phys<NY>(ox, buf, scr, A.twM, lane));
+  pf_store<NY>(A.T[4], P, p, lane, cheb_to_phys<NY>(omy, buf, scr, A.twM, lane));
+  pf_store<NY>(A.T[5], P, p, lane, cheb_to_phys<NY>(oz, buf, scr, A.twM, lane));
+  if (SCALAR) {
+    PF tx = pf_imul(sc, kx), tz = pf_imul(sc, kz);
+    PF ty = cderiv<NY>(sc, dy, lane);
+    pf_store<NY>(A.T[6], P, p, lane, cheb_to_phys<NY>(tx, buf, scr, A.twM, lane));
+    pf_store<NY>(A.T[7], P, p, lane, cheb_to_phys<NY>(ty, buf, scr, A.twM, lane));
+    pf_store<NY>(A.T[8], P, p, lane, cheb_to_phys<NY>(tz, buf, scr, A.twM, lane));
+  }
+}
+
+// -------------------------------------------------- parity kernels --
+template <int NY>
+__global__ __launch_bounds__(PW * 64) void k_cderiv(int P, double dyde, const double2* __restrict__ in,
+                                                    double2* __restrict__ out) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int p = blockIdx.x * PW + wv;
+  if (p >= P) return;
+  PF f = pf_load<NY>(in, P, p, lane);
+  pf_store<NY>(out, P, p, lane, cderiv<NY>(f, dyde, lane));
+}
+
+template <int NY>
+__global__ __launch_bounds__(PW * 64) void k_cheb(int P, const double2* __restrict__ tw, const double2* __restrict__ in,
+                                                  double2* __restrict__ out, int inverse) {
+  extern __shared__ double2 smem[];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int p = blockIdx.x * PW + wv;
+  if (p >= P) return;
+  double2* buf = wave_lds<NY>(smem, wv);
+  PF f = pf_load<NY>(in, P, p, lane);
+  PF r = inverse ? cheb_to_phys<NY>(f, buf, buf + 2 * NY, tw, lane) : phys_to_cheb<NY>(f, buf, buf + 2 * NY, tw, lane);
+  pf_store<NY>(out, P, p, lane, r);
+}
+
+template <int NY>
+__global__ __launch_bounds__(PW * 64) void k_solve(int P, double dyde, const double2* __restrict__ tab,
+                                                   const double* __restrict__ wn, double2* __restrict__ u,
+                                                   const double2* __restrict__ bct, const double2* __restrict__ bcb,
+                                                   int ib, const int* __restrict__ kzero) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int p = blockIdx.x * PW + wv;
+  if (p >= P) return;
+  PF f = pf_load<NY>(u, P, p, lane);
+  PF r = tau_solve<NY>(f, wn[p], dyde, tab + (size_t)p * 65, bct[p], bcb[p], ib, lane);
+  if (kzero && kzero[p]) r = pf_zero();
+  pf_store<NY>(u, P, p, lane, r);
+}
+
+template <int NY>
+__global__ __launch_bounds__(PW * 64) void k_penta(int P, double dyde, const double2* __restrict__ tab,
+                                                   const double* __restrict__ wn, double2* __restrict__ u,
+                                                   double at, double bt, const double2* __restrict__ gtop,
+                                                   double ab, double bb, const double2* __restrict__ gbot,
+                                                   double bsign, const int* __restrict__ kzero) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int p = blockIdx.x * PW + wv;
+  if (p >= P) return;
+  PF f = pf_load<NY>(u, P, p, lane);
+  auto tt = [=](int n) -> double { return at + dyde * bt * (double)n * (double)n; };
+  auto tb = [=](int n) -> double { double t = ab + bsign * dyde * bb * (double)n * (double)n; return (n & 1) ? -t : t; };
+  PF r = tau_penta<NY>(f, wn[p], dyde, tab + (size_t)p * 65, tt, tb, gtop[p], gbot[p], lane);
+  if (kzero && kzero[p]) r = pf_zero();
+  pf_store<NY>(u, P, p, lane, r);
+}
+
+// ------------------------------------------------------------ launchers --
+#define KMM_NY_DISPATCH(ny, ...) \
+  switch (ny) {                   \
+    case 32: { constexpr int NY = 32; __VA_ARGS__; } break; \
+    case 48: { constexpr int NY = 48; __VA_ARGS__; } break; \
+    case 64: { constexpr int NY = 64; __VA_ARGS__; } break; \
+    case 96: { constexpr int NY = 96; __VA_ARGS__; } break; \
+    case 128: { constexpr int NY = 128; __VA_ARGS__; } break; \
+    default: return hipErrorInvalidValue; \
+  }
+
+static inline int nblk(int P) { return (P + PW - 1) / PW; }
+template <int NY>
+static inline size_t lds_bytes() { return (size_t)PW * 4 * NY * sizeof(double2); }
+
+hipError_t launch_tau_tables(int ny, const PencilArgs& a, double g, double2* tab, hipStream_t s) {
+  KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_tau_tables<NY>), dim3((a.P + 127) / 128), dim3(128), 0, s, a.P, a.nz,
+                                          a.alpha, a.beta, a.dyde, g, (const double*)nullptr, tab));
+  return hipGetLastError();
+}
+hipError_t launch_tau_tables_wn(int ny, int P, double dyde, const double* wn, double2* tab, hipStream_t s) {
+  KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_tau_tables<NY>), dim3((P + 127) / 128), dim3(128), 0, s, P, 1, 0.0,
+                                          0.0, dyde, 0.0, wn, tab));
+  return hipGetLastError();
+}
+hipError_t launch_green(int ny, const PencilArgs& a, double pb1, double pt1, double pb2, double pt2, double2* G1,
+                        double2* G2, double4* gk, hipStream_t s) {
+  KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_green<NY>), dim3(nblk(a.P)), dim3(PW * 64), 0, s, a, pb1, pt1, pb2,
+                                          pt2, G1, G2, gk));
+  return hipGetLastError();
+}
+template <int MODE>
+static hipError_t launch_pencil_mode(int ny, const PencilArgs& a, hipStream_t s) {
+  if (a.scalar) {
+    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_pencil<NY, true, MODE>), dim3(nblk(a.P)), dim3(PW * 64),
+                                            lds_bytes<NY>(), s, a));
+  } else {
+    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_pencil<NY, false, MODE>), dim3(nblk(a.P)), dim3(PW * 64),
+                                            lds_bytes<NY>(), s, a));
+  }
+  return hipGetLastError();
+}
+hipError_t launch_pencil_step(int ny, const PencilArgs& a, hipStream_t s) { return launch_pencil_mode<0>(ny, a, s); }
+hipError_t launch_pencil_startup(int ny, const PencilArgs& a, hipStream_t s) { return launch_pencil_mode<1>(ny, a, s); }
+hipError_t launch_pencil_finish(int ny, const PencilArgs& a, hipStream_t s) { return launch_pencil_mode<2>(ny, a, s); }
+
+hipError_t launch_cderiv(int ny, int P, double dyde, const double2* in, double2* out, hipStream_t s) {
+  KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_cderiv<NY>), dim3(nblk(P)), dim3(PW * 64), 0, s, P, dyde, in, out));
+  return hipGetLastError();
+}
+hipError_t launch_cheb(int ny, int P, const double2* tw, const double2* in, double2* out, int inverse,
+                       hipStream_t s) {
+  KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_cheb<NY>), dim3(nblk(P)), dim3(PW * 64), lds_bytes<NY>(), s, P, tw,
+                                          in, out, inverse));
+  return hipGetLastError();
+}
+hipError_t launch_solve(int ny, const PencilArgs& a, const double2* tab, const double* wn, double2* u,
+                        const double2* bctop, const double2* bcbot, int ib, const int* kzero, hipStream_t s) {
+  KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_solve<NY>), dim3(nblk(a.P)), dim3(PW * 64), 0, s, a.P, a.dyde, tab,
+                                          wn, u, bctop, bcbot, ib, kzero));
+  return hipGetLastError();
+}
+hipError_t launch_penta(int ny, const PencilArgs& a, const double2* tab, const double* wn, double2* u, double at,
+                        double bt, const double2* gtop, double ab, double bb, const double2* gbot, int bottom_sign,
+                        const int* kzero, hipStream_t s) {
+  KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_penta<NY>), dim3(nblk(a.P)), dim3(PW * 64), 0, s, a.P, a.dyde, tab,
+                                          wn, u, at, bt, gtop, ab, bb, gbot, (double)bottom_sign, kzero));
+  return hipGetLastError();
+}
+
+}  // namespace kmm

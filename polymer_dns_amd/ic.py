@@ -1,0 +1,127 @@
+"""Seeded initial conditions (no reference counterpart).
+
+The reference starts channel runs from a pure laminar profile
+(init_flow.f90:82-139) and trips turbulence with temporary IB bricks; the
+seeded, divergence-free perturbation here is new.  It is delivered to either
+code through a ``restart`` file (IRSTRT=1), so both start from identical
+bytes.
+
+Recipe: laminar u0 = Uc (1 - y^2) (Uc = -re dPdx yl^2/8), and for
+0 <= kx <= kmax, |kz| <= kmax, (kx,kz) != (0,0):
+    v_hat   = (1 - y^2)^2 * sum_{n<=4} r_n  T_n(y)
+    omy_hat = (1 - y^2)   * sum_{n<=4} r'_n T_n(y)
+with r, r' complex N(0,1) drawn from SplitMix64 + Box-Muller.  v = v' = 0 and
+omega_y = 0 at both walls, so u, w from continuity satisfy no slip.  The
+amplitude is scaled so that max |u'| on the collocation grid is
+``amp * Uc``.  kx = 0 modes are made Hermitian in kz.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+from numpy.polynomial import chebyshev as cheb
+
+
+class SplitMix64:
+    """SplitMix64 (Steele, Lea, Flood 2014) with Box-Muller normals."""
+
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed: int):
+        self.s = seed & self.MASK
+
+    def next_u64(self) -> int:
+        self.s = (self.s + 0x9E3779B97F4A7C15) & self.MASK
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & self.MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & self.MASK
+        return z ^ (z >> 31)
+
+    def uniform(self) -> float:
+        return ((self.next_u64() >> 11) + 0.5) * (1.0 / 9007199254740992.0)
+
+    def normal(self) -> float:
+        u1 = self.uniform()
+        u2 = self.uniform()
+        return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+
+
+def laminar_u0(ny, re, R_tau, yl=2.0):
+    """Chebyshev coefficients of the CPG laminar profile (init_flow.f90:117-138)."""
+    dPdx = -2.0 / yl * (2.0 * R_tau / (yl * re)) ** 2
+    Uc = -re * dPdx * yl * yl / 8.0
+    u0 = np.zeros(ny + 1)
+    u0[0] = 0.5 * Uc
+    u0[2] = -0.5 * Uc
+    return u0, Uc, dPdx
+
+
+def _phys_max_speed(v, omy, u0, nx, nz, xl, zl, yl):
+    """max over the collocation grid of |(u', v', w')| for the perturbation."""
+    nxh, nzz, nyp = v.shape
+    ny = nyp - 1
+    dyde = 2.0 / yl
+    alpha = 2.0 * math.pi / xl
+    beta = 2.0 * math.pi / zl
+    kx = (np.arange(nxh) * alpha)[:, None, None]
+    kzi = np.arange(nz)
+    kzi[nz // 2:] -= nz
+    kz = (kzi * beta)[None, :, None]
+    dv = np.zeros_like(v)
+    for a in range(nxh):
+        for b in range(nz):
+            d = cheb.chebder(v[a, b], scl=dyde)
+            dv[a, b, :d.size] = d
+    k2 = kx ** 2 + kz ** 2
+    k2[0, 0, 0] = 1.0
+    u = -(-1j * kx * dv + 1j * kz * omy) / k2
+    w = -(-1j * kz * dv - 1j * kx * omy) / k2
+    u[0, 0] = 0.0
+    w[0, 0] = 0.0
+    y = np.cos(np.arange(nyp) * math.pi / ny)
+    T = np.cos(np.outer(np.arange(nyp), np.arccos(y)))  # T_n(y_k): [n, k]
+    speed2 = 0.0
+    comps = []
+    for a in (u, v, w):
+        py = a @ T                                       # (nxh, nz, nyp) physical y
+        full = np.zeros((nz, nx // 2 + 1, nyp), dtype=np.complex128)
+        full[:, :nxh, :] = np.transpose(py, (1, 0, 2))
+        zt = np.fft.ifft(full, axis=0) * nz
+        ph = np.fft.irfft(zt, n=nx, axis=1) * nx
+        comps.append(ph)
+    speed2 = comps[0] ** 2 + comps[1] ** 2 + comps[2] ** 2
+    return float(np.sqrt(speed2.max()))
+
+
+def seeded_state(nx, ny, nz, xl, zl, re, R_tau, seed, amp=0.05, kmax=8, yl=2.0):
+    """Returns dict(v, omy, u0, w0) in reference layout (see kmm.py)."""
+    nxh, nyp = nx // 2, ny + 1
+    rng = SplitMix64(seed)
+    w1 = np.array([0.5, 0.0, -0.5])
+    w2 = cheb.chebmul(w1, w1)
+    v = np.zeros((nxh, nz, nyp), dtype=np.complex128)
+    omy = np.zeros_like(v)
+    for kx in range(0, min(kmax, nxh - 1) + 1):
+        for kzs in range(-kmax, kmax + 1):
+            if kx == 0 and kzs <= 0:
+                continue
+            if abs(kzs) >= nz // 2:
+                continue
+            r = np.array([complex(rng.normal(), rng.normal()) for _ in range(5)])
+            rp = np.array([complex(rng.normal(), rng.normal()) for _ in range(5)])
+            cv = cheb.chebmul(w2, r.real) + 1j * cheb.chebmul(w2, r.imag)
+            co = cheb.chebmul(w1, rp.real) + 1j * cheb.chebmul(w1, rp.imag)
+            j = kzs % nz
+            v[kx, j, :cv.size] = cv
+            omy[kx, j, :co.size] = co
+            if kx == 0:
+                v[0, (-kzs) % nz, :cv.size] = np.conj(cv)
+                omy[0, (-kzs) % nz, :co.size] = np.conj(co)
+    u0, Uc, dPdx = laminar_u0(ny, re, R_tau, yl)
+    smax = _phys_max_speed(v, omy, u0, nx, nz, xl, zl, yl)
+    if smax > 0:
+        s = amp * Uc / smax
+        v *= s
+        omy *= s
+    return dict(v=v, omy=omy, u0=u0, w0=np.zeros(nyp))

@@ -1,0 +1,284 @@
+"""ctypes binding of libkmm.so and a host-side mirror of the reference's
+solver interface for the KMM channel step.
+
+Arrays crossing this boundary use the reference's memory layout: a spectral
+field is complex128 of shape (nxh, nz, nyp) in C order, which is the same
+bytes as Fortran ``complex(nyp,nz,nxh)``; mean profiles are float64 (nyp,);
+IB integrals and the geometry are (mx, mz, nyp) C order == Fortran
+(nyp, mz, mx).
+
+There is no CPU fallback: constructing a :class:`Channel` without the built
+library (or without a GPU) raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkmm.so")
+
+# kmm_status (include/kmm.h)
+KMM_OK, KMM_E_CFL, KMM_E_PARTICLE, KMM_E_HIP, KMM_E_CONFIG, KMM_E_IO, KMM_E_ARG = range(7)
+
+# field ids (include/kmm.h)
+FIELDS = ["v", "fn", "fnm1", "omy", "gn", "gnm1", "u0", "h1n", "h1nm1", "w0", "h3n", "h3nm1",
+          "fxintg", "fyintg", "fzintg", "scalar", "scn", "scnm1"]
+FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
+SPECTRAL = {"v", "fn", "fnm1", "omy", "gn", "gnm1", "scalar", "scn", "scnm1"}
+MEAN = {"u0", "h1n", "h1nm1", "w0", "h3n", "h3nm1"}
+PHYS = {"fxintg", "fyintg", "fzintg"}
+
+EXPORTS = [
+    "kmm_status_string", "kmm_config_read", "kmm_create", "kmm_destroy", "kmm_set_geometry",
+    "kmm_put_field", "kmm_get_field", "kmm_startup", "kmm_step", "kmm_last_cfl",
+    "kmm_write_restart", "kmm_read_restart", "kmm_write_crestart", "kmm_read_crestart",
+    "kmm_cderiv", "kmm_solve", "kmm_penta", "kmm_vcw3dp", "kmm_stage_times",
+    "kmm_set_timing", "kmm_kernel_times",
+]
+
+
+class KmmConfig(C.Structure):
+    """Mirror of ``kmm_config`` (include/kmm.h)."""
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("nsteps", C.c_int), ("iprnfrq", C.c_int),
+        ("dt", C.c_double), ("gain", C.c_double), ("ugain", C.c_double), ("theta", C.c_double),
+        ("yl", C.c_double), ("zl", C.c_double), ("xl", C.c_double),
+        ("re", C.c_double), ("uinf", C.c_double),
+        ("kwall", C.c_int), ("perturbtime", C.c_int),
+        ("r_tau", C.c_double),
+        ("irstrt", C.c_int), ("crstrt", C.c_int), ("flow_select", C.c_int),
+        ("scalar", C.c_int), ("scl_flag", C.c_int),
+        ("diff", C.c_double), ("deltaT", C.c_double),
+        ("src_start", C.c_int), ("src_stop", C.c_int), ("cadence", C.c_int),
+    ]
+
+
+class KmmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"kmm error {code}: {msg}")
+        self.code = code
+
+
+class CFLError(KmmError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path: str | None = None):
+    """Load libkmm.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FileNotFoundError(f"{p} not built: run `python -m polymer_dns_amd.build`")
+    lib = C.CDLL(p)
+    dp = C.POINTER(C.c_double)
+    lib.kmm_status_string.restype = C.c_char_p
+    lib.kmm_status_string.argtypes = [C.c_int]
+    lib.kmm_config_read.argtypes = [C.c_char_p, C.POINTER(KmmConfig)]
+    lib.kmm_create.argtypes = [C.POINTER(KmmConfig), C.c_int, C.POINTER(C.c_void_p)]
+    lib.kmm_destroy.argtypes = [C.c_void_p]
+    lib.kmm_set_geometry.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+    lib.kmm_put_field.argtypes = [C.c_void_p, C.c_int, dp]
+    lib.kmm_get_field.argtypes = [C.c_void_p, C.c_int, dp]
+    lib.kmm_startup.argtypes = [C.c_void_p]
+    lib.kmm_step.argtypes = [C.c_void_p, C.c_int]
+    lib.kmm_last_cfl.argtypes = [C.c_void_p, dp, C.POINTER(C.c_int)]
+    for n in ("kmm_write_restart", "kmm_read_restart", "kmm_write_crestart", "kmm_read_crestart"):
+        getattr(lib, n).argtypes = [C.c_void_p, C.c_char_p]
+    lib.kmm_cderiv.argtypes = [C.c_void_p, dp, dp]
+    lib.kmm_solve.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, dp, dp, C.c_int]
+    lib.kmm_penta.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_int,
+                              C.c_double, C.c_double, dp, C.c_double, C.c_double, dp]
+    lib.kmm_vcw3dp.argtypes = [C.c_void_p] + [dp] * 13 + [C.c_int]
+    lib.kmm_stage_times.argtypes = [C.c_void_p, dp, dp]
+    lib.kmm_set_timing.argtypes = [C.c_void_p, C.c_int]
+    lib.kmm_kernel_times.argtypes = [C.c_void_p, dp, C.POINTER(C.c_int)]
+    for n in EXPORTS:
+        if n != "kmm_status_string":
+            getattr(lib, n).restype = C.c_int
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _check(lib, rc):
+    if rc != KMM_OK:
+        msg = lib.kmm_status_string(rc).decode(errors="replace")
+        if rc == KMM_E_CFL:
+            raise CFLError(rc, msg)
+        raise KmmError(rc, msg)
+
+
+def read_config(path: str) -> KmmConfig:
+    """Positional dns.config reader of the library (main_dns.f90:934-1068)."""
+    lib = load_library()
+    cfg = KmmConfig()
+    _check(lib, lib.kmm_config_read(path.encode(), C.byref(cfg)))
+    return cfg
+
+
+def make_config(nx, ny, nz, **kw) -> KmmConfig:
+    cfg = KmmConfig()
+    defaults = dict(nsteps=1, iprnfrq=1000, dt=0.005, gain=0.0, ugain=0.0, theta=0.5,
+                    yl=2.0, zl=2.0 * np.pi, xl=4.0 * np.pi, re=2800.0, uinf=0.0, kwall=0,
+                    perturbtime=0, r_tau=float(np.sqrt(3.0 * 2800.0)), irstrt=1, crstrt=0,
+                    flow_select=1, scalar=0, scl_flag=0, diff=1.0, deltaT=0.0,
+                    src_start=0, src_stop=-1, cadence=10)
+    defaults.update(kw)
+    cfg.nx, cfg.ny, cfg.nz = nx, ny, nz
+    for k, v in defaults.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+class Channel:
+    """One channel resident on a GPU, driven through libkmm's C ABI.
+
+    Method names follow the reference: ``startup`` is the new-run branch of
+    ``initial`` (main_dns.f90:1602-2231), ``step`` the time-loop body
+    (main_dns.f90:198-811), ``cderiv``/``solve``/``penta``/``vcw3dp`` the
+    routines of derivatives.f90 and solvers.f90 with their argument meaning.
+    """
+
+    def __init__(self, cfg: KmmConfig, device: int = 0):
+        self.lib = load_library()
+        self.cfg = cfg
+        self.nx, self.ny, self.nz = cfg.nx, cfg.ny, cfg.nz
+        self.nyp, self.nxh = self.ny + 1, self.nx // 2
+        self.mx, self.mz = 3 * self.nx // 2, 3 * self.nz // 2
+        h = C.c_void_p()
+        _check(self.lib, self.lib.kmm_create(C.byref(cfg), device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.kmm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- shapes
+    def shape(self, name):
+        if name in SPECTRAL:
+            return (self.nxh, self.nz, self.nyp), np.complex128
+        if name in MEAN:
+            return (self.nyp,), np.float64
+        if name in PHYS:
+            return (self.mx, self.mz, self.nyp), np.float64
+        raise KeyError(name)
+
+    def put(self, name, arr):
+        shp, dt = self.shape(name)
+        a = np.ascontiguousarray(arr, dtype=dt)
+        if a.shape != shp:
+            raise ValueError(f"{name}: expected {shp}, got {a.shape}")
+        _check(self.lib, self.lib.kmm_put_field(self.h, FIELD_ID[name], _dp(a)))
+
+    def get(self, name):
+        shp, dt = self.shape(name)
+        a = np.empty(shp, dtype=dt)
+        _check(self.lib, self.lib.kmm_get_field(self.h, FIELD_ID[name], _dp(a)))
+        return a
+
+    def set_geometry(self, imatrix):
+        a = np.ascontiguousarray(imatrix, dtype=np.int32)
+        if a.shape != (self.mx, self.mz, self.nyp):
+            raise ValueError("imatrix must have shape (mx, mz, nyp)")
+        _check(self.lib, self.lib.kmm_set_geometry(self.h, a.ctypes.data_as(C.POINTER(C.c_int32))))
+
+    # ---- time stepping
+    def startup(self):
+        _check(self.lib, self.lib.kmm_startup(self.h))
+
+    def step(self, nsteps=1):
+        _check(self.lib, self.lib.kmm_step(self.h, int(nsteps)))
+
+    def last_cfl(self):
+        v = C.c_double()
+        it = C.c_int()
+        _check(self.lib, self.lib.kmm_last_cfl(self.h, C.byref(v), C.byref(it)))
+        return v.value, it.value
+
+    def stage_times(self):
+        a, b = C.c_double(), C.c_double()
+        _check(self.lib, self.lib.kmm_stage_times(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    KERNELS = ("k_pencil", "k_zinv", "k_ibdrag", "k_xplane", "k_zfwd")
+
+    def set_timing(self, on=True):
+        _check(self.lib, self.lib.kmm_set_timing(self.h, int(bool(on))))
+
+    def kernel_times(self):
+        """Average ms per launch of each step kernel over the last step() call."""
+        a = (C.c_double * 5)()
+        n = C.c_int()
+        _check(self.lib, self.lib.kmm_kernel_times(self.h, a, C.byref(n)))
+        return dict(zip(self.KERNELS, list(a))), n.value
+
+    def write_restart(self, path):
+        _check(self.lib, self.lib.kmm_write_restart(self.h, str(path).encode()))
+
+    def read_restart(self, path):
+        _check(self.lib, self.lib.kmm_read_restart(self.h, str(path).encode()))
+
+    def write_crestart(self, path):
+        _check(self.lib, self.lib.kmm_write_crestart(self.h, str(path).encode()))
+
+    def read_crestart(self, path):
+        _check(self.lib, self.lib.kmm_read_crestart(self.h, str(path).encode()))
+
+    # ---- per-routine entries
+    def cderiv(self, f):
+        f = np.ascontiguousarray(f, dtype=np.complex128)
+        df = np.empty_like(f)
+        _check(self.lib, self.lib.kmm_cderiv(self.h, _dp(f), _dp(df)))
+        return df
+
+    def solve(self, u, x, g, dyde, bctop, bcbot, ib):
+        """solve(u,x,g,dyde,bctop,bcbot,ib) on one kx slab u complex (nz, nyp)."""
+        u = np.ascontiguousarray(u, dtype=np.complex128).copy()
+        bt = np.ascontiguousarray(np.broadcast_to(bctop, (self.nz,)), dtype=np.complex128)
+        bb = np.ascontiguousarray(np.broadcast_to(bcbot, (self.nz,)), dtype=np.complex128)
+        _check(self.lib, self.lib.kmm_solve(self.h, _dp(u), x, g, dyde, _dp(bt), _dp(bb), int(ib)))
+        return u
+
+    def penta(self, u, x, g, dyde, ib, at, bt, gtop, ab, bb, gbot):
+        u = np.ascontiguousarray(u, dtype=np.complex128).copy()
+        gt = np.ascontiguousarray(np.broadcast_to(gtop, (self.nz,)), dtype=np.complex128)
+        gb = np.ascontiguousarray(np.broadcast_to(gbot, (self.nz,)), dtype=np.complex128)
+        _check(self.lib, self.lib.kmm_penta(self.h, _dp(u), x, g, dyde, int(ib), at, bt, _dp(gt),
+                                            ab, bb, _dp(gb)))
+        return u
+
+    def vcw3dp(self, u, v, w, omx, omy, omz, sclx=None, scly=None, sclz=None, it=1):
+        ins = [np.ascontiguousarray(a, dtype=np.complex128) for a in (u, v, w, omx, omy, omz)]
+        sc = [np.ascontiguousarray(a, dtype=np.complex128) for a in (sclx, scly, sclz)] \
+            if sclx is not None else []
+        outs = [np.empty_like(ins[0]) for _ in range(4 if sc else 3)]
+        ptrs = [_dp(a) for a in ins] + ([_dp(a) for a in sc] if sc else [None] * 3) + \
+               [_dp(a) for a in outs] + ([None] if not sc else [])
+        _check(self.lib, self.lib.kmm_vcw3dp(self.h, *ptrs, int(it)))
+        return outs  # gn, fn, omz (, scn)

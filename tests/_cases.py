@@ -1,0 +1,35 @@
+"""Shared parity cases: seeded states and oracle runs (test helpers)."""
+import math
+
+import numpy as np
+
+from oracle import kmm_oracle as O
+from polymer_dns_amd import ic
+
+
+def params(nx, ny, nz, scalar=False, **kw):
+    d = dict(xl=4.0 * math.pi, zl=2.0 * math.pi, re=2800.0, dt=0.005,
+             R_tau=math.sqrt(3.0 * 2800.0))
+    d.update(kw)
+    return O.Params(nx=nx, ny=ny, nz=nz, scalar=scalar, **d)
+
+
+def seeded(p, seed, amp=0.05, kmax=8):
+    st = ic.seeded_state(p.nx, p.ny, p.nz, p.xl, p.zl, p.re, p.R_tau, seed, amp=amp,
+                         kmax=min(kmax, p.nxh - 1, p.nzh - 1), yl=p.yl)
+    st = {k: np.array(v) for k, v in st.items()}
+    if p.scalar:
+        rng = np.random.default_rng(seed)
+        sc = np.zeros((p.nxh, p.nz, p.nyp), dtype=np.complex128)
+        sc[0, 0, 0] = 0.5
+        sc[1, 2, :4] = 0.01 * (rng.standard_normal(4) + 1j * rng.standard_normal(4))
+        sc[0, 3, :4] = 0.01 * rng.standard_normal(4)
+        sc[0, p.nz - 3, :4] = np.conj(sc[0, 3, :4])
+        st["scalar"] = sc
+    return st
+
+
+def rel(a, b):
+    d = np.abs(np.asarray(a) - np.asarray(b)).max()
+    s = np.abs(np.asarray(b)).max()
+    return d / s if s > 0 else d

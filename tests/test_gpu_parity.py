@@ -1,0 +1,152 @@
+"""HIP path (through the C ABI) vs the CPU oracle on seeded inputs."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import kmm_oracle as O
+from polymer_dns_amd import kmm
+from tests._cases import params, seeded, rel
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10  # relative, per spectral field (FP64 path)
+
+
+def cfg_for(p, **kw):
+    return kmm.make_config(p.nx, p.ny, p.nz, xl=p.xl, zl=p.zl, yl=p.yl, re=p.re, dt=p.dt,
+                           theta=p.theta, r_tau=p.R_tau, gain=p.gain, ugain=p.ugain,
+                           scalar=int(p.scalar), diff=p.diff, scl_flag=p.scl_flag,
+                           deltaT=p.deltaT, perturbtime=p.perturbtime, **kw)
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("ny", [32, 64, 128])
+def test_cderiv(ny):
+    p = params(32, ny, 32)
+    rng = np.random.default_rng(ny)
+    f = rng.standard_normal((p.nxh, p.nz, p.nyp)) + 1j * rng.standard_normal((p.nxh, p.nz, p.nyp))
+    with kmm.Channel(cfg_for(p)) as ch:
+        d = ch.cderiv(f)
+    assert rel(d, O.cderiv(f, p.dyde)) < 1e-13
+
+
+@pytest.mark.parametrize("ny,g,ib", [(32, 0.0, 0), (64, 1.1e6, 0), (128, 3.7, 0), (64, 0.0, 1), (32, 2.0, 1)])
+def test_solve(ny, g, ib):
+    p = params(32, ny, 16)
+    rng = np.random.default_rng(ny + ib)
+    f = rng.standard_normal((p.nz, p.nyp)) + 1j * rng.standard_normal((p.nz, p.nyp))
+    bt = rng.standard_normal(p.nz) + 1j * rng.standard_normal(p.nz)
+    bb = rng.standard_normal(p.nz) + 1j * rng.standard_normal(p.nz)
+    x = p.wavx[3] ** 2
+    with kmm.Channel(cfg_for(p)) as ch:
+        u = ch.solve(f, x, g, p.dyde, bt, bb, ib)
+    ref = O.solve(f, p.wavz ** 2 + x + g, p.dyde, bt, bb, ib)
+    assert rel(u, ref) < 1e-11
+
+
+@pytest.mark.parametrize("ny,g,robin", [(32, 5.0, (1.0, 0.0, 1.0, 0.0)), (64, 1.1e6, (0.0, 1.0, 0.0, 1.0)),
+                                        (128, 0.0, (1.0, 0.3, 0.5, 0.2))])
+def test_penta(ny, g, robin):
+    p = params(32, ny, 16)
+    at, bt, ab, bb = robin
+    rng = np.random.default_rng(ny)
+    f = rng.standard_normal((p.nz, p.nyp)) + 1j * rng.standard_normal((p.nz, p.nyp))
+    gt = rng.standard_normal(p.nz) + 1j * rng.standard_normal(p.nz)
+    gb = rng.standard_normal(p.nz) + 1j * rng.standard_normal(p.nz)
+    x = p.wavx[2] ** 2
+    with kmm.Channel(cfg_for(p)) as ch:
+        u = ch.penta(f, x, g, p.dyde, 0, at, bt, gt, ab, bb, gb)
+    ref = O.penta(f, p.wavz ** 2 + x + g, p.dyde, 0, at, bt, gt, ab, bb, gb)
+    assert rel(u, ref) < 1e-11
+
+
+@pytest.mark.parametrize("grid", [(32, 32, 32), (64, 64, 64), (128, 128, 128)])
+def test_vcw3dp(grid):
+    nx, ny, nz = grid
+    p = params(nx, ny, nz)
+    st = seeded(p, 7)
+    u, v, w = O.veloc(st["v"], st["omy"], st["u0"], st["w0"], p)
+    omx, omz = O.vort(u, v, w, p)
+    for a in (u, v, w, omx, omz):
+        O.norm(a)
+    ost = dict(fxintg=np.zeros((p.mx, p.mz, p.nyp)), fyintg=np.zeros((p.mx, p.mz, p.nyp)),
+               fzintg=np.zeros((p.mx, p.mz, p.nyp)))
+    gn, fn, oz, _, _ = O.vcw3dp(ost, p, 1, dict(u=u, v=v, w=w, omx=omx, omy=st["omy"], omz=omz))
+    for a in (gn, fn, oz):
+        O.norm(a)
+    with kmm.Channel(cfg_for(p)) as ch:
+        g2, f2, z2 = ch.vcw3dp(u, v, w, omx, st["omy"], omz)
+    for a, b in ((g2, gn), (f2, fn), (z2, oz)):
+        assert rel(a, b) < 1e-12
+
+
+def run_pair(p, nsteps, seed=11, imatrix=None, kw=None):
+    st = seeded(p, seed)
+    sol = O.Solver(p, imatrix=imatrix)
+    ost = {k: v.copy() for k, v in st.items()}
+    sol.startup(ost)
+    ch = kmm.Channel(cfg_for(p, **(kw or {})))
+    if imatrix is not None:
+        ch.set_geometry(imatrix)
+    for k in ("v", "omy", "u0", "w0"):
+        ch.put(k, st[k])
+    if p.scalar:
+        ch.put("scalar", st["scalar"])
+    ch.startup()
+    return sol, ost, ch
+
+
+FIELDS_CHECK = ["v", "omy", "fn", "fnm1", "gn", "gnm1", "u0", "w0", "h1n", "h3n"]
+
+
+def compare(ost, ch, names, tol, tag):
+    worst = 0.0
+    for k in names:
+        r = rel(ch.get(k), ost[k])
+        log(f"  {tag} {k}: rel {r:.3e}")
+        worst = max(worst, r)
+        assert r < tol, (tag, k, r)
+    return worst
+
+
+@pytest.mark.parametrize("grid", [(32, 32, 32), (64, 64, 64)])
+def test_startup_and_steps(grid):
+    p = params(*grid)
+    sol, ost, ch = run_pair(p, 0)
+    compare(ost, ch, FIELDS_CHECK, TOL, "startup")
+    for it in range(10):
+        sol.step(ost)
+        ch.step(1)
+        compare(ost, ch, FIELDS_CHECK, TOL, f"step{it + 1}")
+    ch.close()
+
+
+def test_scalar_steps():
+    p = params(32, 32, 32, scalar=True, diff=0.71, scl_flag=6, deltaT=0.3)
+    sol, ost, ch = run_pair(p, 0)
+    compare(ost, ch, FIELDS_CHECK + ["scalar", "scn", "scnm1"], TOL, "startup")
+    for it in range(5):
+        sol.step(ost)
+        ch.step(1)
+        compare(ost, ch, FIELDS_CHECK + ["scalar", "scn", "scnm1"], TOL, f"step{it + 1}")
+    ch.close()
+
+
+def test_ib_rib_steps():
+    from polymer_dns_amd.restart import rib_geometry
+    p = params(64, 32, 32, gain=20.0, ugain=100.0, dt=0.002)
+    im = rib_geometry(p.nx, p.ny, p.nz, y_cells=4, x_center=int(0.4 * p.mx), x_width=3)
+    sol, ost, ch = run_pair(p, 0, imatrix=im)
+    compare(ost, ch, FIELDS_CHECK, TOL, "startup")
+    for it in range(5):
+        sol.step(ost)
+        ch.step(1)
+        compare(ost, ch, FIELDS_CHECK, TOL, f"step{it + 1}")
+    fx = ch.get("fxintg")
+    assert rel(fx, ost["fxintg"]) < 1e-10
+    ch.close()
