@@ -89,6 +89,26 @@ def cpu_baseline(nsteps=2):
                 sample=f"{nsteps} FP64 steps of the same workload (numpy restatement, single thread)")
 
 
+def run_mode(kmm, precision, steps, warmup):
+    cfg = kmm.make_config(NX, NY, NZ, xl=XL, zl=ZL, re=RE, dt=DT, r_tau=RTAU, gain=GAIN, ugain=UGAIN,
+                          scalar=1, diff=PR, scl_flag=6, deltaT=SRC, precision=precision)
+    ch = kmm.Channel(cfg)
+    ch.set_geometry(rib(NX, NY, NZ))
+    st = workload_state()
+    for k in ("v", "omy", "u0", "w0", "scalar"):
+        ch.put(k, st[k])
+    ch.startup()
+    ch.step(warmup)
+    ch.set_timing(True)
+    t0 = time.perf_counter()
+    ch.step(steps)            # returns after a device synchronisation
+    t = time.perf_counter() - t0
+    ktimes, _ = ch.kernel_times()
+    cfl, it = ch.last_cfl()
+    ch.close()
+    return t, ktimes, cfl, it
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,22 +120,9 @@ def main():
         print(json.dumps({"error": "this build runs one channel on one GPU; --gpus must be 1"}))
         sys.exit(2)
     from polymer_dns_amd import kmm
-    cfg = kmm.make_config(NX, NY, NZ, xl=XL, zl=ZL, re=RE, dt=DT, r_tau=RTAU, gain=GAIN, ugain=UGAIN,
-                          scalar=1, diff=PR, scl_flag=6, deltaT=SRC)
     G = NX * (NY + 1) * NZ
-    ch = kmm.Channel(cfg)
-    ch.set_geometry(rib(NX, NY, NZ))
-    st = workload_state()
-    for k in ("v", "omy", "u0", "w0", "scalar"):
-        ch.put(k, st[k])
-    ch.startup()
-    ch.step(args.warmup)
-    ch.set_timing(True)
-    t0 = time.perf_counter()
-    ch.step(args.steps)            # returns after a device synchronisation
-    t = time.perf_counter() - t0
-    ktimes, nsamp = ch.kernel_times()
-    cfl, it = ch.last_cfl()
+    t, ktimes, cfl, it = run_mode(kmm, 0, args.steps, args.warmup)
+    tm, ktm, _, _ = run_mode(kmm, 1, args.steps, args.warmup)
     ms = 1000.0 * t / args.steps
     value = G * args.steps / t / 1e6
     kb = kernel_bytes(G)
@@ -134,8 +141,11 @@ def main():
         "step_roofline": {"bytes_per_gridpt": 320, "achieved_GBs": round(step_bytes / (ms * 1e-3) / 1e9, 1),
                           "frac": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "cfl_last": cfl, "it": it,
+        "mixed": {"value": round(G * args.steps / tm / 1e6, 2), "ms_per_step": round(1000.0 * tm / args.steps, 4),
+                  "dtype": "f32 transforms/products + f64 state and tau solves",
+                  "kernel_ms": {k: round(v, 4) for k, v in ktm.items()},
+                  "speedup_vs_fp64": round(t / tm, 3)},
     }
-    ch.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

@@ -57,6 +57,8 @@ typedef struct kmm_config {
   double deltaT;          /* scalar source rate for scl_flag 6 */
   int src_start, src_stop;
   int cadence;
+  int precision;          /* 0: FP64 throughout; 1: mixed (FP32 transforms, products
+                             and transfer fields; FP64 state and tau solves) */
 } kmm_config;
 
 typedef struct kmm_ctx kmm_ctx;

@@ -50,13 +50,14 @@ struct kmm_ctx {
   // state
   double2 *V, *OMY, *FN, *FNM1, *GN, *GNM1, *TH, *SCN, *SCNM1;
   double *u0, *w0, *h1n, *h1nm1, *h3n, *h3nm1;
-  double2* NL[4];
-  double2* T[9];
-  double2 *Z, *ZF;
+  void* NL[4];
+  void* T[9];
+  void *Z, *ZF;
   // tables
   double2 *tabH, *tabP, *tabS, *G1, *G2;
   double4* gk;
-  double2 *twM, *twZ, *twX;
+  void *twM, *twZ, *twX;   // transfer-precision twiddles
+  size_t vsz = 16;         // bytes per complex transfer element (16 FP64, 8 mixed)
   double *smz, *smy, *seght_d;
   unsigned long long* cfl_d;
   int cfl_cap = 0;
@@ -205,6 +206,13 @@ static std::vector<double2> twiddles(int n) {
   return t;
 }
 
+static std::vector<float2> twiddles_f(int n) {
+  std::vector<double2> d = twiddles(n);
+  std::vector<float2> f(n);
+  for (int k = 0; k < n; ++k) f[k] = make_float2((float)d[k].x, (float)d[k].y);
+  return f;
+}
+
 static bool fft_len_ok(int n) {
   int m = n;
   while (m % 2 == 0) m /= 2;
@@ -225,6 +233,7 @@ static PencilArgs pencil_args(kmm_ctx* c) {
   if (c->cfg.flow_select == 2 || c->cfg.flow_select == 12 || c->cfg.flow_select == 3) a.gtu = c->cfg.uinf;
   if (c->cfg.flow_select == 3) a.gtw = c->cfg.uinf;
   a.scalar = c->cfg.scalar;
+  a.mixed = c->cfg.precision == 1;
   a.smooth = 1;
   a.first_nl = 1;
   a.smz = c->smz; a.smy = c->smy; a.twM = c->twM;
@@ -243,6 +252,7 @@ static PlaneArgs plane_args(kmm_ctx* c, int it, unsigned long long* cfl) {
   a.nxh = c->nxh; a.nz = c->nz; a.nzh = c->nzh; a.nyp = c->nyp; a.mx = c->mx; a.mz = c->mz; a.P = c->P;
   a.nfin = c->cfg.scalar ? 9 : 6;
   a.nfout = c->cfg.scalar ? 4 : 3;
+  a.mixed = c->cfg.precision == 1;
   a.dt = c->cfg.dt; a.dPdx = c->dPdx;
   a.delxm = c->cfg.xl / c->mx; a.delzm = c->cfg.zl / c->mz;
   a.src_uniform = (c->cfg.scalar && c->cfg.scl_flag == 6) ? c->cfg.deltaT : 0.0;
@@ -270,6 +280,7 @@ extern "C" int kmm_create(const kmm_config* cfg, int device, kmm_ctx** out) {
   if ((nx / 2) % 8 != 0) return fail(KMM_E_CONFIG, "nx/2 must be a multiple of 8");
   if (!fft_len_ok(3 * nx / 2) || !fft_len_ok(3 * nz / 2)) return fail(KMM_E_CONFIG, "3/2 grid not 2^a 3^b");
   if (cfg->theta > 0.99) return fail(KMM_E_CONFIG, "THETA > 0.99 is not supported");
+  if (cfg->precision != 0 && cfg->precision != 1) return fail(KMM_E_CONFIG, "precision must be 0 (FP64) or 1 (mixed)");
   if (!(cfg->flow_select == 1 || cfg->flow_select == 2 || cfg->flow_select == 3 || cfg->flow_select == 11 ||
         cfg->flow_select == 12))
     return fail(KMM_E_CONFIG, "flow_select must be a wall-bounded case with nt=nb=1 (1,2,3,11,12)");
@@ -306,9 +317,11 @@ extern "C" int kmm_create(const kmm_config* cfg, int device, kmm_ctx** out) {
     for (auto p : st) if ((r = dalloc(c, (void**)p, fb))) goto fail_;
     for (auto p : {&c->u0, &c->w0, &c->h1n, &c->h1nm1, &c->h3n, &c->h3nm1})
       if ((r = dalloc(c, (void**)p, c->nyp * sizeof(double)))) goto fail_;
-    for (int i = 0; i < 4; ++i) if ((r = dalloc(c, (void**)&c->NL[i], fb))) goto fail_;
-    for (int i = 0; i < 9; ++i) if ((r = dalloc(c, (void**)&c->T[i], fb))) goto fail_;
-    const size_t zb = (size_t)c->nyp * c->mz * c->nxh * sizeof(double2);
+    c->vsz = (cfg->precision == 1) ? sizeof(float2) : sizeof(double2);
+    const size_t tbv = c->S * c->vsz;
+    for (int i = 0; i < 4; ++i) if ((r = dalloc(c, (void**)&c->NL[i], tbv))) goto fail_;
+    for (int i = 0; i < 9; ++i) if ((r = dalloc(c, (void**)&c->T[i], tbv))) goto fail_;
+    const size_t zb = (size_t)c->nyp * c->mz * c->nxh * c->vsz;
     if ((r = dalloc(c, (void**)&c->Z, 9 * zb))) goto fail_;
     if ((r = dalloc(c, (void**)&c->ZF, 4 * zb))) goto fail_;
     const size_t tb = (size_t)c->P * 65 * sizeof(double2);
@@ -326,9 +339,16 @@ extern "C" int kmm_create(const kmm_config* cfg, int device, kmm_ctx** out) {
       return KMM_OK;
     };
     auto tM = twiddles(2 * ny), tZ = twiddles(c->mz), tX = twiddles(c->mx);
-    if ((r = up((void**)&c->twM, tM.data(), tM.size() * sizeof(double2)))) goto fail_;
-    if ((r = up((void**)&c->twZ, tZ.data(), tZ.size() * sizeof(double2)))) goto fail_;
-    if ((r = up((void**)&c->twX, tX.data(), tX.size() * sizeof(double2)))) goto fail_;
+    if (cfg->precision == 1) {
+      auto fM = twiddles_f(2 * ny), fZ = twiddles_f(c->mz), fX = twiddles_f(c->mx);
+      if ((r = up((void**)&c->twM, fM.data(), fM.size() * sizeof(float2)))) goto fail_;
+      if ((r = up((void**)&c->twZ, fZ.data(), fZ.size() * sizeof(float2)))) goto fail_;
+      if ((r = up((void**)&c->twX, fX.data(), fX.size() * sizeof(float2)))) goto fail_;
+    } else {
+      if ((r = up((void**)&c->twM, tM.data(), tM.size() * sizeof(double2)))) goto fail_;
+      if ((r = up((void**)&c->twZ, tZ.data(), tZ.size() * sizeof(double2)))) goto fail_;
+      if ((r = up((void**)&c->twX, tX.data(), tX.size() * sizeof(double2)))) goto fail_;
+    }
     std::vector<double> smz(nz), smy(c->nyp);
     for (int j = 0; j < nz; ++j) {  // main_dns.f90:712-717
       const int jj = (j < c->nzh) ? j : nz - j;
@@ -849,11 +869,14 @@ extern "C" int kmm_read_crestart(kmm_ctx* c, const char* path) {
 extern "C" int kmm_cderiv(kmm_ctx* c, const double* f, double* df) {
   if (!c || !f || !df) return fail(KMM_E_ARG, "null argument");
   KMM_HIP(hipSetDevice(c->device));
-  int r = upload_field(c, c->T[0], f);
+  // the z-intermediate buffer holds at least two FP64 spectral fields
+  double2* a = static_cast<double2*>(c->ZF);
+  double2* b = a + c->S;
+  int r = upload_field(c, a, f);
   if (r) return r;
-  hipError_t e = kmm::launch_cderiv(c->ny, c->P, c->dyde, c->T[0], c->T[1], c->stream);
+  hipError_t e = kmm::launch_cderiv(c->ny, c->P, c->dyde, a, b, c->stream);
   if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
-  return download_field(c, c->T[1], df);
+  return download_field(c, b, df);
 }
 
 // one kx slab: u complex (nyp,nz) -> device [y][kz] with P = nz pencils
@@ -942,9 +965,9 @@ extern "C" int kmm_vcw3dp(kmm_ctx* c, const double* u, const double* v, const do
   const int nin = sc ? 9 : 6;
   // spectral inputs -> NL buffers (scratch) -> Chebyshev-to-physical -> T
   for (int f = 0; f < nin; ++f) {
-    int r = upload_field(c, c->ZF, ins[f]);
+    int r = upload_field(c, static_cast<double2*>(c->ZF), ins[f]);
     if (r) return r;
-    hipError_t e = kmm::launch_cheb(c->ny, c->P, c->twM, c->ZF, c->T[f], 1, c->stream);
+    hipError_t e = kmm::launch_cheb(c->ny, c->P, c->twM, c->ZF, c->T[f], 1, c->cfg.precision, c->stream);
     if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
   }
   KMM_HIP(hipMemsetAsync(c->cfl_d, 0, sizeof(unsigned long long), c->stream));
@@ -953,9 +976,9 @@ extern "C" int kmm_vcw3dp(kmm_ctx* c, const double* u, const double* v, const do
   double* outs[4] = {gn, fn, omz_out, scn};
   const int nout = sc ? 4 : 3;
   for (int o = 0; o < nout; ++o) {
-    hipError_t e = kmm::launch_cheb(c->ny, c->P, c->twM, c->NL[o], c->T[o], 0, c->stream);
+    hipError_t e = kmm::launch_cheb(c->ny, c->P, c->twM, c->NL[o], c->ZF, 0, c->cfg.precision, c->stream);
     if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
-    r = download_field(c, c->T[o], outs[o]);
+    r = download_field(c, static_cast<double2*>(c->ZF), outs[o]);
     if (r) return r;
   }
   c->nl_valid = false;

@@ -10,20 +10,34 @@ namespace kmm {
 constexpr int WAVE = 64;
 
 // ---------------------------------------------------------------- complex --
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+// Complex arithmetic on double2 (FP64 path) and float2 (FP32 transforms of
+// the mixed-precision path).
+template <typename V> struct Scal;
+template <> struct Scal<double2> { using T = double; };
+template <> struct Scal<float2> { using T = float; };
+template <typename V>
+__device__ __forceinline__ V mkv(typename Scal<V>::T x, typename Scal<V>::T y) { V r; r.x = x; r.y = y; return r; }
+
+template <typename V> __device__ __forceinline__ V cadd(V a, V b) { return mkv<V>(a.x + b.x, a.y + b.y); }
+template <typename V> __device__ __forceinline__ V csub(V a, V b) { return mkv<V>(a.x - b.x, a.y - b.y); }
+template <typename V> __device__ __forceinline__ V cscale(V a, typename Scal<V>::T s) { return mkv<V>(a.x * s, a.y * s); }
+template <typename V> __device__ __forceinline__ V cmul(V a, V b) {
+  return mkv<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+template <typename V> __device__ __forceinline__ V cconj(V a) { return mkv<V>(a.x, -a.y); }
 // i*s*a for real s
-__device__ __forceinline__ double2 cimul(double2 a, double s) { return make_double2(-a.y * s, a.x * s); }
+template <typename V> __device__ __forceinline__ V cimul(V a, typename Scal<V>::T s) { return mkv<V>(-a.y * s, a.x * s); }
 __device__ __forceinline__ double2 czero() { return make_double2(0.0, 0.0); }
+template <typename V> __device__ __forceinline__ V vzero() { return mkv<V>(0, 0); }
 // a + s*b
 __device__ __forceinline__ double2 cfma(double s, double2 b, double2 a) {
   return make_double2(fma(s, b.x, a.x), fma(s, b.y, a.y));
 }
+// precision conversion
+template <typename V> __device__ __forceinline__ V to_v(double2 a) {
+  return mkv<V>((typename Scal<V>::T)a.x, (typename Scal<V>::T)a.y);
+}
+template <typename V> __device__ __forceinline__ double2 to_d(V a) { return make_double2((double)a.x, (double)a.y); }
 
 // ------------------------------------------------------------ wave sync --
 // Orders LDS traffic between lanes of one wavefront (each wave owns its own
@@ -136,37 +150,38 @@ __host__ __device__ constexpr FFTPlan make_plan(int n) {
 
 // twiddle w^k with w = exp(-2 pi i / N) from a table of N entries; DIR=+1
 // (FFTW_BACKWARD) uses the conjugate.
-template <int DIR>
-__device__ __forceinline__ double2 twid(const double2* __restrict__ tw, int k) {
-  double2 t = tw[k];
+template <int DIR, typename V>
+__device__ __forceinline__ V twid(const V* __restrict__ tw, int k) {
+  V t = tw[k];
   if (DIR > 0) t.y = -t.y;
   return t;
 }
 
-template <int DIR>
-__device__ __forceinline__ void dft2(double2& a, double2& b) {
-  double2 t = a;
+template <int DIR, typename V>
+__device__ __forceinline__ void dft2(V& a, V& b) {
+  V t = a;
   a = cadd(t, b);
   b = csub(t, b);
 }
-template <int DIR>
-__device__ __forceinline__ void dft3(double2& a, double2& b, double2& c) {
+template <int DIR, typename V>
+__device__ __forceinline__ void dft3(V& a, V& b, V& c) {
+  using T = typename Scal<V>::T;
   // forward: w = exp(-2 pi i/3); backward conjugate
-  const double s3 = 0.86602540378443864676 * (DIR < 0 ? 1.0 : -1.0);
-  double2 t1 = cadd(b, c);
-  double2 t2 = make_double2(a.x - 0.5 * t1.x, a.y - 0.5 * t1.y);
-  double2 d = csub(b, c);  // -i*s3*d for forward
-  double2 t3 = make_double2(s3 * d.y, -s3 * d.x);
+  const T s3 = (T)0.86602540378443864676 * (DIR < 0 ? (T)1 : (T)-1);
+  V t1 = cadd(b, c);
+  V t2 = mkv<V>(a.x - (T)0.5 * t1.x, a.y - (T)0.5 * t1.y);
+  V d = csub(b, c);
+  V t3 = mkv<V>(s3 * d.y, -s3 * d.x);
   a = cadd(a, t1);
   b = cadd(t2, t3);
   c = csub(t2, t3);
 }
-template <int DIR>
-__device__ __forceinline__ void dft4(double2& a, double2& b, double2& c, double2& d) {
-  double2 t0 = cadd(a, c), t1 = csub(a, c);
-  double2 t2 = cadd(b, d), t3 = csub(b, d);
+template <int DIR, typename V>
+__device__ __forceinline__ void dft4(V& a, V& b, V& c, V& d) {
+  V t0 = cadd(a, c), t1 = csub(a, c);
+  V t2 = cadd(b, d), t3 = csub(b, d);
   // forward: y1 = t1 - i t3, y3 = t1 + i t3
-  double2 it3 = (DIR < 0) ? make_double2(t3.y, -t3.x) : make_double2(-t3.y, t3.x);
+  V it3 = (DIR < 0) ? mkv<V>(t3.y, -t3.x) : mkv<V>(-t3.y, t3.x);
   a = cadd(t0, t2);
   c = csub(t0, t2);
   b = cadd(t1, it3);
@@ -176,8 +191,8 @@ __device__ __forceinline__ void dft4(double2& a, double2& b, double2& c, double2
 // One-wavefront Stockham FFT of length N held in LDS buffer x (scratch y of
 // the same size).  Unnormalised; DIR=-1 is FFTW_FORWARD (exp(-i..)),
 // DIR=+1 is FFTW_BACKWARD.  Returns the buffer holding the result.
-template <int N, int DIR>
-__device__ double2* wave_fft(double2* x, double2* y, const double2* __restrict__ tw, int lane) {
+template <int N, int DIR, typename V>
+__device__ V* wave_fft(V* x, V* y, const V* __restrict__ tw, int lane) {
   constexpr FFTPlan P = make_plan(N);
   int Ns = 1;
 #pragma unroll
@@ -189,7 +204,7 @@ __device__ double2* wave_fft(double2* x, double2* y, const double2* __restrict__
       const int k = j % Ns;
       const int idxD = (j / Ns) * Ns * R + k;
       if (R == 4) {
-        double2 v0 = x[j], v1 = x[j + nb], v2 = x[j + 2 * nb], v3 = x[j + 3 * nb];
+        V v0 = x[j], v1 = x[j + nb], v2 = x[j + 2 * nb], v3 = x[j + 3 * nb];
         if (Ns > 1) {
           v1 = cmul(v1, twid<DIR>(tw, k * tstride));
           v2 = cmul(v2, twid<DIR>(tw, 2 * k * tstride));
@@ -198,7 +213,7 @@ __device__ double2* wave_fft(double2* x, double2* y, const double2* __restrict__
         dft4<DIR>(v0, v1, v2, v3);
         y[idxD] = v0; y[idxD + Ns] = v1; y[idxD + 2 * Ns] = v2; y[idxD + 3 * Ns] = v3;
       } else if (R == 3) {
-        double2 v0 = x[j], v1 = x[j + nb], v2 = x[j + 2 * nb];
+        V v0 = x[j], v1 = x[j + nb], v2 = x[j + 2 * nb];
         if (Ns > 1) {
           v1 = cmul(v1, twid<DIR>(tw, k * tstride));
           v2 = cmul(v2, twid<DIR>(tw, 2 * k * tstride));
@@ -206,14 +221,14 @@ __device__ double2* wave_fft(double2* x, double2* y, const double2* __restrict__
         dft3<DIR>(v0, v1, v2);
         y[idxD] = v0; y[idxD + Ns] = v1; y[idxD + 2 * Ns] = v2;
       } else {
-        double2 v0 = x[j], v1 = x[j + nb];
+        V v0 = x[j], v1 = x[j + nb];
         if (Ns > 1) v1 = cmul(v1, twid<DIR>(tw, k * tstride));
         dft2<DIR>(v0, v1);
         y[idxD] = v0; y[idxD + Ns] = v1;
       }
     }
     wave_sync();
-    double2* t = x; x = y; y = t;
+    V* t = x; x = y; y = t;
     Ns *= R;
   }
   return x;

@@ -17,35 +17,36 @@ struct PencilArgs {
   int scalar;
   int smooth;       // apply spectral smoothing to the NL terms
   int first_nl;     // NL terms come from the NL buffers (else from FN/GN/SCN)
-  double src_uniform;  // unused here (kept for layout stability)
+  int mixed;           // 1: transfer/NL fields and their transforms in FP32
   const double* smz;   // [nz] z smoothing factors
   const double* smy;   // [nyp] y smoothing factors
-  const double2* twM;  // FFT twiddles for length 2*ny
+  const void* twM;     // FFT twiddles for length 2*ny (transfer precision)
   const double2* tabH; // [P][65] 1/d tables, wn = k2 + gH
   const double2* tabP; // wn = k2
   const double2* tabS; // wn = k2 + gS
   const double2* G1;   // [P][ny/2+1] Green's functions (real, lane-pair packed)
   const double2* G2;
   const double4* gk;   // [P] influence coefficients (k1b,k1t,k2b,k2t)
-  const double2 *NL1, *NL2, *NL3, *NLS;   // physical-y NL: (u x w)_x,_y,_z, scalar
+  const void *NL1, *NL2, *NL3, *NLS;   // physical-y NL: (u x w)_x,_y,_z, scalar (transfer precision)
   double2 *V, *OMY, *FN, *FNM1, *GN, *GNM1, *TH, *SCN, *SCNM1;
   double *u0, *w0, *h1n, *h1nm1, *h3n, *h3nm1;
-  double2* T[9];       // transfer: u v w omx omy omz thx thy thz (physical y)
+  void* T[9];          // transfer: u v w omx omy omz thx thy thz (physical y, transfer precision)
 };
 
 // Plane-stage arguments.
 struct PlaneArgs {
   int nxh, nz, nzh, nyp, mx, mz, P;
   int nfin, nfout;     // input / output field counts (6/3 NS, 9/4 with scalar)
+  int mixed;           // 1: FP32 transforms and products
   double dt, dPdx, delxm, delzm, src_uniform;
   int flow_select;
   const double* seght;
-  const double2* twZ;  // length mz twiddles
-  const double2* twX;  // length mx twiddles
-  const double2* in[9];   // spectral-xz, physical-y inputs [y][kx][kz]
-  double2* Z;             // [nfin][nyp][mz][nxh]
-  double2* ZF;            // [nfout][nyp][mz][nxh]
-  double2* out[4];        // NL [y][kx][kz]
+  const void* twZ;     // length mz twiddles
+  const void* twX;     // length mx twiddles
+  const void* in[9];   // spectral-xz, physical-y inputs [y][kx][kz]
+  void* Z;             // [nfin][nyp][mz][nxh]
+  void* ZF;            // [nfout][nyp][mz][nxh]
+  void* out[4];        // NL [y][kx][kz]
   unsigned long long* cfl;  // bit pattern of the max CFL number for this call
   // immersed boundary (Goldstein feedback, solvers.f90:2996-3108)
   int nsrc;
@@ -82,7 +83,7 @@ hipError_t launch_penta(int ny, const PencilArgs& a, const double2* tab, const d
                         double at, double bt, const double2* gtop, double ab, double bb, const double2* gbot,
                         int bottom_sign, const int* kzero, hipStream_t s);
 hipError_t launch_tau_tables_wn(int ny, int P, double dyde, const double* wn, double2* tab, hipStream_t s);
-hipError_t launch_cheb(int ny, int P, const double2* tw, const double2* in, double2* out, int inverse,
+hipError_t launch_cheb(int ny, int P, const void* tw, const void* in, void* out, int inverse, int mixed,
                        hipStream_t s);
 
 }  // namespace kmm

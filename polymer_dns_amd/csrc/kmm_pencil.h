@@ -112,47 +112,53 @@ __device__ __forceinline__ void wall_derivs(const PF& f, double dyde, int lane, 
 
 // ----------------------------------------------- Chebyshev transforms --
 // Inverse (solvers.f90:2241-2546): values at y_k = cos(pi k/NY) of
-// sum_n a_n T_n via a length-2NY FFT of the even extension of c_n/2 * a_n.
-template <int NY>
-__device__ PF cheb_to_phys(const PF& f, double2* buf, double2* scr, const double2* __restrict__ tw, int lane) {
+// sum_n a_n T_n via a length-2NY FFT of the even extension of c_n/2 * a_n,
+// computed in the transfer precision V and stored to dst[y][pencil].
+template <int NY, typename V>
+__device__ void cheb_to_phys_store(const PF& f, V* buf, V* scr, const V* tw, int lane, V* __restrict__ dst,
+                                   size_t P, size_t p) {
   constexpr int L = NY / 2, M = 2 * NY;
   if (lane < L) {
-    double2 a = (lane == 0) ? f.a : cscale(f.a, 0.5);
-    double2 b = cscale(f.b, 0.5);
+    V a = to_v<V>((lane == 0) ? f.a : cscale(f.a, 0.5));
+    V b = to_v<V>(cscale(f.b, 0.5));
     buf[2 * lane] = a;
     buf[2 * lane + 1] = b;
     if (lane > 0) buf[M - 2 * lane] = a;
     buf[M - 2 * lane - 1] = b;
   }
-  if (lane == 0) buf[NY] = f.e;
+  if (lane == 0) buf[NY] = to_v<V>(f.e);
   wave_sync();
-  double2* r = wave_fft<M, -1>(buf, scr, tw, lane);
-  PF o;
-  if (lane < L) { o.a = r[2 * lane]; o.b = r[2 * lane + 1]; } else { o.a = o.b = czero(); }
-  o.e = r[NY];
+  V* r = wave_fft<M, -1>(buf, scr, tw, lane);
+  if (lane < L) {
+    dst[(size_t)(2 * lane) * P + p] = r[2 * lane];
+    dst[(size_t)(2 * lane + 1) * P + p] = r[2 * lane + 1];
+  }
+  if (lane == 0) dst[(size_t)NY * P + p] = r[NY];
   wave_sync();
-  return o;
 }
-// Forward (solvers.f90:3279-3416): DCT-I, /NY, ends halved.
-template <int NY>
-__device__ PF phys_to_cheb(const PF& f, double2* buf, double2* scr, const double2* __restrict__ tw, int lane) {
+// Forward (solvers.f90:3279-3416): DCT-I, /NY, ends halved; input in the
+// transfer precision V, result in FP64.
+template <int NY, typename V>
+__device__ PF phys_to_cheb_load(const V* __restrict__ src, size_t P, size_t p, V* buf, V* scr, const V* tw,
+                                int lane) {
   constexpr int L = NY / 2, M = 2 * NY;
   if (lane < L) {
-    buf[2 * lane] = f.a;
-    buf[2 * lane + 1] = f.b;
-    if (lane > 0) buf[M - 2 * lane] = f.a;
-    buf[M - 2 * lane - 1] = f.b;
+    const V a = src[(size_t)(2 * lane) * P + p], b = src[(size_t)(2 * lane + 1) * P + p];
+    buf[2 * lane] = a;
+    buf[2 * lane + 1] = b;
+    if (lane > 0) buf[M - 2 * lane] = a;
+    buf[M - 2 * lane - 1] = b;
   }
-  if (lane == 0) buf[NY] = f.e;
+  if (lane == 0) buf[NY] = src[(size_t)NY * P + p];
   wave_sync();
-  double2* r = wave_fft<M, -1>(buf, scr, tw, lane);
+  V* r = wave_fft<M, -1>(buf, scr, tw, lane);
   const double s = 1.0 / NY;
   PF o;
   if (lane < L) {
-    o.a = cscale(r[2 * lane], lane == 0 ? 0.5 * s : s);
-    o.b = cscale(r[2 * lane + 1], s);
+    o.a = cscale(to_d(r[2 * lane]), lane == 0 ? 0.5 * s : s);
+    o.b = cscale(to_d(r[2 * lane + 1]), s);
   } else { o.a = o.b = czero(); }
-  o.e = cscale(r[NY], 0.5 * s);
+  o.e = cscale(to_d(r[NY]), 0.5 * s);
   wave_sync();
   return o;
 }

@@ -18,9 +18,16 @@ namespace kmm {
 
 constexpr int PW = 8;  // wavefronts (pencils) per workgroup
 
-template <int NY>
-__device__ __forceinline__ double2* wave_lds(double2* smem, int wv) {
-  return smem + (size_t)wv * 4 * NY;  // 2 buffers of 2*NY
+template <int NY, typename V>
+__device__ __forceinline__ V* wave_lds(V* smem, int wv) {
+  return smem + 2 * NY + (size_t)wv * 4 * NY;  // twiddles [2NY], then 2 buffers of 2*NY per wave
+}
+// block-wide copy of the length-2NY twiddles into LDS (before any early exit)
+template <int NY, typename V>
+__device__ __forceinline__ const V* lds_twiddles(V* smem, const V* __restrict__ tw) {
+  for (int e = threadIdx.x; e < 2 * NY; e += blockDim.x) smem[e] = tw[e];
+  __syncthreads();
+  return smem;
 }
 
 // ---------------------------------------------------------- 1/d tables --
@@ -150,15 +157,18 @@ __device__ __forceinline__ PF smooth_apply(const PF& f, const double* __restrict
 // MODE 0: full step (NL from buffers or FN/GN/SCN), MODE 1: startup
 // (derived fields of the current state only), MODE 2: finish (NL -> FN, GN,
 // SCN, h1n, h3n).
-template <int NY, bool SCALAR, int MODE>
+template <int NY, bool SCALAR, int MODE, typename V>
 __global__ __launch_bounds__(PW * 64) void k_pencil(PencilArgs A) {
   constexpr int L = NY / 2;
-  extern __shared__ double2 smem[];
+  extern __shared__ __align__(16) unsigned char smraw[];
+  V* smem = reinterpret_cast<V*>(smraw);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int p = blockIdx.x * PW + wv;
+  const V* twl = lds_twiddles<NY, V>(smem, static_cast<const V*>(A.twM));
   if (p >= A.P) return;
-  double2* buf = wave_lds<NY>(smem, wv);
-  double2* scr = buf + 2 * NY;
+  V* buf = wave_lds<NY, V>(smem, wv);
+  V* scr = buf + 2 * NY;
+  V* const* TT = reinterpret_cast<V* const*>(A.T);
   const size_t P = (size_t)A.P;
   double kx, kz;
   int kxi, kzi;
@@ -181,7 +191,11 @@ __global__ __launch_bounds__(PW * 64) void k_pencil(PencilArgs A) {
       if (SCALAR) pf_store<NY>(A.SCN, P, p, lane, z);
     }
     if (MODE != 2)
-      for (int t = 0; t < nT; ++t) pf_store<NY>(A.T[t], P, p, lane, z);
+      for (int t = 0; t < nT; ++t) {
+        V* d = TT[t];
+        if (lane < L) { d[(size_t)(2 * lane) * P + p] = vzero<V>(); d[(size_t)(2 * lane + 1) * P + p] = vzero<V>(); }
+        if (lane == 0) d[(size_t)NY * P + p] = vzero<V>();
+      }
     return;
   }
 
@@ -191,9 +205,9 @@ __global__ __launch_bounds__(PW * 64) void k_pencil(PencilArgs A) {
   if (MODE == 0 || MODE == 2) {
     if (MODE == 2 || A.first_nl) {
       const double sz = A.smooth ? A.smz[kzi] : 1.0;
-      PF h1 = phys_to_cheb<NY>(pf_load<NY>(A.NL1, P, p, lane), buf, scr, A.twM, lane);
-      PF h3 = phys_to_cheb<NY>(pf_load<NY>(A.NL3, P, p, lane), buf, scr, A.twM, lane);
-      PF h2 = phys_to_cheb<NY>(pf_load<NY>(A.NL2, P, p, lane), buf, scr, A.twM, lane);
+      PF h1 = phys_to_cheb_load<NY, V>(static_cast<const V*>(A.NL1), P, p, buf, scr, twl, lane);
+      PF h3 = phys_to_cheb_load<NY, V>(static_cast<const V*>(A.NL3), P, p, buf, scr, twl, lane);
+      PF h2 = phys_to_cheb_load<NY, V>(static_cast<const V*>(A.NL2), P, p, buf, scr, twl, lane);
       if (A.smooth) {
         h1 = smooth_apply<NY>(h1, A.smy, sz, lane);
         h2 = smooth_apply<NY>(h2, A.smy, sz, lane);
@@ -208,7 +222,7 @@ __global__ __launch_bounds__(PW * 64) void k_pencil(PencilArgs A) {
       fn = pf_sub(pf_scale(h2, -k2), dx);
       gn = pf_sub(pf_imul(h1, kz), pf_imul(h3, kx));
       if (SCALAR) {
-        sn = phys_to_cheb<NY>(pf_load<NY>(A.NLS, P, p, lane), buf, scr, A.twM, lane);
+        sn = phys_to_cheb_load<NY, V>(static_cast<const V*>(A.NLS), P, p, buf, scr, twl, lane);
         if (A.smooth) sn = smooth_apply<NY>(sn, A.smy, sz, lane);
         if (is00) sn = pf_real(sn);
       }
@@ -351,18 +365,18 @@ __global__ __launch_bounds__(PW * 64) void k_pencil(PencilArgs A) {
   }
   PF ox = pf_sub(cderiv<NY>(w, dy, lane), pf_imul(v, kz));
   PF oz = pf_sub(pf_imul(v, kx), cderiv<NY>(u, dy, lane));
-  pf_store<NY>(A.T[0], P, p, lane, cheb_to_phys<NY>(u, buf, scr, A.twM, lane));
-  pf_store<NY>(A.T[1], P, p, lane, cheb_to_phys<NY>(v, buf, scr, A.twM, lane));
-  pf_store<NY>(A.T[2], P, p, lane, cheb_to_phys<NY>(w, buf, scr, A.twM, lane));
-  pf_store<NY>(A.T[3], P, p, lane, cheb_to_phys<NY>(ox, buf, scr, A.twM, lane));
-  pf_store<NY>(A.T[4], P, p, lane, cheb_to_phys<NY>(omy, buf, scr, A.twM, lane));
-  pf_store<NY>(A.T[5], P, p, lane, cheb_to_phys<NY>(oz, buf, scr, A.twM, lane));
+  cheb_to_phys_store<NY, V>(u, buf, scr, twl, lane, TT[0], P, p);
+  cheb_to_phys_store<NY, V>(v, buf, scr, twl, lane, TT[1], P, p);
+  cheb_to_phys_store<NY, V>(w, buf, scr, twl, lane, TT[2], P, p);
+  cheb_to_phys_store<NY, V>(ox, buf, scr, twl, lane, TT[3], P, p);
+  cheb_to_phys_store<NY, V>(omy, buf, scr, twl, lane, TT[4], P, p);
+  cheb_to_phys_store<NY, V>(oz, buf, scr, twl, lane, TT[5], P, p);
   if (SCALAR) {
     PF tx = pf_imul(sc, kx), tz = pf_imul(sc, kz);
     PF ty = cderiv<NY>(sc, dy, lane);
-    pf_store<NY>(A.T[6], P, p, lane, cheb_to_phys<NY>(tx, buf, scr, A.twM, lane));
-    pf_store<NY>(A.T[7], P, p, lane, cheb_to_phys<NY>(ty, buf, scr, A.twM, lane));
-    pf_store<NY>(A.T[8], P, p, lane, cheb_to_phys<NY>(tz, buf, scr, A.twM, lane));
+    cheb_to_phys_store<NY, V>(tx, buf, scr, twl, lane, TT[6], P, p);
+    cheb_to_phys_store<NY, V>(ty, buf, scr, twl, lane, TT[7], P, p);
+    cheb_to_phys_store<NY, V>(tz, buf, scr, twl, lane, TT[8], P, p);
   }
 }
 
@@ -377,17 +391,24 @@ __global__ __launch_bounds__(PW * 64) void k_cderiv(int P, double dyde, const do
   pf_store<NY>(out, P, p, lane, cderiv<NY>(f, dyde, lane));
 }
 
-template <int NY>
-__global__ __launch_bounds__(PW * 64) void k_cheb(int P, const double2* __restrict__ tw, const double2* __restrict__ in,
-                                                  double2* __restrict__ out, int inverse) {
-  extern __shared__ double2 smem[];
+// inverse: FP64 spectral in -> V physical-y out; forward: V in -> FP64 out
+template <int NY, typename V>
+__global__ __launch_bounds__(PW * 64) void k_cheb(int P, const V* __restrict__ tw, const void* in, void* out,
+                                                  int inverse) {
+  extern __shared__ __align__(16) unsigned char smraw[];
+  V* smem = reinterpret_cast<V*>(smraw);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int p = blockIdx.x * PW + wv;
+  const V* twl = lds_twiddles<NY, V>(smem, tw);
   if (p >= P) return;
-  double2* buf = wave_lds<NY>(smem, wv);
-  PF f = pf_load<NY>(in, P, p, lane);
-  PF r = inverse ? cheb_to_phys<NY>(f, buf, buf + 2 * NY, tw, lane) : phys_to_cheb<NY>(f, buf, buf + 2 * NY, tw, lane);
-  pf_store<NY>(out, P, p, lane, r);
+  V* buf = wave_lds<NY, V>(smem, wv);
+  if (inverse) {
+    PF f = pf_load<NY>(static_cast<const double2*>(in), P, p, lane);
+    cheb_to_phys_store<NY, V>(f, buf, buf + 2 * NY, twl, lane, static_cast<V*>(out), P, p);
+  } else {
+    PF r = phys_to_cheb_load<NY, V>(static_cast<const V*>(in), P, p, buf, buf + 2 * NY, twl, lane);
+    pf_store<NY>(static_cast<double2*>(out), P, p, lane, r);
+  }
 }
 
 template <int NY>
@@ -433,8 +454,8 @@ __global__ __launch_bounds__(PW * 64) void k_penta(int P, double dyde, const dou
   }
 
 static inline int nblk(int P) { return (P + PW - 1) / PW; }
-template <int NY>
-static inline size_t lds_bytes() { return (size_t)PW * 4 * NY * sizeof(double2); }
+template <int NY, typename V>
+static inline size_t lds_bytes() { return (size_t)(PW * 4 * NY + 2 * NY) * sizeof(V); }
 
 hipError_t launch_tau_tables(int ny, const PencilArgs& a, double g, double2* tab, hipStream_t s) {
   KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_tau_tables<NY>), dim3((a.P + 127) / 128), dim3(128), 0, s, a.P, a.nz,
@@ -452,16 +473,20 @@ hipError_t launch_green(int ny, const PencilArgs& a, double pb1, double pt1, dou
                                           pt2, G1, G2, gk));
   return hipGetLastError();
 }
-template <int MODE>
-static hipError_t launch_pencil_mode(int ny, const PencilArgs& a, hipStream_t s) {
+template <int MODE, typename V>
+static hipError_t launch_pencil_mode_v(int ny, const PencilArgs& a, hipStream_t s) {
   if (a.scalar) {
-    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_pencil<NY, true, MODE>), dim3(nblk(a.P)), dim3(PW * 64),
-                                            lds_bytes<NY>(), s, a));
+    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_pencil<NY, true, MODE, V>), dim3(nblk(a.P)), dim3(PW * 64),
+                                           (lds_bytes<NY, V>()), s, a));
   } else {
-    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_pencil<NY, false, MODE>), dim3(nblk(a.P)), dim3(PW * 64),
-                                            lds_bytes<NY>(), s, a));
+    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_pencil<NY, false, MODE, V>), dim3(nblk(a.P)), dim3(PW * 64),
+                                           (lds_bytes<NY, V>()), s, a));
   }
   return hipGetLastError();
+}
+template <int MODE>
+static hipError_t launch_pencil_mode(int ny, const PencilArgs& a, hipStream_t s) {
+  return a.mixed ? launch_pencil_mode_v<MODE, float2>(ny, a, s) : launch_pencil_mode_v<MODE, double2>(ny, a, s);
 }
 hipError_t launch_pencil_step(int ny, const PencilArgs& a, hipStream_t s) { return launch_pencil_mode<0>(ny, a, s); }
 hipError_t launch_pencil_startup(int ny, const PencilArgs& a, hipStream_t s) { return launch_pencil_mode<1>(ny, a, s); }
@@ -471,10 +496,17 @@ hipError_t launch_cderiv(int ny, int P, double dyde, const double2* in, double2*
   KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_cderiv<NY>), dim3(nblk(P)), dim3(PW * 64), 0, s, P, dyde, in, out));
   return hipGetLastError();
 }
-hipError_t launch_cheb(int ny, int P, const double2* tw, const double2* in, double2* out, int inverse,
+hipError_t launch_cheb(int ny, int P, const void* tw, const void* in, void* out, int inverse, int mixed,
                        hipStream_t s) {
-  KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_cheb<NY>), dim3(nblk(P)), dim3(PW * 64), lds_bytes<NY>(), s, P, tw,
-                                          in, out, inverse));
+  if (mixed) {
+    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_cheb<NY, float2>), dim3(nblk(P)), dim3(PW * 64),
+                                           (lds_bytes<NY, float2>()), s, P, static_cast<const float2*>(tw), in, out,
+                                           inverse));
+  } else {
+    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_cheb<NY, double2>), dim3(nblk(P)), dim3(PW * 64),
+                                           (lds_bytes<NY, double2>()), s, P, static_cast<const double2*>(tw), in,
+                                           out, inverse));
+  }
   return hipGetLastError();
 }
 hipError_t launch_solve(int ny, const PencilArgs& a, const double2* tab, const double* wn, double2* u,

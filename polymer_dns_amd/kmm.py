@@ -54,6 +54,7 @@ class KmmConfig(C.Structure):
         ("scalar", C.c_int), ("scl_flag", C.c_int),
         ("diff", C.c_double), ("deltaT", C.c_double),
         ("src_start", C.c_int), ("src_stop", C.c_int), ("cadence", C.c_int),
+        ("precision", C.c_int),
     ]
 
 
@@ -135,7 +136,7 @@ def make_config(nx, ny, nz, **kw) -> KmmConfig:
                     yl=2.0, zl=2.0 * np.pi, xl=4.0 * np.pi, re=2800.0, uinf=0.0, kwall=0,
                     perturbtime=0, r_tau=float(np.sqrt(3.0 * 2800.0)), irstrt=1, crstrt=0,
                     flow_select=1, scalar=0, scl_flag=0, diff=1.0, deltaT=0.0,
-                    src_start=0, src_stop=-1, cadence=10)
+                    src_start=0, src_stop=-1, cadence=10, precision=0)
     defaults.update(kw)
     cfg.nx, cfg.ny, cfg.nz = nx, ny, nz
     for k, v in defaults.items():

@@ -150,3 +150,28 @@ def test_ib_rib_steps():
     fx = ch.get("fxintg")
     assert rel(fx, ost["fxintg"]) < 1e-10
     ch.close()
+
+
+MIXED_TOL = 1e-4   # FP32 transforms/products, FP64 state and tau solves
+
+
+@pytest.mark.parametrize("scalar", [False, True])
+def test_mixed_precision_steps(scalar):
+    """Mixed mode vs the FP64 oracle: per-field relative error stated and
+    bounded over 10 steps (64^3, or 32^3 with the scalar)."""
+    if scalar:
+        p = params(32, 32, 32, scalar=True, diff=0.71, scl_flag=6, deltaT=0.3)
+    else:
+        p = params(64, 64, 64)
+    names = FIELDS_CHECK + (["scalar", "scn", "scnm1"] if scalar else [])
+    sol, ost, ch = run_pair(p, 0, kw=dict(precision=1))
+    errs = {}
+    for it in range(10):
+        sol.step(ost)
+        ch.step(1)
+    for k in names:
+        errs[k] = rel(ch.get(k), ost[k])
+        log(f"  mixed step10 {k}: rel {errs[k]:.3e}")
+    ch.close()
+    assert max(errs.values()) < MIXED_TOL, errs
+    assert max(errs.values()) > 1e-12   # the FP32 path really ran
