@@ -59,6 +59,9 @@ typedef struct kmm_config {
   int cadence;
   int precision;          /* 0: FP64 throughout; 1: mixed (FP32 transforms, products
                              and transfer fields; FP64 state and tau solves) */
+  int particle_flag;      /* 0: tracers (the only kind built) */
+  double ratio, ap, c_mu; /* particle density ratio, radius, C_mu */
+  int cd_switch;          /* drag law; sets the tracer sub-step (part_track.f90:338-346) */
 } kmm_config;
 
 typedef struct kmm_ctx kmm_ctx;
@@ -140,6 +143,42 @@ int kmm_stage_times(kmm_ctx* ctx, double* pencil_ms, double* plane_ms);
  * (ms[5]) over the last kmm_step call. */
 int kmm_set_timing(kmm_ctx* ctx, int on);
 int kmm_kernel_times(kmm_ctx* ctx, double* ms, int* nsamples);
+
+/* ---- Lagrangian tracers (part_track.f90) ----
+ * Positions are on the 3/2 physical grid: x in [0, xl), y in [-yl/2, yl/2],
+ * z in [0, zl).  After kmm_set_particles / kmm_read_particles the next flow
+ * evaluation (kmm_startup, or the next step) sets each particle velocity to
+ * the fluid velocity there (init_part, part_track.f90:229-292, run when
+ * it == irstrt); every later step moves the particles by one explicit Euler
+ * step of dt/substep and re-interpolates (tracer_update, :360-433, called from
+ * vcw3dp, solvers.f90:3463-3466).  A particle leaving the grid makes
+ * kmm_startup / kmm_step return KMM_E_PARTICLE. */
+
+/* Replaces the particles.dat read of init_part (part_track.f90:259-267):
+ * a header line, then one particle per line "x y z u v w tp" (u, v, w and
+ * the release time tp are read and ignored, as in the reference). */
+int kmm_read_particles(kmm_ctx* ctx, const char* path);
+/* Same from arrays (xpart, ypart, zpart of grid_size.f90:39); n = 0 removes
+ * all particles. */
+int kmm_set_particles(kmm_ctx* ctx, int n, const double* x, const double* y, const double* z);
+/* Number of particles set (npart of grid_size.f90:2, a runtime value here). */
+int kmm_particle_count(kmm_ctx* ctx, int* n);
+/* Current xpart..wpart (any pointer may be NULL). */
+int kmm_get_particles(kmm_ctx* ctx, int n, double* x, double* y, double* z, double* u, double* v, double* w);
+/* part-t-XXXXXX.dat writer of part_track (part_track.f90:196-206): Tecplot
+ * point format, columns x y z u v w swirl in E14.6 (swirl is written as 0). */
+int kmm_write_particles(kmm_ctx* ctx, const char* path, int it);
+/* fluid_interp / fluid_interp1 (part_track.f90:437-613) + interpolate3
+ * (:1184-1222): trilinear interpolation of nf physical fields given in
+ * reference layout real(8) (nyp, mz, mx), stored back to back, at n points.
+ * out is (n, nf) Fortran order (out[f*n + j]).  cells (may be NULL) receives
+ * the 0-based imin, jmin, kmax of each point as (n, 3) Fortran order. */
+int kmm_fluid_interp(kmm_ctx* ctx, int n, const double* xp, const double* yp, const double* zp, int nf,
+                     const double* fields, double* out, int* cells);
+/* up3d / vp3d / wp3d (solvers.f90:3113-3115) of the last flow evaluation in
+ * reference layout real(8) (nyp, mz, mx); comp 0, 1, 2 = u, v, w.  Kept only
+ * while particles are set. */
+int kmm_get_physical(kmm_ctx* ctx, int comp, double* out);
 
 #ifdef __cplusplus
 }

@@ -736,6 +736,19 @@ class Solver:
         self.smooth = smoothing_factors(p)
         self.imatrix = imatrix
         self.it = 0
+        self.particles = None
+        self.part_state = 0
+        self.ap = 0.0
+        self.substep = 1
+
+    def set_particles(self, x, y, z, ap=0.0, cd_switch=0, ratio=0.0):
+        """Tracers initialised at the next evaluation (init_part)."""
+        self.particles = dict(x=np.asarray(x, float).copy(), y=np.asarray(y, float).copy(),
+                              z=np.asarray(z, float).copy(), u=np.zeros(len(x)), v=np.zeros(len(x)),
+                              w=np.zeros(len(x)))
+        self.part_state = 1
+        self.ap = ap
+        self.substep = substep_check(self.p, cd_switch, ratio, ap)
 
     # main_dns.f90:1602-2231 (new-run branch, from spectral u/v/w)
     def startup(self, st):
@@ -775,6 +788,15 @@ class Solver:
         gn, fn, omz_o, scn, phys = vcw3dp(st, p, it, fields)
         st.pop("imatrix", None)
         self.last_phys = phys
+        if self.particles is not None:
+            # part_track (part_track.f90:131-168): initialise at the first
+            # evaluation, Euler-advance afterwards
+            uvw = (phys["up"], phys["vp"], phys["wp"])
+            if self.part_state == 1:
+                self.particles = tracer_init(self.particles, uvw, p)
+            else:
+                self.particles = tracer_update(self.particles, uvw, p, self.ap, self.substep)
+            self.part_state = 2
         self.last_fields = fields
         if st["cflmax"] > 1.0:
             raise CFLError(f"CFL failure at time step {it}: {st['cflmax']}")
@@ -866,3 +888,87 @@ class SplitMix64:
         u1 = self.uniform()
         u2 = self.uniform()
         return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+
+
+# --------------------------------------------------------------------------- #
+#  Lagrangian tracers (FFTW_DNS/code/src/dns/part_track.f90)
+# --------------------------------------------------------------------------- #
+def interpolate3(x, y, z, x0, x1, y0, y1, z0, z1, c000, c100, c001, c101, c010, c110, c011, c111):
+    """part_track.f90:1184-1222 -- trilinear; a degenerate axis (x1 <= x0,
+    the periodic wrap cell) gets weight 0, i.e. the lower-corner value."""
+    xd = np.where(x1 > x0, (x - x0) / np.where(x1 > x0, x1 - x0, 1.0), 0.0)
+    yd = np.where(y1 > y0, (y - y0) / np.where(y1 > y0, y1 - y0, 1.0), 0.0)
+    zd = np.where(z1 > z0, (z - z0) / np.where(z1 > z0, z1 - z0, 1.0), 0.0)
+    c00 = c000 * (1.0 - xd) + c100 * xd
+    c01 = c001 * (1.0 - xd) + c101 * xd
+    c10 = c010 * (1.0 - xd) + c110 * xd
+    c11 = c011 * (1.0 - xd) + c111 * xd
+    c0 = c00 * (1.0 - yd) + c10 * yd
+    c1 = c01 * (1.0 - yd) + c11 * yd
+    return c0 * (1.0 - zd) + c1 * zd
+
+
+def tracer_cells(xp, yp, zp, p: Params):
+    """part_track.f90:537-553 -- 0-based cell indices on the 3/2 grid:
+    imin/imax (x), jmin/jmax (z), kmax/kmin (y, kmin = kmax + 1)."""
+    pi = 2.0 * math.acos(0.0)
+    delxm = p.xl / p.mx
+    delzm = p.zl / p.mz
+    imin = np.mod(np.floor(xp / delxm).astype(np.int64) + p.mx, p.mx)
+    imax = np.mod(imin + 1, p.mx)
+    jmin = np.mod(np.floor(zp / delzm).astype(np.int64) + p.mz, p.mz)
+    jmax = np.mod(jmin + 1, p.mz)
+    kmax = np.floor(1.0 + float(p.ny) / pi * np.arccos(2.0 * yp / p.yl)).astype(np.int64) - 1
+    kmin = kmax + 1
+    return imin, imax, jmin, jmax, kmin, kmax
+
+
+def fluid_interp(xp, yp, zp, fields, p: Params):
+    """part_track.f90:518-613 -- trilinear interpolation of physical fields
+    (each (mx, mz, nyp)) at the particle positions."""
+    imin, imax, jmin, jmax, kmin, kmax = tracer_cells(xp, yp, zp, p)
+    if np.any(kmin > p.ny) or np.any(kmax < 0):
+        raise IndexError("particle outside the y range of the grid")
+    delxm = p.xl / p.mx
+    delzm = p.zl / p.mz
+    xmin, xmax = imin * delxm, imax * delxm
+    zmin, zmax = jmin * delzm, jmax * delzm
+    ymin, ymax = p.ycoord[kmin], p.ycoord[kmax]
+    out = []
+    for u in fields:
+        out.append(interpolate3(xp, yp, zp, xmin, xmax, ymin, ymax, zmin, zmax,
+                                u[imin, jmin, kmin], u[imax, jmin, kmin], u[imin, jmax, kmin], u[imax, jmax, kmin],
+                                u[imin, jmin, kmax], u[imax, jmin, kmax], u[imin, jmax, kmax], u[imax, jmax, kmax]))
+    return out
+
+
+def tracer_update(part, phys, p: Params, ap, substep=1):
+    """part_track.f90:360-433 -- one explicit Euler step with dt/substep
+    (part_track.f90:45-50 divides dt by substep and calls tracer_update
+    once), then the velocity is re-interpolated at the new position."""
+    dt = p.dt / float(substep)
+    height = p.ycoord[0] - ap
+    # Fortran MOD of reals truncates toward zero, like fmod
+    x = np.fmod(p.xl + part["x"] + part["u"] * dt, p.xl)
+    y = np.maximum(np.minimum(part["y"] + part["v"] * dt, height), ap + p.ycoord[-1])
+    z = np.fmod(p.zl + part["z"] + part["w"] * dt, p.zl)
+    u, v, w = fluid_interp(x, y, z, phys, p)
+    return dict(x=x, y=y, z=z, u=u, v=v, w=w)
+
+
+def substep_check(p: Params, cd_switch, ratio, ap):
+    """part_track.f90:335-350 -- number of particle sub-steps per flow step."""
+    pi_f = 1.0 / (ratio + 0.5)
+    if cd_switch == 0:
+        tau = 1.0
+    elif cd_switch == 1:
+        tau = p.re * ap ** 2 / (6.0 * pi_f)
+    else:
+        tau = 2.0 * p.re * ap ** 2 / (9.0 * pi_f)
+    return int(math.ceil(max(1.0, 2.0 * p.dt / tau)))
+
+
+def tracer_init(part, phys, p: Params):
+    """init_part (part_track.f90:229-292): particle velocity = fluid velocity."""
+    u, v, w = fluid_interp(part["x"], part["y"], part["z"], phys, p)
+    return dict(x=part["x"].copy(), y=part["y"].copy(), z=part["z"].copy(), u=u, v=v, w=w)

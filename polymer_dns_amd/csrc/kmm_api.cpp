@@ -82,6 +82,16 @@ struct kmm_ctx {
   double ktime[5] = {0, 0, 0, 0, 0};
   int ksamples = 0;
   std::vector<void*> allocs;
+  // Lagrangian tracers
+  int npart = 0;
+  int part_cap = 0;
+  double* part = nullptr;    // [6][part_cap]: x, y, z, u, v, w
+  double* cap = nullptr;     // [3][nyp][mz][mx] physical u, v, w of the last evaluation
+  double* ycoord_d = nullptr;
+  int* perr = nullptr;
+  int part_state = 0;        // 0 none, 1 initialise at the next evaluation, 2 tracking
+  int substep = 1;
+  bool cap_valid = false;
 };
 
 static int dalloc(kmm_ctx* c, void** p, size_t bytes) {
@@ -154,6 +164,8 @@ extern "C" int kmm_config_read(const char* path, kmm_config* cfg) {
   RD(21, perturbtime, int); RD(22, r_tau, double); RD(29, irstrt, int); RD(30, crstrt, int);
   RD(34, flow_select, int); RD(38, scl_flag, int); RD(63, diff, double); RD(64, deltaT, double);
   RD(66, src_start, int); RD(67, src_stop, int); RD(74, cadence, int);
+  RD(33, particle_flag, int); RD(46, ratio, double); RD(47, ap, double); RD(48, c_mu, double);
+  RD(49, cd_switch, int);
 #undef RD
   if (cfg->src_stop < cfg->src_start) cfg->src_stop = cfg->nsteps;  // main_dns.f90:1071-1075
   cfg->scalar = 0;
@@ -358,6 +370,8 @@ extern "C" int kmm_create(const kmm_config* cfg, int device, kmm_ctx** out) {
     if ((r = up((void**)&c->smz, smz.data(), nz * sizeof(double)))) goto fail_;
     if ((r = up((void**)&c->smy, smy.data(), c->nyp * sizeof(double)))) goto fail_;
     if ((r = up((void**)&c->seght_d, c->seght.data(), c->nyp * sizeof(double)))) goto fail_;
+    if ((r = up((void**)&c->ycoord_d, c->ycoord.data(), c->nyp * sizeof(double)))) goto fail_;
+    if ((r = dalloc(c, (void**)&c->perr, sizeof(int)))) goto fail_;
     for (int d = 0; d < 3; ++d) c->fint_full[d].assign((size_t)c->nyp * c->mz * c->mx, 0.0);
     // tau tables and Green's functions (gfcn, main_dns.f90:151-161)
     PencilArgs a = pencil_args(c);
@@ -381,6 +395,8 @@ extern "C" int kmm_destroy(kmm_ctx* c) {
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   for (void* p : c->allocs) hipFree(p);
+  if (c->part) hipFree(c->part);
+  if (c->cap) hipFree(c->cap);
   for (auto& e : c->ev) if (e) hipEventDestroy(e);
   for (auto& e : c->tev) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -604,10 +620,51 @@ extern "C" int kmm_get_field(kmm_ctx* c, int f, double* h) {
 }
 
 // -------------------------------------------------------------- step ----
-static int run_plane(kmm_ctx* c, int it, unsigned long long* cfl, hipEvent_t* ev = nullptr) {
+static kmm::TracerArgs tracer_args(kmm_ctx* c) {
+  kmm::TracerArgs t;
+  std::memset(&t, 0, sizeof(t));
+  t.mx = c->mx; t.mz = c->mz; t.ny = c->ny;
+  t.xl = c->cfg.xl; t.yl = c->cfg.yl; t.zl = c->cfg.zl;
+  t.delxm = c->cfg.xl / c->mx; t.delzm = c->cfg.zl / c->mz;
+  t.pi = 2.0 * std::acos(0.0);
+  t.dt = c->cfg.dt / (double)c->substep;
+  t.ap = c->cfg.ap;
+  t.ycoord = c->ycoord_d;
+  t.err = c->perr;
+  return t;
+}
+
+// One evaluation of vcw3dp's plane stage.  With track set and particles
+// present, the physical velocity is kept and the tracers are initialised or
+// advanced (part_track, solvers.f90:3463-3466).
+static int run_plane(kmm_ctx* c, int it, unsigned long long* cfl, hipEvent_t* ev = nullptr, bool track = false) {
   PlaneArgs pa = plane_args(c, it, cfl);
+  const bool trk = track && c->npart > 0;
+  if (trk) pa.cap = c->cap;
   hipError_t e = ev ? kmm::launch_plane_timed(pa, c->stream, ev) : kmm::launch_plane(pa, c->stream);
   if (e != hipSuccess) return fail(KMM_E_HIP, std::string("plane stage: ") + hipGetErrorString(e));
+  if (trk) {
+    kmm::TracerArgs t = tracer_args(c);
+    t.n = c->npart; t.nf = 3;
+    t.fld = c->cap;
+    t.fstride = (long long)c->nyp * c->mz * c->mx;
+    t.sy = (long long)c->mz * c->mx; t.sz = c->mx; t.sx = 1;
+    t.update = (c->part_state == 2) ? 1 : 0;
+    t.part = c->part;
+    if ((e = kmm::launch_tracer(t, c->stream)) != hipSuccess)
+      return fail(KMM_E_HIP, std::string("tracers: ") + hipGetErrorString(e));
+    c->part_state = 2;
+    c->cap_valid = true;
+  }
+  return KMM_OK;
+}
+
+static int check_particles(kmm_ctx* c) {
+  if (c->npart == 0) return KMM_OK;
+  int h = 0;
+  KMM_HIP(hipMemcpyAsync(&h, c->perr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  KMM_HIP(hipStreamSynchronize(c->stream));
+  if (h) return fail(KMM_E_PARTICLE, "a particle left the grid (cell index out of range or not finite)");
   return KMM_OK;
 }
 
@@ -649,7 +706,7 @@ extern "C" int kmm_startup(kmm_ctx* c) {
   if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
   KMM_HIP(hipMemsetAsync(c->cfl_d, 0, sizeof(unsigned long long), c->stream));
   c->it = 0;
-  int r = run_plane(c, 0, c->cfl_d);
+  int r = run_plane(c, 0, c->cfl_d, nullptr, true);
   if (r) return r;
   a.smooth = 0;  // initial() applies no smoothing (main_dns.f90:2155-2206)
   e = kmm::launch_pencil_finish(c->ny, a, c->stream);
@@ -661,7 +718,8 @@ extern "C" int kmm_startup(kmm_ctx* c) {
   KMM_HIP(hipMemcpyAsync(c->h1nm1, c->h1n, c->nyp * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   KMM_HIP(hipMemcpyAsync(c->h3nm1, c->h3n, c->nyp * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   c->nl_valid = false;
-  return check_cfl(c, 1, 0);
+  r = check_cfl(c, 1, 0);
+  return r ? r : check_particles(c);
 }
 
 extern "C" int kmm_step(kmm_ctx* c, int nsteps) {
@@ -694,7 +752,7 @@ extern "C" int kmm_step(kmm_ctx* c, int nsteps) {
       if (e != hipSuccess) return fail(KMM_E_HIP, std::string("pencil step: ") + hipGetErrorString(e));
       if (ev) hipEventRecord(ev[1], c->stream);
       else if (s == 0) hipEventRecord(c->ev[1], c->stream);
-      int r = run_plane(c, c->it, c->cfl_d + s, ev ? ev + 2 : nullptr);
+      int r = run_plane(c, c->it, c->cfl_d + s, ev ? ev + 2 : nullptr, true);
       if (r) return r;
       if (!ev && s == 0) hipEventRecord(c->ev[2], c->stream);
       c->nl_valid = true;
@@ -718,6 +776,7 @@ extern "C" int kmm_step(kmm_ctx* c, int nsteps) {
       c->t_plane = t2;
     }
     if (r) return r;
+    if ((r = check_particles(c))) return r;
     done += n;
   }
   return KMM_OK;
@@ -983,4 +1042,206 @@ extern "C" int kmm_vcw3dp(kmm_ctx* c, const double* u, const double* v, const do
   }
   c->nl_valid = false;
   return check_cfl(c, 1, it);
+}
+
+// --------------------------------------------------------- particles ----
+static int part_free(kmm_ctx* c) {
+  if (c->part) hipFree(c->part);
+  if (c->cap) hipFree(c->cap);
+  c->part = nullptr;
+  c->cap = nullptr;
+  c->part_cap = 0;
+  return KMM_OK;
+}
+
+extern "C" int kmm_set_particles(kmm_ctx* c, int n, const double* x, const double* y, const double* z) {
+  if (!c || n < 0 || (n > 0 && (!x || !y || !z))) return fail(KMM_E_ARG, "bad argument");
+  KMM_HIP(hipSetDevice(c->device));
+  KMM_HIP(hipStreamSynchronize(c->stream));
+  if (n == 0) {
+    c->npart = 0;
+    c->part_state = 0;
+    c->cap_valid = false;
+    return part_free(c);
+  }
+  if (c->cfg.particle_flag != 0) return fail(KMM_E_CONFIG, "only tracer particles (particle_flag 0) are supported");
+  // substep_check, part_track.f90:335-350
+  const kmm_config& g = c->cfg;
+  double tau = 1.0;
+  if (g.cd_switch != 0) {
+    const double pi_f = 1.0 / (g.ratio + 0.5);
+    tau = (g.cd_switch == 1) ? g.re * (g.ap * g.ap) / (6.0 * pi_f) : 2.0 * g.re * (g.ap * g.ap) / (9.0 * pi_f);
+  }
+  if (!(tau > 0.0) || !std::isfinite(2.0 * g.dt / tau))
+    return fail(KMM_E_CONFIG, "particle time scale is zero (set the particle radius or CD_switch 0)");
+  c->substep = (int)std::ceil(std::max(1.0, 2.0 * g.dt / tau));
+  if (n > c->part_cap) {
+    if (c->part) { hipFree(c->part); c->part = nullptr; }
+    KMM_HIP(hipMalloc((void**)&c->part, (size_t)6 * n * sizeof(double)));
+    c->part_cap = n;
+  }
+  if (!c->cap) KMM_HIP(hipMalloc((void**)&c->cap, (size_t)3 * c->nyp * c->mz * c->mx * sizeof(double)));
+  const size_t b = (size_t)n * sizeof(double);
+  KMM_HIP(hipMemcpyAsync(c->part, x, b, hipMemcpyHostToDevice, c->stream));
+  KMM_HIP(hipMemcpyAsync(c->part + n, y, b, hipMemcpyHostToDevice, c->stream));
+  KMM_HIP(hipMemcpyAsync(c->part + 2 * (size_t)n, z, b, hipMemcpyHostToDevice, c->stream));
+  KMM_HIP(hipMemsetAsync(c->part + 3 * (size_t)n, 0, 3 * b, c->stream));
+  KMM_HIP(hipMemsetAsync(c->perr, 0, sizeof(int), c->stream));
+  KMM_HIP(hipStreamSynchronize(c->stream));
+  c->npart = n;
+  c->part_state = 1;
+  c->cap_valid = false;
+  return KMM_OK;
+}
+
+extern "C" int kmm_read_particles(kmm_ctx* c, const char* path) {
+  if (!c || !path) return fail(KMM_E_ARG, "null argument");
+  std::ifstream f(path);
+  if (!f) return fail(KMM_E_IO, std::string("cannot open ") + path);
+  std::string ln;
+  if (!std::getline(f, ln)) return fail(KMM_E_IO, std::string(path) + ": missing header line");
+  std::vector<double> vals;
+  while (std::getline(f, ln)) {
+    for (auto& ch : ln) {
+      if (ch == 'd' || ch == 'D') ch = 'e';
+      if (ch == ',') ch = ' ';
+    }
+    std::istringstream is(ln);
+    std::string tok;
+    while (is >> tok) {
+      char* end = nullptr;
+      const double v = std::strtod(tok.c_str(), &end);
+      if (end == tok.c_str() || *end != '\0') return fail(KMM_E_IO, std::string(path) + ": bad number '" + tok + "'");
+      vals.push_back(v);
+    }
+  }
+  if (vals.size() % 7 != 0) return fail(KMM_E_IO, std::string(path) + ": expected 7 values per particle");
+  const int n = (int)(vals.size() / 7);
+  std::vector<double> x(n), y(n), z(n);
+  for (int j = 0; j < n; ++j) {
+    x[j] = vals[7 * (size_t)j];
+    y[j] = vals[7 * (size_t)j + 1];
+    z[j] = vals[7 * (size_t)j + 2];
+  }
+  return kmm_set_particles(c, n, x.data(), y.data(), z.data());
+}
+
+extern "C" int kmm_particle_count(kmm_ctx* c, int* n) {
+  if (!c || !n) return fail(KMM_E_ARG, "null argument");
+  *n = c->npart;
+  return KMM_OK;
+}
+
+extern "C" int kmm_get_particles(kmm_ctx* c, int n, double* x, double* y, double* z, double* u, double* v,
+                                 double* w) {
+  if (!c) return fail(KMM_E_ARG, "null argument");
+  if (n != c->npart) return fail(KMM_E_ARG, "n differs from the number of particles set");
+  if (n == 0) return KMM_OK;
+  KMM_HIP(hipSetDevice(c->device));
+  double* dst[6] = {x, y, z, u, v, w};
+  for (int k = 0; k < 6; ++k)
+    if (dst[k])
+      KMM_HIP(hipMemcpyAsync(dst[k], c->part + (size_t)k * n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+  KMM_HIP(hipStreamSynchronize(c->stream));
+  return KMM_OK;
+}
+
+// Fortran Ew.d edit descriptor with w = 14, d = 6: [-]0.ddddddE+ee
+static std::string fortran_e14_6(double v) {
+  char out[32];
+  if (std::isnan(v)) { std::snprintf(out, sizeof out, "%14s", "NaN"); return out; }
+  if (std::isinf(v)) { std::snprintf(out, sizeof out, "%14s", v > 0 ? "Infinity" : "-Infinity"); return out; }
+  std::string digits = "000000";
+  int ex = 0;
+  const bool neg = std::signbit(v) && v != 0.0;
+  if (v != 0.0) {
+    char t[32];
+    std::snprintf(t, sizeof t, "%.5E", std::fabs(v));  // d.dddddE+xx: same 6 significant digits
+    digits = std::string(1, t[0]) + std::string(t + 2, 5);
+    ex = std::atoi(std::strchr(t, 'E') + 1) + 1;
+  }
+  char e[8];
+  if (std::abs(ex) <= 99) std::snprintf(e, sizeof e, "E%c%02d", ex < 0 ? '-' : '+', std::abs(ex));
+  else std::snprintf(e, sizeof e, "%c%03d", ex < 0 ? '-' : '+', std::abs(ex));
+  const std::string body = std::string(neg ? "-" : "") + "0." + digits + e;
+  std::snprintf(out, sizeof out, "%14s", body.c_str());
+  return out;
+}
+
+extern "C" int kmm_write_particles(kmm_ctx* c, const char* path, int it) {
+  if (!c || !path) return fail(KMM_E_ARG, "null argument");
+  const int n = c->npart;
+  std::vector<double> a((size_t)6 * n);
+  if (n) {
+    int r = kmm_get_particles(c, n, a.data(), a.data() + n, a.data() + 2 * (size_t)n, a.data() + 3 * (size_t)n,
+                              a.data() + 4 * (size_t)n, a.data() + 5 * (size_t)n);
+    if (r) return r;
+  }
+  FILE* f = std::fopen(path, "w");
+  if (!f) return fail(KMM_E_IO, std::string("cannot write ") + path);
+  std::fprintf(f, " variables = \"x\", \"y\", \"z\", \"u\", \"v\", \"w\", \"swirl\"\n");
+  std::fprintf(f, " zone f=point t=\"Particles\", solutiontime=%12d \n\n", it);
+  for (int j = 0; j < n; ++j) {
+    std::string line;
+    for (int k = 0; k < 6; ++k) line += fortran_e14_6(a[(size_t)k * n + j]) + " ";
+    line += fortran_e14_6(0.0) + " ";
+    std::fprintf(f, "%s\n", line.c_str());
+  }
+  const bool ok = std::fclose(f) == 0;
+  return ok ? KMM_OK : fail(KMM_E_IO, std::string("cannot write ") + path);
+}
+
+extern "C" int kmm_fluid_interp(kmm_ctx* c, int n, const double* xp, const double* yp, const double* zp, int nf,
+                                const double* fields, double* out, int* cells) {
+  if (!c || n < 0 || nf < 1 || (n > 0 && (!xp || !yp || !zp || !fields || !out))) return fail(KMM_E_ARG, "bad argument");
+  if (n == 0) return KMM_OK;
+  KMM_HIP(hipSetDevice(c->device));
+  const size_t pf = (size_t)c->nyp * c->mz * c->mx;
+  double *df = nullptr, *dp = nullptr, *dout = nullptr;
+  int *dcell = nullptr, *derr = nullptr;
+  hipError_t e = hipMalloc((void**)&df, (size_t)nf * pf * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&dp, (size_t)3 * n * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&dout, (size_t)nf * n * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&dcell, (size_t)3 * n * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc((void**)&derr, sizeof(int));
+  if (e == hipSuccess) e = hipMemcpyAsync(df, fields, (size_t)nf * pf * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dp, xp, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dp + n, yp, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dp + 2 * (size_t)n, zp, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(derr, 0, sizeof(int), c->stream);
+  int herr = 0;
+  if (e == hipSuccess) {
+    kmm::TracerArgs t = tracer_args(c);
+    t.n = n; t.nf = nf;
+    t.fld = df; t.fstride = (long long)pf;
+    t.sy = 1; t.sz = c->nyp; t.sx = (long long)c->nyp * c->mz;  // Fortran (nyp, mz, mx)
+    t.px = dp; t.py = dp + n; t.pz = dp + 2 * (size_t)n;
+    t.out = dout; t.cells = dcell; t.err = derr;
+    e = kmm::launch_tracer(t, c->stream);
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout, (size_t)nf * n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess && cells)
+    e = hipMemcpyAsync(cells, dcell, (size_t)3 * n * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(df); hipFree(dp); hipFree(dout); hipFree(dcell); hipFree(derr);
+  if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
+  if (herr) return fail(KMM_E_PARTICLE, "a point lies outside the grid");
+  return KMM_OK;
+}
+
+extern "C" int kmm_get_physical(kmm_ctx* c, int comp, double* out) {
+  if (!c || !out || comp < 0 || comp > 2) return fail(KMM_E_ARG, "bad argument");
+  if (!c->cap_valid) return fail(KMM_E_ARG, "no physical velocity kept (set particles and evaluate the flow first)");
+  KMM_HIP(hipSetDevice(c->device));
+  const size_t pf = (size_t)c->nyp * c->mz * c->mx;
+  std::vector<double> h(pf);
+  KMM_HIP(hipMemcpyAsync(h.data(), c->cap + (size_t)comp * pf, pf * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  KMM_HIP(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < c->nyp; ++k)
+    for (int jz = 0; jz < c->mz; ++jz)
+      for (int ix = 0; ix < c->mx; ++ix)
+        out[k + (size_t)c->nyp * (jz + (size_t)c->mz * ix)] = h[((size_t)k * c->mz + jz) * c->mx + ix];
+  return KMM_OK;
 }

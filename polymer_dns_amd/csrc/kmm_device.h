@@ -49,6 +49,38 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // ------------------------------------------------------------- shuffles --
+// DPP moves (register-to-register lane exchange, no LDS): lanes whose
+// source is outside the row/wave read 0.
+constexpr int DPP_ROW_SHL = 0x100;   // lane i <- lane i+k within a row of 16
+constexpr int DPP_ROW_SHR = 0x110;   // lane i <- lane i-k within a row of 16
+constexpr int DPP_WAVE_SHL1 = 0x130; // lane i <- lane i+1 across the wave
+constexpr int DPP_WAVE_SHR1 = 0x138; // lane i <- lane i-1 across the wave
+
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+  const long long b = __double_as_longlong(v);
+  int lo = (int)(unsigned long long)b, hi = (int)((unsigned long long)b >> 32);
+  lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, true);
+  hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, true);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+template <int CTRL>
+__device__ __forceinline__ double2 dpp(double2 v) { return make_double2(dpp<CTRL>(v.x), dpp<CTRL>(v.y)); }
+
+__device__ __forceinline__ double rdlane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_readlane((int)(unsigned long long)b, l);
+  int hi = __builtin_amdgcn_readlane((int)((unsigned long long)b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double2 rdlane(double2 v, int l) { return make_double2(rdlane(v.x, l), rdlane(v.y, l)); }
+
+// neighbour exchange (lane+1 / lane-1; 0 beyond the wave ends)
+__device__ __forceinline__ double shdn1(double v) { return dpp<DPP_WAVE_SHL1>(v); }
+__device__ __forceinline__ double2 shdn1(double2 v) { return dpp<DPP_WAVE_SHL1>(v); }
+__device__ __forceinline__ double shup1(double v) { return dpp<DPP_WAVE_SHR1>(v); }
+__device__ __forceinline__ double2 shup1(double2 v) { return dpp<DPP_WAVE_SHR1>(v); }
+
 __device__ __forceinline__ double shdn(double v, int off) { return __shfl_down(v, off, WAVE); }
 __device__ __forceinline__ double shup(double v, int off) { return __shfl_up(v, off, WAVE); }
 __device__ __forceinline__ double2 shdn(double2 v, int off) {
@@ -57,78 +89,104 @@ __device__ __forceinline__ double2 shdn(double2 v, int off) {
 __device__ __forceinline__ double2 shup(double2 v, int off) {
   return make_double2(__shfl_up(v.x, off, WAVE), __shfl_up(v.y, off, WAVE));
 }
-__device__ __forceinline__ double bcast(double v, int src) { return __shfl(v, src, WAVE); }
-__device__ __forceinline__ double2 bcast(double2 v, int src) {
-  return make_double2(__shfl(v.x, src, WAVE), __shfl(v.y, src, WAVE));
+__device__ __forceinline__ double bcast(double v, int src) { return rdlane(v, src); }
+__device__ __forceinline__ double2 bcast(double2 v, int src) { return rdlane(v, src); }
+
+// ---------------------------------------------------------------- scans --
+// All scans run four DPP levels inside each row of 16 lanes and combine the
+// four row aggregates read with v_readlane.
+
+// inclusive suffix sum: r_l = sum_{l' >= l} v_l'
+__device__ __forceinline__ double2 suffix_sum(double2 v, int lane) {
+  const int r = lane & 15;
+  double2 t;
+  t = dpp<DPP_ROW_SHL | 1>(v); if (r + 1 < 16) v = cadd(v, t);
+  t = dpp<DPP_ROW_SHL | 2>(v); if (r + 2 < 16) v = cadd(v, t);
+  t = dpp<DPP_ROW_SHL | 4>(v); if (r + 4 < 16) v = cadd(v, t);
+  t = dpp<DPP_ROW_SHL | 8>(v); if (r + 8 < 16) v = cadd(v, t);
+  const double2 a1 = rdlane(v, 16), a2 = rdlane(v, 32), a3 = rdlane(v, 48);
+  const int row = lane >> 4;
+  double2 add = czero();
+  if (row < 3) add = cadd(add, a3);
+  if (row < 2) add = cadd(add, a2);
+  if (row < 1) add = cadd(add, a1);
+  return cadd(v, add);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, WAVE);
-  return v;
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15;
+  double t;
+  t = dpp<DPP_ROW_SHL | 1>(v); if (r + 1 < 16) v += t;
+  t = dpp<DPP_ROW_SHL | 2>(v); if (r + 2 < 16) v += t;
+  t = dpp<DPP_ROW_SHL | 4>(v); if (r + 4 < 16) v += t;
+  t = dpp<DPP_ROW_SHL | 8>(v); if (r + 8 < 16) v += t;
+  return (rdlane(v, 0) + rdlane(v, 16)) + (rdlane(v, 32) + rdlane(v, 48));
 }
-__device__ __forceinline__ double2 wave_sum(double2 v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    v.x += __shfl_xor(v.x, off, WAVE);
-    v.y += __shfl_xor(v.y, off, WAVE);
-  }
-  return v;
-}
+__device__ __forceinline__ double2 wave_sum(double2 v) { return make_double2(wave_sum(v.x), wave_sum(v.y)); }
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, WAVE));
   return v;
 }
 
-// inclusive suffix sum: r_l = sum_{l' >= l} v_l'
-__device__ __forceinline__ double2 suffix_sum(double2 v, int lane) {
-#pragma unroll
-  for (int off = 1; off < WAVE; off <<= 1) {
-    double2 t = shdn(v, off);
-    if (lane + off < WAVE) v = cadd(v, t);
-  }
-  return v;
-}
-
 // Suffix affine recurrence y_l = w_l + c_l * y_{l+1} over all 64 lanes
-// (y_64 = 0).  A lane with c_l = 0 starts a chain.
-__device__ __forceinline__ double2 suffix_affine(double c, double2 w, int lane) {
-#pragma unroll
-  for (int off = 1; off < WAVE; off <<= 1) {
-    double c2 = shdn(c, off);
-    double2 w2 = shdn(w, off);
-    if (lane + off < WAVE) {
-      w = cfma(c, w2, w);
-      c = c * c2;
-    }
+// (y_64 = 0).  A lane with c_l = 0 starts a chain.  The map of lane l is
+// x -> w_l + c_l x; the scan composes maps over increasing lanes.
+template <typename W>
+__device__ __forceinline__ W affine_apply(double c, W w, W x);
+template <>
+__device__ __forceinline__ double2 affine_apply<double2>(double c, double2 w, double2 x) { return cfma(c, x, w); }
+template <>
+__device__ __forceinline__ double affine_apply<double>(double c, double w, double x) { return fma(c, x, w); }
+
+template <typename W>
+__device__ __forceinline__ W suffix_affine_t(double c, W w, int lane) {
+  const int r = lane & 15;
+#define KMM_SFX_LEVEL(K)                                  \
+  {                                                       \
+    const double c2 = dpp<DPP_ROW_SHL | K>(c);            \
+    const W w2 = dpp<DPP_ROW_SHL | K>(w);                 \
+    if (r + K < 16) { w = affine_apply(c, w, w2); c = c * c2; } \
   }
-  return w;
+  KMM_SFX_LEVEL(1) KMM_SFX_LEVEL(2) KMM_SFX_LEVEL(4) KMM_SFX_LEVEL(8)
+#undef KMM_SFX_LEVEL
+  // row aggregates (maps of whole rows 1..3), composed from the top
+  const double c1 = rdlane(c, 16), c2 = rdlane(c, 32), c3 = rdlane(c, 48);
+  const W w1 = rdlane(w, 16), w2 = rdlane(w, 32), w3 = rdlane(w, 48);
+  // value entering row k from above: y at lane 16k = row map k applied to the rows after it
+  const W y3 = w3;                              // y_64 = 0
+  const W y2 = affine_apply(c2, w2, y3);
+  const W y1 = affine_apply(c1, w1, y2);
+  const int row = lane >> 4;
+  if (row == 3) return w;
+  const W yin = (row == 0) ? y1 : ((row == 1) ? y2 : y3);
+  return affine_apply(c, w, yin);
 }
-__device__ __forceinline__ double suffix_affine_r(double c, double w, int lane) {
-#pragma unroll
-  for (int off = 1; off < WAVE; off <<= 1) {
-    double c2 = shdn(c, off);
-    double w2 = shdn(w, off);
-    if (lane + off < WAVE) {
-      w = fma(c, w2, w);
-      c = c * c2;
-    }
-  }
-  return w;
-}
+__device__ __forceinline__ double2 suffix_affine(double c, double2 w, int lane) { return suffix_affine_t<double2>(c, w, lane); }
+__device__ __forceinline__ double suffix_affine_r(double c, double w, int lane) { return suffix_affine_t<double>(c, w, lane); }
+
 // Prefix affine recurrence y_l = a_l + b_l * y_{l-1} (y_{-1} = 0).
 __device__ __forceinline__ double2 prefix_affine(double b, double2 a, int lane) {
-#pragma unroll
-  for (int off = 1; off < WAVE; off <<= 1) {
-    double b2 = shup(b, off);
-    double2 a2 = shup(a, off);
-    if (lane >= off) {
-      a = cfma(b, a2, a);
-      b = b * b2;
-    }
+  const int r = lane & 15;
+#define KMM_PFX_LEVEL(K)                                  \
+  {                                                       \
+    const double b2 = dpp<DPP_ROW_SHR | K>(b);            \
+    const double2 a2 = dpp<DPP_ROW_SHR | K>(a);           \
+    if (r >= K) { a = cfma(b, a2, a); b = b * b2; }       \
   }
-  return a;
+  KMM_PFX_LEVEL(1) KMM_PFX_LEVEL(2) KMM_PFX_LEVEL(4) KMM_PFX_LEVEL(8)
+#undef KMM_PFX_LEVEL
+  const double b0 = rdlane(b, 15), b1 = rdlane(b, 31), b2 = rdlane(b, 47);
+  const double2 a0 = rdlane(a, 15), a1 = rdlane(a, 31), a2 = rdlane(a, 47);
+  const double2 y0 = a0;                       // y at lane 15 (y_{-1} = 0)
+  const double2 y1 = cfma(b1, y0, a1);         // y at lane 31
+  const double2 y2 = cfma(b2, y1, a2);         // y at lane 47
+  (void)b0;
+  const int row = lane >> 4;
+  if (row == 0) return a;
+  const double2 yin = (row == 1) ? y0 : (row == 2 ? y1 : y2);
+  return cfma(b, yin, a);
 }
 
 // ------------------------------------------------------------------ FFT --
@@ -188,8 +246,13 @@ __device__ __forceinline__ void dft4(V& a, V& b, V& c, V& d) {
   d = csub(t1, it3);
 }
 
+// FFT buffers in LDS are padded by one element every 16 so that the
+// stride-R writes of the Stockham passes do not pile onto the same banks.
+__device__ __forceinline__ int fpad(int i) { return i + (i >> 4); }
+template <int N> struct FFTBuf { static constexpr int len = N + N / 16 + 1; };
+
 // One-wavefront Stockham FFT of length N held in LDS buffer x (scratch y of
-// the same size).  Unnormalised; DIR=-1 is FFTW_FORWARD (exp(-i..)),
+// the same padded size FFTBuf<N>::len, indexed through fpad).  Unnormalised; DIR=-1 is FFTW_FORWARD (exp(-i..)),
 // DIR=+1 is FFTW_BACKWARD.  Returns the buffer holding the result.
 template <int N, int DIR, typename V>
 __device__ V* wave_fft(V* x, V* y, const V* __restrict__ tw, int lane) {
@@ -204,27 +267,27 @@ __device__ V* wave_fft(V* x, V* y, const V* __restrict__ tw, int lane) {
       const int k = j % Ns;
       const int idxD = (j / Ns) * Ns * R + k;
       if (R == 4) {
-        V v0 = x[j], v1 = x[j + nb], v2 = x[j + 2 * nb], v3 = x[j + 3 * nb];
+        V v0 = x[fpad(j)], v1 = x[fpad(j + nb)], v2 = x[fpad(j + 2 * nb)], v3 = x[fpad(j + 3 * nb)];
         if (Ns > 1) {
           v1 = cmul(v1, twid<DIR>(tw, k * tstride));
           v2 = cmul(v2, twid<DIR>(tw, 2 * k * tstride));
           v3 = cmul(v3, twid<DIR>(tw, 3 * k * tstride));
         }
         dft4<DIR>(v0, v1, v2, v3);
-        y[idxD] = v0; y[idxD + Ns] = v1; y[idxD + 2 * Ns] = v2; y[idxD + 3 * Ns] = v3;
+        y[fpad(idxD)] = v0; y[fpad(idxD + Ns)] = v1; y[fpad(idxD + 2 * Ns)] = v2; y[fpad(idxD + 3 * Ns)] = v3;
       } else if (R == 3) {
-        V v0 = x[j], v1 = x[j + nb], v2 = x[j + 2 * nb];
+        V v0 = x[fpad(j)], v1 = x[fpad(j + nb)], v2 = x[fpad(j + 2 * nb)];
         if (Ns > 1) {
           v1 = cmul(v1, twid<DIR>(tw, k * tstride));
           v2 = cmul(v2, twid<DIR>(tw, 2 * k * tstride));
         }
         dft3<DIR>(v0, v1, v2);
-        y[idxD] = v0; y[idxD + Ns] = v1; y[idxD + 2 * Ns] = v2;
+        y[fpad(idxD)] = v0; y[fpad(idxD + Ns)] = v1; y[fpad(idxD + 2 * Ns)] = v2;
       } else {
-        V v0 = x[j], v1 = x[j + nb];
+        V v0 = x[fpad(j)], v1 = x[fpad(j + nb)];
         if (Ns > 1) v1 = cmul(v1, twid<DIR>(tw, k * tstride));
         dft2<DIR>(v0, v1);
-        y[idxD] = v0; y[idxD + Ns] = v1;
+        y[fpad(idxD)] = v0; y[fpad(idxD + Ns)] = v1;
       }
     }
     wave_sync();

@@ -62,7 +62,30 @@ struct PlaneArgs {
   const double* ent_w;    // fspread weight per entry
   const int* fixrow;      // [nyp*mz+1] solid cells per (y,z) row (no -dPdx)
   const int* fixx;        // x index per solid cell
+  double* cap;            // nullptr, or [3][nyp][mz][mx] physical u, v, w (FP64)
 };
+
+// Lagrangian tracers (part_track.f90): particle j has x, y, z, u, v, w at
+// part[0..5][j]; fields are read at fld[f*fstride + k*sy + jz*sz + ix*sx]
+// (k: y index, jz: z index, ix: x index on the 3/2 grid).
+struct TracerArgs {
+  int n, mx, mz, ny;
+  double xl, yl, zl, delxm, delzm, pi;
+  double dt, ap;            // tracer step dt/substep; particle radius
+  const double* ycoord;     // [ny+1]
+  const double* fld;
+  long long fstride, sy, sz, sx;
+  int nf;                   // fields interpolated (3 for tracers)
+  int update;               // 1: Euler position update before interpolation
+  double* part;             // [6][n] (tracer mode)
+  const double* px;         // query positions (interpolation mode, part == nullptr)
+  const double* py;
+  const double* pz;
+  double* out;              // [nf][n] (interpolation mode)
+  int* cells;               // nullptr or [3][n]: imin, jmin, kmax (0-based)
+  int* err;                 // set to 1 when a particle leaves the grid
+};
+hipError_t launch_tracer(const TracerArgs& a, hipStream_t s);
 
 hipError_t launch_tau_tables(int ny, const PencilArgs& a, double g, double2* tab, hipStream_t s);
 hipError_t launch_green(int ny, const PencilArgs& a, double pb1, double pt1, double pb2, double pt2,

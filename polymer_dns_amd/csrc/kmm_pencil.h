@@ -87,8 +87,7 @@ __device__ __forceinline__ PF cderiv(const PF& f, double dyde, int lane) {
   PF d;
   d.a = suffix_sum(go, lane);
   double2 se = suffix_sum(ge, lane);
-  double2 sx = shdn(se, 1);
-  if (lane == WAVE - 1) sx = czero();
+  double2 sx = shdn1(se);  // 0 at lane 63
   d.b = cfma(2.0 * NY * dyde, f.e, sx);
   d.e = czero();
   if (lane == 0) d.a = cscale(d.a, 0.5);
@@ -121,19 +120,19 @@ __device__ void cheb_to_phys_store(const PF& f, V* buf, V* scr, const V* tw, int
   if (lane < L) {
     V a = to_v<V>((lane == 0) ? f.a : cscale(f.a, 0.5));
     V b = to_v<V>(cscale(f.b, 0.5));
-    buf[2 * lane] = a;
-    buf[2 * lane + 1] = b;
-    if (lane > 0) buf[M - 2 * lane] = a;
-    buf[M - 2 * lane - 1] = b;
+    buf[fpad(2 * lane)] = a;
+    buf[fpad(2 * lane + 1)] = b;
+    if (lane > 0) buf[fpad(M - 2 * lane)] = a;
+    buf[fpad(M - 2 * lane - 1)] = b;
   }
-  if (lane == 0) buf[NY] = to_v<V>(f.e);
+  if (lane == 0) buf[fpad(NY)] = to_v<V>(f.e);
   wave_sync();
   V* r = wave_fft<M, -1>(buf, scr, tw, lane);
   if (lane < L) {
-    dst[(size_t)(2 * lane) * P + p] = r[2 * lane];
-    dst[(size_t)(2 * lane + 1) * P + p] = r[2 * lane + 1];
+    dst[(size_t)(2 * lane) * P + p] = r[fpad(2 * lane)];
+    dst[(size_t)(2 * lane + 1) * P + p] = r[fpad(2 * lane + 1)];
   }
-  if (lane == 0) dst[(size_t)NY * P + p] = r[NY];
+  if (lane == 0) dst[(size_t)NY * P + p] = r[fpad(NY)];
   wave_sync();
 }
 // Forward (solvers.f90:3279-3416): DCT-I, /NY, ends halved; input in the
@@ -144,21 +143,21 @@ __device__ PF phys_to_cheb_load(const V* __restrict__ src, size_t P, size_t p, V
   constexpr int L = NY / 2, M = 2 * NY;
   if (lane < L) {
     const V a = src[(size_t)(2 * lane) * P + p], b = src[(size_t)(2 * lane + 1) * P + p];
-    buf[2 * lane] = a;
-    buf[2 * lane + 1] = b;
-    if (lane > 0) buf[M - 2 * lane] = a;
-    buf[M - 2 * lane - 1] = b;
+    buf[fpad(2 * lane)] = a;
+    buf[fpad(2 * lane + 1)] = b;
+    if (lane > 0) buf[fpad(M - 2 * lane)] = a;
+    buf[fpad(M - 2 * lane - 1)] = b;
   }
-  if (lane == 0) buf[NY] = src[(size_t)NY * P + p];
+  if (lane == 0) buf[fpad(NY)] = src[(size_t)NY * P + p];
   wave_sync();
   V* r = wave_fft<M, -1>(buf, scr, tw, lane);
   const double s = 1.0 / NY;
   PF o;
   if (lane < L) {
-    o.a = cscale(to_d(r[2 * lane]), lane == 0 ? 0.5 * s : s);
-    o.b = cscale(to_d(r[2 * lane + 1]), s);
+    o.a = cscale(to_d(r[fpad(2 * lane)]), lane == 0 ? 0.5 * s : s);
+    o.b = cscale(to_d(r[fpad(2 * lane + 1)]), s);
   } else { o.a = o.b = czero(); }
-  o.e = cscale(to_d(r[NY]), 0.5 * s);
+  o.e = cscale(to_d(r[fpad(NY)]), 0.5 * s);
   wave_sync();
   return o;
 }
@@ -191,8 +190,8 @@ template <int NY>
 __device__ __forceinline__ void tau_rhs(const PF& s, int lane, double2& we, double2& wo, double2& wex) {
   constexpr int L = NY / 2;
   const int i = lane;
-  double2 sme = shup(s.a, 1), spe = shdn(s.a, 1);
-  double2 smo = shup(s.b, 1), spo = shdn(s.b, 1);
+  double2 sme = shup1(s.a), spe = shdn1(s.a);
+  double2 smo = shup1(s.b), spo = shdn1(s.b);
   we = czero();
   wo = czero();
   if (i >= 1 && i <= L - 1) {
@@ -239,7 +238,7 @@ __device__ __forceinline__ TauPar tau_eliminate(bool even, double wn, double dyd
   if (even) coef_even<NY>(lane < 1 ? 1 : lane, wn, dyde, a1, a2, a3);
   else coef_odd<NY>(lane < 1 ? 1 : lane, wn, dyde, a1, a2, a3);
   const bool row = (lane >= 1 && lane <= L - 1);
-  double invd_next = shdn(invd, 1);
+  double invd_next = shdn1(invd);
   double rho = row ? a3 * invd_next : 0.0;
   double2 wv = row ? w : czero();
   t.wp = suffix_affine(-rho, wv, lane);
@@ -274,7 +273,7 @@ __device__ __forceinline__ double2 tau_bcsum(const TauPar& t, TW tq, int lane, d
   double wv = row ? T(lane) : 0.0;
   if (lane == L - 1) { wv += k * carry; c = 0.0; }
   double F = suffix_affine_r(c, wv, lane);
-  double Fn = shdn(F, 1);
+  double Fn = shdn1(F);
   if (lane == L - 1) Fn = carry;
   double2 term = row ? cscale(t.wp, Fn * t.invd) : czero();
   double2 s = wave_sum(term);

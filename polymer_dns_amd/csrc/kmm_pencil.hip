@@ -20,7 +20,8 @@ constexpr int PW = 8;  // wavefronts (pencils) per workgroup
 
 template <int NY, typename V>
 __device__ __forceinline__ V* wave_lds(V* smem, int wv) {
-  return smem + 2 * NY + (size_t)wv * 4 * NY;  // twiddles [2NY], then 2 buffers of 2*NY per wave
+  // twiddles [2NY], then two padded FFT buffers of length 2NY per wave
+  return smem + 2 * NY + (size_t)wv * 2 * FFTBuf<2 * NY>::len;
 }
 // block-wide copy of the length-2NY twiddles into LDS (before any early exit)
 template <int NY, typename V>
@@ -167,7 +168,7 @@ __global__ __launch_bounds__(PW * 64) void k_pencil(PencilArgs A) {
   const V* twl = lds_twiddles<NY, V>(smem, static_cast<const V*>(A.twM));
   if (p >= A.P) return;
   V* buf = wave_lds<NY, V>(smem, wv);
-  V* scr = buf + 2 * NY;
+  V* scr = buf + FFTBuf<2 * NY>::len;
   V* const* TT = reinterpret_cast<V* const*>(A.T);
   const size_t P = (size_t)A.P;
   double kx, kz;
@@ -404,9 +405,9 @@ __global__ __launch_bounds__(PW * 64) void k_cheb(int P, const V* __restrict__ t
   V* buf = wave_lds<NY, V>(smem, wv);
   if (inverse) {
     PF f = pf_load<NY>(static_cast<const double2*>(in), P, p, lane);
-    cheb_to_phys_store<NY, V>(f, buf, buf + 2 * NY, twl, lane, static_cast<V*>(out), P, p);
+    cheb_to_phys_store<NY, V>(f, buf, buf + FFTBuf<2 * NY>::len, twl, lane, static_cast<V*>(out), P, p);
   } else {
-    PF r = phys_to_cheb_load<NY, V>(static_cast<const V*>(in), P, p, buf, buf + 2 * NY, twl, lane);
+    PF r = phys_to_cheb_load<NY, V>(static_cast<const V*>(in), P, p, buf, buf + FFTBuf<2 * NY>::len, twl, lane);
     pf_store<NY>(static_cast<double2*>(out), P, p, lane, r);
   }
 }
@@ -455,7 +456,7 @@ __global__ __launch_bounds__(PW * 64) void k_penta(int P, double dyde, const dou
 
 static inline int nblk(int P) { return (P + PW - 1) / PW; }
 template <int NY, typename V>
-static inline size_t lds_bytes() { return (size_t)(PW * 4 * NY + 2 * NY) * sizeof(V); }
+static inline size_t lds_bytes() { return (size_t)(PW * 2 * FFTBuf<2 * NY>::len + 2 * NY) * sizeof(V); }
 
 hipError_t launch_tau_tables(int ny, const PencilArgs& a, double g, double2* tab, hipStream_t s) {
   KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_tau_tables<NY>), dim3((a.P + 127) / 128), dim3(128), 0, s, a.P, a.nz,

@@ -25,24 +25,25 @@ __global__ __launch_bounds__(512) void k_zinv(PlaneArgs A) {
   const int ngx = A.nxh / 8;
   const int y = blockIdx.x / ngx, kx0 = (blockIdx.x % ngx) * 8;
   const int kx = kx0 + wv;
-  V* buf = sm + wv * 2 * MZ;
-  V* scr = buf + MZ;
-  V* tile = sm + 16 * MZ;  // [MZ][8]
-  V* tw = sm + 24 * MZ;     // [MZ] twiddles
+  constexpr int BL = FFTBuf<MZ>::len;
+  V* buf = sm + wv * 2 * BL;
+  V* scr = buf + BL;
+  V* tile = sm + 16 * BL;       // [MZ][8]
+  V* tw = tile + 8 * MZ;        // [MZ] twiddles
   for (int e = threadIdx.x; e < MZ; e += 512) tw[e] = static_cast<const V*>(A.twZ)[e];
   __syncthreads();
 #pragma unroll
   for (int f = 0; f < NFIN; ++f) {
     const V* src = static_cast<const V*>(A.in[f]) + ((size_t)y * A.nxh + kx) * A.nz;
-    for (int z = lane; z < MZ; z += 64) buf[z] = vzero<V>();
+    for (int z = lane; z < MZ; z += 64) buf[fpad(z)] = vzero<V>();
     wave_sync();
     for (int j = lane; j < A.nz; j += 64) {
       const int jj = (j < A.nzh) ? j : (MZ - A.nz + j);
-      buf[jj] = src[j];
+      buf[fpad(jj)] = src[j];
     }
     wave_sync();
     V* r = wave_fft<MZ, +1>(buf, scr, tw, lane);
-    for (int z = lane; z < MZ; z += 64) tile[z * 8 + wv] = r[z];
+    for (int z = lane; z < MZ; z += 64) tile[z * 8 + wv] = r[fpad(z)];
     __syncthreads();
     V* dst = static_cast<V*>(A.Z) + ((size_t)f * A.nyp + y) * MZ * A.nxh;
     for (int e = threadIdx.x; e < MZ * 8; e += 512) {
@@ -100,9 +101,10 @@ __global__ __launch_bounds__(WPB * 64) void k_xplane(PlaneArgs A) {
   const int nzb = A.mz / (2 * WPB);
   const int y = blockIdx.x / nzb;
   const int zA = (blockIdx.x % nzb) * 2 * WPB + 2 * wv, zB = zA + 1;
+  constexpr int BL = FFTBuf<MX>::len;
   V* tw = sm;                              // [MX] twiddles
-  V* buf = sm + MX + (size_t)wv * 2 * MX;
-  V* scr = buf + MX;
+  V* buf = sm + MX + (size_t)wv * 2 * BL;
+  V* scr = buf + BL;
   for (int e = threadIdx.x; e < MX; e += WPB * 64) tw[e] = static_cast<const V*>(A.twX)[e];
   __syncthreads();
   const int nxh = A.nxh;
@@ -113,15 +115,15 @@ __global__ __launch_bounds__(WPB * 64) void k_xplane(PlaneArgs A) {
     const V* zr = static_cast<const V*>(A.Z) + ((size_t)f * A.nyp + y) * A.mz * nxh;
     const V* ra = zr + (size_t)zA * nxh;
     const V* rb = zr + (size_t)zB * nxh;
-    for (int x = lane; x < MX; x += 64) buf[x] = vzero<V>();
+    for (int x = lane; x < MX; x += 64) buf[fpad(x)] = vzero<V>();
     wave_sync();
     for (int k = lane; k < nxh; k += 64) {
       const V ua = ra[k], ub = rb[k];
       if (k == 0) {
         buf[0] = mkv<V>(ua.x, ub.x);
       } else {
-        buf[k] = mkv<V>(ua.x - ub.y, ua.y + ub.x);
-        buf[MX - k] = mkv<V>(ua.x + ub.y, ub.x - ua.y);
+        buf[fpad(k)] = mkv<V>(ua.x - ub.y, ua.y + ub.x);
+        buf[fpad(MX - k)] = mkv<V>(ua.x + ub.y, ub.x - ua.y);
       }
     }
     wave_sync();
@@ -129,9 +131,24 @@ __global__ __launch_bounds__(WPB * 64) void k_xplane(PlaneArgs A) {
 #pragma unroll
     for (int m = 0; m < XPL; ++m) {
       const int x = lane + 64 * m;
-      ph[f][m] = (x < MX) ? r[x] : vzero<V>();
+      ph[f][m] = (x < MX) ? r[fpad(x)] : vzero<V>();
     }
     wave_sync();
+  }
+  // physical u, v, w for particle tracking (up3d/vp3d/wp3d, solvers.f90:3113-3115)
+  if (A.cap) {
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      double* ca = A.cap + (((size_t)f * A.nyp + y) * A.mz + zA) * MX;
+#pragma unroll
+      for (int m = 0; m < XPL; ++m) {
+        const int x = lane + 64 * m;
+        if (x < MX) {
+          ca[x] = (double)ph[f][m].x;
+          ca[MX + x] = (double)ph[f][m].y;
+        }
+      }
+    }
   }
   // pointwise products (solvers.f90:2891-2898), CFL (2985-2986), dPdx (3044)
   const T sgy = (T)A.seght[y];
@@ -205,13 +222,13 @@ __global__ __launch_bounds__(WPB * 64) void k_xplane(PlaneArgs A) {
 #pragma unroll
     for (int m = 0; m < XPL; ++m) {
       const int x = lane + 64 * m;
-      if (x < MX) buf[x] = ph[o][m];
+      if (x < MX) buf[fpad(x)] = ph[o][m];
     }
     wave_sync();
     V* r = wave_fft<MX, -1>(buf, scr, tw, lane);
     V* zo = static_cast<V*>(A.ZF) + ((size_t)o * A.nyp + y) * A.mz * nxh;
     for (int k = lane; k < nxh; k += 64) {
-      const V zk = r[k], zm = r[(MX - k) % MX];
+      const V zk = r[fpad(k)], zm = r[fpad((MX - k) % MX)];
       const V ua = mkv<V>((T)0.5 * (zk.x + zm.x), (T)0.5 * (zk.y - zm.y));
       const V ub = mkv<V>((T)0.5 * (zk.y + zm.y), (T)-0.5 * (zk.x - zm.x));
       zo[(size_t)zA * nxh + k] = ua;
@@ -230,10 +247,11 @@ __global__ __launch_bounds__(512) void k_zfwd(PlaneArgs A) {
   const int ngx = A.nxh / 8;
   const int y = blockIdx.x / ngx, kx0 = (blockIdx.x % ngx) * 8;
   const int kx = kx0 + wv;
-  V* buf = sm + wv * 2 * MZ;
-  V* scr = buf + MZ;
-  V* tile = sm + 16 * MZ;
-  V* tw = sm + 24 * MZ;
+  constexpr int BL = FFTBuf<MZ>::len;
+  V* buf = sm + wv * 2 * BL;
+  V* scr = buf + BL;
+  V* tile = sm + 16 * BL;
+  V* tw = tile + 8 * MZ;
   for (int e = threadIdx.x; e < MZ; e += 512) tw[e] = static_cast<const V*>(A.twZ)[e];
   __syncthreads();
   const T scale = (T)(1.0 / ((double)A.mx * (double)MZ));
@@ -246,13 +264,13 @@ __global__ __launch_bounds__(512) void k_zfwd(PlaneArgs A) {
       tile[e] = srcp[(size_t)z * A.nxh + kx0 + c];
     }
     __syncthreads();
-    for (int z = lane; z < MZ; z += 64) buf[z] = tile[z * 8 + wv];
+    for (int z = lane; z < MZ; z += 64) buf[fpad(z)] = tile[z * 8 + wv];
     wave_sync();
     V* r = wave_fft<MZ, -1>(buf, scr, tw, lane);
     V* o2 = (r == buf) ? scr : buf;
     for (int j = lane; j < nz; j += 64) {
       const int jj = (j < nzh) ? j : (MZ - nz + j);
-      o2[j] = (j == nzh) ? vzero<V>() : cscale(r[jj], scale);
+      o2[j] = (j == nzh) ? vzero<V>() : cscale(r[fpad(jj)], scale);
     }
     wave_sync();
     V* dst = static_cast<V*>(A.out[o]) + ((size_t)y * A.nxh + kx) * nz;
@@ -272,7 +290,7 @@ __global__ __launch_bounds__(512) void k_zfwd(PlaneArgs A) {
 template <int MX, int MZ, int NFIN, int NFOUT, typename V>
 static hipError_t plane_impl(const PlaneArgs& a, hipStream_t s, bool inverse_part, hipEvent_t* ev) {
   const int ngx = a.nxh / 8;
-  const size_t zlds = (size_t)(25 * MZ) * sizeof(V);
+  const size_t zlds = (size_t)(16 * FFTBuf<MZ>::len + 9 * MZ) * sizeof(V);
   if (inverse_part) {
     hipLaunchKernelGGL((k_zinv<MZ, NFIN, V>), dim3(a.nyp * ngx), dim3(512), zlds, s, a);
     if (ev) hipEventRecord(ev[0], s);
@@ -280,7 +298,7 @@ static hipError_t plane_impl(const PlaneArgs& a, hipStream_t s, bool inverse_par
       hipLaunchKernelGGL((k_ibdrag<MX, V>), dim3((a.nsrc + 3) / 4), dim3(256), 0, s, a);
     if (ev) hipEventRecord(ev[1], s);
     constexpr int WPB = 4;
-    const size_t xlds = (size_t)(MX + WPB * 2 * MX) * sizeof(V);
+    const size_t xlds = (size_t)(MX + WPB * 2 * FFTBuf<MX>::len) * sizeof(V);
     hipLaunchKernelGGL((k_xplane<MX, NFIN, NFOUT, WPB, V>), dim3(a.nyp * (a.mz / (2 * WPB))), dim3(WPB * 64), xlds,
                        s, a);
     if (ev) hipEventRecord(ev[2], s);

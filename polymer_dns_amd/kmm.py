@@ -37,6 +37,8 @@ EXPORTS = [
     "kmm_write_restart", "kmm_read_restart", "kmm_write_crestart", "kmm_read_crestart",
     "kmm_cderiv", "kmm_solve", "kmm_penta", "kmm_vcw3dp", "kmm_stage_times",
     "kmm_set_timing", "kmm_kernel_times",
+    "kmm_read_particles", "kmm_set_particles", "kmm_particle_count", "kmm_get_particles", "kmm_write_particles",
+    "kmm_fluid_interp", "kmm_get_physical",
 ]
 
 
@@ -55,6 +57,9 @@ class KmmConfig(C.Structure):
         ("diff", C.c_double), ("deltaT", C.c_double),
         ("src_start", C.c_int), ("src_stop", C.c_int), ("cadence", C.c_int),
         ("precision", C.c_int),
+        ("particle_flag", C.c_int),
+        ("ratio", C.c_double), ("ap", C.c_double), ("c_mu", C.c_double),
+        ("cd_switch", C.c_int),
     ]
 
 
@@ -102,6 +107,14 @@ def load_library(path: str | None = None):
     lib.kmm_stage_times.argtypes = [C.c_void_p, dp, dp]
     lib.kmm_set_timing.argtypes = [C.c_void_p, C.c_int]
     lib.kmm_kernel_times.argtypes = [C.c_void_p, dp, C.POINTER(C.c_int)]
+    ip = C.POINTER(C.c_int)
+    lib.kmm_read_particles.argtypes = [C.c_void_p, C.c_char_p]
+    lib.kmm_particle_count.argtypes = [C.c_void_p, ip]
+    lib.kmm_set_particles.argtypes = [C.c_void_p, C.c_int, dp, dp, dp]
+    lib.kmm_get_particles.argtypes = [C.c_void_p, C.c_int] + [dp] * 6
+    lib.kmm_write_particles.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+    lib.kmm_fluid_interp.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, C.c_int, dp, dp, ip]
+    lib.kmm_get_physical.argtypes = [C.c_void_p, C.c_int, dp]
     for n in EXPORTS:
         if n != "kmm_status_string":
             getattr(lib, n).restype = C.c_int
@@ -136,7 +149,8 @@ def make_config(nx, ny, nz, **kw) -> KmmConfig:
                     yl=2.0, zl=2.0 * np.pi, xl=4.0 * np.pi, re=2800.0, uinf=0.0, kwall=0,
                     perturbtime=0, r_tau=float(np.sqrt(3.0 * 2800.0)), irstrt=1, crstrt=0,
                     flow_select=1, scalar=0, scl_flag=0, diff=1.0, deltaT=0.0,
-                    src_start=0, src_stop=-1, cadence=10, precision=0)
+                    src_start=0, src_stop=-1, cadence=10, precision=0,
+                    particle_flag=0, ratio=0.0, ap=0.0, c_mu=1.0, cd_switch=0)
     defaults.update(kw)
     cfg.nx, cfg.ny, cfg.nz = nx, ny, nz
     for k, v in defaults.items():
@@ -283,3 +297,51 @@ class Channel:
                [_dp(a) for a in outs] + ([None] if not sc else [])
         _check(self.lib, self.lib.kmm_vcw3dp(self.h, *ptrs, int(it)))
         return outs  # gn, fn, omz (, scn)
+
+    # ---- Lagrangian tracers (part_track.f90) ----
+    def set_particles(self, x, y, z):
+        """xpart, ypart, zpart; velocities are taken from the fluid at the
+        next flow evaluation (init_part, part_track.f90:229-292)."""
+        x, y, z = (np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (x, y, z))
+        if not (len(x) == len(y) == len(z)):
+            raise ValueError("x, y, z must have the same length")
+        _check(self.lib, self.lib.kmm_set_particles(self.h, len(x), _dp(x), _dp(y), _dp(z)))
+
+    def read_particles(self, path):
+        """particles.dat: a header line, then x y z u v w tp per particle."""
+        _check(self.lib, self.lib.kmm_read_particles(self.h, str(path).encode()))
+
+    def particle_count(self):
+        n = C.c_int(0)
+        _check(self.lib, self.lib.kmm_particle_count(self.h, C.byref(n)))
+        return n.value
+
+    def particles(self):
+        """dict of x, y, z, u, v, w arrays (xpart..wpart)."""
+        n = self.particle_count()
+        out = {k: np.empty(n) for k in ("x", "y", "z", "u", "v", "w")}
+        if n:
+            _check(self.lib, self.lib.kmm_get_particles(self.h, n, *[_dp(out[k]) for k in "xyzuvw"]))
+        return out
+
+    def write_particles(self, path, it):
+        _check(self.lib, self.lib.kmm_write_particles(self.h, str(path).encode(), int(it)))
+
+    def fluid_interp(self, xp, yp, zp, fields, return_cells=False):
+        """Trilinear interpolation (fluid_interp / interpolate3) of physical
+        fields shaped (mx, mz, nyp) in C order == Fortran (nyp, mz, mx)."""
+        xp, yp, zp = (np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (xp, yp, zp))
+        fl = np.ascontiguousarray(np.stack([np.asarray(f, dtype=np.float64) for f in fields]))
+        nf, n = fl.shape[0], len(xp)
+        out = np.empty((nf, n))
+        cells = np.empty((3, n), dtype=np.int32)
+        _check(self.lib, self.lib.kmm_fluid_interp(self.h, n, _dp(xp), _dp(yp), _dp(zp), nf, _dp(fl), _dp(out),
+                                                   cells.ctypes.data_as(C.POINTER(C.c_int))))
+        return (out, cells) if return_cells else out
+
+    def physical(self, comp):
+        """up3d / vp3d / wp3d of the last flow evaluation, shape (mx, mz, nyp)."""
+        mx, mz = 3 * self.nx // 2, 3 * self.nz // 2
+        out = np.empty((mx, mz, self.nyp))
+        _check(self.lib, self.lib.kmm_get_physical(self.h, int(comp), _dp(out)))
+        return out

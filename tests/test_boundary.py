@@ -50,6 +50,9 @@ def test_config_reader_positional():
     assert cfg.deltaT == 1.0          # Fortran d exponent
     assert cfg.src_stop == cfg.nsteps  # src_stop < src_start -> nsteps
     assert cfg.cadence == 10
+    # particle block (main_dns.f90:987, 1013-1016)
+    assert cfg.particle_flag == 0 and cfg.cd_switch == 2
+    assert cfg.ratio == 0.0 and cfg.ap == pytest.approx(0.0085) and cfg.c_mu == 1.0
 
 
 def test_config_reader_rejects_short_file(tmp_path):

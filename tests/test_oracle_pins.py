@@ -332,3 +332,89 @@ def test_ib_spreading_matches_literal_reference_loop():
     for a, b in zip(vw, ref):
         assert np.abs(a - b).max() < 1e-13
     assert np.abs(fx1 - fx2).max() == 0
+
+
+# ------------------------------------------------------------------ tracers
+def _grid(p):
+    X = (np.arange(p.mx) * (p.xl / p.mx))[:, None, None]
+    Z = (np.arange(p.mz) * (p.zl / p.mz))[None, :, None]
+    Y = p.ycoord[None, None, :]
+    return X, Y, Z
+
+
+def test_trilinear_fields_are_reproduced_exactly():
+    p = params(16, 16, 12)
+    X, Y, Z = _grid(p)
+    f = 0.3 + 1.7 * X - 0.4 * Z + 2.2 * Y + 0.9 * X * Y - 0.6 * Y * Z + 0.25 * X * Z + 0.1 * X * Y * Z
+    rng = np.random.default_rng(5)
+    n = 2000
+    # stay out of the periodic wrap cell, where the rule is nearest-lower
+    x = rng.uniform(0, p.xl * (p.mx - 1) / p.mx, n)
+    z = rng.uniform(0, p.zl * (p.mz - 1) / p.mz, n)
+    y = rng.uniform(-0.999, 0.999, n)
+    (g,) = O.fluid_interp(x, y, z, [f], p)
+    exact = 0.3 + 1.7 * x - 0.4 * z + 2.2 * y + 0.9 * x * y - 0.6 * y * z + 0.25 * x * z + 0.1 * x * y * z
+    assert np.abs(g - exact).max() < 1e-12
+
+
+def test_wrap_cell_takes_lower_value_and_nodes_return_node_values():
+    p = params(8, 8, 8)
+    rng = np.random.default_rng(6)
+    f = rng.standard_normal((p.mx, p.mz, p.nyp))
+    dx, dz = p.xl / p.mx, p.zl / p.mz
+    # x inside the last cell (imax wraps to 0): value is independent of x
+    k, j = 3, 4
+    y, z = p.ycoord[k], j * dz
+    xs = (p.mx - 1) * dx + np.array([0.0, 0.3, 0.7, 0.99]) * dx
+    (g,) = O.fluid_interp(xs, np.full(4, y), np.full(4, z), [f], p)
+    assert np.all(g == f[p.mx - 1, j, k])
+    # nodes
+    ii, jj, kk = rng.integers(0, p.mx, 50), rng.integers(0, p.mz, 50), rng.integers(1, p.ny, 50)
+    (g,) = O.fluid_interp(ii * dx, p.ycoord[kk], jj * dz, [f], p)
+    assert np.abs(g - f[ii, jj, kk]).max() < 1e-13
+
+
+def test_y_cell_search_matches_brute_force():
+    """kmax = floor(1 + ny/pi acos(2y/yl)) is the Gauss-Lobatto interval
+    holding y: ycoord[kmin] <= y <= ycoord[kmax] (0-based, kmin = kmax + 1)."""
+    for ny in (16, 32, 64, 128):
+        p = params(8, ny, 8)
+        y = np.random.default_rng(ny).uniform(-1.0, 1.0, 5000)
+        _, _, _, _, kmin, kmax = O.tracer_cells(np.zeros_like(y), y, np.zeros_like(y), p)
+        brute = np.array([np.nonzero(p.ycoord >= yy)[0].max() for yy in y])
+        brute = np.minimum(brute, ny - 1)
+        assert np.array_equal(kmax, brute)
+        assert np.all(p.ycoord[kmin] <= y) and np.all(y <= p.ycoord[kmax])
+
+
+def test_tracer_uniform_flow_displacement_and_wall_clamp():
+    p = params(8, 8, 8, dt=0.01)
+    U, V, W = 1.3, 0.7, -2.1
+    shape = (p.mx, p.mz, p.nyp)
+    phys = (np.full(shape, U), np.full(shape, V), np.full(shape, W))
+    rng = np.random.default_rng(7)
+    n = 300
+    part = dict(x=rng.uniform(0, p.xl, n), y=rng.uniform(-0.9, 0.9, n), z=rng.uniform(0, p.zl, n))
+    part = O.tracer_init(part, phys, p)
+    assert np.allclose(part["u"], U, rtol=0, atol=1e-14) and np.allclose(part["w"], W, rtol=0, atol=1e-14)
+    x0, y0, z0 = part["x"].copy(), part["y"].copy(), part["z"].copy()
+    ap = 0.01
+    for s in range(1, 40):
+        part = O.tracer_update(part, phys, p, ap=ap)
+        dxs = np.angle(np.exp(2j * np.pi * (part["x"] - x0 - s * U * p.dt) / p.xl)) * p.xl / (2 * np.pi)
+        dzs = np.angle(np.exp(2j * np.pi * (part["z"] - z0 - s * W * p.dt) / p.zl)) * p.zl / (2 * np.pi)
+        assert np.abs(dxs).max() < 1e-12 and np.abs(dzs).max() < 1e-12
+        assert np.all((part["x"] >= 0) & (part["x"] < p.xl)) and np.all((part["z"] >= 0) & (part["z"] < p.zl))
+        yexp = np.minimum(y0 + s * V * p.dt, 1.0 - ap)  # clamp is idempotent for a constant V > 0
+        assert np.abs(part["y"] - yexp).max() < 1e-12
+
+
+def test_substep_closed_forms():
+    p = params(8, 8, 8, dt=0.01, re=1000.0)
+    assert O.substep_check(p, 0, 0.0, 0.0) == 1
+    # Stokes time tau = re a^2 / (6 Pi_f), Pi_f = 1 / (ratio + 1/2)
+    a, ratio = 0.002, 1.5
+    tau = 1000.0 * a * a / (6.0 / (ratio + 0.5))
+    assert O.substep_check(p, 1, ratio, a) == math.ceil(2 * 0.01 / tau)
+    tau2 = 2 * 1000.0 * a * a / (9.0 / (ratio + 0.5))
+    assert O.substep_check(p, 2, ratio, a) == math.ceil(2 * 0.01 / tau2)
