@@ -64,6 +64,21 @@ class Params:
     diff: float = 1.0
     scl_flag: int = 0
     deltaT: float = 0.0
+    # polymer (POLYMER build, FENE-P; dns.config lines 35-37, 55-62, 66)
+    polymer: bool = False
+    zlmax: float = 30.0       # L (maximum extensibility); Peterlin uses L^2
+    tpoly: float = 0.1        # relaxation time lambda
+    alpha_poly: float = 0.0
+    diffpoly: float = 0.5     # artificial diffusivity 1/(re*diffpoly)
+    c11z: float = 1.0
+    c22z: float = 1.0
+    c33z: float = 1.0
+    qbeta: float = 0.9        # solvent viscosity ratio beta
+    itarget: int = 0          # 0: uniform beta = qbeta
+    ipeter: int = 1           # 1: FENE-P Peterlin function, 0: Oldroyd-B
+    src_start: int = 0
+    irstrt: int = 1
+    crstrt: int = 0
 
     def __post_init__(self):
         assert self.ny % 2 == 0 and self.nz % 2 == 0 and self.nx % 2 == 0
@@ -687,7 +702,120 @@ def vcw3dp(st, p: Params, it, fields):
     omz_o = to_spectral(vwz, p)
     scn = to_spectral(vc, p) if vc is not None else None
     phys = dict(up=up, vp=vp, wp=wp)
+    if "C" in fields:
+        gv = fields["gradv"]
+        gup = [[to_physical(gv[f"u{i}{k}"], p) for k in (1, 2, 3)] for i in (1, 2, 3)]
+        cp = {ij: to_physical(fields["C"][ij], p) for ij in POLY}
+        dcp = {ij: tuple(to_physical(a, p) for a in fields["dC"][ij]) for ij in POLY}
+        cn, strp, qpp = polymer_products(cp, dcp, (up, vp, wp), gup, p)
+        phys["poly"] = dict(cn={k: to_spectral(v, p) for k, v in cn.items()},
+                            str={k: to_spectral(v, p) for k, v in strp.items()},
+                            qp={k: to_spectral(v, p) for k, v in qpp.items()})
     return gn, fn, omz_o, scn, phys
+
+
+# --------------------------------------------------------------------------- #
+#  FENE-P conformation tensor (POLYMER build)
+# --------------------------------------------------------------------------- #
+POLY = ("11", "12", "13", "22", "23", "33")     # independent components (c21=c12, ...)
+_CIDX = {(0, 0): "11", (0, 1): "12", (0, 2): "13", (1, 0): "12", (1, 1): "22",
+         (1, 2): "23", (2, 0): "13", (2, 1): "23", (2, 2): "33"}
+_PAIRS = ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))
+
+
+def derivscji(C, p: Params):
+    """derivatives.f90:148-254 -- dC_ij/dx_k for the independent components:
+    {ij: (i kx C, D C, i kz C)} (dc_ij1 = i kx C_ij, dc_ij2 = cderiv, dc_ij3 = i kz)."""
+    kx = p.wavx[:, None, None]
+    kz = p.wavz[None, :, None]
+    return {ij: (1j * kx * C[ij], cderiv(C[ij], p.dyde), 1j * kz * C[ij]) for ij in POLY}
+
+
+def polymer_products(cp, dcp, up3, gup, p: Params):
+    """solvers.f90:2901-2983, pointwise on the 3/2 grid:
+      c_ij^n = sum_k C_ik du_j/dx_k + sum_k C_kj du_i/dx_k - u . grad C_ij
+    (the literal c11np..c33np lines: e.g. c12np = c11 u21 + c12 u22 + c13 u23
+    + c12 u11 + c22 u12 + c32 u13 - (u dc121 + v dc122 + w dc123)),
+    trC, Peterlin factor (L^2-3)/(L^2-trC) (ipeter=1) or 1, str = f C - I,
+    qp = (1-beta)/(re beta lambda) str with beta = qbeta (itarget 0).
+    cp: {ij: phys}, dcp: {ij: (dx, dy, dz) phys}, up3 = (u, v, w),
+    gup[i][k] = du_i/dx_k phys.  Returns ({ij: c_n}, {ij: str}, {ij: qp})."""
+    c = lambda i, j: cp[_CIDX[(i, j)]]
+    cn = {}
+    for i, j in _PAIRS:
+        s = c(i, 0) * gup[j][0] + c(i, 1) * gup[j][1] + c(i, 2) * gup[j][2] \
+            + c(0, j) * gup[i][0] + c(1, j) * gup[i][1] + c(2, j) * gup[i][2]
+        d = dcp[_CIDX[(i, j)]]
+        cn[_CIDX[(i, j)]] = s - (up3[0] * d[0] + up3[1] * d[1] + up3[2] * d[2])
+    tr = cp["11"] + cp["22"] + cp["33"]
+    if p.ipeter == 0:
+        fac = 1.0
+    else:
+        L2 = p.zlmax * p.zlmax
+        fac = (L2 - 3.0) / (L2 - tr)
+    st = {ij: fac * cp[ij] for ij in POLY}
+    for ij in ("11", "22", "33"):
+        st[ij] = st[ij] - 1.0
+    if p.itarget != 0:
+        raise NotImplementedError("itarget 1/2 (scalar-dependent beta) is not restated")
+    zbeta1 = (1.0 - p.qbeta) / (p.re * p.qbeta * p.tpoly)
+    qp = {ij: zbeta1 * st[ij] for ij in POLY}
+    return cn, st, qp
+
+
+def polyNL(cn, cnm1, strn, strnm1, p: Params):
+    """helpers.f90:610-674 -- 3 (c_n - str_n/lambda) - (c_nm1 - str_nm1/lambda)."""
+    ti = 1.0 / p.tpoly
+    return {ij: 3.0 * (cn[ij] - ti * strn[ij]) - (cnm1[ij] - ti * strnm1[ij]) for ij in POLY}
+
+
+def polymer_implicit(C, cNL, p: Params):
+    """main_dns.f90:240-321 -- phirhs(C, cNL, 2 cNL, diffpoly) (AB2 through
+    phirhs's 3 fn - fnm1 = cNL) and the Neumann penta solve with
+    g = re diffpoly/(dt theta); returns the new independent components."""
+    g = p.re * p.diffpoly / (p.dt * p.theta)
+    out = {}
+    for ij in POLY:
+        rhs = phirhs(C[ij], cNL[ij], 2.0 * cNL[ij], p.diffpoly, p)
+        c = penta(rhs, p.k2 + g, p.dyde, 0, 0.0, 1.0, 0.0, 0.0, 1.0, 0.0)
+        norm(c)
+        out[ij] = c
+    return out
+
+
+def polyforce(qp, p: Params):
+    """helpers.f90:498-560 -- t = div(qp): t1 = i(kx q11 + kz q13) + D q12,
+    t2 = i(kx q12 + kz q23) + D q22, t3 = i(kx q13 + kz q33) + D q23."""
+    kx = p.wavx[:, None, None]
+    kz = p.wavz[None, :, None]
+    d = lambda a: cderiv(a, p.dyde)
+    t1 = 1j * (kx * qp["11"] + kz * qp["13"]) + d(qp["12"])
+    t2 = 1j * (kx * qp["12"] + kz * qp["23"]) + d(qp["22"])
+    t3 = 1j * (kx * qp["13"] + kz * qp["33"]) + d(qp["23"])
+    return t1, t2, t3
+
+
+def polymer_init(p: Params):
+    """main_dns.f90:206-236 -- C = diag(c11z, c22z, c33z) through xyzfft."""
+    out = {}
+    for ij in POLY:
+        val = {"11": p.c11z, "22": p.c22z, "33": p.c33z}.get(ij, 0.0)
+        out[ij] = xyzfft(np.full((p.nx, p.nz, p.nyp), val), p)
+    return out
+
+
+def polymer_flags(it, p: Params):
+    """Time-line of the POLYMER branches of the main loop (main_dns.f90:206,
+    240, 637, 674, 682) and of vcw3dp (solvers.f90:2900): init C at the start
+    of the step; polyNL + implicit solve; polymer terms in vcw3dp; polymer
+    force; the n -> n-1 copy before smoothing (nested inside the force
+    branch, so its it == src_start-1 arm never fires)."""
+    init = (it == p.src_start - 1) or (it == p.irstrt and p.crstrt == 0)
+    solve = it >= p.src_start
+    evaluate = it >= p.src_start - 1
+    force = it >= p.src_start
+    copy = force and init
+    return dict(init=init, solve=solve, evaluate=evaluate, force=force, copy=copy)
 
 
 # --------------------------------------------------------------------------- #
@@ -764,9 +892,16 @@ class Solver:
         st.setdefault("fxintg", np.zeros((p.mx, p.mz, p.nyp)))
         st.setdefault("fyintg", np.zeros((p.mx, p.mz, p.nyp)))
         st.setdefault("fzintg", np.zeros((p.mx, p.mz, p.nyp)))
+        if p.polymer:
+            self.polymer_defaults(st)
         self._nonlinear(st, u, v, w, it=0, smooth=False)
         st["fnm1"] = st["fn"].copy()
         st["gnm1"] = st["gn"].copy()
+        if p.polymer:
+            # initial(): the n-1 polymer terms start equal to the n terms
+            for ij in POLY:
+                st["c%snm1" % ij] = st["c%sn" % ij].copy()
+                st["str%snm1" % ij] = st["str%sn" % ij].copy()
         st["h1nm1"] = st["h1n"].copy()
         st["h3nm1"] = st["h3n"].copy()
         if p.scalar:
@@ -783,6 +918,12 @@ class Solver:
             sx, sy, sz = gradscl(st["scalar"], p)
             norm(sx); norm(sy); norm(sz)
             fields.update(sclx=sx, scly=sy, sclz=sz)
+        pf = polymer_flags(it, p) if p.polymer else None
+        if p.polymer and pf["evaluate"]:
+            C = {ij: st["c" + ij] for ij in POLY}
+            fields["C"] = C
+            fields["dC"] = derivscji(C, p)
+            fields["gradv"] = gradv(u, v, w, p)
         if self.imatrix is not None:
             st["imatrix"] = self.imatrix
         gn, fn, omz_o, scn, phys = vcw3dp(st, p, it, fields)
@@ -804,12 +945,34 @@ class Solver:
             norm(a)
         if scn is not None:
             norm(scn)
+        if p.polymer and pf["evaluate"]:
+            pol = phys["poly"]
+            for ij in POLY:
+                for key, d in (("cn", "c%sn"), ("str", "str%sn")):
+                    norm(pol[key][ij])
+                    st[d % ij] = pol[key][ij]
+                norm(pol["qp"][ij])
+            if pf["force"]:
+                t1, t2, t3 = polyforce(pol["qp"], p)
+                for a in (t1, t2, t3):
+                    norm(a)
+                gn = gn + t1
+                fn = fn + t2
+                omz_o = omz_o + t3
+                if pf["copy"]:
+                    for ij in POLY:
+                        st["c%snm1" % ij] = st["c%sn" % ij].copy()
+                        st["str%snm1" % ij] = st["str%sn" % ij].copy()
         if smooth:
             omz_o *= self.smooth
             gn *= self.smooth
             fn *= self.smooth
             if scn is not None:
                 scn *= self.smooth
+            if p.polymer and pf["evaluate"]:
+                for ij in POLY:
+                    st["c%sn" % ij] = st["c%sn" % ij] * self.smooth
+                    st["str%sn" % ij] = st["str%sn" % ij] * self.smooth
         st["h1n"] = gn[0, 0, :].real.copy()
         st["h3n"] = omz_o[0, 0, :].real.copy()
         for a in (fn, gn, omz_o):
@@ -823,11 +986,33 @@ class Solver:
             st["scn"] = scn
 
     # main_dns.f90:198-811 (NS + scalar)
+    def polymer_defaults(self, st):
+        """Zero conformation / history arrays that the caller did not supply."""
+        z = lambda: np.zeros((self.p.nxh, self.p.nz, self.p.nyp), dtype=np.complex128)
+        for ij in POLY:
+            for key in ("c%s", "c%sn", "c%snm1", "str%sn", "str%snm1"):
+                st.setdefault(key % ij, z())
+
     def step(self, st):
         p = self.p
         self.it += 1
         it = self.it
         g = p.re / (p.dt * p.theta)
+        if p.polymer:
+            self.polymer_defaults(st)
+            pf = polymer_flags(it, p)
+            if pf["init"]:
+                for ij, c in polymer_init(p).items():
+                    st["c" + ij] = c
+            if pf["solve"]:
+                get = lambda key: {ij: st[key % ij] for ij in POLY}
+                cNL = polyNL(get("c%sn"), get("c%snm1"), get("str%sn"), get("str%snm1"), p)
+                for ij in POLY:
+                    st["c%snm1" % ij] = st["c%sn" % ij].copy()
+                    st["str%snm1" % ij] = st["str%sn" % ij].copy()
+                newc = polymer_implicit(get("c%s"), cNL, p)
+                for ij in POLY:
+                    st["c" + ij] = newc[ij]
         if p.scalar:
             rhs = phirhs(st["scalar"], st["scn"], st["scnm1"], p.diff, p)
             st["scnm1"] = st["scn"].copy()

@@ -418,3 +418,143 @@ def test_substep_closed_forms():
     assert O.substep_check(p, 1, ratio, a) == math.ceil(2 * 0.01 / tau)
     tau2 = 2 * 1000.0 * a * a / (9.0 / (ratio + 0.5))
     assert O.substep_check(p, 2, ratio, a) == math.ceil(2 * 0.01 / tau2)
+
+
+# ------------------------------------------------------------------ polymer
+def _uniform_poly(shape, C, u=(0.0, 0.0, 0.0), gradu=None):
+    cp = {ij: np.full(shape, C[ij]) for ij in O.POLY}
+    dcp = {ij: (np.zeros(shape), np.zeros(shape), np.zeros(shape)) for ij in O.POLY}
+    up3 = tuple(np.full(shape, a) for a in u)
+    g = np.zeros((3, 3)) if gradu is None else gradu
+    gup = [[np.full(shape, g[i, k]) for k in range(3)] for i in range(3)]
+    return cp, dcp, up3, gup
+
+
+@pytest.mark.parametrize("ipeter", [0, 1])
+def test_simple_shear_steady_state(ipeter):
+    """Homogeneous shear u = gdot y: the FENE-P / Oldroyd-B steady state
+    C22 = C33 = 1/f, C12 = lam gdot / f^2, C11 = (1 + 2 lam gdot C12)/f with
+    f = (L^2-3)/(L^2-trC) (f = 1 for Oldroyd-B) makes c_n - str/lam vanish."""
+    from scipy.optimize import brentq
+    p = params(8, 8, 8, polymer=True, tpoly=0.7, zlmax=12.0, ipeter=ipeter)
+    lam, gd, L2 = p.tpoly, 3.0, p.zlmax ** 2
+
+    def comps(f):
+        c22 = 1.0 / f
+        c12 = lam * gd / f ** 2
+        c11 = (1.0 + 2.0 * lam * gd * c12) / f
+        return c11, c12, c22
+
+    if ipeter:
+        f = brentq(lambda f: f - (L2 - 3.0) / (L2 - (comps(f)[0] + 2 * comps(f)[2])), 1.0 + 1e-12, 1e3)
+    else:
+        f = 1.0
+    c11, c12, c22 = comps(f)
+    C = {"11": c11, "12": c12, "13": 0.0, "22": c22, "23": 0.0, "33": c22}
+    g = np.zeros((3, 3))
+    g[0, 1] = gd                      # du/dy
+    cn, strp, _ = O.polymer_products(*_uniform_poly((2, 2, 2), C, gradu=g), p)
+    for ij in O.POLY:
+        assert np.abs(cn[ij] - strp[ij] / lam).max() < 1e-12 * max(1.0, c11), ij
+    if not ipeter:
+        assert abs(c11 - (1 + 2 * (lam * gd) ** 2)) < 1e-12 and abs(c12 - lam * gd) < 1e-12
+    # the transposed gradient (dv/dx) is not a steady state
+    gt = g.T.copy()
+    cn2, strp2, _ = O.polymer_products(*_uniform_poly((2, 2, 2), C, gradu=gt), p)
+    assert np.abs(cn2["12"] - strp2["12"] / lam).max() > 1e-3
+
+
+def test_polymer_identity_at_rest_and_advection_sign():
+    p = params(8, 8, 8, polymer=True)
+    I = {"11": 1.0, "12": 0.0, "13": 0.0, "22": 1.0, "23": 0.0, "33": 1.0}
+    cn, strp, qp = O.polymer_products(*_uniform_poly((2, 3, 2), I), p)
+    for ij in O.POLY:
+        assert np.abs(cn[ij]).max() == 0.0 and np.abs(strp[ij]).max() < 1e-15 and np.abs(qp[ij]).max() < 1e-15
+    # pure advection: c_n = -u . grad C
+    rng = np.random.default_rng(3)
+    shape = (4, 4, 4)
+    cp = {ij: rng.standard_normal(shape) for ij in O.POLY}
+    dcp = {ij: tuple(rng.standard_normal(shape) for _ in range(3)) for ij in O.POLY}
+    U = (0.3, -1.2, 2.0)
+    up3 = tuple(np.full(shape, a) for a in U)
+    gup = [[np.zeros(shape)] * 3 for _ in range(3)]
+    cn, _, _ = O.polymer_products(cp, dcp, up3, gup, p)
+    for ij in O.POLY:
+        d = dcp[ij]
+        assert np.abs(cn[ij] + (U[0] * d[0] + U[1] * d[1] + U[2] * d[2])).max() < 1e-14
+
+
+def test_polyforce_is_the_divergence_of_an_analytic_tensor():
+    """t_i = d q_ik / d x_k for q11 = cos(a x) T2(y), q12 = sin(b z) T3(y),
+    q13 = cos(a x + b z), q22 = T4(y), q23 = sin(a x) T1(y), q33 = cos(2 b z) T2(y)."""
+    p = params(16, 16, 16)
+    x = (np.arange(p.mx) * p.xl / p.mx)[:, None, None]
+    z = (np.arange(p.mz) * p.zl / p.mz)[None, :, None]
+    y = (p.ycoord / (p.yl / 2))[None, None, :]
+    a, b, dy = p.alpha, p.beta, p.dyde
+    T = [np.ones_like(y), y, 2 * y ** 2 - 1, 4 * y ** 3 - 3 * y, 8 * y ** 4 - 8 * y ** 2 + 1]
+    dT = [0 * y, np.ones_like(y), 4 * y, 12 * y ** 2 - 3, 32 * y ** 3 - 16 * y]
+    phys = {"11": np.cos(a * x) * T[2], "12": np.sin(b * z) * T[3], "13": np.cos(a * x + b * z) * T[0],
+            "22": T[4] + 0 * x * z, "23": np.sin(a * x) * T[1], "33": np.cos(2 * b * z) * T[2]}
+    q = {k: O.to_spectral(np.broadcast_to(v, (p.mx, p.mz, p.nyp)).copy(), p) for k, v in phys.items()}
+    t1, t2, t3 = O.polyforce(q, p)
+    e1 = -a * np.sin(a * x) * T[2] + np.sin(b * z) * dT[3] * dy - b * np.sin(a * x + b * z)
+    e2 = dT[4] * dy + 0 * x * z          # d q12/dx = 0, d q23/dz = 0
+    e3 = -a * np.sin(a * x + b * z) + np.sin(a * x) * dT[1] * dy - 2 * b * np.sin(2 * b * z) * T[2]
+    for t, e in ((t1, e1), (t2, e2), (t3, e3)):
+        assert np.abs(O.to_physical(t, p) - e).max() < 1e-11
+
+
+def test_polymer_neumann_diffusion_conserves_the_mean():
+    """With no source the implicit step keeps int C dy of the mean mode and
+    dC/dy = 0 at both walls."""
+    p = params(8, 32, 8, polymer=True, dt=0.01, diffpoly=0.02, re=100.0)
+    rng = np.random.default_rng(4)
+    C = {}
+    for ij in O.POLY:
+        # a + b (T2 - T4/4) + c (T3 - 9 T1 / 1): zero wall slope (T_n'(1) = n^2)
+        c = np.zeros((p.nxh, p.nz, p.nyp), dtype=np.complex128)
+        a, b, d = rng.standard_normal(3)
+        c[0, 0, 0] = a
+        c[0, 0, 2], c[0, 0, 4] = b, -b / 4.0
+        c[0, 0, 3], c[0, 0, 1] = d, -9.0 * d
+        C[ij] = c
+    zero = {ij: np.zeros_like(C[ij]) for ij in O.POLY}
+    n = np.arange(p.nyp)
+    wts = np.zeros(p.nyp)
+    wts[n % 2 == 0] = 2.0 / (1.0 - n[n % 2 == 0] ** 2.0)
+    for ij in O.POLY:
+        assert abs(O.c1dertw(C[ij][0, 0], p.dyde)) < 1e-14 and abs(O.c1derbw(C[ij][0, 0], p.dyde)) < 1e-14
+    m0 = {ij: (C[ij][0, 0].real * wts).sum() for ij in O.POLY}
+    for _ in range(5):
+        C = O.polymer_implicit(C, zero, p)
+    for ij in O.POLY:
+        c = C[ij][0, 0].real
+        assert abs((c * wts).sum() - m0[ij]) < 1e-10 * max(1, abs(m0[ij]))  # tau residual
+        assert abs(O.c1dertw(c, p.dyde)) < 1e-9 and abs(O.c1derbw(c, p.dyde)) < 1e-9
+
+
+def test_polymer_relaxation_at_rest_follows_exp_decay():
+    """u = 0, uniform C0 = diag(2.5, 0.5, 1.5): Oldroyd-B relaxes as
+    I + (C0 - I) exp(-t/lambda); AB2 (Euler start) error is O(dt^2)."""
+    dt, lam, nsteps = 0.002, 0.05, 50
+    p = params(8, 8, 8, polymer=True, dt=dt, R_tau=0.0, tpoly=lam, ipeter=0, crstrt=1, src_start=0)
+    st = {k: np.zeros((p.nxh, p.nz, p.nyp), dtype=np.complex128) for k in ("v", "omy")}
+    st["u0"] = np.zeros(p.nyp)
+    st["w0"] = np.zeros(p.nyp)
+    diag = {"11": 2.5, "22": 0.5, "33": 1.5}
+    for ij in O.POLY:
+        c = np.zeros((p.nxh, p.nz, p.nyp), dtype=np.complex128)
+        c[0, 0, 0] = diag.get(ij, 0.0)
+        st["c" + ij] = c
+    sol = O.Solver(p)
+    sol.startup(st)
+    for _ in range(nsteps):
+        sol.step(st)
+    t = nsteps * dt
+    for ij, c0 in diag.items():
+        exact = 1.0 + (c0 - 1.0) * math.exp(-t / lam)
+        got = st["c" + ij][0, 0, 0].real
+        assert abs(got - exact) < 5 * (dt / lam) ** 2 * abs(c0 - 1.0), (ij, got, exact)
+        assert np.abs(st["c" + ij][0, 0, 1:]).max() < 1e-11
+    assert np.abs(st["c12"]).max() < 1e-14
