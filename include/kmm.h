@@ -62,6 +62,17 @@ typedef struct kmm_config {
   int particle_flag;      /* 0: tracers (the only kind built) */
   double ratio, ap, c_mu; /* particle density ratio, radius, C_mu */
   int cd_switch;          /* drag law; sets the tracer sub-step (part_track.f90:338-346) */
+  /* FENE-P conformation tensor (POLYMER build; dns.config lines 35-37, 55-62) */
+  int polymer;            /* 1 = POLYMER build */
+  int itarget;            /* 0: uniform beta = qbeta (the only mode built) */
+  int ipeter;             /* 1: Peterlin function (FENE-P), 0: Oldroyd-B */
+  int ipolyflag;
+  double zlmax;           /* maximum extensibility L */
+  double tpoly;           /* relaxation time lambda */
+  double alpha_poly;
+  double diffpoly;        /* artificial diffusivity 1/(re diffpoly) */
+  double c11z, c22z, c33z;/* initial diagonal of C */
+  double qbeta;           /* solvent viscosity ratio beta */
 } kmm_config;
 
 typedef struct kmm_ctx kmm_ctx;
@@ -72,6 +83,12 @@ enum {
   KMM_U0, KMM_H1N, KMM_H1NM1, KMM_W0, KMM_H3N, KMM_H3NM1,   /* mean profiles */
   KMM_FXINTG, KMM_FYINTG, KMM_FZINTG,                       /* restart record 2 */
   KMM_SCALAR, KMM_SCN, KMM_SCNM1,                           /* SCALAR build */
+  /* POLYMER build: independent components 11, 12, 13, 22, 23, 33 each */
+  KMM_C11, KMM_C12, KMM_C13, KMM_C22, KMM_C23, KMM_C33,     /* conformation C */
+  KMM_C11N, KMM_C12N, KMM_C13N, KMM_C22N, KMM_C23N, KMM_C33N,
+  KMM_C11NM1, KMM_C12NM1, KMM_C13NM1, KMM_C22NM1, KMM_C23NM1, KMM_C33NM1,
+  KMM_STR11N, KMM_STR12N, KMM_STR13N, KMM_STR22N, KMM_STR23N, KMM_STR33N,
+  KMM_STR11NM1, KMM_STR12NM1, KMM_STR13NM1, KMM_STR22NM1, KMM_STR23NM1, KMM_STR33NM1,
   KMM_NFIELDS
 };
 
@@ -108,8 +125,12 @@ int kmm_last_cfl(kmm_ctx* ctx, double* cflmax, int* it);
 /* outputs/last-restart (main_dns.f90:776-781) / setup/restart (1744-1747) */
 int kmm_write_restart(kmm_ctx* ctx, const char* path);
 int kmm_read_restart(kmm_ctx* ctx, const char* path);
-/* scalar state as the first three records of c-restart
- * (scalar / scn / scnm1; main_dns.f90:782-793, 1910-1916) */
+/* c-restart (main_dns.f90:782-793, 1910-1916).  SCALAR only: the three
+ * records scalar / scn / scnm1.  POLYMER: the reference's five records
+ * (scalar, c11..c33 | scn, c11n..c33n | scnm1, c11nm1..c33nm1 |
+ * str11n..str33n | str11nm1..str33nm1), the scalar fields written as zeros
+ * when there is no scalar; c21, c31, c32 are written as c12, c13, c23 and
+ * ignored on reading. */
 int kmm_write_crestart(kmm_ctx* ctx, const char* path);
 int kmm_read_crestart(kmm_ctx* ctx, const char* path);
 
@@ -139,8 +160,9 @@ int kmm_vcw3dp(kmm_ctx* ctx, const double* u, const double* v, const double* w, 
 int kmm_stage_times(kmm_ctx* ctx, double* pencil_ms, double* plane_ms);
 
 /* Per-kernel HIP-event timing of every step of later kmm_step calls:
- * average ms per launch of k_pencil, k_zinv, k_ibdrag, k_xplane, k_zfwd
- * (ms[5]) over the last kmm_step call. */
+ * average ms per step of k_pencil, k_zinv, k_ibdrag, k_xplane, k_zfwd,
+ * the polymer pencil kernel, the polymer plane kernels and the tracer kernel
+ * (ms[8]) over the last kmm_step call. */
 int kmm_set_timing(kmm_ctx* ctx, int on);
 int kmm_kernel_times(kmm_ctx* ctx, double* ms, int* nsamples);
 

@@ -79,7 +79,7 @@ struct kmm_ctx {
   // per-kernel timing (pencil, zinv, ibdrag, xplane, zfwd) over a kmm_step call
   int timing = 0;
   std::vector<hipEvent_t> tev;
-  double ktime[5] = {0, 0, 0, 0, 0};
+  double ktime[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int ksamples = 0;
   std::vector<void*> allocs;
   // Lagrangian tracers
@@ -92,6 +92,15 @@ struct kmm_ctx {
   int part_state = 0;        // 0 none, 1 initialise at the next evaluation, 2 tracking
   int substep = 1;
   bool cap_valid = false;
+  // FENE-P conformation tensor: independent components 11 12 13 22 23 33
+  double2 *C[6] = {}, *CN[6] = {}, *CNM1[6] = {}, *STRN[6] = {}, *STRNM1[6] = {}, *TF[3] = {};
+  double2* tabC = nullptr;
+  void* NLP[12] = {};   // plane outputs c_n, str_n (physical y, transfer precision)
+  void* TP[12] = {};    // transfer C, D C
+  void* TPD[3] = {};    // transfer D u, D v, D w
+  void *ZP = nullptr, *ZFP = nullptr;
+  struct PolyFlags { bool init = false, solve = false, evaluate = false, force = false, copy = false; };
+  PolyFlags pprev;      // flags of the evaluation whose results sit in NLP
 };
 
 static int dalloc(kmm_ctx* c, void** p, size_t bytes) {
@@ -166,9 +175,13 @@ extern "C" int kmm_config_read(const char* path, kmm_config* cfg) {
   RD(66, src_start, int); RD(67, src_stop, int); RD(74, cadence, int);
   RD(33, particle_flag, int); RD(46, ratio, double); RD(47, ap, double); RD(48, c_mu, double);
   RD(49, cd_switch, int);
+  RD(35, itarget, int); RD(36, ipeter, int); RD(37, ipolyflag, int);
+  RD(55, zlmax, double); RD(56, tpoly, double); RD(57, alpha_poly, double); RD(58, diffpoly, double);
+  RD(59, c11z, double); RD(60, c22z, double); RD(61, c33z, double); RD(62, qbeta, double);
 #undef RD
   if (cfg->src_stop < cfg->src_start) cfg->src_stop = cfg->nsteps;  // main_dns.f90:1071-1075
   cfg->scalar = 0;
+  cfg->polymer = 0;
   return KMM_OK;
 }
 
@@ -298,6 +311,13 @@ extern "C" int kmm_create(const kmm_config* cfg, int device, kmm_ctx** out) {
     return fail(KMM_E_CONFIG, "flow_select must be a wall-bounded case with nt=nb=1 (1,2,3,11,12)");
   if (cfg->scalar && !(cfg->scl_flag == 0 || cfg->scl_flag == 1 || cfg->scl_flag == 3 || cfg->scl_flag == 6))
     return fail(KMM_E_CONFIG, "scl_flag must be 0, 1, 3 or 6 (uniform source)");
+  if (cfg->polymer) {
+    if (cfg->itarget != 0) return fail(KMM_E_CONFIG, "polymer: only itarget 0 (uniform beta) is supported");
+    if (!(cfg->tpoly > 0.0) || !(cfg->diffpoly > 0.0) || !(cfg->qbeta > 0.0 && cfg->qbeta <= 1.0))
+      return fail(KMM_E_CONFIG, "polymer: tpoly, diffpoly must be positive and 0 < qbeta <= 1");
+    if (cfg->ipeter != 0 && !(cfg->zlmax * cfg->zlmax > 3.0))
+      return fail(KMM_E_CONFIG, "polymer: zlmax^2 must exceed 3 for the Peterlin function");
+  }
   kmm_ctx* c = new kmm_ctx();
   c->cfg = *cfg;
   c->device = device;
@@ -341,6 +361,20 @@ extern "C" int kmm_create(const kmm_config* cfg, int device, kmm_ctx** out) {
     const size_t gb = (size_t)c->P * (ny / 2 + 1) * sizeof(double2);
     for (auto p : {&c->G1, &c->G2}) if ((r = dalloc(c, (void**)p, gb))) goto fail_;
     if ((r = dalloc(c, (void**)&c->gk, (size_t)c->P * sizeof(double4)))) goto fail_;
+    if (cfg->polymer) {
+      for (int q = 0; q < 6; ++q)
+        for (double2** b : {&c->C[q], &c->CN[q], &c->CNM1[q], &c->STRN[q], &c->STRNM1[q]})
+          if ((r = dalloc(c, (void**)b, fb))) goto fail_;
+      for (int q = 0; q < 3; ++q) if ((r = dalloc(c, (void**)&c->TF[q], fb))) goto fail_;
+      for (int q = 0; q < 12; ++q) {
+        if ((r = dalloc(c, &c->NLP[q], tbv))) goto fail_;
+        if ((r = dalloc(c, &c->TP[q], tbv))) goto fail_;
+      }
+      for (int q = 0; q < 3; ++q) if ((r = dalloc(c, &c->TPD[q], tbv))) goto fail_;
+      if ((r = dalloc(c, &c->ZP, 24 * zb))) goto fail_;
+      if ((r = dalloc(c, &c->ZFP, 12 * zb))) goto fail_;
+      if ((r = dalloc(c, (void**)&c->tabC, tb))) goto fail_;
+    }
     if ((r = dalloc(c, (void**)&c->cfl_d, 4096 * sizeof(unsigned long long)))) goto fail_;
     c->cfl_cap = 4096;
     // twiddles and smoothing
@@ -379,6 +413,11 @@ extern "C" int kmm_create(const kmm_config* cfg, int device, kmm_ctx** out) {
     if ((e = kmm::launch_tau_tables(ny, a, c->gH, c->tabH, c->stream)) != hipSuccess) { r = fail(KMM_E_HIP, hipGetErrorString(e)); goto fail_; }
     if ((e = kmm::launch_tau_tables(ny, a, 0.0, c->tabP, c->stream)) != hipSuccess) { r = fail(KMM_E_HIP, hipGetErrorString(e)); goto fail_; }
     if ((e = kmm::launch_tau_tables(ny, a, c->gS, c->tabS, c->stream)) != hipSuccess) { r = fail(KMM_E_HIP, hipGetErrorString(e)); goto fail_; }
+    if (cfg->polymer &&
+        (e = kmm::launch_tau_tables(ny, a, cfg->re * cfg->diffpoly / (cfg->dt * cfg->theta), c->tabC, c->stream)) != hipSuccess) {
+      r = fail(KMM_E_HIP, hipGetErrorString(e));
+      goto fail_;
+    }
     // p1b=1, p1t=-1, p2b=-1, p2t=0.5 (setbcs, main_dns.f90:1406-1409)
     if ((e = kmm::launch_green(ny, a, 1.0, -1.0, -1.0, 0.5, c->G1, c->G2, c->gk, c->stream)) != hipSuccess) { r = fail(KMM_E_HIP, hipGetErrorString(e)); goto fail_; }
     if (hipStreamSynchronize(c->stream) != hipSuccess) { r = fail(KMM_E_HIP, "setup kernels failed"); goto fail_; }
@@ -559,6 +598,14 @@ static double2* spectral_ptr(kmm_ctx* c, int f) {
     case KMM_SCALAR: return c->TH;
     case KMM_SCN: return c->SCN;
     case KMM_SCNM1: return c->SCNM1;
+    default: break;
+  }
+  if (c->cfg.polymer && f >= KMM_C11 && f <= KMM_STR33NM1) {
+    const int g = (f - KMM_C11) / 6, q = (f - KMM_C11) % 6;
+    double2** tab[5] = {c->C, c->CN, c->CNM1, c->STRN, c->STRNM1};
+    return tab[g][q];
+  }
+  switch (f) {
     default: return nullptr;
   }
 }
@@ -576,10 +623,16 @@ static double* mean_ptr(kmm_ctx* c, int f) {
 
 static int materialize(kmm_ctx* c);
 
+// fields the next step rebuilds from the NL buffers while they are valid
+static bool nl_derived(int f) {
+  return f == KMM_FN || f == KMM_GN || f == KMM_SCN || f == KMM_H1N || f == KMM_H3N ||
+         (f >= KMM_C11N && f <= KMM_STR33NM1);
+}
+
 extern "C" int kmm_put_field(kmm_ctx* c, int f, const double* h) {
   if (!c || !h) return fail(KMM_E_ARG, "null argument");
   KMM_HIP(hipSetDevice(c->device));
-  if (c->nl_valid && (f == KMM_FN || f == KMM_GN || f == KMM_SCN || f == KMM_H1N || f == KMM_H3N)) {
+  if (c->nl_valid && nl_derived(f)) {
     int r = materialize(c);
     if (r) return r;
     c->nl_valid = false;
@@ -600,7 +653,7 @@ extern "C" int kmm_put_field(kmm_ctx* c, int f, const double* h) {
 extern "C" int kmm_get_field(kmm_ctx* c, int f, double* h) {
   if (!c || !h) return fail(KMM_E_ARG, "null argument");
   KMM_HIP(hipSetDevice(c->device));
-  if (c->nl_valid && (f == KMM_FN || f == KMM_GN || f == KMM_SCN || f == KMM_H1N || f == KMM_H3N)) {
+  if (c->nl_valid && nl_derived(f)) {
     int r = materialize(c);
     if (r) return r;
   }
@@ -620,6 +673,67 @@ extern "C" int kmm_get_field(kmm_ctx* c, int f, double* h) {
 }
 
 // -------------------------------------------------------------- step ----
+// Time-line of the POLYMER branches (main_dns.f90:206, 240, 637, 674, 682):
+// init C at the start of step it; polyNL + implicit solve; polymer terms in
+// vcw3dp; polymer force; the n -> n-1 copy before smoothing, which sits inside
+// the force branch so that its it == src_start-1 arm never fires.
+static kmm_ctx::PolyFlags poly_flags(const kmm_ctx* c, int it) {
+  kmm_ctx::PolyFlags f;
+  if (!c->cfg.polymer) return f;
+  const kmm_config& g = c->cfg;
+  f.init = (it == g.src_start - 1) || (it == g.irstrt && g.crstrt == 0);
+  f.solve = it >= g.src_start;
+  f.evaluate = it >= g.src_start - 1;
+  f.force = it >= g.src_start;
+  f.copy = f.force && f.init;
+  return f;
+}
+
+static kmm::PolyArgs poly_args(kmm_ctx* c) {
+  kmm::PolyArgs a;
+  std::memset(&a, 0, sizeof(a));
+  const kmm_config& g = c->cfg;
+  a.nxh = c->nxh; a.nz = c->nz; a.nzh = c->nzh; a.P = c->P; a.ny = c->ny;
+  a.alpha = c->alpha; a.beta = c->beta; a.dyde = c->dyde; a.re = g.re; a.dt = g.dt; a.theta = g.theta;
+  a.diffpoly = g.diffpoly; a.tpoly = g.tpoly;
+  a.zbeta1 = (1.0 - g.qbeta) / (g.re * g.qbeta * g.tpoly);   // solvers.f90:2971
+  a.gC = g.re * g.diffpoly / (g.dt * g.theta);
+  a.c0[0] = g.c11z; a.c0[1] = g.c22z; a.c0[2] = g.c33z;
+  a.smz = c->smz; a.smy = c->smy; a.twM = c->twM; a.tabC = c->tabC;
+  for (int q = 0; q < 12; ++q) { a.NLP[q] = c->NLP[q]; a.TP[q] = c->TP[q]; }
+  for (int q = 0; q < 6; ++q) {
+    a.C[q] = c->C[q]; a.CN[q] = c->CN[q]; a.CNM1[q] = c->CNM1[q]; a.STRN[q] = c->STRN[q]; a.STRNM1[q] = c->STRNM1[q];
+  }
+  for (int q = 0; q < 3; ++q) a.TF[q] = c->TF[q];
+  return a;
+}
+
+static int run_poly_pencil(kmm_ctx* c, int mode, const kmm::PolyArgs& a) {
+  hipError_t e = kmm::launch_poly_pencil(c->ny, mode, a, c->cfg.precision == 1, c->stream);
+  if (e != hipSuccess) return fail(KMM_E_HIP, std::string("polymer pencil: ") + hipGetErrorString(e));
+  return KMM_OK;
+}
+
+static int run_poly_plane(kmm_ctx* c) {
+  kmm::PolyPlaneArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.nxh = c->nxh; a.nz = c->nz; a.nzh = c->nzh; a.nyp = c->nyp; a.mx = c->mx; a.mz = c->mz;
+  a.mixed = c->cfg.precision == 1;
+  a.alpha = c->alpha; a.beta = c->beta;
+  a.ipeter = c->cfg.ipeter;
+  a.L2 = c->cfg.zlmax * c->cfg.zlmax;
+  a.twZ = c->twZ; a.twX = c->twX;
+  a.U = c->Z;
+  // z-stage inputs: dz u(3), dy u(3), C(6), dz C(6), dy C(6)
+  for (int q = 0; q < 3; ++q) { a.in[q] = c->T[q]; a.in[3 + q] = c->TPD[q]; }
+  for (int q = 0; q < 6; ++q) { a.in[6 + q] = c->TP[q]; a.in[12 + q] = c->TP[q]; a.in[18 + q] = c->TP[6 + q]; }
+  a.ZP = c->ZP; a.ZFP = c->ZFP;
+  for (int q = 0; q < 12; ++q) a.out[q] = c->NLP[q];
+  hipError_t e = kmm::launch_poly_plane(a, c->stream);
+  if (e != hipSuccess) return fail(KMM_E_HIP, std::string("polymer plane: ") + hipGetErrorString(e));
+  return KMM_OK;
+}
+
 static kmm::TracerArgs tracer_args(kmm_ctx* c) {
   kmm::TracerArgs t;
   std::memset(&t, 0, sizeof(t));
@@ -637,12 +751,18 @@ static kmm::TracerArgs tracer_args(kmm_ctx* c) {
 // One evaluation of vcw3dp's plane stage.  With track set and particles
 // present, the physical velocity is kept and the tracers are initialised or
 // advanced (part_track, solvers.f90:3463-3466).
-static int run_plane(kmm_ctx* c, int it, unsigned long long* cfl, hipEvent_t* ev = nullptr, bool track = false) {
+static int run_plane(kmm_ctx* c, int it, unsigned long long* cfl, hipEvent_t* ev = nullptr, bool track = false,
+                     bool poly = false) {
   PlaneArgs pa = plane_args(c, it, cfl);
   const bool trk = track && c->npart > 0;
   if (trk) pa.cap = c->cap;
   hipError_t e = ev ? kmm::launch_plane_timed(pa, c->stream, ev) : kmm::launch_plane(pa, c->stream);
   if (e != hipSuccess) return fail(KMM_E_HIP, std::string("plane stage: ") + hipGetErrorString(e));
+  if (poly) {
+    int r = run_poly_plane(c);
+    if (r) return r;
+  }
+  if (ev) hipEventRecord(ev[4], c->stream);
   if (trk) {
     kmm::TracerArgs t = tracer_args(c);
     t.n = c->npart; t.nf = 3;
@@ -656,6 +776,7 @@ static int run_plane(kmm_ctx* c, int it, unsigned long long* cfl, hipEvent_t* ev
     c->part_state = 2;
     c->cap_valid = true;
   }
+  if (ev) hipEventRecord(ev[5], c->stream);
   return KMM_OK;
 }
 
@@ -673,6 +794,13 @@ static int check_particles(kmm_ctx* c) {
 static int materialize(kmm_ctx* c) {
   PencilArgs a = pencil_args(c);
   a.smooth = 1;
+  if (c->cfg.polymer && c->pprev.evaluate) {
+    kmm::PolyArgs pa = poly_args(c);
+    pa.partB = 1; pa.force = c->pprev.force; pa.copy = c->pprev.copy; pa.smooth = 1;
+    int r = run_poly_pencil(c, 2, pa);
+    if (r) return r;
+    if (c->pprev.force) for (int q = 0; q < 3; ++q) a.TF[q] = c->TF[q];
+  }
   hipError_t e = kmm::launch_pencil_finish(c->ny, a, c->stream);
   if (e != hipSuccess) return fail(KMM_E_HIP, std::string("pencil finish: ") + hipGetErrorString(e));
   KMM_HIP(hipStreamSynchronize(c->stream));
@@ -702,15 +830,45 @@ extern "C" int kmm_startup(kmm_ctx* c) {
   if (!c) return fail(KMM_E_ARG, "null argument");
   KMM_HIP(hipSetDevice(c->device));
   PencilArgs a = pencil_args(c);
+  // initial() evaluates vcw3dp at it = 0: no C initialisation or solve there
+  const kmm_ctx::PolyFlags pf = poly_flags(c, 0);
+  int r = 0;
+  if (pf.evaluate) {
+    kmm::PolyArgs pa = poly_args(c);
+    pa.evaluate = 1;
+    if ((r = run_poly_pencil(c, 1, pa))) return r;
+    for (int q = 0; q < 3; ++q) a.TPD[q] = c->TPD[q];
+  }
   hipError_t e = kmm::launch_pencil_startup(c->ny, a, c->stream);
   if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
+  for (int q = 0; q < 3; ++q) a.TPD[q] = nullptr;
   KMM_HIP(hipMemsetAsync(c->cfl_d, 0, sizeof(unsigned long long), c->stream));
   c->it = 0;
-  int r = run_plane(c, 0, c->cfl_d, nullptr, true);
+  r = run_plane(c, 0, c->cfl_d, nullptr, true, pf.evaluate);
   if (r) return r;
   a.smooth = 0;  // initial() applies no smoothing (main_dns.f90:2155-2206)
+  if (c->cfg.polymer) {
+    if (pf.evaluate) {
+      kmm::PolyArgs pa = poly_args(c);
+      pa.partB = 1; pa.force = pf.force; pa.smooth = 0;
+      if ((r = run_poly_pencil(c, 2, pa))) return r;
+      if (pf.force) for (int q = 0; q < 3; ++q) a.TF[q] = c->TF[q];
+    } else {
+      for (int q = 0; q < 6; ++q) {  // no polymer terms before release
+        KMM_HIP(hipMemsetAsync(c->CN[q], 0, c->S * sizeof(double2), c->stream));
+        KMM_HIP(hipMemsetAsync(c->STRN[q], 0, c->S * sizeof(double2), c->stream));
+      }
+    }
+  }
   e = kmm::launch_pencil_finish(c->ny, a, c->stream);
   if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
+  if (c->cfg.polymer) {  // initial(): n-1 polymer terms = n terms (main_dns.f90:2200-2225)
+    for (int q = 0; q < 6; ++q) {
+      KMM_HIP(hipMemcpyAsync(c->CNM1[q], c->CN[q], c->S * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
+      KMM_HIP(hipMemcpyAsync(c->STRNM1[q], c->STRN[q], c->S * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
+    }
+  }
+  c->pprev = pf;
   const size_t fb = c->S * sizeof(double2);
   KMM_HIP(hipMemcpyAsync(c->FNM1, c->FN, fb, hipMemcpyDeviceToDevice, c->stream));
   KMM_HIP(hipMemcpyAsync(c->GNM1, c->GN, fb, hipMemcpyDeviceToDevice, c->stream));
@@ -733,7 +891,7 @@ extern "C" int kmm_step(kmm_ctx* c, int nsteps) {
     const int n = std::min(nsteps - done, c->cfl_cap);
     const int it0 = c->it + 1;
     if (c->timing) {
-      const size_t need = (size_t)n * 6;
+      const size_t need = (size_t)n * 9;
       while (c->tev.size() < need) {
         hipEvent_t e;
         KMM_HIP(hipEventCreate(&e));
@@ -745,26 +903,43 @@ extern "C" int kmm_step(kmm_ctx* c, int nsteps) {
       c->it += 1;
       PencilArgs a = pencil_args(c);
       a.first_nl = c->nl_valid ? 1 : 0;
-      hipEvent_t* ev = c->timing ? &c->tev[(size_t)s * 6] : nullptr;
+      hipEvent_t* ev = c->timing ? &c->tev[(size_t)s * 9] : nullptr;
       if (ev) hipEventRecord(ev[0], c->stream);
       else if (s == 0) hipEventRecord(c->ev[0], c->stream);
+      const kmm_ctx::PolyFlags pf = poly_flags(c, c->it);
+      if (c->cfg.polymer) {
+        kmm::PolyArgs pa = poly_args(c);
+        pa.partB = c->nl_valid && c->pprev.evaluate;
+        pa.force = pa.partB && c->pprev.force;
+        pa.copy = pa.partB && c->pprev.copy;
+        pa.smooth = 1;
+        pa.init = pf.init; pa.solve = pf.solve; pa.evaluate = pf.evaluate;
+        int r = run_poly_pencil(c, 0, pa);
+        if (r) return r;
+        if (pa.force) for (int q = 0; q < 3; ++q) a.TF[q] = c->TF[q];
+        if (pf.evaluate) for (int q = 0; q < 3; ++q) a.TPD[q] = c->TPD[q];
+      }
+      if (ev) hipEventRecord(ev[1], c->stream);
       hipError_t e = kmm::launch_pencil_step(c->ny, a, c->stream);
       if (e != hipSuccess) return fail(KMM_E_HIP, std::string("pencil step: ") + hipGetErrorString(e));
-      if (ev) hipEventRecord(ev[1], c->stream);
+      if (ev) hipEventRecord(ev[2], c->stream);
       else if (s == 0) hipEventRecord(c->ev[1], c->stream);
-      int r = run_plane(c, c->it, c->cfl_d + s, ev ? ev + 2 : nullptr, true);
+      int r = run_plane(c, c->it, c->cfl_d + s, ev ? ev + 3 : nullptr, true, pf.evaluate);
       if (r) return r;
       if (!ev && s == 0) hipEventRecord(c->ev[2], c->stream);
+      c->pprev = pf;
       c->nl_valid = true;
     }
     int r = check_cfl(c, n, it0);
     if (c->timing) {
+      // intervals: poly pencil, pencil, zinv, ibdrag, xplane, zfwd, poly plane, tracers
+      static const int slot[8] = {5, 0, 1, 2, 3, 4, 6, 7};
       for (int s = 0; s < n; ++s) {
-        hipEvent_t* ev = &c->tev[(size_t)s * 6];
-        for (int k = 0; k < 5; ++k) {
+        hipEvent_t* ev = &c->tev[(size_t)s * 9];
+        for (int k = 0; k < 8; ++k) {
           float t = 0.0f;
           hipEventElapsedTime(&t, ev[k], ev[k + 1]);
-          c->ktime[k] += t;
+          c->ktime[slot[k]] += t;
         }
       }
       c->ksamples += n;
@@ -790,7 +965,7 @@ extern "C" int kmm_set_timing(kmm_ctx* c, int on) {
 
 extern "C" int kmm_kernel_times(kmm_ctx* c, double* ms, int* nsamples) {
   if (!c || !ms) return fail(KMM_E_ARG, "null argument");
-  for (int k = 0; k < 5; ++k) ms[k] = c->ksamples ? c->ktime[k] / c->ksamples : 0.0;
+  for (int k = 0; k < 8; ++k) ms[k] = c->ksamples ? c->ktime[k] / c->ksamples : 0.0;
   if (nsamples) *nsamples = c->ksamples;
   return KMM_OK;
 }
@@ -886,41 +1061,96 @@ extern "C" int kmm_read_restart(kmm_ctx* c, const char* path) {
   c->fint_full[1] = p1;
   c->fint_full[2] = p2;
   c->it = 0;
+  c->pprev = kmm_ctx::PolyFlags();
+  if (c->cfg.polymer && c->cfg.crstrt == 0) {
+    // initial() with crstrt = 0: no polymer state until it is released
+    // (main_dns.f90:1887-1907)
+    for (int q = 0; q < 6; ++q)
+      for (double2* b : {c->C[q], c->CN[q], c->CNM1[q], c->STRN[q], c->STRNM1[q]})
+        KMM_HIP(hipMemsetAsync(b, 0, c->S * sizeof(double2), c->stream));
+  }
   return sync_fint_to_dev(c);
 }
 
+// c-restart records: SCALAR only: scalar | scn | scnm1.  POLYMER (the
+// reference's SCALAR+POLYMER layout, main_dns.f90:782-793, 1910-1916):
+// scalar, c11..c33 | scn, c11n.. | scnm1, c11nm1.. | str11n.. | str11nm1..
+static const int kPolyFull[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};  // c11 c12 c13 c21 c22 c23 c31 c32 c33
+
 extern "C" int kmm_write_crestart(kmm_ctx* c, const char* path) {
   if (!c || !path) return fail(KMM_E_ARG, "null argument");
-  if (!c->cfg.scalar) return fail(KMM_E_CONFIG, "no scalar in this configuration");
-  const size_t fs = c->S * 2;
-  std::vector<double> a(fs), b(fs), d(fs);
+  if (!c->cfg.scalar && !c->cfg.polymer) return fail(KMM_E_CONFIG, "no scalar or polymer in this configuration");
+  const size_t fs = c->S * 2, nb = fs * sizeof(double);
   int r;
-  if ((r = kmm_get_field(c, KMM_SCALAR, a.data())) || (r = kmm_get_field(c, KMM_SCN, b.data())) ||
-      (r = kmm_get_field(c, KMM_SCNM1, d.data())))
-    return r;
+  std::vector<double> sc(fs, 0.0), scn(fs, 0.0), scm(fs, 0.0);
+  if (c->cfg.scalar) {
+    if ((r = kmm_get_field(c, KMM_SCALAR, sc.data())) || (r = kmm_get_field(c, KMM_SCN, scn.data())) ||
+        (r = kmm_get_field(c, KMM_SCNM1, scm.data())))
+      return r;
+  }
+  std::vector<std::vector<double>> pol;
+  if (c->cfg.polymer) {
+    pol.assign(30, std::vector<double>(fs));
+    for (int q = 0; q < 30; ++q)
+      if ((r = kmm_get_field(c, KMM_C11 + q, pol[q].data()))) return r;
+  }
   FILE* f = fopen(path, "wb");
   if (!f) return fail(KMM_E_IO, std::string("cannot write ") + path);
-  const size_t nb = fs * sizeof(double);
-  bool ok = write_record(f, {{a.data(), nb}}) && write_record(f, {{b.data(), nb}}) && write_record(f, {{d.data(), nb}});
+  bool ok;
+  if (!c->cfg.polymer) {
+    ok = write_record(f, {{sc.data(), nb}}) && write_record(f, {{scn.data(), nb}}) && write_record(f, {{scm.data(), nb}});
+  } else {
+    std::vector<std::pair<const void*, size_t>> r1 = {{sc.data(), nb}}, r2 = {{scn.data(), nb}}, r3 = {{scm.data(), nb}}, r4,
+                                                   r5;
+    for (int q = 0; q < 9; ++q) r1.push_back({pol[kPolyFull[q]].data(), nb});
+    for (int q = 0; q < 6; ++q) {
+      r2.push_back({pol[6 + q].data(), nb});
+      r3.push_back({pol[12 + q].data(), nb});
+      r4.push_back({pol[18 + q].data(), nb});
+      r5.push_back({pol[24 + q].data(), nb});
+    }
+    ok = write_record(f, r1) && write_record(f, r2) && write_record(f, r3) && write_record(f, r4) &&
+         write_record(f, r5);
+  }
   fclose(f);
   return ok ? KMM_OK : fail(KMM_E_IO, "c-restart write failed");
 }
 
 extern "C" int kmm_read_crestart(kmm_ctx* c, const char* path) {
   if (!c || !path) return fail(KMM_E_ARG, "null argument");
-  if (!c->cfg.scalar) return fail(KMM_E_CONFIG, "no scalar in this configuration");
-  const size_t fs = c->S * 2;
-  std::vector<double> a(fs), b(fs), d(fs);
+  if (!c->cfg.scalar && !c->cfg.polymer) return fail(KMM_E_CONFIG, "no scalar or polymer in this configuration");
+  const size_t fs = c->S * 2, nb = fs * sizeof(double);
+  std::vector<double> sc(fs), scn(fs), scm(fs);
   FILE* f = fopen(path, "rb");
   if (!f) return fail(KMM_E_IO, std::string("cannot open ") + path);
-  const size_t nb = fs * sizeof(double);
-  bool ok = read_record(f, {{a.data(), nb}}) && read_record(f, {{b.data(), nb}}) && read_record(f, {{d.data(), nb}});
+  bool ok;
+  std::vector<std::vector<double>> pol;
+  if (!c->cfg.polymer) {
+    ok = read_record(f, {{sc.data(), nb}}) && read_record(f, {{scn.data(), nb}}) && read_record(f, {{scm.data(), nb}});
+  } else {
+    pol.assign(33, std::vector<double>(fs));  // 0..29 as KMM_C11.., 30..32 c21 c31 c32 (ignored)
+    static const int full_dst[9] = {0, 1, 2, 30, 3, 4, 31, 32, 5};
+    std::vector<std::pair<void*, size_t>> r1 = {{sc.data(), nb}}, r2 = {{scn.data(), nb}}, r3 = {{scm.data(), nb}}, r4, r5;
+    for (int q = 0; q < 9; ++q) r1.push_back({pol[full_dst[q]].data(), nb});
+    for (int q = 0; q < 6; ++q) {
+      r2.push_back({pol[6 + q].data(), nb});
+      r3.push_back({pol[12 + q].data(), nb});
+      r4.push_back({pol[18 + q].data(), nb});
+      r5.push_back({pol[24 + q].data(), nb});
+    }
+    ok = read_record(f, r1) && read_record(f, r2) && read_record(f, r3) && read_record(f, r4) && read_record(f, r5);
+  }
   fclose(f);
   if (!ok) return fail(KMM_E_IO, std::string("c-restart file ") + path + " is malformed");
   int r;
-  if ((r = kmm_put_field(c, KMM_SCALAR, a.data())) || (r = kmm_put_field(c, KMM_SCN, b.data())) ||
-      (r = kmm_put_field(c, KMM_SCNM1, d.data())))
-    return r;
+  if (c->cfg.scalar) {
+    if ((r = kmm_put_field(c, KMM_SCALAR, sc.data())) || (r = kmm_put_field(c, KMM_SCN, scn.data())) ||
+        (r = kmm_put_field(c, KMM_SCNM1, scm.data())))
+      return r;
+  }
+  if (c->cfg.polymer)
+    for (int q = 0; q < 30; ++q)
+      if ((r = kmm_put_field(c, KMM_C11 + q, pol[q].data()))) return r;
   return KMM_OK;
 }
 

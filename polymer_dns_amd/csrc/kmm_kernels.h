@@ -31,7 +31,39 @@ struct PencilArgs {
   double2 *V, *OMY, *FN, *FNM1, *GN, *GNM1, *TH, *SCN, *SCNM1;
   double *u0, *w0, *h1n, *h1nm1, *h3n, *h3nm1;
   void* T[9];          // transfer: u v w omx omy omz thx thy thz (physical y, transfer precision)
+  // FENE-P coupling (nullptr when off)
+  const double2* TF[3];  // polymer force div(qp) (spectral) added to NL1..NL3 before smoothing
+  void* TPD[3];          // transfer: D u, D v, D w (physical y, transfer precision)
 };
+
+// Per-pencil FENE-P stage (main_dns.f90:206-321, helpers.f90:498-674 and the
+// spectral half of the polymer terms of vcw3dp).
+struct PolyArgs {
+  int nxh, nz, nzh, P, ny;
+  double alpha, beta, dyde, re, dt, theta;
+  double diffpoly, tpoly, zbeta1, gC;  // gC = re diffpoly/(dt theta)
+  double c0[3];                        // c11z, c22z, c33z
+  int partB;     // forward-transform NLP into c_n, str_n (else load CN, STRN)
+  int force;     // write TF = div(zbeta1 str_n) (unsmoothed)
+  int copy;      // CNM1/STRNM1 = unsmoothed c_n/str_n (n -> n-1 before smoothing)
+  int smooth;
+  int init;      // C = diag(c11z, c22z, c33z) before the solve
+  int solve;     // polyNL + phirhs + Neumann penta
+  int evaluate;  // write the transfer fields C, D C
+  const double* smz;
+  const double* smy;
+  const void* twM;
+  const double2* tabC;  // [P][65] 1/d tables, wn = k2 + gC
+  const void* NLP[12];  // c_n (6), str_n (6): physical y, transfer precision
+  double2* C[6];
+  double2* CN[6];
+  double2* CNM1[6];
+  double2* STRN[6];
+  double2* STRNM1[6];
+  double2* TF[3];
+  void* TP[12];         // transfer: C (6), D C (6)
+};
+hipError_t launch_poly_pencil(int ny, int mode, const PolyArgs& a, bool mixed, hipStream_t s);
 
 // Plane-stage arguments.
 struct PlaneArgs {
@@ -63,7 +95,27 @@ struct PlaneArgs {
   const int* fixrow;      // [nyp*mz+1] solid cells per (y,z) row (no -dPdx)
   const int* fixx;        // x index per solid cell
   double* cap;            // nullptr, or [3][nyp][mz][mx] physical u, v, w (FP64)
+  unsigned dzmask;        // k_zinv: bit f set -> input f is multiplied by i kz first
+  double beta;            // z wavenumber unit
 };
+
+// Plane stage of the FENE-P terms (solvers.f90:2901-2983 and the transforms
+// around them).  Z-transformed inputs, [f][nyp][mz][nxh] at transfer precision.
+struct PolyPlaneArgs {
+  int nxh, nz, nzh, nyp, mx, mz;
+  int mixed;
+  double alpha, beta;
+  int ipeter;
+  double L2;              // zlmax^2
+  const void* twZ;
+  const void* twX;
+  const void* U;          // u, v, w (fields 0..2 of the NS intermediates)
+  const void* in[24];     // spectral-xz inputs of the z stage (see kmm_api.cpp)
+  void* ZP;               // [24]: dz u(3), dy u(3), C(6), dz C(6), dy C(6)
+  void* ZFP;              // [12]: c_n(6), str_n(6) after the forward x transform
+  void* out[12];          // NLP outputs [y][kx][kz]
+};
+hipError_t launch_poly_plane(const PolyPlaneArgs& a, hipStream_t s);
 
 // Lagrangian tracers (part_track.f90): particle j has x, y, z, u, v, w at
 // part[0..5][j]; fields are read at fld[f*fstride + k*sy + jz*sz + ix*sx]

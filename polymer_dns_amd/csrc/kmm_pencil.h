@@ -382,4 +382,58 @@ __device__ __forceinline__ PF phirhs(const PF& v, const PF& d2v, const PF& fn, c
   return r;
 }
 
+
+// ------------------------------------------------ shared by the pencil kernels --
+constexpr int PW = 8;  // wavefronts (pencils) per workgroup
+
+template <int NY, typename V>
+__device__ __forceinline__ V* wave_lds(V* smem, int wv) {
+  // twiddles [2NY], then two padded FFT buffers of length 2NY per wave
+  return smem + 2 * NY + (size_t)wv * 2 * FFTBuf<2 * NY>::len;
+}
+// block-wide copy of the length-2NY twiddles into LDS (before any early exit)
+template <int NY, typename V>
+__device__ __forceinline__ const V* lds_twiddles(V* smem, const V* __restrict__ tw) {
+  for (int e = threadIdx.x; e < 2 * NY; e += blockDim.x) smem[e] = tw[e];
+  __syncthreads();
+  return smem;
+}
+
+// (kx, kz) wavenumbers of pencil p = kx_index * nz + kz_index
+template <class Args>
+__device__ __forceinline__ void pencil_k(const Args& A, int p, double& kx, double& kz, int& kxi, int& kzi) {
+  kxi = p / A.nz;
+  kzi = p % A.nz;
+  const int kzs = (kzi < A.nzh) ? kzi : kzi - A.nz;
+  kx = kxi * A.alpha;
+  kz = kzs * A.beta;
+}
+
+template <int NY>
+__device__ __forceinline__ PF smooth_apply(const PF& f, const double* __restrict__ smy, double sz, int lane) {
+  constexpr int L = NY / 2;
+  PF r = f;
+  if (lane < L) {
+    r.a = cscale(f.a, sz * smy[2 * lane]);
+    r.b = cscale(f.b, sz * smy[2 * lane + 1]);
+  }
+  r.e = cscale(f.e, sz * smy[NY]);
+  return r;
+}
+
+// host-side helpers of the pencil launchers
+#define KMM_NY_DISPATCH(ny, ...) \
+  switch (ny) {                   \
+    case 32: { constexpr int NY = 32; __VA_ARGS__; } break; \
+    case 48: { constexpr int NY = 48; __VA_ARGS__; } break; \
+    case 64: { constexpr int NY = 64; __VA_ARGS__; } break; \
+    case 96: { constexpr int NY = 96; __VA_ARGS__; } break; \
+    case 128: { constexpr int NY = 128; __VA_ARGS__; } break; \
+    default: return hipErrorInvalidValue; \
+  }
+
+static inline int nblk(int P) { return (P + PW - 1) / PW; }
+template <int NY, typename V>
+static inline size_t lds_bytes() { return (size_t)(PW * 2 * FFTBuf<2 * NY>::len + 2 * NY) * sizeof(V); }
+
 }  // namespace kmm

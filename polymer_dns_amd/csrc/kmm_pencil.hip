@@ -16,21 +16,6 @@
 
 namespace kmm {
 
-constexpr int PW = 8;  // wavefronts (pencils) per workgroup
-
-template <int NY, typename V>
-__device__ __forceinline__ V* wave_lds(V* smem, int wv) {
-  // twiddles [2NY], then two padded FFT buffers of length 2NY per wave
-  return smem + 2 * NY + (size_t)wv * 2 * FFTBuf<2 * NY>::len;
-}
-// block-wide copy of the length-2NY twiddles into LDS (before any early exit)
-template <int NY, typename V>
-__device__ __forceinline__ const V* lds_twiddles(V* smem, const V* __restrict__ tw) {
-  for (int e = threadIdx.x; e < 2 * NY; e += blockDim.x) smem[e] = tw[e];
-  __syncthreads();
-  return smem;
-}
-
 // ---------------------------------------------------------- 1/d tables --
 // d recurrence of eqsolve/pntsolve (solvers.f90:500-553, 931-935) per pencil
 // and parity; thread per pencil.
@@ -78,13 +63,6 @@ __global__ void k_tau_tables(int P, int nz, double alpha, double beta, double dy
   }
 }
 
-__device__ __forceinline__ void pencil_k(const PencilArgs& A, int p, double& kx, double& kz, int& kxi, int& kzi) {
-  kxi = p / A.nz;
-  kzi = p % A.nz;
-  const int kzs = (kzi < A.nzh) ? kzi : kzi - A.nz;
-  kx = kxi * A.alpha;
-  kz = kzs * A.beta;
-}
 
 // ------------------------------------------------- Green's functions --
 // gfcn (solvers.f90:5-172) with real boundary data: the reference's complex
@@ -143,17 +121,6 @@ __device__ __forceinline__ PF pf_cmul_real(double2 c, const PF& g) {
   return r;
 }
 
-template <int NY>
-__device__ __forceinline__ PF smooth_apply(const PF& f, const double* __restrict__ smy, double sz, int lane) {
-  constexpr int L = NY / 2;
-  PF r = f;
-  if (lane < L) {
-    r.a = cscale(f.a, sz * smy[2 * lane]);
-    r.b = cscale(f.b, sz * smy[2 * lane + 1]);
-  }
-  r.e = cscale(f.e, sz * smy[NY]);
-  return r;
-}
 
 // MODE 0: full step (NL from buffers or FN/GN/SCN), MODE 1: startup
 // (derived fields of the current state only), MODE 2: finish (NL -> FN, GN,
@@ -192,8 +159,9 @@ __global__ __launch_bounds__(PW * 64) void k_pencil(PencilArgs A) {
       if (SCALAR) pf_store<NY>(A.SCN, P, p, lane, z);
     }
     if (MODE != 2)
-      for (int t = 0; t < nT; ++t) {
-        V* d = TT[t];
+      for (int t = 0; t < nT + 3; ++t) {
+        V* d = (t < nT) ? TT[t] : static_cast<V*>(A.TPD[t - nT]);
+        if (!d) continue;
         if (lane < L) { d[(size_t)(2 * lane) * P + p] = vzero<V>(); d[(size_t)(2 * lane + 1) * P + p] = vzero<V>(); }
         if (lane == 0) d[(size_t)NY * P + p] = vzero<V>();
       }
@@ -209,6 +177,11 @@ __global__ __launch_bounds__(PW * 64) void k_pencil(PencilArgs A) {
       PF h1 = phys_to_cheb_load<NY, V>(static_cast<const V*>(A.NL1), P, p, buf, scr, twl, lane);
       PF h3 = phys_to_cheb_load<NY, V>(static_cast<const V*>(A.NL3), P, p, buf, scr, twl, lane);
       PF h2 = phys_to_cheb_load<NY, V>(static_cast<const V*>(A.NL2), P, p, buf, scr, twl, lane);
+      if (A.TF[0]) {  // polymer force, subforce (helpers.f90:564-606), before smoothing
+        h1 = pf_add(h1, pf_load<NY>(A.TF[0], P, p, lane));
+        h2 = pf_add(h2, pf_load<NY>(A.TF[1], P, p, lane));
+        h3 = pf_add(h3, pf_load<NY>(A.TF[2], P, p, lane));
+      }
       if (A.smooth) {
         h1 = smooth_apply<NY>(h1, A.smy, sz, lane);
         h2 = smooth_apply<NY>(h2, A.smy, sz, lane);
@@ -364,8 +337,14 @@ __global__ __launch_bounds__(PW * 64) void k_pencil(PencilArgs A) {
     u = pf_scale(pf_sub(pf_imul(dv, kx), pf_imul(omy, kz)), ik2);
     w = pf_scale(pf_add(pf_imul(dv, kz), pf_imul(omy, kx)), ik2);
   }
-  PF ox = pf_sub(cderiv<NY>(w, dy, lane), pf_imul(v, kz));
-  PF oz = pf_sub(pf_imul(v, kx), cderiv<NY>(u, dy, lane));
+  const PF du = cderiv<NY>(u, dy, lane), dw = cderiv<NY>(w, dy, lane);
+  PF ox = pf_sub(dw, pf_imul(v, kz));
+  PF oz = pf_sub(pf_imul(v, kx), du);
+  if (A.TPD[0]) {  // D u, D v, D w for the velocity-gradient tensor (gradv, derivatives.f90:6-86)
+    cheb_to_phys_store<NY, V>(du, buf, scr, twl, lane, static_cast<V*>(A.TPD[0]), P, p);
+    cheb_to_phys_store<NY, V>(cderiv<NY>(v, dy, lane), buf, scr, twl, lane, static_cast<V*>(A.TPD[1]), P, p);
+    cheb_to_phys_store<NY, V>(dw, buf, scr, twl, lane, static_cast<V*>(A.TPD[2]), P, p);
+  }
   cheb_to_phys_store<NY, V>(u, buf, scr, twl, lane, TT[0], P, p);
   cheb_to_phys_store<NY, V>(v, buf, scr, twl, lane, TT[1], P, p);
   cheb_to_phys_store<NY, V>(w, buf, scr, twl, lane, TT[2], P, p);
@@ -444,20 +423,6 @@ __global__ __launch_bounds__(PW * 64) void k_penta(int P, double dyde, const dou
 }
 
 // ------------------------------------------------------------ launchers --
-#define KMM_NY_DISPATCH(ny, ...) \
-  switch (ny) {                   \
-    case 32: { constexpr int NY = 32; __VA_ARGS__; } break; \
-    case 48: { constexpr int NY = 48; __VA_ARGS__; } break; \
-    case 64: { constexpr int NY = 64; __VA_ARGS__; } break; \
-    case 96: { constexpr int NY = 96; __VA_ARGS__; } break; \
-    case 128: { constexpr int NY = 128; __VA_ARGS__; } break; \
-    default: return hipErrorInvalidValue; \
-  }
-
-static inline int nblk(int P) { return (P + PW - 1) / PW; }
-template <int NY, typename V>
-static inline size_t lds_bytes() { return (size_t)(PW * 2 * FFTBuf<2 * NY>::len + 2 * NY) * sizeof(V); }
-
 hipError_t launch_tau_tables(int ny, const PencilArgs& a, double g, double2* tab, hipStream_t s) {
   KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_tau_tables<NY>), dim3((a.P + 127) / 128), dim3(128), 0, s, a.P, a.nz,
                                           a.alpha, a.beta, a.dyde, g, (const double*)nullptr, tab));

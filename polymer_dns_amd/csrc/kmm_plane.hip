@@ -12,6 +12,8 @@
 //   k_zfwd   : forward c2c along z, keep nz modes, 1/(mx mz), and the +-kz
 //              projection of ``norm`` (helpers.f90:91-158).
 // Intermediate layout Z[f][y][z][kx] keeps the x lines contiguous.
+#include <cstring>
+
 #include "kmm_kernels.h"
 #include "kmm_device.h"
 
@@ -37,9 +39,16 @@ __global__ __launch_bounds__(512) void k_zinv(PlaneArgs A) {
     const V* src = static_cast<const V*>(A.in[f]) + ((size_t)y * A.nxh + kx) * A.nz;
     for (int z = lane; z < MZ; z += 64) buf[fpad(z)] = vzero<V>();
     wave_sync();
+    const bool dz = (A.dzmask >> f) & 1u;  // d/dz = multiplication by i kz
     for (int j = lane; j < A.nz; j += 64) {
       const int jj = (j < A.nzh) ? j : (MZ - A.nz + j);
-      buf[fpad(jj)] = src[j];
+      V val = src[j];
+      if (dz) {
+        using T = typename Scal<V>::T;
+        const T wz = (T)(A.beta * (double)((j < A.nzh) ? j : j - A.nz));
+        val = mkv<V>(-val.y * wz, val.x * wz);
+      }
+      buf[fpad(jj)] = val;
     }
     wave_sync();
     V* r = wave_fft<MZ, +1>(buf, scr, tw, lane);
@@ -287,6 +296,189 @@ __global__ __launch_bounds__(512) void k_zfwd(PlaneArgs A) {
 }
 
 // ------------------------------------------------------------ launchers --
+// ------------------------------------------------------------ FENE-P --
+// One wavefront per (y, z) row.  Two real fields of the row share one
+// complex x transform: spectra a_k, b_k (k < nxh) of the two fields are
+// packed as a + i b with the Hermitian extension; after the inverse
+// transform the real part is field a, the imaginary part field b.  With
+// dxa/dxb the spectrum is first multiplied by i alpha k (d/dx).
+template <int MX, typename V>
+__device__ __forceinline__ void xrow_pair_inv(const V* __restrict__ ra, bool dxa, const V* __restrict__ rb, bool dxb,
+                                              int nxh, double alpha, V* buf, V* scr, const V* tw, int lane,
+                                              typename Scal<V>::T (&oa)[(MX + 63) / 64],
+                                              typename Scal<V>::T (&ob)[(MX + 63) / 64]) {
+  using T = typename Scal<V>::T;
+  constexpr int XPL = (MX + 63) / 64;
+  for (int x = lane; x < MX; x += 64) buf[fpad(x)] = vzero<V>();
+  wave_sync();
+  for (int k = lane; k < nxh; k += 64) {
+    V ua = ra[k], ub = rb[k];
+    const T wk = (T)(alpha * (double)k);
+    if (dxa) ua = mkv<V>(-ua.y * wk, ua.x * wk);
+    if (dxb) ub = mkv<V>(-ub.y * wk, ub.x * wk);
+    if (k == 0) {
+      buf[0] = mkv<V>(ua.x, ub.x);
+    } else {
+      buf[fpad(k)] = mkv<V>(ua.x - ub.y, ua.y + ub.x);
+      buf[fpad(MX - k)] = mkv<V>(ua.x + ub.y, ub.x - ua.y);
+    }
+  }
+  wave_sync();
+  V* r = wave_fft<MX, +1>(buf, scr, tw, lane);
+#pragma unroll
+  for (int m = 0; m < XPL; ++m) {
+    const int x = lane + 64 * m;
+    const V v = (x < MX) ? r[fpad(x)] : vzero<V>();
+    oa[m] = v.x;
+    ob[m] = v.y;
+  }
+  wave_sync();
+}
+
+// forward r2c of two real rows through one complex transform; writes the
+// nxh kept modes of each (the 1/(mx mz) scale is applied in k_zfwd)
+template <int MX, typename V>
+__device__ __forceinline__ void xrow_pair_fwd(const typename Scal<V>::T (&a)[(MX + 63) / 64],
+                                              const typename Scal<V>::T (&b)[(MX + 63) / 64], V* __restrict__ oa,
+                                              V* __restrict__ ob, int nxh, V* buf, V* scr, const V* tw, int lane) {
+  using T = typename Scal<V>::T;
+  constexpr int XPL = (MX + 63) / 64;
+#pragma unroll
+  for (int m = 0; m < XPL; ++m) {
+    const int x = lane + 64 * m;
+    if (x < MX) buf[fpad(x)] = mkv<V>(a[m], b[m]);
+  }
+  wave_sync();
+  V* r = wave_fft<MX, -1>(buf, scr, tw, lane);
+  for (int k = lane; k < nxh; k += 64) {
+    const V zk = r[fpad(k)], zm = r[fpad((MX - k) % MX)];
+    oa[k] = mkv<V>((T)0.5 * (zk.x + zm.x), (T)0.5 * (zk.y - zm.y));
+    ob[k] = mkv<V>((T)0.5 * (zk.y + zm.y), (T)-0.5 * (zk.x - zm.x));
+  }
+  wave_sync();
+}
+
+// Pointwise FENE-P terms of vcw3dp (solvers.f90:2901-2983) on one row:
+//   c_ij^n = sum_k C_ik du_j/dx_k + sum_k C_kj du_i/dx_k - u . grad C_ij
+//   str_ij = f C_ij - delta_ij,  f = (L^2-3)/(L^2-trC) (ipeter 1) or 1
+// Field order of the independent components: 11, 12, 13, 22, 23, 33.
+template <int MX, typename V>
+__global__ __launch_bounds__(256) void k_xplane_poly(PolyPlaneArgs A) {
+  using T = typename Scal<V>::T;
+  constexpr int XPL = (MX + 63) / 64;
+  constexpr int WPB = 4;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  V* sm = reinterpret_cast<V*>(smraw);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  constexpr int BL = FFTBuf<MX>::len;
+  V* tw = sm;
+  V* buf = sm + MX + (size_t)wv * 2 * BL;
+  V* scr = buf + BL;
+  for (int e = threadIdx.x; e < MX; e += WPB * 64) tw[e] = static_cast<const V*>(A.twX)[e];
+  __syncthreads();
+  const int row = blockIdx.x * WPB + wv;  // grid covers nyp * mz rows exactly
+  const int y = row / A.mz, z = row % A.mz;
+  const int nxh = A.nxh;
+  const size_t plane = (size_t)A.nyp * A.mz * nxh;
+  const size_t roff = ((size_t)y * A.mz + z) * nxh;
+  const V* U = static_cast<const V*>(A.U);
+  const V* ZP = static_cast<const V*>(A.ZP);
+  auto urow = [&](int f) { return U + (size_t)f * plane + roff; };
+  auto zrow = [&](int f) { return ZP + (size_t)f * plane + roff; };
+  const double al = A.alpha;
+  // velocity and its gradient g[i][k] = du_i/dx_k
+  T u[3][XPL], g[3][3][XPL];
+  xrow_pair_inv<MX, V>(urow(0), false, urow(0), true, nxh, al, buf, scr, tw, lane, u[0], g[0][0]);
+  xrow_pair_inv<MX, V>(urow(1), false, urow(1), true, nxh, al, buf, scr, tw, lane, u[1], g[1][0]);
+  xrow_pair_inv<MX, V>(urow(2), false, urow(2), true, nxh, al, buf, scr, tw, lane, u[2], g[2][0]);
+  xrow_pair_inv<MX, V>(zrow(3), false, zrow(4), false, nxh, al, buf, scr, tw, lane, g[0][1], g[1][1]);
+  xrow_pair_inv<MX, V>(zrow(5), false, zrow(0), false, nxh, al, buf, scr, tw, lane, g[2][1], g[0][2]);
+  xrow_pair_inv<MX, V>(zrow(1), false, zrow(2), false, nxh, al, buf, scr, tw, lane, g[1][2], g[2][2]);
+  T c[6][XPL];
+  xrow_pair_inv<MX, V>(zrow(6), false, zrow(7), false, nxh, al, buf, scr, tw, lane, c[0], c[1]);
+  xrow_pair_inv<MX, V>(zrow(8), false, zrow(9), false, nxh, al, buf, scr, tw, lane, c[2], c[3]);
+  xrow_pair_inv<MX, V>(zrow(10), false, zrow(11), false, nxh, al, buf, scr, tw, lane, c[4], c[5]);
+  T s[6][XPL], st[6][XPL];
+  const T L2 = (T)A.L2;
+#pragma unroll
+  for (int m = 0; m < XPL; ++m) {
+    const T c11 = c[0][m], c12 = c[1][m], c13 = c[2][m], c22 = c[3][m], c23 = c[4][m], c33 = c[5][m];
+    // C_ik (i = row, k = column), symmetric
+    const T C[3][3] = {{c11, c12, c13}, {c12, c22, c23}, {c13, c23, c33}};
+    const int I[6] = {0, 0, 0, 1, 1, 2}, J[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int i = I[q], j = J[q];
+      s[q][m] = C[i][0] * g[j][0][m] + C[i][1] * g[j][1][m] + C[i][2] * g[j][2][m] +
+                C[0][j] * g[i][0][m] + C[1][j] * g[i][1][m] + C[2][j] * g[i][2][m];
+    }
+    const T tr = c11 + c22 + c33;
+    const T fac = A.ipeter ? (L2 - (T)3) / (L2 - tr) : (T)1;
+    st[0][m] = fac * c11 - (T)1;
+    st[1][m] = fac * c12;
+    st[2][m] = fac * c13;
+    st[3][m] = fac * c22 - (T)1;
+    st[4][m] = fac * c23;
+    st[5][m] = fac * c33 - (T)1;
+  }
+  // - u . grad C_ij, streamed two components per transform
+  T da[XPL], db[XPL];
+#pragma unroll
+  for (int dir = 0; dir < 3; ++dir) {
+#pragma unroll
+    for (int q = 0; q < 6; q += 2) {
+      // dir 0: d/dx of C (fields 6+q), dir 1: d/dy C (18+q), dir 2: d/dz C (12+q)
+      const int fa = (dir == 0) ? 6 + q : (dir == 1 ? 18 + q : 12 + q);
+      xrow_pair_inv<MX, V>(zrow(fa), dir == 0, zrow(fa + 1), dir == 0, nxh, al, buf, scr, tw, lane, da, db);
+#pragma unroll
+      for (int m = 0; m < XPL; ++m) {
+        s[q][m] -= u[dir][m] * da[m];
+        s[q + 1][m] -= u[dir][m] * db[m];
+      }
+    }
+  }
+  V* ZF = static_cast<V*>(A.ZFP);
+  auto orow = [&](int f) { return ZF + (size_t)f * plane + roff; };
+#pragma unroll
+  for (int q = 0; q < 6; q += 2) {
+    xrow_pair_fwd<MX, V>(s[q], s[q + 1], orow(q), orow(q + 1), nxh, buf, scr, tw, lane);
+    xrow_pair_fwd<MX, V>(st[q], st[q + 1], orow(6 + q), orow(7 + q), nxh, buf, scr, tw, lane);
+  }
+}
+
+template <int MX, int MZ, typename V>
+static hipError_t poly_plane_impl(const PolyPlaneArgs& pa, hipStream_t s) {
+  // z inverse of the 24 polymer inputs (8 per launch, d/dz by mask)
+  PlaneArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.nxh = pa.nxh; a.nz = pa.nz; a.nzh = pa.nzh; a.nyp = pa.nyp; a.mx = pa.mx; a.mz = pa.mz;
+  a.twZ = pa.twZ; a.twX = pa.twX; a.beta = pa.beta;
+  const int ngx = a.nxh / 8;
+  const size_t zlds = (size_t)(16 * FFTBuf<MZ>::len + 9 * MZ) * sizeof(V);
+  const size_t zb = (size_t)pa.nyp * pa.mz * pa.nxh;
+  // ZP layout: dz u(3), dy u(3), C(6), dz C(6), dy C(6); the dz fields are
+  // [0..2] and [12..17]
+  for (int g = 0; g < 3; ++g) {
+    a.dzmask = 0;
+    for (int f = 0; f < 8; ++f) {
+      const int o = 8 * g + f;
+      a.in[f] = pa.in[o];
+      if (o < 3 || (o >= 12 && o < 18)) a.dzmask |= 1u << f;
+    }
+    a.Z = static_cast<V*>(pa.ZP) + (size_t)8 * g * zb;
+    hipLaunchKernelGGL((k_zinv<MZ, 8, V>), dim3(a.nyp * ngx), dim3(512), zlds, s, a);
+  }
+  const size_t xlds = (size_t)(MX + 4 * 2 * FFTBuf<MX>::len) * sizeof(V);
+  hipLaunchKernelGGL((k_xplane_poly<MX, V>), dim3(pa.nyp * pa.mz / 4), dim3(256), xlds, s, pa);
+  // forward z of the 12 outputs, 4 per launch
+  for (int g = 0; g < 3; ++g) {
+    a.ZF = static_cast<V*>(pa.ZFP) + (size_t)4 * g * zb;
+    for (int o = 0; o < 4; ++o) a.out[o] = pa.out[4 * g + o];
+    hipLaunchKernelGGL((k_zfwd<MZ, 4, V>), dim3(a.nyp * ngx), dim3(512), zlds, s, a);
+  }
+  return hipGetLastError();
+}
+
 template <int MX, int MZ, int NFIN, int NFOUT, typename V>
 static hipError_t plane_impl(const PlaneArgs& a, hipStream_t s, bool inverse_part, hipEvent_t* ev) {
   const int ngx = a.nxh / 8;
@@ -333,6 +525,23 @@ static hipError_t plane_dispatch(const PlaneArgs& a, hipStream_t s, bool inv, hi
   KMM_PLANE_CASE(288, 288)
   KMM_PLANE_CASE(384, 192)
 #undef KMM_PLANE_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_poly_plane(const PolyPlaneArgs& a, hipStream_t s) {
+  if (a.nxh % 8 != 0 || (a.nyp * a.mz) % 4 != 0) return hipErrorInvalidValue;
+#define KMM_POLY_CASE(X, Zz)                                                              \
+  if (a.mx == X && a.mz == Zz)                                                            \
+    return a.mixed ? poly_plane_impl<X, Zz, float2>(a, s) : poly_plane_impl<X, Zz, double2>(a, s);
+  KMM_POLY_CASE(48, 48)
+  KMM_POLY_CASE(96, 48)
+  KMM_POLY_CASE(96, 96)
+  KMM_POLY_CASE(144, 144)
+  KMM_POLY_CASE(192, 96)
+  KMM_POLY_CASE(192, 192)
+  KMM_POLY_CASE(288, 288)
+  KMM_POLY_CASE(384, 192)
+#undef KMM_POLY_CASE
   return hipErrorInvalidValue;
 }
 

@@ -1,0 +1,169 @@
+// Pencil stage of the FENE-P conformation tensor (POLYMER build of the
+// reference), one wavefront per (kx,kz) pencil, the six independent
+// components c11, c12, c13, c22, c23, c33 one after another:
+//
+//   part B  forward Chebyshev transform of the plane stage's c_n and str_n
+//           (solvers.f90:3279-3416); the polymer force
+//           t = div(qp), qp = zbeta1 str_n (polyforce, helpers.f90:498-560,
+//           before smoothing as in main_dns.f90:674-680); the n -> n-1 copy
+//           that the reference makes before smoothing (main_dns.f90:682-703);
+//           smoothing (main_dns.f90:726-737)
+//   init    C = diag(c11z, c22z, c33z) (main_dns.f90:206-236; the constant's
+//           only Chebyshev-Fourier coefficient is the (0,0,0) one)
+//   solve   polyNL (helpers.f90:610-674), phirhs with fn = cNL,
+//           fnm1 = 2 cNL (main_dns.f90:249-261), n -> n-1 copy (:263-281)
+//           and the Neumann penta solve with g = re diffpoly/(dt theta)
+//           (:286-304)
+//   evaluate  C and dC/dy to physical y for the plane stage (derivscji's
+//           y-derivatives, derivatives.f90:148-254; the x and z derivatives
+//           are taken in the plane stage).
+//
+// MODE 0: step, MODE 1: startup (evaluate only), MODE 2: finish (part B only).
+#include "kmm_kernels.h"
+#include "kmm_pencil.h"
+
+namespace kmm {
+
+template <int NY, int MODE, typename V>
+__global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
+  constexpr int L = NY / 2;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  V* smem = reinterpret_cast<V*>(smraw);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int p = blockIdx.x * PW + wv;
+  const V* twl = lds_twiddles<NY, V>(smem, static_cast<const V*>(A.twM));
+  if (p >= A.P) return;
+  V* buf = wave_lds<NY, V>(smem, wv);
+  V* scr = buf + FFTBuf<2 * NY>::len;
+  const size_t P = (size_t)A.P;
+  double kx, kz;
+  int kxi, kzi;
+  pencil_k(A, p, kx, kz, kxi, kzi);
+  const double k2 = kx * kx + kz * kz;
+  const bool is00 = (kxi == 0 && kzi == 0);
+  const double dy = A.dyde;
+
+  if (kzi == A.nzh) {  // norm zeroes the z-Nyquist row (helpers.f90:149-156)
+    const PF z = pf_zero();
+    for (int c = 0; c < 6; ++c) {
+      if (MODE == 0 || MODE == 2) {
+        if (A.partB || MODE == 2) { pf_store<NY>(A.CN[c], P, p, lane, z); pf_store<NY>(A.STRN[c], P, p, lane, z); }
+        if (A.copy || A.solve) { pf_store<NY>(A.CNM1[c], P, p, lane, z); pf_store<NY>(A.STRNM1[c], P, p, lane, z); }
+      }
+      if (MODE == 0 && (A.init || A.solve)) pf_store<NY>(A.C[c], P, p, lane, z);
+      if (MODE != 2 && A.evaluate)
+        for (int h = 0; h < 2; ++h) {
+          V* d = static_cast<V*>(A.TP[c + 6 * h]);
+          if (lane < L) { d[(size_t)(2 * lane) * P + p] = vzero<V>(); d[(size_t)(2 * lane + 1) * P + p] = vzero<V>(); }
+          if (lane == 0) d[(size_t)NY * P + p] = vzero<V>();
+        }
+    }
+    if (A.force && (MODE == 2 || A.partB))
+      for (int t = 0; t < 3; ++t) pf_store<NY>(A.TF[t], P, p, lane, z);
+    return;
+  }
+
+  const double sz = A.smooth ? A.smz[kzi] : 1.0;
+  const double ti = 1.0 / A.tpoly;
+  PF t1 = pf_zero(), t2 = pf_zero(), t3 = pf_zero();
+#pragma unroll 1
+  for (int c = 0; c < 6; ++c) {
+    // component c: 11, 12, 13, 22, 23, 33
+    PF cn, sn;
+    PF cnm1_copy, snm1_copy;
+    const bool haveB = (MODE == 2) || (MODE == 0 && A.partB);
+    if (haveB) {
+      cn = phys_to_cheb_load<NY, V>(static_cast<const V*>(A.NLP[c]), P, p, buf, scr, twl, lane);
+      sn = phys_to_cheb_load<NY, V>(static_cast<const V*>(A.NLP[6 + c]), P, p, buf, scr, twl, lane);
+      if (is00) { cn = pf_real(cn); sn = pf_real(sn); }
+      if (A.force) {
+        // t_i = d q_ik / d x_k with q = zbeta1 str (symmetric)
+        const PF q = pf_scale(sn, A.zbeta1);
+        switch (c) {
+          case 0: t1 = pf_add(t1, pf_imul(q, kx)); break;
+          case 1: t1 = pf_add(t1, cderiv<NY>(q, dy, lane)); t2 = pf_add(t2, pf_imul(q, kx)); break;
+          case 2: t1 = pf_add(t1, pf_imul(q, kz)); t3 = pf_add(t3, pf_imul(q, kx)); break;
+          case 3: t2 = pf_add(t2, cderiv<NY>(q, dy, lane)); break;
+          case 4: t2 = pf_add(t2, pf_imul(q, kz)); t3 = pf_add(t3, cderiv<NY>(q, dy, lane)); break;
+          default: t3 = pf_add(t3, pf_imul(q, kz)); break;
+        }
+      }
+      if (A.copy) {
+        cnm1_copy = cn;
+        snm1_copy = sn;
+        pf_store<NY>(A.CNM1[c], P, p, lane, cn);
+        pf_store<NY>(A.STRNM1[c], P, p, lane, sn);
+      }
+      if (A.smooth) {
+        cn = smooth_apply<NY>(cn, A.smy, sz, lane);
+        sn = smooth_apply<NY>(sn, A.smy, sz, lane);
+      }
+      pf_store<NY>(A.CN[c], P, p, lane, cn);
+      pf_store<NY>(A.STRN[c], P, p, lane, sn);
+      if (MODE == 2) continue;
+    }
+    PF C;
+    if (MODE == 0) {
+      if (A.init) {
+        C = pf_zero();
+        if (is00 && (c == 0 || c == 3 || c == 5)) {
+          const double v0 = A.c0[c == 0 ? 0 : (c == 3 ? 1 : 2)];
+          if (lane == 0) C.a = make_double2(v0, 0.0);
+        }
+      } else {
+        C = pf_load<NY>(A.C[c], P, p, lane);
+      }
+      if (A.solve) {
+        if (!haveB) {
+          cn = pf_load<NY>(A.CN[c], P, p, lane);
+          sn = pf_load<NY>(A.STRN[c], P, p, lane);
+        }
+        const PF cm = A.copy && haveB ? cnm1_copy : pf_load<NY>(A.CNM1[c], P, p, lane);
+        const PF sm = A.copy && haveB ? snm1_copy : pf_load<NY>(A.STRNM1[c], P, p, lane);
+        // cNL = 3 (c_n - str_n/lambda) - (c_nm1 - str_nm1/lambda)
+        const PF nl = pf_sub(pf_scale(pf_axpy(-ti, sn, cn), 3.0), pf_axpy(-ti, sm, cm));
+        pf_store<NY>(A.CNM1[c], P, p, lane, cn);
+        pf_store<NY>(A.STRNM1[c], P, p, lane, sn);
+        const PF d1 = cderiv<NY>(C, dy, lane);
+        const PF d2 = cderiv<NY>(d1, dy, lane);
+        const PF rhs = phirhs<NY>(C, d2, nl, pf_scale(nl, 2.0), k2, A.re, A.diffpoly, A.dt, A.theta);
+        // Neumann rows at both walls (at = ab = 0, bt = bb = 1, zero data)
+        auto tt = [dy](int n) -> double { return dy * (double)n * (double)n; };
+        auto tb = [dy](int n) -> double { double v = dy * (double)n * (double)n; return (n & 1) ? -v : v; };
+        C = tau_penta<NY>(rhs, k2 + A.gC, dy, A.tabC + (size_t)p * 65, tt, tb, czero(), czero(), lane);
+        if (is00) C = pf_real(C);
+      }
+      if (A.init || A.solve) pf_store<NY>(A.C[c], P, p, lane, C);
+    } else {
+      C = pf_load<NY>(A.C[c], P, p, lane);
+    }
+    if (A.evaluate) {
+      cheb_to_phys_store<NY, V>(C, buf, scr, twl, lane, static_cast<V*>(A.TP[c]), P, p);
+      cheb_to_phys_store<NY, V>(cderiv<NY>(C, dy, lane), buf, scr, twl, lane, static_cast<V*>(A.TP[6 + c]), P, p);
+    }
+  }
+  if (A.force && ((MODE == 0 && A.partB) || MODE == 2)) {
+    if (is00) { t1 = pf_real(t1); t2 = pf_real(t2); t3 = pf_real(t3); }
+    pf_store<NY>(A.TF[0], P, p, lane, t1);
+    pf_store<NY>(A.TF[1], P, p, lane, t2);
+    pf_store<NY>(A.TF[2], P, p, lane, t3);
+  }
+}
+
+template <int MODE, typename V>
+static hipError_t launch_poly_v(int ny, const PolyArgs& a, hipStream_t s) {
+  KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_poly_pencil<NY, MODE, V>), dim3(nblk(a.P)), dim3(PW * 64),
+                                         (lds_bytes<NY, V>()), s, a));
+  return hipGetLastError();
+}
+
+hipError_t launch_poly_pencil(int ny, int mode, const PolyArgs& a, bool mixed, hipStream_t s) {
+  switch (mode) {
+    case 0: return mixed ? launch_poly_v<0, float2>(ny, a, s) : launch_poly_v<0, double2>(ny, a, s);
+    case 1: return mixed ? launch_poly_v<1, float2>(ny, a, s) : launch_poly_v<1, double2>(ny, a, s);
+    case 2: return mixed ? launch_poly_v<2, float2>(ny, a, s) : launch_poly_v<2, double2>(ny, a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace kmm

@@ -24,10 +24,13 @@ LIB_PATH = os.path.join(_HERE, "libkmm.so")
 KMM_OK, KMM_E_CFL, KMM_E_PARTICLE, KMM_E_HIP, KMM_E_CONFIG, KMM_E_IO, KMM_E_ARG = range(7)
 
 # field ids (include/kmm.h)
+POLY = ("11", "12", "13", "22", "23", "33")   # independent conformation components
+POLY_FIELDS = (["c" + q for q in POLY] + ["c%sn" % q for q in POLY] + ["c%snm1" % q for q in POLY]
+               + ["str%sn" % q for q in POLY] + ["str%snm1" % q for q in POLY])
 FIELDS = ["v", "fn", "fnm1", "omy", "gn", "gnm1", "u0", "h1n", "h1nm1", "w0", "h3n", "h3nm1",
-          "fxintg", "fyintg", "fzintg", "scalar", "scn", "scnm1"]
+          "fxintg", "fyintg", "fzintg", "scalar", "scn", "scnm1"] + POLY_FIELDS
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
-SPECTRAL = {"v", "fn", "fnm1", "omy", "gn", "gnm1", "scalar", "scn", "scnm1"}
+SPECTRAL = {"v", "fn", "fnm1", "omy", "gn", "gnm1", "scalar", "scn", "scnm1"} | set(POLY_FIELDS)
 MEAN = {"u0", "h1n", "h1nm1", "w0", "h3n", "h3nm1"}
 PHYS = {"fxintg", "fyintg", "fzintg"}
 
@@ -60,6 +63,9 @@ class KmmConfig(C.Structure):
         ("particle_flag", C.c_int),
         ("ratio", C.c_double), ("ap", C.c_double), ("c_mu", C.c_double),
         ("cd_switch", C.c_int),
+        ("polymer", C.c_int), ("itarget", C.c_int), ("ipeter", C.c_int), ("ipolyflag", C.c_int),
+        ("zlmax", C.c_double), ("tpoly", C.c_double), ("alpha_poly", C.c_double), ("diffpoly", C.c_double),
+        ("c11z", C.c_double), ("c22z", C.c_double), ("c33z", C.c_double), ("qbeta", C.c_double),
     ]
 
 
@@ -150,7 +156,9 @@ def make_config(nx, ny, nz, **kw) -> KmmConfig:
                     perturbtime=0, r_tau=float(np.sqrt(3.0 * 2800.0)), irstrt=1, crstrt=0,
                     flow_select=1, scalar=0, scl_flag=0, diff=1.0, deltaT=0.0,
                     src_start=0, src_stop=-1, cadence=10, precision=0,
-                    particle_flag=0, ratio=0.0, ap=0.0, c_mu=1.0, cd_switch=0)
+                    particle_flag=0, ratio=0.0, ap=0.0, c_mu=1.0, cd_switch=0,
+                    polymer=0, itarget=0, ipeter=1, ipolyflag=1, zlmax=30.0, tpoly=0.1, alpha_poly=0.0,
+                    diffpoly=0.5, c11z=1.0, c22z=1.0, c33z=1.0, qbeta=0.9)
     defaults.update(kw)
     cfg.nx, cfg.ny, cfg.nz = nx, ny, nz
     for k, v in defaults.items():
@@ -241,14 +249,14 @@ class Channel:
         _check(self.lib, self.lib.kmm_stage_times(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
-    KERNELS = ("k_pencil", "k_zinv", "k_ibdrag", "k_xplane", "k_zfwd")
+    KERNELS = ("k_pencil", "k_zinv", "k_ibdrag", "k_xplane", "k_zfwd", "k_poly_pencil", "k_poly_plane", "k_tracer")
 
     def set_timing(self, on=True):
         _check(self.lib, self.lib.kmm_set_timing(self.h, int(bool(on))))
 
     def kernel_times(self):
-        """Average ms per launch of each step kernel over the last step() call."""
-        a = (C.c_double * 5)()
+        """Average ms per step of each kernel group over the last step() call."""
+        a = (C.c_double * 8)()
         n = C.c_int()
         _check(self.lib, self.lib.kmm_kernel_times(self.h, a, C.byref(n)))
         return dict(zip(self.KERNELS, list(a))), n.value

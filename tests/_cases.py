@@ -33,3 +33,29 @@ def rel(a, b):
     d = np.abs(np.asarray(a) - np.asarray(b)).max()
     s = np.abs(np.asarray(b)).max()
     return d / s if s > 0 else d
+
+
+def poly_state(p, seed, amp=1e-3, kmax=3, nmax=6):
+    """Conformation tensor = identity + a smooth seeded perturbation of
+    amplitude amp (low Fourier modes times low Chebyshev polynomials), as the
+    six independent spectral components {"c11": ..., ...}."""
+    rng = np.random.default_rng(seed)
+    x = (np.arange(p.nx) * p.xl / p.nx)[:, None, None]
+    z = (np.arange(p.nz) * p.zl / p.nz)[None, :, None]
+    yy = (p.ycoord / (p.yl / 2))[None, None, :]
+    T = [np.ones_like(yy), yy]
+    for n in range(2, nmax + 1):
+        T.append(2 * yy * T[-1] - T[-2])
+    out = {}
+    for ij in O.POLY:
+        f = np.zeros((p.nx, p.nz, p.nyp))
+        for _ in range(6):
+            kx, kz, n = rng.integers(0, kmax + 1), rng.integers(-kmax, kmax + 1), rng.integers(0, nmax + 1)
+            ph = rng.uniform(0, 2 * np.pi)
+            f = f + rng.standard_normal() * np.cos(kx * p.alpha * x + kz * p.beta * z + ph) * T[n]
+        f = amp * f / max(1e-30, np.abs(f).max())
+        if ij in ("11", "22", "33"):
+            f = f + 1.0
+        out["c" + ij] = O.xyzfft(f, p)
+        O.norm(out["c" + ij])
+    return out

@@ -1,0 +1,125 @@
+"""GPU parity of the FENE-P conformation-tensor path (POLYMER build) through
+the C ABI: the full channel step with the polymer against the oracle's step
+on the same seeded fields (FP64 to 1e-10 relative per spectral field), the
+release time-line (C initialised at it == irstrt), Oldroyd-B, the mixed
+precision mode, and the c-restart round trip."""
+import numpy as np
+import pytest
+
+from oracle import kmm_oracle as O
+from tests._cases import params, poly_state, rel, seeded
+
+pytestmark = pytest.mark.gpu
+
+kmm = pytest.importorskip("polymer_dns_amd.kmm")
+
+POLY_KW = dict(polymer=True, zlmax=30.0, tpoly=0.5, diffpoly=0.5, qbeta=0.9)
+
+
+def channel(p, precision=0):
+    cfg = kmm.make_config(p.nx, p.ny, p.nz, xl=p.xl, zl=p.zl, re=p.re, dt=p.dt, r_tau=p.R_tau,
+                          polymer=1, zlmax=p.zlmax, tpoly=p.tpoly, diffpoly=p.diffpoly, qbeta=p.qbeta,
+                          c11z=p.c11z, c22z=p.c22z, c33z=p.c33z, ipeter=p.ipeter, itarget=0,
+                          src_start=p.src_start, irstrt=p.irstrt, crstrt=p.crstrt, precision=precision)
+    return kmm.Channel(cfg)
+
+
+def start_pair(p, st, precision=0):
+    ch = channel(p, precision)
+    for k in ("v", "omy", "u0", "w0"):
+        ch.put(k, st[k])
+    for ij in O.POLY:
+        if "c" + ij in st:
+            ch.put("c" + ij, st["c" + ij])
+    ch.startup()
+    sol = O.Solver(p)
+    ost = {k: v.copy() for k, v in st.items()}
+    sol.startup(ost)
+    return ch, sol, ost
+
+
+def compare(ch, ost, tol, keys=None):
+    keys = keys or (["v", "omy", "fn", "gn"] + ["c" + q for q in O.POLY] + ["c%sn" % q for q in O.POLY]
+                    + ["str%sn" % q for q in O.POLY] + ["c%snm1" % q for q in O.POLY])
+    worst = {}
+    for k in keys:
+        worst[k] = rel(ch.get(k), ost[k])
+    bad = {k: v for k, v in worst.items() if not v < tol}
+    assert not bad, bad
+    return worst
+
+
+def test_polymer_step_matches_oracle_fp64():
+    p = params(32, 32, 32, dt=0.005, crstrt=1, src_start=0, **POLY_KW)
+    st = seeded(p, 31, amp=0.2)
+    st.update(poly_state(p, 32))
+    ch, sol, ost = start_pair(p, st)
+    compare(ch, ost, 1e-10)
+    for _ in range(6):
+        ch.step(1)
+        sol.step(ost)
+    compare(ch, ost, 1e-10)
+    ch.close()
+
+
+def test_polymer_release_initialises_at_irstrt():
+    """crstrt = 0, irstrt = 1: C starts at zero, becomes diag(c11z, c22z,
+    c33z) at step 1, and the n -> n-1 copy of step 1 is taken before smoothing."""
+    p = params(32, 32, 32, dt=0.005, crstrt=0, irstrt=1, src_start=0, c11z=1.0, c22z=1.0, c33z=1.0, **POLY_KW)
+    st = seeded(p, 33, amp=0.2)
+    ch, sol, ost = start_pair(p, st)
+    for _ in range(4):
+        ch.step(1)
+        sol.step(ost)
+    compare(ch, ost, 1e-10)
+    ch.close()
+
+
+def test_oldroyd_b_matches_oracle():
+    p = params(32, 32, 32, dt=0.005, crstrt=1, src_start=0, ipeter=0, **POLY_KW)
+    st = seeded(p, 35, amp=0.2)
+    st.update(poly_state(p, 36))
+    ch, sol, ost = start_pair(p, st)
+    for _ in range(3):
+        ch.step(1)
+        sol.step(ost)
+    compare(ch, ost, 1e-10)
+    ch.close()
+
+
+def test_polymer_mixed_precision_error():
+    p = params(32, 32, 32, dt=0.005, crstrt=1, src_start=0, **POLY_KW)
+    st = seeded(p, 37, amp=0.2)
+    st.update(poly_state(p, 38))
+    ch, sol, ost = start_pair(p, st, precision=1)
+    for _ in range(6):
+        ch.step(1)
+        sol.step(ost)
+    w = compare(ch, ost, 1e-4, keys=["v", "omy"] + ["c" + q for q in O.POLY])
+    # the conformation tensor keeps FP64 state and solves: its error stays small
+    assert max(w["c" + q] for q in O.POLY) < 1e-5, w
+    ch.close()
+
+
+def test_polymer_crestart_round_trip(tmp_path):
+    p = params(32, 32, 32, dt=0.005, crstrt=1, src_start=0, **POLY_KW)
+    st = seeded(p, 39, amp=0.2)
+    st.update(poly_state(p, 40))
+    ch, _, _ = start_pair(p, st)
+    ch.step(3)
+    ch.write_restart(tmp_path / "restart")
+    ch.write_crestart(tmp_path / "c-restart")
+    raw = (tmp_path / "c-restart").read_bytes()
+    fb = 16 * p.nxh * p.nz * p.nyp
+    assert int.from_bytes(raw[:4], "little") == 10 * fb          # scalar + 9 C components
+    ch2 = channel(p)
+    ch2.read_restart(tmp_path / "restart")
+    ch2.read_crestart(tmp_path / "c-restart")
+    for k in ["c" + q for q in O.POLY] + ["str%snm1" % q for q in O.POLY]:
+        assert np.array_equal(ch.get(k), ch2.get(k)), k
+    ch.step(2)
+    ch2.step(2)
+    for k in ["v", "omy", "c12", "c22n"]:
+        assert np.array_equal(ch.get(k), ch2.get(k)), k
+    ch.close()
+    ch2.close()
