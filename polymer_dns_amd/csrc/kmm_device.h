@@ -197,12 +197,38 @@ struct FFTPlan {
   int nst;
   int r[16];
 };
+// Radix plan for one wavefront: each stage runs ceil((N/R)/64) butterfly
+// rounds, so the plan minimises the total number of rounds (then the number
+// of stages) over factorisations into radices 8, 6, 4, 3, 2.
+struct PlanSearch {
+  int best_cost, best_nst, cur_nst;
+  int best[16], cur[16];
+};
+__host__ __device__ constexpr void plan_dfs(PlanSearch& s, int n, int m, int cost) {
+  if (m == 1) {
+    if (cost < s.best_cost || (cost == s.best_cost && s.cur_nst < s.best_nst)) {
+      s.best_cost = cost;
+      s.best_nst = s.cur_nst;
+      for (int i = 0; i < s.cur_nst; ++i) s.best[i] = s.cur[i];
+    }
+    return;
+  }
+  if (s.cur_nst >= 16 || cost >= s.best_cost) return;
+  const int rad[5] = {8, 6, 4, 3, 2};
+  for (int i = 0; i < 5; ++i) {
+    const int r = rad[i];
+    if (m % r) continue;
+    const int nb = n / r;
+    s.cur[s.cur_nst++] = r;
+    plan_dfs(s, n, m / r, cost + (nb + 63) / 64);
+    --s.cur_nst;
+  }
+}
 __host__ __device__ constexpr FFTPlan make_plan(int n) {
-  FFTPlan p{n, 0, {0}};
-  int m = n;
-  while (m % 3 == 0) { p.r[p.nst++] = 3; m /= 3; }
-  while (m % 4 == 0) { p.r[p.nst++] = 4; m /= 4; }
-  while (m % 2 == 0) { p.r[p.nst++] = 2; m /= 2; }
+  PlanSearch s{1 << 30, 99, 0, {0}, {0}};
+  plan_dfs(s, n, n, 0);
+  FFTPlan p{n, s.best_nst, {0}};
+  for (int i = 0; i < s.best_nst; ++i) p.r[i] = s.best[i];
   return p;
 }
 
@@ -245,6 +271,45 @@ __device__ __forceinline__ void dft4(V& a, V& b, V& c, V& d) {
   b = cadd(t1, it3);
   d = csub(t1, it3);
 }
+// 6-point DFT, natural order: two 3-point DFTs of the even/odd samples
+template <int DIR, typename V>
+__device__ __forceinline__ void dft6(V& v0, V& v1, V& v2, V& v3, V& v4, V& v5) {
+  using T = typename Scal<V>::T;
+  const T sg = (DIR < 0) ? (T)-1 : (T)1;
+  const T h = (T)0.5, s3 = (T)0.86602540378443864676 * sg;
+  V e0 = v0, e1 = v2, e2 = v4, o0 = v1, o1 = v3, o2 = v5;
+  dft3<DIR>(e0, e1, e2);
+  dft3<DIR>(o0, o1, o2);
+  const V w1o1 = cmul(o1, mkv<V>(h, s3));    // W6^1 o1
+  const V w2o2 = cmul(o2, mkv<V>(-h, s3));   // W6^2 o2
+  v0 = cadd(e0, o0);
+  v3 = csub(e0, o0);
+  v1 = cadd(e1, w1o1);
+  v4 = csub(e1, w1o1);
+  v2 = cadd(e2, w2o2);
+  v5 = csub(e2, w2o2);
+}
+// 8-point DFT, natural order: two 4-point DFTs of the even/odd samples
+template <int DIR, typename V>
+__device__ __forceinline__ void dft8(V& v0, V& v1, V& v2, V& v3, V& v4, V& v5, V& v6, V& v7) {
+  using T = typename Scal<V>::T;
+  const T sg = (DIR < 0) ? (T)-1 : (T)1;
+  const T r = (T)0.70710678118654752440;
+  V e0 = v0, e1 = v2, e2 = v4, e3 = v6, o0 = v1, o1 = v3, o2 = v5, o3 = v7;
+  dft4<DIR>(e0, e1, e2, e3);
+  dft4<DIR>(o0, o1, o2, o3);
+  const V w1 = cmul(o1, mkv<V>(r, sg * r));          // W8^1 o1
+  const V w2 = mkv<V>(-sg * o2.y, sg * o2.x);        // W8^2 o2 = sg i o2
+  const V w3 = cmul(o3, mkv<V>(-r, sg * r));         // W8^3 o3
+  v0 = cadd(e0, o0);
+  v4 = csub(e0, o0);
+  v1 = cadd(e1, w1);
+  v5 = csub(e1, w1);
+  v2 = cadd(e2, w2);
+  v6 = csub(e2, w2);
+  v3 = cadd(e3, w3);
+  v7 = csub(e3, w3);
+}
 
 // FFT buffers in LDS are padded by one element every 16 so that the
 // stride-R writes of the Stockham passes do not pile onto the same banks.
@@ -266,7 +331,29 @@ __device__ V* wave_fft(V* x, V* y, const V* __restrict__ tw, int lane) {
     for (int j = lane; j < nb; j += WAVE) {
       const int k = j % Ns;
       const int idxD = (j / Ns) * Ns * R + k;
-      if (R == 4) {
+      if (R == 8) {
+        V v[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) v[m] = x[fpad(j + m * nb)];
+        if (Ns > 1) {
+#pragma unroll
+          for (int m = 1; m < 8; ++m) v[m] = cmul(v[m], twid<DIR>(tw, m * k * tstride));
+        }
+        dft8<DIR>(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) y[fpad(idxD + m * Ns)] = v[m];
+      } else if (R == 6) {
+        V v[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) v[m] = x[fpad(j + m * nb)];
+        if (Ns > 1) {
+#pragma unroll
+          for (int m = 1; m < 6; ++m) v[m] = cmul(v[m], twid<DIR>(tw, m * k * tstride));
+        }
+        dft6<DIR>(v[0], v[1], v[2], v[3], v[4], v[5]);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) y[fpad(idxD + m * Ns)] = v[m];
+      } else if (R == 4) {
         V v0 = x[fpad(j)], v1 = x[fpad(j + nb)], v2 = x[fpad(j + 2 * nb)], v3 = x[fpad(j + 3 * nb)];
         if (Ns > 1) {
           v1 = cmul(v1, twid<DIR>(tw, k * tstride));

@@ -13,6 +13,7 @@
 //              projection of ``norm`` (helpers.f90:91-158).
 // Intermediate layout Z[f][y][z][kx] keeps the x lines contiguous.
 #include <cstring>
+#include <utility>
 
 #include "kmm_kernels.h"
 #include "kmm_device.h"
@@ -302,39 +303,6 @@ __global__ __launch_bounds__(512) void k_zfwd(PlaneArgs A) {
 // packed as a + i b with the Hermitian extension; after the inverse
 // transform the real part is field a, the imaginary part field b.  With
 // dxa/dxb the spectrum is first multiplied by i alpha k (d/dx).
-template <int MX, typename V>
-__device__ __forceinline__ void xrow_pair_inv(const V* __restrict__ ra, bool dxa, const V* __restrict__ rb, bool dxb,
-                                              int nxh, double alpha, V* buf, V* scr, const V* tw, int lane,
-                                              typename Scal<V>::T (&oa)[(MX + 63) / 64],
-                                              typename Scal<V>::T (&ob)[(MX + 63) / 64]) {
-  using T = typename Scal<V>::T;
-  constexpr int XPL = (MX + 63) / 64;
-  for (int x = lane; x < MX; x += 64) buf[fpad(x)] = vzero<V>();
-  wave_sync();
-  for (int k = lane; k < nxh; k += 64) {
-    V ua = ra[k], ub = rb[k];
-    const T wk = (T)(alpha * (double)k);
-    if (dxa) ua = mkv<V>(-ua.y * wk, ua.x * wk);
-    if (dxb) ub = mkv<V>(-ub.y * wk, ub.x * wk);
-    if (k == 0) {
-      buf[0] = mkv<V>(ua.x, ub.x);
-    } else {
-      buf[fpad(k)] = mkv<V>(ua.x - ub.y, ua.y + ub.x);
-      buf[fpad(MX - k)] = mkv<V>(ua.x + ub.y, ub.x - ua.y);
-    }
-  }
-  wave_sync();
-  V* r = wave_fft<MX, +1>(buf, scr, tw, lane);
-#pragma unroll
-  for (int m = 0; m < XPL; ++m) {
-    const int x = lane + 64 * m;
-    const V v = (x < MX) ? r[fpad(x)] : vzero<V>();
-    oa[m] = v.x;
-    ob[m] = v.y;
-  }
-  wave_sync();
-}
-
 // forward r2c of two real rows through one complex transform; writes the
 // nxh kept modes of each (the 1/(mx mz) scale is applied in k_zfwd)
 template <int MX, typename V>
@@ -358,92 +326,225 @@ __device__ __forceinline__ void xrow_pair_fwd(const typename Scal<V>::T (&a)[(MX
   wave_sync();
 }
 
+// Spectral inputs of one pair transform, kept in registers so that the
+// loads of the next pair overlap the current transform.
+template <int MX, typename V>
+struct PairIn {
+  static constexpr int KPL = (MX / 3 + 63) / 64;  // nxh = MX/3 modes over 64 lanes
+  V a[KPL], b[KPL];
+};
+template <int MX, typename V>
+__device__ __forceinline__ void pair_load(PairIn<MX, V>& r, const V* __restrict__ ra, const V* __restrict__ rb,
+                                          int nxh, int lane) {
+#pragma unroll
+  for (int j = 0; j < PairIn<MX, V>::KPL; ++j) {
+    const int k = lane + 64 * j;
+    if (k < nxh) {
+      r.a[j] = ra[k];
+      r.b[j] = rb[k];
+    }
+  }
+}
+template <int MX, typename V>
+__device__ __forceinline__ void pair_inv(const PairIn<MX, V>& r, bool dxa, bool dxb, int nxh, double alpha, V* buf,
+                                         V* scr, const V* tw, int lane, typename Scal<V>::T (&oa)[(MX + 63) / 64],
+                                         typename Scal<V>::T (&ob)[(MX + 63) / 64]) {
+  using T = typename Scal<V>::T;
+  constexpr int XPL = (MX + 63) / 64;
+  for (int x = lane; x < MX; x += 64) buf[fpad(x)] = vzero<V>();
+  wave_sync();
+#pragma unroll
+  for (int j = 0; j < PairIn<MX, V>::KPL; ++j) {
+    const int k = lane + 64 * j;
+    if (k < nxh) {
+      V ua = r.a[j], ub = r.b[j];
+      const T wk = (T)(alpha * (double)k);
+      if (dxa) ua = mkv<V>(-ua.y * wk, ua.x * wk);
+      if (dxb) ub = mkv<V>(-ub.y * wk, ub.x * wk);
+      if (k == 0) {
+        buf[0] = mkv<V>(ua.x, ub.x);
+      } else {
+        buf[fpad(k)] = mkv<V>(ua.x - ub.y, ua.y + ub.x);
+        buf[fpad(MX - k)] = mkv<V>(ua.x + ub.y, ub.x - ua.y);
+      }
+    }
+  }
+  wave_sync();
+  V* res = wave_fft<MX, +1>(buf, scr, tw, lane);
+#pragma unroll
+  for (int m = 0; m < XPL; ++m) {
+    const int x = lane + 64 * m;
+    const V v = (x < MX) ? res[fpad(x)] : vzero<V>();
+    oa[m] = v.x;
+    ob[m] = v.y;
+  }
+  wave_sync();
+}
+
+// The 18 pair transforms of a row: field ids >= 0 index the z-stage outputs
+// (dz u 0-2, dy u 3-5, C 6-11, dz C 12-17, dy C 18-23), ids < 0 the velocity
+// intermediates (-1 - f); dx = multiply by i alpha k first.
+struct PairSpec {
+  int fa, fb;
+  bool dxa, dxb;
+};
+
 // Pointwise FENE-P terms of vcw3dp (solvers.f90:2901-2983) on one row:
 //   c_ij^n = sum_k C_ik du_j/dx_k + sum_k C_kj du_i/dx_k - u . grad C_ij
 //   str_ij = f C_ij - delta_ij,  f = (L^2-3)/(L^2-trC) (ipeter 1) or 1
 // Field order of the independent components: 11, 12, 13, 22, 23, 33.
+// s_ij += C_ik du_j/dx_k + C_kj du_i/dx_k for one gradient component
+// gv = du_a/dx_k (all six independent (i,j); both terms when i = j = a)
+template <typename T>
+__device__ __forceinline__ void stretch_add(T (&s)[6], const T (&c)[6], int a, int k, T gv) {
+  const int I[6] = {0, 0, 0, 1, 1, 2}, J[6] = {0, 1, 2, 1, 2, 2};
+  // symmetric C_ik from the independent components
+  auto C = [&](int i, int j) -> T {
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    return c[lo == 0 ? hi : (lo == 1 ? 2 + hi : 5)];
+  };
+#pragma unroll
+  for (int e = 0; e < 6; ++e) {
+    if (J[e] == a) s[e] += C(I[e], k) * gv;
+    if (I[e] == a) s[e] += C(k, J[e]) * gv;
+  }
+}
+
+// Order chosen for register pressure: C, then the nine velocity-gradient
+// components (accumulated straight into the stretching terms; C dies after
+// them), u v w, the 18 dC components (advection), and C once more for the
+// Peterlin stress after the c_n transforms have freed their registers.
+constexpr int kNP = 21;
+__host__ __device__ constexpr PairSpec poly_pair(int q) {
+  constexpr PairSpec SP[kNP] = {
+      {6, 7, false, false},   {8, 9, false, false},   {10, 11, false, false},   // C11 C12 | C13 C22 | C23 C33
+      {-1, -2, true, true},   {-3, 3, true, false},   {4, 5, false, false},     // du/dx dv/dx | dw/dx du/dy | dv/dy dw/dy
+      {0, 1, false, false},   {2, -1, false, false},  {-2, -3, false, false},   // du/dz dv/dz | dw/dz u | v w
+      {6, 7, true, true},     {8, 9, true, true},     {10, 11, true, true},     // dC/dx
+      {18, 19, false, false}, {20, 21, false, false}, {22, 23, false, false},   // dC/dy
+      {12, 13, false, false}, {14, 15, false, false}, {16, 17, false, false},   // dC/dz
+      {6, 7, false, false},   {8, 9, false, false},   {10, 11, false, false}};  // C again
+  return SP[q];
+}
+
+template <int MX, typename V>
+struct PolyRow {
+  using T = typename Scal<V>::T;
+  static constexpr int XPL = (MX + 63) / 64;
+  T u[3][XPL], c[XPL][6], s[XPL][6];
+  PairIn<MX, V> pin[2];
+  const V* U;
+  const V* ZP;
+  V* ZF;
+  size_t plane, roff;
+  int nxh, lane, ipeter;
+  double alpha;
+  T L2;
+  V *buf, *scr;
+  const V* tw;
+  __device__ const V* rowp(int id) const {
+    return id < 0 ? U + (size_t)(-1 - id) * plane + roff : ZP + (size_t)id * plane + roff;
+  }
+  __device__ V* orow(int f) const { return ZF + (size_t)f * plane + roff; }
+
+  template <int Q>
+  __device__ __forceinline__ void step() {
+    constexpr PairSpec sp = poly_pair(Q);
+    if constexpr (Q + 1 < kNP) {
+      constexpr PairSpec nx = poly_pair(Q + 1);
+      pair_load<MX, V>(pin[(Q + 1) & 1], rowp(nx.fa), rowp(nx.fb), nxh, lane);
+    }
+    if constexpr (Q == 18) {  // c_n complete: forward transforms of its three pairs
+      fwd_s<0>();
+      fwd_s<2>();
+      fwd_s<4>();
+    }
+    T oa[XPL], ob[XPL];
+    pair_inv<MX, V>(pin[Q & 1], sp.dxa, sp.dxb, nxh, alpha, buf, scr, tw, lane, oa, ob);
+#pragma unroll
+    for (int m = 0; m < XPL; ++m) {
+      if constexpr (Q == 0 || Q == 18) { c[m][0] = oa[m]; c[m][1] = ob[m]; }
+      else if constexpr (Q == 1 || Q == 19) { c[m][2] = oa[m]; c[m][3] = ob[m]; }
+      else if constexpr (Q == 2 || Q == 20) { c[m][4] = oa[m]; c[m][5] = ob[m]; }
+      else if constexpr (Q == 3) { stretch_add(s[m], c[m], 0, 0, oa[m]); stretch_add(s[m], c[m], 1, 0, ob[m]); }
+      else if constexpr (Q == 4) { stretch_add(s[m], c[m], 2, 0, oa[m]); stretch_add(s[m], c[m], 0, 1, ob[m]); }
+      else if constexpr (Q == 5) { stretch_add(s[m], c[m], 1, 1, oa[m]); stretch_add(s[m], c[m], 2, 1, ob[m]); }
+      else if constexpr (Q == 6) { stretch_add(s[m], c[m], 0, 2, oa[m]); stretch_add(s[m], c[m], 1, 2, ob[m]); }
+      else if constexpr (Q == 7) { stretch_add(s[m], c[m], 2, 2, oa[m]); u[0][m] = ob[m]; }
+      else if constexpr (Q == 8) { u[1][m] = oa[m]; u[2][m] = ob[m]; }
+      else {
+        // - u . grad C_ij: pairs 9-11 d/dx, 12-14 d/dy, 15-17 d/dz
+        constexpr int dir = (Q - 9) / 3, cq = 2 * ((Q - 9) % 3);
+        s[m][cq] -= u[dir][m] * oa[m];
+        s[m][cq + 1] -= u[dir][m] * ob[m];
+      }
+    }
+  }
+  template <int E>
+  __device__ __forceinline__ void fwd_s() {
+    T sa[XPL], sb[XPL];
+#pragma unroll
+    for (int m = 0; m < XPL; ++m) { sa[m] = s[m][E]; sb[m] = s[m][E + 1]; }
+    xrow_pair_fwd<MX, V>(sa, sb, orow(E), orow(E + 1), nxh, buf, scr, tw, lane);
+  }
+  // Peterlin stress str = f C - I and its forward transforms
+  template <int E>
+  __device__ __forceinline__ void fwd_str() {
+    T sa[XPL], sb[XPL];
+#pragma unroll
+    for (int m = 0; m < XPL; ++m) {
+      const T tr = c[m][0] + c[m][3] + c[m][5];
+      const T fac = ipeter ? (L2 - (T)3) / (L2 - tr) : (T)1;
+      sa[m] = fac * c[m][E] - ((E == 0) ? (T)1 : (T)0);
+      sb[m] = fac * c[m][E + 1] - ((E + 1 == 3 || E + 1 == 5) ? (T)1 : (T)0);
+    }
+    xrow_pair_fwd<MX, V>(sa, sb, orow(6 + E), orow(7 + E), nxh, buf, scr, tw, lane);
+  }
+  template <int... Q>
+  __device__ __forceinline__ void run(std::integer_sequence<int, Q...>) {
+    (step<Q>(), ...);
+  }
+};
+
 template <int MX, typename V>
 __global__ __launch_bounds__(256) void k_xplane_poly(PolyPlaneArgs A) {
   using T = typename Scal<V>::T;
-  constexpr int XPL = (MX + 63) / 64;
   constexpr int WPB = 4;
   extern __shared__ __align__(16) unsigned char smraw[];
   V* sm = reinterpret_cast<V*>(smraw);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   constexpr int BL = FFTBuf<MX>::len;
   V* tw = sm;
-  V* buf = sm + MX + (size_t)wv * 2 * BL;
-  V* scr = buf + BL;
   for (int e = threadIdx.x; e < MX; e += WPB * 64) tw[e] = static_cast<const V*>(A.twX)[e];
   __syncthreads();
   const int row = blockIdx.x * WPB + wv;  // grid covers nyp * mz rows exactly
   const int y = row / A.mz, z = row % A.mz;
-  const int nxh = A.nxh;
-  const size_t plane = (size_t)A.nyp * A.mz * nxh;
-  const size_t roff = ((size_t)y * A.mz + z) * nxh;
-  const V* U = static_cast<const V*>(A.U);
-  const V* ZP = static_cast<const V*>(A.ZP);
-  auto urow = [&](int f) { return U + (size_t)f * plane + roff; };
-  auto zrow = [&](int f) { return ZP + (size_t)f * plane + roff; };
-  const double al = A.alpha;
-  // velocity and its gradient g[i][k] = du_i/dx_k
-  T u[3][XPL], g[3][3][XPL];
-  xrow_pair_inv<MX, V>(urow(0), false, urow(0), true, nxh, al, buf, scr, tw, lane, u[0], g[0][0]);
-  xrow_pair_inv<MX, V>(urow(1), false, urow(1), true, nxh, al, buf, scr, tw, lane, u[1], g[1][0]);
-  xrow_pair_inv<MX, V>(urow(2), false, urow(2), true, nxh, al, buf, scr, tw, lane, u[2], g[2][0]);
-  xrow_pair_inv<MX, V>(zrow(3), false, zrow(4), false, nxh, al, buf, scr, tw, lane, g[0][1], g[1][1]);
-  xrow_pair_inv<MX, V>(zrow(5), false, zrow(0), false, nxh, al, buf, scr, tw, lane, g[2][1], g[0][2]);
-  xrow_pair_inv<MX, V>(zrow(1), false, zrow(2), false, nxh, al, buf, scr, tw, lane, g[1][2], g[2][2]);
-  T c[6][XPL];
-  xrow_pair_inv<MX, V>(zrow(6), false, zrow(7), false, nxh, al, buf, scr, tw, lane, c[0], c[1]);
-  xrow_pair_inv<MX, V>(zrow(8), false, zrow(9), false, nxh, al, buf, scr, tw, lane, c[2], c[3]);
-  xrow_pair_inv<MX, V>(zrow(10), false, zrow(11), false, nxh, al, buf, scr, tw, lane, c[4], c[5]);
-  T s[6][XPL], st[6][XPL];
-  const T L2 = (T)A.L2;
+  PolyRow<MX, V> R;
+  R.U = static_cast<const V*>(A.U);
+  R.ZP = static_cast<const V*>(A.ZP);
+  R.ZF = static_cast<V*>(A.ZFP);
+  R.nxh = A.nxh;
+  R.plane = (size_t)A.nyp * A.mz * A.nxh;
+  R.roff = ((size_t)y * A.mz + z) * A.nxh;
+  R.lane = lane;
+  R.ipeter = A.ipeter;
+  R.alpha = A.alpha;
+  R.L2 = (T)A.L2;
+  R.buf = sm + MX + (size_t)wv * 2 * BL;
+  R.scr = R.buf + BL;
+  R.tw = tw;
 #pragma unroll
-  for (int m = 0; m < XPL; ++m) {
-    const T c11 = c[0][m], c12 = c[1][m], c13 = c[2][m], c22 = c[3][m], c23 = c[4][m], c33 = c[5][m];
-    // C_ik (i = row, k = column), symmetric
-    const T C[3][3] = {{c11, c12, c13}, {c12, c22, c23}, {c13, c23, c33}};
-    const int I[6] = {0, 0, 0, 1, 1, 2}, J[6] = {0, 1, 2, 1, 2, 2};
+  for (int m = 0; m < PolyRow<MX, V>::XPL; ++m)
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      const int i = I[q], j = J[q];
-      s[q][m] = C[i][0] * g[j][0][m] + C[i][1] * g[j][1][m] + C[i][2] * g[j][2][m] +
-                C[0][j] * g[i][0][m] + C[1][j] * g[i][1][m] + C[2][j] * g[i][2][m];
-    }
-    const T tr = c11 + c22 + c33;
-    const T fac = A.ipeter ? (L2 - (T)3) / (L2 - tr) : (T)1;
-    st[0][m] = fac * c11 - (T)1;
-    st[1][m] = fac * c12;
-    st[2][m] = fac * c13;
-    st[3][m] = fac * c22 - (T)1;
-    st[4][m] = fac * c23;
-    st[5][m] = fac * c33 - (T)1;
+    for (int e = 0; e < 6; ++e) R.s[m][e] = (T)0;
+  {
+    constexpr PairSpec p0 = poly_pair(0);
+    pair_load<MX, V>(R.pin[0], R.rowp(p0.fa), R.rowp(p0.fb), R.nxh, lane);
   }
-  // - u . grad C_ij, streamed two components per transform
-  T da[XPL], db[XPL];
-#pragma unroll
-  for (int dir = 0; dir < 3; ++dir) {
-#pragma unroll
-    for (int q = 0; q < 6; q += 2) {
-      // dir 0: d/dx of C (fields 6+q), dir 1: d/dy C (18+q), dir 2: d/dz C (12+q)
-      const int fa = (dir == 0) ? 6 + q : (dir == 1 ? 18 + q : 12 + q);
-      xrow_pair_inv<MX, V>(zrow(fa), dir == 0, zrow(fa + 1), dir == 0, nxh, al, buf, scr, tw, lane, da, db);
-#pragma unroll
-      for (int m = 0; m < XPL; ++m) {
-        s[q][m] -= u[dir][m] * da[m];
-        s[q + 1][m] -= u[dir][m] * db[m];
-      }
-    }
-  }
-  V* ZF = static_cast<V*>(A.ZFP);
-  auto orow = [&](int f) { return ZF + (size_t)f * plane + roff; };
-#pragma unroll
-  for (int q = 0; q < 6; q += 2) {
-    xrow_pair_fwd<MX, V>(s[q], s[q + 1], orow(q), orow(q + 1), nxh, buf, scr, tw, lane);
-    xrow_pair_fwd<MX, V>(st[q], st[q + 1], orow(6 + q), orow(7 + q), nxh, buf, scr, tw, lane);
-  }
+  R.run(std::make_integer_sequence<int, kNP>{});
+  R.template fwd_str<0>();
+  R.template fwd_str<2>();
+  R.template fwd_str<4>();
 }
 
 template <int MX, int MZ, typename V>

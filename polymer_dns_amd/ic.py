@@ -125,3 +125,36 @@ def seeded_state(nx, ny, nz, xl, zl, re, R_tau, seed, amp=0.05, kmax=8, yl=2.0):
         v *= s
         omy *= s
     return dict(v=v, omy=omy, u0=u0, w0=np.zeros(nyp))
+
+
+POLY = ("11", "12", "13", "22", "23", "33")
+
+
+def conformation_state(nx, ny, nz, seed, amp=1e-3, kmax=4, nmax=6):
+    """Conformation tensor C = I + seeded noise, as the six independent
+    spectral components {"c11": ..., "c33": ...} (complex (nx/2, nz, ny+1)).
+
+    Noise: for 0 <= kx <= kmax, |kz| <= kmax and Chebyshev n <= nmax, complex
+    N(0,1) coefficients from SplitMix64 scaled by ``amp / (1 + kx + |kz| + n)``;
+    the kx = 0 plane is made Hermitian in kz and the (0,0) pencil real, so the
+    physical field is real.  The identity enters as 1 in the (0,0,0)
+    coefficient of c11, c22, c33."""
+    rng = SplitMix64(seed)
+    nxh, nyp = nx // 2, ny + 1
+    out = {}
+    for ij in POLY:
+        a = np.zeros((nxh, nz, nyp), dtype=np.complex128)
+        for kx in range(0, min(kmax, nxh - 1) + 1):
+            for kzs in range(-kmax, kmax + 1):
+                kzi = kzs % nz
+                for n in range(0, min(nmax, ny) + 1):
+                    r = complex(rng.normal(), rng.normal())
+                    a[kx, kzi, n] = amp * r / (1.0 + kx + abs(kzs) + n)
+        j = np.arange(1, nz // 2)
+        a[0, nz - j, :] = np.conj(a[0, j, :])
+        a[0, 0, :] = a[0, 0, :].real
+        a[:, nz // 2, :] = 0.0
+        if ij in ("11", "22", "33"):
+            a[0, 0, 0] += 1.0
+        out["c" + ij] = a
+    return out
