@@ -73,6 +73,11 @@ typedef struct kmm_config {
   double diffpoly;        /* artificial diffusivity 1/(re diffpoly) */
   double c11z, c22z, c33z;/* initial diagonal of C */
   double qbeta;           /* solvent viscosity ratio beta */
+  /* constant mass flux (no reference counterpart): every step the mean
+   * pressure gradient is corrected so that the bulk velocity stays ubulk
+   * (ubulk <= 0: the bulk velocity of the state at kmm_startup) */
+  int cmf;
+  double ubulk;
 } kmm_config;
 
 typedef struct kmm_ctx kmm_ctx;
@@ -118,6 +123,25 @@ int kmm_startup(kmm_ctx* ctx);
 /* Advance nsteps time steps (main_dns.f90:198-811).  Returns KMM_E_CFL with
  * the failing step in kmm_last_cfl if the CFL number exceeds 1. */
 int kmm_step(kmm_ctx* ctx, int nsteps);
+
+/* gradv (derivatives.f90:6-86): the nine gradients u_ij = d u_i / d x_j
+ * and the Laplacians of u, v, w, all spectral (nyp, nz, nxh). */
+int kmm_gradv(kmm_ctx* ctx, const double* u, const double* v, const double* w, double* u11, double* u12,
+              double* u13, double* u21, double* u22, double* u23, double* u31, double* u32, double* u33,
+              double* lu, double* lv, double* lw);
+/* calcswirl (helpers.f90:360-404): swirl strength sqrt(sum Im(lambda)^2) of
+ * the velocity-gradient tensor at every point of physical arrays
+ * real(8) (nyp, mz, mx), argument order as the reference's dummy list; 0
+ * where the eigenvalues are all real (the reference returns rounding noise
+ * there) or the result is not finite. */
+int kmm_calcswirl(kmm_ctx* ctx, const double* u11, const double* u21, const double* u31, const double* u12,
+                  const double* u22, const double* u32, const double* u13, const double* u23, const double* u33,
+                  double* swirl);
+
+/* Mean pressure gradient dPdx (init_flow.f90:117; updated every step when
+ * cmf is set) and its override. */
+int kmm_get_dpdx(kmm_ctx* ctx, double* dpdx);
+int kmm_set_dpdx(kmm_ctx* ctx, double dpdx);
 
 /* Largest CFL number of the last step run and the step counter it. */
 int kmm_last_cfl(kmm_ctx* ctx, double* cflmax, int* it);

@@ -79,6 +79,11 @@ class Params:
     src_start: int = 0
     irstrt: int = 1
     crstrt: int = 0
+    # constant mass flux (extension, no reference counterpart): dPdx is
+    # corrected every step so that the bulk velocity stays at ubulk
+    # (ubulk <= 0: the bulk velocity of the state at startup)
+    cmf: bool = False
+    ubulk: float = 0.0
 
     def __post_init__(self):
         assert self.ny % 2 == 0 and self.nz % 2 == 0 and self.nx % 2 == 0
@@ -457,6 +462,34 @@ def uzero(u0, h1, h1l, uflag, p: Params):
     return psolve1d(rhs, g, p.dyde, p.atw, p.btu, p.gtw, p.abw, p.bbw, p.gbw)
 
 
+def bulk(a):
+    """Mean over y in [-yl/2, yl/2] of a Chebyshev series a[..., n]:
+    (1/2) int_{-1}^{1} T_n = 1/(1-n^2) for even n, 0 for odd n."""
+    n = np.arange(a.shape[-1])
+    w = np.zeros(a.shape[-1])
+    w[n % 2 == 0] = 1.0 / (1.0 - n[n % 2 == 0] ** 2.0)
+    return (np.asarray(a) * w).sum(axis=-1)
+
+
+def cmf_unit_response(p: Params):
+    """Mean-flow response phi to a unit streamwise forcing held over one step:
+    the uzero system (solvers.f90:1065-1176) with right-hand side -re/theta
+    (the forcing term -re (3 f - f)/(2 theta) for f = 1) and homogeneous
+    wall data."""
+    rhs = np.zeros(p.nyp)
+    rhs[0] = -p.re / p.theta
+    g = p.re / (p.theta * p.dt)
+    return psolve1d(rhs, g, p.dyde, p.atu, p.btu, 0.0, p.abu, p.bbu, 0.0)
+
+
+def cmf_correct(u0, p: Params):
+    """Constant mass flux: u0 += f phi with f = (ubulk - bulk(u0))/bulk(phi);
+    the pressure gradient of the following steps becomes dPdx - f."""
+    phi = cmf_unit_response(p)
+    f = (p.ubulk - bulk(u0)) / bulk(phi)
+    return u0 + f * phi, f
+
+
 def veloc(v, omy, u0, w0, p: Params):
     """solvers.f90:1576-1659 -- u, w from continuity and omega_y."""
     dv = cderiv(v, p.dyde)
@@ -511,6 +544,38 @@ def gradv(u, v, w, p: Params):
                 u21=1j * kx * v, u22=u22, u23=1j * kz * v,
                 u31=1j * kx * w, u32=u32, u33=1j * kz * w,
                 Lu=d(u12) - k2 * u, Lv=d(u22) - k2 * v, Lw=d(u32) - k2 * w)
+
+
+def eig_cardano(g):
+    """helpers.f90:197-319 -- the three eigenvalues of the velocity-gradient
+    tensor g[i][k] (arrays) by Cardano's closed form in complex arithmetic
+    with principal powers: A = tr^2/9 - (sum of principal 2x2 minors)/3,
+    B = tr^3/27 - tr m2/6 + det/2, S = (B + (B^2 - A^3)^(1/2))^(1/3),
+    l1 = tr/3 + A/S + S, l2,3 = tr/3 -+ i sqrt(3) (A/S - S)/2 - A/(2S) - S/2."""
+    u11, u12, u13 = (np.asarray(g[0][k], dtype=np.complex128) for k in range(3))
+    u21, u22, u23 = (np.asarray(g[1][k], dtype=np.complex128) for k in range(3))
+    u31, u32, u33 = (np.asarray(g[2][k], dtype=np.complex128) for k in range(3))
+    tr = u11 + u22 + u33
+    A = tr ** 2 / 9 - (u11 * u22) / 3 + (u12 * u21) / 3 - (u11 * u33) / 3 + (u13 * u31) / 3 \
+        - (u22 * u33) / 3 + (u23 * u32) / 3
+    B = tr ** 3 / 27 - (tr * (u11 * u22 - u12 * u21 + u11 * u33 - u13 * u31 + u22 * u33 - u23 * u32)) / 6 \
+        + (u11 * u22 * u33) / 2 - (u11 * u23 * u32) / 2 - (u12 * u21 * u33) / 2 + (u12 * u23 * u31) / 2 \
+        + (u13 * u21 * u32) / 2 - (u13 * u22 * u31) / 2
+    with np.errstate(all="ignore"):
+        S = (B + (B ** 2 - A ** 3) ** 0.5) ** (1.0 / 3.0)
+        l1 = tr / 3 + A / S + S
+        l2 = tr / 3 - (math.sqrt(3.0) * (A / S - S) * 1j) / 2 - A / (2 * S) - S / 2
+        l3 = tr / 3 + (math.sqrt(3.0) * (A / S - S) * 1j) / 2 - A / (2 * S) - S / 2
+    return l1, l2, l3
+
+
+def calcswirl(g):
+    """helpers.f90:360-404 -- sqrt(sum Im(lambda)^2); 0 where that is 0 or NaN."""
+    l1, l2, l3 = eig_cardano(g)
+    with np.errstate(all="ignore"):
+        sw = np.sqrt(l1.imag ** 2 + l2.imag ** 2 + l3.imag ** 2)
+        bad = ~(np.floor(sw / sw) == 1)
+    return np.where(bad, 0.0, sw)
 
 
 def smoothing_factors(p: Params):
@@ -883,6 +948,8 @@ class Solver:
         """Given v, omy, u0, w0 (and scalar), fill the derived fields and the
         AB2 history exactly as ``initial`` does at it=0."""
         p = self.p
+        if p.cmf and p.ubulk <= 0.0:
+            p.ubulk = float(bulk(np.asarray(st["u0"], dtype=np.float64)))
         for k in ("v", "omy"):
             norm(st[k])
         u, v, w = veloc(st["v"], st["omy"], st["u0"], st["w0"], p)
@@ -1033,6 +1100,9 @@ class Solver:
         v = vp + c1[:, :, None] * self.gf1 + c2[:, :, None] * self.gf2
         omy = penta(orhs, p.k2 + g, p.dyde, 0, p.atu, p.btu, 0.0, p.abu, p.bbu, 0.0)
         st["u0"] = uzero(st["u0"], st["h1n"], st["h1nm1"], 1, p)
+        if p.cmf:
+            st["u0"], fc = cmf_correct(st["u0"], p)
+            p.dPdx = p.dPdx - fc
         st["w0"] = uzero(st["w0"], st["h3n"], st["h3nm1"], 3, p)
         omy[0, 0, :] = 0.0
         st["h1nm1"] = st["h1n"].copy()

@@ -92,6 +92,8 @@ struct kmm_ctx {
   int part_state = 0;        // 0 none, 1 initialise at the next evaluation, 2 tracking
   int substep = 1;
   bool cap_valid = false;
+  double* dpdx_d = nullptr;  // mean pressure gradient (device; updated under constant mass flux)
+  double ubulk = 0.0;        // bulk-velocity target of the constant-mass-flux correction
   // FENE-P conformation tensor: independent components 11 12 13 22 23 33
   double2 *C[6] = {}, *CN[6] = {}, *CNM1[6] = {}, *STRN[6] = {}, *STRNM1[6] = {}, *TF[3] = {};
   double2* tabC = nullptr;
@@ -268,6 +270,9 @@ static PencilArgs pencil_args(kmm_ctx* c) {
   a.TH = c->TH; a.SCN = c->SCN; a.SCNM1 = c->SCNM1;
   a.u0 = c->u0; a.w0 = c->w0; a.h1n = c->h1n; a.h1nm1 = c->h1nm1; a.h3n = c->h3n; a.h3nm1 = c->h3nm1;
   for (int t = 0; t < 9; ++t) a.T[t] = c->T[t];
+  a.cmf = (c->cfg.cmf && c->cfg.flow_select == 1) ? 1 : 0;
+  a.ubulk = c->ubulk;
+  a.dpdx = c->dpdx_d;
   return a;
 }
 
@@ -278,7 +283,7 @@ static PlaneArgs plane_args(kmm_ctx* c, int it, unsigned long long* cfl) {
   a.nfin = c->cfg.scalar ? 9 : 6;
   a.nfout = c->cfg.scalar ? 4 : 3;
   a.mixed = c->cfg.precision == 1;
-  a.dt = c->cfg.dt; a.dPdx = c->dPdx;
+  a.dt = c->cfg.dt; a.dPdx = c->dPdx; a.dpdx_d = c->dpdx_d;
   a.delxm = c->cfg.xl / c->mx; a.delzm = c->cfg.zl / c->mz;
   a.src_uniform = (c->cfg.scalar && c->cfg.scl_flag == 6) ? c->cfg.deltaT : 0.0;
   a.flow_select = c->cfg.flow_select;
@@ -318,6 +323,7 @@ extern "C" int kmm_create(const kmm_config* cfg, int device, kmm_ctx** out) {
     if (cfg->ipeter != 0 && !(cfg->zlmax * cfg->zlmax > 3.0))
       return fail(KMM_E_CONFIG, "polymer: zlmax^2 must exceed 3 for the Peterlin function");
   }
+  if (cfg->cmf && cfg->flow_select != 1) return fail(KMM_E_CONFIG, "constant mass flux needs flow_select 1 (channel)");
   kmm_ctx* c = new kmm_ctx();
   c->cfg = *cfg;
   c->device = device;
@@ -405,6 +411,8 @@ extern "C" int kmm_create(const kmm_config* cfg, int device, kmm_ctx** out) {
     if ((r = up((void**)&c->smy, smy.data(), c->nyp * sizeof(double)))) goto fail_;
     if ((r = up((void**)&c->seght_d, c->seght.data(), c->nyp * sizeof(double)))) goto fail_;
     if ((r = up((void**)&c->ycoord_d, c->ycoord.data(), c->nyp * sizeof(double)))) goto fail_;
+    if ((r = up((void**)&c->dpdx_d, &c->dPdx, sizeof(double)))) goto fail_;
+    c->ubulk = cfg->ubulk;
     if ((r = dalloc(c, (void**)&c->perr, sizeof(int)))) goto fail_;
     for (int d = 0; d < 3; ++d) c->fint_full[d].assign((size_t)c->nyp * c->mz * c->mx, 0.0);
     // tau tables and Green's functions (gfcn, main_dns.f90:151-161)
@@ -826,9 +834,21 @@ static int check_cfl(kmm_ctx* c, int n, int it0) {
   return KMM_OK;
 }
 
+static double host_bulk(const std::vector<double>& a) {
+  double s = 0.0;
+  for (size_t n = 0; n < a.size(); n += 2) s += a[n] / (1.0 - (double)n * (double)n);
+  return s;
+}
+
 extern "C" int kmm_startup(kmm_ctx* c) {
   if (!c) return fail(KMM_E_ARG, "null argument");
   KMM_HIP(hipSetDevice(c->device));
+  if (c->cfg.cmf && c->cfg.ubulk <= 0.0) {  // target = bulk velocity of the initial mean flow
+    std::vector<double> u0(c->nyp);
+    KMM_HIP(hipMemcpyAsync(u0.data(), c->u0, c->nyp * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    KMM_HIP(hipStreamSynchronize(c->stream));
+    c->ubulk = host_bulk(u0);
+  }
   PencilArgs a = pencil_args(c);
   // initial() evaluates vcw3dp at it = 0: no C initialisation or solve there
   const kmm_ctx::PolyFlags pf = poly_flags(c, 0);
@@ -1474,4 +1494,77 @@ extern "C" int kmm_get_physical(kmm_ctx* c, int comp, double* out) {
       for (int ix = 0; ix < c->mx; ++ix)
         out[k + (size_t)c->nyp * (jz + (size_t)c->mz * ix)] = h[((size_t)k * c->mz + jz) * c->mx + ix];
   return KMM_OK;
+}
+
+extern "C" int kmm_get_dpdx(kmm_ctx* c, double* dpdx) {
+  if (!c || !dpdx) return fail(KMM_E_ARG, "null argument");
+  KMM_HIP(hipSetDevice(c->device));
+  KMM_HIP(hipMemcpyAsync(dpdx, c->dpdx_d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  KMM_HIP(hipStreamSynchronize(c->stream));
+  return KMM_OK;
+}
+
+extern "C" int kmm_set_dpdx(kmm_ctx* c, double dpdx) {
+  if (!c) return fail(KMM_E_ARG, "null argument");
+  KMM_HIP(hipSetDevice(c->device));
+  c->dPdx = dpdx;
+  KMM_HIP(hipMemcpyAsync(c->dpdx_d, &c->dPdx, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  KMM_HIP(hipStreamSynchronize(c->stream));
+  return KMM_OK;
+}
+
+extern "C" int kmm_gradv(kmm_ctx* c, const double* u, const double* v, const double* w, double* u11, double* u12,
+                         double* u13, double* u21, double* u22, double* u23, double* u31, double* u32, double* u33,
+                         double* lu, double* lv, double* lw) {
+  double* outs[12] = {u11, u12, u13, u21, u22, u23, u31, u32, u33, lu, lv, lw};
+  if (!c || !u || !v || !w) return fail(KMM_E_ARG, "null argument");
+  for (double* o : outs)
+    if (!o) return fail(KMM_E_ARG, "null argument");
+  KMM_HIP(hipSetDevice(c->device));
+  const size_t fb = c->S * sizeof(double2);
+  double2* d = nullptr;
+  double2** dptr = nullptr;
+  hipError_t e = hipMalloc((void**)&d, 15 * fb);
+  if (e == hipSuccess) e = hipMalloc((void**)&dptr, 12 * sizeof(double2*));
+  if (e != hipSuccess) {
+    hipFree(d);
+    return fail(KMM_E_HIP, hipGetErrorString(e));
+  }
+  int r = upload_field(c, d, u);
+  if (!r) r = upload_field(c, d + c->S, v);
+  if (!r) r = upload_field(c, d + 2 * c->S, w);
+  double2* hp[12];
+  for (int q = 0; q < 12; ++q) hp[q] = d + (3 + q) * c->S;
+  if (!r) {
+    e = hipMemcpyAsync(dptr, hp, sizeof(hp), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+      e = kmm::launch_gradv(c->ny, c->P, c->nz, c->alpha, c->beta, c->dyde, d, d + c->S, d + 2 * c->S, dptr,
+                            c->stream);
+    if (e != hipSuccess) r = fail(KMM_E_HIP, hipGetErrorString(e));
+  }
+  for (int q = 0; q < 12 && !r; ++q) r = download_field(c, hp[q], outs[q]);
+  hipFree(d);
+  hipFree(dptr);
+  return r;
+}
+
+extern "C" int kmm_calcswirl(kmm_ctx* c, const double* u11, const double* u21, const double* u31,
+                             const double* u12, const double* u22, const double* u32, const double* u13,
+                             const double* u23, const double* u33, double* swirl) {
+  // tensor components in u11 u12 u13 u21 u22 u23 u31 u32 u33 order
+  const double* in[9] = {u11, u12, u13, u21, u22, u23, u31, u32, u33};
+  if (!c || !swirl) return fail(KMM_E_ARG, "null argument");
+  for (const double* a : in)
+    if (!a) return fail(KMM_E_ARG, "null argument");
+  KMM_HIP(hipSetDevice(c->device));
+  const size_t n = (size_t)c->nyp * c->mz * c->mx;
+  double* d = nullptr;
+  hipError_t e = hipMalloc((void**)&d, 10 * n * sizeof(double));
+  for (int k = 0; k < 9 && e == hipSuccess; ++k)
+    e = hipMemcpyAsync(d + k * n, in[k], n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = kmm::launch_swirl((int)n, d, d + 9 * n, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(swirl, d + 9 * n, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(d);
+  return e == hipSuccess ? KMM_OK : fail(KMM_E_HIP, hipGetErrorString(e));
 }

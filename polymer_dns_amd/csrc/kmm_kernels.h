@@ -34,6 +34,10 @@ struct PencilArgs {
   // FENE-P coupling (nullptr when off)
   const double2* TF[3];  // polymer force div(qp) (spectral) added to NL1..NL3 before smoothing
   void* TPD[3];          // transfer: D u, D v, D w (physical y, transfer precision)
+  // constant mass flux: correct u0 and the device-resident dPdx
+  int cmf;
+  double ubulk;
+  double* dpdx;
 };
 
 // Per-pencil FENE-P stage (main_dns.f90:206-321, helpers.f90:498-674 and the
@@ -71,6 +75,7 @@ struct PlaneArgs {
   int nfin, nfout;     // input / output field counts (6/3 NS, 9/4 with scalar)
   int mixed;           // 1: FP32 transforms and products
   double dt, dPdx, delxm, delzm, src_uniform;
+  const double* dpdx_d;  // device-resident dPdx (read instead of dPdx when set)
   int flow_select;
   const double* seght;
   const void* twZ;     // length mz twiddles
@@ -138,6 +143,11 @@ struct TracerArgs {
   int* err;                 // set to 1 when a particle leaves the grid
 };
 hipError_t launch_tracer(const TracerArgs& a, hipStream_t s);
+hipError_t launch_gradv(int ny, int P, int nz, double alpha, double beta, double dyde, const double2* u,
+                        const double2* v, const double2* w, double2* const* out, hipStream_t s);
+// swirl strength of n velocity-gradient tensors; g[k*n + j] = component k
+// (u11 u12 u13 u21 ... u33 order) of point j
+hipError_t launch_swirl(int n, const double* g, double* swirl, hipStream_t s);
 
 hipError_t launch_tau_tables(int ny, const PencilArgs& a, double g, double2* tab, hipStream_t s);
 hipError_t launch_green(int ny, const PencilArgs& a, double pb1, double pt1, double pb2, double pt2,

@@ -383,6 +383,16 @@ __device__ __forceinline__ PF phirhs(const PF& v, const PF& d2v, const PF& fn, c
 }
 
 
+// mean over y of a Chebyshev series: sum over even n of a_n / (1 - n^2)
+template <int NY>
+__device__ __forceinline__ double pf_bulk(const PF& f, int lane) {
+  constexpr int L = NY / 2;
+  const double n = 2.0 * lane;
+  double v = (lane < L) ? f.a.x / (1.0 - n * n) : 0.0;
+  v = wave_sum(v);
+  return v + f.e.x / (1.0 - (double)NY * (double)NY);
+}
+
 // ------------------------------------------------ shared by the pencil kernels --
 constexpr int PW = 8;  // wavefronts (pencils) per workgroup
 

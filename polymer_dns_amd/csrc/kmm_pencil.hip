@@ -307,6 +307,18 @@ __global__ __launch_bounds__(PW * 64) void k_pencil(PencilArgs A) {
         PF un = tau_penta<NY>(r, A.gH, dy, A.tabH + (size_t)p * 65, tt, tb, make_double2(gt, 0.0),
                               make_double2(gb, 0.0), lane);
         un = pf_real(un);
+        if (w == 0 && A.cmf) {
+          // constant mass flux: u0 += f phi, phi = response to a unit forcing
+          // held over the step (rhs -re/theta, homogeneous wall data)
+          PF unit = pf_zero();
+          if (lane == 0) unit.a = make_double2(-A.re / th, 0.0);
+          PF phi = tau_penta<NY>(unit, A.gH, dy, A.tabH + (size_t)p * 65, tt, tb, czero(), czero(), lane);
+          phi = pf_real(phi);
+          const double bu = pf_bulk<NY>(un, lane), bp = pf_bulk<NY>(phi, lane);
+          const double fc = (A.ubulk - bu) / bp;
+          un = pf_axpy(fc, phi, un);
+          if (lane == 0) *A.dpdx = *A.dpdx - fc;
+        }
         pf_store_real<NY>(um, lane, un);
         pf_store_real<NY>(hl, lane, hcur);
         pf_store_real<NY>(hn, lane, hcur);
@@ -369,6 +381,30 @@ __global__ __launch_bounds__(PW * 64) void k_cderiv(int P, double dyde, const do
   if (p >= P) return;
   PF f = pf_load<NY>(in, P, p, lane);
   pf_store<NY>(out, P, p, lane, cderiv<NY>(f, dyde, lane));
+}
+
+// gradv (derivatives.f90:6-86): out[3 i + k] = d u_i / d x_k for i, k in
+// x, y, z, out[9 + i] = Laplacian of u_i (D^2 - k^2); wave per pencil
+template <int NY>
+__global__ __launch_bounds__(PW * 64) void k_gradv(int P, int nz, int nzh, double alpha, double beta, double dyde,
+                                                   const double2* __restrict__ u, const double2* __restrict__ v,
+                                                   const double2* __restrict__ w, double2* const* __restrict__ out) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int p = blockIdx.x * PW + wv;
+  if (p >= P) return;
+  const int kxi = p / nz, kzi = p % nz;
+  const double kx = kxi * alpha, kz = ((kzi < nzh) ? kzi : kzi - nz) * beta;
+  const double k2 = kx * kx + kz * kz;
+  const double2* in[3] = {u, v, w};
+#pragma unroll 1
+  for (int i = 0; i < 3; ++i) {
+    const PF f = pf_load<NY>(in[i], P, p, lane);
+    const PF d = cderiv<NY>(f, dyde, lane);
+    pf_store<NY>(out[3 * i + 0], P, p, lane, pf_imul(f, kx));
+    pf_store<NY>(out[3 * i + 1], P, p, lane, d);
+    pf_store<NY>(out[3 * i + 2], P, p, lane, pf_imul(f, kz));
+    pf_store<NY>(out[9 + i], P, p, lane, pf_axpy(-k2, f, cderiv<NY>(d, dyde, lane)));
+  }
 }
 
 // inverse: FP64 spectral in -> V physical-y out; forward: V in -> FP64 out
@@ -458,6 +494,12 @@ hipError_t launch_pencil_step(int ny, const PencilArgs& a, hipStream_t s) { retu
 hipError_t launch_pencil_startup(int ny, const PencilArgs& a, hipStream_t s) { return launch_pencil_mode<1>(ny, a, s); }
 hipError_t launch_pencil_finish(int ny, const PencilArgs& a, hipStream_t s) { return launch_pencil_mode<2>(ny, a, s); }
 
+hipError_t launch_gradv(int ny, int P, int nz, double alpha, double beta, double dyde, const double2* u,
+                        const double2* v, const double2* w, double2* const* out, hipStream_t s) {
+  KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_gradv<NY>), dim3(nblk(P)), dim3(PW * 64), 0, s, P, nz, nz / 2, alpha,
+                                         beta, dyde, u, v, w, out));
+  return hipGetLastError();
+}
 hipError_t launch_cderiv(int ny, int P, double dyde, const double2* in, double2* out, hipStream_t s) {
   KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_cderiv<NY>), dim3(nblk(P)), dim3(PW * 64), 0, s, P, dyde, in, out));
   return hipGetLastError();

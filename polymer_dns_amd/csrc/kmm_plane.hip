@@ -162,7 +162,7 @@ __global__ __launch_bounds__(WPB * 64) void k_xplane(PlaneArgs A) {
   }
   // pointwise products (solvers.f90:2891-2898), CFL (2985-2986), dPdx (3044)
   const T sgy = (T)A.seght[y];
-  const T dpdx = (T)((A.flow_select == 1) ? A.dPdx : 0.0);
+  const T dpdx = (T)((A.flow_select == 1) ? (A.dpdx_d ? *A.dpdx_d : A.dPdx) : 0.0);
   const T delxm = (T)A.delxm, delzm = (T)A.delzm, dt = (T)A.dt, src_u = (T)A.src_uniform;
   double cmax = 0.0;
 #pragma unroll

@@ -41,7 +41,8 @@ EXPORTS = [
     "kmm_cderiv", "kmm_solve", "kmm_penta", "kmm_vcw3dp", "kmm_stage_times",
     "kmm_set_timing", "kmm_kernel_times",
     "kmm_read_particles", "kmm_set_particles", "kmm_particle_count", "kmm_get_particles", "kmm_write_particles",
-    "kmm_fluid_interp", "kmm_get_physical",
+    "kmm_fluid_interp", "kmm_get_physical", "kmm_get_dpdx", "kmm_set_dpdx",
+    "kmm_gradv", "kmm_calcswirl",
 ]
 
 
@@ -66,6 +67,7 @@ class KmmConfig(C.Structure):
         ("polymer", C.c_int), ("itarget", C.c_int), ("ipeter", C.c_int), ("ipolyflag", C.c_int),
         ("zlmax", C.c_double), ("tpoly", C.c_double), ("alpha_poly", C.c_double), ("diffpoly", C.c_double),
         ("c11z", C.c_double), ("c22z", C.c_double), ("c33z", C.c_double), ("qbeta", C.c_double),
+        ("cmf", C.c_int), ("ubulk", C.c_double),
     ]
 
 
@@ -121,6 +123,10 @@ def load_library(path: str | None = None):
     lib.kmm_write_particles.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
     lib.kmm_fluid_interp.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, C.c_int, dp, dp, ip]
     lib.kmm_get_physical.argtypes = [C.c_void_p, C.c_int, dp]
+    lib.kmm_get_dpdx.argtypes = [C.c_void_p, dp]
+    lib.kmm_gradv.argtypes = [C.c_void_p] + [dp] * 15
+    lib.kmm_calcswirl.argtypes = [C.c_void_p] + [dp] * 10
+    lib.kmm_set_dpdx.argtypes = [C.c_void_p, C.c_double]
     for n in EXPORTS:
         if n != "kmm_status_string":
             getattr(lib, n).restype = C.c_int
@@ -158,7 +164,7 @@ def make_config(nx, ny, nz, **kw) -> KmmConfig:
                     src_start=0, src_stop=-1, cadence=10, precision=0,
                     particle_flag=0, ratio=0.0, ap=0.0, c_mu=1.0, cd_switch=0,
                     polymer=0, itarget=0, ipeter=1, ipolyflag=1, zlmax=30.0, tpoly=0.1, alpha_poly=0.0,
-                    diffpoly=0.5, c11z=1.0, c22z=1.0, c33z=1.0, qbeta=0.9)
+                    diffpoly=0.5, c11z=1.0, c22z=1.0, c33z=1.0, qbeta=0.9, cmf=0, ubulk=0.0)
     defaults.update(kw)
     cfg.nx, cfg.ny, cfg.nz = nx, ny, nz
     for k, v in defaults.items():
@@ -352,4 +358,31 @@ class Channel:
         mx, mz = 3 * self.nx // 2, 3 * self.nz // 2
         out = np.empty((mx, mz, self.nyp))
         _check(self.lib, self.lib.kmm_get_physical(self.h, int(comp), _dp(out)))
+        return out
+
+    # ---- mean pressure gradient ----
+    def dpdx(self):
+        v = C.c_double(0.0)
+        _check(self.lib, self.lib.kmm_get_dpdx(self.h, C.byref(v)))
+        return v.value
+
+    def set_dpdx(self, value):
+        _check(self.lib, self.lib.kmm_set_dpdx(self.h, float(value)))
+
+    # ---- velocity gradient and swirl ----
+    def gradv(self, u, v, w):
+        """dict u11..u33, Lu, Lv, Lw (spectral) of gradv (derivatives.f90:6-86)."""
+        ins = [np.ascontiguousarray(a, dtype=np.complex128) for a in (u, v, w)]
+        names = ["u11", "u12", "u13", "u21", "u22", "u23", "u31", "u32", "u33", "Lu", "Lv", "Lw"]
+        outs = {k: np.empty_like(ins[0]) for k in names}
+        _check(self.lib, self.lib.kmm_gradv(self.h, *[_dp(a) for a in ins], *[_dp(outs[k]) for k in names]))
+        return outs
+
+    def calcswirl(self, g):
+        """Swirl strength of physical gradients g[i][k] = d u_i/d x_k, each
+        (mx, mz, nyp) (calcswirl, helpers.f90:360-404)."""
+        a = [[np.ascontiguousarray(g[i][k], dtype=np.float64) for k in range(3)] for i in range(3)]
+        out = np.empty_like(a[0][0])
+        order = [a[0][0], a[1][0], a[2][0], a[0][1], a[1][1], a[2][1], a[0][2], a[1][2], a[2][2]]
+        _check(self.lib, self.lib.kmm_calcswirl(self.h, *[_dp(x) for x in order], _dp(out)))
         return out

@@ -13,7 +13,7 @@ import scipy.linalg as sla
 from numpy.polynomial import chebyshev as cheb
 
 from oracle import kmm_oracle as O
-from tests._cases import params, rel
+from tests._cases import params, rel, seeded
 
 RNG = np.random.default_rng(1234)
 
@@ -558,3 +558,66 @@ def test_polymer_relaxation_at_rest_follows_exp_decay():
         assert abs(got - exact) < 5 * (dt / lam) ** 2 * abs(c0 - 1.0), (ij, got, exact)
         assert np.abs(st["c" + ij][0, 0, 1:]).max() < 1e-11
     assert np.abs(st["c12"]).max() < 1e-14
+
+
+# ------------------------------------------------------- constant mass flux
+def test_cmf_unit_response_closed_form():
+    """phi'' - g phi = -re/theta, phi(+-1) = 0, g = re/(theta dt):
+    phi = dt (1 - cosh(sqrt(g) y)/cosh(sqrt(g)))."""
+    p = params(8, 64, 8, re=10.0, dt=0.1)
+    phi = O.cmf_unit_response(p)
+    y = np.cos(np.arange(p.nyp) * np.pi / p.ny)
+    k = math.sqrt(p.re / (p.theta * p.dt))
+    exact = p.dt * (1.0 - np.cosh(k * y) / math.cosh(k))
+    got = np.polynomial.chebyshev.chebval(y, phi)
+    assert np.abs(got - exact).max() < 1e-12
+    # bulk(): mean of the Chebyshev series
+    assert abs(O.bulk(phi) - p.dt * (1.0 - math.tanh(k) / k)) < 1e-12
+
+
+def test_cmf_keeps_bulk_velocity_and_laminar_fixed_point():
+    p = params(16, 32, 16, cmf=True)
+    st = seeded(p, 41, amp=0.3)
+    sol = O.Solver(p)
+    sol.startup(st)
+    ub = p.ubulk
+    for _ in range(5):
+        sol.step(st)
+        assert abs(O.bulk(st["u0"]) - ub) < 1e-12 * abs(ub)
+    # laminar Poiseuille under its own pressure gradient: no correction
+    q = params(16, 32, 16, cmf=True)
+    lam = {k: np.zeros((q.nxh, q.nz, q.nyp), dtype=np.complex128) for k in ("v", "omy")}
+    lam["u0"] = seeded(q, 1, amp=0.0)["u0"]
+    lam["w0"] = np.zeros(q.nyp)
+    dp0 = q.dPdx
+    s2 = O.Solver(q)
+    s2.startup(lam)
+    for _ in range(3):
+        s2.step(lam)
+    assert abs(q.dPdx - dp0) < 1e-10 * abs(dp0)  # rounding of bulk(u0) amplified by 1/bulk(phi) ~ 1/dt
+
+
+# ------------------------------------------------------------------ swirl
+def test_cardano_eigenvalues_match_eigvals():
+    rng = np.random.default_rng(8)
+    n = 2000
+    M = rng.standard_normal((n, 3, 3))
+    g = [[M[:, i, k] for k in range(3)] for i in range(3)]
+    lams = np.stack(O.eig_cardano(g), axis=1)
+    ref = np.linalg.eigvals(M)
+    for j in range(n):
+        # every computed root has a reference root next to it, and vice versa
+        d = np.abs(lams[j][:, None] - ref[j][None, :])
+        tol = 1e-8 * max(1.0, np.abs(ref[j]).max())
+        assert d.min(axis=1).max() < tol and d.min(axis=0).max() < tol, j
+    # swirl = sqrt(2) |Im| of the complex pair, 0 for three real roots
+    sw = O.calcswirl(g)
+    im = np.abs(ref.imag).max(axis=1)
+    cplx = im > 1e-6
+    assert np.allclose(sw[cplx], math.sqrt(2.0) * im[cplx], rtol=1e-8)
+    assert np.all(sw[~cplx] < 1e-6)
+    # a pure rotation about z: eigenvalues 0, +-i w -> swirl sqrt(2) w
+    w = 0.7
+    rot = [[np.array([0.0]), np.array([-w]), np.array([0.0])], [np.array([w]), np.array([0.0]), np.array([0.0])],
+           [np.array([0.0]), np.array([0.0]), np.array([0.0])]]
+    assert abs(O.calcswirl(rot)[0] - math.sqrt(2.0) * w) < 1e-12
