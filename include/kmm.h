@@ -78,6 +78,8 @@ typedef struct kmm_config {
    * (ubulk <= 0: the bulk velocity of the state at kmm_startup) */
   int cmf;
   double ubulk;
+  int particle_swirl;     /* 1: swirl strength at the particles every step (swirl_part,
+                             solvers.f90:3479-3481); kmm_config_read sets 1 */
 } kmm_config;
 
 typedef struct kmm_ctx kmm_ctx;
@@ -137,6 +139,20 @@ int kmm_gradv(kmm_ctx* ctx, const double* u, const double* v, const double* w, d
 int kmm_calcswirl(kmm_ctx* ctx, const double* u11, const double* u21, const double* u31, const double* u12,
                   const double* u22, const double* u32, const double* u13, const double* u23, const double* u33,
                   double* swirl);
+
+/* Ensemble of independent channels (any grid sizes, one context each, same
+ * device): advances every member by nsteps.  Each round queues one chunk of
+ * steps (a graph replay where possible) on every member's stream before it
+ * waits for any of them, so the members run concurrently.  Returns the
+ * first error; its message names the member.  No reference counterpart. */
+int kmm_ensemble_step(kmm_ctx** ctxs, int n, int nsteps);
+
+/* Capture `steps` consecutive time steps into a HIP graph and replay it
+ * whenever kmm_step runs steps whose launches are all alike (NL terms
+ * valid, polymer and IB time-lines past their switch points, tracers
+ * tracking); the rest run as ordinary launches.  0 turns it off.  The CFL
+ * buffer is cleared outside the graph.  No reference counterpart. */
+int kmm_set_graph(kmm_ctx* ctx, int steps);
 
 /* Mean pressure gradient dPdx (init_flow.f90:117; updated every step when
  * cmf is set) and its override. */
@@ -211,8 +227,11 @@ int kmm_set_particles(kmm_ctx* ctx, int n, const double* x, const double* y, con
 int kmm_particle_count(kmm_ctx* ctx, int* n);
 /* Current xpart..wpart (any pointer may be NULL). */
 int kmm_get_particles(kmm_ctx* ctx, int n, double* x, double* y, double* z, double* u, double* v, double* w);
+/* swirl_part: swirl strength at the particles (particle_swirl = 1) */
+int kmm_get_particle_swirl(kmm_ctx* ctx, int n, double* swirl);
 /* part-t-XXXXXX.dat writer of part_track (part_track.f90:196-206): Tecplot
- * point format, columns x y z u v w swirl in E14.6 (swirl is written as 0). */
+ * point format, columns x y z u v w swirl in E14.6 (swirl 0 when
+ * particle_swirl is off). */
 int kmm_write_particles(kmm_ctx* ctx, const char* path, int it);
 /* fluid_interp / fluid_interp1 (part_track.f90:437-613) + interpolate3
  * (:1184-1222): trilinear interpolation of nf physical fields given in

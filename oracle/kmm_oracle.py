@@ -578,6 +578,14 @@ def calcswirl(g):
     return np.where(bad, 0.0, sw)
 
 
+def swirl_eig(g):
+    """sqrt(2) |Im| of the complex eigenvalue pair of g[i][k] from LAPACK
+    (numpy.linalg.eigvals): the swirl the closed form of calcswirl computes,
+    without its cancellation near repeated roots."""
+    M = np.stack([np.stack([np.asarray(g[i][k]) for k in range(3)], axis=-1) for i in range(3)], axis=-2)
+    return math.sqrt(2.0) * np.abs(np.linalg.eigvals(M).imag).max(axis=-1)
+
+
 def smoothing_factors(p: Params):
     """main_dns.f90:710-745 -- exp(-3 (|kz|/nzh)^10) * exp(-3 (n/ny)^10)."""
     j = np.arange(p.nz)
@@ -933,6 +941,8 @@ class Solver:
         self.part_state = 0
         self.ap = 0.0
         self.substep = 1
+        self.part_swirl = True
+        self.swirl_fn = calcswirl
 
     def set_particles(self, x, y, z, ap=0.0, cd_switch=0, ratio=0.0):
         """Tracers initialised at the next evaluation (init_part)."""
@@ -1005,6 +1015,14 @@ class Solver:
             else:
                 self.particles = tracer_update(self.particles, uvw, p, self.ap, self.substep)
             self.part_state = 2
+            if self.part_swirl:
+                # swirl_part = swirl_3d at the particles (solvers.f90:3479-3481)
+                gv = gradv(u, v, w, p)
+                gp = [[to_physical(gv[f"u{i}{k}"], p) for k in (1, 2, 3)] for i in (1, 2, 3)]
+                sw = self.swirl_fn(gp)
+                pt = self.particles
+                (pt["swirl"],) = fluid_interp(pt["x"], pt["y"], pt["z"], [sw], p)
+                self.last_swirl = sw
         self.last_fields = fields
         if st["cflmax"] > 1.0:
             raise CFLError(f"CFL failure at time step {it}: {st['cflmax']}")

@@ -85,14 +85,20 @@ struct kmm_ctx {
   // Lagrangian tracers
   int npart = 0;
   int part_cap = 0;
-  double* part = nullptr;    // [6][part_cap]: x, y, z, u, v, w
-  double* cap = nullptr;     // [3][nyp][mz][mx] physical u, v, w of the last evaluation
+  double* part = nullptr;    // [7][part_cap]: x, y, z, u, v, w, swirl
+  double* cap = nullptr;     // [4][nyp][mz][mx] physical u, v, w (and swirl) of the last evaluation
+  bool part_swirl = false;   // swirl strength at the particles (solvers.f90:3479-3481)
   double* ycoord_d = nullptr;
   int* perr = nullptr;
   int part_state = 0;        // 0 none, 1 initialise at the next evaluation, 2 tracking
   int substep = 1;
   bool cap_valid = false;
   double* dpdx_d = nullptr;  // mean pressure gradient (device; updated under constant mass flux)
+  // captured steady steps (kmm_set_graph)
+  int graph_steps = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  long long graph_sig[4] = {0, 0, 0, 0};
   double ubulk = 0.0;        // bulk-velocity target of the constant-mass-flux correction
   // FENE-P conformation tensor: independent components 11 12 13 22 23 33
   double2 *C[6] = {}, *CN[6] = {}, *CNM1[6] = {}, *STRN[6] = {}, *STRNM1[6] = {}, *TF[3] = {};
@@ -104,6 +110,8 @@ struct kmm_ctx {
   struct PolyFlags { bool init = false, solve = false, evaluate = false, force = false, copy = false; };
   PolyFlags pprev;      // flags of the evaluation whose results sit in NLP
 };
+
+static void drop_graph(kmm_ctx* c);
 
 static int dalloc(kmm_ctx* c, void** p, size_t bytes) {
   KMM_HIP(hipMalloc(p, bytes > 0 ? bytes : 16));
@@ -184,6 +192,7 @@ extern "C" int kmm_config_read(const char* path, kmm_config* cfg) {
   if (cfg->src_stop < cfg->src_start) cfg->src_stop = cfg->nsteps;  // main_dns.f90:1071-1075
   cfg->scalar = 0;
   cfg->polymer = 0;
+  cfg->particle_swirl = 1;  // part_track's caller always interpolates swirl_3d
   return KMM_OK;
 }
 
@@ -441,6 +450,8 @@ extern "C" int kmm_destroy(kmm_ctx* c) {
   if (!c) return KMM_OK;
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->graph) hipGraphDestroy(c->graph);
   for (void* p : c->allocs) hipFree(p);
   if (c->part) hipFree(c->part);
   if (c->cap) hipFree(c->cap);
@@ -460,6 +471,7 @@ extern "C" int kmm_destroy(kmm_ctx* c) {
 // contribution survives only at linear addresses not after its source
 // (quirk Q6); z offsets are not wrapped (quirk Q5).
 extern "C" int kmm_set_geometry(kmm_ctx* c, const int32_t* im) {
+  if (c) drop_graph(c);  // the IB tables are reallocated
   if (!c || !im) return fail(KMM_E_ARG, "null argument");
   const int nyp = c->nyp, mz = c->mz, mx = c->mx;
   auto at = [&](int i, int j, int k) { return im[(size_t)i + (size_t)nyp * ((size_t)j + (size_t)mz * k)]; };
@@ -770,10 +782,25 @@ static int run_plane(kmm_ctx* c, int it, unsigned long long* cfl, hipEvent_t* ev
     int r = run_poly_plane(c);
     if (r) return r;
   }
+  if (trk && c->part_swirl) {
+    // swirl_3d (solvers.f90:3147-3150 via calcswirl) for swirl_part
+    kmm::PolyPlaneArgs sa;
+    std::memset(&sa, 0, sizeof(sa));
+    sa.nxh = c->nxh; sa.nz = c->nz; sa.nzh = c->nzh; sa.nyp = c->nyp; sa.mx = c->mx; sa.mz = c->mz;
+    sa.mixed = c->cfg.precision == 1;
+    sa.alpha = c->alpha; sa.beta = c->beta;
+    sa.twZ = c->twZ; sa.twX = c->twX;
+    sa.U = c->Z;
+    for (int q = 0; q < 3; ++q) { sa.in[q] = c->T[q]; sa.in[3 + q] = c->TPD[q]; }
+    sa.ZP = c->ZP;
+    sa.swirl = c->cap + (size_t)3 * c->nyp * c->mz * c->mx;
+    if ((e = kmm::launch_swirl_plane(sa, !poly, c->stream)) != hipSuccess)
+      return fail(KMM_E_HIP, std::string("swirl: ") + hipGetErrorString(e));
+  }
   if (ev) hipEventRecord(ev[4], c->stream);
   if (trk) {
     kmm::TracerArgs t = tracer_args(c);
-    t.n = c->npart; t.nf = 3;
+    t.n = c->npart; t.nf = c->part_swirl ? 4 : 3;
     t.fld = c->cap;
     t.fstride = (long long)c->nyp * c->mz * c->mx;
     t.sy = (long long)c->mz * c->mx; t.sz = c->mx; t.sx = 1;
@@ -857,8 +884,9 @@ extern "C" int kmm_startup(kmm_ctx* c) {
     kmm::PolyArgs pa = poly_args(c);
     pa.evaluate = 1;
     if ((r = run_poly_pencil(c, 1, pa))) return r;
-    for (int q = 0; q < 3; ++q) a.TPD[q] = c->TPD[q];
   }
+  if (pf.evaluate || (c->npart > 0 && c->part_swirl))
+    for (int q = 0; q < 3; ++q) a.TPD[q] = c->TPD[q];
   hipError_t e = kmm::launch_pencil_startup(c->ny, a, c->stream);
   if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
   for (int q = 0; q < 3; ++q) a.TPD[q] = nullptr;
@@ -900,79 +928,211 @@ extern "C" int kmm_startup(kmm_ctx* c) {
   return r ? r : check_particles(c);
 }
 
-extern "C" int kmm_step(kmm_ctx* c, int nsteps) {
-  if (!c || nsteps < 0) return fail(KMM_E_ARG, "bad argument");
+// Launch one time step on the library stream (no synchronisation).  s is the
+// step's slot in the CFL buffer; ev (or nullptr) receives per-kernel events;
+// stage_ev records the first-step stage events of the untimed path.
+static int enqueue_step(kmm_ctx* c, int s, hipEvent_t* ev, bool stage_ev) {
+  c->it += 1;
+  PencilArgs a = pencil_args(c);
+  a.first_nl = c->nl_valid ? 1 : 0;
+  if (ev) hipEventRecord(ev[0], c->stream);
+  else if (stage_ev) hipEventRecord(c->ev[0], c->stream);
+  const kmm_ctx::PolyFlags pf = poly_flags(c, c->it);
+  if (c->cfg.polymer) {
+    kmm::PolyArgs pa = poly_args(c);
+    pa.partB = c->nl_valid && c->pprev.evaluate;
+    pa.force = pa.partB && c->pprev.force;
+    pa.copy = pa.partB && c->pprev.copy;
+    pa.smooth = 1;
+    pa.init = pf.init; pa.solve = pf.solve; pa.evaluate = pf.evaluate;
+    int r = run_poly_pencil(c, 0, pa);
+    if (r) return r;
+    if (pa.force) for (int q = 0; q < 3; ++q) a.TF[q] = c->TF[q];
+  }
+  if (pf.evaluate || (c->npart > 0 && c->part_swirl)) for (int q = 0; q < 3; ++q) a.TPD[q] = c->TPD[q];
+  if (ev) hipEventRecord(ev[1], c->stream);
+  hipError_t e = kmm::launch_pencil_step(c->ny, a, c->stream);
+  if (e != hipSuccess) return fail(KMM_E_HIP, std::string("pencil step: ") + hipGetErrorString(e));
+  if (ev) hipEventRecord(ev[2], c->stream);
+  else if (stage_ev) hipEventRecord(c->ev[1], c->stream);
+  int r = run_plane(c, c->it, c->cfl_d + s, ev ? ev + 3 : nullptr, true, pf.evaluate);
+  if (r) return r;
+  if (!ev && stage_ev) hipEventRecord(c->ev[2], c->stream);
+  c->pprev = pf;
+  c->nl_valid = true;
+  return KMM_OK;
+}
+
+// Steps it+1 .. it+k launch the same kernels with the same arguments (apart
+// from the IB activation step, constant here) -- the condition for replaying
+// a captured graph.
+static bool steady(const kmm_ctx* c, int k) {
+  if (!c->nl_valid) return false;
+  if (c->npart > 0 && c->part_state != 2) return false;
+  if (c->nsrc > 0 && c->it < c->cfg.perturbtime + 1) return false;
+  if (c->cfg.polymer) {
+    for (int j = 0; j <= k; ++j) {
+      const kmm_ctx::PolyFlags f = poly_flags(c, c->it + j);
+      const kmm_ctx::PolyFlags& g = c->pprev;
+      if (f.init != g.init || f.solve != g.solve || f.evaluate != g.evaluate || f.force != g.force || f.copy != g.copy)
+        return false;
+    }
+  }
+  return true;
+}
+
+static void drop_graph(kmm_ctx* c) {
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+}
+
+extern "C" int kmm_set_graph(kmm_ctx* c, int steps) {
+  if (!c || steps < 0 || steps > 4096) return fail(KMM_E_ARG, "steps per graph must be in 0..4096");
   KMM_HIP(hipSetDevice(c->device));
-  int done = 0;
+  drop_graph(c);
+  c->graph_steps = steps;
+  return KMM_OK;
+}
+
+// One chunk of kmm_step: launches up to `remaining` steps (a graph replay
+// when the steps are steady) without waiting; step_chunk_finish waits and
+// checks CFL and particles.
+struct StepChunk {
+  int n = 0, it0 = 0;
+  bool graph = false;
+};
+
+static int step_chunk_enqueue(kmm_ctx* c, int remaining, StepChunk& ch) {
+  const int K = c->graph_steps;
+  ch.graph = K > 0 && !c->timing && remaining >= K && steady(c, K);
+  ch.n = ch.graph ? K : std::min(remaining, c->cfl_cap);
+  ch.it0 = c->it + 1;
+  const int n = ch.n;
+  if (c->timing) {
+    const size_t need = (size_t)n * 9;
+    while (c->tev.size() < need) {
+      hipEvent_t e;
+      KMM_HIP(hipEventCreate(&e));
+      c->tev.push_back(e);
+    }
+  }
+  KMM_HIP(hipMemsetAsync(c->cfl_d, 0, n * sizeof(unsigned long long), c->stream));
+  if (ch.graph) {
+    // kernel arguments and buffers of a steady step are the same every time;
+    // the signature guards what could change between calls
+    const long long sig[4] = {(long long)c->npart, (long long)(uintptr_t)c->part, (long long)c->nsrc,
+                              (long long)c->part_swirl};
+    if (c->gexec && std::memcmp(sig, c->graph_sig, sizeof(sig)) != 0) drop_graph(c);
+    if (!c->gexec) {
+      KMM_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int r = 0;
+      const int itsave = c->it;
+      for (int s = 0; s < n && !r; ++s) r = enqueue_step(c, s, nullptr, false);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (r) {
+        if (g) hipGraphDestroy(g);
+        return r;
+      }
+      if (e != hipSuccess) return fail(KMM_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+      c->graph = g;
+      KMM_HIP(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+      std::memcpy(c->graph_sig, sig, sizeof(sig));
+      c->it = itsave;  // the capture recorded the steps; the replay below runs them
+    }
+    KMM_HIP(hipGraphLaunch(c->gexec, c->stream));
+    c->it += n;
+  } else {
+    for (int s = 0; s < n; ++s) {
+      hipEvent_t* ev = c->timing ? &c->tev[(size_t)s * 9] : nullptr;
+      int r = enqueue_step(c, s, ev, s == 0);
+      if (r) return r;
+    }
+  }
+  return KMM_OK;
+}
+
+static int step_chunk_finish(kmm_ctx* c, const StepChunk& ch) {
+  const int n = ch.n;
+  int r = check_cfl(c, n, ch.it0);
+  if (c->timing) {
+    // intervals: poly pencil, pencil, zinv, ibdrag, xplane, zfwd, poly plane, tracers
+    static const int slot[8] = {5, 0, 1, 2, 3, 4, 6, 7};
+    for (int s = 0; s < n; ++s) {
+      hipEvent_t* ev = &c->tev[(size_t)s * 9];
+      for (int k = 0; k < 8; ++k) {
+        float t = 0.0f;
+        hipEventElapsedTime(&t, ev[k], ev[k + 1]);
+        c->ktime[slot[k]] += t;
+      }
+    }
+    c->ksamples += n;
+  } else if (!ch.graph) {
+    float t1 = 0, t2 = 0;
+    hipEventElapsedTime(&t1, c->ev[0], c->ev[1]);
+    hipEventElapsedTime(&t2, c->ev[1], c->ev[2]);
+    c->t_pencil = t1;
+    c->t_plane = t2;
+  }
+  if (r) return r;
+  return check_particles(c);
+}
+
+static void reset_step_timers(kmm_ctx* c) {
   c->t_pencil = c->t_plane = 0.0;
   for (double& t : c->ktime) t = 0.0;
   c->ksamples = 0;
+}
+
+extern "C" int kmm_step(kmm_ctx* c, int nsteps) {
+  if (!c || nsteps < 0) return fail(KMM_E_ARG, "bad argument");
+  KMM_HIP(hipSetDevice(c->device));
+  reset_step_timers(c);
+  int done = 0;
   while (done < nsteps) {
-    const int n = std::min(nsteps - done, c->cfl_cap);
-    const int it0 = c->it + 1;
-    if (c->timing) {
-      const size_t need = (size_t)n * 9;
-      while (c->tev.size() < need) {
-        hipEvent_t e;
-        KMM_HIP(hipEventCreate(&e));
-        c->tev.push_back(e);
-      }
-    }
-    KMM_HIP(hipMemsetAsync(c->cfl_d, 0, n * sizeof(unsigned long long), c->stream));
-    for (int s = 0; s < n; ++s) {
-      c->it += 1;
-      PencilArgs a = pencil_args(c);
-      a.first_nl = c->nl_valid ? 1 : 0;
-      hipEvent_t* ev = c->timing ? &c->tev[(size_t)s * 9] : nullptr;
-      if (ev) hipEventRecord(ev[0], c->stream);
-      else if (s == 0) hipEventRecord(c->ev[0], c->stream);
-      const kmm_ctx::PolyFlags pf = poly_flags(c, c->it);
-      if (c->cfg.polymer) {
-        kmm::PolyArgs pa = poly_args(c);
-        pa.partB = c->nl_valid && c->pprev.evaluate;
-        pa.force = pa.partB && c->pprev.force;
-        pa.copy = pa.partB && c->pprev.copy;
-        pa.smooth = 1;
-        pa.init = pf.init; pa.solve = pf.solve; pa.evaluate = pf.evaluate;
-        int r = run_poly_pencil(c, 0, pa);
-        if (r) return r;
-        if (pa.force) for (int q = 0; q < 3; ++q) a.TF[q] = c->TF[q];
-        if (pf.evaluate) for (int q = 0; q < 3; ++q) a.TPD[q] = c->TPD[q];
-      }
-      if (ev) hipEventRecord(ev[1], c->stream);
-      hipError_t e = kmm::launch_pencil_step(c->ny, a, c->stream);
-      if (e != hipSuccess) return fail(KMM_E_HIP, std::string("pencil step: ") + hipGetErrorString(e));
-      if (ev) hipEventRecord(ev[2], c->stream);
-      else if (s == 0) hipEventRecord(c->ev[1], c->stream);
-      int r = run_plane(c, c->it, c->cfl_d + s, ev ? ev + 3 : nullptr, true, pf.evaluate);
-      if (r) return r;
-      if (!ev && s == 0) hipEventRecord(c->ev[2], c->stream);
-      c->pprev = pf;
-      c->nl_valid = true;
-    }
-    int r = check_cfl(c, n, it0);
-    if (c->timing) {
-      // intervals: poly pencil, pencil, zinv, ibdrag, xplane, zfwd, poly plane, tracers
-      static const int slot[8] = {5, 0, 1, 2, 3, 4, 6, 7};
-      for (int s = 0; s < n; ++s) {
-        hipEvent_t* ev = &c->tev[(size_t)s * 9];
-        for (int k = 0; k < 8; ++k) {
-          float t = 0.0f;
-          hipEventElapsedTime(&t, ev[k], ev[k + 1]);
-          c->ktime[slot[k]] += t;
-        }
-      }
-      c->ksamples += n;
-    } else {
-      float t1 = 0, t2 = 0;
-      hipEventElapsedTime(&t1, c->ev[0], c->ev[1]);
-      hipEventElapsedTime(&t2, c->ev[1], c->ev[2]);
-      c->t_pencil = t1;
-      c->t_plane = t2;
-    }
+    StepChunk ch;
+    int r = step_chunk_enqueue(c, nsteps - done, ch);
+    if (!r) r = step_chunk_finish(c, ch);
     if (r) return r;
-    if ((r = check_particles(c))) return r;
-    done += n;
+    done += ch.n;
+  }
+  return KMM_OK;
+}
+
+extern "C" int kmm_ensemble_step(kmm_ctx** ctxs, int n, int nsteps) {
+  if (!ctxs || n <= 0 || nsteps < 0) return fail(KMM_E_ARG, "bad argument");
+  for (int m = 0; m < n; ++m) {
+    if (!ctxs[m]) return fail(KMM_E_ARG, "null member");
+    if (m && ctxs[m]->device != ctxs[0]->device) return fail(KMM_E_ARG, "ensemble members must share a device");
+    reset_step_timers(ctxs[m]);
+  }
+  KMM_HIP(hipSetDevice(ctxs[0]->device));
+  std::vector<int> done(n, 0);
+  std::vector<StepChunk> ch(n);
+  for (;;) {
+    bool any = false;
+    // queue one chunk on every member's stream, then wait for them in turn
+    for (int m = 0; m < n; ++m) {
+      ch[m].n = 0;
+      if (done[m] >= nsteps) continue;
+      any = true;
+      int r = step_chunk_enqueue(ctxs[m], nsteps - done[m], ch[m]);
+      if (r) return r;
+    }
+    if (!any) break;
+    for (int m = 0; m < n; ++m) {
+      if (!ch[m].n) continue;
+      int r = step_chunk_finish(ctxs[m], ch[m]);
+      if (r) {
+        char msg[64];
+        std::snprintf(msg, sizeof msg, " (ensemble member %d)", m);
+        g_last_error += msg;
+        return r;
+      }
+      done[m] += ch[m].n;
+    }
   }
   return KMM_OK;
 }
@@ -1308,6 +1468,7 @@ extern "C" int kmm_set_particles(kmm_ctx* c, int n, const double* x, const doubl
   if (!c || n < 0 || (n > 0 && (!x || !y || !z))) return fail(KMM_E_ARG, "bad argument");
   KMM_HIP(hipSetDevice(c->device));
   KMM_HIP(hipStreamSynchronize(c->stream));
+  drop_graph(c);  // particle buffers may be reallocated
   if (n == 0) {
     c->npart = 0;
     c->part_state = 0;
@@ -1327,15 +1488,25 @@ extern "C" int kmm_set_particles(kmm_ctx* c, int n, const double* x, const doubl
   c->substep = (int)std::ceil(std::max(1.0, 2.0 * g.dt / tau));
   if (n > c->part_cap) {
     if (c->part) { hipFree(c->part); c->part = nullptr; }
-    KMM_HIP(hipMalloc((void**)&c->part, (size_t)6 * n * sizeof(double)));
+    KMM_HIP(hipMalloc((void**)&c->part, (size_t)7 * n * sizeof(double)));
     c->part_cap = n;
   }
-  if (!c->cap) KMM_HIP(hipMalloc((void**)&c->cap, (size_t)3 * c->nyp * c->mz * c->mx * sizeof(double)));
+  if (!c->cap) KMM_HIP(hipMalloc((void**)&c->cap, (size_t)4 * c->nyp * c->mz * c->mx * sizeof(double)));
+  c->part_swirl = g.particle_swirl != 0;
+  if (c->part_swirl && !c->ZP) {  // gradient intermediates (the polymer stage allocates them otherwise)
+    const size_t tbv = c->S * c->vsz, zb = (size_t)c->nyp * c->mz * c->nxh * c->vsz;
+    for (int q = 0; q < 3; ++q) {
+      KMM_HIP(hipMalloc(&c->TPD[q], tbv));
+      c->allocs.push_back(c->TPD[q]);
+    }
+    KMM_HIP(hipMalloc(&c->ZP, 6 * zb));
+    c->allocs.push_back(c->ZP);
+  }
   const size_t b = (size_t)n * sizeof(double);
   KMM_HIP(hipMemcpyAsync(c->part, x, b, hipMemcpyHostToDevice, c->stream));
   KMM_HIP(hipMemcpyAsync(c->part + n, y, b, hipMemcpyHostToDevice, c->stream));
   KMM_HIP(hipMemcpyAsync(c->part + 2 * (size_t)n, z, b, hipMemcpyHostToDevice, c->stream));
-  KMM_HIP(hipMemsetAsync(c->part + 3 * (size_t)n, 0, 3 * b, c->stream));
+  KMM_HIP(hipMemsetAsync(c->part + 3 * (size_t)n, 0, 4 * b, c->stream));
   KMM_HIP(hipMemsetAsync(c->perr, 0, sizeof(int), c->stream));
   KMM_HIP(hipStreamSynchronize(c->stream));
   c->npart = n;
@@ -1397,6 +1568,18 @@ extern "C" int kmm_get_particles(kmm_ctx* c, int n, double* x, double* y, double
   return KMM_OK;
 }
 
+extern "C" int kmm_get_particle_swirl(kmm_ctx* c, int n, double* swirl) {
+  if (!c || !swirl) return fail(KMM_E_ARG, "null argument");
+  if (n != c->npart) return fail(KMM_E_ARG, "n differs from the number of particles set");
+  if (!c->part_swirl) return fail(KMM_E_CONFIG, "particle swirl is off (particle_swirl = 0)");
+  if (n == 0) return KMM_OK;
+  KMM_HIP(hipSetDevice(c->device));
+  KMM_HIP(hipMemcpyAsync(swirl, c->part + (size_t)6 * n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                         c->stream));
+  KMM_HIP(hipStreamSynchronize(c->stream));
+  return KMM_OK;
+}
+
 // Fortran Ew.d edit descriptor with w = 14, d = 6: [-]0.ddddddE+ee
 static std::string fortran_e14_6(double v) {
   char out[32];
@@ -1422,10 +1605,11 @@ static std::string fortran_e14_6(double v) {
 extern "C" int kmm_write_particles(kmm_ctx* c, const char* path, int it) {
   if (!c || !path) return fail(KMM_E_ARG, "null argument");
   const int n = c->npart;
-  std::vector<double> a((size_t)6 * n);
+  std::vector<double> a((size_t)7 * n, 0.0);
   if (n) {
     int r = kmm_get_particles(c, n, a.data(), a.data() + n, a.data() + 2 * (size_t)n, a.data() + 3 * (size_t)n,
                               a.data() + 4 * (size_t)n, a.data() + 5 * (size_t)n);
+    if (!r && c->part_swirl) r = kmm_get_particle_swirl(c, n, a.data() + 6 * (size_t)n);
     if (r) return r;
   }
   FILE* f = std::fopen(path, "w");
@@ -1435,7 +1619,7 @@ extern "C" int kmm_write_particles(kmm_ctx* c, const char* path, int it) {
   for (int j = 0; j < n; ++j) {
     std::string line;
     for (int k = 0; k < 6; ++k) line += fortran_e14_6(a[(size_t)k * n + j]) + " ";
-    line += fortran_e14_6(0.0) + " ";
+    line += fortran_e14_6(a[(size_t)6 * n + j]) + " ";
     std::fprintf(f, "%s\n", line.c_str());
   }
   const bool ok = std::fclose(f) == 0;

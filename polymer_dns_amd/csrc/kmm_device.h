@@ -384,4 +384,34 @@ __device__ V* wave_fft(V* x, V* y, const V* __restrict__ tw, int lane) {
   return x;
 }
 
+// Swirl strength (calcswirl + eig, helpers.f90:197-319, 360-404): the
+// reference evaluates the three eigenvalues of the velocity-gradient tensor
+// with Cardano's closed form in complex arithmetic and returns
+// sqrt(sum Im(lambda)^2), 0 where that is 0 or NaN.  For a real 3x3 matrix
+// with one real root and a complex pair that is sqrt(2) |Im| of the pair.
+// Here: characteristic polynomial t^3 + p t + q of the deviatoric part,
+// discriminant D = (q/2)^2 + (p/3)^3; D > 0: Im of the pair is
+// (sqrt(3)/2) |u - v| with u = cbrt(-q/2 - sign(q) sqrt(D)) and
+// v = -p/(3u) (no cancellation, unlike the reference's complex form, whose
+// S = (B + sqrt(B^2 - A^3))^(1/3) loses digits when B < 0), so
+// swirl = sqrt(3/2) |u - v|; D <= 0 (three real roots): 0.
+__device__ __forceinline__ double swirl_of(const double (&a)[9]) {
+  const double tr = a[0] + a[4] + a[8];
+  // sum of principal 2x2 minors and determinant
+  const double m2 = a[0] * a[4] - a[1] * a[3] + a[0] * a[8] - a[2] * a[6] + a[4] * a[8] - a[5] * a[7];
+  const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                     a[2] * (a[3] * a[7] - a[4] * a[6]);
+  // lambda^3 - tr lambda^2 + m2 lambda - det = 0, lambda = t + tr/3
+  const double pp = m2 - tr * tr / 3.0;
+  const double qq = -2.0 * tr * tr * tr / 27.0 + tr * m2 / 3.0 - det;
+  const double disc = 0.25 * qq * qq + pp * pp * pp / 27.0;
+  if (!(disc > 0.0)) return 0.0;
+  // the cube root whose argument does not cancel, then v = -p/(3u)
+  const double sd = sqrt(disc);
+  const double uu = cbrt(-0.5 * qq + (qq > 0.0 ? -sd : sd));
+  const double vv = -pp / (3.0 * uu);
+  const double sw = 1.2247448713915890491 * fabs(uu - vv);  // sqrt(3/2)
+  return (sw > 0.0 && isfinite(sw)) ? sw : 0.0;
+}
+
 }  // namespace kmm

@@ -119,8 +119,13 @@ struct PolyPlaneArgs {
   void* ZP;               // [24]: dz u(3), dy u(3), C(6), dz C(6), dy C(6)
   void* ZFP;              // [12]: c_n(6), str_n(6) after the forward x transform
   void* out[12];          // NLP outputs [y][kx][kz]
+  double* swirl;          // [nyp][mz][mx] swirl strength (launch_swirl_plane)
 };
 hipError_t launch_poly_plane(const PolyPlaneArgs& a, hipStream_t s);
+// swirl strength on the 3/2 grid from the velocity gradient; with zstage the
+// z transforms of dz u (from in[0..2]) and dy u (in[3..5]) into ZP[0..5] run
+// first (they are already there when the polymer stage ran)
+hipError_t launch_swirl_plane(const PolyPlaneArgs& a, bool zstage, hipStream_t s);
 
 // Lagrangian tracers (part_track.f90): particle j has x, y, z, u, v, w at
 // part[0..5][j]; fields are read at fld[f*fstride + k*sy + jz*sz + ix*sx]

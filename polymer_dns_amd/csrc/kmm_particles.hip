@@ -18,6 +18,7 @@
 // of bounds or stop, part_track.f90:582-590) raises *err; its indices are
 // clamped so every access stays inside the field.
 #include "kmm_kernels.h"
+#include "kmm_device.h"
 
 namespace kmm {
 
@@ -104,32 +105,6 @@ __global__ __launch_bounds__(256) void k_tracer(TracerArgs A) {
   }
 }
 
-// Swirl strength (calcswirl + eig, helpers.f90:197-319, 360-404): the
-// reference evaluates the three eigenvalues of the velocity-gradient tensor
-// with Cardano's closed form in complex arithmetic and returns
-// sqrt(sum Im(lambda)^2), 0 where that is 0 or NaN.  For a real 3x3 matrix
-// with one real root and a complex pair that is sqrt(2) |Im| of the pair.
-// Here: characteristic polynomial t^3 + p t + q of the deviatoric part,
-// discriminant D = (q/2)^2 + (p/3)^3; D > 0: Im of the pair is
-// (sqrt(3)/2) |u - v| with u, v = cbrt(-q/2 +- sqrt(D)), so
-// swirl = sqrt(3/2) |u - v|; D <= 0 (three real roots): 0.
-__device__ __forceinline__ double swirl_of(const double (&a)[9]) {
-  const double tr = a[0] + a[4] + a[8];
-  // sum of principal 2x2 minors and determinant
-  const double m2 = a[0] * a[4] - a[1] * a[3] + a[0] * a[8] - a[2] * a[6] + a[4] * a[8] - a[5] * a[7];
-  const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
-                     a[2] * (a[3] * a[7] - a[4] * a[6]);
-  // lambda^3 - tr lambda^2 + m2 lambda - det = 0, lambda = t + tr/3
-  const double pp = m2 - tr * tr / 3.0;
-  const double qq = -2.0 * tr * tr * tr / 27.0 + tr * m2 / 3.0 - det;
-  const double disc = 0.25 * qq * qq + pp * pp * pp / 27.0;
-  if (!(disc > 0.0)) return 0.0;
-  const double sd = sqrt(disc);
-  const double uu = cbrt(-0.5 * qq + sd), vv = cbrt(-0.5 * qq - sd);
-  const double sw = 1.2247448713915890491 * fabs(uu - vv);  // sqrt(3/2)
-  return (sw > 0.0 && isfinite(sw)) ? sw : 0.0;
-}
-
 __global__ __launch_bounds__(256) void k_swirl(int n, const double* __restrict__ g, double* __restrict__ out) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
@@ -147,7 +122,7 @@ hipError_t launch_swirl(int n, const double* g, double* swirl, hipStream_t s) {
 
 hipError_t launch_tracer(const TracerArgs& a, hipStream_t s) {
   if (a.n <= 0) return hipSuccess;
-  if (a.part && a.nf != 3) return hipErrorInvalidValue;
+  if (a.part && a.nf != 3 && a.nf != 4) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_tracer, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
   return hipGetLastError();
 }

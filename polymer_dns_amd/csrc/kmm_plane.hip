@@ -547,6 +547,73 @@ __global__ __launch_bounds__(256) void k_xplane_poly(PolyPlaneArgs A) {
   R.template fwd_str<4>();
 }
 
+// Swirl strength (calcswirl, helpers.f90:360-404) of the velocity gradient on
+// one row: five pair transforms give the nine components d u_i / d x_k.
+template <int MX, typename V>
+__global__ __launch_bounds__(256) void k_xplane_swirl(PolyPlaneArgs A) {
+  using T = typename Scal<V>::T;
+  constexpr int XPL = (MX + 63) / 64;
+  constexpr int WPB = 4;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  V* sm = reinterpret_cast<V*>(smraw);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  constexpr int BL = FFTBuf<MX>::len;
+  V* tw = sm;
+  V* buf = sm + MX + (size_t)wv * 2 * BL;
+  V* scr = buf + BL;
+  for (int e = threadIdx.x; e < MX; e += WPB * 64) tw[e] = static_cast<const V*>(A.twX)[e];
+  __syncthreads();
+  const int row = blockIdx.x * WPB + wv;  // grid covers nyp * mz rows exactly
+  const int y = row / A.mz, z = row % A.mz;
+  const int nxh = A.nxh;
+  const size_t plane = (size_t)A.nyp * A.mz * nxh;
+  const size_t roff = ((size_t)y * A.mz + z) * nxh;
+  const V* U = static_cast<const V*>(A.U);
+  const V* ZP = static_cast<const V*>(A.ZP);
+  auto urow = [&](int f) { return U + (size_t)f * plane + roff; };
+  auto zrow = [&](int f) { return ZP + (size_t)f * plane + roff; };
+  T g[9][XPL];  // g[3 i + k] = d u_i / d x_k
+  PairIn<MX, V> pin;
+  pair_load<MX, V>(pin, urow(0), urow(1), nxh, lane);
+  pair_inv<MX, V>(pin, true, true, nxh, A.alpha, buf, scr, tw, lane, g[0], g[3]);
+  pair_load<MX, V>(pin, urow(2), zrow(3), nxh, lane);
+  pair_inv<MX, V>(pin, true, false, nxh, A.alpha, buf, scr, tw, lane, g[6], g[1]);
+  pair_load<MX, V>(pin, zrow(4), zrow(5), nxh, lane);
+  pair_inv<MX, V>(pin, false, false, nxh, A.alpha, buf, scr, tw, lane, g[4], g[7]);
+  pair_load<MX, V>(pin, zrow(0), zrow(1), nxh, lane);
+  pair_inv<MX, V>(pin, false, false, nxh, A.alpha, buf, scr, tw, lane, g[2], g[5]);
+  T dummy[XPL];
+  pair_load<MX, V>(pin, zrow(2), zrow(2), nxh, lane);
+  pair_inv<MX, V>(pin, false, false, nxh, A.alpha, buf, scr, tw, lane, g[8], dummy);
+  double* out = A.swirl + ((size_t)y * A.mz + z) * MX;
+#pragma unroll
+  for (int m = 0; m < XPL; ++m) {
+    const int x = lane + 64 * m;
+    double a[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a[k] = (double)g[k][m];
+    if (x < MX) out[x] = swirl_of(a);
+  }
+}
+
+template <int MX, int MZ, typename V>
+static hipError_t swirl_plane_impl(const PolyPlaneArgs& pa, bool zstage, hipStream_t s) {
+  if (zstage) {
+    PlaneArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.nxh = pa.nxh; a.nz = pa.nz; a.nzh = pa.nzh; a.nyp = pa.nyp; a.mx = pa.mx; a.mz = pa.mz;
+    a.twZ = pa.twZ; a.twX = pa.twX; a.beta = pa.beta;
+    for (int f = 0; f < 6; ++f) a.in[f] = pa.in[f];
+    a.dzmask = 0x7u;  // dz u, dz v, dz w
+    a.Z = pa.ZP;
+    const size_t zlds = (size_t)(16 * FFTBuf<MZ>::len + 9 * MZ) * sizeof(V);
+    hipLaunchKernelGGL((k_zinv<MZ, 6, V>), dim3(a.nyp * (a.nxh / 8)), dim3(512), zlds, s, a);
+  }
+  const size_t xlds = (size_t)(MX + 4 * 2 * FFTBuf<MX>::len) * sizeof(V);
+  hipLaunchKernelGGL((k_xplane_swirl<MX, V>), dim3(pa.nyp * pa.mz / 4), dim3(256), xlds, s, pa);
+  return hipGetLastError();
+}
+
 template <int MX, int MZ, typename V>
 static hipError_t poly_plane_impl(const PolyPlaneArgs& pa, hipStream_t s) {
   // z inverse of the 24 polymer inputs (8 per launch, d/dz by mask)
@@ -643,6 +710,23 @@ hipError_t launch_poly_plane(const PolyPlaneArgs& a, hipStream_t s) {
   KMM_POLY_CASE(288, 288)
   KMM_POLY_CASE(384, 192)
 #undef KMM_POLY_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_swirl_plane(const PolyPlaneArgs& a, bool zstage, hipStream_t s) {
+  if (a.nxh % 8 != 0 || (a.nyp * a.mz) % 4 != 0) return hipErrorInvalidValue;
+#define KMM_SWIRL_CASE(X, Zz)                                                                          \
+  if (a.mx == X && a.mz == Zz)                                                                         \
+    return a.mixed ? swirl_plane_impl<X, Zz, float2>(a, zstage, s) : swirl_plane_impl<X, Zz, double2>(a, zstage, s);
+  KMM_SWIRL_CASE(48, 48)
+  KMM_SWIRL_CASE(96, 48)
+  KMM_SWIRL_CASE(96, 96)
+  KMM_SWIRL_CASE(144, 144)
+  KMM_SWIRL_CASE(192, 96)
+  KMM_SWIRL_CASE(192, 192)
+  KMM_SWIRL_CASE(288, 288)
+  KMM_SWIRL_CASE(384, 192)
+#undef KMM_SWIRL_CASE
   return hipErrorInvalidValue;
 }
 

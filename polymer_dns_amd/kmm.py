@@ -42,7 +42,8 @@ EXPORTS = [
     "kmm_set_timing", "kmm_kernel_times",
     "kmm_read_particles", "kmm_set_particles", "kmm_particle_count", "kmm_get_particles", "kmm_write_particles",
     "kmm_fluid_interp", "kmm_get_physical", "kmm_get_dpdx", "kmm_set_dpdx",
-    "kmm_gradv", "kmm_calcswirl",
+    "kmm_gradv", "kmm_calcswirl", "kmm_get_particle_swirl", "kmm_set_graph",
+    "kmm_ensemble_step",
 ]
 
 
@@ -67,7 +68,7 @@ class KmmConfig(C.Structure):
         ("polymer", C.c_int), ("itarget", C.c_int), ("ipeter", C.c_int), ("ipolyflag", C.c_int),
         ("zlmax", C.c_double), ("tpoly", C.c_double), ("alpha_poly", C.c_double), ("diffpoly", C.c_double),
         ("c11z", C.c_double), ("c22z", C.c_double), ("c33z", C.c_double), ("qbeta", C.c_double),
-        ("cmf", C.c_int), ("ubulk", C.c_double),
+        ("cmf", C.c_int), ("ubulk", C.c_double), ("particle_swirl", C.c_int),
     ]
 
 
@@ -125,6 +126,9 @@ def load_library(path: str | None = None):
     lib.kmm_get_physical.argtypes = [C.c_void_p, C.c_int, dp]
     lib.kmm_get_dpdx.argtypes = [C.c_void_p, dp]
     lib.kmm_gradv.argtypes = [C.c_void_p] + [dp] * 15
+    lib.kmm_get_particle_swirl.argtypes = [C.c_void_p, C.c_int, dp]
+    lib.kmm_set_graph.argtypes = [C.c_void_p, C.c_int]
+    lib.kmm_ensemble_step.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int]
     lib.kmm_calcswirl.argtypes = [C.c_void_p] + [dp] * 10
     lib.kmm_set_dpdx.argtypes = [C.c_void_p, C.c_double]
     for n in EXPORTS:
@@ -164,7 +168,7 @@ def make_config(nx, ny, nz, **kw) -> KmmConfig:
                     src_start=0, src_stop=-1, cadence=10, precision=0,
                     particle_flag=0, ratio=0.0, ap=0.0, c_mu=1.0, cd_switch=0,
                     polymer=0, itarget=0, ipeter=1, ipolyflag=1, zlmax=30.0, tpoly=0.1, alpha_poly=0.0,
-                    diffpoly=0.5, c11z=1.0, c22z=1.0, c33z=1.0, qbeta=0.9, cmf=0, ubulk=0.0)
+                    diffpoly=0.5, c11z=1.0, c22z=1.0, c33z=1.0, qbeta=0.9, cmf=0, ubulk=0.0, particle_swirl=1)
     defaults.update(kw)
     cfg.nx, cfg.ny, cfg.nz = nx, ny, nz
     for k, v in defaults.items():
@@ -336,6 +340,9 @@ class Channel:
         out = {k: np.empty(n) for k in ("x", "y", "z", "u", "v", "w")}
         if n:
             _check(self.lib, self.lib.kmm_get_particles(self.h, n, *[_dp(out[k]) for k in "xyzuvw"]))
+            if self.cfg.particle_swirl:
+                out["swirl"] = np.empty(n)
+                _check(self.lib, self.lib.kmm_get_particle_swirl(self.h, n, _dp(out["swirl"])))
         return out
 
     def write_particles(self, path, it):
@@ -386,3 +393,15 @@ class Channel:
         order = [a[0][0], a[1][0], a[2][0], a[0][1], a[1][1], a[2][1], a[0][2], a[1][2], a[2][2]]
         _check(self.lib, self.lib.kmm_calcswirl(self.h, *[_dp(x) for x in order], _dp(out)))
         return out
+
+    def set_graph(self, steps):
+        """Replay steady steps as a captured HIP graph of `steps` steps (0: off)."""
+        _check(self.lib, self.lib.kmm_set_graph(self.h, int(steps)))
+
+
+def ensemble_step(channels, nsteps):
+    """Advance independent channels together (kmm_ensemble_step): each round
+    queues work on every member's stream before waiting for any of them."""
+    lib = load_library()
+    arr = (C.c_void_p * len(channels))(*[ch.h for ch in channels])
+    _check(lib, lib.kmm_ensemble_step(arr, len(channels), int(nsteps)))

@@ -82,6 +82,7 @@ def test_tracers_follow_the_channel_step(precision):
     ch.set_particles(x, y, z)
     ch.startup()
     sol = O.Solver(p)
+    sol.swirl_fn = O.swirl_eig   # the GPU swirl has no cancellation near repeated roots
     sol.set_particles(x, y, z, ap=ap)
     ost = {k: v.copy() for k, v in st.items()}
     sol.startup(ost)
@@ -100,6 +101,9 @@ def test_tracers_follow_the_channel_step(precision):
     # up3d / vp3d / wp3d of the last evaluation
     for c, key in enumerate(("up", "vp", "wp")):
         assert rel(ch.physical(c), sol.last_phys[key]) < tol
+    # swirl_part: swirl_3d (from eigenvalues) interpolated at the particles
+    scale = np.abs(sol.last_swirl).max()
+    assert np.abs(got["swirl"] - sol.particles["swirl"]).max() < (1e-9 if precision == 0 else 1e-3) * scale
     ch.close()
 
 
@@ -155,7 +159,7 @@ def test_particles_dat_read_and_part_file_write(tmp_path):
             mant = s.strip().lstrip("-")
             assert mant.startswith("0.") and mant[8] == "E", s
         vals = np.array([float(s) for s in fields])
-        ref = np.array([got[k][j] for k in "xyzuvw"] + [0.0])
+        ref = np.array([got[k][j] for k in ("x", "y", "z", "u", "v", "w", "swirl")])
         assert np.allclose(vals, ref, rtol=5.1e-6, atol=0)  # 6 significant digits
 
 
