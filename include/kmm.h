@@ -245,6 +245,31 @@ int kmm_fluid_interp(kmm_ctx* ctx, int n, const double* xp, const double* yp, co
  * while particles are set. */
 int kmm_get_physical(kmm_ctx* ctx, int comp, double* out);
 
+/* ---- learned polymer-stress closure (no reference counterpart) ----
+ * A 22 -> 128 -> 128 -> 128 -> 128 -> 6 perceptron, GELU after the hidden
+ * layers, bf16 weights and FP32 accumulation on the matrix cores.  Inputs per
+ * point: the nine velocity gradients du_i/dx_j (row-major), the wall distance
+ * 1 - |y|, the six stress components (11 12 13 22 23 33) of the two previous
+ * steps; output: the six stress components of the current step.  Arrays are
+ * point-major: X[n][22], Y[n][6].  Parameters: FP32 master vector of
+ * kmm_mlp_param_count() floats, W_l (n_out x n_in, row-major) then b_l for
+ * each layer. */
+typedef struct kmm_mlp kmm_mlp;
+int kmm_mlp_param_count(void);
+int kmm_mlp_create(int device, unsigned long long seed, kmm_mlp** out);
+int kmm_mlp_destroy(kmm_mlp* mlp);
+int kmm_mlp_set_params(kmm_mlp* mlp, int np, const float* params);
+int kmm_mlp_get_params(kmm_mlp* mlp, int np, float* params);
+/* input/output normalisation x_n = (x - mu_x)/sd_x, y = y_n sd_y + mu_y */
+int kmm_mlp_set_norm(kmm_mlp* mlp, const float* mu_x, const float* sd_x, const float* mu_y, const float* sd_y);
+int kmm_mlp_infer(kmm_mlp* mlp, int n, const float* X, float* Y);
+/* loss (mean over points of the summed squared normalised error) and its
+ * gradient; mode 0: FP32, mode 1: bf16 rounding as in inference */
+int kmm_mlp_grad(kmm_mlp* mlp, int n, const float* X, const float* T, int mode, float* grad, float* loss);
+/* Adam (lr, beta 0.9/0.999) on seeded mini-batches; losses[steps] optional */
+int kmm_mlp_train(kmm_mlp* mlp, int n, const float* X, const float* T, int steps, int batch, float lr,
+                  unsigned long long seed, float* losses);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,114 @@
+"""CPU restatement (numpy) of the learned polymer-stress closure: a 22-input,
+4 x 128 GELU, 6-output multilayer perceptron (no reference counterpart; the
+network is new in this build).
+
+TEST INFRASTRUCTURE ONLY: only tests/ may import this module.
+
+Conventions shared with polymer_dns_amd/csrc/kmm_mlp.hip:
+  * parameters: W[l] (n_out x n_in, row-major), b[l] (n_out); layer sizes
+    22 -> 128 -> 128 -> 128 -> 128 -> 6; GELU(x) = x/2 (1 + erf(x/sqrt 2))
+    after the four hidden layers, none after the last;
+  * input and output affine normalisation: x_n = (x - mu_x)/sd_x,
+    y = y_n sd_y + mu_y;
+  * "bf16" mode rounds weights and the input of every layer to bfloat16
+    (round to nearest even), accumulates in FP32;
+  * loss = mean over samples of sum over the 6 outputs of
+    (y_n - t_n)^2 with t_n = (t - mu_y)/sd_y.
+
+Pins: analytic gradients vs central finite differences (FP64); bf16
+rounding vs the bit pattern definition; GELU derivative vs differences.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+SIZES = (22, 128, 128, 128, 128, 6)
+NL = len(SIZES) - 1
+
+
+def bf16(x):
+    """Round float32 values to bfloat16 (nearest even) and return float32."""
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    u = a.view(np.uint32).astype(np.uint64)
+    r = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    out = r.astype(np.uint32).view(np.float32)
+    return np.where(np.isfinite(a), out, a)
+
+
+def gelu(x):
+    from scipy.special import erf
+    return 0.5 * x * (1.0 + erf(x / math.sqrt(2.0)))
+
+
+def gelu_grad(x):
+    from scipy.special import erf
+    return 0.5 * (1.0 + erf(x / math.sqrt(2.0))) + x * np.exp(-0.5 * x * x) / math.sqrt(2.0 * math.pi)
+
+
+def init_params(seed):
+    """He-style uniform initialisation from a seed (same recipe as the host
+    library: U(-a, a), a = sqrt(6/n_in), zero biases)."""
+    rng = np.random.default_rng(seed)
+    W, b = [], []
+    for l in range(NL):
+        a = math.sqrt(6.0 / SIZES[l])
+        W.append(rng.uniform(-a, a, (SIZES[l + 1], SIZES[l])).astype(np.float32))
+        b.append(np.zeros(SIZES[l + 1], dtype=np.float32))
+    return W, b
+
+
+def forward(W, b, X, mu_x, sd_x, mu_y, sd_y, mode="bf16", keep=False):
+    """Returns y (N x 6) and, with keep, the list of layer inputs and
+    pre-activations (for the backward pass)."""
+    dt = np.float64 if mode == "fp64" else np.float32
+    a = ((np.asarray(X, dtype=dt) - mu_x) / sd_x).astype(dt)
+    acts, pres = [], []
+    for l in range(NL):
+        if mode == "bf16":
+            a_in, w = bf16(a), bf16(W[l])
+            z = (a_in.astype(np.float64) @ w.astype(np.float64).T).astype(np.float32) + b[l]
+        else:
+            a_in, w = a, np.asarray(W[l], dtype=dt)
+            z = a_in @ w.T + np.asarray(b[l], dtype=dt)
+        acts.append(a_in)
+        pres.append(z)
+        a = gelu(z).astype(z.dtype) if l < NL - 1 else z
+    y = a * sd_y + mu_y
+    return (y, acts, pres) if keep else y
+
+
+def loss_and_grads(W, b, X, T, mu_x, sd_x, mu_y, sd_y, mode="fp64"):
+    """MSE loss on normalised outputs and its gradients w.r.t. W and b."""
+    y, acts, pres = forward(W, b, X, mu_x, sd_x, mu_y, sd_y, mode=mode, keep=True)
+    n = X.shape[0]
+    yn = (y - mu_y) / sd_y
+    tn = (np.asarray(T, dtype=yn.dtype) - mu_y) / sd_y
+    loss = float(((yn - tn) ** 2).sum() / n)
+    d = 2.0 * (yn - tn) / n                      # d loss / d z_last
+    gW, gb = [None] * NL, [None] * NL
+    for l in range(NL - 1, -1, -1):
+        if l < NL - 1:
+            d = d * gelu_grad(pres[l])
+        gW[l] = d.T @ acts[l]
+        gb[l] = d.sum(axis=0)
+        if l > 0:
+            w = bf16(W[l]) if mode == "bf16" else np.asarray(W[l], dtype=d.dtype)
+            d = d @ w
+    return loss, gW, gb
+
+
+def adam(W, b, gW, gb, state, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+    """One Adam update of the FP32 master parameters (Kingma & Ba 2015)."""
+    t = state.setdefault("t", 0) + 1
+    state["t"] = t
+    for l in range(NL):
+        for key, P, G in (("W", W, gW), ("b", b, gb)):
+            m = state.setdefault(key + "m%d" % l, np.zeros_like(P[l]))
+            v = state.setdefault(key + "v%d" % l, np.zeros_like(P[l]))
+            m[:] = beta1 * m + (1 - beta1) * G[l]
+            v[:] = beta2 * v + (1 - beta2) * G[l] * G[l]
+            mh = m / (1 - beta1 ** t)
+            vh = v / (1 - beta2 ** t)
+            P[l] = (P[l] - lr * mh / (np.sqrt(vh) + eps)).astype(np.float32)

@@ -143,6 +143,12 @@ static int fail(int code, const std::string& msg) {
   return code;
 }
 
+// shared with the other translation units of the library
+const char* kmm_last_error_set(const std::string& msg) {
+  g_last_error = msg;
+  return g_last_error.c_str();
+}
+
 // ----------------------------------------------------------- dns.config --
 static bool parse_num(const std::string& line, int nvals, double* out) {
   std::string s = line;

@@ -148,6 +148,17 @@ struct TracerArgs {
   int* err;                 // set to 1 when a particle leaves the grid
 };
 hipError_t launch_tracer(const TracerArgs& a, hipStream_t s);
+// learned closure (kmm_mlp.hip)
+int mlp_param_count();
+int mlp_wb_count();
+int mlp_fb_blocks(int n);
+hipError_t launch_mlp_infer(int n, const float* X, float* Y, const void* wb, const float* par, const float* norm,
+                            hipStream_t s);
+hipError_t launch_mlp_pack(const float* par, void* wb, hipStream_t s);
+hipError_t launch_mlp_grad(int n, const float* X, const float* T, const float* par, const float* norm, int mode,
+                           float* gpart, float* lpart, float* g, float* loss, hipStream_t s);
+hipError_t launch_mlp_adam(float* par, float* m, float* v, const float* g, float lr, int t, void* wb,
+                           hipStream_t s);
 hipError_t launch_gradv(int ny, int P, int nz, double alpha, double beta, double dyde, const double2* u,
                         const double2* v, const double2* w, double2* const* out, hipStream_t s);
 // swirl strength of n velocity-gradient tensors; g[k*n + j] = component k

@@ -44,6 +44,8 @@ EXPORTS = [
     "kmm_fluid_interp", "kmm_get_physical", "kmm_get_dpdx", "kmm_set_dpdx",
     "kmm_gradv", "kmm_calcswirl", "kmm_get_particle_swirl", "kmm_set_graph",
     "kmm_ensemble_step",
+    "kmm_mlp_param_count", "kmm_mlp_create", "kmm_mlp_destroy", "kmm_mlp_set_params", "kmm_mlp_get_params",
+    "kmm_mlp_set_norm", "kmm_mlp_infer", "kmm_mlp_grad", "kmm_mlp_train",
 ]
 
 
@@ -129,6 +131,16 @@ def load_library(path: str | None = None):
     lib.kmm_get_particle_swirl.argtypes = [C.c_void_p, C.c_int, dp]
     lib.kmm_set_graph.argtypes = [C.c_void_p, C.c_int]
     lib.kmm_ensemble_step.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int]
+    fp = C.POINTER(C.c_float)
+    lib.kmm_mlp_param_count.argtypes = []
+    lib.kmm_mlp_create.argtypes = [C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]
+    lib.kmm_mlp_destroy.argtypes = [C.c_void_p]
+    lib.kmm_mlp_set_params.argtypes = [C.c_void_p, C.c_int, fp]
+    lib.kmm_mlp_get_params.argtypes = [C.c_void_p, C.c_int, fp]
+    lib.kmm_mlp_set_norm.argtypes = [C.c_void_p, fp, fp, fp, fp]
+    lib.kmm_mlp_infer.argtypes = [C.c_void_p, C.c_int, fp, fp]
+    lib.kmm_mlp_grad.argtypes = [C.c_void_p, C.c_int, fp, fp, C.c_int, fp, fp]
+    lib.kmm_mlp_train.argtypes = [C.c_void_p, C.c_int, fp, fp, C.c_int, C.c_int, C.c_float, C.c_ulonglong, fp]
     lib.kmm_calcswirl.argtypes = [C.c_void_p] + [dp] * 10
     lib.kmm_set_dpdx.argtypes = [C.c_void_p, C.c_double]
     for n in EXPORTS:
@@ -405,3 +417,85 @@ def ensemble_step(channels, nsteps):
     lib = load_library()
     arr = (C.c_void_p * len(channels))(*[ch.h for ch in channels])
     _check(lib, lib.kmm_ensemble_step(arr, len(channels), int(nsteps)))
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Mlp:
+    """Learned polymer-stress closure (kmm_mlp_*): 22 inputs, 4 x 128 GELU,
+    6 outputs, bf16 weights with FP32 accumulation on the GPU."""
+
+    SIZES = (22, 128, 128, 128, 128, 6)
+
+    def __init__(self, seed=0, device=0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        _check(self.lib, self.lib.kmm_mlp_create(int(device), int(seed), C.byref(h)))
+        self.h = h
+        self.np = self.lib.kmm_mlp_param_count()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.kmm_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def params(self):
+        p = np.empty(self.np, dtype=np.float32)
+        _check(self.lib, self.lib.kmm_mlp_get_params(self.h, self.np, _fp(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(self.lib, self.lib.kmm_mlp_set_params(self.h, self.np, _fp(p)))
+
+    def unpack(self, p=None):
+        """Parameter vector -> lists W, b (layer order)."""
+        p = self.params() if p is None else p
+        W, b, o = [], [], 0
+        for l in range(5):
+            ni, no = self.SIZES[l], self.SIZES[l + 1]
+            W.append(p[o:o + no * ni].reshape(no, ni))
+            o += no * ni
+            b.append(p[o:o + no])
+            o += no
+        return W, b
+
+    def set_norm(self, mu_x, sd_x, mu_y, sd_y):
+        a = [np.ascontiguousarray(x, dtype=np.float32) for x in (mu_x, sd_x, mu_y, sd_y)]
+        _check(self.lib, self.lib.kmm_mlp_set_norm(self.h, *[_fp(x) for x in a]))
+
+    def infer(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.empty((X.shape[0], 6), dtype=np.float32)
+        _check(self.lib, self.lib.kmm_mlp_infer(self.h, X.shape[0], _fp(X), _fp(Y)))
+        return Y
+
+    def grad(self, X, T, mode=0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.empty(self.np, dtype=np.float32)
+        loss = np.empty(1, dtype=np.float32)
+        _check(self.lib, self.lib.kmm_mlp_grad(self.h, X.shape[0], _fp(X), _fp(T), int(mode), _fp(g), _fp(loss)))
+        return float(loss[0]), g
+
+    def train(self, X, T, steps, batch, lr=1e-3, seed=0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        losses = np.empty(steps, dtype=np.float32)
+        _check(self.lib, self.lib.kmm_mlp_train(self.h, X.shape[0], _fp(X), _fp(T), int(steps), int(batch),
+                                                float(lr), int(seed), _fp(losses)))
+        return losses

@@ -1,0 +1,57 @@
+"""Pins for the MLP-closure oracle (oracle/mlp_oracle.py): bfloat16 rounding
+against its bit-level definition, the GELU derivative and the analytic
+gradients against central finite differences."""
+import math
+
+import numpy as np
+
+from oracle import mlp_oracle as M
+
+
+def test_bf16_round_to_nearest_even():
+    one = np.float32(1.0)
+    ulp = 2.0 ** -7                       # bf16 spacing at 1
+    x = np.array([1.0, 1.0 + ulp / 2, 1.0 + 3 * ulp / 2, 1.0 + ulp / 4, -2.5, 3.0e38], dtype=np.float32)
+    got = M.bf16(x)
+    # ties go to the even mantissa: 1 + ulp/2 -> 1, 1 + 3ulp/2 -> 1 + 2ulp
+    exp = np.array([1.0, 1.0, 1.0 + 2 * ulp, 1.0, -2.5, 3.0e38], dtype=np.float32)
+    assert np.array_equal(got[:5], exp[:5])
+    assert (got.view(np.uint32) & 0xFFFF).max() == 0
+    assert abs(got[5] / exp[5] - 1) < 2 ** -8
+    assert M.bf16(one) == one
+
+
+def test_gelu_derivative():
+    x = np.linspace(-6, 6, 101)
+    h = 1e-6
+    fd = (M.gelu(x + h) - M.gelu(x - h)) / (2 * h)
+    assert np.abs(fd - M.gelu_grad(x)).max() < 1e-8
+    assert abs(M.gelu(np.array([1.0]))[0] - 0.5 * (1 + math.erf(1 / math.sqrt(2)))) < 1e-15
+
+
+def test_gradients_match_finite_differences():
+    rng = np.random.default_rng(3)
+    W, b = M.init_params(7)
+    W = [w.astype(np.float64) for w in W]
+    b = [bb.astype(np.float64) + 0.01 * rng.standard_normal(bb.shape) for bb in b]
+    X = rng.standard_normal((5, 22))
+    T = rng.standard_normal((5, 6))
+    mx, sx, my, sy = np.zeros(22), np.ones(22), np.zeros(6), np.ones(6)
+    loss, gW, gb = M.loss_and_grads(W, b, X, T, mx, sx, my, sy, mode="fp64")
+    h = 1e-6
+    for l, (i, j) in [(0, (3, 5)), (1, (10, 100)), (2, (0, 0)), (3, (127, 64)), (4, (5, 17))]:
+        Wp = [w.copy() for w in W]
+        Wm = [w.copy() for w in W]
+        Wp[l][i, j] += h
+        Wm[l][i, j] -= h
+        fd = (M.loss_and_grads(Wp, b, X, T, mx, sx, my, sy, "fp64")[0]
+              - M.loss_and_grads(Wm, b, X, T, mx, sx, my, sy, "fp64")[0]) / (2 * h)
+        assert abs(fd - gW[l][i, j]) < 1e-7 * max(1.0, abs(fd)), (l, i, j, fd, gW[l][i, j])
+    for l, i in [(0, 7), (4, 2)]:
+        bp = [x.copy() for x in b]
+        bm = [x.copy() for x in b]
+        bp[l][i] += h
+        bm[l][i] -= h
+        fd = (M.loss_and_grads(W, bp, X, T, mx, sx, my, sy, "fp64")[0]
+              - M.loss_and_grads(W, bm, X, T, mx, sx, my, sy, "fp64")[0]) / (2 * h)
+        assert abs(fd - gb[l][i]) < 1e-7 * max(1.0, abs(fd))
