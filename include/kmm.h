@@ -270,6 +270,22 @@ int kmm_mlp_grad(kmm_mlp* mlp, int n, const float* X, const float* T, int mode, 
 int kmm_mlp_train(kmm_mlp* mlp, int n, const float* X, const float* T, int steps, int batch, float lr,
                   unsigned long long seed, float* losses);
 
+/* Learned closure in the channel step (polymer = 1):
+ * kmm_set_poly_capture keeps, at every FENE-P evaluation, the physical
+ * velocity gradient and Peterlin stress (FP32, 3/2 grid) of the last three
+ * steps; kmm_poly_samples draws n seeded (features, target) pairs from the
+ * last three captured steps: features = du_i/dx_j (9), wall distance,
+ * str(n-1) (6) and str(n-1) - str(n-2) (6, formed in FP32), target = the
+ * correction to linear extrapolation str(n) - 2 str(n-1) + str(n-2); in the
+ * step the network's output is added to 2 str(n-1) - str(n-2) in FP32.  kmm_set_closure(ctx, mlp) replaces the conformation
+ * transport by the network from the next step on (needs the stress of the
+ * two previous steps), NULL returns to FENE-P; kmm_get_poly_stress returns
+ * the last stress component on the 3/2 grid (nyp, mz, mx). */
+int kmm_set_poly_capture(kmm_ctx* ctx, int on);
+int kmm_poly_samples(kmm_ctx* ctx, int n, unsigned long long seed, float* X, float* T);
+int kmm_set_closure(kmm_ctx* ctx, kmm_mlp* mlp);
+int kmm_get_poly_stress(kmm_ctx* ctx, int comp, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -99,6 +99,7 @@ struct kmm_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   long long graph_sig[4] = {0, 0, 0, 0};
+  int graph_slot = 0;
   double ubulk = 0.0;        // bulk-velocity target of the constant-mass-flux correction
   // FENE-P conformation tensor: independent components 11 12 13 22 23 33
   double2 *C[6] = {}, *CN[6] = {}, *CNM1[6] = {}, *STRN[6] = {}, *STRNM1[6] = {}, *TF[3] = {};
@@ -109,6 +110,13 @@ struct kmm_ctx {
   void *ZP = nullptr, *ZFP = nullptr;
   struct PolyFlags { bool init = false, solve = false, evaluate = false, force = false, copy = false; };
   PolyFlags pprev;      // flags of the evaluation whose results sit in NLP
+  // learned closure: physical captures of the last evaluations (FP32)
+  bool poly_capture = false;
+  float* hist[3] = {nullptr, nullptr, nullptr};  // [6][Pphys] stress, slot it % 3
+  float* capg = nullptr;                          // [9][Pphys] du_i/dx_k of the last evaluation
+  int hist_n = 0, hist_it = -1000;                // consecutive captured evaluations, last one
+  kmm_mlp* closure = nullptr;                     // network replacing the conformation transport
+  bool nlp_closure = false;                       // NLP holds only a closure stress (no c_n)
 };
 
 static void drop_graph(kmm_ctx* c);
@@ -740,7 +748,55 @@ static int run_poly_pencil(kmm_ctx* c, int mode, const kmm::PolyArgs& a) {
   return KMM_OK;
 }
 
-static int run_poly_plane(kmm_ctx* c) {
+static inline int hslot(int it) { return ((it % 3) + 3) % 3; }
+
+static void hist_note(kmm_ctx* c, int it) {
+  c->hist_n = (it == c->hist_it + 1) ? c->hist_n + 1 : 1;
+  c->hist_it = it;
+}
+
+// learned closure at step it: velocity gradient on the 3/2 grid, the network
+// on every point (inputs: gradient, wall distance, stress of it-1 and it-2),
+// forward transforms of the predicted stress into NLP[6..11]
+static int run_closure_plane(kmm_ctx* c, int it, bool swirl) {
+  kmm::PolyPlaneArgs sa;
+  std::memset(&sa, 0, sizeof(sa));
+  sa.nxh = c->nxh; sa.nz = c->nz; sa.nzh = c->nzh; sa.nyp = c->nyp; sa.mx = c->mx; sa.mz = c->mz;
+  sa.mixed = c->cfg.precision == 1;
+  sa.alpha = c->alpha; sa.beta = c->beta;
+  sa.twZ = c->twZ; sa.twX = c->twX;
+  sa.U = c->Z;
+  for (int q = 0; q < 3; ++q) { sa.in[q] = c->T[q]; sa.in[3 + q] = c->TPD[q]; }
+  sa.ZP = c->ZP;
+  sa.ZFP = c->ZFP;
+  sa.capg = c->capg;
+  sa.swirl = swirl ? c->cap + (size_t)3 * c->nyp * c->mz * c->mx : nullptr;
+  hipError_t e = kmm::launch_swirl_plane(sa, true, c->stream);
+  if (e != hipSuccess) return fail(KMM_E_HIP, std::string("closure gradient: ") + hipGetErrorString(e));
+  const void* wb;
+  const float *par, *nrm;
+  int dev;
+  kmm_mlp_view(c->closure, &wb, &par, &nrm, &dev);
+  kmm::MlpFieldIO io;
+  io.P = (size_t)c->nyp * c->mz * c->mx;
+  io.plane_pts = c->mz * c->mx;
+  io.half = 0.5 * c->cfg.yl;
+  io.ycoord = c->ycoord_d;
+  io.g = c->capg;
+  io.h1 = c->hist[hslot(it - 1)];
+  io.h2 = c->hist[hslot(it - 2)];
+  io.out = c->hist[hslot(it)];
+  if ((e = kmm::launch_mlp_field(io, wb, par, nrm, c->stream)) != hipSuccess)
+    return fail(KMM_E_HIP, std::string("closure network: ") + hipGetErrorString(e));
+  sa.caps = c->hist[hslot(it)];
+  for (int q = 0; q < 12; ++q) sa.out[q] = c->NLP[q];
+  if ((e = kmm::launch_closure_fwd(sa, c->stream)) != hipSuccess)
+    return fail(KMM_E_HIP, std::string("closure transforms: ") + hipGetErrorString(e));
+  hist_note(c, it);
+  return KMM_OK;
+}
+
+static int run_poly_plane(kmm_ctx* c, int it) {
   kmm::PolyPlaneArgs a;
   std::memset(&a, 0, sizeof(a));
   a.nxh = c->nxh; a.nz = c->nz; a.nzh = c->nzh; a.nyp = c->nyp; a.mx = c->mx; a.mz = c->mz;
@@ -755,8 +811,13 @@ static int run_poly_plane(kmm_ctx* c) {
   for (int q = 0; q < 6; ++q) { a.in[6 + q] = c->TP[q]; a.in[12 + q] = c->TP[q]; a.in[18 + q] = c->TP[6 + q]; }
   a.ZP = c->ZP; a.ZFP = c->ZFP;
   for (int q = 0; q < 12; ++q) a.out[q] = c->NLP[q];
+  if (c->poly_capture) {
+    a.capg = c->capg;
+    a.caps = c->hist[hslot(it)];
+  }
   hipError_t e = kmm::launch_poly_plane(a, c->stream);
   if (e != hipSuccess) return fail(KMM_E_HIP, std::string("polymer plane: ") + hipGetErrorString(e));
+  if (c->poly_capture) hist_note(c, it);
   return KMM_OK;
 }
 
@@ -778,17 +839,20 @@ static kmm::TracerArgs tracer_args(kmm_ctx* c) {
 // present, the physical velocity is kept and the tracers are initialised or
 // advanced (part_track, solvers.f90:3463-3466).
 static int run_plane(kmm_ctx* c, int it, unsigned long long* cfl, hipEvent_t* ev = nullptr, bool track = false,
-                     bool poly = false) {
+                     bool poly = false, bool closure = false) {
   PlaneArgs pa = plane_args(c, it, cfl);
   const bool trk = track && c->npart > 0;
   if (trk) pa.cap = c->cap;
   hipError_t e = ev ? kmm::launch_plane_timed(pa, c->stream, ev) : kmm::launch_plane(pa, c->stream);
   if (e != hipSuccess) return fail(KMM_E_HIP, std::string("plane stage: ") + hipGetErrorString(e));
   if (poly) {
-    int r = run_poly_plane(c);
+    int r = run_poly_plane(c, it);
     if (r) return r;
   }
-  if (trk && c->part_swirl) {
+  if (closure) {
+    int r = run_closure_plane(c, it, trk && c->part_swirl);
+    if (r) return r;
+  } else if (trk && c->part_swirl) {
     // swirl_3d (solvers.f90:3147-3150 via calcswirl) for swirl_part
     kmm::PolyPlaneArgs sa;
     std::memset(&sa, 0, sizeof(sa));
@@ -838,6 +902,7 @@ static int materialize(kmm_ctx* c) {
   if (c->cfg.polymer && c->pprev.evaluate) {
     kmm::PolyArgs pa = poly_args(c);
     pa.partB = 1; pa.force = c->pprev.force; pa.copy = c->pprev.copy; pa.smooth = 1;
+    pa.closure = c->nlp_closure ? 1 : 0;
     int r = run_poly_pencil(c, 2, pa);
     if (r) return r;
     if (c->pprev.force) for (int q = 0; q < 3; ++q) a.TF[q] = c->TF[q];
@@ -943,28 +1008,36 @@ static int enqueue_step(kmm_ctx* c, int s, hipEvent_t* ev, bool stage_ev) {
   a.first_nl = c->nl_valid ? 1 : 0;
   if (ev) hipEventRecord(ev[0], c->stream);
   else if (stage_ev) hipEventRecord(c->ev[0], c->stream);
-  const kmm_ctx::PolyFlags pf = poly_flags(c, c->it);
+  const bool clo = c->closure != nullptr;
+  kmm_ctx::PolyFlags pf = poly_flags(c, c->it);
+  if (clo) {  // learned closure: no conformation transport; the stress comes from the network
+    pf = kmm_ctx::PolyFlags();
+    pf.evaluate = true;
+    pf.force = true;
+  }
   if (c->cfg.polymer) {
     kmm::PolyArgs pa = poly_args(c);
     pa.partB = c->nl_valid && c->pprev.evaluate;
     pa.force = pa.partB && c->pprev.force;
     pa.copy = pa.partB && c->pprev.copy;
+    pa.closure = c->nlp_closure ? 1 : 0;
     pa.smooth = 1;
-    pa.init = pf.init; pa.solve = pf.solve; pa.evaluate = pf.evaluate;
+    if (!clo) { pa.init = pf.init; pa.solve = pf.solve; pa.evaluate = pf.evaluate; }
     int r = run_poly_pencil(c, 0, pa);
     if (r) return r;
     if (pa.force) for (int q = 0; q < 3; ++q) a.TF[q] = c->TF[q];
   }
-  if (pf.evaluate || (c->npart > 0 && c->part_swirl)) for (int q = 0; q < 3; ++q) a.TPD[q] = c->TPD[q];
+  if (pf.evaluate || clo || (c->npart > 0 && c->part_swirl)) for (int q = 0; q < 3; ++q) a.TPD[q] = c->TPD[q];
   if (ev) hipEventRecord(ev[1], c->stream);
   hipError_t e = kmm::launch_pencil_step(c->ny, a, c->stream);
   if (e != hipSuccess) return fail(KMM_E_HIP, std::string("pencil step: ") + hipGetErrorString(e));
   if (ev) hipEventRecord(ev[2], c->stream);
   else if (stage_ev) hipEventRecord(c->ev[1], c->stream);
-  int r = run_plane(c, c->it, c->cfl_d + s, ev ? ev + 3 : nullptr, true, pf.evaluate);
+  int r = run_plane(c, c->it, c->cfl_d + s, ev ? ev + 3 : nullptr, true, pf.evaluate && !clo, clo);
   if (r) return r;
   if (!ev && stage_ev) hipEventRecord(c->ev[2], c->stream);
   c->pprev = pf;
+  c->nlp_closure = clo;
   c->nl_valid = true;
   return KMM_OK;
 }
@@ -974,6 +1047,8 @@ static int enqueue_step(kmm_ctx* c, int s, hipEvent_t* ev, bool stage_ev) {
 // a captured graph.
 static bool steady(const kmm_ctx* c, int k) {
   if (!c->nl_valid) return false;
+  if (c->closure) return c->nlp_closure && k % 3 == 0;  // history slots repeat every 3 steps
+  if (c->poly_capture) return false;                     // capture bookkeeping runs per launch
   if (c->npart > 0 && c->part_state != 2) return false;
   if (c->nsrc > 0 && c->it < c->cfg.perturbtime + 1) return false;
   if (c->cfg.polymer) {
@@ -1028,9 +1103,11 @@ static int step_chunk_enqueue(kmm_ctx* c, int remaining, StepChunk& ch) {
   if (ch.graph) {
     // kernel arguments and buffers of a steady step are the same every time;
     // the signature guards what could change between calls
-    const long long sig[4] = {(long long)c->npart, (long long)(uintptr_t)c->part, (long long)c->nsrc,
-                              (long long)c->part_swirl};
-    if (c->gexec && std::memcmp(sig, c->graph_sig, sizeof(sig)) != 0) drop_graph(c);
+    const long long sig[4] = {(long long)c->npart, (long long)(uintptr_t)c->part,
+                              (long long)c->nsrc * 2 + (c->part_swirl ? 1 : 0), (long long)(uintptr_t)c->closure};
+    if (c->gexec && (std::memcmp(sig, c->graph_sig, sizeof(sig)) != 0 ||
+                     (c->closure && hslot(c->it + 1) != c->graph_slot)))
+      drop_graph(c);
     if (!c->gexec) {
       KMM_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       int r = 0;
@@ -1046,9 +1123,13 @@ static int step_chunk_enqueue(kmm_ctx* c, int remaining, StepChunk& ch) {
       c->graph = g;
       KMM_HIP(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
       std::memcpy(c->graph_sig, sig, sizeof(sig));
+      c->graph_slot = hslot(itsave + 1);
       c->it = itsave;  // the capture recorded the steps; the replay below runs them
+      if (c->closure) c->hist_it = itsave;
     }
     KMM_HIP(hipGraphLaunch(c->gexec, c->stream));
+    if (c->closure)
+      for (int q = 1; q <= n; ++q) hist_note(c, c->it + q);
     c->it += n;
   } else {
     for (int s = 0; s < n; ++s) {
@@ -1757,4 +1838,85 @@ extern "C" int kmm_calcswirl(kmm_ctx* c, const double* u11, const double* u21, c
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   hipFree(d);
   return e == hipSuccess ? KMM_OK : fail(KMM_E_HIP, hipGetErrorString(e));
+}
+
+// ------------------------------------------------------ learned closure --
+extern "C" int kmm_set_poly_capture(kmm_ctx* c, int on) {
+  if (!c) return fail(KMM_E_ARG, "null argument");
+  if (!c->cfg.polymer) return fail(KMM_E_CONFIG, "capture needs the polymer (polymer = 1)");
+  KMM_HIP(hipSetDevice(c->device));
+  if (on && !c->capg) {
+    const size_t P = (size_t)c->nyp * c->mz * c->mx;
+    KMM_HIP(hipMalloc((void**)&c->capg, 9 * P * sizeof(float)));
+    c->allocs.push_back(c->capg);
+    for (int q = 0; q < 3; ++q) {
+      KMM_HIP(hipMalloc((void**)&c->hist[q], 6 * P * sizeof(float)));
+      c->allocs.push_back(c->hist[q]);
+    }
+  }
+  c->poly_capture = on != 0;
+  drop_graph(c);
+  return KMM_OK;
+}
+
+extern "C" int kmm_poly_samples(kmm_ctx* c, int n, unsigned long long seed, float* X, float* T) {
+  if (!c || n <= 0 || !X || !T) return fail(KMM_E_ARG, "bad argument");
+  if (!c->capg || c->hist_n < 3)
+    return fail(KMM_E_ARG, "needs three consecutive captured evaluations (kmm_set_poly_capture, then step)");
+  KMM_HIP(hipSetDevice(c->device));
+  const int it = c->hist_it;
+  kmm::MlpFieldIO io;
+  io.P = (size_t)c->nyp * c->mz * c->mx;
+  io.plane_pts = c->mz * c->mx;
+  io.half = 0.5 * c->cfg.yl;
+  io.ycoord = c->ycoord_d;
+  io.g = c->capg;
+  io.h1 = c->hist[hslot(it - 1)];
+  io.h2 = c->hist[hslot(it - 2)];
+  io.out = nullptr;
+  float *dX = nullptr, *dT = nullptr;
+  hipError_t e = hipMalloc((void**)&dX, (size_t)n * 22 * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc((void**)&dT, (size_t)n * 6 * sizeof(float));
+  if (e == hipSuccess) e = kmm::launch_mlp_sample(io, c->hist[hslot(it)], n, seed, dX, dT, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(X, dX, (size_t)n * 22 * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(T, dT, (size_t)n * 6 * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(dX);
+  hipFree(dT);
+  return e == hipSuccess ? KMM_OK : fail(KMM_E_HIP, hipGetErrorString(e));
+}
+
+extern "C" int kmm_set_closure(kmm_ctx* c, kmm_mlp* mlp) {
+  if (!c) return fail(KMM_E_ARG, "null argument");
+  if (mlp) {
+    if (!c->cfg.polymer) return fail(KMM_E_CONFIG, "the closure replaces the polymer (polymer = 1)");
+    const void* wb;
+    const float *par, *nrm;
+    int dev = -1;
+    kmm_mlp_view(mlp, &wb, &par, &nrm, &dev);
+    if (dev != c->device) return fail(KMM_E_ARG, "network and channel on different devices");
+    if (!c->capg || c->hist_n < 2 || c->hist_it != c->it)
+      return fail(KMM_E_ARG, "the closure needs the stress of the last two steps (capture on for two steps)");
+  }
+  c->closure = mlp;
+  drop_graph(c);
+  return KMM_OK;
+}
+
+// stress of the last evaluation on the 3/2 grid, reference layout (nyp, mz, mx)
+// per component (11 12 13 22 23 33)
+extern "C" int kmm_get_poly_stress(kmm_ctx* c, int comp, double* out) {
+  if (!c || !out || comp < 0 || comp > 5) return fail(KMM_E_ARG, "bad argument");
+  if (!c->capg || c->hist_n < 1) return fail(KMM_E_ARG, "no captured stress");
+  KMM_HIP(hipSetDevice(c->device));
+  const size_t P = (size_t)c->nyp * c->mz * c->mx;
+  std::vector<float> h(P);
+  KMM_HIP(hipMemcpyAsync(h.data(), c->hist[hslot(c->hist_it)] + (size_t)comp * P, P * sizeof(float),
+                         hipMemcpyDeviceToHost, c->stream));
+  KMM_HIP(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < c->nyp; ++k)
+    for (int jz = 0; jz < c->mz; ++jz)
+      for (int ix = 0; ix < c->mx; ++ix)
+        out[k + (size_t)c->nyp * (jz + (size_t)c->mz * ix)] = h[((size_t)k * c->mz + jz) * c->mx + ix];
+  return KMM_OK;
 }

@@ -48,6 +48,7 @@ struct PolyArgs {
   double diffpoly, tpoly, zbeta1, gC;  // gC = re diffpoly/(dt theta)
   double c0[3];                        // c11z, c22z, c33z
   int partB;     // forward-transform NLP into c_n, str_n (else load CN, STRN)
+  int closure;   // learned closure: NLP holds only str_n (no c_n, no conformation)
   int force;     // write TF = div(zbeta1 str_n) (unsmoothed)
   int copy;      // CNM1/STRNM1 = unsmoothed c_n/str_n (n -> n-1 before smoothing)
   int smooth;
@@ -120,12 +121,37 @@ struct PolyPlaneArgs {
   void* ZFP;              // [12]: c_n(6), str_n(6) after the forward x transform
   void* out[12];          // NLP outputs [y][kx][kz]
   double* swirl;          // [nyp][mz][mx] swirl strength (launch_swirl_plane)
+  float* capg;            // nullptr or [9][P] physical du_i/dx_k, P = nyp mz mx, [y][z][x]
+  float* caps;            // nullptr or [6][P] physical Peterlin stress (11 12 13 22 23 33)
 };
 hipError_t launch_poly_plane(const PolyPlaneArgs& a, hipStream_t s);
 // swirl strength on the 3/2 grid from the velocity gradient; with zstage the
 // z transforms of dz u (from in[0..2]) and dy u (in[3..5]) into ZP[0..5] run
 // first (they are already there when the polymer stage ran)
 hipError_t launch_swirl_plane(const PolyPlaneArgs& a, bool zstage, hipStream_t s);
+// learned closure: forward transforms of the stress fields caps[6][P] into out[6..11]
+hipError_t launch_closure_fwd(const PolyPlaneArgs& a, hipStream_t s);
+
+// MLP on field-major inputs: features of point p are g[k][p] (k < 9), the
+// wall distance 1 - |ycoord[p / (mz mx)]| / (yl/2), h1[j][p] and
+// h1[j][p] - h2[j][p] (j < 6); the network gives the correction to linear
+// extrapolation: out[j][p] = 2 h1[j][p] - h2[j][p] + y_j
+struct MlpFieldIO {
+  size_t P;
+  int plane_pts;           // mz * mx
+  double half;             // yl / 2
+  const double* ycoord;
+  const float* g;
+  const float* h1;
+  const float* h2;
+  float* out;
+};
+hipError_t launch_mlp_field(const MlpFieldIO& io, const void* wb, const float* par, const float* norm,
+                            hipStream_t s);
+// training samples from captured fields: for i < n, point p_i = hash(seed, i)
+// mod P; X[i] = features of p_i, T[i] = t - 2 h1 + h2 at p_i
+hipError_t launch_mlp_sample(const MlpFieldIO& io, const float* t, int n, unsigned long long seed, float* X,
+                             float* T, hipStream_t s);
 
 // Lagrangian tracers (part_track.f90): particle j has x, y, z, u, v, w at
 // part[0..5][j]; fields are read at fld[f*fstride + k*sy + jz*sz + ix*sx]
@@ -188,3 +214,7 @@ hipError_t launch_cheb(int ny, int P, const void* tw, const void* in, void* out,
                        hipStream_t s);
 
 }  // namespace kmm
+
+// device view of a learned-closure network (kmm_mlp_api.cpp)
+struct kmm_mlp;
+bool kmm_mlp_view(kmm_mlp* m, const void** wb, const float** par, const float** norm, int* device);

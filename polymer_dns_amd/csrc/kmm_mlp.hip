@@ -20,6 +20,7 @@
 // W_l (n_out x n_in row-major) then b_l; sizes 22,128,128,128,128,6.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "kmm_kernels.h"
 #include "kmm_device.h"
@@ -73,9 +74,20 @@ int mlp_param_count() { return kParams; }
 int mlp_wb_count() { return kWb; }
 
 // -------------------------------------------------------------- inference --
+// feature k of point p from field-major captures (learned-closure layout)
+__device__ __forceinline__ float field_feature(const MlpFieldIO& io, size_t p, int k) {
+  if (k < 9) return io.g[(size_t)k * io.P + p];
+  if (k == 9) return (float)(1.0 - fabs(io.ycoord[p / (size_t)io.plane_pts]) / io.half);
+  if (k < 16) return io.h1[(size_t)(k - 10) * io.P + p];
+  // the last step's change, formed in FP32 before the bf16 rounding of the inputs
+  return io.h1[(size_t)(k - 16) * io.P + p] - io.h2[(size_t)(k - 16) * io.P + p];
+}
+
+// FIELD = false: X[n][22] -> Y[n][6]; FIELD = true: io fields -> io.out[6][P]
+template <bool FIELD>
 __global__ __launch_bounds__(256) void k_mlp_infer(int n, const float* __restrict__ X, float* __restrict__ Y,
-                                                   const __bf16* __restrict__ wb, const float* __restrict__ par,
-                                                   const float* __restrict__ norm) {
+                                                   MlpFieldIO io, const __bf16* __restrict__ wb,
+                                                   const float* __restrict__ par, const float* __restrict__ norm) {
   extern __shared__ __align__(16) unsigned char smraw[];
   __bf16* sw = reinterpret_cast<__bf16*>(smraw);                 // [kWb]
   float* sb = reinterpret_cast<float*>(sw + kWb);                 // biases [kBias]
@@ -96,11 +108,17 @@ __global__ __launch_bounds__(256) void k_mlp_infer(int n, const float* __restric
   const int wstride = gridDim.x * (blockDim.x >> 6);
   for (int t = blockIdx.x * (blockDim.x >> 6) + wv; t < ntiles; t += wstride) {
     const int p0 = t * 16;
-    // input tile: 16 points x 32 (22 used), normalised, bf16
+    // input tile: 16 points x 32 (22 used), normalised, bf16; field inputs
+    // are read with consecutive lanes on consecutive points
     for (int e = lane; e < 16 * 32; e += 64) {
-      const int r = e >> 5, k = e & 31;
+      int r, k;
+      if (FIELD) { k = e >> 4; r = e & 15; }
+      else { r = e >> 5; k = e & 31; }
       float v = 0.0f;
-      if (k < kIn && p0 + r < n) v = (X[(size_t)(p0 + r) * kIn + k] - mu_x[k]) / sd_x[k];
+      if (k < kIn && p0 + r < n) {
+        const float xv = FIELD ? field_feature(io, (size_t)(p0 + r), k) : X[(size_t)(p0 + r) * kIn + k];
+        v = (xv - mu_x[k]) / sd_x[k];
+      }
       act[r * (kH + 8) + k] = (__bf16)v;
     }
     wave_sync();
@@ -138,7 +156,16 @@ __global__ __launch_bounds__(256) void k_mlp_infer(int n, const float* __restric
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const int p = p0 + 4 * kq + i;
-            if (p < n) Y[(size_t)p * kOut + col] = (acc[0][i] + sb[512 + col]) * sd_y[col] + mu_y[col];
+            if (p < n) {
+              const float yv = (acc[0][i] + sb[512 + col]) * sd_y[col] + mu_y[col];
+              // the closure network predicts the correction to linear extrapolation
+              // of the two previous steps: out = 2 h1 - h2 + y
+              if (FIELD) {
+                const size_t q = (size_t)col * io.P + p;
+                io.out[q] = io.h1[q] + (io.h1[q] - io.h2[q]) + yv;
+              }
+              else Y[(size_t)p * kOut + col] = yv;
+            }
           }
         }
       }
@@ -149,20 +176,60 @@ __global__ __launch_bounds__(256) void k_mlp_infer(int n, const float* __restric
 
 size_t mlp_infer_lds() { return (size_t)kWb * 2 + (size_t)kBias * 4 + (size_t)4 * 16 * (kH + 8) * 2; }
 
-hipError_t launch_mlp_infer(int n, const float* X, float* Y, const void* wb, const float* par, const float* norm,
-                            hipStream_t s) {
+template <bool FIELD>
+static hipError_t mlp_infer_launch(int n, const float* X, float* Y, const MlpFieldIO& io, const void* wb,
+                                   const float* par, const float* norm, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const int tiles = (n + 15) / 16;
   const int blocks = std::min((tiles + 3) / 4, 256 * 8);
   static bool attr = false;  // > 64 KB of dynamic LDS (160 KB per CU on gfx950)
   if (!attr) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_mlp_infer, hipFuncAttributeMaxDynamicSharedMemorySize,
+    hipError_t e = hipFuncSetAttribute((const void*)k_mlp_infer<FIELD>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)mlp_infer_lds());
     if (e != hipSuccess) return e;
     attr = true;
   }
-  hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), mlp_infer_lds(), s, n, X, Y,
+  hipLaunchKernelGGL((k_mlp_infer<FIELD>), dim3(blocks), dim3(256), mlp_infer_lds(), s, n, X, Y, io,
                      static_cast<const __bf16*>(wb), par, norm);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_infer(int n, const float* X, float* Y, const void* wb, const float* par, const float* norm,
+                            hipStream_t s) {
+  MlpFieldIO io;
+  std::memset(&io, 0, sizeof(io));
+  return mlp_infer_launch<false>(n, X, Y, io, wb, par, norm, s);
+}
+
+hipError_t launch_mlp_field(const MlpFieldIO& io, const void* wb, const float* par, const float* norm,
+                            hipStream_t s) {
+  return mlp_infer_launch<true>((int)io.P, nullptr, nullptr, io, wb, par, norm, s);
+}
+
+// SplitMix64 of (seed, i): point index of sample i
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ void k_mlp_sample(MlpFieldIO io, const float* __restrict__ t, int n, unsigned long long seed,
+                             float* __restrict__ X, float* __restrict__ T) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t p = (size_t)(mix64(seed ^ mix64((unsigned long long)i)) % (unsigned long long)io.P);
+  for (int k = 0; k < kIn; ++k) X[(size_t)i * kIn + k] = field_feature(io, p, k);
+  for (int j = 0; j < kOut; ++j) {
+    const size_t q = (size_t)j * io.P + p;
+    T[(size_t)i * kOut + j] = (t[q] - io.h1[q]) - (io.h1[q] - io.h2[q]);
+  }
+}
+
+hipError_t launch_mlp_sample(const MlpFieldIO& io, const float* t, int n, unsigned long long seed, float* X,
+                             float* T, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mlp_sample, dim3((n + 255) / 256), dim3(256), 0, s, io, t, n, seed, X, T);
   return hipGetLastError();
 }
 

@@ -215,3 +215,12 @@ extern "C" int kmm_mlp_train(kmm_mlp* n, int npts, const float* X, const float* 
   }
   return KMM_OK;
 }
+
+bool kmm_mlp_view(kmm_mlp* m, const void** wb, const float** par, const float** norm, int* device) {
+  if (!m) return false;
+  *wb = m->wb;
+  *par = m->par;
+  *norm = m->norm;
+  *device = m->device;
+  return true;
+}

@@ -442,10 +442,20 @@ struct PolyRow {
   T L2;
   V *buf, *scr;
   const V* tw;
+  float* capg;   // nullptr or [9][P] physical du_i/dx_k (FP32)
+  float* caps;   // nullptr or [6][P] physical Peterlin stress (FP32)
+  size_t P, prow;  // physical points, offset of this row
   __device__ const V* rowp(int id) const {
     return id < 0 ? U + (size_t)(-1 - id) * plane + roff : ZP + (size_t)id * plane + roff;
   }
   __device__ V* orow(int f) const { return ZF + (size_t)f * plane + roff; }
+  __device__ void cap(float* base, int f, const T (&v)[XPL]) const {
+#pragma unroll
+    for (int m = 0; m < XPL; ++m) {
+      const int x = lane + 64 * m;
+      if (x < MX) base[(size_t)f * P + prow + x] = (float)v[m];
+    }
+  }
 
   template <int Q>
   __device__ __forceinline__ void step() {
@@ -461,6 +471,13 @@ struct PolyRow {
     }
     T oa[XPL], ob[XPL];
     pair_inv<MX, V>(pin[Q & 1], sp.dxa, sp.dxb, nxh, alpha, buf, scr, tw, lane, oa, ob);
+    if (capg) {  // g[3 i + k] = du_i/dx_k
+      if constexpr (Q == 3) { cap(capg, 0, oa); cap(capg, 3, ob); }
+      if constexpr (Q == 4) { cap(capg, 6, oa); cap(capg, 1, ob); }
+      if constexpr (Q == 5) { cap(capg, 4, oa); cap(capg, 7, ob); }
+      if constexpr (Q == 6) { cap(capg, 2, oa); cap(capg, 5, ob); }
+      if constexpr (Q == 7) { cap(capg, 8, oa); }
+    }
 #pragma unroll
     for (int m = 0; m < XPL; ++m) {
       if constexpr (Q == 0 || Q == 18) { c[m][0] = oa[m]; c[m][1] = ob[m]; }
@@ -498,6 +515,7 @@ struct PolyRow {
       sa[m] = fac * c[m][E] - ((E == 0) ? (T)1 : (T)0);
       sb[m] = fac * c[m][E + 1] - ((E + 1 == 3 || E + 1 == 5) ? (T)1 : (T)0);
     }
+    if (caps) { cap(caps, E, sa); cap(caps, E + 1, sb); }
     xrow_pair_fwd<MX, V>(sa, sb, orow(6 + E), orow(7 + E), nxh, buf, scr, tw, lane);
   }
   template <int... Q>
@@ -533,6 +551,10 @@ __global__ __launch_bounds__(256) void k_xplane_poly(PolyPlaneArgs A) {
   R.buf = sm + MX + (size_t)wv * 2 * BL;
   R.scr = R.buf + BL;
   R.tw = tw;
+  R.capg = A.capg;
+  R.caps = A.caps;
+  R.P = (size_t)A.nyp * A.mz * MX;
+  R.prow = ((size_t)y * A.mz + z) * MX;
 #pragma unroll
   for (int m = 0; m < PolyRow<MX, V>::XPL; ++m)
 #pragma unroll
@@ -585,15 +607,81 @@ __global__ __launch_bounds__(256) void k_xplane_swirl(PolyPlaneArgs A) {
   T dummy[XPL];
   pair_load<MX, V>(pin, zrow(2), zrow(2), nxh, lane);
   pair_inv<MX, V>(pin, false, false, nxh, A.alpha, buf, scr, tw, lane, g[8], dummy);
-  double* out = A.swirl + ((size_t)y * A.mz + z) * MX;
+  const size_t prow = ((size_t)y * A.mz + z) * MX;
+  if (A.capg) {  // du_i/dx_k for the learned closure, [9][P] FP32
+    const size_t P = (size_t)A.nyp * A.mz * MX;
 #pragma unroll
-  for (int m = 0; m < XPL; ++m) {
-    const int x = lane + 64 * m;
-    double a[9];
+    for (int k = 0; k < 9; ++k)
 #pragma unroll
-    for (int k = 0; k < 9; ++k) a[k] = (double)g[k][m];
-    if (x < MX) out[x] = swirl_of(a);
+      for (int m = 0; m < XPL; ++m) {
+        const int x = lane + 64 * m;
+        if (x < MX) A.capg[(size_t)k * P + prow + x] = (float)g[k][m];
+      }
   }
+  if (A.swirl) {
+    double* out = A.swirl + prow;
+#pragma unroll
+    for (int m = 0; m < XPL; ++m) {
+      const int x = lane + 64 * m;
+      double a[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) a[k] = (double)g[k][m];
+      if (x < MX) out[x] = swirl_of(a);
+    }
+  }
+}
+
+// forward x transforms of the six closure stress rows (FP32 [6][P]) into the
+// z-stage inputs ZFP[6..11]
+template <int MX, typename V>
+__global__ __launch_bounds__(256) void k_xrow_fwd_str(PolyPlaneArgs A) {
+  using T = typename Scal<V>::T;
+  constexpr int XPL = (MX + 63) / 64;
+  constexpr int WPB = 4;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  V* sm = reinterpret_cast<V*>(smraw);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  constexpr int BL = FFTBuf<MX>::len;
+  V* tw = sm;
+  V* buf = sm + MX + (size_t)wv * 2 * BL;
+  V* scr = buf + BL;
+  for (int e = threadIdx.x; e < MX; e += WPB * 64) tw[e] = static_cast<const V*>(A.twX)[e];
+  __syncthreads();
+  const int row = blockIdx.x * WPB + wv;
+  const int y = row / A.mz, z = row % A.mz;
+  const size_t P = (size_t)A.nyp * A.mz * MX, prow = ((size_t)y * A.mz + z) * MX;
+  const size_t plane = (size_t)A.nyp * A.mz * A.nxh, roff = ((size_t)y * A.mz + z) * A.nxh;
+  V* ZF = static_cast<V*>(A.ZFP);
+#pragma unroll
+  for (int e = 0; e < 6; e += 2) {
+    T sa[XPL], sb[XPL];
+#pragma unroll
+    for (int m = 0; m < XPL; ++m) {
+      const int x = lane + 64 * m;
+      sa[m] = (x < MX) ? (T)A.caps[(size_t)e * P + prow + x] : (T)0;
+      sb[m] = (x < MX) ? (T)A.caps[(size_t)(e + 1) * P + prow + x] : (T)0;
+    }
+    xrow_pair_fwd<MX, V>(sa, sb, ZF + (size_t)(6 + e) * plane + roff, ZF + (size_t)(7 + e) * plane + roff, A.nxh,
+                         buf, scr, tw, lane);
+  }
+}
+
+template <int MX, int MZ, typename V>
+static hipError_t closure_fwd_impl(const PolyPlaneArgs& pa, hipStream_t s) {
+  const size_t xlds = (size_t)(MX + 4 * 2 * FFTBuf<MX>::len) * sizeof(V);
+  hipLaunchKernelGGL((k_xrow_fwd_str<MX, V>), dim3(pa.nyp * pa.mz / 4), dim3(256), xlds, s, pa);
+  PlaneArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.nxh = pa.nxh; a.nz = pa.nz; a.nzh = pa.nzh; a.nyp = pa.nyp; a.mx = pa.mx; a.mz = pa.mz;
+  a.twZ = pa.twZ; a.twX = pa.twX;
+  const size_t zb = (size_t)pa.nyp * pa.mz * pa.nxh;
+  const size_t zlds = (size_t)(16 * FFTBuf<MZ>::len + 9 * MZ) * sizeof(V);
+  for (int g = 0; g < 2; ++g) {
+    a.ZF = static_cast<V*>(pa.ZFP) + (size_t)(6 + 3 * g) * zb;
+    for (int o = 0; o < 3; ++o) a.out[o] = pa.out[6 + 3 * g + o];
+    hipLaunchKernelGGL((k_zfwd<MZ, 3, V>), dim3(a.nyp * (a.nxh / 8)), dim3(512), zlds, s, a);
+  }
+  return hipGetLastError();
 }
 
 template <int MX, int MZ, typename V>
@@ -727,6 +815,22 @@ hipError_t launch_swirl_plane(const PolyPlaneArgs& a, bool zstage, hipStream_t s
   KMM_SWIRL_CASE(288, 288)
   KMM_SWIRL_CASE(384, 192)
 #undef KMM_SWIRL_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_closure_fwd(const PolyPlaneArgs& a, hipStream_t s) {
+  if (a.nxh % 8 != 0 || (a.nyp * a.mz) % 4 != 0) return hipErrorInvalidValue;
+#define KMM_CLO_CASE(X, Zz) \
+  if (a.mx == X && a.mz == Zz) return a.mixed ? closure_fwd_impl<X, Zz, float2>(a, s) : closure_fwd_impl<X, Zz, double2>(a, s);
+  KMM_CLO_CASE(48, 48)
+  KMM_CLO_CASE(96, 48)
+  KMM_CLO_CASE(96, 96)
+  KMM_CLO_CASE(144, 144)
+  KMM_CLO_CASE(192, 96)
+  KMM_CLO_CASE(192, 192)
+  KMM_CLO_CASE(288, 288)
+  KMM_CLO_CASE(384, 192)
+#undef KMM_CLO_CASE
   return hipErrorInvalidValue;
 }
 

@@ -47,7 +47,10 @@ __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
     const PF z = pf_zero();
     for (int c = 0; c < 6; ++c) {
       if (MODE == 0 || MODE == 2) {
-        if (A.partB || MODE == 2) { pf_store<NY>(A.CN[c], P, p, lane, z); pf_store<NY>(A.STRN[c], P, p, lane, z); }
+        if (A.partB || MODE == 2) {
+          if (!A.closure) pf_store<NY>(A.CN[c], P, p, lane, z);
+          pf_store<NY>(A.STRN[c], P, p, lane, z);
+        }
         if (A.copy || A.solve) { pf_store<NY>(A.CNM1[c], P, p, lane, z); pf_store<NY>(A.STRNM1[c], P, p, lane, z); }
       }
       if (MODE == 0 && (A.init || A.solve)) pf_store<NY>(A.C[c], P, p, lane, z);
@@ -73,7 +76,7 @@ __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
     PF cnm1_copy, snm1_copy;
     const bool haveB = (MODE == 2) || (MODE == 0 && A.partB);
     if (haveB) {
-      cn = phys_to_cheb_load<NY, V>(static_cast<const V*>(A.NLP[c]), P, p, buf, scr, twl, lane);
+      cn = A.closure ? pf_zero() : phys_to_cheb_load<NY, V>(static_cast<const V*>(A.NLP[c]), P, p, buf, scr, twl, lane);
       sn = phys_to_cheb_load<NY, V>(static_cast<const V*>(A.NLP[6 + c]), P, p, buf, scr, twl, lane);
       if (is00) { cn = pf_real(cn); sn = pf_real(sn); }
       if (A.force) {
@@ -98,7 +101,7 @@ __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
         cn = smooth_apply<NY>(cn, A.smy, sz, lane);
         sn = smooth_apply<NY>(sn, A.smy, sz, lane);
       }
-      pf_store<NY>(A.CN[c], P, p, lane, cn);
+      if (!A.closure) pf_store<NY>(A.CN[c], P, p, lane, cn);
       pf_store<NY>(A.STRN[c], P, p, lane, sn);
       if (MODE == 2) continue;
     }

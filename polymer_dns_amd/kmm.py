@@ -46,6 +46,7 @@ EXPORTS = [
     "kmm_ensemble_step",
     "kmm_mlp_param_count", "kmm_mlp_create", "kmm_mlp_destroy", "kmm_mlp_set_params", "kmm_mlp_get_params",
     "kmm_mlp_set_norm", "kmm_mlp_infer", "kmm_mlp_grad", "kmm_mlp_train",
+    "kmm_set_poly_capture", "kmm_poly_samples", "kmm_set_closure", "kmm_get_poly_stress",
 ]
 
 
@@ -141,6 +142,10 @@ def load_library(path: str | None = None):
     lib.kmm_mlp_infer.argtypes = [C.c_void_p, C.c_int, fp, fp]
     lib.kmm_mlp_grad.argtypes = [C.c_void_p, C.c_int, fp, fp, C.c_int, fp, fp]
     lib.kmm_mlp_train.argtypes = [C.c_void_p, C.c_int, fp, fp, C.c_int, C.c_int, C.c_float, C.c_ulonglong, fp]
+    lib.kmm_set_poly_capture.argtypes = [C.c_void_p, C.c_int]
+    lib.kmm_poly_samples.argtypes = [C.c_void_p, C.c_int, C.c_ulonglong, fp, fp]
+    lib.kmm_set_closure.argtypes = [C.c_void_p, C.c_void_p]
+    lib.kmm_get_poly_stress.argtypes = [C.c_void_p, C.c_int, dp]
     lib.kmm_calcswirl.argtypes = [C.c_void_p] + [dp] * 10
     lib.kmm_set_dpdx.argtypes = [C.c_void_p, C.c_double]
     for n in EXPORTS:
@@ -404,6 +409,27 @@ class Channel:
         out = np.empty_like(a[0][0])
         order = [a[0][0], a[1][0], a[2][0], a[0][1], a[1][1], a[2][1], a[0][2], a[1][2], a[2][2]]
         _check(self.lib, self.lib.kmm_calcswirl(self.h, *[_dp(x) for x in order], _dp(out)))
+        return out
+
+    # ---- learned closure ----
+    def set_poly_capture(self, on=True):
+        _check(self.lib, self.lib.kmm_set_poly_capture(self.h, int(bool(on))))
+
+    def poly_samples(self, n, seed=0):
+        X = np.empty((n, 22), dtype=np.float32)
+        T = np.empty((n, 6), dtype=np.float32)
+        _check(self.lib, self.lib.kmm_poly_samples(self.h, int(n), int(seed),
+                                                   X.ctypes.data_as(C.POINTER(C.c_float)),
+                                                   T.ctypes.data_as(C.POINTER(C.c_float))))
+        return X, T
+
+    def set_closure(self, mlp):
+        _check(self.lib, self.lib.kmm_set_closure(self.h, mlp.h if mlp is not None else None))
+        self._closure = mlp   # keep the network alive while the channel uses it
+
+    def poly_stress(self, comp):
+        out = np.empty((self.mx, self.mz, self.nyp))
+        _check(self.lib, self.lib.kmm_get_poly_stress(self.h, int(comp), _dp(out)))
         return out
 
     def set_graph(self, steps):

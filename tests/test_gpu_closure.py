@@ -1,0 +1,80 @@
+"""The learned polymer-stress closure in the channel step: training data
+drawn by the GPU from its own FENE-P run, GPU training, the held-out gain over
+the closure without the network (zero increment, i.e. frozen stress) and over
+the untrained network, and a closure run that tracks the FENE-P run."""
+import numpy as np
+import pytest
+
+from tests._cases import params, poly_state, rel, seeded
+
+pytestmark = pytest.mark.gpu
+
+kmm = pytest.importorskip("polymer_dns_amd.kmm")
+
+POLY = dict(polymer=1, zlmax=30.0, tpoly=0.5, diffpoly=0.5, qbeta=0.9, crstrt=1, src_start=0)
+
+
+def channel(p, st):
+    cfg = kmm.make_config(p.nx, p.ny, p.nz, xl=p.xl, zl=p.zl, re=p.re, dt=p.dt, r_tau=p.R_tau, **POLY)
+    ch = kmm.Channel(cfg)
+    for k, v in st.items():
+        ch.put(k, v)
+    ch.startup()
+    return ch
+
+
+def collect(ch, steps, n, seed):
+    Xs, Ts = [], []
+    for s in range(steps):
+        ch.step(1)
+        X, T = ch.poly_samples(n, seed=seed + s)
+        Xs.append(X)
+        Ts.append(T)
+    return np.concatenate(Xs), np.concatenate(Ts)
+
+
+def test_closure_training_gain_and_run():
+    p = params(32, 32, 32, dt=0.005, polymer=True, crstrt=1, src_start=0)
+    st = seeded(p, 61, amp=0.3)
+    st.update(poly_state(p, 62, amp=0.05))
+    st = {k: v for k, v in st.items() if k in ("v", "omy", "u0", "w0") or k.startswith("c")}
+    ch = channel(p, st)
+    ch.set_poly_capture(True)
+    ch.step(2)
+    Xtr, Ttr = collect(ch, 24, 2000, seed=100)
+    Xte, Tte = collect(ch, 4, 2000, seed=900)
+    mx, sx = Xtr.mean(0), Xtr.std(0) + 1e-12
+    my, sy = Ttr.mean(0), Ttr.std(0) + 1e-12
+    net = kmm.Mlp(seed=3)
+    net.set_norm(mx, sx, my, sy)
+    untrained = net.infer(Xte)
+    losses = net.train(Xtr, Ttr, steps=1500, batch=512, lr=1e-3, seed=5)
+    pred = net.infer(Xte)
+
+    def mse(a):
+        return float((((a - Tte) / sy) ** 2).mean())
+
+    # targets are corrections to linear extrapolation: predicting 0 is the
+    # closure without the network; -(str(n-1) - str(n-2)) freezes the stress
+    m_net, m_extrap, m_untr = mse(pred), mse(np.zeros_like(Tte)), mse(untrained)
+    m_frozen = mse(-Xte[:, 16:22])
+    print(f"held-out normalised MSE: network {m_net:.3e}, linear extrapolation {m_extrap:.3e}, "
+          f"frozen stress {m_frozen:.3e}, untrained {m_untr:.3e}", flush=True)
+    assert np.isfinite(losses).all() and losses[-100:].mean() < 0.5 * losses[:20].mean()
+    assert m_net < 0.5 * m_extrap
+    assert m_net < 1e-3 * m_frozen
+
+    # closure run vs the FENE-P run from the same state, 6 steps
+    ref = channel(p, st)
+    ref.set_poly_capture(True)
+    ref.step(2 + 24 + 4)
+    ch.set_closure(net)
+    ch.step(6)
+    ref.step(6)
+    s12 = rel(ch.poly_stress(1), ref.poly_stress(1))
+    v = rel(ch.get("v"), ref.get("v"))
+    print(f"closure vs FENE-P after 6 steps: stress12 rel {s12:.3e}, v rel {v:.3e}", flush=True)
+    assert s12 < 0.05 and v < 1e-3
+    ch.close()
+    ref.close()
+    net.close()
