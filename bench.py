@@ -67,7 +67,7 @@ NPART = 100_000
 AP = 1e-3   # tracer radius: keeps the wall clamp of tracer_update inside the grid
 
 
-def polymer_mode(kmm, precision, steps, warmup):
+def polymer_mode(kmm, precision, steps, warmup, learned=False):
     from polymer_dns_amd import ic
     utau = RTAU / RE                    # friction velocity in bulk units (half height 1)
     tpoly = WI_TAU / (RE * utau * utau)   # Wi_tau = lambda u_tau^2 / nu
@@ -83,6 +83,28 @@ def polymer_mode(kmm, precision, steps, warmup):
     rng = np.random.default_rng(SEED + 4)
     ch.set_particles(rng.uniform(0, XL, NPART), rng.uniform(-0.99, 0.99, NPART), rng.uniform(0, ZL, NPART))
     ch.startup()
+    net = None
+    extra = {}
+    if learned:
+        # training pairs drawn by the GPU from its own FENE-P steps, GPU training,
+        # then the network replaces the conformation transport
+        ch.set_poly_capture(True)
+        ch.step(2)
+        Xs, Ts = [], []
+        for s in range(6):
+            ch.step(1)
+            X, T = ch.poly_samples(20000, seed=SEED + 10 + s)
+            Xs.append(X)
+            Ts.append(T)
+        X, T = np.concatenate(Xs), np.concatenate(Ts)
+        net = kmm.Mlp(seed=SEED + 5)
+        net.set_norm(X.mean(0), X.std(0) + 1e-12, T.mean(0), T.std(0) + 1e-12)
+        t0 = time.perf_counter()
+        losses = net.train(X, T, steps=400, batch=1024, lr=1e-3, seed=SEED + 6)
+        extra = {"train_s": round(time.perf_counter() - t0, 2), "train_loss_first_last":
+                 [round(float(losses[:10].mean()), 4), round(float(losses[-10:].mean()), 4)],
+                 "samples": int(X.shape[0])}
+        ch.set_closure(net)
     ch.step(warmup)
     ch.set_timing(True)
     t0 = time.perf_counter()
@@ -90,9 +112,13 @@ def polymer_mode(kmm, precision, steps, warmup):
     t = time.perf_counter() - t0
     ktimes, _ = ch.kernel_times()
     ch.close()
+    if net is not None:
+        net.close()
     G = PNX * (PNY + 1) * PNZ
-    return {"value": round(G * steps / t / 1e6, 2), "ms_per_step": round(1000.0 * t / steps, 4),
-            "kernel_ms": {k: round(v, 4) for k, v in ktimes.items()}}
+    out = {"value": round(G * steps / t / 1e6, 2), "ms_per_step": round(1000.0 * t / steps, 4),
+           "kernel_ms": {k: round(v, 4) for k, v in ktimes.items()}}
+    out.update(extra)
+    return out
 
 
 def kernel_bytes(G):
@@ -186,10 +212,13 @@ def main():
     if args.poly_steps > 0:
         pf = polymer_mode(kmm, 0, args.poly_steps, 5)
         pm = polymer_mode(kmm, 1, args.poly_steps, 5)
+        pl = polymer_mode(kmm, 1, args.poly_steps, 5, learned=True)
         out["polymer"] = {"workload": f"viscoelastic channel {PNX}x{PNY + 1}x{PNZ}, FENE-P Wi_tau {WI_TAU:g}, "
                                       f"beta {QBETA}, L^2 {ZLMAX ** 2:g}, Sc {SC}, {NPART} tracers",
-                          "unit": "Mgridpt-steps/s", "fp64": pf, "mixed": pm,
-                          "speedup_mixed_vs_fp64": round(pf["ms_per_step"] / pm["ms_per_step"], 3)}
+                          "unit": "Mgridpt-steps/s", "fp64": pf, "mixed": pm, "mixed_learned": pl,
+                          "speedup_mixed_vs_fp64": round(pf["ms_per_step"] / pm["ms_per_step"], 3),
+                          "speedup_learned_vs_fenep_fp64": round(pf["ms_per_step"] / pl["ms_per_step"], 3),
+                          "speedup_learned_vs_fenep_mixed": round(pm["ms_per_step"] / pl["ms_per_step"], 3)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

@@ -246,11 +246,12 @@ int kmm_fluid_interp(kmm_ctx* ctx, int n, const double* xp, const double* yp, co
 int kmm_get_physical(kmm_ctx* ctx, int comp, double* out);
 
 /* ---- learned polymer-stress closure (no reference counterpart) ----
- * A 22 -> 128 -> 128 -> 128 -> 128 -> 6 perceptron, GELU after the hidden
- * layers, bf16 weights and FP32 accumulation on the matrix cores.  Inputs per
- * point: the nine velocity gradients du_i/dx_j (row-major), the wall distance
- * 1 - |y|, the six stress components (11 12 13 22 23 33) of the two previous
- * steps; output: the six stress components of the current step.  Arrays are
+ * A 22 -> 128 -> 128 -> 128 -> 128 -> 6 perceptron, ReLU after the hidden
+ * layers, bf16 weights and FP32 accumulation on the matrix cores.  Inputs per point: the
+ * nine velocity gradients du_i/dx_j (row-major), the wall distance 1 - |y|,
+ * the six stress components (11 12 13 22 23 33) of the previous step and their
+ * change over the step before; output: six values (in the channel closure,
+ * the correction to linear extrapolation of the stress).  Arrays are
  * point-major: X[n][22], Y[n][6].  Parameters: FP32 master vector of
  * kmm_mlp_param_count() floats, W_l (n_out x n_in, row-major) then b_l for
  * each layer. */

@@ -1,13 +1,13 @@
 """CPU restatement (numpy) of the learned polymer-stress closure: a 22-input,
-4 x 128 GELU, 6-output multilayer perceptron (no reference counterpart; the
-network is new in this build).
+4 x 128 ReLU, 6-output multilayer perceptron (no reference counterpart; the
+network is new here).
 
 TEST INFRASTRUCTURE ONLY: only tests/ may import this module.
 
 Conventions shared with polymer_dns_amd/csrc/kmm_mlp.hip:
   * parameters: W[l] (n_out x n_in, row-major), b[l] (n_out); layer sizes
-    22 -> 128 -> 128 -> 128 -> 128 -> 6; GELU(x) = x/2 (1 + erf(x/sqrt 2))
-    after the four hidden layers, none after the last;
+    22 -> 128 -> 128 -> 128 -> 128 -> 6; ReLU after the four hidden layers,
+    none after the last;
   * input and output affine normalisation: x_n = (x - mu_x)/sd_x,
     y = y_n sd_y + mu_y;
   * "bf16" mode rounds weights and the input of every layer to bfloat16
@@ -16,7 +16,8 @@ Conventions shared with polymer_dns_amd/csrc/kmm_mlp.hip:
     (y_n - t_n)^2 with t_n = (t - mu_y)/sd_y.
 
 Pins: analytic gradients vs central finite differences (FP64); bf16
-rounding vs the bit pattern definition; GELU derivative vs differences.
+rounding vs the bit pattern definition; the activation's values and
+derivative vs differences away from the kink.
 """
 from __future__ import annotations
 
@@ -37,14 +38,14 @@ def bf16(x):
     return np.where(np.isfinite(a), out, a)
 
 
-def gelu(x):
-    from scipy.special import erf
-    return 0.5 * x * (1.0 + erf(x / math.sqrt(2.0)))
+def act(x):
+    """ReLU."""
+    return np.maximum(x, 0.0)
 
 
-def gelu_grad(x):
-    from scipy.special import erf
-    return 0.5 * (1.0 + erf(x / math.sqrt(2.0))) + x * np.exp(-0.5 * x * x) / math.sqrt(2.0 * math.pi)
+def act_grad(x):
+    """d act / dx (0 at the kink, as the GPU training kernel takes it)."""
+    return (x > 0).astype(np.asarray(x).dtype)
 
 
 def init_params(seed):
@@ -74,7 +75,7 @@ def forward(W, b, X, mu_x, sd_x, mu_y, sd_y, mode="bf16", keep=False):
             z = a_in @ w.T + np.asarray(b[l], dtype=dt)
         acts.append(a_in)
         pres.append(z)
-        a = gelu(z).astype(z.dtype) if l < NL - 1 else z
+        a = act(z).astype(z.dtype) if l < NL - 1 else z
     y = a * sd_y + mu_y
     return (y, acts, pres) if keep else y
 
@@ -90,7 +91,7 @@ def loss_and_grads(W, b, X, T, mu_x, sd_x, mu_y, sd_y, mode="fp64"):
     gW, gb = [None] * NL, [None] * NL
     for l in range(NL - 1, -1, -1):
         if l < NL - 1:
-            d = d * gelu_grad(pres[l])
+            d = d * act_grad(pres[l])
         gW[l] = d.T @ acts[l]
         gb[l] = d.sum(axis=0)
         if l > 0:

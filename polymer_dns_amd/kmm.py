@@ -450,7 +450,7 @@ def _fp(a):
 
 
 class Mlp:
-    """Learned polymer-stress closure (kmm_mlp_*): 22 inputs, 4 x 128 GELU,
+    """Learned polymer-stress closure (kmm_mlp_*): 22 inputs, 4 x 128 ReLU,
     6 outputs, bf16 weights with FP32 accumulation on the GPU."""
 
     SIZES = (22, 128, 128, 128, 128, 6)

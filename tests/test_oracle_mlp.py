@@ -1,5 +1,5 @@
 """Pins for the MLP-closure oracle (oracle/mlp_oracle.py): bfloat16 rounding
-against its bit-level definition, the GELU derivative and the analytic
+against its bit-level definition, the activation (closed form and derivative) and the analytic
 gradients against central finite differences."""
 import math
 
@@ -21,12 +21,16 @@ def test_bf16_round_to_nearest_even():
     assert M.bf16(one) == one
 
 
-def test_gelu_derivative():
-    x = np.linspace(-6, 6, 101)
+def test_activation_and_derivative():
+    x = np.array([-3.0, -0.5, 0.0, 0.5, 2.5])
+    assert np.array_equal(M.act(x), [0.0, 0.0, 0.0, 0.5, 2.5])
+    # bf16 rounding and the activation commute (the matrix-core path rounds first)
+    z = np.random.default_rng(0).standard_normal(1000).astype(np.float32)
+    assert np.array_equal(M.bf16(M.act(z)), M.act(M.bf16(z)))
+    xs = np.linspace(-3, 3, 120)          # no sample on the kink
     h = 1e-6
-    fd = (M.gelu(x + h) - M.gelu(x - h)) / (2 * h)
-    assert np.abs(fd - M.gelu_grad(x)).max() < 1e-8
-    assert abs(M.gelu(np.array([1.0]))[0] - 0.5 * (1 + math.erf(1 / math.sqrt(2)))) < 1e-15
+    fd = (M.act(xs + h) - M.act(xs - h)) / (2 * h)
+    assert np.abs(fd - M.act_grad(xs)).max() < 1e-8
 
 
 def test_gradients_match_finite_differences():
