@@ -287,6 +287,27 @@ int kmm_poly_samples(kmm_ctx* ctx, int n, unsigned long long seed, float* X, flo
 int kmm_set_closure(kmm_ctx* ctx, kmm_mlp* mlp);
 int kmm_get_poly_stress(kmm_ctx* ctx, int comp, double* out);
 
+/* Residual gate and audit steps of the learned closure (no reference
+ * counterpart).  With every > 0, step closure_start + m*every (m >= 1) opens
+ * an audit window of `window` (>= 3, < every) steps that run the full FENE-P
+ * transport instead of the network: C is rebuilt from the closure's last
+ * stress (the inverse of str = f C - I: f = (L^2 + tr str)/L^2 for ipeter 1,
+ * 1 otherwise) and stands for C at the window's first step; the next step
+ * starts the transport history afresh.  From the window's third step on the
+ * network predicts each step's stress from the FENE-P history and gradient;
+ * the relative residual ||pred - str|| / ||str|| over those steps is the gate:
+ * at or below `gate` the closure resumes from the FENE-P stress history, above
+ * it the run stays on FENE-P (fallback) until kmm_set_closure is called again.
+ * every = 0 turns audits off.  Needs the captures (kmm_set_poly_capture). */
+int kmm_set_closure_audit(kmm_ctx* ctx, int every, int window, double gate);
+/* active: 1 while the network carries the stress; audits: completed windows;
+ * last_residual: the last window's residual (-1 before the first);
+ * fallback_it: the first FENE-P step after a failed audit, -1 if none */
+int kmm_closure_status(kmm_ctx* ctx, int* active, int* audits, double* last_residual, int* fallback_it);
+/* C replaced by the tensor rebuilt from the last captured stress (the audit's
+ * restart of the transport); the pending plane outputs are kept */
+int kmm_poly_conf_from_stress(kmm_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

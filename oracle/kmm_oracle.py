@@ -836,6 +836,23 @@ def polymer_products(cp, dcp, up3, gup, p: Params):
     return cn, st, qp
 
 
+def conf_from_stress(st, p: Params):
+    """Inverse of the stress map of solvers.f90:2925-2949 (str = f C - I,
+    f = (L^2-3)/(L^2-trC) for ipeter 1, 1 for ipeter 0), the restart of the
+    transport in the learned closure's audit steps: with s = tr str,
+    f = (L^2 + s)/L^2 and C = (str + I)/f.  st: {ij: phys}; returns {ij: C}."""
+    tr = st["11"] + st["22"] + st["33"]
+    if p.ipeter == 0:
+        inv = 1.0
+    else:
+        L2 = p.zlmax * p.zlmax
+        inv = L2 / (L2 + tr)
+    out = {ij: st[ij] * inv for ij in POLY}
+    for ij in ("11", "22", "33"):
+        out[ij] = out[ij] + inv
+    return out
+
+
 def polyNL(cn, cnm1, strn, strnm1, p: Params):
     """helpers.f90:610-674 -- 3 (c_n - str_n/lambda) - (c_nm1 - str_nm1/lambda)."""
     ti = 1.0 / p.tpoly

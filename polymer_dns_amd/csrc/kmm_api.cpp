@@ -117,6 +117,18 @@ struct kmm_ctx {
   int hist_n = 0, hist_it = -1000;                // consecutive captured evaluations, last one
   kmm_mlp* closure = nullptr;                     // network replacing the conformation transport
   bool nlp_closure = false;                       // NLP holds only a closure stress (no c_n)
+  // audits of the closure: windows of full FENE-P transport every audit_every
+  // steps, restarted from C rebuilt from the closure's stress
+  bool closure_active = false;                    // false after a failed audit (FENE-P fallback)
+  int closure_it0 = 0;                            // first step of the current closure period
+  int audit_every = 0, audit_window = 0, audit_w = -1;  // audit_w: position in a running window
+  double audit_gate = 0.0, audit_last = -1.0;
+  int audits = 0, fallback_it = -1;
+  float* audit_pred = nullptr;                    // [6][Pphys] network prediction in a window
+  float* conf_phys = nullptr;                     // [6][Pphys] rebuilt C on the 3/2 grid
+  void* conf_nlp[6] = {};                         // its x-z transforms (physical y)
+  double* audit_part = nullptr;                   // residual partial sums
+  double* audit_acc = nullptr;                    // [2] residual sums of the running window
 };
 
 static void drop_graph(kmm_ctx* c);
@@ -796,6 +808,87 @@ static int run_closure_plane(kmm_ctx* c, int it, bool swirl) {
   return KMM_OK;
 }
 
+// buffers of the conformation rebuild and the audit residual (allocated once)
+static int audit_buffers(kmm_ctx* c) {
+  if (c->conf_phys) return KMM_OK;
+  const size_t P = (size_t)c->nyp * c->mz * c->mx;
+  const size_t tbv = c->S * c->vsz;
+  KMM_HIP(hipMalloc((void**)&c->conf_phys, 6 * P * sizeof(float)));
+  c->allocs.push_back(c->conf_phys);
+  KMM_HIP(hipMalloc((void**)&c->audit_pred, 6 * P * sizeof(float)));
+  c->allocs.push_back(c->audit_pred);
+  for (int q = 0; q < 6; ++q) {
+    KMM_HIP(hipMalloc(&c->conf_nlp[q], tbv));
+    c->allocs.push_back(c->conf_nlp[q]);
+  }
+  KMM_HIP(hipMalloc((void**)&c->audit_part, 2 * (size_t)kmm::audit_part_count() * sizeof(double)));
+  c->allocs.push_back(c->audit_part);
+  KMM_HIP(hipMalloc((void**)&c->audit_acc, 2 * sizeof(double)));
+  c->allocs.push_back(c->audit_acc);
+  return KMM_OK;
+}
+
+// C (spectral) rebuilt from a physical stress field on the 3/2 grid: the
+// inverse of the stress map pointwise, forward transforms in x and z, and the
+// polymer pencil's forward Chebyshev transform (mode 3) into C.  The pending
+// plane outputs in NLP are left untouched.
+static int rebuild_conf(kmm_ctx* c, const float* str) {
+  int r = audit_buffers(c);
+  if (r) return r;
+  const size_t P = (size_t)c->nyp * c->mz * c->mx;
+  const double L2 = c->cfg.zlmax * c->cfg.zlmax;
+  hipError_t e = kmm::launch_conf_from_stress(str, c->conf_phys, P, L2, c->cfg.ipeter, c->stream);
+  if (e != hipSuccess) return fail(KMM_E_HIP, std::string("conformation rebuild: ") + hipGetErrorString(e));
+  kmm::PolyPlaneArgs sa;
+  std::memset(&sa, 0, sizeof(sa));
+  sa.nxh = c->nxh; sa.nz = c->nz; sa.nzh = c->nzh; sa.nyp = c->nyp; sa.mx = c->mx; sa.mz = c->mz;
+  sa.mixed = c->cfg.precision == 1;
+  sa.alpha = c->alpha; sa.beta = c->beta;
+  sa.twZ = c->twZ; sa.twX = c->twX;
+  sa.ZFP = c->ZFP;
+  sa.caps = c->conf_phys;
+  for (int q = 0; q < 6; ++q) sa.out[6 + q] = c->conf_nlp[q];
+  if ((e = kmm::launch_closure_fwd(sa, c->stream)) != hipSuccess)
+    return fail(KMM_E_HIP, std::string("conformation rebuild transforms: ") + hipGetErrorString(e));
+  kmm::PolyArgs pa = poly_args(c);
+  for (int q = 0; q < 6; ++q) pa.NLP[q] = c->conf_nlp[q];
+  if ((e = kmm::launch_poly_pencil(c->ny, 3, pa, c->cfg.precision == 1, c->stream)) != hipSuccess)
+    return fail(KMM_E_HIP, std::string("conformation rebuild pencil: ") + hipGetErrorString(e));
+  return KMM_OK;
+}
+
+// network prediction of step it's stress from the FENE-P history of the
+// window, residual sums against the FENE-P stress just captured
+static int audit_compare(kmm_ctx* c, int it) {
+  const void* wb;
+  const float *par, *nrm;
+  int dev;
+  kmm_mlp_view(c->closure, &wb, &par, &nrm, &dev);
+  kmm::MlpFieldIO io;
+  io.P = (size_t)c->nyp * c->mz * c->mx;
+  io.plane_pts = c->mz * c->mx;
+  io.half = 0.5 * c->cfg.yl;
+  io.ycoord = c->ycoord_d;
+  io.g = c->capg;
+  io.h1 = c->hist[hslot(it - 1)];
+  io.h2 = c->hist[hslot(it - 2)];
+  io.out = c->audit_pred;
+  hipError_t e = kmm::launch_mlp_field(io, wb, par, nrm, c->stream);
+  if (e == hipSuccess)
+    e = kmm::launch_audit_residual(c->audit_pred, c->hist[hslot(it)], 6 * io.P, c->audit_part, c->audit_acc,
+                                   c->stream);
+  if (e != hipSuccess) return fail(KMM_E_HIP, std::string("closure audit: ") + hipGetErrorString(e));
+  return KMM_OK;
+}
+
+// first step after step `it` that opens an audit window: closure_it0 +
+// m * audit_every, m >= 1
+static int next_audit_from(const kmm_ctx* c, int it) {
+  const int e = c->audit_every;
+  const int d = it + 1 - c->closure_it0;
+  return c->closure_it0 + (d <= e ? e : ((d + e - 1) / e) * e);
+}
+
 static int run_poly_plane(kmm_ctx* c, int it) {
   kmm::PolyPlaneArgs a;
   std::memset(&a, 0, sizeof(a));
@@ -811,13 +904,14 @@ static int run_poly_plane(kmm_ctx* c, int it) {
   for (int q = 0; q < 6; ++q) { a.in[6 + q] = c->TP[q]; a.in[12 + q] = c->TP[q]; a.in[18 + q] = c->TP[6 + q]; }
   a.ZP = c->ZP; a.ZFP = c->ZFP;
   for (int q = 0; q < 12; ++q) a.out[q] = c->NLP[q];
-  if (c->poly_capture) {
+  const bool cap = c->poly_capture || c->audit_w >= 0;  // audit windows capture for the comparison
+  if (cap) {
     a.capg = c->capg;
     a.caps = c->hist[hslot(it)];
   }
   hipError_t e = kmm::launch_poly_plane(a, c->stream);
   if (e != hipSuccess) return fail(KMM_E_HIP, std::string("polymer plane: ") + hipGetErrorString(e));
-  if (c->poly_capture) hist_note(c, it);
+  if (cap) hist_note(c, it);
   return KMM_OK;
 }
 
@@ -1008,12 +1102,25 @@ static int enqueue_step(kmm_ctx* c, int s, hipEvent_t* ev, bool stage_ev) {
   a.first_nl = c->nl_valid ? 1 : 0;
   if (ev) hipEventRecord(ev[0], c->stream);
   else if (stage_ev) hipEventRecord(c->ev[0], c->stream);
-  const bool clo = c->closure != nullptr;
+  const bool att = c->closure != nullptr && c->closure_active;
+  if (att && c->audit_every > 0 && c->audit_w < 0 && c->it == next_audit_from(c, c->it - 1)) c->audit_w = 0;
+  const bool clo = att && c->audit_w < 0;
   kmm_ctx::PolyFlags pf = poly_flags(c, c->it);
   if (clo) {  // learned closure: no conformation transport; the stress comes from the network
     pf = kmm_ctx::PolyFlags();
     pf.evaluate = true;
     pf.force = true;
+  } else if (att && c->audit_w == 0) {
+    // audit window opens: C rebuilt from the closure's last stress stands for
+    // C at this step (no solve); the next step starts the transport history
+    // afresh (the n -> n-1 copy of an initial step)
+    int r = rebuild_conf(c, c->hist[hslot(c->it - 1)]);
+    if (r) return r;
+    KMM_HIP(hipMemsetAsync(c->audit_acc, 0, 2 * sizeof(double), c->stream));
+    pf = kmm_ctx::PolyFlags();
+    pf.evaluate = true;
+    pf.force = true;
+    pf.copy = true;
   }
   if (c->cfg.polymer) {
     kmm::PolyArgs pa = poly_args(c);
@@ -1039,6 +1146,24 @@ static int enqueue_step(kmm_ctx* c, int s, hipEvent_t* ev, bool stage_ev) {
   c->pprev = pf;
   c->nlp_closure = clo;
   c->nl_valid = true;
+  if (att && c->audit_w >= 0) {
+    if (c->audit_w >= 2 && (r = audit_compare(c, c->it))) return r;
+    if (++c->audit_w == c->audit_window) {
+      // the gate: relative residual of the network's one-step predictions
+      double acc[2] = {0.0, 0.0};
+      KMM_HIP(hipMemcpyAsync(acc, c->audit_acc, sizeof(acc), hipMemcpyDeviceToHost, c->stream));
+      KMM_HIP(hipStreamSynchronize(c->stream));
+      c->audit_last = acc[1] > 0.0 ? std::sqrt(acc[0] / acc[1]) : (acc[0] > 0.0 ? INFINITY : 0.0);
+      c->audits += 1;
+      c->audit_w = -1;
+      if (c->audit_last <= c->audit_gate) {
+        c->closure_it0 = c->it + 1;  // the closure resumes from the FENE-P stress history
+      } else {
+        c->closure_active = false;   // fallback: the transport carries on
+        c->fallback_it = c->it + 1;
+      }
+    }
+  }
   return KMM_OK;
 }
 
@@ -1047,7 +1172,11 @@ static int enqueue_step(kmm_ctx* c, int s, hipEvent_t* ev, bool stage_ev) {
 // a captured graph.
 static bool steady(const kmm_ctx* c, int k) {
   if (!c->nl_valid) return false;
-  if (c->closure) return c->nlp_closure && k % 3 == 0;  // history slots repeat every 3 steps
+  if (c->closure && c->closure_active) {
+    if (c->audit_every > 0 && (c->audit_w >= 0 || next_audit_from(c, c->it) <= c->it + k)) return false;
+    return c->nlp_closure && k % 3 == 0;  // history slots repeat every 3 steps
+  }
+  if (c->audit_w >= 0) return false;
   if (c->poly_capture) return false;                     // capture bookkeeping runs per launch
   if (c->npart > 0 && c->part_state != 2) return false;
   if (c->nsrc > 0 && c->it < c->cfg.perturbtime + 1) return false;
@@ -1899,7 +2028,50 @@ extern "C" int kmm_set_closure(kmm_ctx* c, kmm_mlp* mlp) {
       return fail(KMM_E_ARG, "the closure needs the stress of the last two steps (capture on for two steps)");
   }
   c->closure = mlp;
+  c->closure_active = mlp != nullptr;
+  c->closure_it0 = c->it + 1;
+  c->audit_w = -1;
+  c->fallback_it = -1;
   drop_graph(c);
+  return KMM_OK;
+}
+
+extern "C" int kmm_set_closure_audit(kmm_ctx* c, int every, int window, double gate) {
+  if (!c) return fail(KMM_E_ARG, "null argument");
+  if (every < 0 || (every > 0 && (window < 3 || window >= every)) || !(gate >= 0.0))
+    return fail(KMM_E_ARG, "audit: every = 0 (off) or window >= 3 and every > window; gate >= 0");
+  if (every > 0) {
+    if (!c->cfg.polymer) return fail(KMM_E_CONFIG, "audits need the polymer (polymer = 1)");
+    if (!c->capg) return fail(KMM_E_ARG, "audits need the captures (kmm_set_poly_capture)");
+    KMM_HIP(hipSetDevice(c->device));
+    int r = audit_buffers(c);
+    if (r) return r;
+  }
+  if (c->audit_w >= 0) return fail(KMM_E_ARG, "an audit window is running");
+  c->audit_every = every;
+  c->audit_window = window;
+  c->audit_gate = gate;
+  drop_graph(c);
+  return KMM_OK;
+}
+
+extern "C" int kmm_closure_status(kmm_ctx* c, int* active, int* audits, double* last_residual, int* fallback_it) {
+  if (!c || !active || !audits || !last_residual || !fallback_it) return fail(KMM_E_ARG, "null argument");
+  *active = (c->closure && c->closure_active) ? 1 : 0;
+  *audits = c->audits;
+  *last_residual = c->audit_last;
+  *fallback_it = c->fallback_it;
+  return KMM_OK;
+}
+
+extern "C" int kmm_poly_conf_from_stress(kmm_ctx* c) {
+  if (!c) return fail(KMM_E_ARG, "null argument");
+  if (!c->cfg.polymer) return fail(KMM_E_CONFIG, "needs the polymer (polymer = 1)");
+  if (!c->capg || c->hist_n < 1) return fail(KMM_E_ARG, "no captured stress");
+  KMM_HIP(hipSetDevice(c->device));
+  int r = rebuild_conf(c, c->hist[hslot(c->hist_it)]);
+  if (r) return r;
+  KMM_HIP(hipStreamSynchronize(c->stream));
   return KMM_OK;
 }
 

@@ -68,6 +68,8 @@ struct PolyArgs {
   double2* TF[3];
   void* TP[12];         // transfer: C (6), D C (6)
 };
+// mode 0: step, 1: startup (evaluate), 2: finish (part B), 3: C from the
+// physical-y fields in NLP[0..5] (forward Chebyshev transform only)
 hipError_t launch_poly_pencil(int ny, int mode, const PolyArgs& a, bool mixed, hipStream_t s);
 
 // Plane-stage arguments.
@@ -174,6 +176,13 @@ struct TracerArgs {
   int* err;                 // set to 1 when a particle leaves the grid
 };
 hipError_t launch_tracer(const TracerArgs& a, hipStream_t s);
+// closure audits (kmm_audit.hip): C = (str + I)/f over P points of [6][P]
+// arrays; residual sums acc[0] += sum (pred - ref)^2, acc[1] += sum ref^2
+// over n values (part: 2 * audit_part_count() doubles)
+hipError_t launch_conf_from_stress(const float* str, float* C, size_t P, double L2, int ipeter, hipStream_t s);
+int audit_part_count();
+hipError_t launch_audit_residual(const float* pred, const float* ref, size_t n, double* part, double* acc,
+                                 hipStream_t s);
 // learned closure (kmm_mlp.hip)
 int mlp_param_count();
 int mlp_wb_count();

@@ -18,7 +18,9 @@
 //           y-derivatives, derivatives.f90:148-254; the x and z derivatives
 //           are taken in the plane stage).
 //
-// MODE 0: step, MODE 1: startup (evaluate only), MODE 2: finish (part B only).
+// MODE 0: step, MODE 1: startup (evaluate only), MODE 2: finish (part B only),
+// MODE 3: C from the physical-y fields in NLP[0..5] (a conformation tensor
+// rebuilt on the 3/2 grid and forward transformed in x and z).
 #include "kmm_kernels.h"
 #include "kmm_pencil.h"
 
@@ -43,6 +45,18 @@ __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
   const bool is00 = (kxi == 0 && kzi == 0);
   const double dy = A.dyde;
 
+  if (MODE == 3) {
+#pragma unroll 1
+    for (int c = 0; c < 6; ++c) {
+      PF C = pf_zero();
+      if (kzi != A.nzh) {
+        C = phys_to_cheb_load<NY, V>(static_cast<const V*>(A.NLP[c]), P, p, buf, scr, twl, lane);
+        if (is00) C = pf_real(C);
+      }
+      pf_store<NY>(A.C[c], P, p, lane, C);
+    }
+    return;
+  }
   if (kzi == A.nzh) {  // norm zeroes the z-Nyquist row (helpers.f90:149-156)
     const PF z = pf_zero();
     for (int c = 0; c < 6; ++c) {
@@ -165,6 +179,7 @@ hipError_t launch_poly_pencil(int ny, int mode, const PolyArgs& a, bool mixed, h
     case 0: return mixed ? launch_poly_v<0, float2>(ny, a, s) : launch_poly_v<0, double2>(ny, a, s);
     case 1: return mixed ? launch_poly_v<1, float2>(ny, a, s) : launch_poly_v<1, double2>(ny, a, s);
     case 2: return mixed ? launch_poly_v<2, float2>(ny, a, s) : launch_poly_v<2, double2>(ny, a, s);
+    case 3: return mixed ? launch_poly_v<3, float2>(ny, a, s) : launch_poly_v<3, double2>(ny, a, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -47,6 +47,7 @@ EXPORTS = [
     "kmm_mlp_param_count", "kmm_mlp_create", "kmm_mlp_destroy", "kmm_mlp_set_params", "kmm_mlp_get_params",
     "kmm_mlp_set_norm", "kmm_mlp_infer", "kmm_mlp_grad", "kmm_mlp_train",
     "kmm_set_poly_capture", "kmm_poly_samples", "kmm_set_closure", "kmm_get_poly_stress",
+    "kmm_set_closure_audit", "kmm_closure_status", "kmm_poly_conf_from_stress",
 ]
 
 
@@ -146,6 +147,9 @@ def load_library(path: str | None = None):
     lib.kmm_poly_samples.argtypes = [C.c_void_p, C.c_int, C.c_ulonglong, fp, fp]
     lib.kmm_set_closure.argtypes = [C.c_void_p, C.c_void_p]
     lib.kmm_get_poly_stress.argtypes = [C.c_void_p, C.c_int, dp]
+    lib.kmm_set_closure_audit.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double]
+    lib.kmm_closure_status.argtypes = [C.c_void_p, ip, ip, dp, ip]
+    lib.kmm_poly_conf_from_stress.argtypes = [C.c_void_p]
     lib.kmm_calcswirl.argtypes = [C.c_void_p] + [dp] * 10
     lib.kmm_set_dpdx.argtypes = [C.c_void_p, C.c_double]
     for n in EXPORTS:
@@ -426,6 +430,22 @@ class Channel:
     def set_closure(self, mlp):
         _check(self.lib, self.lib.kmm_set_closure(self.h, mlp.h if mlp is not None else None))
         self._closure = mlp   # keep the network alive while the channel uses it
+
+    def set_closure_audit(self, every, window=4, gate=0.1):
+        """Audit windows of full FENE-P transport every `every` steps (0: off);
+        the network's residual over a window gates the closure (above `gate`
+        the run falls back to FENE-P)."""
+        _check(self.lib, self.lib.kmm_set_closure_audit(self.h, int(every), int(window), float(gate)))
+
+    def closure_status(self):
+        a, n, f = C.c_int(), C.c_int(), C.c_int()
+        r = C.c_double()
+        _check(self.lib, self.lib.kmm_closure_status(self.h, C.byref(a), C.byref(n), C.byref(r), C.byref(f)))
+        return {"active": bool(a.value), "audits": n.value, "residual": r.value, "fallback_it": f.value}
+
+    def conf_from_stress(self):
+        """Replace C by the tensor rebuilt from the last captured stress."""
+        _check(self.lib, self.lib.kmm_poly_conf_from_stress(self.h))
 
     def poly_stress(self, comp):
         out = np.empty((self.mx, self.mz, self.nyp))

@@ -484,6 +484,33 @@ def test_polymer_identity_at_rest_and_advection_sign():
         assert np.abs(cn[ij] + (U[0] * d[0] + U[1] * d[1] + U[2] * d[2])).max() < 1e-14
 
 
+@pytest.mark.parametrize("ipeter", [0, 1])
+def test_conformation_rebuild_inverts_the_stress_map(ipeter):
+    """conf_from_stress undoes polymer_products' str = f C - I on random SPD
+    tensors with tr C < L^2 (the map is one-to-one there), and the identity
+    (str = 0 at rest for ipeter 0) comes back as the identity."""
+    p = params(8, 8, 8, polymer=True)
+    p.ipeter = ipeter
+    p.zlmax = 10.0
+    rng = np.random.default_rng(11)
+    shape = (3, 4, 5)
+    A = rng.standard_normal(shape + (3, 3)) * 0.8
+    S = np.eye(3) + np.einsum("...ij,...kj->...ik", A, A)   # SPD, trace well below L^2 = 100
+    idx = {"11": (0, 0), "12": (0, 1), "13": (0, 2), "22": (1, 1), "23": (1, 2), "33": (2, 2)}
+    C = {ij: S[..., a, b] for ij, (a, b) in idx.items()}
+    zero = tuple(np.zeros(shape) for _ in range(3))
+    gup = [[np.zeros(shape)] * 3 for _ in range(3)]
+    _, st, _ = O.polymer_products(C, {ij: zero for ij in O.POLY}, zero, gup, p)
+    back = O.conf_from_stress(st, p)
+    for ij in O.POLY:
+        assert np.abs(back[ij] - C[ij]).max() < 1e-12 * np.abs(C[ij]).max(), ij
+    I = {ij: np.full(shape, 1.0 if ij in ("11", "22", "33") else 0.0) for ij in O.POLY}
+    _, st0, _ = O.polymer_products(I, {ij: zero for ij in O.POLY}, zero, gup, p)
+    back0 = O.conf_from_stress(st0, p)
+    for ij in O.POLY:
+        assert np.abs(back0[ij] - I[ij]).max() < 1e-14
+
+
 def test_polyforce_is_the_divergence_of_an_analytic_tensor():
     """t_i = d q_ik / d x_k for q11 = cos(a x) T2(y), q12 = sin(b z) T3(y),
     q13 = cos(a x + b z), q22 = T4(y), q23 = sin(a x) T1(y), q33 = cos(2 b z) T2(y)."""
