@@ -52,7 +52,8 @@ typedef struct kmm_config {
   int irstrt, crstrt;     /* restart flags */
   int flow_select;        /* 1 channel, 2 2-D Couette, 3 3-D Couette */
   int scalar;             /* 1 = SCALAR build (passive scalar) */
-  int scl_flag;           /* 0,1,3: no source; 6: uniform source deltaT */
+  int scl_flag;           /* 0,1,3: no source; 2: Gaussian blobs of rate deltaT at the
+                             particles between src_start and src_stop; 6: uniform source deltaT */
   double diff;            /* DIFF (Prandtl/Schmidt number) */
   double deltaT;          /* scalar source rate for scl_flag 6 */
   int src_start, src_stop;
@@ -80,6 +81,8 @@ typedef struct kmm_config {
   double ubulk;
   int particle_swirl;     /* 1: swirl strength at the particles every step (swirl_part,
                              solvers.f90:3479-3481); kmm_config_read sets 1 */
+  double sigmax, sigmay, sigmaz;  /* Gaussian blob widths of the scl_flag 2 source
+                                     (dns.config line 65) */
 } kmm_config;
 
 typedef struct kmm_ctx kmm_ctx;

@@ -64,6 +64,10 @@ class Params:
     diff: float = 1.0
     scl_flag: int = 0
     deltaT: float = 0.0
+    # scl_flag 2: Gaussian blobs at the particles (dns.config line 65)
+    sigmax: float = 0.0
+    sigmay: float = 0.0
+    sigmaz: float = 0.0
     # polymer (POLYMER build, FENE-P; dns.config lines 35-37, 55-62, 66)
     polymer: bool = False
     zlmax: float = 30.0       # L (maximum extensibility); Peterlin uses L^2
@@ -77,6 +81,7 @@ class Params:
     itarget: int = 0          # 0: uniform beta = qbeta
     ipeter: int = 1           # 1: FENE-P Peterlin function, 0: Oldroyd-B
     src_start: int = 0
+    src_stop: int = 10 ** 9   # last step of the scl_flag 2 source window
     irstrt: int = 1
     crstrt: int = 0
     # constant mass flux (extension, no reference counterpart): dPdx is
@@ -724,9 +729,35 @@ class CFLError(RuntimeError):
     pass
 
 
-def vcw3dp(st, p: Params, it, fields):
+def scalar_blob_source(part, p: Params):
+    """solvers.f90:2078-2118 -- scsource(i,j,k) = sum over particles of
+    deltaT exp(-(betax + betay + betaz)), betax = (x_k - x_n)^2/(2 sigmax^2)
+    with x_k = (k-1) delxm (no periodic wrap), betaz likewise in z, betay with
+    ycoord(i); a term counts only where betax < 20, betax + betaz < 20 and
+    betax + betay + betaz < 18.  Returns the (mx, mz, nyp) physical field."""
+    delxm = p.xl / p.mx
+    delzm = p.zl / p.mz
+    xk = np.arange(p.mx) * delxm
+    zj = np.arange(p.mz) * delzm
+    src = np.zeros((p.mx, p.mz, p.nyp))
+    for xc, yc, zc in zip(part["x"], part["y"], part["z"]):
+        bx = (xk - xc) ** 2 / (2.0 * p.sigmax ** 2)
+        bz = (zj - zc) ** 2 / (2.0 * p.sigmaz ** 2)
+        by = (p.ycoord - yc) ** 2 / (2.0 * p.sigmay ** 2)
+        ok = (bx[:, None] < 20.0) & ((bx[:, None] + bz[None, :]) < 20.0)
+        b = (bx[:, None, None] + by[None, None, :]) + bz[None, :, None]
+        keep = ok[:, :, None] & (b < 18.0)
+        src += np.where(keep, p.deltaT * np.exp(-np.where(keep, b, 0.0)), 0.0)
+    return src
+
+
+def vcw3dp(st, p: Params, it, fields, particles=None):
     """Returns (gn, fn, omz, scn) = spectral (u x omega)_x, _y, _z and the
-    scalar nonlinear term, and updates the IB integrals in st."""
+    scalar nonlinear term, and updates the IB integrals in st.  With
+    scl_flag 2 inside [src_start, src_stop] the blob source at `particles`
+    enters as scsource/dt (solvers.f90:3061-3063); outside the window the
+    reference reads an uninitialised local array, restated here as no
+    source."""
     u, v, w, omx, omy, omz = (fields[k] for k in ("u", "v", "w", "omx", "omy", "omz"))
     up, vp, wp = to_physical(u, p), to_physical(v, p), to_physical(w, p)
     wxp, wyp, wzp = to_physical(omx, p), to_physical(omy, p), to_physical(omz, p)
@@ -748,6 +779,8 @@ def vcw3dp(st, p: Params, it, fields):
         vc = -(up * cxp + vp * cyp + wp * czp)
         if p.scl_flag == 6:
             vc = vc + p.deltaT
+        if p.scl_flag == 2 and particles is not None and p.src_start <= it <= p.src_stop:
+            vc = vc + scalar_blob_source(particles, p) / p.dt
     imat = st.get("imatrix")
     if p.flow_select == 1:
         if imat is None:
@@ -1020,7 +1053,7 @@ class Solver:
             fields["gradv"] = gradv(u, v, w, p)
         if self.imatrix is not None:
             st["imatrix"] = self.imatrix
-        gn, fn, omz_o, scn, phys = vcw3dp(st, p, it, fields)
+        gn, fn, omz_o, scn, phys = vcw3dp(st, p, it, fields, self.particles)
         st.pop("imatrix", None)
         self.last_phys = phys
         if self.particles is not None:

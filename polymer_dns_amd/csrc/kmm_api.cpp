@@ -86,6 +86,7 @@ struct kmm_ctx {
   int npart = 0;
   int part_cap = 0;
   double* part = nullptr;    // [7][part_cap]: x, y, z, u, v, w, swirl
+  double* scsrc = nullptr;   // [nyp][mz][mx] Gaussian-blob scalar source (scl_flag 2)
   double* cap = nullptr;     // [4][nyp][mz][mx] physical u, v, w (and swirl) of the last evaluation
   bool part_swirl = false;   // swirl strength at the particles (solvers.f90:3479-3481)
   double* ycoord_d = nullptr;
@@ -209,6 +210,11 @@ extern "C" int kmm_config_read(const char* path, kmm_config* cfg) {
   RD(21, perturbtime, int); RD(22, r_tau, double); RD(29, irstrt, int); RD(30, crstrt, int);
   RD(34, flow_select, int); RD(38, scl_flag, int); RD(63, diff, double); RD(64, deltaT, double);
   RD(66, src_start, int); RD(67, src_stop, int); RD(74, cadence, int);
+  {
+    double sg[3];
+    if (!get(65, sg, 3)) return fail(KMM_E_IO, "dns.config line 65");
+    cfg->sigmax = sg[0]; cfg->sigmay = sg[1]; cfg->sigmaz = sg[2];
+  }
   RD(33, particle_flag, int); RD(46, ratio, double); RD(47, ap, double); RD(48, c_mu, double);
   RD(49, cd_switch, int);
   RD(35, itarget, int); RD(36, ipeter, int); RD(37, ipolyflag, int);
@@ -349,8 +355,11 @@ extern "C" int kmm_create(const kmm_config* cfg, int device, kmm_ctx** out) {
   if (!(cfg->flow_select == 1 || cfg->flow_select == 2 || cfg->flow_select == 3 || cfg->flow_select == 11 ||
         cfg->flow_select == 12))
     return fail(KMM_E_CONFIG, "flow_select must be a wall-bounded case with nt=nb=1 (1,2,3,11,12)");
-  if (cfg->scalar && !(cfg->scl_flag == 0 || cfg->scl_flag == 1 || cfg->scl_flag == 3 || cfg->scl_flag == 6))
-    return fail(KMM_E_CONFIG, "scl_flag must be 0, 1, 3 or 6 (uniform source)");
+  if (cfg->scalar && !(cfg->scl_flag == 0 || cfg->scl_flag == 1 || cfg->scl_flag == 2 || cfg->scl_flag == 3 ||
+                      cfg->scl_flag == 6))
+    return fail(KMM_E_CONFIG, "scl_flag must be 0, 1, 2 (blobs at the particles), 3 or 6 (uniform source)");
+  if (cfg->scalar && cfg->scl_flag == 2 && !(cfg->sigmax > 0.0 && cfg->sigmay > 0.0 && cfg->sigmaz > 0.0))
+    return fail(KMM_E_CONFIG, "scl_flag 2 needs positive sigmax, sigmay, sigmaz");
   if (cfg->polymer) {
     if (cfg->itarget != 0) return fail(KMM_E_CONFIG, "polymer: only itarget 0 (uniform beta) is supported");
     if (!(cfg->tpoly > 0.0) || !(cfg->diffpoly > 0.0) || !(cfg->qbeta > 0.0 && cfg->qbeta <= 1.0))
@@ -416,6 +425,8 @@ extern "C" int kmm_create(const kmm_config* cfg, int device, kmm_ctx** out) {
       if ((r = dalloc(c, &c->ZFP, 12 * zb))) goto fail_;
       if ((r = dalloc(c, (void**)&c->tabC, tb))) goto fail_;
     }
+    if (cfg->scalar && cfg->scl_flag == 2)
+      if ((r = dalloc(c, (void**)&c->scsrc, (size_t)c->nyp * c->mz * c->mx * sizeof(double)))) goto fail_;
     if ((r = dalloc(c, (void**)&c->cfl_d, 4096 * sizeof(unsigned long long)))) goto fail_;
     c->cfl_cap = 4096;
     // twiddles and smoothing
@@ -932,11 +943,31 @@ static kmm::TracerArgs tracer_args(kmm_ctx* c) {
 // One evaluation of vcw3dp's plane stage.  With track set and particles
 // present, the physical velocity is kept and the tracers are initialised or
 // advanced (part_track, solvers.f90:3463-3466).
+// The scl_flag 2 source is computed for src_start <= it <= src_stop; outside
+// that window the reference adds its uninitialised local array, here nothing.
+static bool blob_active(const kmm_ctx* c, int it) {
+  return c->scsrc && it >= c->cfg.src_start && it <= c->cfg.src_stop;
+}
+
 static int run_plane(kmm_ctx* c, int it, unsigned long long* cfl, hipEvent_t* ev = nullptr, bool track = false,
                      bool poly = false, bool closure = false) {
   PlaneArgs pa = plane_args(c, it, cfl);
   const bool trk = track && c->npart > 0;
   if (trk) pa.cap = c->cap;
+  if (blob_active(c, it)) {
+    // solvers.f90:2078-2118 at the particle positions before this step's update
+    kmm::BlobArgs b;
+    b.n = c->npart; b.mx = c->mx; b.mz = c->mz; b.nyp = c->nyp;
+    b.delxm = c->cfg.xl / c->mx; b.delzm = c->cfg.zl / c->mz;
+    b.sigmax = c->cfg.sigmax; b.sigmay = c->cfg.sigmay; b.sigmaz = c->cfg.sigmaz;
+    b.deltaT = c->cfg.deltaT;
+    b.ycoord = c->ycoord_d;
+    b.part = c->part;
+    b.src = c->scsrc;
+    hipError_t eb = kmm::launch_blob_source(b, c->stream);
+    if (eb != hipSuccess) return fail(KMM_E_HIP, std::string("scalar source: ") + hipGetErrorString(eb));
+    pa.scsrc = c->scsrc;
+  }
   hipError_t e = ev ? kmm::launch_plane_timed(pa, c->stream, ev) : kmm::launch_plane(pa, c->stream);
   if (e != hipSuccess) return fail(KMM_E_HIP, std::string("plane stage: ") + hipGetErrorString(e));
   if (poly) {
@@ -1180,6 +1211,10 @@ static bool steady(const kmm_ctx* c, int k) {
   if (c->poly_capture) return false;                     // capture bookkeeping runs per launch
   if (c->npart > 0 && c->part_state != 2) return false;
   if (c->nsrc > 0 && c->it < c->cfg.perturbtime + 1) return false;
+  if (c->scsrc) {  // the source window must not open or close inside the graph
+    const int a = c->it + 1, b = c->it + k, s0 = c->cfg.src_start, s1 = c->cfg.src_stop;
+    if ((s0 > a && s0 <= b) || (s1 >= a && s1 < b)) return false;
+  }
   if (c->cfg.polymer) {
     for (int j = 0; j <= k; ++j) {
       const kmm_ctx::PolyFlags f = poly_flags(c, c->it + j);

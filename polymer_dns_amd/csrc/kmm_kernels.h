@@ -103,6 +103,7 @@ struct PlaneArgs {
   const int* fixrow;      // [nyp*mz+1] solid cells per (y,z) row (no -dPdx)
   const int* fixx;        // x index per solid cell
   double* cap;            // nullptr, or [3][nyp][mz][mx] physical u, v, w (FP64)
+  const double* scsrc;    // nullptr, or [nyp][mz][mx] blob source of the scalar (added / dt)
   unsigned dzmask;        // k_zinv: bit f set -> input f is multiplied by i kz first
   double beta;            // z wavenumber unit
 };
@@ -176,6 +177,16 @@ struct TracerArgs {
   int* err;                 // set to 1 when a particle leaves the grid
 };
 hipError_t launch_tracer(const TracerArgs& a, hipStream_t s);
+// scalar source of Gaussian blobs at the particles (scl_flag 2), summed into
+// src[nyp][mz][mx] (cleared first)
+struct BlobArgs {
+  int n, mx, mz, nyp;
+  double delxm, delzm, sigmax, sigmay, sigmaz, deltaT;
+  const double* ycoord;
+  const double* part;       // [6][n]: x, y, z first
+  double* src;
+};
+hipError_t launch_blob_source(const BlobArgs& a, hipStream_t s);
 // closure audits (kmm_audit.hip): C = (str + I)/f over P points of [6][P]
 // arrays; residual sums acc[0] += sum (pred - ref)^2, acc[1] += sum ref^2
 // over n values (part: 2 * audit_part_count() doubles)

@@ -120,6 +120,53 @@ hipError_t launch_swirl(int n, const double* g, double* swirl, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Scalar source of Gaussian blobs at the particles (scl_flag 2,
+// solvers.f90:2078-2118): scsource(i,j,k) = sum_n deltaT exp(-(bx + by + bz)),
+// bx = (x_k - x_n)^2 / (2 sigmax^2) with x_k = k delxm (no periodic wrap), and
+// the reference's cut-offs bx < 20, bx + bz < 20, bx + by + bz < 18.  One
+// wavefront per particle scatters into its box (FP64 atomics: the sum order
+// over particles is free, as in the reference's OpenMP reduction); lanes run
+// over y.
+__global__ void k_fill0(double* __restrict__ p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+__global__ __launch_bounds__(64) void k_blob(BlobArgs A) {
+  const int q = blockIdx.x, lane = threadIdx.x;
+  const double xc = A.part[q], yc = A.part[A.n + q], zc = A.part[2 * A.n + q];
+  const double sx = 2.0 * (A.sigmax * A.sigmax), sy = 2.0 * (A.sigmay * A.sigmay);
+  const double sz = 2.0 * (A.sigmaz * A.sigmaz);
+  // index boxes holding every bx < 20 (bz < 20): |x_k - x_n| < sqrt(40) sigmax,
+  // widened by one cell; the exact tests below decide
+  const double rx = sqrt(40.0) * A.sigmax, rz = sqrt(40.0) * A.sigmaz;
+  auto clampi = [](double v, int hi) { return (int)fmin(fmax(v, 0.0), (double)hi); };
+  const int k0 = clampi(floor((xc - rx) / A.delxm) - 1.0, A.mx - 1), k1 = clampi(ceil((xc + rx) / A.delxm) + 1.0, A.mx - 1);
+  const int j0 = clampi(floor((zc - rz) / A.delzm) - 1.0, A.mz - 1), j1 = clampi(ceil((zc + rz) / A.delzm) + 1.0, A.mz - 1);
+  for (int k = k0; k <= k1; ++k) {
+    const double dx = (double)k * A.delxm - xc;
+    const double bx = (dx * dx) / sx;
+    if (!(bx < 20.0)) continue;
+    for (int j = j0; j <= j1; ++j) {
+      const double dz = (double)j * A.delzm - zc;
+      const double bz = (dz * dz) / sz;
+      if (!(bx + bz < 20.0)) continue;
+      for (int i = lane; i < A.nyp; i += 64) {
+        const double dy = A.ycoord[i] - yc;
+        const double b = (bx + (dy * dy) / sy) + bz;
+        if (b < 18.0) atomicAdd(&A.src[((size_t)i * A.mz + j) * A.mx + k], A.deltaT * exp(-b));
+      }
+    }
+  }
+}
+
+hipError_t launch_blob_source(const BlobArgs& a, hipStream_t s) {
+  const size_t n = (size_t)a.nyp * a.mz * a.mx;
+  hipLaunchKernelGGL(k_fill0, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a.src, n);
+  if (a.n > 0) hipLaunchKernelGGL(k_blob, dim3(a.n), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_tracer(const TracerArgs& a, hipStream_t s) {
   if (a.n <= 0) return hipSuccess;
   if (a.part && a.nf != 3 && a.nf != 4) return hipErrorInvalidValue;

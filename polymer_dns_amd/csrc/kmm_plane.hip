@@ -178,6 +178,8 @@ __global__ __launch_bounds__(WPB * 64) void k_xplane(PlaneArgs A) {
         const T tx = rr ? ph[6][m].y : ph[6][m].x, ty = rr ? ph[7][m].y : ph[7][m].x;
         const T tz = rr ? ph[8][m].y : ph[8][m].x;
         vc = -(u * tx + v * ty + w * tz) + src_u;
+        if (A.scsrc && ok)  // vc += scsource/dt (solvers.f90:3061-3063)
+          vc += (T)(A.scsrc[((size_t)y * A.mz + (rr ? zB : zA)) * MX + lane + 64 * m] / A.dt);
       }
       const double c = (double)((fabs(u) / delxm + fabs(v) / sgy + fabs(w) / delzm) * dt);
       if (ok) cmax = fmax(cmax, c);

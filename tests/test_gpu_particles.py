@@ -171,3 +171,36 @@ def test_bad_particle_file_is_an_io_error(tmp_path):
         with pytest.raises(kmm.KmmError) as e:
             ch.read_particles(bad)
         assert e.value.code == kmm.KMM_E_IO
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_blob_scalar_source_matches_oracle(precision):
+    """scl_flag 2: Gaussian blobs of rate deltaT at the tracers feed the
+    scalar between src_start and src_stop (solvers.f90:2078-2118,
+    3061-3063); the run crosses both ends of the window."""
+    p = params(32, 32, 32, scalar=True, dt=0.01, diff=0.71, scl_flag=2, deltaT=0.4,
+               sigmax=0.5, sigmay=0.08, sigmaz=0.3, src_start=2, src_stop=4)
+    print(f"case: blob source, 32^3, precision {precision}", flush=True)
+    st = seeded(p, 23, amp=0.2)
+    rng = np.random.default_rng(24)
+    x, y, z = random_points(p, 40, rng, wall=0.9)
+    ch = channel(p, precision=precision, ap=0.005, cd_switch=0, scalar=1, diff=p.diff, scl_flag=2,
+                 deltaT=p.deltaT, sigmax=p.sigmax, sigmay=p.sigmay, sigmaz=p.sigmaz,
+                 src_start=p.src_start, src_stop=p.src_stop)
+    for k in ("v", "omy", "u0", "w0", "scalar"):
+        ch.put(k, st[k])
+    ch.set_particles(x, y, z)
+    ch.startup()
+    sol = O.Solver(p)
+    sol.swirl_fn = O.swirl_eig
+    sol.set_particles(x, y, z, ap=0.005)
+    ost = {k: v.copy() for k, v in st.items()}
+    sol.startup(ost)
+    tol = 1e-10 if precision == 0 else 1e-4
+    for step in range(6):
+        ch.step(1)
+        sol.step(ost)
+        errs = {k: rel(ch.get(k), ost[k]) for k in ("scalar", "scn", "v")}
+        print(f"  step {step + 1}:", {k: f"{e:.2e}" for k, e in errs.items()}, flush=True)
+        assert max(errs.values()) < tol, (step, errs)
+    ch.close()

@@ -484,6 +484,34 @@ def test_polymer_identity_at_rest_and_advection_sign():
         assert np.abs(cn[ij] + (U[0] * d[0] + U[1] * d[1] + U[2] * d[2])).max() < 1e-14
 
 
+def test_blob_source_node_values_cutoffs_and_no_wrap():
+    """scalar_blob_source: deltaT at a particle sitting on a node, the
+    Gaussian factor one cell away, nothing where 18 <= beta (cut-off), sums
+    over particles, and no periodic wrap in x."""
+    p = params(16, 16, 16, scalar=True, scl_flag=2, deltaT=0.7)
+    dx, dz = p.xl / p.mx, p.zl / p.mz
+    p.sigmax, p.sigmay, p.sigmaz = 0.9 * dx, 0.05, 0.8 * dz
+    i0, j0, k0 = 5, 3, 7
+    part = {"x": np.array([k0 * dx]), "y": np.array([p.ycoord[i0]]), "z": np.array([j0 * dz])}
+    s = O.scalar_blob_source(part, p)
+    assert s[k0, j0, i0] == p.deltaT
+    assert abs(s[k0 + 1, j0, i0] - p.deltaT * math.exp(-dx * dx / (2 * p.sigmax ** 2))) < 1e-15
+    assert abs(s[k0, j0 + 1, i0] - p.deltaT * math.exp(-dz * dz / (2 * p.sigmaz ** 2))) < 1e-15
+    # x offsets of 5 and 6 cells: beta 15.4 (kept) and 22.2 (beyond the x cut-off)
+    assert s[k0 + 5, j0, i0] > 0 and s[k0 + 6, j0, i0] == 0.0
+    # a node with 18 <= bx + bz < 20 is cut although its Gaussian is not 0
+    bx, bz = 5 * 5 / (2 * 0.81), 2 * 2 / (2 * 0.64)
+    assert 18.0 <= bx + bz < 20.0 and s[k0 + 5, j0 + 2, i0] == 0.0 and s[k0 + 5, j0 + 1, i0] > 0
+    # superposition
+    two = {k: np.concatenate([v, v + (0.0 if k == "y" else 1.0)]) for k, v in part.items()}
+    one_b = {k: v + (0.0 if k == "y" else 1.0) for k, v in part.items()}
+    assert np.abs(O.scalar_blob_source(two, p) - s - O.scalar_blob_source(one_b, p)).max() < 1e-15
+    # a blob at x = 0 does not reach the last x node (no periodic wrap)
+    edge = {"x": np.array([0.0]), "y": part["y"], "z": part["z"]}
+    se = O.scalar_blob_source(edge, p)
+    assert se[0, j0, i0] == p.deltaT and se[p.mx - 1, j0, i0] == 0.0
+
+
 @pytest.mark.parametrize("ipeter", [0, 1])
 def test_conformation_rebuild_inverts_the_stress_map(ipeter):
     """conf_from_stress undoes polymer_products' str = f C - I on random SPD
