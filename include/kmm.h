@@ -311,6 +311,17 @@ int kmm_closure_status(kmm_ctx* ctx, int* active, int* audits, double* last_resi
  * restart of the transport); the pending plane outputs are kept */
 int kmm_poly_conf_from_stress(kmm_ctx* ctx);
 
+/* Autotuner (no reference counterpart; the role of FFTW's PATIENT planning,
+ * solvers.f90:2057-2069): occupancy variants of the pencil step kernel and of
+ * the x-plane kernel (0: the compiler's register budget, 1: one more wave per
+ * SIMD) are timed over `steps` steps on a twin of the context (same
+ * configuration, geometry, tracers and state; the context itself does not
+ * advance) and the faster of each is kept.  ms (or NULL) receives
+ * {pencil v0, pencil v1, xplane v0, xplane v1} in ms per step. */
+int kmm_autotune(kmm_ctx* ctx, int steps, double* ms);
+int kmm_set_tuning(kmm_ctx* ctx, int pencil, int xplane);
+int kmm_get_tuning(kmm_ctx* ctx, int* pencil, int* xplane);
+
 #ifdef __cplusplus
 }
 #endif

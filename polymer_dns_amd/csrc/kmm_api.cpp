@@ -87,6 +87,8 @@ struct kmm_ctx {
   int part_cap = 0;
   double* part = nullptr;    // [7][part_cap]: x, y, z, u, v, w, swirl
   double* scsrc = nullptr;   // [nyp][mz][mx] Gaussian-blob scalar source (scl_flag 2)
+  int tune_pencil = 0, tune_xplane = 0;  // occupancy variants (kmm_autotune)
+  std::vector<int32_t> imatrix_h;        // geometry as set (the autotuner's twin needs it)
   double* cap = nullptr;     // [4][nyp][mz][mx] physical u, v, w (and swirl) of the last evaluation
   bool part_swirl = false;   // swirl strength at the particles (solvers.f90:3479-3481)
   double* ycoord_d = nullptr;
@@ -293,6 +295,7 @@ static PencilArgs pencil_args(kmm_ctx* c) {
   PencilArgs a;
   std::memset(&a, 0, sizeof(a));
   a.nxh = c->nxh; a.nz = c->nz; a.nzh = c->nzh; a.P = c->P; a.ny = c->ny;
+  a.tune = c->tune_pencil;
   a.alpha = c->alpha; a.beta = c->beta; a.dyde = c->dyde; a.re = c->cfg.re; a.dt = c->cfg.dt;
   a.theta = c->cfg.theta; a.diff = c->cfg.diff; a.gH = c->gH; a.gS = c->gS;
   // setbcs (main_dns.f90:1240-1265): no-slip for channel / Couette
@@ -323,6 +326,7 @@ static PlaneArgs plane_args(kmm_ctx* c, int it, unsigned long long* cfl) {
   a.nxh = c->nxh; a.nz = c->nz; a.nzh = c->nzh; a.nyp = c->nyp; a.mx = c->mx; a.mz = c->mz; a.P = c->P;
   a.nfin = c->cfg.scalar ? 9 : 6;
   a.nfout = c->cfg.scalar ? 4 : 3;
+  a.tune = c->tune_xplane;
   a.mixed = c->cfg.precision == 1;
   a.dt = c->cfg.dt; a.dPdx = c->dPdx; a.dpdx_d = c->dpdx_d;
   a.delxm = c->cfg.xl / c->mx; a.delzm = c->cfg.zl / c->mz;
@@ -511,6 +515,7 @@ extern "C" int kmm_set_geometry(kmm_ctx* c, const int32_t* im) {
   if (c) drop_graph(c);  // the IB tables are reallocated
   if (!c || !im) return fail(KMM_E_ARG, "null argument");
   const int nyp = c->nyp, mz = c->mz, mx = c->mx;
+  c->imatrix_h.assign(im, im + (size_t)nyp * mz * mx);
   auto at = [&](int i, int j, int k) { return im[(size_t)i + (size_t)nyp * ((size_t)j + (size_t)mz * k)]; };
   int kmax = 0;
   std::vector<int4> src;
@@ -2125,5 +2130,76 @@ extern "C" int kmm_get_poly_stress(kmm_ctx* c, int comp, double* out) {
     for (int jz = 0; jz < c->mz; ++jz)
       for (int ix = 0; ix < c->mx; ++ix)
         out[k + (size_t)c->nyp * (jz + (size_t)c->mz * ix)] = h[((size_t)k * c->mz + jz) * c->mx + ix];
+  return KMM_OK;
+}
+
+// ------------------------------------------------------------ autotuner --
+// Occupancy variants of the two heaviest kernels (the pencil step kernel and
+// the x-plane kernel) are timed on a twin of the context -- same
+// configuration, geometry, tracers and state, its own buffers -- and the
+// faster variant of each is kept (the role FFTW's PATIENT planning plays for
+// the reference's transforms, solvers.f90:2057-2069).
+extern "C" int kmm_set_tuning(kmm_ctx* c, int pencil, int xplane) {
+  if (!c || pencil < 0 || pencil > 1 || xplane < 0 || xplane > 1) return fail(KMM_E_ARG, "variants are 0 or 1");
+  c->tune_pencil = pencil;
+  c->tune_xplane = xplane;
+  drop_graph(c);
+  return KMM_OK;
+}
+
+extern "C" int kmm_get_tuning(kmm_ctx* c, int* pencil, int* xplane) {
+  if (!c || !pencil || !xplane) return fail(KMM_E_ARG, "null argument");
+  *pencil = c->tune_pencil;
+  *xplane = c->tune_xplane;
+  return KMM_OK;
+}
+
+extern "C" int kmm_autotune(kmm_ctx* c, int steps, double* ms) {
+  if (!c || steps < 1 || steps > 1000) return fail(KMM_E_ARG, "steps must be in 1..1000");
+  KMM_HIP(hipSetDevice(c->device));
+  KMM_HIP(hipStreamSynchronize(c->stream));
+  kmm_ctx* t = nullptr;
+  int r = kmm_create(&c->cfg, c->device, &t);
+  if (r) return r;
+  if (!c->imatrix_h.empty()) r = kmm_set_geometry(t, c->imatrix_h.data());
+  if (!r && c->npart > 0) {
+    std::vector<double> x(c->npart), y(c->npart), z(c->npart);
+    r = kmm_get_particles(c, c->npart, x.data(), y.data(), z.data(), nullptr, nullptr, nullptr);
+    if (!r) r = kmm_set_particles(t, c->npart, x.data(), y.data(), z.data());
+  }
+  std::vector<int> ids = {KMM_V, KMM_OMY, KMM_U0, KMM_W0};
+  if (c->cfg.scalar) ids.push_back(KMM_SCALAR);
+  if (c->cfg.polymer)
+    for (int q = KMM_C11; q <= KMM_C33; ++q) ids.push_back(q);
+  std::vector<double> buf(2 * c->S + 16);
+  for (size_t k = 0; k < ids.size() && !r; ++k) {
+    r = kmm_get_field(c, ids[k], buf.data());
+    if (!r) r = kmm_put_field(t, ids[k], buf.data());
+  }
+  if (!r) r = kmm_startup(t);
+  double tm[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+  for (int v = 0; v < 2 && !r; ++v) {
+    t->tune_pencil = v;
+    t->tune_xplane = v;
+    t->timing = 0;
+    r = kmm_step(t, 2);
+    if (r) break;
+    t->timing = 1;
+    r = kmm_step(t, steps);
+    if (r) break;
+    double kms[8];
+    kmm_kernel_times(t, kms, nullptr);
+    tm[0][v] = kms[0];  // k_pencil
+    tm[1][v] = kms[3];  // k_xplane
+  }
+  const std::string why = r ? g_last_error : std::string();
+  kmm_destroy(t);
+  if (r) return fail(r, "autotune: " + why);
+  c->tune_pencil = tm[0][1] < tm[0][0] ? 1 : 0;
+  c->tune_xplane = tm[1][1] < tm[1][0] ? 1 : 0;
+  drop_graph(c);
+  if (ms) {
+    ms[0] = tm[0][0]; ms[1] = tm[0][1]; ms[2] = tm[1][0]; ms[3] = tm[1][1];
+  }
   return KMM_OK;
 }

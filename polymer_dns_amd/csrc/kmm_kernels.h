@@ -10,6 +10,7 @@ namespace kmm {
 // "pencil" p = kx*nz + kz; P = nxh*nz pencils per y-plane.
 struct PencilArgs {
   int nxh, nz, nzh, P, ny;
+  int tune;             // occupancy variant of the step kernel (autotuner)
   double alpha, beta, dyde, re, dt, theta, diff;
   double gH, gS;  // re/(dt theta), re diff/(dt theta)
   // mean-mode Robin data (setbcs, main_dns.f90:1175-1418)
@@ -105,6 +106,7 @@ struct PlaneArgs {
   double* cap;            // nullptr, or [3][nyp][mz][mx] physical u, v, w (FP64)
   const double* scsrc;    // nullptr, or [nyp][mz][mx] blob source of the scalar (added / dt)
   unsigned dzmask;        // k_zinv: bit f set -> input f is multiplied by i kz first
+  int tune;               // occupancy variant of k_xplane (autotuner)
   double beta;            // z wavenumber unit
 };
 

@@ -125,8 +125,10 @@ __device__ __forceinline__ PF pf_cmul_real(double2 c, const PF& g) {
 // MODE 0: full step (NL from buffers or FN/GN/SCN), MODE 1: startup
 // (derived fields of the current state only), MODE 2: finish (NL -> FN, GN,
 // SCN, h1n, h3n).
-template <int NY, bool SCALAR, int MODE, typename V>
-__global__ __launch_bounds__(PW * 64) void k_pencil(PencilArgs A) {
+// TV: occupancy variant picked by the autotuner (0: the compiler's register
+// budget, 2 waves/SIMD at NY = 128; 1: at least 3 waves/SIMD, <= 168 VGPRs)
+template <int NY, bool SCALAR, int MODE, typename V, int TV = 0>
+__global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(TV ? 3 : 1))) void k_pencil(PencilArgs A) {
   constexpr int L = NY / 2;
   extern __shared__ __align__(16) unsigned char smraw[];
   V* smem = reinterpret_cast<V*>(smraw);
@@ -475,16 +477,22 @@ hipError_t launch_green(int ny, const PencilArgs& a, double pb1, double pt1, dou
                                           pt2, G1, G2, gk));
   return hipGetLastError();
 }
-template <int MODE, typename V>
-static hipError_t launch_pencil_mode_v(int ny, const PencilArgs& a, hipStream_t s) {
+template <int MODE, typename V, int TV>
+static hipError_t launch_pencil_tv(int ny, const PencilArgs& a, hipStream_t s) {
   if (a.scalar) {
-    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_pencil<NY, true, MODE, V>), dim3(nblk(a.P)), dim3(PW * 64),
+    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_pencil<NY, true, MODE, V, TV>), dim3(nblk(a.P)), dim3(PW * 64),
                                            (lds_bytes<NY, V>()), s, a));
   } else {
-    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_pencil<NY, false, MODE, V>), dim3(nblk(a.P)), dim3(PW * 64),
+    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_pencil<NY, false, MODE, V, TV>), dim3(nblk(a.P)), dim3(PW * 64),
                                            (lds_bytes<NY, V>()), s, a));
   }
   return hipGetLastError();
+}
+template <int MODE, typename V>
+static hipError_t launch_pencil_mode_v(int ny, const PencilArgs& a, hipStream_t s) {
+  // only the step kernel has tuning variants
+  if (MODE == 0 && a.tune == 1) return launch_pencil_tv<MODE, V, 1>(ny, a, s);
+  return launch_pencil_tv<MODE, V, 0>(ny, a, s);
 }
 template <int MODE>
 static hipError_t launch_pencil_mode(int ny, const PencilArgs& a, hipStream_t s) {

@@ -101,8 +101,12 @@ __global__ __launch_bounds__(256) void k_ibdrag(PlaneArgs A) {
   }
 }
 
-template <int MX, int NFIN, int NFOUT, int WPB, typename V>
-__global__ __launch_bounds__(WPB * 64) void k_xplane(PlaneArgs A) {
+// TV: occupancy variant picked by the autotuner (0: the compiler's register
+// budget; 1: one more wave per SIMD than FP64/FP32 need at MX = 192, i.e.
+// >= 3 (FP64) or >= 5 (FP32) waves/SIMD)
+template <int MX, int NFIN, int NFOUT, int WPB, typename V, int TV = 0>
+__global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(TV ? (sizeof(V) == 16 ? 3 : 5) : 1)))
+void k_xplane(PlaneArgs A) {
   using T = typename Scal<V>::T;
   constexpr int XPL = (MX + 63) / 64;  // x positions per lane: x = lane + 64 m
   extern __shared__ __align__(16) unsigned char smraw[];
@@ -749,8 +753,12 @@ static hipError_t plane_impl(const PlaneArgs& a, hipStream_t s, bool inverse_par
     if (ev) hipEventRecord(ev[1], s);
     constexpr int WPB = 4;
     const size_t xlds = (size_t)(MX + WPB * 2 * FFTBuf<MX>::len) * sizeof(V);
-    hipLaunchKernelGGL((k_xplane<MX, NFIN, NFOUT, WPB, V>), dim3(a.nyp * (a.mz / (2 * WPB))), dim3(WPB * 64), xlds,
-                       s, a);
+    if (a.tune == 1)
+      hipLaunchKernelGGL((k_xplane<MX, NFIN, NFOUT, WPB, V, 1>), dim3(a.nyp * (a.mz / (2 * WPB))), dim3(WPB * 64),
+                         xlds, s, a);
+    else
+      hipLaunchKernelGGL((k_xplane<MX, NFIN, NFOUT, WPB, V, 0>), dim3(a.nyp * (a.mz / (2 * WPB))), dim3(WPB * 64),
+                         xlds, s, a);
     if (ev) hipEventRecord(ev[2], s);
   }
   hipLaunchKernelGGL((k_zfwd<MZ, NFOUT, V>), dim3(a.nyp * ngx), dim3(512), zlds, s, a);

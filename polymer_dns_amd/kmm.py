@@ -48,6 +48,7 @@ EXPORTS = [
     "kmm_mlp_set_norm", "kmm_mlp_infer", "kmm_mlp_grad", "kmm_mlp_train",
     "kmm_set_poly_capture", "kmm_poly_samples", "kmm_set_closure", "kmm_get_poly_stress",
     "kmm_set_closure_audit", "kmm_closure_status", "kmm_poly_conf_from_stress",
+    "kmm_autotune", "kmm_set_tuning", "kmm_get_tuning",
 ]
 
 
@@ -151,6 +152,9 @@ def load_library(path: str | None = None):
     lib.kmm_set_closure_audit.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double]
     lib.kmm_closure_status.argtypes = [C.c_void_p, ip, ip, dp, ip]
     lib.kmm_poly_conf_from_stress.argtypes = [C.c_void_p]
+    lib.kmm_autotune.argtypes = [C.c_void_p, C.c_int, dp]
+    lib.kmm_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.kmm_get_tuning.argtypes = [C.c_void_p, ip, ip]
     lib.kmm_calcswirl.argtypes = [C.c_void_p] + [dp] * 10
     lib.kmm_set_dpdx.argtypes = [C.c_void_p, C.c_double]
     for n in EXPORTS:
@@ -432,6 +436,21 @@ class Channel:
     def set_closure(self, mlp):
         _check(self.lib, self.lib.kmm_set_closure(self.h, mlp.h if mlp is not None else None))
         self._closure = mlp   # keep the network alive while the channel uses it
+
+    def autotune(self, steps=10):
+        """Time the occupancy variants of the pencil and x-plane kernels on a
+        twin of this channel and keep the faster; returns the timings."""
+        ms = np.zeros(4)
+        _check(self.lib, self.lib.kmm_autotune(self.h, int(steps), _dp(ms)))
+        return {"pencil_ms": ms[:2].tolist(), "xplane_ms": ms[2:].tolist(), "choice": self.tuning()}
+
+    def set_tuning(self, pencil, xplane):
+        _check(self.lib, self.lib.kmm_set_tuning(self.h, int(pencil), int(xplane)))
+
+    def tuning(self):
+        a, b = C.c_int(), C.c_int()
+        _check(self.lib, self.lib.kmm_get_tuning(self.h, C.byref(a), C.byref(b)))
+        return {"pencil": a.value, "xplane": b.value}
 
     def set_closure_audit(self, every, window=4, gate=0.1):
         """Audit windows of full FENE-P transport every `every` steps (0: off);
