@@ -94,16 +94,16 @@ __device__ __forceinline__ double2 bcast(double2 v, int src) { return rdlane(v, 
 
 // ---------------------------------------------------------------- scans --
 // All scans run four DPP levels inside each row of 16 lanes and combine the
-// four row aggregates read with v_readlane.
+// four row aggregates read with v_readlane.  A DPP move reads 0 where its
+// source lane lies outside the row, which makes the additive updates of
+// those lanes no-ops without a select.
 
 // inclusive suffix sum: r_l = sum_{l' >= l} v_l'
 __device__ __forceinline__ double2 suffix_sum(double2 v, int lane) {
-  const int r = lane & 15;
-  double2 t;
-  t = dpp<DPP_ROW_SHL | 1>(v); if (r + 1 < 16) v = cadd(v, t);
-  t = dpp<DPP_ROW_SHL | 2>(v); if (r + 2 < 16) v = cadd(v, t);
-  t = dpp<DPP_ROW_SHL | 4>(v); if (r + 4 < 16) v = cadd(v, t);
-  t = dpp<DPP_ROW_SHL | 8>(v); if (r + 8 < 16) v = cadd(v, t);
+  v = cadd(v, dpp<DPP_ROW_SHL | 1>(v));
+  v = cadd(v, dpp<DPP_ROW_SHL | 2>(v));
+  v = cadd(v, dpp<DPP_ROW_SHL | 4>(v));
+  v = cadd(v, dpp<DPP_ROW_SHL | 8>(v));
   const double2 a1 = rdlane(v, 16), a2 = rdlane(v, 32), a3 = rdlane(v, 48);
   const int row = lane >> 4;
   double2 add = czero();
@@ -114,13 +114,10 @@ __device__ __forceinline__ double2 suffix_sum(double2 v, int lane) {
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
-  const int lane = threadIdx.x & 63;
-  const int r = lane & 15;
-  double t;
-  t = dpp<DPP_ROW_SHL | 1>(v); if (r + 1 < 16) v += t;
-  t = dpp<DPP_ROW_SHL | 2>(v); if (r + 2 < 16) v += t;
-  t = dpp<DPP_ROW_SHL | 4>(v); if (r + 4 < 16) v += t;
-  t = dpp<DPP_ROW_SHL | 8>(v); if (r + 8 < 16) v += t;
+  v += dpp<DPP_ROW_SHL | 1>(v);
+  v += dpp<DPP_ROW_SHL | 2>(v);
+  v += dpp<DPP_ROW_SHL | 4>(v);
+  v += dpp<DPP_ROW_SHL | 8>(v);
   return (rdlane(v, 0) + rdlane(v, 16)) + (rdlane(v, 32) + rdlane(v, 48));
 }
 __device__ __forceinline__ double2 wave_sum(double2 v) { return make_double2(wave_sum(v.x), wave_sum(v.y)); }
@@ -146,8 +143,8 @@ __device__ __forceinline__ W suffix_affine_t(double c, W w, int lane) {
 #define KMM_SFX_LEVEL(K)                                  \
   {                                                       \
     const double c2 = dpp<DPP_ROW_SHL | K>(c);            \
-    const W w2 = dpp<DPP_ROW_SHL | K>(w);                 \
-    if (r + K < 16) { w = affine_apply(c, w, w2); c = c * c2; } \
+    w = affine_apply(c, w, dpp<DPP_ROW_SHL | K>(w));      \
+    if (r + K < 16) c = c * c2;                           \
   }
   KMM_SFX_LEVEL(1) KMM_SFX_LEVEL(2) KMM_SFX_LEVEL(4) KMM_SFX_LEVEL(8)
 #undef KMM_SFX_LEVEL
@@ -172,8 +169,8 @@ __device__ __forceinline__ double2 prefix_affine(double b, double2 a, int lane) 
 #define KMM_PFX_LEVEL(K)                                  \
   {                                                       \
     const double b2 = dpp<DPP_ROW_SHR | K>(b);            \
-    const double2 a2 = dpp<DPP_ROW_SHR | K>(a);           \
-    if (r >= K) { a = cfma(b, a2, a); b = b * b2; }       \
+    a = cfma(b, dpp<DPP_ROW_SHR | K>(a), a);              \
+    if (r >= K) b = b * b2;                               \
   }
   KMM_PFX_LEVEL(1) KMM_PFX_LEVEL(2) KMM_PFX_LEVEL(4) KMM_PFX_LEVEL(8)
 #undef KMM_PFX_LEVEL
