@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import re
 import math
 import os
 import sys
@@ -141,6 +142,22 @@ def kernel_bytes(G):
     }
 
 
+def measured_traffic(mode, kernel, tune):
+    """HBM bytes per launch of `kernel` (read + write, FETCH_SIZE x 2 and
+    WRITE_SIZE x 1 after the copy calibration) from the committed counter
+    summary, for the occupancy variant this run's autotuner chose; None when
+    the summary has no entry."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "kmm_step_c2_hbm_traffic.md")
+    try:
+        with open(path) as fh:
+            m = re.search(r"```json\n(.*?)\n```", fh.read(), re.S)
+        table = json.loads(m.group(1)) if m else {}
+    except (OSError, ValueError):
+        return None
+    var = {"k_pencil": tune["choice"]["pencil"], "k_xplane": tune["choice"]["xplane"]}.get(kernel, 0)
+    return table.get(mode, {}).get(kernel, {}).get(str(var))
+
+
 def cpu_baseline(nsteps=8):
     from threadpoolctl import threadpool_limits
     from oracle import kmm_oracle as O
@@ -208,7 +225,8 @@ def main():
         "config": {"workload": "Re_tau 180 channel 128x129x128, FP64, scalar Pr 0.71 + uniform source, "
                                "Goldstein rib IB", "grid": [NX, NY + 1, NZ], "dt": DT, "mode": "fp64"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic("fp64", dom, tune), "algorithmic_bytes": kb[dom],
                      "kernel_ms": {k: round(v, 4) for k, v in ktimes.items()}},
         "step_roofline": {"bytes_per_gridpt": 320, "achieved_GBs": round(step_bytes / (ms * 1e-3) / 1e9, 1),
                           "frac": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
