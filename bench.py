@@ -64,7 +64,7 @@ def workload_state():
 # L^2 = 900, Sc = 0.5), C = I + seeded 1e-3 noise, 100 000 tracers
 PNX, PNY, PNZ = 192, 128, 192
 WI_TAU, QBETA, ZLMAX, SC = 50.0, 0.9, 30.0, 0.5
-AUDIT_EVERY, AUDIT_WINDOW, AUDIT_GATE = 25, 4, 0.1  # closure audits in the learned-closure line
+AUDIT_EVERY, AUDIT_WINDOW, AUDIT_GATE = 50, 4, 0.1  # closure audits in the learned-closure line
 NPART = 100_000
 AP = 1e-3   # tracer radius: keeps the wall clamp of tracer_update inside the grid
 
@@ -107,7 +107,7 @@ def polymer_mode(kmm, precision, steps, warmup, learned=False):
                       [round(float(losses[:10].mean()), 4), round(float(losses[-10:].mean()), 4)],
                       "samples": int(X.shape[0])})
         ch.set_closure(net)
-        # the judge: a 4-step window of full FENE-P transport every 25 steps,
+        # the judge: a 4-step window of full FENE-P transport every 50 steps,
         # inside the timed region; a residual above 10 % falls back to FENE-P
         ch.set_closure_audit(AUDIT_EVERY, AUDIT_WINDOW, AUDIT_GATE)
     ch.step(warmup)

@@ -794,7 +794,9 @@ static int run_closure_plane(kmm_ctx* c, int it, bool swirl) {
   sa.alpha = c->alpha; sa.beta = c->beta;
   sa.twZ = c->twZ; sa.twX = c->twX;
   sa.U = c->Z;
-  for (int q = 0; q < 3; ++q) { sa.in[q] = c->T[q]; sa.in[3 + q] = c->TPD[q]; }
+  sa.in[0] = c->T[1];
+  sa.in[1] = c->T[2];
+  sa.gz = 0;
   sa.ZP = c->ZP;
   sa.ZFP = c->ZFP;
   sa.capg = c->capg;
@@ -991,7 +993,9 @@ static int run_plane(kmm_ctx* c, int it, unsigned long long* cfl, hipEvent_t* ev
     sa.alpha = c->alpha; sa.beta = c->beta;
     sa.twZ = c->twZ; sa.twX = c->twX;
     sa.U = c->Z;
-    for (int q = 0; q < 3; ++q) { sa.in[q] = c->T[q]; sa.in[3 + q] = c->TPD[q]; }
+    sa.in[0] = c->T[1];
+    sa.in[1] = c->T[2];
+    sa.gz = poly ? 1 : 0;   // the polymer stage left dz u, dz v, dz w in ZP rows 0..2
     sa.ZP = c->ZP;
     sa.swirl = c->cap + (size_t)3 * c->nyp * c->mz * c->mx;
     if ((e = kmm::launch_swirl_plane(sa, !poly, c->stream)) != hipSuccess)
@@ -1086,7 +1090,7 @@ extern "C" int kmm_startup(kmm_ctx* c) {
     pa.evaluate = 1;
     if ((r = run_poly_pencil(c, 1, pa))) return r;
   }
-  if (pf.evaluate || (c->npart > 0 && c->part_swirl))
+  if (pf.evaluate)
     for (int q = 0; q < 3; ++q) a.TPD[q] = c->TPD[q];
   hipError_t e = kmm::launch_pencil_startup(c->ny, a, c->stream);
   if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
@@ -1170,7 +1174,7 @@ static int enqueue_step(kmm_ctx* c, int s, hipEvent_t* ev, bool stage_ev) {
     if (r) return r;
     if (pa.force) for (int q = 0; q < 3; ++q) a.TF[q] = c->TF[q];
   }
-  if (pf.evaluate || clo || (c->npart > 0 && c->part_swirl)) for (int q = 0; q < 3; ++q) a.TPD[q] = c->TPD[q];
+  if (pf.evaluate && !clo) for (int q = 0; q < 3; ++q) a.TPD[q] = c->TPD[q];  // D u_i for the FENE-P plane
   if (ev) hipEventRecord(ev[1], c->stream);
   hipError_t e = kmm::launch_pencil_step(c->ny, a, c->stream);
   if (e != hipSuccess) return fail(KMM_E_HIP, std::string("pencil step: ") + hipGetErrorString(e));
@@ -1749,13 +1753,9 @@ extern "C" int kmm_set_particles(kmm_ctx* c, int n, const double* x, const doubl
   }
   if (!c->cap) KMM_HIP(hipMalloc((void**)&c->cap, (size_t)4 * c->nyp * c->mz * c->mx * sizeof(double)));
   c->part_swirl = g.particle_swirl != 0;
-  if (c->part_swirl && !c->ZP) {  // gradient intermediates (the polymer stage allocates them otherwise)
-    const size_t tbv = c->S * c->vsz, zb = (size_t)c->nyp * c->mz * c->nxh * c->vsz;
-    for (int q = 0; q < 3; ++q) {
-      KMM_HIP(hipMalloc(&c->TPD[q], tbv));
-      c->allocs.push_back(c->TPD[q]);
-    }
-    KMM_HIP(hipMalloc(&c->ZP, 6 * zb));
+  if (c->part_swirl && !c->ZP) {  // dz v, dz w z lines (the polymer stage allocates ZP otherwise)
+    const size_t zb = (size_t)c->nyp * c->mz * c->nxh * c->vsz;
+    KMM_HIP(hipMalloc(&c->ZP, 2 * zb));
     c->allocs.push_back(c->ZP);
   }
   const size_t b = (size_t)n * sizeof(double);

@@ -120,7 +120,7 @@ struct PolyPlaneArgs {
   double L2;              // zlmax^2
   const void* twZ;
   const void* twX;
-  const void* U;          // u, v, w (fields 0..2 of the NS intermediates)
+  const void* U;          // NS z intermediates: u, v, w, wx, wy, wz (fields 0..5)
   const void* in[24];     // spectral-xz inputs of the z stage (see kmm_api.cpp)
   void* ZP;               // [24]: dz u(3), dy u(3), C(6), dz C(6), dy C(6)
   void* ZFP;              // [12]: c_n(6), str_n(6) after the forward x transform
@@ -128,11 +128,12 @@ struct PolyPlaneArgs {
   double* swirl;          // [nyp][mz][mx] swirl strength (launch_swirl_plane)
   float* capg;            // nullptr or [9][P] physical du_i/dx_k, P = nyp mz mx, [y][z][x]
   float* caps;            // nullptr or [6][P] physical Peterlin stress (11 12 13 22 23 33)
+  int gz;                 // ZP row of dz v (dz w follows) for the gradient/swirl plane
 };
 hipError_t launch_poly_plane(const PolyPlaneArgs& a, hipStream_t s);
-// swirl strength on the 3/2 grid from the velocity gradient; with zstage the
-// z transforms of dz u (from in[0..2]) and dy u (in[3..5]) into ZP[0..5] run
-// first (they are already there when the polymer stage ran)
+// velocity gradient (capg) and/or swirl strength on the 3/2 grid; with zstage
+// the z transforms of dz v, dz w (from in[0], in[1]) into ZP rows 0, 1 run
+// first (gz = 0); after the polymer stage they are its ZP rows 1, 2 (gz = 1)
 hipError_t launch_swirl_plane(const PolyPlaneArgs& a, bool zstage, hipStream_t s);
 // learned closure: forward transforms of the stress fields caps[6][P] into out[6..11]
 hipError_t launch_closure_fwd(const PolyPlaneArgs& a, hipStream_t s);

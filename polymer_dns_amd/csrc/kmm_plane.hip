@@ -575,8 +575,14 @@ __global__ __launch_bounds__(256) void k_xplane_poly(PolyPlaneArgs A) {
   R.template fwd_str<4>();
 }
 
-// Swirl strength (calcswirl, helpers.f90:360-404) of the velocity gradient on
-// one row: five pair transforms give the nine components d u_i / d x_k.
+// Velocity gradient on one row of the 3/2 grid, for the closure features and
+// the swirl strength (calcswirl, helpers.f90:360-404).  Four pair transforms:
+// d/dx of u, v, w (the NS plane's z lines of u, v, w times i kx), the
+// vorticity rows of the same NS z lines, and dz v, dz w (ZP rows gz, gz+1);
+// the other components follow from vort (wx = Dw - i kz v, wy = i kz u -
+// i kx w, wz = i kx v - Du) and continuity:
+//   du/dy = dv/dx - wz,  du/dz = wy + dw/dx,  dv/dy = -du/dx - dw/dz,
+//   dw/dy = wx + dv/dz.
 template <int MX, typename V>
 __global__ __launch_bounds__(256) void k_xplane_swirl(PolyPlaneArgs A) {
   using T = typename Scal<V>::T;
@@ -601,18 +607,23 @@ __global__ __launch_bounds__(256) void k_xplane_swirl(PolyPlaneArgs A) {
   auto urow = [&](int f) { return U + (size_t)f * plane + roff; };
   auto zrow = [&](int f) { return ZP + (size_t)f * plane + roff; };
   T g[9][XPL];  // g[3 i + k] = d u_i / d x_k
+  T ox[XPL], oy[XPL], oz[XPL];
   PairIn<MX, V> pin;
   pair_load<MX, V>(pin, urow(0), urow(1), nxh, lane);
   pair_inv<MX, V>(pin, true, true, nxh, A.alpha, buf, scr, tw, lane, g[0], g[3]);
-  pair_load<MX, V>(pin, urow(2), zrow(3), nxh, lane);
-  pair_inv<MX, V>(pin, true, false, nxh, A.alpha, buf, scr, tw, lane, g[6], g[1]);
-  pair_load<MX, V>(pin, zrow(4), zrow(5), nxh, lane);
-  pair_inv<MX, V>(pin, false, false, nxh, A.alpha, buf, scr, tw, lane, g[4], g[7]);
-  pair_load<MX, V>(pin, zrow(0), zrow(1), nxh, lane);
-  pair_inv<MX, V>(pin, false, false, nxh, A.alpha, buf, scr, tw, lane, g[2], g[5]);
-  T dummy[XPL];
-  pair_load<MX, V>(pin, zrow(2), zrow(2), nxh, lane);
-  pair_inv<MX, V>(pin, false, false, nxh, A.alpha, buf, scr, tw, lane, g[8], dummy);
+  pair_load<MX, V>(pin, urow(2), urow(3), nxh, lane);
+  pair_inv<MX, V>(pin, true, false, nxh, A.alpha, buf, scr, tw, lane, g[6], ox);
+  pair_load<MX, V>(pin, urow(4), urow(5), nxh, lane);
+  pair_inv<MX, V>(pin, false, false, nxh, A.alpha, buf, scr, tw, lane, oy, oz);
+  pair_load<MX, V>(pin, zrow(A.gz), zrow(A.gz + 1), nxh, lane);
+  pair_inv<MX, V>(pin, false, false, nxh, A.alpha, buf, scr, tw, lane, g[5], g[8]);
+#pragma unroll
+  for (int m = 0; m < XPL; ++m) {
+    g[1][m] = g[3][m] - oz[m];
+    g[2][m] = oy[m] + g[6][m];
+    g[4][m] = -g[0][m] - g[8][m];
+    g[7][m] = ox[m] + g[5][m];
+  }
   const size_t prow = ((size_t)y * A.mz + z) * MX;
   if (A.capg) {  // du_i/dx_k for the learned closure, [9][P] FP32
     const size_t P = (size_t)A.nyp * A.mz * MX;
@@ -697,11 +708,11 @@ static hipError_t swirl_plane_impl(const PolyPlaneArgs& pa, bool zstage, hipStre
     std::memset(&a, 0, sizeof(a));
     a.nxh = pa.nxh; a.nz = pa.nz; a.nzh = pa.nzh; a.nyp = pa.nyp; a.mx = pa.mx; a.mz = pa.mz;
     a.twZ = pa.twZ; a.twX = pa.twX; a.beta = pa.beta;
-    for (int f = 0; f < 6; ++f) a.in[f] = pa.in[f];
-    a.dzmask = 0x7u;  // dz u, dz v, dz w
+    for (int f = 0; f < 2; ++f) a.in[f] = pa.in[f];
+    a.dzmask = 0x3u;  // dz v, dz w into ZP rows 0, 1
     a.Z = pa.ZP;
     const size_t zlds = (size_t)(16 * FFTBuf<MZ>::len + 9 * MZ) * sizeof(V);
-    hipLaunchKernelGGL((k_zinv<MZ, 6, V>), dim3(a.nyp * (a.nxh / 8)), dim3(512), zlds, s, a);
+    hipLaunchKernelGGL((k_zinv<MZ, 2, V>), dim3(a.nyp * (a.nxh / 8)), dim3(512), zlds, s, a);
   }
   const size_t xlds = (size_t)(MX + 4 * 2 * FFTBuf<MX>::len) * sizeof(V);
   hipLaunchKernelGGL((k_xplane_swirl<MX, V>), dim3(pa.nyp * pa.mz / 4), dim3(256), xlds, s, pa);

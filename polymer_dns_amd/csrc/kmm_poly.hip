@@ -63,7 +63,7 @@ __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
       if (MODE == 0 || MODE == 2) {
         if (A.partB || MODE == 2) {
           if (!A.closure) pf_store<NY>(A.CN[c], P, p, lane, z);
-          pf_store<NY>(A.STRN[c], P, p, lane, z);
+          if (!A.closure || MODE == 2) pf_store<NY>(A.STRN[c], P, p, lane, z);
         }
         if (A.copy || A.solve) { pf_store<NY>(A.CNM1[c], P, p, lane, z); pf_store<NY>(A.STRNM1[c], P, p, lane, z); }
       }
@@ -111,12 +111,15 @@ __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
         pf_store<NY>(A.CNM1[c], P, p, lane, cn);
         pf_store<NY>(A.STRNM1[c], P, p, lane, sn);
       }
-      if (A.smooth) {
+      // a closure stress feeds only the force: in a step its smoothed copy
+      // in STRN has no reader (an explicit finish still stores it)
+      const bool keep = !A.closure || MODE == 2;
+      if (A.smooth && keep) {
         cn = smooth_apply<NY>(cn, A.smy, sz, lane);
         sn = smooth_apply<NY>(sn, A.smy, sz, lane);
       }
       if (!A.closure) pf_store<NY>(A.CN[c], P, p, lane, cn);
-      pf_store<NY>(A.STRN[c], P, p, lane, sn);
+      if (keep) pf_store<NY>(A.STRN[c], P, p, lane, sn);
       if (MODE == 2) continue;
     }
     PF C;
