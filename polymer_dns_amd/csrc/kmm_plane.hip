@@ -20,6 +20,16 @@
 
 namespace kmm {
 
+// one wave's line of nz spectral values, NJ per lane
+template <int NJ, typename V>
+__device__ __forceinline__ void zinv_line_load(V (&pre)[NJ], const V* __restrict__ src, int nz, int lane) {
+#pragma unroll
+  for (int q = 0; q < NJ; ++q) {
+    const int j = lane + 64 * q;
+    pre[q] = (j < nz) ? src[j] : vzero<V>();
+  }
+}
+
 template <int MZ, int NFIN, typename V>
 __global__ __launch_bounds__(512) void k_zinv(PlaneArgs A) {
   extern __shared__ __align__(16) unsigned char smraw[];
@@ -35,22 +45,32 @@ __global__ __launch_bounds__(512) void k_zinv(PlaneArgs A) {
   V* tw = tile + 8 * MZ;        // [MZ] twiddles
   for (int e = threadIdx.x; e < MZ; e += 512) tw[e] = static_cast<const V*>(A.twZ)[e];
   __syncthreads();
+  // each field's line is loaded into registers one field ahead, so its
+  // latency hides behind the previous field's transform and store
+  constexpr int NJ = (2 * MZ / 3 + 63) / 64;  // values of a line per lane (nz = 2 mz / 3)
+  V pre[NJ];
+  zinv_line_load<NJ, V>(pre, static_cast<const V*>(A.in[0]) + ((size_t)y * A.nxh + kx) * A.nz, A.nz, lane);
 #pragma unroll
   for (int f = 0; f < NFIN; ++f) {
-    const V* src = static_cast<const V*>(A.in[f]) + ((size_t)y * A.nxh + kx) * A.nz;
     for (int z = lane; z < MZ; z += 64) buf[fpad(z)] = vzero<V>();
     wave_sync();
     const bool dz = (A.dzmask >> f) & 1u;  // d/dz = multiplication by i kz
-    for (int j = lane; j < A.nz; j += 64) {
-      const int jj = (j < A.nzh) ? j : (MZ - A.nz + j);
-      V val = src[j];
-      if (dz) {
-        using T = typename Scal<V>::T;
-        const T wz = (T)(A.beta * (double)((j < A.nzh) ? j : j - A.nz));
-        val = mkv<V>(-val.y * wz, val.x * wz);
+#pragma unroll
+    for (int q = 0; q < NJ; ++q) {
+      const int j = lane + 64 * q;
+      if (j < A.nz) {
+        const int jj = (j < A.nzh) ? j : (MZ - A.nz + j);
+        V val = pre[q];
+        if (dz) {
+          using T = typename Scal<V>::T;
+          const T wz = (T)(A.beta * (double)((j < A.nzh) ? j : j - A.nz));
+          val = mkv<V>(-val.y * wz, val.x * wz);
+        }
+        buf[fpad(jj)] = val;
       }
-      buf[fpad(jj)] = val;
     }
+    if (f + 1 < NFIN)
+      zinv_line_load<NJ, V>(pre, static_cast<const V*>(A.in[f + 1]) + ((size_t)y * A.nxh + kx) * A.nz, A.nz, lane);
     wave_sync();
     V* r = wave_fft<MZ, +1>(buf, scr, tw, lane);
     for (int z = lane; z < MZ; z += 64) tile[z * 8 + wv] = r[fpad(z)];
@@ -254,6 +274,20 @@ void k_xplane(PlaneArgs A) {
   }
 }
 
+// one block's [MZ][8] tile of a z-line group, TL values per thread (real
+// and imaginary parts kept as scalars so that they stay in registers)
+template <int MZ, int TL, typename V, typename T>
+__device__ __forceinline__ void zfwd_tile_load(T (&px)[TL], T (&py)[TL], const V* __restrict__ srcp, int nxh,
+                                               int kx0) {
+#pragma unroll
+  for (int q = 0; q < TL; ++q) {
+    const int e = min((int)threadIdx.x + 512 * q, MZ * 8 - 1);  // in range; extra values are not stored
+    const V v = srcp[(size_t)(e >> 3) * nxh + kx0 + (e & 7)];
+    px[q] = v.x;
+    py[q] = v.y;
+  }
+}
+
 template <int MZ, int NFOUT, typename V>
 __global__ __launch_bounds__(512) void k_zfwd(PlaneArgs A) {
   using T = typename Scal<V>::T;
@@ -272,14 +306,23 @@ __global__ __launch_bounds__(512) void k_zfwd(PlaneArgs A) {
   __syncthreads();
   const T scale = (T)(1.0 / ((double)A.mx * (double)MZ));
   const int nz = A.nz, nzh = A.nzh;
+  // the next field's tile is loaded into registers during this field's
+  // transform
+  constexpr int TL = (MZ * 8 + 511) / 512;
+  T prx[TL], pry[TL];
+  zfwd_tile_load<MZ, TL, V, T>(prx, pry, static_cast<const V*>(A.ZF) + (size_t)y * MZ * A.nxh, A.nxh, kx0);
 #pragma unroll
   for (int o = 0; o < NFOUT; ++o) {
-    const V* srcp = static_cast<const V*>(A.ZF) + ((size_t)o * A.nyp + y) * MZ * A.nxh;
-    for (int e = threadIdx.x; e < MZ * 8; e += 512) {
-      const int z = e >> 3, c = e & 7;
-      tile[e] = srcp[(size_t)z * A.nxh + kx0 + c];
+#pragma unroll
+    for (int q = 0; q < TL; ++q) {
+      const int e = threadIdx.x + 512 * q;
+      if (e < MZ * 8) tile[e] = mkv<V>(prx[q], pry[q]);
     }
     __syncthreads();
+    if (o + 1 < NFOUT)
+      zfwd_tile_load<MZ, TL, V, T>(prx, pry,
+                                   static_cast<const V*>(A.ZF) + ((size_t)(o + 1) * A.nyp + y) * MZ * A.nxh, A.nxh,
+                                   kx0);
     for (int z = lane; z < MZ; z += 64) buf[fpad(z)] = tile[z * 8 + wv];
     wave_sync();
     V* r = wave_fft<MZ, -1>(buf, scr, tw, lane);
