@@ -121,6 +121,19 @@ __global__ __launch_bounds__(256) void k_ibdrag(PlaneArgs A) {
   }
 }
 
+// the two z rows (A, B) of one field, nxh spectral values each, NK per lane
+template <int NK, typename V, typename T>
+__device__ __forceinline__ void xrow_pair_load(T (&ax)[NK], T (&ay)[NK], T (&bx)[NK], T (&by)[NK],
+                                               const V* __restrict__ ra, const V* __restrict__ rb, int nxh,
+                                               int lane) {
+#pragma unroll
+  for (int q = 0; q < NK; ++q) {
+    const int k = min(lane + 64 * q, nxh - 1);  // in range; lanes past nxh do not use theirs
+    const V a = ra[k], b = rb[k];
+    ax[q] = a.x; ay[q] = a.y; bx[q] = b.x; by[q] = b.y;
+  }
+}
+
 // TV: occupancy variant picked by the autotuner (0: the compiler's register
 // budget; 1: one more wave per SIMD than FP64/FP32 need at MX = 192, i.e.
 // >= 3 (FP64) or >= 5 (FP32) waves/SIMD)
@@ -144,15 +157,24 @@ void k_xplane(PlaneArgs A) {
   const int nxh = A.nxh;
   // ph[f][m] = (row A, row B) physical values at x = lane + 64 m
   V ph[NFIN][XPL];
+  // the next field's two rows are loaded into registers during this field's
+  // transform (nxh = MX/3 values per row)
+  constexpr int NK = (MX / 3 + 63) / 64;
+  T pax[NK], pay[NK], pbx[NK], pby[NK];
+  auto rows = [&](int f) {
+    const V* zr = static_cast<const V*>(A.Z) + ((size_t)f * A.nyp + y) * A.mz * nxh;
+    xrow_pair_load<NK, V, T>(pax, pay, pbx, pby, zr + (size_t)zA * nxh, zr + (size_t)zB * nxh, nxh, lane);
+  };
+  rows(0);
 #pragma unroll
   for (int f = 0; f < NFIN; ++f) {
-    const V* zr = static_cast<const V*>(A.Z) + ((size_t)f * A.nyp + y) * A.mz * nxh;
-    const V* ra = zr + (size_t)zA * nxh;
-    const V* rb = zr + (size_t)zB * nxh;
     for (int x = lane; x < MX; x += 64) buf[fpad(x)] = vzero<V>();
     wave_sync();
-    for (int k = lane; k < nxh; k += 64) {
-      const V ua = ra[k], ub = rb[k];
+#pragma unroll
+    for (int q = 0; q < NK; ++q) {
+      const int k = lane + 64 * q;
+      if (k >= nxh) continue;
+      const V ua = mkv<V>(pax[q], pay[q]), ub = mkv<V>(pbx[q], pby[q]);
       if (k == 0) {
         buf[0] = mkv<V>(ua.x, ub.x);
       } else {
@@ -160,6 +182,7 @@ void k_xplane(PlaneArgs A) {
         buf[fpad(MX - k)] = mkv<V>(ua.x + ub.y, ub.x - ua.y);
       }
     }
+    if (f + 1 < NFIN) rows(f + 1);
     wave_sync();
     V* r = wave_fft<MX, +1>(buf, scr, tw, lane);
 #pragma unroll
