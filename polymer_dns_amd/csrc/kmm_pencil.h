@@ -135,8 +135,50 @@ __device__ void cheb_to_phys_store(const PF& f, V* buf, V* scr, const V* tw, int
   if (lane == 0) dst[(size_t)NY * P + p] = r[fpad(NY)];
   wave_sync();
 }
-// Forward (solvers.f90:3279-3416): DCT-I, /NY, ends halved; input in the
-// transfer precision V, result in FP64.
+// A pencil's physical-y values held in registers (lane l: y = 2l, 2l+1; lane
+// 0 also y = NY), so that a load can be issued ahead of the transform that
+// consumes it.
+template <typename V>
+struct PhysPencil {
+  V a, b, e;
+};
+template <int NY, typename V>
+__device__ __forceinline__ PhysPencil<V> phys_load(const V* __restrict__ src, size_t P, size_t p, int lane) {
+  constexpr int L = NY / 2;
+  PhysPencil<V> q;
+  const int l = lane < L ? lane : L - 1;  // lanes >= L reload a valid row and ignore it
+  q.a = src[(size_t)(2 * l) * P + p];
+  q.b = src[(size_t)(2 * l + 1) * P + p];
+  q.e = src[(size_t)NY * P + p];
+  return q;
+}
+// Forward (solvers.f90:3279-3416): DCT-I through the length-2NY FFT of the
+// even extension, /NY, ends halved; input in the transfer precision V,
+// result in FP64.
+template <int NY, typename V>
+__device__ PF phys_to_cheb(const PhysPencil<V>& q, V* buf, V* scr, const V* tw, int lane) {
+  constexpr int L = NY / 2, M = 2 * NY;
+  if (lane < L) {
+    buf[fpad(2 * lane)] = q.a;
+    buf[fpad(2 * lane + 1)] = q.b;
+    if (lane > 0) buf[fpad(M - 2 * lane)] = q.a;
+    buf[fpad(M - 2 * lane - 1)] = q.b;
+  }
+  if (lane == 0) buf[fpad(NY)] = q.e;
+  wave_sync();
+  V* r = wave_fft<M, -1>(buf, scr, tw, lane);
+  const double s = 1.0 / NY;
+  PF o;
+  if (lane < L) {
+    o.a = cscale(to_d(r[fpad(2 * lane)]), lane == 0 ? 0.5 * s : s);
+    o.b = cscale(to_d(r[fpad(2 * lane + 1)]), s);
+  } else { o.a = o.b = czero(); }
+  o.e = cscale(to_d(r[fpad(NY)]), 0.5 * s);
+  wave_sync();
+  return o;
+}
+
+// the same, loading the pencil first
 template <int NY, typename V>
 __device__ PF phys_to_cheb_load(const V* __restrict__ src, size_t P, size_t p, V* buf, V* scr, const V* tw,
                                 int lane) {

@@ -176,9 +176,15 @@ __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(TV ? 3 
   if (MODE == 0 || MODE == 2) {
     if (MODE == 2 || A.first_nl) {
       const double sz = A.smooth ? A.smz[kzi] : 1.0;
-      PF h1 = phys_to_cheb_load<NY, V>(static_cast<const V*>(A.NL1), P, p, buf, scr, twl, lane);
-      PF h3 = phys_to_cheb_load<NY, V>(static_cast<const V*>(A.NL3), P, p, buf, scr, twl, lane);
-      PF h2 = phys_to_cheb_load<NY, V>(static_cast<const V*>(A.NL2), P, p, buf, scr, twl, lane);
+      // each pencil's load is issued one transform ahead
+      PhysPencil<V> q1 = phys_load<NY, V>(static_cast<const V*>(A.NL1), P, p, lane);
+      PhysPencil<V> q3 = phys_load<NY, V>(static_cast<const V*>(A.NL3), P, p, lane);
+      PF h1 = phys_to_cheb<NY, V>(q1, buf, scr, twl, lane);
+      PhysPencil<V> q2 = phys_load<NY, V>(static_cast<const V*>(A.NL2), P, p, lane);
+      PF h3 = phys_to_cheb<NY, V>(q3, buf, scr, twl, lane);
+      PhysPencil<V> qs;
+      if (SCALAR) qs = phys_load<NY, V>(static_cast<const V*>(A.NLS), P, p, lane);
+      PF h2 = phys_to_cheb<NY, V>(q2, buf, scr, twl, lane);
       if (A.TF[0]) {  // polymer force, subforce (helpers.f90:564-606), before smoothing
         h1 = pf_add(h1, pf_load<NY>(A.TF[0], P, p, lane));
         h2 = pf_add(h2, pf_load<NY>(A.TF[1], P, p, lane));
@@ -198,7 +204,7 @@ __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(TV ? 3 
       fn = pf_sub(pf_scale(h2, -k2), dx);
       gn = pf_sub(pf_imul(h1, kz), pf_imul(h3, kx));
       if (SCALAR) {
-        sn = phys_to_cheb_load<NY, V>(static_cast<const V*>(A.NLS), P, p, buf, scr, twl, lane);
+        sn = phys_to_cheb<NY, V>(qs, buf, scr, twl, lane);
         if (A.smooth) sn = smooth_apply<NY>(sn, A.smy, sz, lane);
         if (is00) sn = pf_real(sn);
       }
