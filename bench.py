@@ -64,7 +64,7 @@ def workload_state():
 # L^2 = 900, Sc = 0.5), C = I + seeded 1e-3 noise, 100 000 tracers
 PNX, PNY, PNZ = 192, 128, 192
 WI_TAU, QBETA, ZLMAX, SC = 50.0, 0.9, 30.0, 0.5
-AUDIT_EVERY, AUDIT_WINDOW, AUDIT_GATE = 50, 4, 0.1  # closure audits in the learned-closure line
+AUDIT_EVERY, AUDIT_WINDOW, AUDIT_GATE = 25, 3, 0.1  # closure audits in the learned-closure line
 NPART = 100_000
 AP = 1e-3   # tracer radius: keeps the wall clamp of tracer_update inside the grid
 

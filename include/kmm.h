@@ -311,6 +311,21 @@ int kmm_closure_status(kmm_ctx* ctx, int* active, int* audits, double* last_resi
  * restart of the transport); the pending plane outputs are kept */
 int kmm_poly_conf_from_stress(kmm_ctx* ctx);
 
+/* writeoutputs (helpers.f90:1227-1436, with the mean u of solvers.f90:
+ * 3487-3495) for the last evaluation: prof[9][nyp] = mean u, u'_rms, v'_rms,
+ * w'_rms, Reynolds shear stress <u'v>, mean viscous shear stress <du/dy>/re,
+ * betay (beta_poly = qbeta in POLYMER builds, else the mean scalar), mean
+ * swirl strength, polymer shear stress <qp12>; y from +1 to -1.  scal[6] =
+ * mass flux (bulk velocity), top and bottom wall shear stress, top and bottom
+ * polymer shear stress, volume mean of tr C (NaN when no conformation tensor
+ * was evaluated).  Second moments come from the spectral coefficients
+ * (Parseval on the 3/2 grid is exact), means from the (0,0) modes. */
+int kmm_statistics(kmm_ctx* ctx, double* prof, double* scal);
+/* the same statistics appended to the reference's output files in dir
+ * (Txy0 urms0 RSS0 vrms0 wrms0 massflux0 mean_u0 betay0 swirly0 twss0 bwss0
+ * tpss0 bpss0 Ckk0 PSS0), created when first != 0 */
+int kmm_write_outputs(kmm_ctx* ctx, const char* dir, int first);
+
 /* Autotuner (no reference counterpart; the role of FFTW's PATIENT planning,
  * solvers.f90:2057-2069): occupancy variants of the pencil step kernel and of
  * the x-plane kernel (0: the compiler's register budget, 1: one more wave per

@@ -816,7 +816,8 @@ def vcw3dp(st, p: Params, it, fields, particles=None):
         cn, strp, qpp = polymer_products(cp, dcp, (up, vp, wp), gup, p)
         phys["poly"] = dict(cn={k: to_spectral(v, p) for k, v in cn.items()},
                             str={k: to_spectral(v, p) for k, v in strp.items()},
-                            qp={k: to_spectral(v, p) for k, v in qpp.items()})
+                            qp={k: to_spectral(v, p) for k, v in qpp.items()},
+                            p12=qpp["12"], trC=cp["11"] + cp["22"] + cp["33"])
     return gn, fn, omz_o, scn, phys
 
 
@@ -867,6 +868,46 @@ def polymer_products(cp, dcp, up3, gup, p: Params):
     zbeta1 = (1.0 - p.qbeta) / (p.re * p.qbeta * p.tpoly)
     qp = {ij: zbeta1 * st[ij] for ij in POLY}
     return cn, st, qp
+
+
+def writeoutputs(up, vp, wp, u12, swirl, beta, p12, trC, p: Params):
+    """helpers.f90:1227-1436 with the mean u of solvers.f90:3487-3495, on the
+    3/2 physical grid.  Arrays are (mx, mz, nyp) with y index 0 at y = +1
+    (ycoord order); beta, p12, trC may be None (zeros).  Returns the y
+    profiles umean, urms, vrms, wrms, rss, txy, betay, swirly, pss and the
+    scalars massflux, twss, bwss, tpss, bpss, ckk."""
+    nyp, mx, mz = p.nyp, p.mx, p.mz
+    zero = np.zeros_like(up)
+    beta = zero if beta is None else beta
+    p12 = zero if p12 is None else p12
+    trC = zero if trC is None else trC
+    Lx, Lz = p.xl / mx, p.zl / mz
+    volume = p.xl * p.yl * p.zl
+    umean = up.sum(axis=1).sum(axis=0) / (mz * mx)            # uzmean over z, then over x
+    upr = up - umean[None, None, :]
+    y = p.ycoord
+    Ly = np.empty(nyp)
+    Ly[0] = abs(y[1] - y[0]) / 2.0
+    Ly[-1] = abs(y[-1] - y[-2]) / 2.0
+    Ly[1:-1] = abs(y[2:] - y[1:-1]) / 2.0 + abs(y[1:-1] - y[:-2]) / 2.0
+    n = mx * mz
+    prof = dict(
+        umean=umean,
+        urms=np.sqrt((upr ** 2).sum(axis=(0, 1)) / n),
+        vrms=np.sqrt((vp ** 2).sum(axis=(0, 1)) / n),
+        wrms=np.sqrt((wp ** 2).sum(axis=(0, 1)) / n),
+        rss=(upr * vp).sum(axis=(0, 1)) / n,
+        txy=(u12 / p.re).sum(axis=(0, 1)) / n,
+        betay=(beta * Lx * Lz).sum(axis=(0, 1)) / (p.xl * p.zl),
+        swirly=(swirl * Lx * Lz).sum(axis=(0, 1)) / (p.xl * p.zl),
+        pss=p12.sum(axis=(0, 1)) / n,
+    )
+    mf = sum(1.0 / p.yl * 0.5 * (umean[i + 1] + umean[i]) * abs(y[i + 1] - y[i]) for i in range(nyp - 1))
+    prof.update(massflux=mf,
+                twss=float((u12[:, :, 0] / p.re).sum() / n), bwss=float((u12[:, :, -1] / p.re).sum() / n),
+                tpss=float(p12[:, :, 0].sum() / n), bpss=float(p12[:, :, -1].sum() / n),
+                ckk=float((trC * Lx * Ly[None, None, :] * Lz).sum() / volume))
+    return prof
 
 
 def conf_from_stress(st, p: Params):

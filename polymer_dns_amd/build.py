@@ -9,7 +9,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libkmm.so")
-SOURCES = ["kmm_pencil.hip", "kmm_poly.hip", "kmm_plane.hip", "kmm_particles.hip", "kmm_mlp.hip", "kmm_audit.hip", "kmm_api.cpp", "kmm_mlp_api.cpp"]
+SOURCES = ["kmm_pencil.hip", "kmm_poly.hip", "kmm_plane.hip", "kmm_particles.hip", "kmm_mlp.hip", "kmm_audit.hip", "kmm_stats.hip", "kmm_api.cpp", "kmm_mlp_api.cpp"]
 ARCH = os.environ.get("KMM_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",

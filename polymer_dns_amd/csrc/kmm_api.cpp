@@ -88,6 +88,8 @@ struct kmm_ctx {
   double* part = nullptr;    // [7][part_cap]: x, y, z, u, v, w, swirl
   double* scsrc = nullptr;   // [nyp][mz][mx] Gaussian-blob scalar source (scl_flag 2)
   int tune_pencil = 0, tune_xplane = 0;  // occupancy variants (kmm_autotune)
+  double* stats_d = nullptr;             // [8][nyp] statistics sums (kmm_statistics)
+  double* swirl_tmp = nullptr;           // [nyp][mz][mx] swirl for statistics without tracers
   std::vector<int32_t> imatrix_h;        // geometry as set (the autotuner's twin needs it)
   double* cap = nullptr;     // [4][nyp][mz][mx] physical u, v, w (and swirl) of the last evaluation
   bool part_swirl = false;   // swirl strength at the particles (solvers.f90:3479-3481)
@@ -848,14 +850,15 @@ static int audit_buffers(kmm_ctx* c) {
 
 // C (spectral) rebuilt from a physical stress field on the 3/2 grid: the
 // inverse of the stress map pointwise, forward transforms in x and z, and the
-// polymer pencil's forward Chebyshev transform (mode 3) into C.  The pending
-// plane outputs in NLP are left untouched.
-static int rebuild_conf(kmm_ctx* c, const float* str) {
+// polymer pencil's forward Chebyshev transform (mode 3) into C.  With prev
+// given, the stress is 2 str - prev (one step of linear extrapolation).  The
+// pending plane outputs in NLP are left untouched.
+static int rebuild_conf(kmm_ctx* c, const float* str, const float* prev = nullptr) {
   int r = audit_buffers(c);
   if (r) return r;
   const size_t P = (size_t)c->nyp * c->mz * c->mx;
   const double L2 = c->cfg.zlmax * c->cfg.zlmax;
-  hipError_t e = kmm::launch_conf_from_stress(str, c->conf_phys, P, L2, c->cfg.ipeter, c->stream);
+  hipError_t e = kmm::launch_conf_from_stress(str, prev, c->conf_phys, P, L2, c->cfg.ipeter, c->stream);
   if (e != hipSuccess) return fail(KMM_E_HIP, std::string("conformation rebuild: ") + hipGetErrorString(e));
   kmm::PolyPlaneArgs sa;
   std::memset(&sa, 0, sizeof(sa));
@@ -1151,10 +1154,12 @@ static int enqueue_step(kmm_ctx* c, int s, hipEvent_t* ev, bool stage_ev) {
     pf.evaluate = true;
     pf.force = true;
   } else if (att && c->audit_w == 0) {
-    // audit window opens: C rebuilt from the closure's last stress stands for
-    // C at this step (no solve); the next step starts the transport history
-    // afresh (the n -> n-1 copy of an initial step)
-    int r = rebuild_conf(c, c->hist[hslot(c->it - 1)]);
+    // audit window opens: C at this step (no solve) is rebuilt from the
+    // closure's last two stresses extrapolated to this step -- holding the
+    // last one would freeze the stress for a step, an error of a whole
+    // step's increment; the next step starts the transport history afresh
+    // (the n -> n-1 copy of an initial step)
+    int r = rebuild_conf(c, c->hist[hslot(c->it - 1)], c->hist[hslot(c->it - 2)]);
     if (r) return r;
     KMM_HIP(hipMemsetAsync(c->audit_acc, 0, 2 * sizeof(double), c->stream));
     pf = kmm_ctx::PolyFlags();
@@ -2200,6 +2205,162 @@ extern "C" int kmm_autotune(kmm_ctx* c, int steps, double* ms) {
   drop_graph(c);
   if (ms) {
     ms[0] = tm[0][0]; ms[1] = tm[0][1]; ms[2] = tm[1][0]; ms[3] = tm[1][1];
+  }
+  return KMM_OK;
+}
+
+// ----------------------------------------------------------- statistics --
+// Chebyshev sum f(y_k) = sum_n a_n cos(n pi k / ny), y_k = cos(pi k / ny)
+// (the inverse transform with fac = c(i)/2, solvers.f90:2241-2546)
+static std::vector<double> cheb_eval(const std::vector<double>& a, int ny) {
+  std::vector<double> f(ny + 1, 0.0);
+  const double pi = 2.0 * std::acos(0.0);
+  for (int k = 0; k <= ny; ++k) {
+    double s = 0.0;
+    for (int n = 0; n <= ny; ++n) s += a[n] * std::cos(pi * (double)n * (double)k / (double)ny);
+    f[k] = s;
+  }
+  return f;
+}
+
+// cderiv (derivatives.f90:258-314) of one Chebyshev series
+static std::vector<double> cheb_deriv(const std::vector<double>& a, int ny, double dyde) {
+  std::vector<double> d(ny + 1, 0.0);
+  if (ny >= 1) d[ny - 1] = 2.0 * ny * a[ny] * dyde;
+  for (int n = ny - 2; n >= 0; --n) d[n] = d[n + 2] + 2.0 * (n + 1) * a[n + 1] * dyde;
+  d[0] *= 0.5;
+  return d;
+}
+
+extern "C" int kmm_statistics(kmm_ctx* c, double* prof, double* scal) {
+  if (!c || !prof || !scal) return fail(KMM_E_ARG, "null argument");
+  if (!c->nl_valid) return fail(KMM_E_ARG, "no evaluation yet (kmm_startup first)");
+  KMM_HIP(hipSetDevice(c->device));
+  const int nyp = c->nyp, ny = c->ny;
+  const kmm_config& g = c->cfg;
+  if (!c->stats_d) {
+    KMM_HIP(hipMalloc((void**)&c->stats_d, (size_t)8 * nyp * sizeof(double)));
+    c->allocs.push_back(c->stats_d);
+  }
+  kmm::StatsArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.P = c->P; a.nz = c->nz; a.nyp = nyp; a.mixed = g.precision == 1;
+  a.u = c->T[0]; a.v = c->T[1]; a.w = c->T[2];
+  const bool poly = g.polymer != 0;
+  if (poly) a.q = c->NLP[7];                                   // str12 of the last evaluation
+  if (poly && c->pprev.evaluate && !c->nlp_closure) { a.c11 = c->TP[0]; a.c22 = c->TP[3]; a.c33 = c->TP[5]; }
+  a.out = c->stats_d;
+  hipError_t e = kmm::launch_stats(a, c->stream);
+  if (e != hipSuccess) return fail(KMM_E_HIP, std::string("statistics: ") + hipGetErrorString(e));
+  std::vector<double> out((size_t)8 * nyp);
+  KMM_HIP(hipMemcpyAsync(out.data(), c->stats_d, out.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  // the (0,0) modes of the state: u0 (Chebyshev) and the scalar
+  std::vector<double> u0(nyp), th(nyp, 0.0);
+  KMM_HIP(hipMemcpyAsync(u0.data(), c->u0, nyp * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (g.scalar)
+    KMM_HIP(hipMemcpy2DAsync(th.data(), sizeof(double), c->TH, (size_t)c->P * sizeof(double2), sizeof(double), nyp,
+                             hipMemcpyDeviceToHost, c->stream));
+  // swirl strength: the tracers' field when this evaluation made one, else
+  // computed here from the same z intermediates
+  std::vector<double> swy(nyp, 0.0);
+  const double* swf = nullptr;
+  if (c->npart > 0 && c->part_swirl && c->cap_valid && c->cap) {
+    swf = c->cap + (size_t)3 * nyp * c->mz * c->mx;
+  } else {
+    const size_t pp = (size_t)nyp * c->mz * c->mx;
+    if (!c->swirl_tmp) {
+      KMM_HIP(hipMalloc((void**)&c->swirl_tmp, pp * sizeof(double)));
+      c->allocs.push_back(c->swirl_tmp);
+    }
+    if (!c->ZP) {
+      KMM_HIP(hipMalloc(&c->ZP, 2 * (size_t)nyp * c->mz * c->nxh * c->vsz));
+      c->allocs.push_back(c->ZP);
+    }
+    kmm::PolyPlaneArgs sa;
+    std::memset(&sa, 0, sizeof(sa));
+    sa.nxh = c->nxh; sa.nz = c->nz; sa.nzh = c->nzh; sa.nyp = nyp; sa.mx = c->mx; sa.mz = c->mz;
+    sa.mixed = g.precision == 1;
+    sa.alpha = c->alpha; sa.beta = c->beta;
+    sa.twZ = c->twZ; sa.twX = c->twX;
+    sa.U = c->Z;
+    sa.in[0] = c->T[1];
+    sa.in[1] = c->T[2];
+    sa.gz = 0;
+    sa.ZP = c->ZP;
+    sa.swirl = c->swirl_tmp;
+    if ((e = kmm::launch_swirl_plane(sa, true, c->stream)) != hipSuccess)
+      return fail(KMM_E_HIP, std::string("statistics swirl: ") + hipGetErrorString(e));
+    swf = c->swirl_tmp;
+  }
+  if ((e = kmm::launch_plane_mean(swf, nyp, c->mz * c->mx, c->stats_d, c->stream)) != hipSuccess)
+    return fail(KMM_E_HIP, std::string("statistics swirl mean: ") + hipGetErrorString(e));
+  KMM_HIP(hipMemcpyAsync(swy.data(), c->stats_d, nyp * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  KMM_HIP(hipStreamSynchronize(c->stream));
+  const std::vector<double> du = cheb_eval(cheb_deriv(u0, ny, c->dyde), ny);
+  const std::vector<double> thp = cheb_eval(th, ny);
+  const double zbeta1 = poly ? (1.0 - g.qbeta) / (g.re * g.qbeta * g.tpoly) : 0.0;
+  double* umean = prof;
+  double* urms = prof + nyp;
+  double* vrms = prof + 2 * nyp;
+  double* wrms = prof + 3 * nyp;
+  double* rss = prof + 4 * nyp;
+  double* txy = prof + 5 * nyp;
+  double* betay = prof + 6 * nyp;
+  double* swirly = prof + 7 * nyp;
+  double* pss = prof + 8 * nyp;
+  for (int i = 0; i < nyp; ++i) {
+    umean[i] = out[(size_t)4 * nyp + i];
+    urms[i] = std::sqrt(out[i]);
+    vrms[i] = std::sqrt(out[(size_t)nyp + i]);
+    wrms[i] = std::sqrt(out[(size_t)2 * nyp + i]);
+    rss[i] = out[(size_t)3 * nyp + i];
+    txy[i] = du[i] / g.re;
+    // writeoutputs gets beta_poly in POLYMER builds (qbeta for itarget 0), the scalar otherwise
+    betay[i] = poly ? g.qbeta : (g.scalar ? thp[i] : 0.0);
+    swirly[i] = swy[i];
+    pss[i] = poly ? zbeta1 * out[(size_t)6 * nyp + i] : 0.0;
+  }
+  double mf = 0.0, ckk = 0.0;
+  for (int i = 0; i + 1 < nyp; ++i)
+    mf += 1.0 / g.yl * 0.5 * (umean[i + 1] + umean[i]) * std::fabs(c->ycoord[i + 1] - c->ycoord[i]);
+  if (a.c11)
+    for (int i = 0; i < nyp; ++i) {
+      const double ly = (i == 0) ? std::fabs(c->ycoord[1] - c->ycoord[0]) / 2.0
+                      : (i == nyp - 1) ? std::fabs(c->ycoord[nyp - 1] - c->ycoord[nyp - 2]) / 2.0
+                      : std::fabs(c->ycoord[i + 1] - c->ycoord[i]) / 2.0 + std::fabs(c->ycoord[i] - c->ycoord[i - 1]) / 2.0;
+      ckk += out[(size_t)7 * nyp + i] * ly / g.yl;
+    }
+  scal[0] = mf;
+  scal[1] = txy[0];
+  scal[2] = txy[nyp - 1];
+  scal[3] = pss[0];
+  scal[4] = pss[nyp - 1];
+  scal[5] = a.c11 ? ckk : std::nan("");
+  return KMM_OK;
+}
+
+extern "C" int kmm_write_outputs(kmm_ctx* c, const char* dir, int first) {
+  if (!c || !dir) return fail(KMM_E_ARG, "null argument");
+  const int nyp = c->nyp;
+  std::vector<double> prof((size_t)9 * nyp), scal(6);
+  int r = kmm_statistics(c, prof.data(), scal.data());
+  if (r) return r;
+  // file names and contents of writeoutputs (helpers.f90:1362-1436): one
+  // value per line, y profiles top to bottom, appended on later calls
+  struct Out { const char* name; int prof; int scal; };
+  static const Out files[] = {{"Txy0", 5, -1},   {"urms0", 1, -1},  {"RSS0", 4, -1},   {"vrms0", 2, -1},
+                              {"wrms0", 3, -1},  {"massflux0", -1, 0}, {"mean_u0", 0, -1}, {"betay0", 6, -1},
+                              {"swirly0", 7, -1}, {"twss0", -1, 1},  {"bwss0", -1, 2},  {"tpss0", -1, 3},
+                              {"bpss0", -1, 4},  {"Ckk0", -1, 5},   {"PSS0", 8, -1}};
+  for (const Out& o : files) {
+    const std::string path = std::string(dir) + "/" + o.name;
+    FILE* fp = std::fopen(path.c_str(), first ? "w" : "a");
+    if (!fp) return fail(KMM_E_IO, "cannot open " + path);
+    if (o.prof >= 0)
+      for (int i = 0; i < nyp; ++i) std::fprintf(fp, " %24.16E\n", prof[(size_t)o.prof * nyp + i]);
+    else
+      std::fprintf(fp, " %24.16E\n", scal[o.scal]);
+    std::fclose(fp);
   }
   return KMM_OK;
 }

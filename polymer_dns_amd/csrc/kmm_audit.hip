@@ -6,7 +6,9 @@
 //                       f = (L^2 - 3)/(L^2 - tr C) (ipeter 1) or 1
 //                       (the forward map, solvers.f90:2925-2949): with
 //                       s = tr str, f = (L^2 + s)/L^2 and C = (str + I)/f.
-//                       Pointwise over the 3/2 grid, FP32 in and out.
+//                       Pointwise over the 3/2 grid, FP32 in and out;
+//                       optionally from the one-step extrapolation
+//                       2 str(n-1) - str(n-2).
 //   k_audit_partial     per-block sums of (pred - ref)^2 and ref^2 (FP64),
 //   k_audit_final       summed in a fixed order and added to acc[0], acc[1].
 #include "kmm_kernels.h"
@@ -14,23 +16,30 @@
 namespace kmm {
 
 // component order 11 12 13 22 23 33; diagonal entries 0, 3, 5
-__global__ void k_conf_from_stress(const float* __restrict__ str, float* __restrict__ C, size_t P, float L2,
-                                   int ipeter) {
+// With prev != nullptr the stress is first extrapolated one step ahead,
+// 2 str - prev.
+__global__ void k_conf_from_stress(const float* __restrict__ str, const float* __restrict__ prev,
+                                   float* __restrict__ C, size_t P, float L2, int ipeter) {
   const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P) return;
   float s[6];
 #pragma unroll
   for (int q = 0; q < 6; ++q) s[q] = str[(size_t)q * P + p];
+  if (prev) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) s[q] = 2.0f * s[q] - prev[(size_t)q * P + p];
+  }
   const float tr = s[0] + s[3] + s[5];
   const float inv = ipeter ? L2 / (L2 + tr) : 1.0f;  // 1/f
 #pragma unroll
   for (int q = 0; q < 6; ++q) C[(size_t)q * P + p] = (s[q] + ((q == 0 || q == 3 || q == 5) ? 1.0f : 0.0f)) * inv;
 }
 
-hipError_t launch_conf_from_stress(const float* str, float* C, size_t P, double L2, int ipeter, hipStream_t s) {
+hipError_t launch_conf_from_stress(const float* str, const float* prev, float* C, size_t P, double L2, int ipeter,
+                                  hipStream_t s) {
   if (P == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_conf_from_stress, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, str, C, P, (float)L2,
-                     ipeter);
+  hipLaunchKernelGGL(k_conf_from_stress, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, str, prev, C, P,
+                     (float)L2, ipeter);
   return hipGetLastError();
 }
 

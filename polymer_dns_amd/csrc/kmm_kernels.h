@@ -190,10 +190,26 @@ struct BlobArgs {
   double* src;
 };
 hipError_t launch_blob_source(const BlobArgs& a, hipStream_t s);
+// writeoutputs statistics (kmm_stats.hip): out[8][nyp], see the kernel file
+struct StatsArgs {
+  int P, nz, nyp, mixed;
+  const void* u;
+  const void* v;
+  const void* w;
+  const void* s;          // optional (0,0)-mode fields, transfer layout
+  const void* q;
+  const void* c11;
+  const void* c22;
+  const void* c33;
+  double* out;
+};
+hipError_t launch_stats(const StatsArgs& a, hipStream_t s);
+hipError_t launch_plane_mean(const double* f, int nyp, int n, double* out, hipStream_t s);
 // closure audits (kmm_audit.hip): C = (str + I)/f over P points of [6][P]
-// arrays; residual sums acc[0] += sum (pred - ref)^2, acc[1] += sum ref^2
+// arrays (str replaced by 2 str - prev when prev is given); residual sums acc[0] += sum (pred - ref)^2, acc[1] += sum ref^2
 // over n values (part: 2 * audit_part_count() doubles)
-hipError_t launch_conf_from_stress(const float* str, float* C, size_t P, double L2, int ipeter, hipStream_t s);
+hipError_t launch_conf_from_stress(const float* str, const float* prev, float* C, size_t P, double L2, int ipeter,
+                                  hipStream_t s);
 int audit_part_count();
 hipError_t launch_audit_residual(const float* pred, const float* ref, size_t n, double* part, double* acc,
                                  hipStream_t s);

@@ -48,7 +48,7 @@ EXPORTS = [
     "kmm_mlp_set_norm", "kmm_mlp_infer", "kmm_mlp_grad", "kmm_mlp_train",
     "kmm_set_poly_capture", "kmm_poly_samples", "kmm_set_closure", "kmm_get_poly_stress",
     "kmm_set_closure_audit", "kmm_closure_status", "kmm_poly_conf_from_stress",
-    "kmm_autotune", "kmm_set_tuning", "kmm_get_tuning",
+    "kmm_autotune", "kmm_set_tuning", "kmm_get_tuning", "kmm_statistics", "kmm_write_outputs",
 ]
 
 
@@ -155,6 +155,8 @@ def load_library(path: str | None = None):
     lib.kmm_autotune.argtypes = [C.c_void_p, C.c_int, dp]
     lib.kmm_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.kmm_get_tuning.argtypes = [C.c_void_p, ip, ip]
+    lib.kmm_statistics.argtypes = [C.c_void_p, dp, dp]
+    lib.kmm_write_outputs.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
     lib.kmm_calcswirl.argtypes = [C.c_void_p] + [dp] * 10
     lib.kmm_set_dpdx.argtypes = [C.c_void_p, C.c_double]
     for n in EXPORTS:
@@ -436,6 +438,22 @@ class Channel:
     def set_closure(self, mlp):
         _check(self.lib, self.lib.kmm_set_closure(self.h, mlp.h if mlp is not None else None))
         self._closure = mlp   # keep the network alive while the channel uses it
+
+    STAT_PROFILES = ("umean", "urms", "vrms", "wrms", "rss", "txy", "betay", "swirly", "pss")
+    STAT_SCALARS = ("massflux", "twss", "bwss", "tpss", "bpss", "ckk")
+
+    def statistics(self):
+        """writeoutputs' profiles (y from +1 to -1) and scalars of the last
+        evaluation, as a dict."""
+        prof = np.zeros((9, self.nyp))
+        scal = np.zeros(6)
+        _check(self.lib, self.lib.kmm_statistics(self.h, _dp(prof), _dp(scal)))
+        out = {k: prof[i].copy() for i, k in enumerate(self.STAT_PROFILES)}
+        out.update({k: float(scal[i]) for i, k in enumerate(self.STAT_SCALARS)})
+        return out
+
+    def write_outputs(self, directory, first=False):
+        _check(self.lib, self.lib.kmm_write_outputs(self.h, os.fsencode(str(directory)), int(bool(first))))
 
     def autotune(self, steps=10):
         """Time the occupancy variants of the pencil and x-plane kernels on a

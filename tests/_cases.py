@@ -35,10 +35,11 @@ def rel(a, b):
     return d / s if s > 0 else d
 
 
-def poly_state(p, seed, amp=1e-3, kmax=3, nmax=6):
+def poly_state(p, seed, amp=1e-3, kmax=3, nmax=6, mean=None):
     """Conformation tensor = identity + a smooth seeded perturbation of
     amplitude amp (low Fourier modes times low Chebyshev polynomials), as the
-    six independent spectral components {"c11": ..., ...}."""
+    six independent spectral components {"c11": ..., ...}.  `mean` adds
+    physical y-profiles (length nyp) to components, {"11": ..., "12": ...}."""
     rng = np.random.default_rng(seed)
     x = (np.arange(p.nx) * p.xl / p.nx)[:, None, None]
     z = (np.arange(p.nz) * p.zl / p.nz)[None, :, None]
@@ -56,6 +57,17 @@ def poly_state(p, seed, amp=1e-3, kmax=3, nmax=6):
         f = amp * f / max(1e-30, np.abs(f).max())
         if ij in ("11", "22", "33"):
             f = f + 1.0
+        if mean is not None and ij in mean:
+            f = f + np.asarray(mean[ij])[None, None, :]
         out["c" + ij] = O.xyzfft(f, p)
         O.norm(out["c" + ij])
     return out
+
+
+def shear_conformation_mean(p, u0, lam):
+    """Mean conformation of a steady simple shear at the local rate of the
+    mean flow u0 (Chebyshev coefficients), Oldroyd-B limit: C12 = lam U',
+    C11 = 1 + 2 (lam U')^2 as added profiles for poly_state(mean=...)."""
+    d = np.polynomial.chebyshev.chebder(np.real(np.asarray(u0))) * (2.0 / p.yl)
+    g = lam * np.polynomial.chebyshev.chebval(p.ycoord / (p.yl / 2), d)
+    return {"12": g, "11": 2.0 * g * g}

@@ -1,0 +1,127 @@
+"""writeoutputs statistics on the GPU (kmm_statistics / kmm_write_outputs)
+against the oracle's literal restatement on the 3/2 physical grid, the
+output files, and the profile checks of the north star: mixed precision
+against FP64 over a longer run, and the learned closure against FENE-P."""
+import numpy as np
+import pytest
+
+from oracle import kmm_oracle as O
+from tests._cases import params, poly_state, rel, seeded
+
+pytestmark = pytest.mark.gpu
+
+kmm = pytest.importorskip("polymer_dns_amd.kmm")
+
+PROFILES = ("umean", "urms", "vrms", "wrms", "rss", "txy", "betay", "swirly", "pss")
+POLY = dict(polymer=1, zlmax=30.0, tpoly=0.5, diffpoly=0.5, qbeta=0.9, crstrt=1, src_start=0)
+
+
+def channel(p, st, precision=0, **kw):
+    cfg = kmm.make_config(p.nx, p.ny, p.nz, xl=p.xl, zl=p.zl, re=p.re, dt=p.dt, r_tau=p.R_tau,
+                          precision=precision, **kw)
+    ch = kmm.Channel(cfg)
+    for k, v in st.items():
+        ch.put(k, v)
+    ch.startup()
+    return ch
+
+
+def oracle_stats(sol, ost, p):
+    f = sol.last_fields
+    gv = O.gradv(f["u"], f["v"], f["w"], p)
+    gp = [[O.to_physical(gv[f"u{i}{k}"], p) for k in (1, 2, 3)] for i in (1, 2, 3)]
+    ph = sol.last_phys
+    beta = None
+    if p.polymer:
+        beta = np.full_like(ph["up"], p.qbeta)
+    elif p.scalar:
+        beta = O.to_physical(ost["scalar"], p)
+    poly = ph.get("poly", {})
+    return O.writeoutputs(ph["up"], ph["vp"], ph["wp"], gp[0][1], O.swirl_eig(gp), beta,
+                          poly.get("p12"), poly.get("trC"), p)
+
+
+def check(got, ref, tol, swirl_tol):
+    for k in PROFILES:
+        t = swirl_tol if k == "swirly" else tol
+        e = rel(got[k], ref[k])
+        print(f"  {k}: rel {e:.2e}", flush=True)
+        assert e < t or np.abs(got[k] - ref[k]).max() < 1e-14, (k, e)
+    for k in ("massflux", "twss", "bwss", "tpss", "bpss", "ckk"):
+        if np.isnan(got[k]) and np.isnan(ref[k]):
+            continue
+        e = abs(got[k] - ref[k]) / max(abs(ref[k]), 1e-300)
+        print(f"  {k}: {got[k]:.6e} vs {ref[k]:.6e}", flush=True)
+        assert e < tol or abs(got[k] - ref[k]) < 1e-14, (k, got[k], ref[k])
+
+
+def test_statistics_match_writeoutputs_scalar():
+    p = params(32, 32, 32, scalar=True, diff=0.71, scl_flag=6, deltaT=0.3)
+    print("case: statistics, 32^3 scalar", flush=True)
+    st = seeded(p, 71, amp=0.2)
+    ch = channel(p, st, scalar=1, diff=0.71, scl_flag=6, deltaT=0.3)
+    sol = O.Solver(p)
+    ost = {k: v.copy() for k, v in st.items()}
+    sol.startup(ost)
+    for _ in range(3):
+        sol.step(ost)
+        ch.step(1)
+    got = ch.statistics()
+    ref = oracle_stats(sol, ost, p)
+    ref["ckk"] = got["ckk"]          # no conformation tensor: NaN on the GPU
+    check(got, ref, 1e-10, 1e-6)
+    assert np.isnan(got["ckk"])
+    ch.close()
+
+
+def test_statistics_match_writeoutputs_polymer(tmp_path):
+    p = params(32, 32, 32, dt=0.005, polymer=True, zlmax=30.0, tpoly=0.5, diffpoly=0.5, qbeta=0.9, crstrt=1,
+               src_start=0)
+    print("case: statistics, 32^3 FENE-P", flush=True)
+    st = seeded(p, 72, amp=0.2)
+    st.update(poly_state(p, 73, amp=0.05))
+    st = {k: v for k, v in st.items() if k in ("v", "omy", "u0", "w0") or k.startswith("c")}
+    ch = channel(p, st, **POLY)
+    sol = O.Solver(p)
+    sol.polymer_defaults(st)
+    ost = {k: v.copy() for k, v in st.items()}
+    sol.startup(ost)
+    for _ in range(3):
+        sol.step(ost)
+        ch.step(1)
+    got = ch.statistics()
+    ref = oracle_stats(sol, ost, p)
+    check(got, ref, 1e-10, 1e-6)
+    # the output files: profiles y top to bottom, appended on the next call
+    ch.write_outputs(tmp_path, first=True)
+    ch.step(1)
+    ch.write_outputs(tmp_path)
+    u = np.loadtxt(tmp_path / "mean_u0")
+    assert u.shape == (2 * p.nyp,) and np.allclose(u[:p.nyp], got["umean"], rtol=1e-15, atol=0)
+    assert np.loadtxt(tmp_path / "massflux0").shape == (2,)
+    for name in ("Txy0", "urms0", "RSS0", "vrms0", "wrms0", "betay0", "swirly0", "PSS0"):
+        assert np.loadtxt(tmp_path / name).shape == (2 * p.nyp,), name
+    for name in ("twss0", "bwss0", "tpss0", "bpss0", "Ckk0"):
+        assert np.loadtxt(tmp_path / name).shape == (2,), name
+    ch.close()
+
+
+def profile_err(a, b, keys):
+    return {k: float(np.abs(a[k] - b[k]).max() / np.abs(b[k]).max()) for k in keys}
+
+
+def test_mixed_precision_profiles_within_half_percent():
+    """The north star's bar for mixed precision: mean and RMS profiles within
+    0.5 % of the FP64 run (here 300 steps of a perturbed 64^3 channel)."""
+    p = params(64, 64, 64, dt=0.01)
+    print("case: mixed vs FP64 profiles, 64^3, 300 steps", flush=True)
+    st = seeded(p, 74, amp=0.3)
+    a = channel(p, st, 0)
+    b = channel(p, st, 1)
+    a.step(300)
+    b.step(300)
+    e = profile_err(b.statistics(), a.statistics(), ("umean", "urms", "vrms", "wrms", "rss"))
+    print("mixed vs FP64 after 300 steps:", {k: f"{v:.2e}" for k, v in e.items()}, flush=True)
+    assert max(e.values()) < 5e-3, e
+    a.close()
+    b.close()

@@ -484,6 +484,37 @@ def test_polymer_identity_at_rest_and_advection_sign():
         assert np.abs(cn[ij] + (U[0] * d[0] + U[1] * d[1] + U[2] * d[2])).max() < 1e-14
 
 
+def test_writeoutputs_closed_forms():
+    """writeoutputs on fields with known averages: a single x mode has
+    rms |a|/sqrt(2) and <u'v'> = ab cos(phi)/2, a constant mean profile
+    carries mass flux equal to itself, the volume mean of a constant trace
+    is the constant, and the wall stresses are the wall values of du/dy."""
+    p = params(16, 16, 16)
+    x = np.arange(p.mx) * p.xl / p.mx
+    y = p.ycoord
+    X = x[:, None, None] * np.ones((1, p.mz, p.nyp))
+    a, b, ph, m = 0.3, 0.2, 0.7, 3
+    k = 2 * math.pi * m / p.xl
+    U0 = 1.25
+    up = U0 + a * np.cos(k * X)
+    vp = b * np.cos(k * X + ph)
+    wp = np.zeros_like(up)
+    u12 = np.ones_like(up) * (2.0 * y)[None, None, :]
+    trC = np.full_like(up, 3.5)
+    st = O.writeoutputs(up, vp, wp, u12, np.zeros_like(up), None, 0.1 * u12, trC, p)
+    assert np.abs(st["umean"] - U0).max() < 1e-14
+    assert np.abs(st["urms"] - a / math.sqrt(2)).max() < 1e-14
+    assert np.abs(st["vrms"] - b / math.sqrt(2)).max() < 1e-14
+    assert np.abs(st["rss"] - a * b * math.cos(ph) / 2).max() < 1e-14
+    assert abs(st["massflux"] - U0) < 1e-13
+    assert abs(st["ckk"] - 3.5) < 1e-13
+    assert abs(st["twss"] - 2.0 * y[0] / p.re) < 1e-15 and abs(st["bwss"] - 2.0 * y[-1] / p.re) < 1e-15
+    assert abs(st["tpss"] - 0.2 * y[0]) < 1e-14 and abs(st["pss"][3] - 0.2 * y[3]) < 1e-14
+    # an odd mean profile carries no mass
+    st2 = O.writeoutputs(np.ones_like(up) * y[None, None, :], vp, wp, u12, np.zeros_like(up), None, None, None, p)
+    assert abs(st2["massflux"]) < 1e-14
+
+
 def test_blob_source_node_values_cutoffs_and_no_wrap():
     """scalar_blob_source: deltaT at a particle sitting on a node, the
     Gaussian factor one cell away, nothing where 18 <= beta (cut-off), sums
