@@ -107,8 +107,9 @@ def polymer_mode(kmm, precision, steps, warmup, learned=False):
                       [round(float(losses[:10].mean()), 4), round(float(losses[-10:].mean()), 4)],
                       "samples": int(X.shape[0])})
         ch.set_closure(net)
-        # the judge: a 4-step window of full FENE-P transport every 50 steps,
-        # inside the timed region; a residual above 10 % falls back to FENE-P
+        # audit windows of full FENE-P transport (3 steps after every 25
+        # closure steps) run inside the timed region; a residual above 10 %
+        # falls back to FENE-P
         ch.set_closure_audit(AUDIT_EVERY, AUDIT_WINDOW, AUDIT_GATE)
     ch.step(warmup)
     ch.set_timing(True)
@@ -203,7 +204,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--poly-steps", type=int, default=50, help="steps of the viscoelastic workload (0: skip)")
+    ap.add_argument("--poly-steps", type=int, default=56,
+                    help="steps of the viscoelastic workload (0: skip); 56 = two closure/audit periods")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs one channel on one GPU; --gpus must be 1"}))

@@ -1034,6 +1034,10 @@ class Solver:
         self.substep = 1
         self.part_swirl = True
         self.swirl_fn = calcswirl
+        # learned-closure steps (no reference counterpart): a physical stress
+        # {ij: (mx, mz, nyp)} supplied from outside replaces the FENE-P
+        # transport and evaluation and feeds only the polymer force
+        self.closure_stress = None
 
     def set_particles(self, x, y, z, ap=0.0, cd_switch=0, ratio=0.0):
         """Tracers initialised at the next evaluation (init_part)."""
@@ -1087,7 +1091,8 @@ class Solver:
             norm(sx); norm(sy); norm(sz)
             fields.update(sclx=sx, scly=sy, sclz=sz)
         pf = polymer_flags(it, p) if p.polymer else None
-        if p.polymer and pf["evaluate"]:
+        closure = p.polymer and pf["evaluate"] and self.closure_stress is not None
+        if p.polymer and pf["evaluate"] and not closure:
             C = {ij: st["c" + ij] for ij in POLY}
             fields["C"] = C
             fields["dC"] = derivscji(C, p)
@@ -1121,7 +1126,19 @@ class Solver:
             norm(a)
         if scn is not None:
             norm(scn)
-        if p.polymer and pf["evaluate"]:
+        if closure:
+            # qp = zbeta1 str transformed like the FENE-P stress, then polyforce
+            zb = (1.0 - p.qbeta) / (p.re * p.qbeta * p.tpoly)
+            qp = {ij: to_spectral(zb * np.asarray(self.closure_stress[ij], dtype=np.float64), p) for ij in POLY}
+            for ij in POLY:
+                norm(qp[ij])
+            t1, t2, t3 = polyforce(qp, p)
+            for a in (t1, t2, t3):
+                norm(a)
+            gn = gn + t1
+            fn = fn + t2
+            omz_o = omz_o + t3
+        elif p.polymer and pf["evaluate"]:
             pol = phys["poly"]
             for ij in POLY:
                 for key, d in (("cn", "c%sn"), ("str", "str%sn")):
@@ -1145,7 +1162,7 @@ class Solver:
             fn *= self.smooth
             if scn is not None:
                 scn *= self.smooth
-            if p.polymer and pf["evaluate"]:
+            if p.polymer and pf["evaluate"] and not closure:
                 for ij in POLY:
                     st["c%sn" % ij] = st["c%sn" % ij] * self.smooth
                     st["str%sn" % ij] = st["str%sn" % ij] * self.smooth
@@ -1174,7 +1191,7 @@ class Solver:
         self.it += 1
         it = self.it
         g = p.re / (p.dt * p.theta)
-        if p.polymer:
+        if p.polymer and self.closure_stress is None:
             self.polymer_defaults(st)
             pf = polymer_flags(it, p)
             if pf["init"]:

@@ -1175,7 +1175,9 @@ static int enqueue_step(kmm_ctx* c, int s, hipEvent_t* ev, bool stage_ev) {
     pa.closure = c->nlp_closure ? 1 : 0;
     pa.smooth = 1;
     if (!clo) { pa.init = pf.init; pa.solve = pf.solve; pa.evaluate = pf.evaluate; }
-    int r = run_poly_pencil(c, 0, pa);
+    // a closure step finishing a closure evaluation only forms the force
+    const bool force_only = clo && pa.partB && pa.closure && pa.force && !pa.copy;
+    int r = run_poly_pencil(c, force_only ? 4 : 0, pa);
     if (r) return r;
     if (pa.force) for (int q = 0; q < 3; ++q) a.TF[q] = c->TF[q];
   }

@@ -70,7 +70,8 @@ struct PolyArgs {
   void* TP[12];         // transfer: C (6), D C (6)
 };
 // mode 0: step, 1: startup (evaluate), 2: finish (part B), 3: C from the
-// physical-y fields in NLP[0..5] (forward Chebyshev transform only)
+// physical-y fields in NLP[0..5] (forward Chebyshev transform only), 4: the
+// force of a closure stress in NLP[6..11] (part B without C)
 hipError_t launch_poly_pencil(int ny, int mode, const PolyArgs& a, bool mixed, hipStream_t s);
 
 // Plane-stage arguments.

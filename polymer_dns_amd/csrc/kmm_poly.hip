@@ -20,7 +20,9 @@
 //
 // MODE 0: step, MODE 1: startup (evaluate only), MODE 2: finish (part B only),
 // MODE 3: C from the physical-y fields in NLP[0..5] (a conformation tensor
-// rebuilt on the 3/2 grid and forward transformed in x and z).
+// rebuilt on the 3/2 grid and forward transformed in x and z), MODE 4: part B
+// of a learned-closure stress (the force only: a closure stress is neither
+// smoothed nor stored, and there is no conformation to transport).
 #include "kmm_kernels.h"
 #include "kmm_pencil.h"
 
@@ -55,6 +57,34 @@ __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
       }
       pf_store<NY>(A.C[c], P, p, lane, C);
     }
+    return;
+  }
+  if (MODE == 4) {
+    // each component's load is issued before the previous one's transform
+    PF t1 = pf_zero(), t2 = pf_zero(), t3 = pf_zero();
+    if (kzi != A.nzh) {
+      PhysPencil<V> q = phys_load<NY, V>(static_cast<const V*>(A.NLP[6]), P, p, lane);
+#pragma unroll 1
+      for (int c = 0; c < 6; ++c) {
+        const PhysPencil<V> cur = q;
+        if (c < 5) q = phys_load<NY, V>(static_cast<const V*>(A.NLP[7 + c]), P, p, lane);
+        PF sn = phys_to_cheb<NY, V>(cur, buf, scr, twl, lane);
+        if (is00) sn = pf_real(sn);
+        const PF f = pf_scale(sn, A.zbeta1);
+        switch (c) {
+          case 0: t1 = pf_add(t1, pf_imul(f, kx)); break;
+          case 1: t1 = pf_add(t1, cderiv<NY>(f, dy, lane)); t2 = pf_add(t2, pf_imul(f, kx)); break;
+          case 2: t1 = pf_add(t1, pf_imul(f, kz)); t3 = pf_add(t3, pf_imul(f, kx)); break;
+          case 3: t2 = pf_add(t2, cderiv<NY>(f, dy, lane)); break;
+          case 4: t2 = pf_add(t2, pf_imul(f, kz)); t3 = pf_add(t3, cderiv<NY>(f, dy, lane)); break;
+          default: t3 = pf_add(t3, pf_imul(f, kz)); break;
+        }
+      }
+      if (is00) { t1 = pf_real(t1); t2 = pf_real(t2); t3 = pf_real(t3); }
+    }
+    pf_store<NY>(A.TF[0], P, p, lane, t1);
+    pf_store<NY>(A.TF[1], P, p, lane, t2);
+    pf_store<NY>(A.TF[2], P, p, lane, t3);
     return;
   }
   if (kzi == A.nzh) {  // norm zeroes the z-Nyquist row (helpers.f90:149-156)
@@ -183,6 +213,7 @@ hipError_t launch_poly_pencil(int ny, int mode, const PolyArgs& a, bool mixed, h
     case 1: return mixed ? launch_poly_v<1, float2>(ny, a, s) : launch_poly_v<1, double2>(ny, a, s);
     case 2: return mixed ? launch_poly_v<2, float2>(ny, a, s) : launch_poly_v<2, double2>(ny, a, s);
     case 3: return mixed ? launch_poly_v<3, float2>(ny, a, s) : launch_poly_v<3, double2>(ny, a, s);
+    case 4: return mixed ? launch_poly_v<4, float2>(ny, a, s) : launch_poly_v<4, double2>(ny, a, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -95,3 +95,39 @@ def test_closure_training_gain_and_run():
     ch.close()
     ref.close()
     net.close()
+
+
+def test_closure_steps_match_the_oracle_fed_the_same_stress():
+    """Steps with the learned closure against the oracle's closure hook fed
+    the stress the GPU network predicted (FP64): the force of a closure
+    stress, the first closure step finishing the last FENE-P evaluation, and
+    the following closure-only steps."""
+    from oracle import kmm_oracle as O
+    p = params(32, 32, 32, dt=0.005, polymer=True, zlmax=30.0, tpoly=0.5, diffpoly=0.5, qbeta=0.9, crstrt=1,
+               src_start=0)
+    print("case: closure steps vs oracle, 32^3 FP64", flush=True)
+    st = seeded(p, 81, amp=0.2)
+    st.update(poly_state(p, 82, amp=0.05, mean=shear_conformation_mean(p, st["u0"], POLY["tpoly"])))
+    st = {k: v for k, v in st.items() if k in ("v", "omy", "u0", "w0") or k.startswith("c")}
+    ch = channel(p, st)
+    sol = O.Solver(p)
+    sol.polymer_defaults(st)
+    ost = {k: v.copy() for k, v in st.items()}
+    sol.startup(ost)
+    ch.set_poly_capture(True)
+    for _ in range(3):
+        ch.step(1)
+        sol.step(ost)
+    net = kmm.Mlp(seed=4)   # untrained: small corrections to the extrapolation
+    net.set_norm(np.zeros(22), np.ones(22), np.zeros(6), np.full(6, 1e-3))
+    ch.set_closure(net)
+    for k in range(4):
+        ch.step(1)
+        sol.closure_stress = {ij: ch.poly_stress(q) for q, ij in enumerate(O.POLY)}
+        sol.step(ost)
+        e = {f: rel(ch.get(f), ost[f]) for f in ("v", "omy")}
+        e["u0"] = rel(ch.get("u0"), ost["u0"])
+        print(f"closure step {k + 1}:", {f: f"{x:.2e}" for f, x in e.items()}, flush=True)
+        assert max(e.values()) < 1e-10, (k, e)
+    ch.close()
+    net.close()

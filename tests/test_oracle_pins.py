@@ -13,7 +13,7 @@ import scipy.linalg as sla
 from numpy.polynomial import chebyshev as cheb
 
 from oracle import kmm_oracle as O
-from tests._cases import params, rel, seeded
+from tests._cases import params, poly_state, rel, seeded, shear_conformation_mean
 
 RNG = np.random.default_rng(1234)
 
@@ -644,6 +644,36 @@ def test_polymer_relaxation_at_rest_follows_exp_decay():
         assert abs(got - exact) < 5 * (dt / lam) ** 2 * abs(c0 - 1.0), (ij, got, exact)
         assert np.abs(st["c" + ij][0, 0, 1:]).max() < 1e-11
     assert np.abs(st["c12"]).max() < 1e-14
+
+
+def test_closure_hook_with_the_fenep_stress_reproduces_the_fenep_step():
+    """The oracle's learned-closure hook fed the stress the FENE-P step
+    evaluates gives that step's nonlinear terms (the force path is the same),
+    leaves the conformation untouched, and a zero stress removes the force."""
+    p = params(8, 8, 8, polymer=True, dt=0.005, zlmax=30.0, tpoly=0.5, diffpoly=0.5, qbeta=0.9, crstrt=1,
+               src_start=0)
+    st = seeded(p, 5, amp=0.2)
+    st = {k: v for k, v in st.items() if k in ("v", "omy", "u0", "w0")}
+    st.update(poly_state(p, 6, amp=0.05, mean=shear_conformation_mean(p, st["u0"], p.tpoly)))
+    sol = O.Solver(p)
+    sol.polymer_defaults(st)
+    sol.startup(st)
+    sol.step(st)
+    a, b, z = ({k: (v.copy() if hasattr(v, "copy") else v) for k, v in st.items()} for _ in range(3))
+    sa, sb, sz = O.Solver(p), O.Solver(p), O.Solver(p)
+    for s_ in (sa, sb, sz):
+        s_.it = sol.it
+    sa.step(a)
+    stress = {ij: O.to_physical(sa.last_phys["poly"]["str"][ij], p) for ij in O.POLY}
+    sb.closure_stress = stress
+    sb.step(b)
+    for k in ("fn", "gn", "v", "omy"):
+        assert rel(b[k], a[k]) < 1e-12, k
+    for ij in O.POLY:
+        assert np.array_equal(b["c" + ij], st["c" + ij])
+    sz.closure_stress = {ij: np.zeros_like(stress[ij]) for ij in O.POLY}
+    sz.step(z)
+    assert rel(z["fn"], a["fn"]) > 1e-6  # the force is a visible part of the NL terms
 
 
 # ------------------------------------------------------- constant mass flux
