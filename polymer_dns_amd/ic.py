@@ -130,9 +130,49 @@ def seeded_state(nx, ny, nz, xl, zl, re, R_tau, seed, amp=0.05, kmax=8, yl=2.0):
 POLY = ("11", "12", "13", "22", "23", "33")
 
 
-def conformation_state(nx, ny, nz, seed, amp=1e-3, kmax=4, nmax=6):
+def shear_equilibrium(u0, tpoly, zlmax, ipeter=1, yl=2.0):
+    """Steady FENE-P conformation of a homogeneous simple shear at the local
+    rate U'(y) of the mean profile u0 (Chebyshev coefficients), at the
+    Gauss-Lobatto points y_k = cos(k pi/ny) (yl/2).  With W = tpoly U' the
+    steady equations L C + C L^T = (f C - I)/tpoly give C22 = C33 = 1/f,
+    C12 = W/f^2, C11 = 1/f + 2 W^2/f^3, and the Peterlin function
+    f = (L^2 - 3)/(L^2 - tr C) (ipeter 1) reduces to the cubic
+    L^2 f^2 (f - 1) = 2 W^2 (root f >= 1, by Newton); ipeter 0: f = 1.
+    Returns {"11", "12", "22", "33"} profiles (length ny+1)."""
+    u0 = np.real(np.asarray(u0, dtype=np.complex128))
+    ny = u0.size - 1
+    y = np.cos(np.arange(ny + 1) * math.pi / ny)
+    W = tpoly * cheb.chebval(y, cheb.chebder(u0)) * (2.0 / yl)
+    f = np.ones_like(W)
+    if ipeter:
+        L2 = zlmax * zlmax
+        f = 1.0 + np.cbrt(2.0 * W * W / L2)          # above the root
+        for _ in range(60):
+            g = L2 * f * f * (f - 1.0) - 2.0 * W * W
+            dg = L2 * (3.0 * f * f - 2.0 * f)
+            f = f - g / dg
+    return {"11": 1.0 / f + 2.0 * W * W / f ** 3, "12": W / f ** 2, "22": 1.0 / f, "33": 1.0 / f}
+
+
+def _cheb_coeffs(vals):
+    """Chebyshev coefficients of values at the Gauss-Lobatto points
+    y_k = cos(k pi/N) (DCT-I, ends halved)."""
+    N = vals.size - 1
+    k = np.arange(N + 1)
+    w = np.ones(N + 1)
+    w[0] = w[N] = 0.5
+    a = (2.0 / N) * (np.cos(np.outer(k, k) * math.pi / N) @ (w * vals))
+    a[0] *= 0.5
+    a[N] *= 0.5
+    return a
+
+
+def conformation_state(nx, ny, nz, seed, amp=1e-3, kmax=4, nmax=6, mean=None):
     """Conformation tensor C = I + seeded noise, as the six independent
     spectral components {"c11": ..., "c33": ...} (complex (nx/2, nz, ny+1)).
+    With ``mean`` (profiles at the Gauss-Lobatto points, e.g. from
+    ``shear_equilibrium``) the mean of each given component is that profile
+    instead of the identity's.
 
     Noise: for 0 <= kx <= kmax, |kz| <= kmax and Chebyshev n <= nmax, complex
     N(0,1) coefficients from SplitMix64 scaled by ``amp / (1 + kx + |kz| + n)``;
@@ -154,7 +194,9 @@ def conformation_state(nx, ny, nz, seed, amp=1e-3, kmax=4, nmax=6):
         a[0, nz - j, :] = np.conj(a[0, j, :])
         a[0, 0, :] = a[0, 0, :].real
         a[:, nz // 2, :] = 0.0
-        if ij in ("11", "22", "33"):
+        if mean is not None and ij in mean:
+            a[0, 0, :] += _cheb_coeffs(np.asarray(mean[ij], dtype=np.float64))
+        elif ij in ("11", "22", "33"):
             a[0, 0, 0] += 1.0
         out["c" + ij] = a
     return out

@@ -464,6 +464,27 @@ def test_simple_shear_steady_state(ipeter):
     assert np.abs(cn2["12"] - strp2["12"] / lam).max() > 1e-3
 
 
+@pytest.mark.parametrize("ipeter", [0, 1])
+def test_seeded_shear_equilibrium_is_a_steady_state(ipeter):
+    """The seeded initial conformation of the mean shear (ic.shear_equilibrium)
+    is, at every y, a steady state of the oracle's FENE-P products for the
+    local shear rate of the laminar profile."""
+    from polymer_dns_amd import ic
+    p = params(8, 16, 8, polymer=True, tpoly=0.2, zlmax=12.0, ipeter=ipeter)
+    u0, _, _ = ic.laminar_u0(p.ny, p.re, p.R_tau)
+    m = ic.shear_equilibrium(u0, p.tpoly, p.zlmax, ipeter)
+    dudy = np.polynomial.chebyshev.chebval(p.ycoord, np.polynomial.chebyshev.chebder(u0))
+    for k in (0, 3, 8, 13, 16):
+        C = {"11": m["11"][k], "12": m["12"][k], "13": 0.0, "22": m["22"][k], "23": 0.0, "33": m["33"][k]}
+        g = np.zeros((3, 3))
+        g[0, 1] = dudy[k]
+        cn, strp, _ = O.polymer_products(*_uniform_poly((2, 2, 2), C, gradu=g), p)
+        for ij in O.POLY:
+            assert np.abs(cn[ij] - strp[ij] / p.tpoly).max() < 1e-11 * max(1.0, m["11"][k]), (k, ij)
+    if not ipeter:
+        assert np.allclose(m["22"], 1.0) and np.allclose(m["12"], p.tpoly * dudy)
+
+
 def test_polymer_identity_at_rest_and_advection_sign():
     p = params(8, 8, 8, polymer=True)
     I = {"11": 1.0, "12": 0.0, "13": 0.0, "22": 1.0, "23": 0.0, "33": 1.0}
