@@ -27,7 +27,8 @@ extern "C" {
 
 typedef enum kmm_status {
   KMM_OK = 0,
-  KMM_E_CFL = 1,      /* CFL > 1 (solvers.f90:3503-3514 "CFL failure") */
+  KMM_E_CFL = 1,      /* CFL > 1 (solvers.f90:3503-3514 "CFL failure"), or the inertial
+                         particle CFL >= 10 (part_track.f90:216-221) */
   KMM_E_PARTICLE = 2, /* particle left the domain (part_track.f90:498-504) */
   KMM_E_HIP = 3,      /* HIP runtime error */
   KMM_E_CONFIG = 4,   /* unsupported or inconsistent configuration */
@@ -60,9 +61,11 @@ typedef struct kmm_config {
   int cadence;
   int precision;          /* 0: FP64 throughout; 1: mixed (FP32 transforms, products
                              and transfer fields; FP64 state and tau solves) */
-  int particle_flag;      /* 0: tracers (the only kind built) */
+  int particle_flag;      /* 0: tracers; > 0: inertial Maxey-Riley particles (RK4,
+                             part_track.f90:617-1075) */
   double ratio, ap, c_mu; /* particle density ratio, radius, C_mu */
-  int cd_switch;          /* drag law; sets the tracer sub-step (part_track.f90:338-346) */
+  int cd_switch;          /* drag law 0-3 (part_track.f90:826-840); sets the particle
+                             sub-step (part_track.f90:338-346) */
   /* FENE-P conformation tensor (POLYMER build; dns.config lines 35-37, 55-62) */
   int polymer;            /* 1 = POLYMER build */
   int itarget;            /* 0: uniform beta = qbeta (the only mode built) */
@@ -83,6 +86,8 @@ typedef struct kmm_config {
                              solvers.f90:3479-3481); kmm_config_read sets 1 */
   double sigmax, sigmay, sigmaz;  /* Gaussian blob widths of the scl_flag 2 source
                                      (dns.config line 65) */
+  double gravity;         /* buoyancy on inertial particles, -gravity (Pi_f - Pi_p) on
+                             dv/dt (dns.config line 23, part_track.f90:873) */
 } kmm_config;
 
 typedef struct kmm_ctx kmm_ctx;

@@ -807,7 +807,7 @@ def vcw3dp(st, p: Params, it, fields, particles=None):
     fn = to_spectral(vwy, p)
     omz_o = to_spectral(vwz, p)
     scn = to_spectral(vc, p) if vc is not None else None
-    phys = dict(up=up, vp=vp, wp=wp)
+    phys = dict(up=up, vp=vp, wp=wp, wxp=wxp, wyp=wyp, wzp=wzp)
     if "C" in fields:
         gv = fields["gradv"]
         gup = [[to_physical(gv[f"u{i}{k}"], p) for k in (1, 2, 3)] for i in (1, 2, 3)]
@@ -1039,14 +1039,19 @@ class Solver:
         # transport and evaluation and feeds only the polymer force
         self.closure_stress = None
 
-    def set_particles(self, x, y, z, ap=0.0, cd_switch=0, ratio=0.0):
-        """Tracers initialised at the next evaluation (init_part)."""
+    def set_particles(self, x, y, z, ap=0.0, cd_switch=0, ratio=0.0, particle_flag=0, c_mu=1.0, gravity=0.0):
+        """Particles initialised at the next evaluation (init_part): tracers
+        (particle_flag 0) or inertial Maxey-Riley particles (> 0)."""
         self.particles = dict(x=np.asarray(x, float).copy(), y=np.asarray(y, float).copy(),
                               z=np.asarray(z, float).copy(), u=np.zeros(len(x)), v=np.zeros(len(x)),
                               w=np.zeros(len(x)))
         self.part_state = 1
         self.ap = ap
         self.substep = substep_check(self.p, cd_switch, ratio, ap)
+        self.particle_flag = particle_flag
+        self.mp = dict(ap=ap, ratio=ratio, c_mu=c_mu, cd_switch=cd_switch, gravity=gravity)
+        self.mr_old = None
+        self.umax1 = 0.0   # saved across steps (part_track.f90:104)
 
     # main_dns.f90:1602-2231 (new-run branch, from spectral u/v/w)
     def startup(self, st):
@@ -1106,10 +1111,24 @@ class Solver:
             # part_track (part_track.f90:131-168): initialise at the first
             # evaluation, Euler-advance afterwards
             uvw = (phys["up"], phys["vp"], phys["wp"])
+            inertial = getattr(self, "particle_flag", 0) > 0
+            fl = mr_fields(u, v, w, phys, p) if inertial else None
             if self.part_state == 1:
                 self.particles = tracer_init(self.particles, uvw, p)
+            elif inertial:
+                # part_track.f90:131-150: substep RK4 sub-steps of dt/substep
+                dts = p.dt / float(self.substep)
+                for _ in range(self.substep):
+                    self.particles, umax = mr_rk4(self.particles, fl, self.mr_old, p, self.mp, dts)
+                    self.umax1 = max(self.umax1, umax)
+                # particle CFL (part_track.f90:211-221)
+                if self.umax1 * p.dt / self.ap >= 10.0:
+                    raise ParticleCFLError(f"particle cfl failure at time step {it}")
             else:
                 self.particles = tracer_update(self.particles, uvw, p, self.ap, self.substep)
+            if inertial:
+                # u_old, Lu_old for the next step (solvers.f90:3468-3474)
+                self.mr_old = dict(u=fl["u"], v=fl["v"], w=fl["w"], L=fl["L"])
             self.part_state = 2
             if self.part_swirl:
                 # swirl_part = swirl_3d at the particles (solvers.f90:3479-3481)
@@ -1353,3 +1372,103 @@ def tracer_init(part, phys, p: Params):
     """init_part (part_track.f90:229-292): particle velocity = fluid velocity."""
     u, v, w = fluid_interp(part["x"], part["y"], part["z"], phys, p)
     return dict(x=part["x"].copy(), y=part["y"].copy(), z=part["z"].copy(), u=u, v=v, w=w)
+
+
+# --------------------------------------------------------------------------- #
+#  Inertial particles: Maxey-Riley equations by RK4 (part_track.f90:617-1075)
+# --------------------------------------------------------------------------- #
+class ParticleCFLError(RuntimeError):
+    pass
+
+
+def mr_fields(u, v, w, phys, p: Params):
+    """The physical fields part_track receives (solvers.f90:3464-3466): u, v,
+    w and the vorticity of vcw3dp, the nine gradients u_ik = du_i/dx_k and
+    the Laplacians of gradv (derivatives.f90:6-86, normed as in
+    main_dns.f90:544-558), all on the 3/2 grid (mx, mz, nyp)."""
+    gv = gradv(u, v, w, p)
+    for a in gv.values():
+        norm(a)
+    return dict(u=phys["up"], v=phys["vp"], w=phys["wp"], ox=phys["wxp"], oy=phys["wyp"], oz=phys["wzp"],
+                g=[to_physical(gv[f"u{i}{k}"], p) for i in (1, 2, 3) for k in (1, 2, 3)],
+                L=[to_physical(gv[k], p) for k in ("Lu", "Lv", "Lw")])
+
+
+def mr_forces(xp, yp, zp, un, vn, wn, fl, old, p: Params, mp, dt):
+    """RKstage (part_track.f90:805-873) and SubDerivative (:896-1075): the
+    Maxey-Riley acceleration (pressure gradient / added mass, drag with the
+    Faxen term, lift, Faxen time derivative, buoyancy).  dt is the particle
+    sub-step: the reference divides dt by substep before the integration,
+    so the fluid time derivatives (u - u_old)/dt use it too."""
+    ap, re = mp["ap"], p.re
+    pi_f = 1.0 / (mp["ratio"] + 0.5)
+    pi_p = mp["ratio"] / (mp["ratio"] + 0.5)
+    c_l = 0.5
+    uf, vf, wf = fluid_interp(xp, yp, zp, [fl["u"], fl["v"], fl["w"]], p)
+    wxf, wyf, wzf = fluid_interp(xp, yp, zp, [fl["ox"], fl["oy"], fl["oz"]], p)
+    luf, lvf, lwf = fluid_interp(xp, yp, zp, fl["L"], p)
+    g = fluid_interp(xp, yp, zp, fl["g"], p)
+    # corner values (u - u_old)/dt interpolated (part_track.f90:1001-1009)
+    dudt = fluid_interp(xp, yp, zp, [(fl[k] - old[k]) / dt for k in ("u", "v", "w")], p)
+    dldt = fluid_interp(xp, yp, zp, [(fl["L"][i] - old["L"][i]) / dt for i in range(3)], p)
+    sdu = dudt[0] + uf * g[0] + vf * g[1] + wf * g[2]
+    sdv = dudt[1] + uf * g[3] + vf * g[4] + wf * g[5]
+    sdw = dudt[2] + uf * g[6] + vf * g[7] + wf * g[8]
+    re_p = 2.0 * ap * re * np.sqrt((uf - un) ** 2 + (vf - vn) ** 2 + (wf - wn) ** 2)
+    cd = mp["cd_switch"]
+    if cd == 0:
+        c_d = np.zeros_like(re_p)
+    elif cd == 1:
+        c_d = np.full_like(re_p, 16.0 / (2.0 * ap * re) * mp["c_mu"])
+    elif cd == 2:
+        c_d = 24.0 / (2.0 * ap * re) * (1.0 + 0.15 * re_p ** 0.687)
+    else:
+        with np.errstate(divide="ignore"):
+            c_d = np.where(re_p > 1.0, 16.31 * re_p ** (-0.8331), 17.28 * re_p ** (-0.9711))
+    a2 = ap * ap / 6.0
+    ru = uf - un + a2 * luf
+    rv = vf - vn + a2 * lvf
+    rw = wf - wn + a2 * lwf
+    dfac = c_d * 3.0 / (8.0 * ap)
+    ffac = 3.0 / 20.0 * (ap * ap)
+    ku = pi_f * (1.5 * sdu + dfac * ru + c_l * (rv * wzf - rw * wyf) + ffac * dldt[0])
+    kv = pi_f * (1.5 * sdv + dfac * rv + c_l * (rw * wxf - ru * wzf) + ffac * dldt[1]) - mp["gravity"] * (pi_f - pi_p)
+    kw = pi_f * (1.5 * sdw + dfac * rw + c_l * (ru * wyf - rv * wxf) + ffac * dldt[2])
+    return ku, kv, kw
+
+
+def mr_rk4(part, fl, old, p: Params, mp, dt):
+    """RK4 (part_track.f90:617-755) of one sub-step dt for every particle:
+    stages at dt/2, dt/2, dt from the start state, positions periodic in x
+    and z and clamped to [ap + y_min, y_max - ap] in y, an elastic bounce of
+    v at a clamped wall.  Returns the new state and max |u_f - u_p| at the new
+    positions."""
+    ap = mp["ap"]
+    height = p.ycoord[0] - ap
+    bottom = ap + p.ycoord[-1]
+    x0, y0, z0 = part["x"], part["y"], part["z"]
+    up, vp, wp = part["u"], part["v"], part["w"]
+    xp, yp, zp, un, vn, wn = x0, y0, z0, up, vp, wp
+    ks, vs = [], []
+    for sn in (1, 2, 3, 4):
+        ku, kv, kw = mr_forces(xp, yp, zp, un, vn, wn, fl, old, p, mp, dt)
+        h = dt if sn == 3 else dt / 2.0
+        un, vn, wn = up + h * ku, vp + h * kv, wp + h * kw
+        xp = np.fmod(p.xl + x0 + h * un, p.xl)
+        yp = np.maximum(np.minimum(y0 + h * vn, height), bottom)
+        zp = np.fmod(p.zl + z0 + h * wn, p.zl)
+        ks.append((ku, kv, kw))
+        vs.append((un, vn, wn))
+    (u2, v2, w2), (u3, v3, w3), (u4, v4, w4) = vs[:3]
+    (k1u, k1v, k1w), (k2u, k2v, k2w), (k3u, k3v, k3w), (k4u, k4v, k4w) = ks
+    x = np.fmod(p.xl + x0 + dt * (up / 6 + u2 / 3 + u3 / 3 + u4 / 6), p.xl)
+    y = np.maximum(np.minimum(y0 + dt * (vp / 6 + v2 / 3 + v3 / 3 + v4 / 6), height), bottom)
+    z = np.fmod(p.zl + z0 + dt * (wp / 6 + w2 / 3 + w3 / 3 + w4 / 6), p.zl)
+    u = up + dt * (k1u / 6 + k2u / 3 + k3u / 3 + k4u / 6)
+    vnew = vp + dt * (k1v / 6 + k2v / 3 + k3v / 3 + k4v / 6)
+    w = wp + dt * (k1w / 6 + k2w / 3 + k3w / 3 + k4w / 6)
+    bounce = ((y == height) & (vnew > 0.0)) | ((y == bottom) & (vnew < 0.0))
+    v = np.where(bounce, -vnew, vnew)
+    uf, vf, wf = fluid_interp(x, y, z, [fl["u"], fl["v"], fl["w"]], p)
+    umax = float(np.sqrt((uf - u) ** 2 + (vf - v) ** 2 + (wf - w) ** 2).max()) if x.size else 0.0
+    return dict(x=x, y=y, z=z, u=u, v=v, w=w), umax

@@ -97,6 +97,13 @@ struct kmm_ctx {
   int* perr = nullptr;
   int part_state = 0;        // 0 none, 1 initialise at the next evaluation, 2 tracking
   int substep = 1;
+  // inertial particles (particle_flag > 0): FP64 fields on the 3/2 grid
+  double* mrf = nullptr;                 // [12][P] vorticity, velocity gradient
+  double* lap[2] = {nullptr, nullptr};   // [3][P] Laplacians, current / previous evaluation
+  int lap_cur = 0;
+  double* uold = nullptr;                // [3][P] u, v, w of the previous evaluation
+  void* lapT = nullptr;                  // [3] Laplacians in physical y (transfer precision)
+  void* lapZ = nullptr;                  // [3] after the z inverse
   bool cap_valid = false;
   double* dpdx_d = nullptr;  // mean pressure gradient (device; updated under constant mass flux)
   // captured steady steps (kmm_set_graph)
@@ -153,7 +160,7 @@ static int dalloc(kmm_ctx* c, void** p, size_t bytes) {
 extern "C" const char* kmm_status_string(int s) {
   switch (s) {
     case KMM_OK: return "ok";
-    case KMM_E_CFL: return "CFL failure";
+    case KMM_E_CFL: return g_last_error.empty() ? "CFL failure" : g_last_error.c_str();
     case KMM_E_PARTICLE: return "particle out of domain";
     case KMM_E_HIP: return g_last_error.empty() ? "HIP error" : g_last_error.c_str();
     case KMM_E_CONFIG: return g_last_error.empty() ? "configuration error" : g_last_error.c_str();
@@ -211,7 +218,8 @@ extern "C" int kmm_config_read(const char* path, kmm_config* cfg) {
   RD(4, nsteps, int); RD(5, iprnfrq, int); RD(6, dt, double); RD(7, gain, double); RD(8, ugain, double);
   RD(9, theta, double); RD(10, ny, int); RD(11, nz, int); RD(12, nx, int); RD(14, yl, double);
   RD(15, zl, double); RD(16, xl, double); RD(17, re, double); RD(19, uinf, double); RD(20, kwall, int);
-  RD(21, perturbtime, int); RD(22, r_tau, double); RD(29, irstrt, int); RD(30, crstrt, int);
+  RD(21, perturbtime, int); RD(22, r_tau, double); RD(23, gravity, double); RD(29, irstrt, int);
+  RD(30, crstrt, int);
   RD(34, flow_select, int); RD(38, scl_flag, int); RD(63, diff, double); RD(64, deltaT, double);
   RD(66, src_start, int); RD(67, src_stop, int); RD(74, cadence, int);
   {
@@ -489,6 +497,8 @@ fail_:
   return r;
 }
 
+static int part_free(kmm_ctx* c);
+
 extern "C" int kmm_destroy(kmm_ctx* c) {
   if (!c) return KMM_OK;
   hipSetDevice(c->device);
@@ -496,8 +506,7 @@ extern "C" int kmm_destroy(kmm_ctx* c) {
   if (c->gexec) hipGraphExecDestroy(c->gexec);
   if (c->graph) hipGraphDestroy(c->graph);
   for (void* p : c->allocs) hipFree(p);
-  if (c->part) hipFree(c->part);
-  if (c->cap) hipFree(c->cap);
+  part_free(c);
   for (auto& e : c->ev) if (e) hipEventDestroy(e);
   for (auto& e : c->tev) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -788,7 +797,7 @@ static void hist_note(kmm_ctx* c, int it) {
 // learned closure at step it: velocity gradient on the 3/2 grid, the network
 // on every point (inputs: gradient, wall distance, stress of it-1 and it-2),
 // forward transforms of the predicted stress into NLP[6..11]
-static int run_closure_plane(kmm_ctx* c, int it, bool swirl) {
+static int run_closure_plane(kmm_ctx* c, int it, bool swirl, double* mrf) {
   kmm::PolyPlaneArgs sa;
   std::memset(&sa, 0, sizeof(sa));
   sa.nxh = c->nxh; sa.nz = c->nz; sa.nzh = c->nzh; sa.nyp = c->nyp; sa.mx = c->mx; sa.mz = c->mz;
@@ -803,6 +812,7 @@ static int run_closure_plane(kmm_ctx* c, int it, bool swirl) {
   sa.ZFP = c->ZFP;
   sa.capg = c->capg;
   sa.swirl = swirl ? c->cap + (size_t)3 * c->nyp * c->mz * c->mx : nullptr;
+  sa.mrf = mrf;
   hipError_t e = kmm::launch_swirl_plane(sa, true, c->stream);
   if (e != hipSuccess) return fail(KMM_E_HIP, std::string("closure gradient: ") + hipGetErrorString(e));
   const void* wb;
@@ -959,6 +969,37 @@ static bool blob_active(const kmm_ctx* c, int it) {
   return c->scsrc && it >= c->cfg.src_start && it <= c->cfg.src_stop;
 }
 
+// inertial (Maxey-Riley) particles are tracked (particle_flag > 0)
+static bool inertial(const kmm_ctx* c) { return c->cfg.particle_flag > 0 && c->npart > 0; }
+
+// Laplacians of u, v, w of this evaluation on the 3/2 grid into lap[lap_cur]
+static int mr_laplacians(kmm_ctx* c) {
+  kmm::MrLapArgs la;
+  std::memset(&la, 0, sizeof(la));
+  la.P = (int)c->P; la.nz = c->nz; la.nzh = c->nzh; la.nxh = c->nxh; la.nyp = c->nyp; la.mx = c->mx; la.mz = c->mz;
+  la.mixed = c->cfg.precision == 1;
+  la.alpha = c->alpha; la.beta = c->beta; la.dyde = c->dyde;
+  la.twM = c->twM; la.twZ = c->twZ; la.twX = c->twX;
+  la.V = c->V; la.OMY = c->OMY; la.u0 = c->u0; la.w0 = c->w0;
+  const size_t tb = (size_t)c->nyp * c->nxh * c->nz * c->vsz;
+  for (int f = 0; f < 3; ++f) la.LT[f] = static_cast<char*>(c->lapT) + f * tb;
+  la.LZ = c->lapZ;
+  la.out = c->lap[c->lap_cur];
+  hipError_t e = kmm::launch_mr_lap_pencil(c->ny, la, c->stream);
+  if (e == hipSuccess) e = kmm::launch_mr_lap_plane(la, c->stream);
+  if (e != hipSuccess) return fail(KMM_E_HIP, std::string("particle Laplacians: ") + hipGetErrorString(e));
+  return KMM_OK;
+}
+
+// u, v, w and the Laplacians of this evaluation become the old ones of the
+// next (solvers.f90:3468-3474)
+static int mr_keep_old(kmm_ctx* c) {
+  const size_t P = (size_t)c->nyp * c->mz * c->mx;
+  KMM_HIP(hipMemcpyAsync(c->uold, c->cap, 3 * P * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  c->lap_cur = 1 - c->lap_cur;
+  return KMM_OK;
+}
+
 static int run_plane(kmm_ctx* c, int it, unsigned long long* cfl, hipEvent_t* ev = nullptr, bool track = false,
                      bool poly = false, bool closure = false) {
   PlaneArgs pa = plane_args(c, it, cfl);
@@ -984,10 +1025,11 @@ static int run_plane(kmm_ctx* c, int it, unsigned long long* cfl, hipEvent_t* ev
     int r = run_poly_plane(c, it);
     if (r) return r;
   }
+  const bool mr = trk && inertial(c);
   if (closure) {
-    int r = run_closure_plane(c, it, trk && c->part_swirl);
+    int r = run_closure_plane(c, it, trk && c->part_swirl, mr ? c->mrf : nullptr);
     if (r) return r;
-  } else if (trk && c->part_swirl) {
+  } else if (trk && (c->part_swirl || mr)) {
     // swirl_3d (solvers.f90:3147-3150 via calcswirl) for swirl_part
     kmm::PolyPlaneArgs sa;
     std::memset(&sa, 0, sizeof(sa));
@@ -1000,12 +1042,40 @@ static int run_plane(kmm_ctx* c, int it, unsigned long long* cfl, hipEvent_t* ev
     sa.in[1] = c->T[2];
     sa.gz = poly ? 1 : 0;   // the polymer stage left dz u, dz v, dz w in ZP rows 0..2
     sa.ZP = c->ZP;
-    sa.swirl = c->cap + (size_t)3 * c->nyp * c->mz * c->mx;
+    sa.swirl = c->part_swirl ? c->cap + (size_t)3 * c->nyp * c->mz * c->mx : nullptr;
+    sa.mrf = mr ? c->mrf : nullptr;  // vorticity and gradient for inertial particles
     if ((e = kmm::launch_swirl_plane(sa, !poly, c->stream)) != hipSuccess)
       return fail(KMM_E_HIP, std::string("swirl: ") + hipGetErrorString(e));
   }
   if (ev) hipEventRecord(ev[4], c->stream);
-  if (trk) {
+  if (mr && c->part_state == 2) {
+    // inertial particles: Laplacians of this evaluation, then RK4 sub-steps
+    // (part_track.f90:131-150)
+    const size_t P = (size_t)c->nyp * c->mz * c->mx;
+    int r = mr_laplacians(c);
+    if (r) return r;
+    kmm::MrArgs m;
+    std::memset(&m, 0, sizeof(m));
+    m.t = tracer_args(c);
+    m.t.n = c->npart;
+    m.t.fstride = (long long)P;
+    m.t.sy = (long long)c->mz * c->mx; m.t.sz = c->mx; m.t.sx = 1;
+    m.t.part = c->part;
+    m.cap = c->cap;
+    m.mrf = c->mrf;
+    m.lap = c->lap[c->lap_cur];
+    m.lold = c->lap[1 - c->lap_cur];
+    m.uold = c->uold;
+    m.substep = c->substep;
+    m.cd_switch = c->cfg.cd_switch;
+    m.swirl = c->part_swirl ? 1 : 0;
+    m.re = c->cfg.re; m.ratio = c->cfg.ratio; m.c_mu = c->cfg.c_mu; m.gravity = c->cfg.gravity;
+    m.dtfull = c->cfg.dt;
+    if ((e = kmm::launch_inertial(m, c->stream)) != hipSuccess)
+      return fail(KMM_E_HIP, std::string("inertial particles: ") + hipGetErrorString(e));
+    if ((r = mr_keep_old(c))) return r;
+    c->cap_valid = true;
+  } else if (trk) {
     kmm::TracerArgs t = tracer_args(c);
     t.n = c->npart; t.nf = c->part_swirl ? 4 : 3;
     t.fld = c->cap;
@@ -1015,6 +1085,11 @@ static int run_plane(kmm_ctx* c, int it, unsigned long long* cfl, hipEvent_t* ev
     t.part = c->part;
     if ((e = kmm::launch_tracer(t, c->stream)) != hipSuccess)
       return fail(KMM_E_HIP, std::string("tracers: ") + hipGetErrorString(e));
+    if (mr) {  // init_part of inertial particles: keep this evaluation's fields as the old ones
+      int r = mr_laplacians(c);
+      if (r) return r;
+      if ((r = mr_keep_old(c))) return r;
+    }
     c->part_state = 2;
     c->cap_valid = true;
   }
@@ -1027,7 +1102,8 @@ static int check_particles(kmm_ctx* c) {
   int h = 0;
   KMM_HIP(hipMemcpyAsync(&h, c->perr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   KMM_HIP(hipStreamSynchronize(c->stream));
-  if (h) return fail(KMM_E_PARTICLE, "a particle left the grid (cell index out of range or not finite)");
+  if (h & 1) return fail(KMM_E_PARTICLE, "a particle left the grid (cell index out of range or not finite)");
+  if (h & 2) return fail(KMM_E_CFL, "particle cfl failure: max |u_f - u_p| dt / a >= 10 (part_track.f90:216-221)");
   return KMM_OK;
 }
 
@@ -1226,6 +1302,7 @@ static bool steady(const kmm_ctx* c, int k) {
   if (c->audit_w >= 0) return false;
   if (c->poly_capture) return false;                     // capture bookkeeping runs per launch
   if (c->npart > 0 && c->part_state != 2) return false;
+  if (inertial(c)) return false;  // per-step copies of the old fields and buffer swaps
   if (c->nsrc > 0 && c->it < c->cfg.perturbtime + 1) return false;
   if (c->scsrc) {  // the source window must not open or close inside the graph
     const int a = c->it + 1, b = c->it + k, s0 = c->cfg.src_start, s1 = c->cfg.src_stop;
@@ -1728,6 +1805,24 @@ static int part_free(kmm_ctx* c) {
   c->part = nullptr;
   c->cap = nullptr;
   c->part_cap = 0;
+  void** mr[6] = {(void**)&c->mrf, (void**)&c->lap[0], (void**)&c->lap[1], (void**)&c->uold, &c->lapT, &c->lapZ};
+  for (void** q : mr) {
+    if (*q) hipFree(*q);
+    *q = nullptr;
+  }
+  return KMM_OK;
+}
+
+// buffers of the inertial-particle fields (allocated once per context)
+static int mr_buffers(kmm_ctx* c) {
+  if (c->mrf) return KMM_OK;
+  const size_t P = (size_t)c->nyp * c->mz * c->mx;
+  KMM_HIP(hipMalloc((void**)&c->mrf, 12 * P * sizeof(double)));
+  for (int b = 0; b < 2; ++b) KMM_HIP(hipMalloc((void**)&c->lap[b], 3 * P * sizeof(double)));
+  KMM_HIP(hipMalloc((void**)&c->uold, 3 * P * sizeof(double)));
+  KMM_HIP(hipMalloc(&c->lapT, 3 * (size_t)c->nyp * c->nxh * c->nz * c->vsz));
+  KMM_HIP(hipMalloc(&c->lapZ, 3 * (size_t)c->nyp * c->mz * c->nxh * c->vsz));
+  c->lap_cur = 0;
   return KMM_OK;
 }
 
@@ -1742,9 +1837,11 @@ extern "C" int kmm_set_particles(kmm_ctx* c, int n, const double* x, const doubl
     c->cap_valid = false;
     return part_free(c);
   }
-  if (c->cfg.particle_flag != 0) return fail(KMM_E_CONFIG, "only tracer particles (particle_flag 0) are supported");
+  if (c->cfg.particle_flag < 0) return fail(KMM_E_CONFIG, "particle_flag must be 0 (tracers) or > 0 (inertial)");
   // substep_check, part_track.f90:335-350
   const kmm_config& g = c->cfg;
+  if (g.particle_flag > 0 && !(g.ap > 0.0))
+    return fail(KMM_E_CONFIG, "inertial particles need a positive radius a");
   double tau = 1.0;
   if (g.cd_switch != 0) {
     const double pi_f = 1.0 / (g.ratio + 0.5);
@@ -1760,7 +1857,11 @@ extern "C" int kmm_set_particles(kmm_ctx* c, int n, const double* x, const doubl
   }
   if (!c->cap) KMM_HIP(hipMalloc((void**)&c->cap, (size_t)4 * c->nyp * c->mz * c->mx * sizeof(double)));
   c->part_swirl = g.particle_swirl != 0;
-  if (c->part_swirl && !c->ZP) {  // dz v, dz w z lines (the polymer stage allocates ZP otherwise)
+  if (g.particle_flag > 0) {
+    int r = mr_buffers(c);
+    if (r) return r;
+  }
+  if ((c->part_swirl || g.particle_flag > 0) && !c->ZP) {  // dz v, dz w z lines (the polymer stage allocates ZP otherwise)
     const size_t zb = (size_t)c->nyp * c->mz * c->nxh * c->vsz;
     KMM_HIP(hipMalloc(&c->ZP, 2 * zb));
     c->allocs.push_back(c->ZP);

@@ -130,6 +130,7 @@ struct PolyPlaneArgs {
   float* capg;            // nullptr or [9][P] physical du_i/dx_k, P = nyp mz mx, [y][z][x]
   float* caps;            // nullptr or [6][P] physical Peterlin stress (11 12 13 22 23 33)
   int gz;                 // ZP row of dz v (dz w follows) for the gradient/swirl plane
+  double* mrf;            // nullptr or [12][P] FP64: wx, wy, wz, then du_i/dx_k at 3 + 3 i + k
 };
 hipError_t launch_poly_plane(const PolyPlaneArgs& a, hipStream_t s);
 // velocity gradient (capg) and/or swirl strength on the 3/2 grid; with zstage
@@ -181,6 +182,43 @@ struct TracerArgs {
   int* err;                 // set to 1 when a particle leaves the grid
 };
 hipError_t launch_tracer(const TracerArgs& a, hipStream_t s);
+// Laplacians of u, v, w (gradv's Lu, Lv, Lw, derivatives.f90:77-79) on the 3/2
+// grid for inertial particles: the pencil stage forms u, w from v, omega_y
+// (veloc) and D^2 f - k^2 f into LT (physical y, transfer precision); the
+// plane stage transforms them in z (LZ) and x into out[3][P] (FP64, [y][z][x])
+struct MrLapArgs {
+  int P, nz, nzh, nxh, nyp, mx, mz, mixed;
+  double alpha, beta, dyde;
+  const void* twM;
+  const void* twZ;
+  const void* twX;
+  const double2* V;         // v, omega_y state (Chebyshev, [y][kx][kz])
+  const double2* OMY;
+  const double* u0;
+  const double* w0;
+  void* LT[3];
+  void* LZ;                 // [3][nyp][mz][nxh]
+  double* out;
+};
+hipError_t launch_mr_lap_pencil(int ny, const MrLapArgs& a, hipStream_t s);
+hipError_t launch_mr_lap_plane(const MrLapArgs& a, hipStream_t s);
+// Inertial (Maxey-Riley) particles, RK4 over substep sub-steps of t.dt
+// (part_track.f90:131-150, 617-1075): one thread per particle; fields on the
+// 3/2 grid [y][z][x]; err bit 1: a particle left the grid, bit 2: particle
+// CFL (max |u_f - u_p| dt / a >= 10, part_track.f90:211-221)
+struct MrArgs {
+  TracerArgs t;             // geometry, part [7][n], err, ycoord, dt / substep, ap
+  const double* cap;        // u, v, w, swirl [4][P]
+  const double* mrf;        // wx, wy, wz, du_i/dx_k [12][P]
+  const double* lap;        // Laplacians [3][P]
+  const double* uold;       // u, v, w of the previous evaluation [3][P]; replaced by the
+                            // rates (u - u_old)/dt (dt = t.dt) before the integration
+  const double* lold;       // Laplacians of the previous evaluation [3][P]; replaced by
+                            // their rates likewise
+  int substep, cd_switch, swirl;
+  double re, ratio, c_mu, gravity, dtfull;
+};
+hipError_t launch_inertial(const MrArgs& a, hipStream_t s);
 // scalar source of Gaussian blobs at the particles (scl_flag 2), summed into
 // src[nyp][mz][mx] (cleared first)
 struct BlobArgs {

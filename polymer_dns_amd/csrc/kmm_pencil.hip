@@ -415,6 +415,51 @@ __global__ __launch_bounds__(PW * 64) void k_gradv(int P, int nz, int nzh, doubl
   }
 }
 
+// Laplacians of u, v, w for inertial particles (gradv's D^2 f - k^2 f,
+// derivatives.f90:77-79, normed as in main_dns.f90:556-558) from the state v,
+// omega_y, u0, w0 through veloc (u = -(-i kx Dv + i kz omy)/k2,
+// w = -(-i kz Dv - i kx omy)/k2; u0, 0, w0 at kx = kz = 0), inverse Chebyshev
+// transformed to physical y; one wave per pencil
+template <int NY, typename V>
+__global__ __launch_bounds__(PW * 64) void k_mr_lap(MrLapArgs A) {
+  extern __shared__ __align__(16) unsigned char smraw[];
+  V* smem = reinterpret_cast<V*>(smraw);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int p = blockIdx.x * PW + wv;
+  const V* twl = lds_twiddles<NY, V>(smem, static_cast<const V*>(A.twM));
+  if (p >= A.P) return;
+  V* buf = wave_lds<NY, V>(smem, wv);
+  V* scr = buf + FFTBuf<2 * NY>::len;
+  const size_t P = (size_t)A.P;
+  double kx, kz;
+  int kxi, kzi;
+  pencil_k(A, p, kx, kz, kxi, kzi);
+  const double k2 = kx * kx + kz * kz;
+  const double dy = A.dyde;
+  const bool is00 = (kxi == 0 && kzi == 0);
+  PF f[3] = {pf_zero(), pf_zero(), pf_zero()};
+  if (kzi != A.nzh) {  // norm zeroes the z-Nyquist row
+    if (is00) {
+      f[0] = pf_load_real<NY>(A.u0, lane);
+      f[2] = pf_load_real<NY>(A.w0, lane);
+    } else {
+      const PF v = pf_load<NY>(A.V, P, p, lane);
+      const PF omy = pf_load<NY>(A.OMY, P, p, lane);
+      const PF dv = cderiv<NY>(v, dy, lane);
+      const double ik2 = 1.0 / k2;
+      f[0] = pf_scale(pf_sub(pf_imul(dv, kx), pf_imul(omy, kz)), ik2);
+      f[1] = v;
+      f[2] = pf_scale(pf_add(pf_imul(dv, kz), pf_imul(omy, kx)), ik2);
+    }
+  }
+#pragma unroll 1
+  for (int i = 0; i < 3; ++i) {
+    PF l = pf_axpy(-k2, f[i], cderiv<NY>(cderiv<NY>(f[i], dy, lane), dy, lane));
+    if (is00) l = pf_real(l);
+    cheb_to_phys_store<NY, V>(l, buf, scr, twl, lane, static_cast<V*>(A.LT[i]), P, p);
+  }
+}
+
 // inverse: FP64 spectral in -> V physical-y out; forward: V in -> FP64 out
 template <int NY, typename V>
 __global__ __launch_bounds__(PW * 64) void k_cheb(int P, const V* __restrict__ tw, const void* in, void* out,
@@ -512,6 +557,16 @@ hipError_t launch_gradv(int ny, int P, int nz, double alpha, double beta, double
                         const double2* v, const double2* w, double2* const* out, hipStream_t s) {
   KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_gradv<NY>), dim3(nblk(P)), dim3(PW * 64), 0, s, P, nz, nz / 2, alpha,
                                          beta, dyde, u, v, w, out));
+  return hipGetLastError();
+}
+hipError_t launch_mr_lap_pencil(int ny, const MrLapArgs& a, hipStream_t s) {
+  if (a.mixed) {
+    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_mr_lap<NY, float2>), dim3(nblk(a.P)), dim3(PW * 64),
+                                           (lds_bytes<NY, float2>()), s, a));
+  } else {
+    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_mr_lap<NY, double2>), dim3(nblk(a.P)), dim3(PW * 64),
+                                           (lds_bytes<NY, double2>()), s, a));
+  }
   return hipGetLastError();
 }
 hipError_t launch_cderiv(int ny, int P, double dyde, const double2* in, double2* out, hipStream_t s) {

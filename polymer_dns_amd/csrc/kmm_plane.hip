@@ -701,6 +701,20 @@ __global__ __launch_bounds__(256) void k_xplane_swirl(PolyPlaneArgs A) {
         if (x < MX) A.capg[(size_t)k * P + prow + x] = (float)g[k][m];
       }
   }
+  if (A.mrf) {  // FP64 vorticity and gradient for inertial particles
+    const size_t P = (size_t)A.nyp * A.mz * MX;
+#pragma unroll
+    for (int m = 0; m < XPL; ++m) {
+      const int x = lane + 64 * m;
+      if (x < MX) {
+        A.mrf[prow + x] = (double)ox[m];
+        A.mrf[P + prow + x] = (double)oy[m];
+        A.mrf[2 * P + prow + x] = (double)oz[m];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) A.mrf[(size_t)(3 + k) * P + prow + x] = (double)g[k][m];
+      }
+    }
+  }
   if (A.swirl) {
     double* out = A.swirl + prow;
 #pragma unroll
@@ -710,6 +724,47 @@ __global__ __launch_bounds__(256) void k_xplane_swirl(PolyPlaneArgs A) {
 #pragma unroll
       for (int k = 0; k < 9; ++k) a[k] = (double)g[k][m];
       if (x < MX) out[x] = swirl_of(a);
+    }
+  }
+}
+
+// x inverse of the three Laplacian z lines of a row (LZ fields 0..2) into
+// FP64 physical rows out[3][P]; the third field is paired with itself
+template <int MX, typename V>
+__global__ __launch_bounds__(256) void k_mr_xinv(MrLapArgs A) {
+  using T = typename Scal<V>::T;
+  constexpr int XPL = (MX + 63) / 64;
+  constexpr int WPB = 4;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  V* sm = reinterpret_cast<V*>(smraw);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  constexpr int BL = FFTBuf<MX>::len;
+  V* tw = sm;
+  V* buf = sm + MX + (size_t)wv * 2 * BL;
+  V* scr = buf + BL;
+  for (int e = threadIdx.x; e < MX; e += WPB * 64) tw[e] = static_cast<const V*>(A.twX)[e];
+  __syncthreads();
+  const int row = blockIdx.x * WPB + wv;  // grid covers nyp * mz rows exactly
+  const int y = row / A.mz, z = row % A.mz;
+  const int nxh = A.nxh;
+  const size_t plane = (size_t)A.nyp * A.mz * nxh;
+  const size_t roff = ((size_t)y * A.mz + z) * nxh;
+  const V* LZ = static_cast<const V*>(A.LZ);
+  T a[XPL], b[XPL], c[XPL], d[XPL];
+  PairIn<MX, V> pin;
+  pair_load<MX, V>(pin, LZ + roff, LZ + plane + roff, nxh, lane);
+  pair_inv<MX, V>(pin, false, false, nxh, A.alpha, buf, scr, tw, lane, a, b);
+  pair_load<MX, V>(pin, LZ + 2 * plane + roff, LZ + 2 * plane + roff, nxh, lane);
+  pair_inv<MX, V>(pin, false, false, nxh, A.alpha, buf, scr, tw, lane, c, d);
+  const size_t P = (size_t)A.nyp * A.mz * MX;
+  const size_t prow = ((size_t)y * A.mz + z) * MX;
+#pragma unroll
+  for (int m = 0; m < XPL; ++m) {
+    const int x = lane + 64 * m;
+    if (x < MX) {
+      A.out[prow + x] = (double)a[m];
+      A.out[P + prow + x] = (double)b[m];
+      A.out[2 * P + prow + x] = (double)c[m];
     }
   }
 }
@@ -782,6 +837,22 @@ static hipError_t swirl_plane_impl(const PolyPlaneArgs& pa, bool zstage, hipStre
   }
   const size_t xlds = (size_t)(MX + 4 * 2 * FFTBuf<MX>::len) * sizeof(V);
   hipLaunchKernelGGL((k_xplane_swirl<MX, V>), dim3(pa.nyp * pa.mz / 4), dim3(256), xlds, s, pa);
+  return hipGetLastError();
+}
+
+template <int MX, int MZ, typename V>
+static hipError_t mr_lap_plane_impl(const MrLapArgs& la, hipStream_t s) {
+  PlaneArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.nxh = la.nxh; a.nz = la.nz; a.nzh = la.nzh; a.nyp = la.nyp; a.mx = la.mx; a.mz = la.mz;
+  a.twZ = la.twZ; a.twX = la.twX; a.beta = la.beta;
+  for (int f = 0; f < 3; ++f) a.in[f] = la.LT[f];
+  a.dzmask = 0u;
+  a.Z = la.LZ;
+  const size_t zlds = (size_t)(16 * FFTBuf<MZ>::len + 9 * MZ) * sizeof(V);
+  hipLaunchKernelGGL((k_zinv<MZ, 3, V>), dim3(a.nyp * (a.nxh / 8)), dim3(512), zlds, s, a);
+  const size_t xlds = (size_t)(MX + 4 * 2 * FFTBuf<MX>::len) * sizeof(V);
+  hipLaunchKernelGGL((k_mr_xinv<MX, V>), dim3(la.nyp * la.mz / 4), dim3(256), xlds, s, la);
   return hipGetLastError();
 }
 
@@ -902,6 +973,23 @@ hipError_t launch_swirl_plane(const PolyPlaneArgs& a, bool zstage, hipStream_t s
   KMM_SWIRL_CASE(288, 288)
   KMM_SWIRL_CASE(384, 192)
 #undef KMM_SWIRL_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_mr_lap_plane(const MrLapArgs& a, hipStream_t s) {
+  if (a.nxh % 8 != 0 || (a.nyp * a.mz) % 4 != 0) return hipErrorInvalidValue;
+#define KMM_MRL_CASE(X, Zz)                                                                            \
+  if (a.mx == X && a.mz == Zz)                                                                         \
+    return a.mixed ? mr_lap_plane_impl<X, Zz, float2>(a, s) : mr_lap_plane_impl<X, Zz, double2>(a, s);
+  KMM_MRL_CASE(48, 48)
+  KMM_MRL_CASE(96, 48)
+  KMM_MRL_CASE(96, 96)
+  KMM_MRL_CASE(144, 144)
+  KMM_MRL_CASE(192, 96)
+  KMM_MRL_CASE(192, 192)
+  KMM_MRL_CASE(288, 288)
+  KMM_MRL_CASE(384, 192)
+#undef KMM_MRL_CASE
   return hipErrorInvalidValue;
 }
 

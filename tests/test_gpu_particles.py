@@ -204,3 +204,64 @@ def test_blob_scalar_source_matches_oracle(precision):
         print(f"  step {step + 1}:", {k: f"{e:.2e}" for k, e in errs.items()}, flush=True)
         assert max(errs.values()) < tol, (step, errs)
     ch.close()
+
+
+@pytest.mark.parametrize("case", [
+    dict(precision=0, cd_switch=2, ap=0.01, ratio=2.0, gravity=0.5, c_mu=1.0),    # Schiller-Naumann, 1 sub-step
+    dict(precision=0, cd_switch=1, ap=0.003, ratio=0.5, gravity=-0.2, c_mu=1.2),  # Stokes, 2 sub-steps
+    dict(precision=1, cd_switch=2, ap=0.01, ratio=2.0, gravity=0.5, c_mu=1.0),
+])
+def test_inertial_particles_follow_the_channel_step(case):
+    """Maxey-Riley particles (particle_flag 1, RK4 with sub-steps) carried by
+    the channel step against the oracle's restatement of part_track/RK4/
+    RKstage/SubDerivative on the same flow."""
+    case = dict(case)
+    precision = case.pop("precision")
+    p = params(32, 32, 32, dt=0.01)
+    print("case: inertial particles, 32^3,", case, "precision", precision, flush=True)
+    st = seeded(p, 31, amp=0.2)
+    rng = np.random.default_rng(32)
+    n = 400
+    x, y, z = random_points(p, n, rng, wall=0.95)
+    ch = channel(p, precision=precision, particle_flag=1, **case)
+    for k in ("v", "omy", "u0", "w0"):
+        ch.put(k, st[k])
+    ch.set_particles(x, y, z)
+    ch.startup()
+    sol = O.Solver(p)
+    sol.swirl_fn = O.swirl_eig
+    sol.set_particles(x, y, z, particle_flag=1, **case)
+    ost = {k: v.copy() for k, v in st.items()}
+    sol.startup(ost)
+    tol = 1e-10 if precision == 0 else 1e-4
+    print("sub-steps:", sol.substep, flush=True)
+    for step in range(5):
+        ch.step(1)
+        sol.step(ost)
+        got = ch.particles()
+        e = {k: float(np.abs(got[k] - sol.particles[k]).max()) for k in "xyz"}
+        e.update({k: rel(got[k], sol.particles[k]) for k in "uvw"})
+        print(f"step {step + 1}:", {k: f"{v:.1e}" for k, v in e.items()}, flush=True)
+        assert max(e.values()) < tol, (step, e)
+    # the particles are inertial: their velocity differs from the fluid's
+    uf = O.fluid_interp(got["x"], got["y"], got["z"], [sol.last_phys["up"]], p)[0]
+    assert np.abs(got["u"] - uf).max() > 1e-6
+    ch.close()
+
+
+def test_inertial_particle_cfl_failure_is_reported():
+    """A heavy particle (density ratio 1e6) under strong buoyancy leaves the
+    fluid velocity by more than 10 a/dt in one step: the step returns the
+    particle-CFL error (part_track.f90:216-221)."""
+    p = params(32, 32, 32, dt=0.01)
+    print("case: inertial particle CFL, 32^3", flush=True)
+    st = seeded(p, 33, amp=0.1)
+    ch = channel(p, particle_flag=1, cd_switch=2, ap=1e-4, ratio=1e6, gravity=100.0)
+    for k in ("v", "omy", "u0", "w0"):
+        ch.put(k, st[k])
+    ch.set_particles([1.0, 2.0], [0.0, 0.2], [1.0, 3.0])
+    ch.startup()
+    with pytest.raises(kmm.KmmError) as ei:
+        ch.step(1)
+    assert ei.value.code == kmm.KMM_E_CFL and "particle cfl" in str(ei.value)
+    ch.close()

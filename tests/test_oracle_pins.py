@@ -758,3 +758,90 @@ def test_cardano_eigenvalues_match_eigvals():
     rot = [[np.array([0.0]), np.array([-w]), np.array([0.0])], [np.array([w]), np.array([0.0]), np.array([0.0])],
            [np.array([0.0]), np.array([0.0]), np.array([0.0])]]
     assert abs(O.calcswirl(rot)[0] - math.sqrt(2.0) * w) < 1e-12
+
+
+# ------------------------------------------------------ inertial particles
+def _mr_uniform(p, u=(0.0, 0.0, 0.0), om=(0.0, 0.0, 0.0), lap=(0.0, 0.0, 0.0), uold=None, lold=None):
+    shape = (p.mx, p.mz, p.nyp)
+    c = lambda v: np.full(shape, float(v))
+    fl = dict(u=c(u[0]), v=c(u[1]), w=c(u[2]), ox=c(om[0]), oy=c(om[1]), oz=c(om[2]),
+              g=[c(0.0) for _ in range(9)], L=[c(a) for a in lap])
+    uo = u if uold is None else uold
+    lo = lap if lold is None else lold
+    old = dict(u=c(uo[0]), v=c(uo[1]), w=c(uo[2]), L=[c(a) for a in lo])
+    return fl, old
+
+
+def _mr_part(x, y, z, u, v, w):
+    a = lambda q: np.array([float(q)])
+    return dict(x=a(x), y=a(y), z=a(z), u=a(u), v=a(v), w=a(w))
+
+
+def test_mr_stokes_drag_relaxation_and_settling_closed_forms():
+    """Quiescent fluid, Stokes drag (CD_switch 1): the velocity relaxes as
+    exp(-t/tau) with 1/tau = Pi_f C_D 3/(8 a), the displacement is
+    u0 tau (1 - exp(-t/tau)), and with gravity the particle drifts to
+    v_t = -g (Pi_f - Pi_p) tau; RK4's error is O((dt/tau)^4)."""
+    p = params(8, 8, 8, re=100.0)
+    ap, ratio, grav = 0.05, 2.0, 0.3
+    mp = dict(ap=ap, ratio=ratio, c_mu=1.0, cd_switch=1, gravity=grav)
+    pi_f, pi_p = 1.0 / (ratio + 0.5), ratio / (ratio + 0.5)
+    c_d = 16.0 / (2.0 * ap * p.re)
+    tau = 1.0 / (pi_f * c_d * 3.0 / (8.0 * ap))
+    dt, n = tau / 20.0, 40
+    fl, old = _mr_uniform(p)
+    part = _mr_part(1.0, 0.1, 0.5, 0.4, 0.0, -0.2)
+    for _ in range(n):
+        part, _ = O.mr_rk4(part, fl, old, p, mp, dt)
+    t = n * dt
+    e = math.exp(-t / tau)
+    vt = -grav * (pi_f - pi_p) * tau
+    assert abs(part["u"][0] - 0.4 * e) < 1e-6 * 0.4
+    assert abs(part["w"][0] + 0.2 * e) < 1e-6 * 0.2
+    assert abs(part["v"][0] - vt * (1.0 - e)) < 1e-6 * abs(vt)
+    assert abs(part["x"][0] - (1.0 + 0.4 * tau * (1.0 - e))) < 1e-7
+    assert abs(part["y"][0] - (0.1 + vt * (t - tau * (1.0 - e)))) < 1e-7
+
+
+def test_mr_faxen_shift_lift_direction_and_fluid_acceleration():
+    """Drag pulls toward u_f + a^2/6 Lap u; lift is C_L Pi_f (u_f - u_p) x omega;
+    a neutrally buoyant particle moving with a uniformly accelerating fluid
+    keeps its acceleration (added mass + pressure gradient, Pi_f 3/2 = 1)."""
+    p = params(8, 8, 8, re=100.0)
+    ap = 0.05
+    mp = dict(ap=ap, ratio=2.0, c_mu=1.0, cd_switch=1, gravity=0.0)
+    pi_f = 1.0 / 2.5
+    # Faxen: quiescent fluid, constant Laplacian -> steady particle velocity a^2/6 Lap u
+    lap = (3.0, -1.0, 2.0)
+    fl, old = _mr_uniform(p, lap=lap)
+    ku, kv, kw = O.mr_forces(np.array([1.0]), np.array([0.1]), np.array([0.5]),
+                             np.array([ap * ap / 6.0 * lap[0]]), np.array([ap * ap / 6.0 * lap[1]]),
+                             np.array([ap * ap / 6.0 * lap[2]]), fl, old, p, mp, 0.01)
+    assert max(abs(ku[0]), abs(kv[0]), abs(kw[0])) < 1e-15
+    # lift: no drag (CD_switch 0), uniform flow and vorticity
+    rng = np.random.default_rng(3)
+    uf, om, upart = rng.standard_normal(3), rng.standard_normal(3), rng.standard_normal(3)
+    fl, old = _mr_uniform(p, u=uf, om=om)
+    k = O.mr_forces(np.array([1.0]), np.array([0.1]), np.array([0.5]), *[np.array([q]) for q in upart],
+                    fl, old, p, dict(mp, cd_switch=0), 0.01)
+    assert np.allclose([q[0] for q in k], 0.5 * pi_f * np.cross(uf - upart, om), rtol=1e-13, atol=1e-15)
+    # neutrally buoyant (ratio 1) co-moving particle in u = U + a t
+    acc, U, dt = np.array([0.7, -0.3, 0.2]), np.array([1.0, 0.2, -0.4]), 0.01
+    fl, old = _mr_uniform(p, u=U + acc * dt, uold=U)
+    k = O.mr_forces(np.array([1.0]), np.array([0.1]), np.array([0.5]), *[np.array([q]) for q in U + acc * dt],
+                    fl, old, p, dict(mp, ratio=1.0), dt)
+    assert np.allclose([q[0] for q in k], acc, rtol=1e-12)
+
+
+def test_mr_wall_bounce_and_periodic_wrap():
+    """A particle driven into the top wall is clamped at y_max - a with v
+    reversed; x and z wrap periodically."""
+    p = params(8, 8, 8, re=100.0)
+    ap = 0.05
+    mp = dict(ap=ap, ratio=2.0, c_mu=1.0, cd_switch=0, gravity=0.0)
+    fl, old = _mr_uniform(p)
+    part = _mr_part(p.xl - 0.01, p.ycoord[0] - ap - 0.001, 0.005, 2.0, 1.0, -1.0)
+    new, _ = O.mr_rk4(part, fl, old, p, mp, 0.01)
+    assert new["y"][0] == p.ycoord[0] - ap and new["v"][0] == -1.0
+    assert abs(new["x"][0] - 0.01) < 1e-12 and abs(new["z"][0] - (p.zl - 0.005)) < 1e-12
+    assert new["u"][0] == 2.0 and new["w"][0] == -1.0
