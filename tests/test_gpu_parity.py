@@ -152,6 +152,22 @@ def test_ib_rib_steps():
     ch.close()
 
 
+def test_ib_generated_brick_steps():
+    """The 2-D brick of the geometry generator (Geom.f90 geomtype 2, type-4
+    cells forced while it <= perturbtime) through the channel step."""
+    from polymer_dns_amd.geometry import make_geometry
+    p = params(64, 32, 32, gain=20.0, ugain=100.0, dt=0.002, perturbtime=3)
+    im = make_geometry(p.nx, p.ny, p.nz, p.yl, p.zl, p.xl, geomtype=2)
+    assert (im == 4).any() and set(np.unique(im)) <= {0, 4}
+    log("case: generated 2-D brick, 64x32x32")
+    sol, ost, ch = run_pair(p, 0, imatrix=im)
+    for it in range(5):
+        sol.step(ost)
+        ch.step(1)
+        compare(ost, ch, FIELDS_CHECK, TOL, f"step{it + 1}")
+    ch.close()
+
+
 MIXED_TOL = 1e-4   # FP32 transforms/products, FP64 state and tau solves
 
 
