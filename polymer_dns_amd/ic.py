@@ -200,3 +200,91 @@ def conformation_state(nx, ny, nz, seed, amp=1e-3, kmax=4, nmax=6, mean=None):
             a[0, 0, 0] += 1.0
         out["c" + ij] = a
     return out
+
+
+# --------------------------------------------------------------------------- #
+#  The reference's new-run initial fields (init_flow.f90) and the conversion of
+#  physical fields to the spectral state (main_dns.f90:1923-2005)
+# --------------------------------------------------------------------------- #
+def solver_ycoord(ny, yl=2.0):
+    """main_dns.f90:1112-1116: ycoord(1) = yl/2, ycoord(k) = cos((k-1) pi/ny) yl/2."""
+    y = np.cos(np.arange(ny + 1) * math.pi / ny) * (yl / 2.0)
+    y[0] = yl / 2.0
+    return y
+
+
+def init_flow(flow_select, nx, ny, nz, yl, re, R_tau, uinf=1.0, kwall=0):
+    """Physical initial velocity of init_flow.f90:23-40 on the unpadded grid,
+    arrays (nx, nz, nyp) C-ordered, y index 0 at ycoord(1) = yl/2:
+    still (flow_select <= 0 or 5), channel (1, init_flow.f90:82-139: the
+    laminar profile re dPdx (d^2/2 - yl d/2), d = |y - y_min|, zero in the
+    kwall rows next to each wall), Couette_2D (2, :147-186: uinf d/yl).
+    Couette_3D (3) and vortex_Couette (12) divide by ycoord(nyp) -
+    ycoord(nyp) = 0 (init_flow.f90:226-228, 818) and are refused; the
+    Blasius and streamwise-vortex starts (4, 10, 11) are not built."""
+    nyp = ny + 1
+    y = solver_ycoord(ny, yl)
+    u = np.zeros((nx, nz, nyp))
+    v = np.zeros_like(u)
+    w = np.zeros_like(u)
+    if flow_select <= 0 or flow_select == 5:
+        return u, v, w
+    if flow_select == 1:
+        dpdx = -2.0 / yl * (2.0 * R_tau / (yl * re)) ** 2
+        d = np.abs(y - y[-1])
+        prof = re * dpdx * (0.5 * d ** 2 - yl / 2.0 * d)
+        i1 = np.arange(1, nyp + 1)
+        prof[(i1 <= kwall) | ((nyp - i1) <= kwall)] = 0.0
+        u[:] = prof
+        return u, v, w
+    if flow_select == 2:
+        u[:] = uinf * (np.abs(y - y[-1]) / abs(y[0] - y[-1]))
+        return u, v, w
+    if flow_select in (3, 12):
+        raise ValueError(f"flow_select {flow_select}: the reference's initial u divides by "
+                         "ycoord(nyp) - ycoord(nyp) = 0 (init_flow.f90:226-228, 818)")
+    raise NotImplementedError(f"flow_select {flow_select}: initial field not built (Blasius / streamwise vortices)")
+
+
+def new_run_state(u, v, w, xl, zl):
+    """The new-run branch of initial (main_dns.f90:1923-2005) from physical
+    u, v, w (nx, nz, nyp): forward transforms (xyzfft, helpers.f90:6-87:
+    DCT-I along y over ny, ends halved; r2c along x over nx keeping nx/2
+    modes; c2c along z over nz), omega_y = i kz u - i kx w with the (0,0)
+    mode zero, the symmetry projection (norm, helpers.f90:91-158), u0 and w0
+    = the real (0,0) pencils of u and w, v(0,0) = 0.  Returns dict(v, omy,
+    u0, w0) in the state layout ((nx/2, nz, nyp) complex; u0, w0 (nyp,))."""
+    from scipy.fft import dct
+    nx, nz, nyp = np.shape(u)
+    ny, nxh = nyp - 1, nx // 2
+
+    def fwd(b):
+        t = dct(np.asarray(b, dtype=np.float64), type=1, axis=-1) / float(ny)
+        t[..., 0] /= 2.0
+        t[..., -1] /= 2.0
+        t = np.transpose(t, (2, 1, 0))                       # (nyp, nz, nx)
+        xt = np.fft.rfft(t, axis=2)[:, :, :nxh] / float(nx)
+        zt = np.fft.fft(xt, axis=1) / float(nz)
+        return np.ascontiguousarray(np.transpose(zt, (2, 1, 0)))
+
+    def norm(a):
+        a[0, 0, :] = a[0, 0, :].real
+        j = np.arange(1, nz // 2)
+        a[0, j, :] = 0.5 * (a[0, j, :] + np.conj(a[0, nz - j, :]))
+        a[0, nz - j, :] = np.conj(a[0, j, :])
+        a[:, nz // 2, :] = 0.0
+        return a
+
+    us, vs, ws = fwd(u), fwd(v), fwd(w)
+    kx = (np.arange(nxh) * (2.0 * math.pi / xl))[:, None, None]
+    kzi = np.arange(nz)
+    kzi[nz // 2:] -= nz
+    kz = (kzi * (2.0 * math.pi / zl))[None, :, None]
+    omy = 1j * kz * us - 1j * kx * ws
+    omy[0, 0, :] = 0.0
+    for a in (us, vs, ws, omy):
+        norm(a)
+    u0 = us[0, 0, :].real.copy()
+    w0 = ws[0, 0, :].real.copy()
+    vs[0, 0, :] = 0.0
+    return dict(v=vs, omy=omy, u0=u0, w0=w0)
