@@ -331,6 +331,16 @@ int kmm_statistics(kmm_ctx* ctx, double* prof, double* scal);
  * tpss0 bpss0 Ckk0 PSS0), created when first != 0 */
 int kmm_write_outputs(kmm_ctx* ctx, const char* dir, int first);
 
+/* Physical fields of the last evaluation on the 3/2 grid, the arguments of
+ * write_flowfield_ascii (helpers.f90:1671-1767): u, v, w, wx, wy, wz, swirl,
+ * and the scalar when scalar = 1; out: 7 or 8 fields of nyp*mz*mx doubles,
+ * each in the reference layout (nyp, mz, mx). */
+int kmm_flowfield(kmm_ctx* ctx, double* out);
+/* write_flowfield_ascii: dir/time-XXXXXX.dat (it = XXXXXX, solution time
+ * it/iprnfrq) in the reference's Tecplot ASCII layout, and dir/grid.dat when
+ * it = 1 */
+int kmm_write_flowfield(kmm_ctx* ctx, const char* dir, int it, int iprnfrq);
+
 /* Autotuner (no reference counterpart; the role of FFTW's PATIENT planning,
  * solvers.f90:2057-2069): occupancy variants of the pencil step kernel and of
  * the x-plane kernel (0: the compiler's register budget, 1: one more wave per

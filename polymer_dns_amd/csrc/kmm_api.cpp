@@ -2442,6 +2442,170 @@ extern "C" int kmm_statistics(kmm_ctx* c, double* prof, double* scal) {
   return KMM_OK;
 }
 
+// Physical fields of the last evaluation on the 3/2 grid, the arguments of
+// write_flowfield_ascii (helpers.f90:1671-1767): u, v, w, wx, wy, wz, swirl
+// and, with the scalar, the scalar; FP64, reference layout (nyp, mz, mx) per
+// field.  The velocity and vorticity come from the transfer fields through
+// the z and x inverse transforms, the swirl from the gradient/swirl plane on
+// the same z intermediates, the scalar from an inverse Chebyshev transform of
+// the state first.
+extern "C" int kmm_flowfield(kmm_ctx* c, double* out) {
+  if (!c || !out) return fail(KMM_E_ARG, "null argument");
+  if (!c->nl_valid) return fail(KMM_E_ARG, "no evaluation yet (kmm_startup first)");
+  KMM_HIP(hipSetDevice(c->device));
+  const int nyp = c->nyp;
+  const size_t P = (size_t)nyp * c->mz * c->mx;
+  const int nf = c->cfg.scalar ? 8 : 7;
+  const size_t zb = (size_t)nyp * c->mz * c->nxh * c->vsz;
+  const size_t tb = (size_t)nyp * c->P * c->vsz;
+  double* d = nullptr;
+  void *lz = nullptr, *lt = nullptr;
+  hipError_t e = hipMalloc((void**)&d, (size_t)(nf + 2) * P * sizeof(double));  // the scalar pass writes 3 fields
+  if (e == hipSuccess) e = hipMalloc(&lz, 3 * zb);
+  if (e == hipSuccess && c->cfg.scalar) e = hipMalloc(&lt, tb);
+  if (e == hipSuccess && !c->ZP) {
+    e = hipMalloc(&c->ZP, 2 * zb);
+    if (e == hipSuccess) c->allocs.push_back(c->ZP);
+  }
+  kmm::MrLapArgs la;
+  std::memset(&la, 0, sizeof(la));
+  la.P = (int)c->P; la.nz = c->nz; la.nzh = c->nzh; la.nxh = c->nxh; la.nyp = nyp; la.mx = c->mx; la.mz = c->mz;
+  la.mixed = c->cfg.precision == 1;
+  la.alpha = c->alpha; la.beta = c->beta; la.dyde = c->dyde;
+  la.twM = c->twM; la.twZ = c->twZ; la.twX = c->twX;
+  la.LZ = lz;
+  for (int g3 = 0; g3 < 2 && e == hipSuccess; ++g3) {  // u, v, w then wx, wy, wz
+    for (int f = 0; f < 3; ++f) la.LT[f] = c->T[3 * g3 + f];
+    la.out = d + (size_t)3 * g3 * P;
+    e = kmm::launch_mr_lap_plane(la, c->stream);
+  }
+  if (e == hipSuccess) {  // swirl_3d (calcswirl) from the NS plane's z intermediates
+    kmm::PolyPlaneArgs sa;
+    std::memset(&sa, 0, sizeof(sa));
+    sa.nxh = c->nxh; sa.nz = c->nz; sa.nzh = c->nzh; sa.nyp = nyp; sa.mx = c->mx; sa.mz = c->mz;
+    sa.mixed = c->cfg.precision == 1;
+    sa.alpha = c->alpha; sa.beta = c->beta;
+    sa.twZ = c->twZ; sa.twX = c->twX;
+    sa.U = c->Z;
+    sa.in[0] = c->T[1];
+    sa.in[1] = c->T[2];
+    sa.gz = 0;
+    sa.ZP = c->ZP;
+    sa.swirl = d + 6 * P;
+    e = kmm::launch_swirl_plane(sa, true, c->stream);
+  }
+  if (e == hipSuccess && c->cfg.scalar) {
+    e = kmm::launch_cheb(c->ny, c->P, c->twM, c->TH, lt, 1, c->cfg.precision, c->stream);
+    for (int f = 0; f < 3; ++f) la.LT[f] = lt;
+    la.out = d + 7 * P;
+    if (e == hipSuccess) e = kmm::launch_mr_lap_plane(la, c->stream);
+  }
+  std::vector<double> h;
+  if (e == hipSuccess) {
+    h.resize((size_t)nf * P);
+    e = hipMemcpyAsync(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(d);
+  hipFree(lz);
+  if (lt) hipFree(lt);
+  if (e != hipSuccess) return fail(KMM_E_HIP, std::string("flowfield: ") + hipGetErrorString(e));
+  for (int f = 0; f < nf; ++f)
+    for (int k = 0; k < nyp; ++k)
+      for (int jz = 0; jz < c->mz; ++jz)
+        for (int ix = 0; ix < c->mx; ++ix)
+          out[(size_t)f * P + k + (size_t)nyp * (jz + (size_t)c->mz * ix)] =
+              h[(size_t)f * P + ((size_t)k * c->mz + jz) * c->mx + ix];
+  return KMM_OK;
+}
+
+// Fortran Ew.d output (d significant digits as 0.ddd...E+ee, right-justified
+// in w columns; exponents beyond two digits drop the E)
+static std::string fortran_e(double v, int w, int dgt) {
+  char m[64];
+  if (v == 0.0) {
+    std::snprintf(m, sizeof m, "0.%0*dE+00", dgt, 0);
+  } else if (!std::isfinite(v)) {
+    std::snprintf(m, sizeof m, "%s", std::isnan(v) ? "NaN" : (v > 0 ? "Infinity" : "-Infinity"));
+  } else {
+    char t[64];
+    std::snprintf(t, sizeof t, "%.*E", dgt - 1, std::fabs(v));  // d.ddd E+xx
+    const char* ep = std::strchr(t, 'E');
+    const int ex = std::atoi(ep + 1) + 1;
+    std::string dig(1, t[0]);
+    dig.append(t + 2, (size_t)(ep - (t + 2)));
+    const int ax = ex < 0 ? -ex : ex;
+    if (ax < 100) std::snprintf(m, sizeof m, "%s0.%sE%c%02d", v < 0 ? "-" : "", dig.c_str(), ex < 0 ? '-' : '+', ax);
+    else std::snprintf(m, sizeof m, "%s0.%s%c%03d", v < 0 ? "-" : "", dig.c_str(), ex < 0 ? '-' : '+', ax);
+  }
+  std::string r(m);
+  return r.size() >= (size_t)w ? r : std::string((size_t)w - r.size(), ' ') + r;
+}
+
+// Fortran Aw output of a shorter string: right-justified
+static std::string fortran_a(const std::string& s, size_t w) {
+  return s.size() >= w ? s.substr(0, w) : std::string(w - s.size(), ' ') + s;
+}
+
+// write_flowfield_ascii (helpers.f90:1671-1767): dir/time-XXXXXX.dat with the
+// fields of kmm_flowfield, one point a line (8(e14.6,1x)), y outermost, then
+// z (all mz planes when nz > 10, else the first), then x; at it = 1 also
+// dir/grid.dat with x, y, z
+extern "C" int kmm_write_flowfield(kmm_ctx* c, const char* dir, int it, int iprnfrq) {
+  if (!c || !dir || iprnfrq <= 0) return fail(KMM_E_ARG, "bad argument");
+  const int nyp = c->nyp, mx = c->mx, mz = c->mz;
+  const size_t P = (size_t)nyp * mz * mx;
+  const int nf = c->cfg.scalar ? 8 : 7;
+  std::vector<double> f((size_t)nf * P);
+  int r = kmm_flowfield(c, f.data());
+  if (r) return r;
+  std::vector<double> y(nyp);
+  KMM_HIP(hipMemcpy(y.data(), c->ycoord_d, nyp * sizeof(double), hipMemcpyDeviceToHost));
+  const int mzc = c->nz > 10 ? mz : 1;
+  const double delxm = c->cfg.xl / mx, delzm = c->cfg.zl / mz;
+  char name[64];
+  std::snprintf(name, sizeof name, "/time-%06d.dat", it);
+  const std::string path = std::string(dir) + name;
+  FILE* fp = std::fopen(path.c_str(), "w");
+  if (!fp) return fail(KMM_E_IO, "cannot open " + path);
+  const std::string vars = c->cfg.scalar
+      ? "filetype = solution, variables = \"swirl\", \"u\", \"v\", \"w\", \"wx\", \"wy\", \"wz\", \"scl\""
+      : "filetype = solution, variables = \"swirl\", \"u\", \"v\", \"w\", \"wx\", \"wy\", \"wz\"";
+  std::fprintf(fp, "%s\n", fortran_a(vars, 93).c_str());
+  std::fprintf(fp, "%s%3d%s%4d%s%4d%s%4d%s\n", fortran_a("zone f=point t=\"Field\", solutiontime=", 40).c_str(),
+               it / iprnfrq, fortran_a(",i=", 5).c_str(), mx, fortran_a("j=", 5).c_str(), 1,
+               fortran_a("k=", 5).c_str(), nyp, " \n");
+  static const int order[8] = {6, 0, 1, 2, 3, 4, 5, 7};  // swirl, u, v, w, wx, wy, wz, scl
+  std::string line;
+  for (int k = 0; k < nyp; ++k)
+    for (int jz = 0; jz < mzc; ++jz)
+      for (int ix = 0; ix < mx; ++ix) {
+        line.clear();
+        const size_t o = k + (size_t)nyp * (jz + (size_t)mz * ix);
+        for (int q = 0; q < nf; ++q) {
+          line += fortran_e(f[(size_t)order[q] * P + o], 14, 6);
+          line += ' ';
+        }
+        line += '\n';
+        std::fputs(line.c_str(), fp);
+      }
+  std::fclose(fp);
+  if (it == 1) {
+    const std::string gpath = std::string(dir) + "/grid.dat";
+    FILE* gp = std::fopen(gpath.c_str(), "w");
+    if (!gp) return fail(KMM_E_IO, "cannot open " + gpath);
+    std::fprintf(gp, " filetype = grid, variables = \"x\", \"y\", \"z\"\n");
+    std::fprintf(gp, " zone f=point t=\"Grid\", i=%12d j=%12d k=%12d\n", mx, 1, nyp);
+    for (int k = 0; k < nyp; ++k)
+      for (int jz = 0; jz < mzc; ++jz)
+        for (int ix = 0; ix < mx; ++ix)
+          std::fprintf(gp, "%s %s %s \n", fortran_e(ix * delxm, 14, 6).c_str(), fortran_e(y[k], 14, 6).c_str(),
+                       fortran_e(jz * delzm, 14, 6).c_str());
+    std::fclose(gp);
+  }
+  return KMM_OK;
+}
+
 extern "C" int kmm_write_outputs(kmm_ctx* c, const char* dir, int first) {
   if (!c || !dir) return fail(KMM_E_ARG, "null argument");
   const int nyp = c->nyp;

@@ -49,6 +49,7 @@ EXPORTS = [
     "kmm_set_poly_capture", "kmm_poly_samples", "kmm_set_closure", "kmm_get_poly_stress",
     "kmm_set_closure_audit", "kmm_closure_status", "kmm_poly_conf_from_stress",
     "kmm_autotune", "kmm_set_tuning", "kmm_get_tuning", "kmm_statistics", "kmm_write_outputs",
+    "kmm_flowfield", "kmm_write_flowfield",
 ]
 
 
@@ -158,6 +159,8 @@ def load_library(path: str | None = None):
     lib.kmm_get_tuning.argtypes = [C.c_void_p, ip, ip]
     lib.kmm_statistics.argtypes = [C.c_void_p, dp, dp]
     lib.kmm_write_outputs.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+    lib.kmm_flowfield.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    lib.kmm_write_flowfield.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int]
     lib.kmm_calcswirl.argtypes = [C.c_void_p] + [dp] * 10
     lib.kmm_set_dpdx.argtypes = [C.c_void_p, C.c_double]
     for n in EXPORTS:
@@ -455,6 +458,19 @@ class Channel:
 
     def write_outputs(self, directory, first=False):
         _check(self.lib, self.lib.kmm_write_outputs(self.h, os.fsencode(str(directory)), int(bool(first))))
+
+    FLOWFIELD = ("u", "v", "w", "wx", "wy", "wz", "swirl", "scalar")
+
+    def flowfield(self):
+        """Physical fields of the last evaluation on the 3/2 grid (write_flowfield_ascii's
+        arguments), each (mx, mz, nyp)."""
+        nf = 8 if self.cfg.scalar else 7
+        out = np.zeros((nf, self.mx, self.mz, self.nyp))
+        _check(self.lib, self.lib.kmm_flowfield(self.h, _dp(out)))
+        return {k: out[i] for i, k in enumerate(self.FLOWFIELD[:nf])}
+
+    def write_flowfield(self, directory, it, iprnfrq):
+        _check(self.lib, self.lib.kmm_write_flowfield(self.h, os.fsencode(str(directory)), int(it), int(iprnfrq)))
 
     def autotune(self, steps=10):
         """Time the occupancy variants of the pencil and x-plane kernels on a

@@ -125,3 +125,50 @@ def test_mixed_precision_profiles_within_half_percent():
     assert max(e.values()) < 5e-3, e
     a.close()
     b.close()
+
+
+def test_flowfield_fields_and_ascii_file(tmp_path):
+    """write_flowfield_ascii's fields (u, v, w, vorticity, swirl, scalar on the
+    3/2 grid) against the oracle's physical fields of the same evaluation, and
+    the Tecplot ASCII file (6 significant digits) against them."""
+    p = params(32, 32, 32, scalar=True, diff=0.71, scl_flag=6, deltaT=0.3)
+    print("case: flowfield, 32^3 scalar", flush=True)
+    st = seeded(p, 74, amp=0.2)
+    ch = channel(p, st, scalar=1, diff=0.71, scl_flag=6, deltaT=0.3)
+    sol = O.Solver(p)
+    ost = {k: v.copy() for k, v in st.items()}
+    sol.startup(ost)
+    for _ in range(3):
+        sol.step(ost)
+        ch.step(1)
+    got = ch.flowfield()
+    ph = sol.last_phys
+    f = sol.last_fields
+    gv = O.gradv(f["u"], f["v"], f["w"], p)
+    sw = O.swirl_eig([[O.to_physical(gv[f"u{i}{k}"], p) for k in (1, 2, 3)] for i in (1, 2, 3)])
+    ref = dict(u=ph["up"], v=ph["vp"], w=ph["wp"], wx=ph["wxp"], wy=ph["wyp"], wz=ph["wzp"], swirl=sw,
+               scalar=O.to_physical(ost["scalar"], p))
+    for k, r in ref.items():
+        e = float(np.abs(got[k] - r).max() / np.abs(r).max())
+        print(f"  {k}: rel {e:.2e}", flush=True)
+        # swirl: the imaginary part of a complex eigenvalue pair, which near a
+        # vanishing discriminant grows like the square root of the gradient's
+        # rounding difference
+        assert e < (1e-7 if k == "swirl" else 1e-10), k
+    ch.write_flowfield(tmp_path, 1, 1)
+    data = np.loadtxt(tmp_path / "time-000001.dat", skiprows=2)
+    order = ("swirl", "u", "v", "w", "wx", "wy", "wz", "scalar")
+    mx, mz, nyp = p.mx, p.mz, p.nyp
+    assert data.shape == (nyp * mz * mx, 8)
+    for q, k in enumerate(order):
+        want = np.transpose(got[k], (2, 1, 0)).reshape(-1)   # y outermost, then z, then x
+        # E14.6: mantissa 0.dddddd, so the rounding is at most 5e-6 of the value
+        assert np.all(np.abs(data[:, q] - want) <= 5.0001e-6 * np.abs(want) + 1e-300), k
+    grid = np.loadtxt(tmp_path / "grid.dat", skiprows=2)
+    assert grid.shape == (nyp * mz * mx, 3)
+    assert np.allclose(grid[:mx, 0], np.arange(mx) * p.xl / mx, rtol=5.0001e-6, atol=1e-12)
+    with open(tmp_path / "time-000001.dat") as fh:
+        head = [fh.readline() for _ in range(3)]
+    assert head[0].rstrip("\n").endswith('"scl"') and len(head[0].rstrip("\n")) == 93
+    assert "solutiontime=  1  ,i= " in head[1] and head[2].strip() == ""
+    ch.close()
