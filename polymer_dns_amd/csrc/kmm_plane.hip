@@ -30,14 +30,18 @@ __device__ __forceinline__ void zinv_line_load(V (&pre)[NJ], const V* __restrict
   }
 }
 
+// Work items are (field, y, group of 8 kx lines), walked by a grid sized to
+// the blocks the CUs hold at once (launch_zitems): with 9 fields per
+// (y, group) the last round of blocks is a small fraction of the work, where a
+// grid of one block per (y, group) would leave a nearly empty third round.
 template <int MZ, int NFIN, typename V>
 __global__ __launch_bounds__(512) void k_zinv(PlaneArgs A) {
   extern __shared__ __align__(16) unsigned char smraw[];
   V* sm = reinterpret_cast<V*>(smraw);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ngx = A.nxh / 8;
-  const int y = blockIdx.x / ngx, kx0 = (blockIdx.x % ngx) * 8;
-  const int kx = kx0 + wv;
+  const int ngroups = A.nyp * ngx;
+  const int nitems = NFIN * ngroups;
   constexpr int BL = FFTBuf<MZ>::len;
   V* buf = sm + wv * 2 * BL;
   V* scr = buf + BL;
@@ -45,13 +49,22 @@ __global__ __launch_bounds__(512) void k_zinv(PlaneArgs A) {
   V* tw = tile + 8 * MZ;        // [MZ] twiddles
   for (int e = threadIdx.x; e < MZ; e += 512) tw[e] = static_cast<const V*>(A.twZ)[e];
   __syncthreads();
-  // each field's line is loaded into registers one field ahead, so its
-  // latency hides behind the previous field's transform and store
+  int it = blockIdx.x;
+  if (it >= nitems) return;
+  auto line = [&](int i) -> const V* {
+    const int f = i / ngroups, g = i - f * ngroups;
+    const int y = g / ngx, kx = (g - y * ngx) * 8 + wv;
+    return static_cast<const V*>(A.in[f]) + ((size_t)y * A.nxh + kx) * A.nz;
+  };
+  // each item's line is loaded into registers one item ahead, so its
+  // latency hides behind the previous item's transform and store
   constexpr int NJ = (2 * MZ / 3 + 63) / 64;  // values of a line per lane (nz = 2 mz / 3)
   V pre[NJ];
-  zinv_line_load<NJ, V>(pre, static_cast<const V*>(A.in[0]) + ((size_t)y * A.nxh + kx) * A.nz, A.nz, lane);
-#pragma unroll
-  for (int f = 0; f < NFIN; ++f) {
+  zinv_line_load<NJ, V>(pre, line(it), A.nz, lane);
+  for (; it < nitems; it += gridDim.x) {
+    const int f = it / ngroups, g = it - f * ngroups;
+    const int y = g / ngx, kx0 = (g - y * ngx) * 8;
+    const int kx = kx0 + wv;
     for (int z = lane; z < MZ; z += 64) buf[fpad(z)] = vzero<V>();
     wave_sync();
     const bool dz = (A.dzmask >> f) & 1u;  // d/dz = multiplication by i kz
@@ -69,8 +82,7 @@ __global__ __launch_bounds__(512) void k_zinv(PlaneArgs A) {
         buf[fpad(jj)] = val;
       }
     }
-    if (f + 1 < NFIN)
-      zinv_line_load<NJ, V>(pre, static_cast<const V*>(A.in[f + 1]) + ((size_t)y * A.nxh + kx) * A.nz, A.nz, lane);
+    if (it + (int)gridDim.x < nitems) zinv_line_load<NJ, V>(pre, line(it + gridDim.x), A.nz, lane);
     wave_sync();
     V* r = wave_fft<MZ, +1>(buf, scr, tw, lane);
     for (int z = lane; z < MZ; z += 64) tile[z * 8 + wv] = r[fpad(z)];
@@ -311,6 +323,7 @@ __device__ __forceinline__ void zfwd_tile_load(T (&px)[TL], T (&py)[TL], const V
   }
 }
 
+// Work items (output field, y, group of 8 kx lines) walked like k_zinv's.
 template <int MZ, int NFOUT, typename V>
 __global__ __launch_bounds__(512) void k_zfwd(PlaneArgs A) {
   using T = typename Scal<V>::T;
@@ -318,8 +331,8 @@ __global__ __launch_bounds__(512) void k_zfwd(PlaneArgs A) {
   V* sm = reinterpret_cast<V*>(smraw);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ngx = A.nxh / 8;
-  const int y = blockIdx.x / ngx, kx0 = (blockIdx.x % ngx) * 8;
-  const int kx = kx0 + wv;
+  const int ngroups = A.nyp * ngx;
+  const int nitems = NFOUT * ngroups;
   constexpr int BL = FFTBuf<MZ>::len;
   V* buf = sm + wv * 2 * BL;
   V* scr = buf + BL;
@@ -327,25 +340,40 @@ __global__ __launch_bounds__(512) void k_zfwd(PlaneArgs A) {
   V* tw = tile + 8 * MZ;
   for (int e = threadIdx.x; e < MZ; e += 512) tw[e] = static_cast<const V*>(A.twZ)[e];
   __syncthreads();
+  int it = blockIdx.x;
+  if (it >= nitems) return;
   const T scale = (T)(1.0 / ((double)A.mx * (double)MZ));
   const int nz = A.nz, nzh = A.nzh;
-  // the next field's tile is loaded into registers during this field's
+  auto src = [&](int i, int& kx0) -> const V* {
+    const int o = i / ngroups, g = i - o * ngroups;
+    const int y = g / ngx;
+    kx0 = (g - y * ngx) * 8;
+    return static_cast<const V*>(A.ZF) + ((size_t)o * A.nyp + y) * MZ * A.nxh;
+  };
+  // the next item's tile is loaded into registers during this item's
   // transform
   constexpr int TL = (MZ * 8 + 511) / 512;
   T prx[TL], pry[TL];
-  zfwd_tile_load<MZ, TL, V, T>(prx, pry, static_cast<const V*>(A.ZF) + (size_t)y * MZ * A.nxh, A.nxh, kx0);
-#pragma unroll
-  for (int o = 0; o < NFOUT; ++o) {
+  {
+    int k0;
+    const V* sp = src(it, k0);
+    zfwd_tile_load<MZ, TL, V, T>(prx, pry, sp, A.nxh, k0);
+  }
+  for (; it < nitems; it += gridDim.x) {
+    const int o = it / ngroups, g = it - o * ngroups;
+    const int y = g / ngx, kx0 = (g - y * ngx) * 8;
+    const int kx = kx0 + wv;
 #pragma unroll
     for (int q = 0; q < TL; ++q) {
       const int e = threadIdx.x + 512 * q;
       if (e < MZ * 8) tile[e] = mkv<V>(prx[q], pry[q]);
     }
     __syncthreads();
-    if (o + 1 < NFOUT)
-      zfwd_tile_load<MZ, TL, V, T>(prx, pry,
-                                   static_cast<const V*>(A.ZF) + ((size_t)(o + 1) * A.nyp + y) * MZ * A.nxh, A.nxh,
-                                   kx0);
+    if (it + (int)gridDim.x < nitems) {
+      int k0;
+      const V* sp = src(it + gridDim.x, k0);
+      zfwd_tile_load<MZ, TL, V, T>(prx, pry, sp, A.nxh, k0);
+    }
     for (int z = lane; z < MZ; z += 64) buf[fpad(z)] = tile[z * 8 + wv];
     wave_sync();
     V* r = wave_fft<MZ, -1>(buf, scr, tw, lane);
@@ -366,6 +394,43 @@ __global__ __launch_bounds__(512) void k_zfwd(PlaneArgs A) {
     }
     __syncthreads();
   }
+}
+
+// Grid of the item-walking z kernels: the blocks the device holds at once
+// (occupancy x CUs, queried once per kernel instantiation and device), at
+// most one per item.
+struct ZGridCache {
+  int dev = -1, blocks = 0;
+};
+template <class K>
+static int zitems_grid(ZGridCache& cache, K kernel, size_t lds, int nitems) {
+  int dev = 0;
+  hipGetDevice(&dev);
+  if (dev != cache.dev) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 512, lds) != hipSuccess || per_cu < 1)
+      per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
+    cache.blocks = per_cu * cus;
+    cache.dev = dev;
+  }
+  return nitems < cache.blocks ? nitems : cache.blocks;
+}
+template <int MZ, int NFIN, typename V>
+static void launch_zinv(const PlaneArgs& a, hipStream_t s) {
+  const size_t zlds = (size_t)(16 * FFTBuf<MZ>::len + 9 * MZ) * sizeof(V);
+  const int nitems = NFIN * a.nyp * (a.nxh / 8);
+  static ZGridCache cache;
+  const int grid = zitems_grid(cache, k_zinv<MZ, NFIN, V>, zlds, nitems);
+  hipLaunchKernelGGL((k_zinv<MZ, NFIN, V>), dim3(grid), dim3(512), zlds, s, a);
+}
+template <int MZ, int NFOUT, typename V>
+static void launch_zfwd(const PlaneArgs& a, hipStream_t s) {
+  const size_t zlds = (size_t)(16 * FFTBuf<MZ>::len + 9 * MZ) * sizeof(V);
+  const int nitems = NFOUT * a.nyp * (a.nxh / 8);
+  static ZGridCache cache;
+  const int grid = zitems_grid(cache, k_zfwd<MZ, NFOUT, V>, zlds, nitems);
+  hipLaunchKernelGGL((k_zfwd<MZ, NFOUT, V>), dim3(grid), dim3(512), zlds, s, a);
 }
 
 // ------------------------------------------------------------ launchers --
@@ -813,11 +878,10 @@ static hipError_t closure_fwd_impl(const PolyPlaneArgs& pa, hipStream_t s) {
   a.nxh = pa.nxh; a.nz = pa.nz; a.nzh = pa.nzh; a.nyp = pa.nyp; a.mx = pa.mx; a.mz = pa.mz;
   a.twZ = pa.twZ; a.twX = pa.twX;
   const size_t zb = (size_t)pa.nyp * pa.mz * pa.nxh;
-  const size_t zlds = (size_t)(16 * FFTBuf<MZ>::len + 9 * MZ) * sizeof(V);
   for (int g = 0; g < 2; ++g) {
     a.ZF = static_cast<V*>(pa.ZFP) + (size_t)(6 + 3 * g) * zb;
     for (int o = 0; o < 3; ++o) a.out[o] = pa.out[6 + 3 * g + o];
-    hipLaunchKernelGGL((k_zfwd<MZ, 3, V>), dim3(a.nyp * (a.nxh / 8)), dim3(512), zlds, s, a);
+    launch_zfwd<MZ, 3, V>(a, s);
   }
   return hipGetLastError();
 }
@@ -832,8 +896,7 @@ static hipError_t swirl_plane_impl(const PolyPlaneArgs& pa, bool zstage, hipStre
     for (int f = 0; f < 2; ++f) a.in[f] = pa.in[f];
     a.dzmask = 0x3u;  // dz v, dz w into ZP rows 0, 1
     a.Z = pa.ZP;
-    const size_t zlds = (size_t)(16 * FFTBuf<MZ>::len + 9 * MZ) * sizeof(V);
-    hipLaunchKernelGGL((k_zinv<MZ, 2, V>), dim3(a.nyp * (a.nxh / 8)), dim3(512), zlds, s, a);
+    launch_zinv<MZ, 2, V>(a, s);
   }
   const size_t xlds = (size_t)(MX + 4 * 2 * FFTBuf<MX>::len) * sizeof(V);
   hipLaunchKernelGGL((k_xplane_swirl<MX, V>), dim3(pa.nyp * pa.mz / 4), dim3(256), xlds, s, pa);
@@ -849,8 +912,7 @@ static hipError_t mr_lap_plane_impl(const MrLapArgs& la, hipStream_t s) {
   for (int f = 0; f < 3; ++f) a.in[f] = la.LT[f];
   a.dzmask = 0u;
   a.Z = la.LZ;
-  const size_t zlds = (size_t)(16 * FFTBuf<MZ>::len + 9 * MZ) * sizeof(V);
-  hipLaunchKernelGGL((k_zinv<MZ, 3, V>), dim3(a.nyp * (a.nxh / 8)), dim3(512), zlds, s, a);
+  launch_zinv<MZ, 3, V>(a, s);
   const size_t xlds = (size_t)(MX + 4 * 2 * FFTBuf<MX>::len) * sizeof(V);
   hipLaunchKernelGGL((k_mr_xinv<MX, V>), dim3(la.nyp * la.mz / 4), dim3(256), xlds, s, la);
   return hipGetLastError();
@@ -863,8 +925,6 @@ static hipError_t poly_plane_impl(const PolyPlaneArgs& pa, hipStream_t s) {
   std::memset(&a, 0, sizeof(a));
   a.nxh = pa.nxh; a.nz = pa.nz; a.nzh = pa.nzh; a.nyp = pa.nyp; a.mx = pa.mx; a.mz = pa.mz;
   a.twZ = pa.twZ; a.twX = pa.twX; a.beta = pa.beta;
-  const int ngx = a.nxh / 8;
-  const size_t zlds = (size_t)(16 * FFTBuf<MZ>::len + 9 * MZ) * sizeof(V);
   const size_t zb = (size_t)pa.nyp * pa.mz * pa.nxh;
   // ZP layout: dz u(3), dy u(3), C(6), dz C(6), dy C(6); the dz fields are
   // [0..2] and [12..17]
@@ -876,7 +936,7 @@ static hipError_t poly_plane_impl(const PolyPlaneArgs& pa, hipStream_t s) {
       if (o < 3 || (o >= 12 && o < 18)) a.dzmask |= 1u << f;
     }
     a.Z = static_cast<V*>(pa.ZP) + (size_t)8 * g * zb;
-    hipLaunchKernelGGL((k_zinv<MZ, 8, V>), dim3(a.nyp * ngx), dim3(512), zlds, s, a);
+    launch_zinv<MZ, 8, V>(a, s);
   }
   const size_t xlds = (size_t)(MX + 4 * 2 * FFTBuf<MX>::len) * sizeof(V);
   hipLaunchKernelGGL((k_xplane_poly<MX, V>), dim3(pa.nyp * pa.mz / 4), dim3(256), xlds, s, pa);
@@ -884,17 +944,15 @@ static hipError_t poly_plane_impl(const PolyPlaneArgs& pa, hipStream_t s) {
   for (int g = 0; g < 3; ++g) {
     a.ZF = static_cast<V*>(pa.ZFP) + (size_t)4 * g * zb;
     for (int o = 0; o < 4; ++o) a.out[o] = pa.out[4 * g + o];
-    hipLaunchKernelGGL((k_zfwd<MZ, 4, V>), dim3(a.nyp * ngx), dim3(512), zlds, s, a);
+    launch_zfwd<MZ, 4, V>(a, s);
   }
   return hipGetLastError();
 }
 
 template <int MX, int MZ, int NFIN, int NFOUT, typename V>
 static hipError_t plane_impl(const PlaneArgs& a, hipStream_t s, bool inverse_part, hipEvent_t* ev) {
-  const int ngx = a.nxh / 8;
-  const size_t zlds = (size_t)(16 * FFTBuf<MZ>::len + 9 * MZ) * sizeof(V);
   if (inverse_part) {
-    hipLaunchKernelGGL((k_zinv<MZ, NFIN, V>), dim3(a.nyp * ngx), dim3(512), zlds, s, a);
+    launch_zinv<MZ, NFIN, V>(a, s);
     if (ev) hipEventRecord(ev[0], s);
     if (a.nsrc > 0)
       hipLaunchKernelGGL((k_ibdrag<MX, V>), dim3((a.nsrc + 3) / 4), dim3(256), 0, s, a);
@@ -909,7 +967,7 @@ static hipError_t plane_impl(const PlaneArgs& a, hipStream_t s, bool inverse_par
                          xlds, s, a);
     if (ev) hipEventRecord(ev[2], s);
   }
-  hipLaunchKernelGGL((k_zfwd<MZ, NFOUT, V>), dim3(a.nyp * ngx), dim3(512), zlds, s, a);
+  launch_zfwd<MZ, NFOUT, V>(a, s);
   if (ev) hipEventRecord(ev[3], s);
   return hipGetLastError();
 }
