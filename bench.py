@@ -162,7 +162,7 @@ def measured_traffic(mode, kernel, tune):
     return table.get(mode, {}).get(kernel, {}).get(str(var))
 
 
-def cpu_baseline(nsteps=8):
+def cpu_baseline(nsteps=20):
     from threadpoolctl import threadpool_limits
     from oracle import kmm_oracle as O
     p = O.Params(nx=NX, ny=NY, nz=NZ, xl=XL, zl=ZL, re=RE, dt=DT, R_tau=RTAU, gain=GAIN, ugain=UGAIN,
