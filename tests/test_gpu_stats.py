@@ -1,7 +1,7 @@
 """writeoutputs statistics on the GPU (kmm_statistics / kmm_write_outputs)
 against the oracle's literal restatement on the 3/2 physical grid, the
-output files, and the profile checks of the north star: mixed precision
-against FP64 over a longer run, and the learned closure against FENE-P."""
+output files, the flowfield writer, and the profile check of mixed precision
+against FP64 over a longer run."""
 import numpy as np
 import pytest
 
@@ -111,8 +111,8 @@ def profile_err(a, b, keys):
 
 
 def test_mixed_precision_profiles_within_half_percent():
-    """The north star's bar for mixed precision: mean and RMS profiles within
-    0.5 % of the FP64 run (here 300 steps of a perturbed 64^3 channel)."""
+    """Mixed precision keeps the mean and RMS profiles within 0.5 % of the FP64
+    run (here 300 steps of a perturbed 64^3 channel)."""
     p = params(64, 64, 64, dt=0.01)
     print("case: mixed vs FP64 profiles, 64^3, 300 steps", flush=True)
     st = seeded(p, 74, amp=0.3)
