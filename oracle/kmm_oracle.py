@@ -57,6 +57,7 @@ class Params:
     ugain: float = 0.0
     R_tau: float = math.sqrt(3.0 * 2800.0)
     flow_select: int = 1
+    uinf: float = 0.0         # Uinf: the moving wall's velocity of the Couette cases
     perturbtime: int = 0
     # scalar (SCALAR build); scl_flag 6 = uniform volumetric source of rate
     # deltaT (extension, not a reference mode)
@@ -123,9 +124,14 @@ class Params:
         self.seght = seg
         # init_flow.f90:117 (constant pressure gradient)
         self.dPdx = -2.0 / self.yl * (2.0 * self.R_tau / (self.yl * self.re)) ** 2
-        # setbcs main_dns.f90:1240-1265 (channel) and 1406-1409
+        # setbcs main_dns.f90:1240-1265 (channel), 1268-1330 (2-D Couette:
+        # u = Uinf at the top wall; 3-D Couette: u = w = Uinf) and 1406-1409
         self.atu, self.btu, self.gtu = 1.0, 0.0, 0.0
         self.atw, self.btw, self.gtw = 1.0, 0.0, 0.0
+        if self.flow_select in (2, 3, 12):
+            self.gtu = self.uinf
+        if self.flow_select == 3:
+            self.gtw = self.uinf
         self.abu, self.bbu, self.gbu = 1.0, 0.0, 0.0
         self.abw, self.bbw, self.gbw = 1.0, 0.0, 0.0
         self.nt = 1

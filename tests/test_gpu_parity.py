@@ -168,6 +168,31 @@ def test_ib_generated_brick_steps():
     ch.close()
 
 
+@pytest.mark.parametrize("flow_select", [2, 3])
+def test_couette_steps(flow_select):
+    """Couette walls (u = Uinf at the top wall; also w for 3-D Couette) from
+    the reference's new-run Couette start plus a seeded perturbation."""
+    from polymer_dns_amd import ic
+    p = params(32, 32, 32, flow_select=flow_select, uinf=1.0)
+    log(f"case: Couette flow_select {flow_select}, 32^3")
+    base = ic.new_run_state(*ic.init_flow(2, p.nx, p.ny, p.nz, p.yl, p.re, p.R_tau, uinf=1.0), p.xl, p.zl)
+    st = seeded(p, 23)
+    st["u0"] = base["u0"]
+    st["w0"] = base["u0"].copy() if flow_select == 3 else np.zeros(p.nyp)
+    sol = O.Solver(p)
+    ost = {k: v.copy() for k, v in st.items()}
+    sol.startup(ost)
+    ch = kmm.Channel(cfg_for(p, flow_select=flow_select, uinf=1.0))
+    for k in ("v", "omy", "u0", "w0"):
+        ch.put(k, st[k])
+    ch.startup()
+    for it in range(5):
+        sol.step(ost)
+        ch.step(1)
+        compare(ost, ch, FIELDS_CHECK, TOL, f"step{it + 1}")
+    ch.close()
+
+
 MIXED_TOL = 1e-4   # FP32 transforms/products, FP64 state and tau solves
 
 

@@ -845,3 +845,22 @@ def test_mr_wall_bounce_and_periodic_wrap():
     assert new["y"][0] == p.ycoord[0] - ap and new["v"][0] == -1.0
     assert abs(new["x"][0] - 0.01) < 1e-12 and abs(new["z"][0] - (p.zl - 0.005)) < 1e-12
     assert new["u"][0] == 2.0 and new["w"][0] == -1.0
+
+
+@pytest.mark.parametrize("flow_select", [2, 3])
+def test_couette_profile_is_steady(flow_select):
+    """Couette walls (setbcs: u = Uinf at the top wall, and w = Uinf for 3-D
+    Couette): the linear profile Uinf (1 + y)/2 is a steady state of the
+    mean-flow solve, with no pressure gradient."""
+    p = params(8, 16, 8, flow_select=flow_select, uinf=1.5)
+    st = {k: np.zeros((p.nxh, p.nz, p.nyp), dtype=np.complex128) for k in ("v", "omy")}
+    lin = np.zeros(p.nyp)
+    lin[0], lin[1] = 0.75, 0.75           # 1.5 (1 + y)/2 in Chebyshev coefficients
+    st["u0"] = lin.copy()
+    st["w0"] = lin.copy() if flow_select == 3 else np.zeros(p.nyp)
+    sol = O.Solver(p)
+    sol.startup(st)
+    for _ in range(3):
+        sol.step(st)
+    assert np.abs(st["u0"] - lin).max() < 1e-12
+    assert np.abs(st["w0"] - (lin if flow_select == 3 else 0.0)).max() < 1e-12
