@@ -168,10 +168,11 @@ def test_ib_generated_brick_steps():
     ch.close()
 
 
-@pytest.mark.parametrize("flow_select", [2, 3])
+@pytest.mark.parametrize("flow_select", [2, 3, 11, 12])
 def test_couette_steps(flow_select):
     """Couette walls (u = Uinf at the top wall; also w for 3-D Couette) from
-    the reference's new-run Couette start plus a seeded perturbation."""
+    the reference's new-run Couette start plus a seeded perturbation; 11 and
+    12 are the channel and Couette walls without the pressure gradient."""
     from polymer_dns_amd import ic
     p = params(32, 32, 32, flow_select=flow_select, uinf=1.0)
     log(f"case: Couette flow_select {flow_select}, 32^3")
