@@ -14,7 +14,11 @@ with the state resident in HBM, bracketed by device synchronisation.
 
 roofline: the dominant kernel's algorithmic bytes per launch divided by its
 average launch time, measured with HIP events on the library's stream over
-the timed steps.  cpu_baseline: the numpy restatement of the reference step
+the timed steps.  Inside the timed steps only the pencil step kernel (the
+dominant one) is bracketed by events: events between every kernel cost
+about 7 % of the step.  The per-kernel breakdown (kernel_ms) comes from a
+second pass of the same number of steps right after, with events around
+every kernel.  cpu_baseline: the numpy restatement of the reference step
 timed on this host for a bounded number of steps of the same workload.
 """
 from __future__ import annotations
@@ -115,13 +119,15 @@ def polymer_mode(kmm, precision, steps, warmup, learned=False):
         # falls back to FENE-P
         ch.set_closure_audit(AUDIT_EVERY, AUDIT_WINDOW, AUDIT_GATE)
     ch.step(warmup)
-    ch.set_timing(True)
+    ch.set_timing(2)
     t0 = time.perf_counter()
     ch.step(steps)
     t = time.perf_counter() - t0
-    ktimes, _ = ch.kernel_times()
     if learned:
         extra["audit"] = dict(ch.closure_status(), every=AUDIT_EVERY, window=AUDIT_WINDOW, gate=AUDIT_GATE)
+    ch.set_timing(True)
+    ch.step(steps)
+    ktimes, _ = ch.kernel_times()
     ch.close()
     if net is not None:
         net.close()
@@ -191,13 +197,17 @@ def run_mode(kmm, precision, steps, warmup):
     ch.startup()
     tune = ch.autotune(5)     # set-up, outside the timed region (like FFTW planning)
     ch.step(warmup)
-    ch.set_timing(True)
+    ch.set_timing(2)          # events around the pencil step kernel only
     t0 = time.perf_counter()
     ch.step(steps)            # returns after a device synchronisation
     t = time.perf_counter() - t0
-    ktimes, _ = ch.kernel_times()
+    kpen, _ = ch.kernel_times()
     cfl, it = ch.last_cfl()
+    ch.set_timing(True)       # per-kernel breakdown: a second pass of the same length
+    ch.step(steps)
+    ktimes, _ = ch.kernel_times()
     ch.close()
+    ktimes["k_pencil_timed_pass"] = kpen["k_pencil"]
     return t, ktimes, cfl, it, tune
 
 
@@ -221,7 +231,10 @@ def main():
     value = G * args.steps / t / 1e6
     kb = kernel_bytes(G)
     dom = max((k for k in ktimes if k in kb), key=lambda k: ktimes[k])
-    achieved = kb[dom] / (ktimes[dom] * 1e-3) / 1e9
+    # the dominant kernel's duration from the timed pass (events bracket only
+    # the pencil there; another dominant kernel would use the breakdown pass)
+    dom_ms = ktimes["k_pencil_timed_pass"] if dom == "k_pencil" else ktimes[dom]
+    achieved = kb[dom] / (dom_ms * 1e-3) / 1e9
     step_bytes = 320.0 * G   # 40 FP64 field transfers per grid point (NS + scalar)
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mgridpt-steps/s", "n_gpus": 1,
@@ -230,7 +243,7 @@ def main():
         "config": {"workload": "Re_tau 180 channel 128x129x128, FP64, scalar Pr 0.71 + uniform source, "
                                "Goldstein rib IB", "grid": [NX, NY + 1, NZ], "dt": DT, "mode": "fp64"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "kernel_avg_ms": round(dom_ms, 4),
                      "traffic": measured_traffic("fp64", dom, tune), "algorithmic_bytes": kb[dom],
                      "kernel_ms": {k: round(v, 4) for k, v in ktimes.items()}},
         "step_roofline": {"bytes_per_gridpt": 320, "achieved_GBs": round(step_bytes / (ms * 1e-3) / 1e9, 1),

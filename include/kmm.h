@@ -210,7 +210,9 @@ int kmm_stage_times(kmm_ctx* ctx, double* pencil_ms, double* plane_ms);
 /* Per-kernel HIP-event timing of every step of later kmm_step calls:
  * average ms per step of k_pencil, k_zinv, k_ibdrag, k_xplane, k_zfwd,
  * the polymer pencil kernel, the polymer plane kernels and the tracer kernel
- * (ms[8]) over the last kmm_step call. */
+ * (ms[8]) over the last kmm_step call.  on = 1: events around every kernel;
+ * on = 2: around the pencil step kernel only (two events per step; the other
+ * entries read 0), which keeps the events' own cost out of the step time. */
 int kmm_set_timing(kmm_ctx* ctx, int on);
 int kmm_kernel_times(kmm_ctx* ctx, double* ms, int* nsamples);
 

@@ -1219,8 +1219,10 @@ static int enqueue_step(kmm_ctx* c, int s, hipEvent_t* ev, bool stage_ev) {
   c->it += 1;
   PencilArgs a = pencil_args(c);
   a.first_nl = c->nl_valid ? 1 : 0;
-  if (ev) hipEventRecord(ev[0], c->stream);
-  else if (stage_ev) hipEventRecord(c->ev[0], c->stream);
+  // timing 2: events around the pencil step kernel only (ev[1], ev[2])
+  const bool all_ev = ev && c->timing != 2;
+  if (all_ev) hipEventRecord(ev[0], c->stream);
+  else if (!ev && stage_ev) hipEventRecord(c->ev[0], c->stream);
   const bool att = c->closure != nullptr && c->closure_active;
   if (att && c->audit_every > 0 && c->audit_w < 0 && c->it == next_audit_from(c, c->it - 1)) c->audit_w = 0;
   const bool clo = att && c->audit_w < 0;
@@ -1263,7 +1265,7 @@ static int enqueue_step(kmm_ctx* c, int s, hipEvent_t* ev, bool stage_ev) {
   if (e != hipSuccess) return fail(KMM_E_HIP, std::string("pencil step: ") + hipGetErrorString(e));
   if (ev) hipEventRecord(ev[2], c->stream);
   else if (stage_ev) hipEventRecord(c->ev[1], c->stream);
-  int r = run_plane(c, c->it, c->cfl_d + s, ev ? ev + 3 : nullptr, true, pf.evaluate && !clo, clo);
+  int r = run_plane(c, c->it, c->cfl_d + s, all_ev ? ev + 3 : nullptr, true, pf.evaluate && !clo, clo);
   if (r) return r;
   if (!ev && stage_ev) hipEventRecord(c->ev[2], c->stream);
   c->pprev = pf;
@@ -1402,11 +1404,13 @@ static int step_chunk_finish(kmm_ctx* c, const StepChunk& ch) {
   const int n = ch.n;
   int r = check_cfl(c, n, ch.it0);
   if (c->timing) {
-    // intervals: poly pencil, pencil, zinv, ibdrag, xplane, zfwd, poly plane, tracers
+    // intervals: poly pencil, pencil, zinv, ibdrag, xplane, zfwd, poly plane,
+    // tracers (timing 2: the pencil only)
     static const int slot[8] = {5, 0, 1, 2, 3, 4, 6, 7};
     for (int s = 0; s < n; ++s) {
       hipEvent_t* ev = &c->tev[(size_t)s * 9];
       for (int k = 0; k < 8; ++k) {
+        if (c->timing == 2 && k != 1) continue;
         float t = 0.0f;
         hipEventElapsedTime(&t, ev[k], ev[k + 1]);
         c->ktime[slot[k]] += t;
@@ -1483,7 +1487,8 @@ extern "C" int kmm_ensemble_step(kmm_ctx** ctxs, int n, int nsteps) {
 
 extern "C" int kmm_set_timing(kmm_ctx* c, int on) {
   if (!c) return fail(KMM_E_ARG, "null argument");
-  c->timing = on ? 1 : 0;
+  if (on < 0 || on > 2) return fail(KMM_E_ARG, "timing mode is 0, 1 or 2");
+  c->timing = on;
   return KMM_OK;
 }
 

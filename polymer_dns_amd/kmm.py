@@ -295,7 +295,10 @@ class Channel:
     KERNELS = ("k_pencil", "k_zinv", "k_ibdrag", "k_xplane", "k_zfwd", "k_poly_pencil", "k_poly_plane", "k_tracer")
 
     def set_timing(self, on=True):
-        _check(self.lib, self.lib.kmm_set_timing(self.h, int(bool(on))))
+        """True / 1: HIP events around every kernel; 2: around the pencil step
+        kernel only; False / 0: off."""
+        mode = on if isinstance(on, int) and not isinstance(on, bool) else int(bool(on))
+        _check(self.lib, self.lib.kmm_set_timing(self.h, mode))
 
     def kernel_times(self):
         """Average ms per step of each kernel group over the last step() call."""
