@@ -219,9 +219,12 @@ def init_flow(flow_select, nx, ny, nz, yl, re, R_tau, uinf=1.0, kwall=0):
     still (flow_select <= 0 or 5), channel (1, init_flow.f90:82-139: the
     laminar profile re dPdx (d^2/2 - yl d/2), d = |y - y_min|, zero in the
     kwall rows next to each wall), Couette_2D (2, :147-186: uinf d/yl).
-    Couette_3D (3) and vortex_Couette (12) divide by ycoord(nyp) -
-    ycoord(nyp) = 0 (init_flow.f90:226-228, 818) and are refused; the
-    Blasius and streamwise-vortex starts (4, 10, 11) are not built."""
+    Refused: Couette_3D (3) and vortex_Couette (12) divide by ycoord(nyp) -
+    ycoord(nyp) = 0 (init_flow.f90:226-228, 818); vortex_channel (11) leaves
+    the vortex centre vyc, vzc unset (init_flow.f90:515-676, where
+    vortex_only sets them); the Blasius (4) and vortex_only (10) starts come
+    with free-slip / outflow walls (setbcs, main_dns.f90:1331-1390) that this
+    solver does not run."""
     nyp = ny + 1
     y = solver_ycoord(ny, yl)
     u = np.zeros((nx, nz, nyp))
@@ -243,7 +246,11 @@ def init_flow(flow_select, nx, ny, nz, yl, re, R_tau, uinf=1.0, kwall=0):
     if flow_select in (3, 12):
         raise ValueError(f"flow_select {flow_select}: the reference's initial u divides by "
                          "ycoord(nyp) - ycoord(nyp) = 0 (init_flow.f90:226-228, 818)")
-    raise NotImplementedError(f"flow_select {flow_select}: initial field not built (Blasius / streamwise vortices)")
+    if flow_select == 11:
+        raise ValueError("flow_select 11: vortex_channel uses the vortex centre vyc, vzc without "
+                         "setting it (init_flow.f90:515-676)")
+    raise NotImplementedError(f"flow_select {flow_select}: free-slip / outflow walls (setbcs) are not run "
+                              "by this solver, so its initial field is not built")
 
 
 def new_run_state(u, v, w, xl, zl):

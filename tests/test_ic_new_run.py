@@ -25,10 +25,12 @@ def test_channel_and_couette_profiles_closed_forms():
     assert np.abs(u[1, 2] - 1.5 * (y + 1.0) / 2.0).max() < 1e-14
     for fs in (0, -1, 5):
         assert not any(a.any() for a in ic.init_flow(fs, 8, ny, 8, 2.0, re, R_tau))
-    with pytest.raises(ValueError):
-        ic.init_flow(3, 8, ny, 8, 2.0, re, R_tau)
-    with pytest.raises(NotImplementedError):
-        ic.init_flow(10, 8, ny, 8, 2.0, re, R_tau)
+    for fs in (3, 11, 12):
+        with pytest.raises(ValueError):
+            ic.init_flow(fs, 8, ny, 8, 2.0, re, R_tau)
+    for fs in (4, 10):
+        with pytest.raises(NotImplementedError):
+            ic.init_flow(fs, 8, ny, 8, 2.0, re, R_tau)
 
 
 def _synthesize(a, p):
