@@ -83,6 +83,40 @@ def test_graph_replay_polymer_mixed():
         c.close()
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_graph_replay_learned_closure_with_audits(precision):
+    """Closure steps replayed as graphs (3-step history period) between audit
+    windows of FENE-P transport, against the same run launched step by step:
+    velocity, stress and audit state agree bit for bit."""
+    p = params(32, 32, 32, dt=0.005, polymer=True, crstrt=1, src_start=0)
+    print("case: graph replay of closure steps, 32^3, precision", precision, flush=True)
+    st = seeded(p, 57, amp=0.2)
+    st.update(poly_state(p, 58, amp=0.05))
+    kw = dict(polymer=1, zlmax=30.0, tpoly=0.5, diffpoly=0.5, qbeta=0.9, crstrt=1, src_start=0,
+              precision=precision)
+    net = kmm.Mlp(seed=6)   # untrained: small corrections to the extrapolation
+    net.set_norm(np.zeros(22), np.ones(22), np.zeros(6), np.full(6, 1e-3))
+    chans = [make(p, **kw), make(p, **kw)]
+    for ch in chans:
+        fill(ch, st, ("v", "omy", "u0", "w0") + tuple("c" + q for q in ("11", "12", "13", "22", "23", "33")))
+        ch.startup()
+        ch.set_poly_capture(True)
+        ch.step(3)
+        ch.set_closure(net)
+        ch.set_closure_audit(12, 3, 0.5)
+    chans[1].set_graph(3)
+    for call in range(4):
+        for c in chans:
+            c.step(9)
+        check_equal(chans[0], chans[1], ["v", "omy", "fn", "gn"])
+        assert np.array_equal(chans[0].poly_stress(1), chans[1].poly_stress(1))
+        assert chans[0].closure_status() == chans[1].closure_status()
+    assert chans[0].closure_status()["audits"] >= 2
+    for c in chans:
+        c.close()
+    net.close()
+
+
 def test_ensemble_matches_members_run_alone():
     """Uneven members (ny 48/64/96, nx = nz 64/96) with seeded Re_tau advance
     through kmm_ensemble_step exactly as each does alone."""
