@@ -205,6 +205,26 @@ __device__ PF phys_to_cheb_load(const V* __restrict__ src, size_t P, size_t p, V
 }
 
 // -------------------------------------------------------- tau systems --
+// Reciprocals of the tau row denominators for lane i (rows of lane 0 repeat
+// lane 1's; lane 0 carries no row): even rows j = 2i, odd rows j = 2i + 1,
+// 1/(4 j (j-1)), 1/(2 (j^2-1)), 1/(4 j (j+1)).  Folded at compile time with
+// IEEE division, so the values equal the run-time quotients bit for bit.
+struct TauRcp {
+  double r[6][64];
+  constexpr TauRcp() : r{} {
+    for (int i = 0; i < 64; ++i) {
+      const double je = 2.0 * (i < 1 ? 1 : i), jo = je + 1.0;
+      r[0][i] = 1.0 / (4.0 * je * (je - 1.0));
+      r[1][i] = 1.0 / (2.0 * (je * je - 1.0));
+      r[2][i] = 1.0 / (4.0 * je * (je + 1.0));
+      r[3][i] = 1.0 / (4.0 * jo * (jo - 1.0));
+      r[4][i] = 1.0 / (2.0 * (jo * jo - 1.0));
+      r[5][i] = 1.0 / (4.0 * jo * (jo + 1.0));
+    }
+  }
+};
+__device__ constexpr TauRcp kTauRcp{};
+
 // Row coefficients a(i,1..3) of evnc/oddc (solvers.f90:324-383) for the row
 // owned by lane i (1 <= i); wn = kx^2 + kz^2 + g.
 template <int NY>
@@ -237,16 +257,14 @@ __device__ __forceinline__ void tau_rhs(const PF& s, int lane, double2& we, doub
   we = czero();
   wo = czero();
   if (i >= 1 && i <= L - 1) {
-    const double j = 2.0 * i;
-    const double r1 = 1.0 / (4.0 * j * (j - 1.0)), r2 = 1.0 / (2.0 * (j * j - 1.0)), r3 = 1.0 / (4.0 * j * (j + 1.0));
+    const double r1 = kTauRcp.r[0][i], r2 = kTauRcp.r[1][i], r3 = kTauRcp.r[2][i];
     if (i <= L - 2) {
       const double cm = (i == 1) ? 2.0 : 1.0;
       we = cadd(cadd(cscale(sme, -cm * r1), cscale(s.a, r2)), cscale(spe, -r3));
     } else {
       we = cadd(cscale(sme, -r1), cscale(s.a, r2));
     }
-    const double jo = 2.0 * i + 1.0;
-    const double q1 = 1.0 / (4.0 * jo * (jo - 1.0)), q2 = 1.0 / (2.0 * (jo * jo - 1.0)), q3 = 1.0 / (4.0 * jo * (jo + 1.0));
+    const double q1 = kTauRcp.r[3][i], q2 = kTauRcp.r[4][i], q3 = kTauRcp.r[5][i];
     if (i <= L - 3) {
       wo = cadd(cadd(cscale(smo, -q1), cscale(s.b, q2)), cscale(spo, -q3));
     } else if (i == L - 2) {
@@ -276,9 +294,18 @@ __device__ __forceinline__ TauPar tau_eliminate(bool even, double wn, double dyd
   constexpr int L = NY / 2;
   TauPar t;
   t.even = even;
-  double a1, a2, a3;
-  if (even) coef_even<NY>(lane < 1 ? 1 : lane, wn, dyde, a1, a2, a3);
-  else coef_odd<NY>(lane < 1 ? 1 : lane, wn, dyde, a1, a2, a3);
+  // coef_even / coef_odd of the lane's row with the tabulated reciprocals
+  // (wn r in place of wn / (1/r): within one rounding of the quotient)
+  const int li = lane < 1 ? 1 : lane;
+  double a1, a3;
+  if (even) {
+    a1 = wn * kTauRcp.r[0][li];
+    if (li == 1) a1 *= 2.0;
+    a3 = (li <= L - 2) ? wn * kTauRcp.r[2][li] : 0.0;
+  } else {
+    a1 = wn * kTauRcp.r[3][li];
+    a3 = (li <= L - 3) ? wn * kTauRcp.r[5][li] : 0.0;
+  }
   const bool row = (lane >= 1 && lane <= L - 1);
   double invd_next = shdn1(invd);
   double rho = row ? a3 * invd_next : 0.0;
@@ -288,9 +315,9 @@ __device__ __forceinline__ TauPar tau_eliminate(bool even, double wn, double dyd
   t.a1 = row ? a1 : 0.0;
   t.invd = row ? invd : 0.0;
   if (even) {
-    double b1, b2, b3;
-    coef_even<NY>(L, wn, dyde, b1, b2, b3);
-    t.a1x = b1;
+    constexpr double jx = 2.0 * L;
+    constexpr double rx1 = 1.0 / (4.0 * jx * (jx - 1.0));
+    t.a1x = wn * rx1;  // coef_even's a1 of the extra row
     t.invdx = invdx;
     t.wpx = wx;
   } else {
