@@ -65,7 +65,8 @@ def workload_state():
 
 
 # Viscoelastic workload: 192 x 129 x 192, FENE-P (Wi_tau = 50, beta = 0.9,
-# L^2 = 900, Sc = 0.5), C = I + seeded 1e-3 noise, 100 000 tracers
+# L^2 = 900, Sc = 0.5), C at the FENE-P equilibrium of the mean shear plus
+# seeded 1e-3 noise, 100 000 tracers
 PNX, PNY, PNZ = 192, 128, 192
 WI_TAU, QBETA, ZLMAX, SC = 50.0, 0.9, 30.0, 0.5
 AUDIT_EVERY, AUDIT_WINDOW, AUDIT_GATE = 25, 3, 0.1  # closure audits in the learned-closure line

@@ -194,6 +194,33 @@ def test_couette_steps(flow_select):
     ch.close()
 
 
+def test_headline_configuration_steps():
+    """The benchmark's workload itself (128 x 129 x 128, Re_tau 180, scalar
+    with a uniform source, Goldstein rib) through start-up and 3 steps."""
+    import bench
+    p = params(bench.NX, bench.NY, bench.NZ, xl=bench.XL, zl=bench.ZL, re=bench.RE, dt=bench.DT,
+               R_tau=bench.RTAU, gain=bench.GAIN, ugain=bench.UGAIN, scalar=True, diff=bench.PR, scl_flag=6,
+               deltaT=bench.SRC)
+    log("case: headline configuration, 128x129x128")
+    im = bench.rib(bench.NX, bench.NY, bench.NZ)
+    st = bench.workload_state()
+    sol = O.Solver(p, imatrix=im)
+    ost = {k: v.copy() for k, v in st.items()}
+    sol.startup(ost)
+    ch = kmm.Channel(cfg_for(p))
+    ch.set_geometry(im)
+    for k in ("v", "omy", "u0", "w0", "scalar"):
+        ch.put(k, st[k])
+    ch.startup()
+    names = FIELDS_CHECK + ["scalar", "scn"]
+    compare(ost, ch, names, TOL, "startup")
+    for it in range(3):
+        sol.step(ost)
+        ch.step(1)
+        compare(ost, ch, names, TOL, f"step{it + 1}")
+    ch.close()
+
+
 MIXED_TOL = 1e-4   # FP32 transforms/products, FP64 state and tau solves
 
 
