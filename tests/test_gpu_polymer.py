@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from oracle import kmm_oracle as O
-from tests._cases import params, poly_state, rel, seeded
+from tests._cases import params, poly_state, rel, seeded, shear_conformation_mean
 
 pytestmark = pytest.mark.gpu
 
@@ -56,6 +56,22 @@ def test_polymer_step_matches_oracle_fp64():
     ch, sol, ost = start_pair(p, st)
     compare(ch, ost, 1e-10)
     for _ in range(6):
+        ch.step(1)
+        sol.step(ost)
+    compare(ch, ost, 1e-10)
+    ch.close()
+
+
+def test_polymer_step_ny128_matches_oracle_fp64():
+    """FENE-P at the benchmark's wall-normal resolution (ny = 128, the pencil
+    templates of the viscoelastic workload) with the conformation at the
+    mean-shear equilibrium."""
+    p = params(32, 128, 32, dt=0.002, crstrt=1, src_start=0, **POLY_KW)
+    print("case: FENE-P 32x128x32 FP64", flush=True)
+    st = seeded(p, 39, amp=0.2)
+    st.update(poly_state(p, 40, amp=0.05, mean=shear_conformation_mean(p, st["u0"], p.tpoly)))
+    ch, sol, ost = start_pair(p, st)
+    for _ in range(3):
         ch.step(1)
         sol.step(ost)
     compare(ch, ost, 1e-10)
