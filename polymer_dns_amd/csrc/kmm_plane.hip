@@ -633,6 +633,13 @@ struct PolyRow {
         s[m][cq + 1] -= u[dir][m] * ob[m];
       }
     }
+    // this pair's updates are done here: left free, the compiler defers them
+    // past the later pairs' transforms and keeps every pending pair's values
+    // live (482 registers per lane in FP64 at MX = 288 instead of 256)
+#pragma unroll
+    for (int m = 0; m < XPL; ++m)
+#pragma unroll
+      for (int e = 0; e < 6; ++e) asm volatile("" : "+v"(s[m][e])::"memory");
   }
   template <int E>
   __device__ __forceinline__ void fwd_s() {
@@ -661,8 +668,11 @@ struct PolyRow {
   }
 };
 
+// Rows up to MX = 288 run two waves per SIMD (256 registers: FP64 spills 3
+// values at MX = 288, FP32 none); FP64 rows of MX = 384 keep one.
 template <int MX, typename V>
-__global__ __launch_bounds__(256) void k_xplane_poly(PolyPlaneArgs A) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX <= 288 ? 2 : 1)))
+void k_xplane_poly(PolyPlaneArgs A) {
   using T = typename Scal<V>::T;
   constexpr int WPB = 4;
   extern __shared__ __align__(16) unsigned char smraw[];
