@@ -30,6 +30,17 @@ __device__ __forceinline__ void zinv_line_load(V (&pre)[NJ], const V* __restrict
   }
 }
 
+// LDS of the z kernels: per wave an FFT buffer and its scratch (the waves'
+// results are read in place by the block's store, and k_zfwd's block load
+// writes straight into them), then the MZ twiddles.  FP64 lines longer than
+// 192 read their twiddles from global memory instead, so that two blocks fit
+// a CU (16 x 307 x 16 B = 78.6 KB at MZ = 288).
+template <int MZ, typename V>
+struct ZLds {
+  static constexpr bool tw_global = sizeof(V) == 16 && MZ > 192;
+  static constexpr size_t bytes = (size_t)(16 * FFTBuf<MZ>::len + (tw_global ? 0 : MZ)) * sizeof(V);
+};
+
 // Work items are (field, y, group of 8 kx lines), walked by a grid sized to
 // the blocks the CUs hold at once (launch_zitems): with 9 fields per
 // (y, group) the last round of blocks is a small fraction of the work, where a
@@ -45,10 +56,15 @@ __global__ __launch_bounds__(512) void k_zinv(PlaneArgs A) {
   constexpr int BL = FFTBuf<MZ>::len;
   V* buf = sm + wv * 2 * BL;
   V* scr = buf + BL;
-  V* tile = sm + 16 * BL;       // [MZ][8]
-  V* tw = tile + 8 * MZ;        // [MZ] twiddles
-  for (int e = threadIdx.x; e < MZ; e += 512) tw[e] = static_cast<const V*>(A.twZ)[e];
-  __syncthreads();
+  const V* tw;
+  if constexpr (ZLds<MZ, V>::tw_global) {
+    tw = static_cast<const V*>(A.twZ);
+  } else {
+    V* twl = sm + 16 * BL;
+    for (int e = threadIdx.x; e < MZ; e += 512) twl[e] = static_cast<const V*>(A.twZ)[e];
+    __syncthreads();
+    tw = twl;
+  }
   int it = blockIdx.x;
   if (it >= nitems) return;
   auto line = [&](int i) -> const V* {
@@ -84,13 +100,14 @@ __global__ __launch_bounds__(512) void k_zinv(PlaneArgs A) {
     }
     if (it + (int)gridDim.x < nitems) zinv_line_load<NJ, V>(pre, line(it + gridDim.x), A.nz, lane);
     wave_sync();
-    V* r = wave_fft<MZ, +1>(buf, scr, tw, lane);
-    for (int z = lane; z < MZ; z += 64) tile[z * 8 + wv] = r[fpad(z)];
+    // every wave's result sits at the same offset (buf or scr) of its own
+    // pair of buffers: the block stores the group's z rows from there
+    const int roff = (int)(wave_fft<MZ, +1>(buf, scr, tw, lane) - buf);
     __syncthreads();
     V* dst = static_cast<V*>(A.Z) + ((size_t)f * A.nyp + y) * MZ * A.nxh;
     for (int e = threadIdx.x; e < MZ * 8; e += 512) {
       const int z = e >> 3, c = e & 7;
-      dst[(size_t)z * A.nxh + kx0 + c] = tile[e];
+      dst[(size_t)z * A.nxh + kx0 + c] = sm[c * 2 * BL + roff + fpad(z)];
     }
     __syncthreads();
   }
@@ -336,10 +353,15 @@ __global__ __launch_bounds__(512) void k_zfwd(PlaneArgs A) {
   constexpr int BL = FFTBuf<MZ>::len;
   V* buf = sm + wv * 2 * BL;
   V* scr = buf + BL;
-  V* tile = sm + 16 * BL;
-  V* tw = tile + 8 * MZ;
-  for (int e = threadIdx.x; e < MZ; e += 512) tw[e] = static_cast<const V*>(A.twZ)[e];
-  __syncthreads();
+  const V* tw;
+  if constexpr (ZLds<MZ, V>::tw_global) {
+    tw = static_cast<const V*>(A.twZ);
+  } else {
+    V* twl = sm + 16 * BL;
+    for (int e = threadIdx.x; e < MZ; e += 512) twl[e] = static_cast<const V*>(A.twZ)[e];
+    __syncthreads();
+    tw = twl;
+  }
   int it = blockIdx.x;
   if (it >= nitems) return;
   const T scale = (T)(1.0 / ((double)A.mx * (double)MZ));
@@ -363,10 +385,12 @@ __global__ __launch_bounds__(512) void k_zfwd(PlaneArgs A) {
     const int o = it / ngroups, g = it - o * ngroups;
     const int y = g / ngx, kx0 = (g - y * ngx) * 8;
     const int kx = kx0 + wv;
+    // the block's [MZ][8] group goes straight into the waves' FFT buffers
+    // (line c of the group to wave c)
 #pragma unroll
     for (int q = 0; q < TL; ++q) {
       const int e = threadIdx.x + 512 * q;
-      if (e < MZ * 8) tile[e] = mkv<V>(prx[q], pry[q]);
+      if (e < MZ * 8) sm[(e & 7) * 2 * BL + fpad(e >> 3)] = mkv<V>(prx[q], pry[q]);
     }
     __syncthreads();
     if (it + (int)gridDim.x < nitems) {
@@ -374,8 +398,6 @@ __global__ __launch_bounds__(512) void k_zfwd(PlaneArgs A) {
       const V* sp = src(it + gridDim.x, k0);
       zfwd_tile_load<MZ, TL, V, T>(prx, pry, sp, A.nxh, k0);
     }
-    for (int z = lane; z < MZ; z += 64) buf[fpad(z)] = tile[z * 8 + wv];
-    wave_sync();
     V* r = wave_fft<MZ, -1>(buf, scr, tw, lane);
     V* o2 = (r == buf) ? scr : buf;
     for (int j = lane; j < nz; j += 64) {
@@ -418,7 +440,7 @@ static int zitems_grid(ZGridCache& cache, K kernel, size_t lds, int nitems) {
 }
 template <int MZ, int NFIN, typename V>
 static void launch_zinv(const PlaneArgs& a, hipStream_t s) {
-  const size_t zlds = (size_t)(16 * FFTBuf<MZ>::len + 9 * MZ) * sizeof(V);
+  const size_t zlds = ZLds<MZ, V>::bytes;
   const int nitems = NFIN * a.nyp * (a.nxh / 8);
   static ZGridCache cache;
   const int grid = zitems_grid(cache, k_zinv<MZ, NFIN, V>, zlds, nitems);
@@ -426,7 +448,7 @@ static void launch_zinv(const PlaneArgs& a, hipStream_t s) {
 }
 template <int MZ, int NFOUT, typename V>
 static void launch_zfwd(const PlaneArgs& a, hipStream_t s) {
-  const size_t zlds = (size_t)(16 * FFTBuf<MZ>::len + 9 * MZ) * sizeof(V);
+  const size_t zlds = ZLds<MZ, V>::bytes;
   const int nitems = NFOUT * a.nyp * (a.nxh / 8);
   static ZGridCache cache;
   const int grid = zitems_grid(cache, k_zfwd<MZ, NFOUT, V>, zlds, nitems);
