@@ -137,6 +137,22 @@ def test_scalar_steps():
     ch.close()
 
 
+def test_long_line_steps():
+    """288-point x and z lines (nx = nz = 192, the viscoelastic workload's
+    plane size): the FP64 z transforms read their twiddles from global
+    memory at this length; start-up + 3 steps with the scalar."""
+    p = params(192, 32, 192, scalar=True, diff=0.71, scl_flag=6, deltaT=0.3)
+    log("case: 192x33x192 FP64 scalar")
+    sol, ost, ch = run_pair(p, 0)
+    names = FIELDS_CHECK + ["scalar", "scn", "scnm1"]
+    compare(ost, ch, names, TOL, "startup")
+    for it in range(3):
+        sol.step(ost)
+        ch.step(1)
+        compare(ost, ch, names, TOL, f"step{it + 1}")
+    ch.close()
+
+
 def test_ib_rib_steps():
     from polymer_dns_amd.restart import rib_geometry
     p = params(64, 32, 32, gain=20.0, ugain=100.0, dt=0.002)

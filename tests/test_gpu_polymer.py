@@ -78,6 +78,22 @@ def test_polymer_step_ny128_matches_oracle_fp64():
     ch.close()
 
 
+def test_polymer_step_long_lines_matches_oracle_fp64():
+    """FENE-P on 288-point x and z lines (nx = nz = 192, the viscoelastic
+    workload's plane size: the polymer x-plane kernel at two waves per SIMD
+    with its largest row, FP64 z twiddles from global memory)."""
+    p = params(192, 32, 192, dt=0.002, crstrt=1, src_start=0, **POLY_KW)
+    print("case: FENE-P 192x33x192 FP64", flush=True)
+    st = seeded(p, 82, amp=0.2)
+    st.update(poly_state(p, 83, amp=0.05))
+    ch, sol, ost = start_pair(p, st)
+    for _ in range(2):
+        ch.step(1)
+        sol.step(ost)
+    compare(ch, ost, 1e-10)
+    ch.close()
+
+
 def test_polymer_release_initialises_at_irstrt():
     """crstrt = 0, irstrt = 1: C starts at zero, becomes diag(c11z, c22z,
     c33z) at step 1, and the n -> n-1 copy of step 1 is taken before smoothing."""
