@@ -153,6 +153,21 @@ def test_long_line_steps():
     ch.close()
 
 
+@pytest.mark.parametrize("nx,nz", [(96, 96), (128, 64), (256, 128)])
+def test_plane_size_steps(nx, nz):
+    """Start-up + 2 FP64 steps on the remaining compiled plane sizes
+    (3/2 grids 144x144, 192x96, 384x192)."""
+    p = params(nx, 32, nz)
+    log(f"case: {nx}x33x{nz} FP64")
+    sol, ost, ch = run_pair(p, 0)
+    compare(ost, ch, FIELDS_CHECK, TOL, "startup")
+    for it in range(2):
+        sol.step(ost)
+        ch.step(1)
+        compare(ost, ch, FIELDS_CHECK, TOL, f"step{it + 1}")
+    ch.close()
+
+
 def test_ib_rib_steps():
     from polymer_dns_amd.restart import rib_geometry
     p = params(64, 32, 32, gain=20.0, ugain=100.0, dt=0.002)

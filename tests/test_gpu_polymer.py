@@ -78,12 +78,14 @@ def test_polymer_step_ny128_matches_oracle_fp64():
     ch.close()
 
 
-def test_polymer_step_long_lines_matches_oracle_fp64():
-    """FENE-P on 288-point x and z lines (nx = nz = 192, the viscoelastic
+@pytest.mark.parametrize("nx,nz", [(192, 192), (256, 128)])
+def test_polymer_step_long_lines_matches_oracle_fp64(nx, nz):
+    """FENE-P on long x lines: 288 points (nx = nz = 192, the viscoelastic
     workload's plane size: the polymer x-plane kernel at two waves per SIMD
-    with its largest row, FP64 z twiddles from global memory)."""
-    p = params(192, 32, 192, dt=0.002, crstrt=1, src_start=0, **POLY_KW)
-    print("case: FENE-P 192x33x192 FP64", flush=True)
+    with its largest such row, FP64 z twiddles from global memory) and 384
+    points (one wave per SIMD in FP64)."""
+    p = params(nx, 32, nz, dt=0.002, crstrt=1, src_start=0, **POLY_KW)
+    print(f"case: FENE-P {nx}x33x{nz} FP64", flush=True)
     st = seeded(p, 82, amp=0.2)
     st.update(poly_state(p, 83, amp=0.05))
     ch, sol, ost = start_pair(p, st)
