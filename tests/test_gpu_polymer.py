@@ -121,12 +121,16 @@ def test_oldroyd_b_matches_oracle():
     ch.close()
 
 
-def test_polymer_mixed_precision_error():
-    p = params(32, 32, 32, dt=0.005, crstrt=1, src_start=0, **POLY_KW)
+@pytest.mark.parametrize("nx,nz,nsteps", [(32, 32, 6), (192, 192, 2)])
+def test_polymer_mixed_precision_error(nx, nz, nsteps):
+    """Mixed mode vs the FP64 oracle; 192 x 192 is the viscoelastic plane
+    size (FP32 polymer x-plane kernel at two waves per SIMD, 288-point lines)."""
+    p = params(nx, 32, nz, dt=0.005 if nx == 32 else 0.002, crstrt=1, src_start=0, **POLY_KW)
+    print(f"case: FENE-P mixed {nx}x33x{nz}", flush=True)
     st = seeded(p, 37, amp=0.2)
     st.update(poly_state(p, 38))
     ch, sol, ost = start_pair(p, st, precision=1)
-    for _ in range(6):
+    for _ in range(nsteps):
         ch.step(1)
         sol.step(ost)
     w = compare(ch, ost, 1e-4, keys=["v", "omy"] + ["c" + q for q in O.POLY])
