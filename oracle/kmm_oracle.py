@@ -29,6 +29,18 @@ absent from the image, and a stand-in library is not allowed):
   * the full step -- laminar Poiseuille fixed point under constant pressure
     gradient, divergence-free velocity, Hermitian symmetry, and the linear
     Orr-Sommerfeld / Squire decay of a small perturbation
+  * the nonlinear path (tests/test_oracle_nonlinear_pins.py) -- the
+    Tollmien-Schlichting wave at Re 7500, alpha 1 grows at Orszag's (1971)
+    rate through the full step; an oblique Orr-Sommerfeld mode with its
+    forced Squire response follows its collocation eigenvalue in v and
+    omega_y; a scalar mode follows its advection-diffusion eigenvalue;
+    u . (u x a) = 0 per plane; u x omega and -u . grad(theta) of analytic
+    modes; nonlin annihilates gradient fields; vort is the curl and veloc
+    inverts it; gradv, gradscl, derivscji against analytic derivatives; the
+    NL filter is exp(-3 eta^10) with eta = 1 at kz = nz/2, n = ny, and the
+    state is not filtered.  Sign flips in any term of the cross product,
+    nonlin, vort, gradscl or gradv, the smoothing normaliser, or the AB2
+    weights each fail at least one of these pins.
 """
 from __future__ import annotations
 
