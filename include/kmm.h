@@ -131,7 +131,11 @@ int kmm_get_field(kmm_ctx* ctx, int field, double* host);
 int kmm_startup(kmm_ctx* ctx);
 
 /* Advance nsteps time steps (main_dns.f90:198-811).  Returns KMM_E_CFL with
- * the failing step in kmm_last_cfl if the CFL number exceeds 1. */
+ * the failing step and its CFL number in kmm_last_cfl if the CFL number
+ * exceeds 1 (solvers.f90:3503-3514, where the reference stops).  Steps are
+ * queued in chunks checked after they complete (64 steps, or one graph
+ * replay of kmm_set_graph's length), so on KMM_E_CFL the state has advanced
+ * to the end of the chunk that holds the failing step. */
 int kmm_step(kmm_ctx* ctx, int nsteps);
 
 /* gradv (derivatives.f90:6-86): the nine gradients u_ij = d u_i / d x_j
@@ -196,12 +200,63 @@ int kmm_solve(kmm_ctx* ctx, double* u, double x, double g, double dyde, const do
 int kmm_penta(kmm_ctx* ctx, double* u, double x, double g, double dyde, int ib, double at, double bt,
               const double* gtop, double ab, double bb, const double* gbot);
 
-/* vcw3dp (solvers.f90:1726), NS (+SCALAR) argument subset: inputs
- * u,v,w,omx,omy,omz (+ sclx,scly,sclz) spectral, outputs gn=(u x om)_x,
- * fn=(u x om)_y, omz_out=(u x om)_z (+ scn).  Updates the IB integrals. */
+/* vcw3dp(u,v,w,omx,omy,omz,fn,gn,u11..u33,[scalar,sclx,scly,sclz,scn],..),
+ * solvers.f90:1726-1740, NS (+SCALAR) argument list in the reference's
+ * order.  Inputs u, v, w, omx, omy, omz (+ sclx, scly, sclz) spectral;
+ * outputs fn = (u x om)_y, gn = (u x om)_x, and omz, overwritten with
+ * (u x om)_z as in the reference (+ scn = -u.grad(scalar) + source).  uij
+ * (the nine velocity gradients u11, u12, ... u33) and scalar may be NULL:
+ * they feed calcswirl and part_track, which kmm_step runs itself.  Updates
+ * the IB integrals; returns KMM_E_CFL if the CFL number exceeds 1. */
 int kmm_vcw3dp(kmm_ctx* ctx, const double* u, const double* v, const double* w, const double* omx,
-               const double* omy, const double* omz, const double* sclx, const double* scly,
-               const double* sclz, double* gn, double* fn, double* omz_out, double* scn, int it);
+               const double* omy, double* omz, double* fn, double* gn, const double* const* uij,
+               const double* scalar, const double* sclx, const double* scly, const double* sclz, double* scn,
+               int it);
+
+/* Spectral helper procedures on whole fields (complex (nyp,nz,nxh) each,
+ * the reference's layout), with the reference's argument meaning.  Work
+ * arrays of the reference (wrkc, wrk1) are internal. */
+/* phirhs(v,wrkc,wrk1,fn,fnm1,bcbot,bctop,var), solvers.f90:555-676
+ * (THETA <= 0.99): the CN/AB2 right-hand side replaces v; bcbot and bctop
+ * (complex (nz,nxh), may be NULL) are zeroed as the reference does. */
+int kmm_phirhs(kmm_ctx* ctx, double* v, const double* fn, const double* fnm1, double* bcbot, double* bctop,
+               double var);
+/* nonlin(fn,omz,gn,wrkc,wrk1), solvers.f90:1497-1572: on entry fn, omz, gn
+ * hold (u x om)_y, _z, _x; on return fn and gn hold the nonlinear terms of
+ * the v and omega_y equations (omz unchanged). */
+int kmm_nonlin(kmm_ctx* ctx, double* fn, const double* omz, double* gn);
+/* veloc(u,v,w,u0,w0,omy,wrkc), solvers.f90:1576-1659: u, w from continuity
+ * and omega_y; the mean mode takes u0, w0 (real (nyp)); v(:,1,1) = 0. */
+int kmm_veloc(kmm_ctx* ctx, double* u, double* v, double* w, const double* u0, const double* w0,
+              const double* omy);
+/* vort(u,v,w,omx,omz), solvers.f90:1663-1722 */
+int kmm_vort(kmm_ctx* ctx, const double* u, const double* v, const double* w, double* omx, double* omz);
+/* gradscl(scalar,sclx,scly,sclz), derivatives.f90:90-144 */
+int kmm_gradscl(kmm_ctx* ctx, const double* scalar, double* sclx, double* scly, double* sclz);
+/* derivscji(c11..c33, dc111..dc333), derivatives.f90:148-254: cij[9] =
+ * c11 c12 c13 c21 c22 c23 c31 c32 c33; dc[27] in the reference's dummy
+ * order dc111 dc112 dc113 dc211 dc212 dc213 dc311 dc312 dc313 dc121 ...
+ * dc323 dc131 ... dc333, with dc_ijk = d c_ij / d x_k. */
+int kmm_derivscji(kmm_ctx* ctx, const double* const* cij, double* const* dc);
+/* polyforce(s11,s12,s13,s22,s23,s33,t1,t2,t3), helpers.f90:498-560 */
+int kmm_polyforce(kmm_ctx* ctx, const double* s11, const double* s12, const double* s13, const double* s22,
+                  const double* s23, const double* s33, double* t1, double* t2, double* t3);
+/* subforce(gn,fn,omz,t1,t2,t3), helpers.f90:564-606: gn += t1, fn += t2,
+ * omz += t3 */
+int kmm_subforce(kmm_ctx* ctx, double* gn, double* fn, double* omz, const double* t1, const double* t2,
+                 const double* t3);
+/* polyNL(c11NL..c33NL, c11n.., c11nm1.., str11n.., str11nm1..),
+ * helpers.f90:610-674: each argument is an array of the six components
+ * 11 12 13 22 23 33; cNL = 3 (c_n - str_n/tpoly) - (c_nm1 - str_nm1/tpoly). */
+int kmm_polyNL(kmm_ctx* ctx, double* const* cNL, const double* const* cn, const double* const* cnm1,
+               const double* const* strn, const double* const* strnm1);
+/* norm(a), helpers.f90:91-158: reality of the mean mode, Hermitian
+ * symmetry of the kx = 0 plane, zero z-Nyquist row; in place. */
+int kmm_norm(kmm_ctx* ctx, double* a);
+/* uzero(u0,h1,h1l,uflag), solvers.f90:1065-1176: the mean-mode CN/AB2
+ * Robin solve; u0, h1, h1l real (nyp); uflag 1 (u0) or 3 (w0, with the
+ * reference's btu in the w rows, solvers.f90:1173).  u0 is replaced. */
+int kmm_uzero(kmm_ctx* ctx, double* u0, const double* h1, const double* h1l, int uflag);
 
 /* time spent in the first step of the last kmm_step call (ms): pencil
  * stage, plane stage */

@@ -9,7 +9,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libkmm.so")
-SOURCES = ["kmm_pencil.hip", "kmm_poly.hip", "kmm_plane.hip", "kmm_particles.hip", "kmm_mlp.hip", "kmm_audit.hip", "kmm_stats.hip", "kmm_api.cpp", "kmm_mlp_api.cpp"]
+SOURCES = ["kmm_pencil.hip", "kmm_poly.hip", "kmm_plane.hip", "kmm_particles.hip", "kmm_mlp.hip", "kmm_audit.hip", "kmm_stats.hip", "kmm_routines.hip", "kmm_api.cpp", "kmm_mlp_api.cpp"]
 ARCH = os.environ.get("KMM_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
@@ -33,6 +33,36 @@ def _compile(src: str) -> str:
     return out
 
 
+ROOT = os.path.dirname(HERE)
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FORTRAN_BIN = os.path.join(ROOT, "tests", "fortran", "bin")
+
+
+def build_fortran(verbose: bool = False) -> str | None:
+    """The Fortran host program of the drop-in test (tests/fortran/dns_kmm.f90
+    over include/kmm.f90), linked to libkmm.so by a path relative to the
+    binary.  Skipped (None) when the image has no flang."""
+    if not os.path.exists(FLANG):
+        if verbose:
+            print("flang not found: Fortran host program not built")
+        return None
+    os.makedirs(FORTRAN_BIN, exist_ok=True)
+    out = os.path.join(FORTRAN_BIN, "dns_kmm")
+    steps = [
+        [FLANG, "-c", os.path.join(ROOT, "include", "kmm.f90"), "-o", "kmm.o"],
+        [FLANG, "-c", os.path.join(ROOT, "include", "kmm_dropin.f90"), "-o", "kmm_dropin.o"],
+        [FLANG, "-I.", os.path.join(ROOT, "tests", "fortran", "dns_kmm.f90"), "kmm.o", "-L" + HERE, "-lkmm",
+         "-Wl,-rpath,$ORIGIN/../../../polymer_dns_amd", "-o", out],
+    ]
+    for cmd in steps:
+        r = subprocess.run(cmd, cwd=FORTRAN_BIN, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"flang failed: {' '.join(cmd)}\n{r.stderr}")
+    if verbose:
+        print("built", out)
+    return out
+
+
 def build(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         objs = list(ex.map(_compile, SOURCES))
@@ -44,6 +74,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_fortran(verbose)
     return LIB
 
 

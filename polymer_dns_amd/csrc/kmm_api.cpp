@@ -141,6 +141,12 @@ struct kmm_ctx {
   void* conf_nlp[6] = {};                         // its x-z transforms (physical y)
   double* audit_part = nullptr;                   // residual partial sums
   double* audit_acc = nullptr;                    // [2] residual sums of the running window
+  // per-routine slab solves (kmm_solve/kmm_penta): one kx slab and its tables,
+  // allocated at the first call and kept until kmm_destroy
+  void* slab = nullptr;
+  // per-routine entries (kmm_phirhs ...): 9 FP64 spectral fields and four
+  // mean profiles, allocated at the first call and kept until kmm_destroy
+  double2* rt = nullptr;
 };
 
 static void drop_graph(kmm_ctx* c);
@@ -1339,6 +1345,7 @@ extern "C" int kmm_set_graph(kmm_ctx* c, int steps) {
 // One chunk of kmm_step: launches up to `remaining` steps (a graph replay
 // when the steps are steady) without waiting; step_chunk_finish waits and
 // checks CFL and particles.
+constexpr int kEagerChunk = 64;
 struct StepChunk {
   int n = 0, it0 = 0;
   bool graph = false;
@@ -1347,7 +1354,9 @@ struct StepChunk {
 static int step_chunk_enqueue(kmm_ctx* c, int remaining, StepChunk& ch) {
   const int K = c->graph_steps;
   ch.graph = K > 0 && !c->timing && remaining >= K && steady(c, K);
-  ch.n = ch.graph ? K : std::min(remaining, c->cfl_cap);
+  // eager steps run in chunks of at most kEagerChunk between CFL checks, so
+  // that a failing step is reported at most that many steps late
+  ch.n = ch.graph ? K : std::min(remaining, kEagerChunk);
   ch.it0 = c->it + 1;
   const int n = ch.n;
   if (c->timing) {
@@ -1697,6 +1706,35 @@ extern "C" int kmm_cderiv(kmm_ctx* c, const double* f, double* df) {
   return download_field(c, b, df);
 }
 
+// Helmholtz constants of one kx slab: wn = kz^2 + x + g per kz, with the
+// Neumann zero-mode guard of solve/penta (wn <= 1e-20 -> 1, flagged).
+// Returns whether any kz was flagged.
+static bool slab_wn(const kmm_ctx* c, double x, double g, int ib, double* wn, int* kz) {
+  bool anyk = false;
+  for (int j = 0; j < c->nz; ++j) {
+    const int js = (j < c->nzh) ? j : j - c->nz;
+    wn[j] = (js * c->beta) * (js * c->beta) + x + g;
+    kz[j] = 0;
+    if (ib != 0 && wn[j] <= 1.0e-20) { wn[j] = 1.0; kz[j] = 1; anyk = true; }
+  }
+  return anyk;
+}
+
+// tau tables and the solve of one slab du [y][kz] (P = nz pencils) whose
+// wn, flags and boundary values are already on the device
+static hipError_t slab_launch(kmm_ctx* c, double2* du, double2* dtab, const double2* dbt, const double2* dbb,
+                              const double* dwn, const int* dkz, bool anyk, double dyde, int ib, bool penta,
+                              double at_, double btt, double ab_, double bbb) {
+  hipError_t e = kmm::launch_tau_tables_wn(c->ny, c->nz, dyde, dwn, dtab, c->stream);
+  PencilArgs a = pencil_args(c);
+  a.P = c->nz;
+  a.dyde = dyde;
+  if (e != hipSuccess) return e;
+  if (!penta) return kmm::launch_solve(c->ny, a, dtab, dwn, du, dbt, dbb, ib, anyk ? dkz : nullptr, c->stream);
+  return kmm::launch_penta(c->ny, a, dtab, dwn, du, at_, btt, dbt, ab_, bbb, dbb, +1, anyk ? dkz : nullptr,
+                           c->stream);
+}
+
 // one kx slab: u complex (nyp,nz) -> device [y][kz] with P = nz pencils
 static int slab_solve(kmm_ctx* c, double* u, double x, double g, double dyde, const double* bt, const double* bb,
                       int ib, bool penta, double at_, double btt, double ab_, double bbb) {
@@ -1710,46 +1748,29 @@ static int slab_solve(kmm_ctx* c, double* u, double x, double g, double dyde, co
     }
   std::vector<double> wn(nz);
   std::vector<int> kz(nz, 0);
-  bool anyk = false;
-  for (int j = 0; j < nz; ++j) {
-    const int js = (j < c->nzh) ? j : j - nz;
-    wn[j] = (js * c->beta) * (js * c->beta) + x + g;
-    if (ib != 0 && wn[j] <= 1.0e-20) { wn[j] = 1.0; kz[j] = 1; anyk = true; }
+  const bool anyk = slab_wn(c, x, g, ib, wn.data(), kz.data());
+  // slab layout: u [nyp][nz], 1/d table [nz][65], bctop, bcbot [nz] (double2),
+  // wn [nz] (double), zero-mode flags [nz] (int)
+  const size_t nu = (size_t)nyp * nz, ntab = (size_t)nz * 65;
+  if (!c->slab) {
+    int r = dalloc(c, &c->slab, (nu + ntab + 2 * (size_t)nz) * sizeof(double2) + (size_t)nz * (sizeof(double) + sizeof(int)));
+    if (r) return r;
   }
-  std::vector<double2> bth(nz), bbh(nz);
-  for (int j = 0; j < nz; ++j) {
-    bth[j] = make_double2(bt[2 * j], bt[2 * j + 1]);
-    bbh[j] = make_double2(bb[2 * j], bb[2 * j + 1]);
-  }
-  double2 *du, *dtab, *dbt, *dbb;
-  double* dwn;
-  int* dkz;
-  KMM_HIP(hipMalloc(&du, h.size() * sizeof(double2)));
-  KMM_HIP(hipMalloc(&dtab, (size_t)nz * 65 * sizeof(double2)));
-  KMM_HIP(hipMalloc(&dbt, nz * sizeof(double2)));
-  KMM_HIP(hipMalloc(&dbb, nz * sizeof(double2)));
-  KMM_HIP(hipMalloc(&dwn, nz * sizeof(double)));
-  KMM_HIP(hipMalloc(&dkz, nz * sizeof(int)));
+  double2* du = static_cast<double2*>(c->slab);
+  double2* dtab = du + nu;
+  double2* dbt = dtab + ntab;
+  double2* dbb = dbt + nz;
+  double* dwn = reinterpret_cast<double*>(dbb + nz);
+  int* dkz = reinterpret_cast<int*>(dwn + nz);
   KMM_HIP(hipMemcpyAsync(du, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
-  KMM_HIP(hipMemcpyAsync(dbt, bth.data(), nz * sizeof(double2), hipMemcpyHostToDevice, c->stream));
-  KMM_HIP(hipMemcpyAsync(dbb, bbh.data(), nz * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  KMM_HIP(hipMemcpyAsync(dbt, bt, nz * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  KMM_HIP(hipMemcpyAsync(dbb, bb, nz * sizeof(double2), hipMemcpyHostToDevice, c->stream));
   KMM_HIP(hipMemcpyAsync(dwn, wn.data(), nz * sizeof(double), hipMemcpyHostToDevice, c->stream));
   KMM_HIP(hipMemcpyAsync(dkz, kz.data(), nz * sizeof(int), hipMemcpyHostToDevice, c->stream));
-  hipError_t e = kmm::launch_tau_tables_wn(c->ny, nz, dyde, dwn, dtab, c->stream);
-  PencilArgs a = pencil_args(c);
-  a.P = nz;
-  a.dyde = dyde;
-  if (e == hipSuccess) {
-    if (!penta)
-      e = kmm::launch_solve(c->ny, a, dtab, dwn, du, dbt, dbb, ib, anyk ? dkz : nullptr, c->stream);
-    else
-      e = kmm::launch_penta(c->ny, a, dtab, dwn, du, at_, btt, dbt, ab_, bbb, dbb, +1, anyk ? dkz : nullptr,
-                            c->stream);
-  }
+  hipError_t e = slab_launch(c, du, dtab, dbt, dbb, dwn, dkz, anyk, dyde, ib, penta, at_, btt, ab_, bbb);
   if (e == hipSuccess)
     e = hipMemcpyAsync(h.data(), du, h.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  hipFree(du); hipFree(dtab); hipFree(dbt); hipFree(dbb); hipFree(dwn); hipFree(dkz);
   if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
   for (int j = 0; j < nz; ++j)
     for (int i = 0; i < nyp; ++i) {
@@ -1772,10 +1793,193 @@ extern "C" int kmm_penta(kmm_ctx* c, double* u, double x, double g, double dyde,
   return slab_solve(c, u, x, g, dyde, gtop, gbot, ib, true, at, bt, ab, bb);
 }
 
+// --------------------------------------------- per-routine field entries --
+static int rt_buffers(kmm_ctx* c) {
+  if (!c->rt) {
+    int r = dalloc(c, (void**)&c->rt, 9 * c->S * sizeof(double2) + 4 * (size_t)c->nyp * sizeof(double));
+    if (r) return r;
+  }
+  return KMM_OK;
+}
+static double* rt_mean(kmm_ctx* c, int i) { return reinterpret_cast<double*>(c->rt + 9 * c->S) + (size_t)i * c->nyp; }
+
+static kmm::RoutineArgs routine_args(kmm_ctx* c, int op) {
+  kmm::RoutineArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.op = op;
+  a.nxh = c->nxh; a.nz = c->nz; a.nzh = c->nzh; a.P = c->P;
+  a.alpha = c->alpha; a.beta = c->beta; a.dyde = c->dyde; a.re = c->cfg.re; a.dt = c->cfg.dt;
+  a.theta = c->cfg.theta;
+  return a;
+}
+
+// Uploads nin spectral fields (reference layout) into rt[0..nin), runs op
+// with outputs rt[nin..nin+nout), downloads them into out[].  m0/m1 are
+// optional mean profiles.
+static int run_routine(kmm_ctx* c, int op, const double* const* in, int nin, double* const* out, int nout,
+                       double par, const double* m0 = nullptr, const double* m1 = nullptr) {
+  for (int i = 0; i < nin; ++i) if (!in[i]) return fail(KMM_E_ARG, "null argument");
+  for (int o = 0; o < nout; ++o) if (!out[o]) return fail(KMM_E_ARG, "null argument");
+  KMM_HIP(hipSetDevice(c->device));
+  int r = rt_buffers(c);
+  if (r) return r;
+  kmm::RoutineArgs a = routine_args(c, op);
+  a.par = par;
+  for (int i = 0; i < nin; ++i) {
+    if ((r = upload_field(c, c->rt + (size_t)i * c->S, in[i]))) return r;
+    a.in[i] = c->rt + (size_t)i * c->S;
+  }
+  for (int o = 0; o < nout; ++o) a.out[o] = c->rt + (size_t)(nin + o) * c->S;
+  if (m0) { a.m0 = rt_mean(c, 0); KMM_HIP(hipMemcpyAsync(a.m0, m0, c->nyp * sizeof(double), hipMemcpyHostToDevice, c->stream)); }
+  if (m1) { a.m1 = rt_mean(c, 1); KMM_HIP(hipMemcpyAsync(a.m1, m1, c->nyp * sizeof(double), hipMemcpyHostToDevice, c->stream)); }
+  hipError_t e = kmm::launch_routine(c->ny, a, c->stream);
+  if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
+  for (int o = 0; o < nout; ++o)
+    if ((r = download_field(c, a.out[o], out[o]))) return r;
+  return KMM_OK;
+}
+
+extern "C" int kmm_phirhs(kmm_ctx* c, double* v, const double* fn, const double* fnm1, double* bcbot, double* bctop,
+                          double var) {
+  if (!c || !v || !fn || !fnm1) return fail(KMM_E_ARG, "null argument");
+  const double* in[3] = {v, fn, fnm1};
+  double* out[1] = {v};
+  int r = run_routine(c, kmm::RT_PHIRHS, in, 3, out, 1, var);
+  if (r) return r;
+  // phirhs zeroes the boundary values (solvers.f90:618-625)
+  const size_t nb = 2 * (size_t)c->nz * c->nxh;
+  if (bcbot) std::memset(bcbot, 0, nb * sizeof(double));
+  if (bctop) std::memset(bctop, 0, nb * sizeof(double));
+  return KMM_OK;
+}
+
+extern "C" int kmm_nonlin(kmm_ctx* c, double* fn, const double* omz, double* gn) {
+  if (!c) return fail(KMM_E_ARG, "null argument");
+  const double* in[3] = {fn, omz, gn};
+  double* out[2] = {fn, gn};
+  return run_routine(c, kmm::RT_NONLIN, in, 3, out, 2, 0.0);
+}
+
+extern "C" int kmm_veloc(kmm_ctx* c, double* u, double* v, double* w, const double* u0, const double* w0,
+                         const double* omy) {
+  if (!c || !u0 || !w0) return fail(KMM_E_ARG, "null argument");
+  const double* in[2] = {v, omy};
+  double* out[3] = {u, w, v};
+  return run_routine(c, kmm::RT_VELOC, in, 2, out, 3, 0.0, u0, w0);
+}
+
+extern "C" int kmm_vort(kmm_ctx* c, const double* u, const double* v, const double* w, double* omx, double* omz) {
+  if (!c) return fail(KMM_E_ARG, "null argument");
+  const double* in[3] = {u, v, w};
+  double* out[2] = {omx, omz};
+  return run_routine(c, kmm::RT_VORT, in, 3, out, 2, 0.0);
+}
+
+extern "C" int kmm_gradscl(kmm_ctx* c, const double* scalar, double* sclx, double* scly, double* sclz) {
+  if (!c) return fail(KMM_E_ARG, "null argument");
+  const double* in[1] = {scalar};
+  double* out[3] = {sclx, scly, sclz};
+  return run_routine(c, kmm::RT_GRAD, in, 1, out, 3, 0.0);
+}
+
+extern "C" int kmm_derivscji(kmm_ctx* c, const double* const* cij, double* const* dc) {
+  if (!c || !cij || !dc) return fail(KMM_E_ARG, "null argument");
+  // cij: c11 c12 c13 c21 c22 c23 c31 c32 c33; dc in derivscji's dummy order
+  // (derivatives.f90:148-151): dc111 dc112 dc113 dc211 dc212 dc213 dc311
+  // dc312 dc313, dc121 ... dc323, dc131 ... dc333; dc_ijk = d c_ij / d x_k
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      const double* in[1] = {cij[3 * i + j]};
+      double* out[3];
+      for (int k = 0; k < 3; ++k) out[k] = dc[9 * j + 3 * i + k];
+      int r = run_routine(c, kmm::RT_GRAD, in, 1, out, 3, 0.0);
+      if (r) return r;
+    }
+  return KMM_OK;
+}
+
+extern "C" int kmm_polyforce(kmm_ctx* c, const double* s11, const double* s12, const double* s13,
+                             const double* s22, const double* s23, const double* s33, double* t1, double* t2,
+                             double* t3) {
+  if (!c) return fail(KMM_E_ARG, "null argument");
+  const double* in[6] = {s11, s12, s13, s22, s23, s33};
+  double* out[3] = {t1, t2, t3};
+  return run_routine(c, kmm::RT_POLYFORCE, in, 6, out, 3, 0.0);
+}
+
+extern "C" int kmm_subforce(kmm_ctx* c, double* gn, double* fn, double* omz, const double* t1, const double* t2,
+                            const double* t3) {
+  if (!c) return fail(KMM_E_ARG, "null argument");
+  const double* in[6] = {gn, fn, omz, t1, t2, t3};
+  double* out[3] = {gn, fn, omz};
+  return run_routine(c, kmm::RT_SUBFORCE, in, 6, out, 3, 0.0);
+}
+
+extern "C" int kmm_polyNL(kmm_ctx* c, double* const* cNL, const double* const* cn, const double* const* cnm1,
+                          const double* const* strn, const double* const* strnm1) {
+  if (!c || !cNL || !cn || !cnm1 || !strn || !strnm1) return fail(KMM_E_ARG, "null argument");
+  if (!(c->cfg.tpoly > 0.0)) return fail(KMM_E_CONFIG, "polyNL needs tpoly > 0");
+  for (int q = 0; q < 6; ++q) {  // components 11 12 13 22 23 33
+    const double* in[4] = {cn[q], cnm1[q], strn[q], strnm1[q]};
+    double* out[1] = {cNL[q]};
+    int r = run_routine(c, kmm::RT_POLYNL, in, 4, out, 1, 1.0 / c->cfg.tpoly);
+    if (r) return r;
+  }
+  return KMM_OK;
+}
+
+extern "C" int kmm_norm(kmm_ctx* c, double* a) {
+  if (!c) return fail(KMM_E_ARG, "null argument");
+  const double* in[1] = {a};
+  double* out[1] = {a};
+  // in place on the device copy: every element is read and written by one wave
+  KMM_HIP(hipSetDevice(c->device));
+  int r = rt_buffers(c);
+  if (r) return r;
+  if (!a) return fail(KMM_E_ARG, "null argument");
+  kmm::RoutineArgs ar = routine_args(c, kmm::RT_NORM);
+  if ((r = upload_field(c, c->rt, in[0]))) return r;
+  ar.in[0] = c->rt;
+  ar.out[0] = c->rt;
+  hipError_t e = kmm::launch_routine(c->ny, ar, c->stream);
+  if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
+  return download_field(c, c->rt, out[0]);
+}
+
+extern "C" int kmm_uzero(kmm_ctx* c, double* u0, const double* h1, const double* h1l, int uflag) {
+  if (!c || !u0 || !h1 || !h1l) return fail(KMM_E_ARG, "null argument");
+  if (uflag != 1 && uflag != 3) return fail(KMM_E_ARG, "uflag must be 1 (u0) or 3 (w0)");
+  KMM_HIP(hipSetDevice(c->device));
+  int r = rt_buffers(c);
+  if (r) return r;
+  kmm::RoutineArgs a = routine_args(c, 0);
+  const PencilArgs pa = pencil_args(c);
+  // Robin rows of psolve1d: uflag 3 passes btu for w (solvers.f90:1173)
+  const double rb[6] = {uflag == 1 ? pa.atu : pa.atw, pa.btu, uflag == 1 ? pa.gtu : pa.gtw,
+                        uflag == 1 ? pa.abu : pa.abw, uflag == 1 ? pa.bbu : pa.bbw, uflag == 1 ? pa.gbu : pa.gbw};
+  std::memcpy(a.rb, rb, sizeof(rb));
+  const double* src[3] = {u0, h1, h1l};
+  double** dst[3] = {&a.m0, &a.m1, &a.m2};
+  for (int i = 0; i < 3; ++i) {
+    *dst[i] = rt_mean(c, i);
+    KMM_HIP(hipMemcpyAsync(*dst[i], src[i], c->nyp * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  }
+  a.m3 = rt_mean(c, 3);
+  hipError_t e = kmm::launch_uzero(c->ny, a, c->tabH, c->stream);
+  if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
+  KMM_HIP(hipMemcpyAsync(u0, a.m3, c->nyp * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  KMM_HIP(hipStreamSynchronize(c->stream));
+  return KMM_OK;
+}
+
 extern "C" int kmm_vcw3dp(kmm_ctx* c, const double* u, const double* v, const double* w, const double* omx,
-                          const double* omy, const double* omz, const double* sclx, const double* scly,
-                          const double* sclz, double* gn, double* fn, double* omz_out, double* scn, int it) {
-  if (!c || !u || !v || !w || !omx || !omy || !omz || !gn || !fn || !omz_out) return fail(KMM_E_ARG, "null argument");
+                          const double* omy, double* omz, double* fn, double* gn, const double* const* uij,
+                          const double* scalar, const double* sclx, const double* scly, const double* sclz,
+                          double* scn, int it) {
+  (void)uij;     // the velocity gradients feed calcswirl/part_track, which kmm_step runs itself
+  (void)scalar;  // the scalar enters only through its gradient
+  double* omz_out = omz;
+  if (!c || !u || !v || !w || !omx || !omy || !omz || !gn || !fn) return fail(KMM_E_ARG, "null argument");
   const bool sc = c->cfg.scalar != 0;
   if (sc && (!sclx || !scly || !sclz || !scn)) return fail(KMM_E_ARG, "scalar arrays required");
   KMM_HIP(hipSetDevice(c->device));

@@ -41,6 +41,24 @@ struct PencilArgs {
   double* dpdx;
 };
 
+// Per-routine entries (kmm_routines.hip): one spectral helper procedure of
+// the reference on whole FP64 fields [y][kx][kz].
+enum RoutineOp {
+  RT_PHIRHS = 0, RT_NONLIN, RT_VELOC, RT_VORT, RT_GRAD, RT_POLYFORCE, RT_SUBFORCE, RT_POLYNL, RT_NORM
+};
+struct RoutineArgs {
+  int op;
+  int nxh, nz, nzh, P;
+  double alpha, beta, dyde, re, dt, theta;
+  double par;                // phirhs: var; polyNL: 1/tpoly
+  const double2* in[6];
+  double2* out[3];
+  double *m0, *m1, *m2, *m3;  // real mean profiles (nyp)
+  double rb[6];              // uzero Robin rows: at, bt, gt, ab, bb, gb
+};
+hipError_t launch_routine(int ny, const RoutineArgs& a, hipStream_t s);
+hipError_t launch_uzero(int ny, const RoutineArgs& a, const double2* tab, hipStream_t s);
+
 // Per-pencil FENE-P stage (main_dns.f90:206-321, helpers.f90:498-674 and the
 // spectral half of the polymer terms of vcw3dp).
 struct PolyArgs {

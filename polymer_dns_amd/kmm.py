@@ -50,6 +50,8 @@ EXPORTS = [
     "kmm_set_closure_audit", "kmm_closure_status", "kmm_poly_conf_from_stress",
     "kmm_autotune", "kmm_set_tuning", "kmm_get_tuning", "kmm_statistics", "kmm_write_outputs",
     "kmm_flowfield", "kmm_write_flowfield",
+    "kmm_phirhs", "kmm_nonlin", "kmm_veloc", "kmm_vort", "kmm_gradscl", "kmm_derivscji", "kmm_polyforce",
+    "kmm_subforce", "kmm_polyNL", "kmm_norm", "kmm_uzero",
 ]
 
 
@@ -120,7 +122,19 @@ def load_library(path: str | None = None):
     lib.kmm_solve.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, dp, dp, C.c_int]
     lib.kmm_penta.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_int,
                               C.c_double, C.c_double, dp, C.c_double, C.c_double, dp]
-    lib.kmm_vcw3dp.argtypes = [C.c_void_p] + [dp] * 13 + [C.c_int]
+    pp = C.POINTER(dp)
+    lib.kmm_vcw3dp.argtypes = [C.c_void_p] + [dp] * 8 + [pp] + [dp] * 5 + [C.c_int]
+    lib.kmm_phirhs.argtypes = [C.c_void_p, dp, dp, dp, dp, dp, C.c_double]
+    lib.kmm_nonlin.argtypes = [C.c_void_p, dp, dp, dp]
+    lib.kmm_veloc.argtypes = [C.c_void_p] + [dp] * 6
+    lib.kmm_vort.argtypes = [C.c_void_p] + [dp] * 5
+    lib.kmm_gradscl.argtypes = [C.c_void_p] + [dp] * 4
+    lib.kmm_derivscji.argtypes = [C.c_void_p, pp, pp]
+    lib.kmm_polyforce.argtypes = [C.c_void_p] + [dp] * 9
+    lib.kmm_subforce.argtypes = [C.c_void_p] + [dp] * 6
+    lib.kmm_polyNL.argtypes = [C.c_void_p] + [pp] * 5
+    lib.kmm_norm.argtypes = [C.c_void_p, dp]
+    lib.kmm_uzero.argtypes = [C.c_void_p, dp, dp, dp, C.c_int]
     lib.kmm_stage_times.argtypes = [C.c_void_p, dp, dp]
     lib.kmm_set_timing.argtypes = [C.c_void_p, C.c_int]
     lib.kmm_kernel_times.argtypes = [C.c_void_p, dp, C.POINTER(C.c_int)]
@@ -172,6 +186,8 @@ def load_library(path: str | None = None):
 
 
 def _dp(a):
+    if a is None:
+        return None
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
@@ -342,14 +358,107 @@ class Channel:
                                             ab, bb, _dp(gb)))
         return u
 
+    # ---- per-routine entries (reference argument meaning) ----
+    def _c(self, a):
+        return np.ascontiguousarray(a, dtype=np.complex128)
+
+    def phirhs(self, v, fn, fnm1, var):
+        """phirhs (solvers.f90:555): the CN/AB2 right-hand side."""
+        v = self._c(v).copy()
+        _check(self.lib, self.lib.kmm_phirhs(self.h, _dp(v), _dp(self._c(fn)), _dp(self._c(fnm1)), None, None,
+                                             float(var)))
+        return v
+
+    def nonlin(self, fn, omz, gn):
+        """nonlin (solvers.f90:1497): returns (fn, gn)."""
+        fn, gn = self._c(fn).copy(), self._c(gn).copy()
+        _check(self.lib, self.lib.kmm_nonlin(self.h, _dp(fn), _dp(self._c(omz)), _dp(gn)))
+        return fn, gn
+
+    def veloc(self, v, omy, u0, w0):
+        """veloc (solvers.f90:1576): returns (u, v, w)."""
+        v = self._c(v).copy()
+        u, w = np.empty_like(v), np.empty_like(v)
+        u0 = np.ascontiguousarray(u0, dtype=np.float64)
+        w0 = np.ascontiguousarray(w0, dtype=np.float64)
+        _check(self.lib, self.lib.kmm_veloc(self.h, _dp(u), _dp(v), _dp(w), _dp(u0), _dp(w0), _dp(self._c(omy))))
+        return u, v, w
+
+    def vort(self, u, v, w):
+        """vort (solvers.f90:1663): returns (omx, omz)."""
+        u = self._c(u)
+        omx, omz = np.empty_like(u), np.empty_like(u)
+        _check(self.lib, self.lib.kmm_vort(self.h, _dp(u), _dp(self._c(v)), _dp(self._c(w)), _dp(omx), _dp(omz)))
+        return omx, omz
+
+    def gradscl(self, sc):
+        """gradscl (derivatives.f90:90): returns (sclx, scly, sclz)."""
+        sc = self._c(sc)
+        out = [np.empty_like(sc) for _ in range(3)]
+        _check(self.lib, self.lib.kmm_gradscl(self.h, _dp(sc), *[_dp(a) for a in out]))
+        return tuple(out)
+
+    def derivscji(self, cij):
+        """derivscji (derivatives.f90:148): cij = 9 fields c11 c12 c13 c21 ..
+        c33; returns {"ijk": d c_ij / d x_k}."""
+        cs = [self._c(a) for a in cij]
+        keys = [f"{i}{j}{k}" for j in (1, 2, 3) for i in (1, 2, 3) for k in (1, 2, 3)]
+        out = {k: np.empty_like(cs[0]) for k in keys}
+        pin = (C.POINTER(C.c_double) * 9)(*[_dp(a) for a in cs])
+        pout = (C.POINTER(C.c_double) * 27)(*[_dp(out[k]) for k in keys])
+        _check(self.lib, self.lib.kmm_derivscji(self.h, pin, pout))
+        return out
+
+    def polyforce(self, s):
+        """polyforce (helpers.f90:498): s = dict of 11 12 13 22 23 33;
+        returns (t1, t2, t3)."""
+        ss = [self._c(s[q]) for q in ("11", "12", "13", "22", "23", "33")]
+        out = [np.empty_like(ss[0]) for _ in range(3)]
+        _check(self.lib, self.lib.kmm_polyforce(self.h, *[_dp(a) for a in ss + out]))
+        return tuple(out)
+
+    def subforce(self, gn, fn, omz, t1, t2, t3):
+        """subforce (helpers.f90:564): returns (gn + t1, fn + t2, omz + t3)."""
+        g, f, o = self._c(gn).copy(), self._c(fn).copy(), self._c(omz).copy()
+        _check(self.lib, self.lib.kmm_subforce(self.h, _dp(g), _dp(f), _dp(o), _dp(self._c(t1)),
+                                               _dp(self._c(t2)), _dp(self._c(t3))))
+        return g, f, o
+
+    def polyNL(self, cn, cnm1, strn, strnm1):
+        """polyNL (helpers.f90:610): dicts by component; returns cNL."""
+        Q = ("11", "12", "13", "22", "23", "33")
+        ins = [[self._c(d[q]) for q in Q] for d in (cn, cnm1, strn, strnm1)]
+        out = [np.empty_like(ins[0][0]) for _ in Q]
+        arr = lambda lst: (C.POINTER(C.c_double) * 6)(*[_dp(a) for a in lst])
+        _check(self.lib, self.lib.kmm_polyNL(self.h, arr(out), *[arr(x) for x in ins]))
+        return dict(zip(Q, out))
+
+    def norm(self, a):
+        """norm (helpers.f90:91): returns the projected field."""
+        a = self._c(a).copy()
+        _check(self.lib, self.lib.kmm_norm(self.h, _dp(a)))
+        return a
+
+    def uzero(self, u0, h1, h1l, uflag):
+        """uzero (solvers.f90:1065): returns the new mean profile."""
+        u = np.ascontiguousarray(u0, dtype=np.float64).copy()
+        _check(self.lib, self.lib.kmm_uzero(self.h, _dp(u), _dp(np.ascontiguousarray(h1, dtype=np.float64)),
+                                            _dp(np.ascontiguousarray(h1l, dtype=np.float64)), int(uflag)))
+        return u
+
     def vcw3dp(self, u, v, w, omx, omy, omz, sclx=None, scly=None, sclz=None, it=1):
-        ins = [np.ascontiguousarray(a, dtype=np.complex128) for a in (u, v, w, omx, omy, omz)]
+        """vcw3dp (solvers.f90:1726): returns gn, fn, omz (, scn) -- the x, y,
+        z components of u x omega (and -u.grad(scalar) + source)."""
+        ins = [np.ascontiguousarray(a, dtype=np.complex128) for a in (u, v, w, omx, omy)]
+        oz = np.ascontiguousarray(omz, dtype=np.complex128).copy()   # in: omega_z, out: (u x om)_z
+        fn, gn = np.empty_like(oz), np.empty_like(oz)
         sc = [np.ascontiguousarray(a, dtype=np.complex128) for a in (sclx, scly, sclz)] \
-            if sclx is not None else []
-        outs = [np.empty_like(ins[0]) for _ in range(4 if sc else 3)]
-        ptrs = [_dp(a) for a in ins] + ([_dp(a) for a in sc] if sc else [None] * 3) + \
-               [_dp(a) for a in outs] + ([None] if not sc else [])
+            if sclx is not None else None
+        scn = np.empty_like(oz) if sc else None
+        ptrs = [_dp(a) for a in ins] + [_dp(oz), _dp(fn), _dp(gn), None, None] + \
+               ([_dp(a) for a in sc] + [_dp(scn)] if sc else [None] * 4)
         _check(self.lib, self.lib.kmm_vcw3dp(self.h, *ptrs, int(it)))
+        outs = [gn, fn, oz] + ([scn] if sc else [])
         return outs  # gn, fn, omz (, scn)
 
     # ---- Lagrangian tracers (part_track.f90) ----

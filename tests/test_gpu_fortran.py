@@ -1,0 +1,54 @@
+"""The Fortran host of the drop-in (tests/fortran/dns_kmm.f90 over the
+iso_c_binding module include/kmm.f90, built by build()) runs the
+reference's restart-run sequence on the GPU -- dns.config, setup/restart,
+NSTEPS steps, last-restart -- and writes the same restart bytes as the same
+sequence from Python through ctypes."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests._cases import params, seeded
+
+pytestmark = pytest.mark.gpu
+
+kmm = pytest.importorskip("polymer_dns_amd.kmm")
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "fortran", "bin", "dns_kmm")
+
+
+def write_config(path, nx, ny, nz, nsteps, dt):
+    txt = open(os.path.join(ROOT, "tests", "golden", "dns.config")).read()
+    for key, val in (("NSTEPS", nsteps), ("DT", dt), ("NY1", ny), ("NZ1", nz), ("NX1", nx)):
+        txt, n = re.subn(r"^(\s*)\S+(\s+! %s\s*)$" % key, r"\g<1>%s\g<2>" % val, txt, count=1, flags=re.M)
+        assert n == 1, key
+    with open(path, "w") as f:
+        f.write(txt)
+
+
+def test_fortran_host_restart_run_matches_python(tmp_path):
+    assert os.path.exists(BIN), "tests/fortran/bin/dns_kmm is built by build()"
+    cfgp, r0, rf, rp = (str(tmp_path / n) for n in ("dns.config", "restart", "restart_f", "restart_p"))
+    write_config(cfgp, 32, 32, 32, 5, 0.002)
+    cfg = kmm.read_config(cfgp)
+    assert (cfg.nx, cfg.ny, cfg.nz, cfg.nsteps) == (32, 32, 32, 5)
+    p = params(32, 32, 32, dt=cfg.dt, re=cfg.re, R_tau=cfg.r_tau, xl=cfg.xl, zl=cfg.zl)
+    st = seeded(p, 61, amp=0.2)
+    with kmm.Channel(cfg) as ch:
+        for k in ("v", "omy", "u0", "w0"):
+            ch.put(k, st[k])
+        ch.startup()
+        ch.write_restart(r0)
+    print("case: Fortran host, 32x33x32, 5 steps from a restart", flush=True)
+    r = subprocess.run([BIN, cfgp, r0, rf], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    with kmm.Channel(kmm.read_config(cfgp)) as ch:
+        ch.read_restart(r0)
+        ch.step(5)
+        ch.write_restart(rp)
+    a, b = open(rf, "rb").read(), open(rp, "rb").read()
+    assert len(a) == len(b) and a == b
+    assert a != open(r0, "rb").read()   # the run advanced the state

@@ -227,32 +227,63 @@ def test_couette_steps(flow_select):
 
 def test_headline_configuration_steps():
     """The benchmark's workload itself (128 x 129 x 128, Re_tau 180, scalar
-    with a uniform source, Goldstein rib) through start-up and 3 steps."""
+    with a uniform source, Goldstein rib) through start-up and 10 steps, in
+    FP64 (1e-10 per field) and in mixed precision (FP32 transforms and
+    products, FP64 state and tau solves; error against the FP64 oracle
+    stated per field, bound MIXED_TOL) beside the same oracle run."""
     import bench
     p = params(bench.NX, bench.NY, bench.NZ, xl=bench.XL, zl=bench.ZL, re=bench.RE, dt=bench.DT,
                R_tau=bench.RTAU, gain=bench.GAIN, ugain=bench.UGAIN, scalar=True, diff=bench.PR, scl_flag=6,
                deltaT=bench.SRC)
-    log("case: headline configuration, 128x129x128")
+    log("case: headline configuration, 128x129x128, FP64 and mixed")
     im = bench.rib(bench.NX, bench.NY, bench.NZ)
     st = bench.workload_state()
     sol = O.Solver(p, imatrix=im)
     ost = {k: v.copy() for k, v in st.items()}
     sol.startup(ost)
-    ch = kmm.Channel(cfg_for(p))
-    ch.set_geometry(im)
-    for k in ("v", "omy", "u0", "w0", "scalar"):
-        ch.put(k, st[k])
-    ch.startup()
+    chans = []
+    for prec in (0, 1):
+        ch = kmm.Channel(cfg_for(p, precision=prec))
+        ch.set_geometry(im)
+        for k in ("v", "omy", "u0", "w0", "scalar"):
+            ch.put(k, st[k])
+        ch.startup()
+        chans.append(ch)
     names = FIELDS_CHECK + ["scalar", "scn"]
-    compare(ost, ch, names, TOL, "startup")
-    for it in range(3):
+    compare(ost, chans[0], names, TOL, "startup")
+    for it in range(10):
         sol.step(ost)
-        ch.step(1)
-        compare(ost, ch, names, TOL, f"step{it + 1}")
-    ch.close()
+        for ch in chans:
+            ch.step(1)
+        compare(ost, chans[0], names, TOL, f"step{it + 1}")
+    errs = {k: rel(chans[1].get(k), ost[k]) for k in names}
+    for k, e in errs.items():
+        log(f"  mixed 128x129x128 step10 {k}: rel {e:.3e}")
+    for ch in chans:
+        ch.close()
+    assert max(errs.values()) < MIXED_TOL, errs
+    assert max(errs.values()) > 1e-12   # the FP32 path really ran
 
 
 MIXED_TOL = 1e-4   # FP32 transforms/products, FP64 state and tau solves
+
+
+@pytest.mark.parametrize("grid,precision", [((64, 48, 64), 0), ((64, 48, 64), 1), ((96, 96, 96), 0),
+                                            ((96, 96, 96), 1), ((128, 96, 128), 1)])
+def test_ensemble_member_sizes_match_oracle(grid, precision):
+    """The pencil templates the ensemble members use (ny = 48 and 96; the
+    members' planes 96 x 96, 144 x 144, 192 x 192) at a seeded Re_tau in
+    [120, 240]: start-up and 3 steps against the oracle, FP64 to 1e-10 and
+    mixed within MIXED_TOL of the FP64 oracle."""
+    rtau = float(np.random.default_rng(sum(grid)).uniform(120.0, 240.0))
+    p = params(*grid, dt=0.004, R_tau=rtau)
+    log(f"case: ensemble member {grid[0]}x{grid[1] + 1}x{grid[2]} precision {precision}, Re_tau {rtau:.1f}")
+    sol, ost, ch = run_pair(p, 0, seed=sum(grid) + precision, kw=dict(precision=precision))
+    for it in range(3):
+        sol.step(ost)
+        ch.step(1)
+    compare(ost, ch, FIELDS_CHECK, TOL if precision == 0 else MIXED_TOL, f"{grid} step3")
+    ch.close()
 
 
 @pytest.mark.parametrize("scalar,grid", [(False, (64, 64, 64)), (True, (32, 32, 32)), (True, (128, 32, 128))])
@@ -339,3 +370,53 @@ def test_calcswirl_matches_oracle():
     ref = O.calcswirl(g)
     assert np.abs(sw - ref).max() < 1e-2 * scale
     assert np.median(np.abs(sw - ref)) < 1e-12 * scale
+
+
+def test_cfl_number_and_stop_match_oracle():
+    """CFL number of every evaluation (solvers.f90:2985-2986) against the
+    oracle's, and the stop when it exceeds 1 (solvers.f90:3503-3514): a
+    start-up above 1 and a run that crosses 1 at its first step return
+    KMM_E_CFL with the failing step and CFL number the oracle reports."""
+    p = params(32, 32, 32, dt=0.005)
+    st = seeded(p, 91, amp=0.3)
+    sol = O.Solver(p)
+    ost = {k: v.copy() for k, v in st.items()}
+    sol.startup(ost)
+    c0 = ost["cflmax"]
+    with kmm.Channel(cfg_for(p)) as ch:
+        for k in ("v", "omy", "u0", "w0"):
+            ch.put(k, st[k])
+        ch.startup()
+        cfl, it = ch.last_cfl()
+        assert it == 0 and abs(cfl - c0) <= 1e-10 * c0
+        for step in range(1, 5):
+            sol.step(ost)
+            ch.step(1)
+            cfl, it = ch.last_cfl()
+            log(f"  step {step}: cfl {cfl:.12f} oracle {ost['cflmax']:.12f}")
+            assert it == step and abs(cfl - ost["cflmax"]) <= 1e-10 * ost["cflmax"]
+    # dt with the start-up CFL number at 0.999: the first step exceeds 1
+    # (the oracle's sequence is 0.999, 1.023), so kmm_step stops there
+    for scale, fail_it in ((0.999, 1), (1.01, 0)):
+        pq = params(32, 32, 32, dt=p.dt * scale / c0)
+        solq = O.Solver(pq)
+        oq = {k: v.copy() for k, v in st.items()}
+        with pytest.raises(O.CFLError):
+            solq.startup(oq)
+            for _ in range(3):
+                solq.step(oq)
+        want = oq["cflmax"]
+        assert want > 1.0
+        with kmm.Channel(cfg_for(pq)) as ch:
+            for k in ("v", "omy", "u0", "w0"):
+                ch.put(k, st[k])
+            if fail_it == 0:
+                with pytest.raises(kmm.CFLError):
+                    ch.startup()
+            else:
+                ch.startup()
+                with pytest.raises(kmm.CFLError):
+                    ch.step(3)
+            cfl, it = ch.last_cfl()
+        log(f"  dt scale {scale}: stopped at step {it} with cfl {cfl:.12f} (oracle {want:.12f})")
+        assert it == fail_it and abs(cfl - want) <= 1e-10 * want
