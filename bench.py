@@ -139,6 +139,41 @@ def polymer_mode(kmm, precision, steps, warmup, learned=False):
     return out
 
 
+ENS_SEED = 20261202 + 4
+
+
+def ensemble_mode(kmm, steps, warmup):
+    """The 16-member ensemble of uneven channels (nx = nz in {64, 96, 128},
+    ny in {48, 64, 80, 96}, Re_tau in [120, 240], seeded perturbed laminar
+    starts), mixed precision, advanced together by kmm_ensemble_step with
+    steady steps replayed as graphs; value = grid points of all members x
+    steps / wall seconds."""
+    from polymer_dns_amd import ic
+    chans, G = [], 0
+    for nx, ny, nz, rt, seed in ic.ensemble_members(ENS_SEED):
+        cfg = kmm.make_config(nx, ny, nz, xl=XL, zl=ZL, re=RE, dt=DT, r_tau=rt, precision=1)
+        ch = kmm.Channel(cfg)
+        st = ic.seeded_state(nx, ny, nz, XL, ZL, RE, rt, seed, amp=0.05, kmax=8)
+        for k in ("v", "omy", "u0", "w0"):
+            ch.put(k, st[k])
+        ch.startup()
+        ch.set_graph(10)
+        chans.append(ch)
+        G += nx * (ny + 1) * nz
+    kmm.ensemble_step(chans, warmup)
+    t0 = time.perf_counter()
+    kmm.ensemble_step(chans, steps)
+    t = time.perf_counter() - t0
+    members = sorted({(c.nx, c.nyp, c.nz) for c in chans})
+    for ch in chans:
+        ch.close()
+    return {"workload": "16 independent channels of uneven size, nx = nz in {64, 96, 128}, "
+                        "ny + 1 in {49, 65, 81, 97}, Re_tau in [120, 240], mixed precision, graphs of 10 steps",
+            "sizes": [list(m) for m in members], "gridpts": G, "steps": steps,
+            "value": round(G * steps / t / 1e6, 2), "unit": "Mgridpt-steps/s (all members)",
+            "ms_per_step": round(1000.0 * t / steps, 4)}
+
+
 def kernel_bytes(G):
     """Algorithmic bytes per launch (FP64, NS + scalar): field transfers x 8 G
     bytes (one spectral field = nyp*nz*nxh complex128 = 8 G bytes); the
@@ -220,6 +255,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--poly-steps", type=int, default=56,
                     help="steps of the viscoelastic workload (0: skip); 56 = two closure/audit periods")
+    ap.add_argument("--ensemble-steps", type=int, default=100,
+                    help="steps of the 16-member uneven ensemble (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs one channel on one GPU; --gpus must be 1"}))
@@ -266,6 +303,8 @@ def main():
                           "speedup_mixed_vs_fp64": round(pf["ms_per_step"] / pm["ms_per_step"], 3),
                           "speedup_learned_vs_fenep_fp64": round(pf["ms_per_step"] / pl["ms_per_step"], 3),
                           "speedup_learned_vs_fenep_mixed": round(pm["ms_per_step"] / pl["ms_per_step"], 3)}
+    if args.ensemble_steps > 0:
+        out["ensemble"] = ensemble_mode(kmm, args.ensemble_steps, 10)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

@@ -35,6 +35,8 @@ module kmm_c
     integer(c_int) :: particle_swirl
     real(c_double) :: sigmax, sigmay, sigmaz
     real(c_double) :: gravity
+    integer(c_int) :: bftail
+    real(c_double) :: xstart
   end type
 
   interface

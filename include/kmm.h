@@ -88,6 +88,10 @@ typedef struct kmm_config {
                                      (dns.config line 65) */
   double gravity;         /* buoyancy on inertial particles, -gravity (Pi_f - Pi_p) on
                              dv/dt (dns.config line 23, part_track.f90:873) */
+  int bftail;             /* BFTAIL: last x index of the buffer region (imatrix 6, 8),
+                             dns.config line 13, forcepre main_dns.f90:1495-1511 */
+  double xstart;          /* XSTART: x origin of the suction profile (imatrix 7),
+                             dns.config line 18, main_dns.f90:1537-1546 */
 } kmm_config;
 
 typedef struct kmm_ctx kmm_ctx;

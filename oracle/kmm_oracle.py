@@ -71,6 +71,9 @@ class Params:
     flow_select: int = 1
     uinf: float = 0.0         # Uinf: the moving wall's velocity of the Couette cases
     perturbtime: int = 0
+    kwall: int = 0            # KWALL: laminar start zero in the kwall rows next to each wall
+    bftail: int = 0           # BFTAIL: last x index of the buffer region (imatrix 6, 8)
+    xstart: float = 0.0       # XSTART: x origin of the suction profile (imatrix 7)
     # scalar (SCALAR build); scl_flag 6 = uniform volumetric source of rate
     # deltaT (extension, not a reference mode)
     scalar: bool = False
@@ -692,10 +695,52 @@ def kmaxsurf(imatrix):
     return int(ks.max()) + 1 if ks.size else 0
 
 
+def forcing_constants(p: Params):
+    """forcepre (main_dns.f90:1490-1546): buffer gains bfgain/bfugain (1200
+    entries, 1-based index i -> [i-1]), the spanwise wall slope length, the
+    suction velocity vdes (mx entries, x spacing xl/(mx-1)) and the initial
+    streamwise profile initu(y) of init_flow (channel: laminar re dPdx
+    (d^2/2 - yl d/2), zero in the kwall rows; 2-D Couette: Uinf d/yl)."""
+    bfhead = 2
+    bfwidth = p.bftail - bfhead
+    bfwh = int(bfwidth / 2) if bfwidth >= 0 else -int(-bfwidth / 2)
+    bfcntr = bfwh + 1
+    bfgain = np.zeros(1200)
+    bfugain = np.zeros(1200)
+    for i in range(1, bfwidth + 2):
+        ib = i - bfcntr
+        if abs(ib) <= int(bfwh / 3):
+            bfgain[i - 1] = 20.0 * math.exp(-60.0 * (ib / bfwh) ** 2)
+        bfugain[i - 1] = 20.0 * math.exp(-6.0 * (ib / bfwh) ** 2)
+    slope = bfwidth / 2.0
+    delxm = p.xl / float(p.mx - 1)
+    xsuction = delxm * (p.bftail - 1)
+    x = delxm * np.arange(p.mx)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        vdes = np.where(x <= xsuction, 0.0,
+                        (-0.8604 * p.uinf) / np.sqrt(p.re * p.uinf * (x + p.xstart - xsuction)))
+    d = np.abs(p.ycoord - p.ycoord[-1])
+    initu = np.zeros(p.nyp)
+    if p.flow_select == 1:
+        initu = p.re * p.dPdx * (0.5 * d ** 2 - p.yl / 2.0 * d)
+        i1 = np.arange(1, p.nyp + 1)
+        initu[(i1 <= p.kwall) | ((p.nyp - i1) <= p.kwall)] = 0.0
+    elif p.flow_select == 2:
+        initu = p.uinf * d / abs(p.ycoord[0] - p.ycoord[-1])
+    return dict(bfhead=bfhead, bfwidth=bfwidth, bfgain=bfgain, bfugain=bfugain, slope=slope, vdes=vdes,
+                initu=initu)
+
+
 def apply_ib_plane(i, up, vp, wp, vw, drag_src, fxi, fyi, fzi, imat, it, p: Params,
                    gain, ugain, kmax):
-    """Goldstein feedback at y-plane i (0-based) with the reference's
-    order-dependent 5x5 spreading (quirks Q5, Q6, Q7).
+    """Goldstein feedback at y-plane i (0-based), solvers.f90:2996-3108, with
+    the reference's order-dependent 5x5 spreading (quirks Q5, Q6, Q7).
+
+    Cell types: 1 (and 4 while it <= perturbtime) solid surface, 3
+    spanwise damping, 8 spanwise moving wall at wdes(x), 11 constant x-force
+    -- these set the drag that is spread; 6 buffer zone and 7 suction region
+    add their force at the cell itself.  initv = initw = 0 (x- and
+    z-independent starts).
 
     up/vp/wp/vw*: (mx, mz) arrays for this plane, x index first; the
     reference's (mz, mx) plane is column-major so the linear address of
@@ -703,25 +748,67 @@ def apply_ib_plane(i, up, vp, wp, vw, drag_src, fxi, fyi, fzi, imat, it, p: Para
     vw = [vwx, vwy, vwz] already hold the assigned products (incl. -dPdx).
     fxi/fyi/fzi: (mx, mz) integral slices, updated in place."""
     mx, mz = up.shape
-    src = np.argwhere((imat == 1) | ((imat == 4) & (it <= p.perturbtime)))
+    fc = forcing_constants(p) if np.any(np.isin(imat, (6, 7, 8))) else None
     dragx = np.zeros((mx, mz))
     dragy = np.zeros((mx, mz))
     dragz = np.zeros((mx, mz))
-    for k, j in src:
-        fxi[k, j] += up[k, j]
-        fyi[k, j] += vp[k, j]
-        fzi[k, j] += wp[k, j]
-        dragx[k, j] = -ugain * up[k, j] - gain * fxi[k, j]
-        dragy[k, j] = -ugain * vp[k, j] - gain * fyi[k, j]
-        dragz[k, j] = -ugain * wp[k, j] - gain * fzi[k, j]
-    other = np.argwhere((imat != 0) & (imat != 1) & (imat != 4))
-    if other.size:
-        raise NotImplementedError("imatrix types other than 0/1/4 are not supported")
-    if (i + 1) > kmax or src.size == 0:
+    flat = [a.reshape(-1) for a in vw]          # linear index = k*mz + j (0-based)
+    srcs = []
+    for k, j in np.argwhere(imat != 0):
+        t = int(imat[k, j])
+        u, v, w = up[k, j], vp[k, j], wp[k, j]
+        ls = k * mz + j
+        if t == 1 or (t == 4 and it <= p.perturbtime):
+            fxi[k, j] += u
+            fyi[k, j] += v
+            fzi[k, j] += w
+            dragx[k, j] = -ugain * u - gain * fxi[k, j]
+            dragy[k, j] = -ugain * v - gain * fyi[k, j]
+            dragz[k, j] = -ugain * w - gain * fzi[k, j]
+        elif t == 3:
+            fzi[k, j] += w
+            dragz[k, j] = -ugain * w - gain * fzi[k, j]
+        elif t == 8:
+            k1 = k + 1
+            bw = fc["bfwidth"]
+            if k1 <= bw * 3 + 1:
+                wdes = (0.3 * p.uinf) / fc["slope"] * (k1 - (bw * 3 + 1 - fc["slope"]))
+            else:
+                wdes = 0.3 * p.uinf
+            fxi[k, j] += u
+            fyi[k, j] += v
+            fzi[k, j] += w - wdes
+            dragx[k, j] = -ugain * u - gain * fxi[k, j]
+            dragy[k, j] = -ugain * v - gain * fyi[k, j]
+            dragz[k, j] = -ugain * (w - wdes) - gain * fzi[k, j]
+        elif t == 6:
+            b = (k + 1) - fc["bfhead"] + 1
+            gu, gg = fc["bfugain"][b - 1], fc["bfgain"][b - 1]
+            iu = fc["initu"][i]
+            fxi[k, j] += u - iu
+            fyi[k, j] += v
+            fzi[k, j] += w
+            flat[0][ls] += -gu * (u - iu) - gg * fxi[k, j]
+            flat[1][ls] += -gu * v - gg * fyi[k, j]
+            flat[2][ls] += -gu * w - gg * fzi[k, j]
+        elif t == 7:
+            vd = fc["vdes"][i]          # indexed by the y index (solvers.f90:3034)
+            fyi[k, j] += v - vd
+            flat[1][ls] += -gain * fyi[k, j] - ugain * (v - vd)
+            fxi[k, j] += u - p.uinf
+            flat[0][ls] += -gain * fxi[k, j] - ugain * (u - p.uinf)
+            fzi[k, j] += w
+            flat[2][ls] += -gain * fzi[k, j] - ugain * w
+        elif t == 11:
+            dragx[k, j] = 1.0
+        elif t != 4:
+            raise NotImplementedError(f"imatrix type {t}")
+        if t in (1, 3, 4, 8, 11):
+            srcs.append((k, j))
+    if (i + 1) > kmax or not srcs:
         return
     fs = fspread_matrix()
-    flat = [a.reshape(-1) for a in vw]          # linear index = k*mz + j (0-based)
-    for k, j in src:
+    for k, j in srcs:
         ls = k * mz + j
         for ii in range(-2, 3):
             ipii = k + ii

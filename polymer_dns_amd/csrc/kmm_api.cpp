@@ -64,6 +64,7 @@ struct kmm_ctx {
   // immersed boundary
   int nsrc = 0;
   int4* src = nullptr;
+  double* srcp = nullptr;   // [nsrc][6] per-type constants of the IB cells
   double *fint = nullptr, *drag = nullptr;
   int *trow = nullptr, *tx = nullptr, *tptr = nullptr, *ent_src = nullptr, *fixrow = nullptr, *fixx = nullptr;
   double* ent_w = nullptr;
@@ -222,7 +223,8 @@ extern "C" int kmm_config_read(const char* path, kmm_config* cfg) {
     cfg->dst = (T)v[0];                                                         \
   } while (0)
   RD(4, nsteps, int); RD(5, iprnfrq, int); RD(6, dt, double); RD(7, gain, double); RD(8, ugain, double);
-  RD(9, theta, double); RD(10, ny, int); RD(11, nz, int); RD(12, nx, int); RD(14, yl, double);
+  RD(9, theta, double); RD(10, ny, int); RD(11, nz, int); RD(12, nx, int); RD(13, bftail, int);
+  RD(14, yl, double); RD(18, xstart, double);
   RD(15, zl, double); RD(16, xl, double); RD(17, re, double); RD(19, uinf, double); RD(20, kwall, int);
   RD(21, perturbtime, int); RD(22, r_tau, double); RD(23, gravity, double); RD(29, irstrt, int);
   RD(30, crstrt, int);
@@ -353,7 +355,7 @@ static PlaneArgs plane_args(kmm_ctx* c, int it, unsigned long long* cfl) {
   a.Z = c->Z; a.ZF = c->ZF;
   for (int o = 0; o < 4; ++o) a.out[o] = c->NL[o];
   a.cfl = cfl;
-  a.nsrc = c->nsrc; a.src = c->src; a.fint = c->fint; a.drag = c->drag;
+  a.nsrc = c->nsrc; a.src = c->src; a.srcp = c->srcp; a.fint = c->fint; a.drag = c->drag;
   a.gain = c->cfg.gain; a.ugain = c->cfg.ugain; a.it = it; a.perturbtime = c->cfg.perturbtime;
   a.trow = c->trow; a.tx = c->tx; a.tptr = c->tptr; a.ent_src = c->ent_src; a.ent_w = c->ent_w;
   a.fixrow = c->fixrow; a.fixx = c->fixx;
@@ -366,8 +368,8 @@ extern "C" int kmm_create(const kmm_config* cfg, int device, kmm_ctx** out) {
   const int nx = cfg->nx, ny = cfg->ny, nz = cfg->nz;
   if (nx <= 0 || ny <= 0 || nz <= 0 || nx % 2 || ny % 2 || nz % 2)
     return fail(KMM_E_CONFIG, "grid sizes must be positive and even");
-  if (!(ny == 32 || ny == 48 || ny == 64 || ny == 96 || ny == 128))
-    return fail(KMM_E_CONFIG, "ny must be one of 32, 48, 64, 96, 128");
+  if (!(ny == 32 || ny == 48 || ny == 64 || ny == 80 || ny == 96 || ny == 128))
+    return fail(KMM_E_CONFIG, "ny must be one of 32, 48, 64, 80, 96, 128");
   if ((nx / 2) % 8 != 0) return fail(KMM_E_CONFIG, "nx/2 must be a multiple of 8");
   if (!fft_len_ok(3 * nx / 2) || !fft_len_ok(3 * nz / 2)) return fail(KMM_E_CONFIG, "3/2 grid not 2^a 3^b");
   if (cfg->theta > 0.99) return fail(KMM_E_CONFIG, "THETA > 0.99 is not supported");
@@ -536,29 +538,78 @@ extern "C" int kmm_set_geometry(kmm_ctx* c, const int32_t* im) {
   auto at = [&](int i, int j, int k) { return im[(size_t)i + (size_t)nyp * ((size_t)j + (size_t)mz * k)]; };
   int kmax = 0;
   std::vector<int4> src;
-  std::vector<int> srcid((size_t)nyp * mz * mx, -1);
+  bool direct = false, moving = false;
   for (int k = 0; k < mx; ++k)
     for (int j = 0; j < mz; ++j)
       for (int i = 0; i < nyp; ++i) {
         const int t = at(i, j, k);
-        if (t != 0 && t != 1 && t != 4)
-          return fail(KMM_E_CONFIG, "imatrix types other than 0, 1, 4 are not supported");
-        if (t == 1 || t == 3 || t == 4) kmax = std::max(kmax, k + 1);
+        if (!(t == 0 || t == 1 || t == 3 || t == 4 || t == 6 || t == 7 || t == 8 || t == 11))
+          return fail(KMM_E_CONFIG, "imatrix types are 0, 1, 3, 4, 6, 7, 8, 11 (solvers.f90:2996-3043)");
+        if (t == 1 || t == 3 || t == 4) kmax = std::max(kmax, k + 1);  // kmaxsurf, main_dns.f90:1526-1535
+        if (t == 6 || t == 7) direct = true;
+        if (t == 8) moving = true;
       }
-  // sources ordered by (y, linear in-plane address)
+  // forcepre (main_dns.f90:1490-1546): buffer gains, spanwise wall slope and
+  // the suction profile, with its x spacing xl/(mx-1)
+  const int bfhead = 2, bfwidth = c->cfg.bftail - bfhead, bfwh = bfwidth / 2;
+  const double slope = bfwidth / 2.0;
+  if (moving && !(slope > 0.0)) return fail(KMM_E_CONFIG, "imatrix 8 needs BFTAIL > 2 (slope length bfwidth/2)");
+  if (direct && !(c->cfg.flow_select == 1 || c->cfg.flow_select == 2))
+    return fail(KMM_E_CONFIG, "imatrix 6/7 need the initial profile of flow_select 1 or 2");
+  std::vector<double> bfgain(1200, 0.0), bfugain(1200, 0.0);
+  for (int i = 1; i <= bfwidth + 1 && i <= 1200; ++i) {
+    const int ib = i - (bfwh + 1);
+    if (bfwh > 0 && std::abs(ib) <= bfwh / 3) bfgain[i - 1] = 20.0 * std::exp(-60.0 * std::pow((double)ib / bfwh, 2));
+    if (bfwh > 0) bfugain[i - 1] = 20.0 * std::exp(-6.0 * std::pow((double)ib / bfwh, 2));
+  }
+  const double delxf = c->cfg.xl / (double)(mx - 1), xsuction = delxf * (c->cfg.bftail - 1);
+  auto vdes = [&](int i1) {  // indexed by the y index, as vcw3dp does (solvers.f90:3034)
+    const double x = delxf * (i1 - 1);
+    return x <= xsuction ? 0.0 : (-0.8604 * c->cfg.uinf) / std::sqrt(c->cfg.re * c->cfg.uinf * (x + c->cfg.xstart - xsuction));
+  };
+  // initial profile of init_flow (x and z independent): channel or Couette
+  std::vector<double> initu(nyp, 0.0);
+  if (direct) {
+    const double yl = c->cfg.yl, ymin = c->ycoord[nyp - 1];
+    for (int i = 0; i < nyp; ++i) {
+      const double d = std::fabs(c->ycoord[i] - ymin);
+      if (c->cfg.flow_select == 2) initu[i] = c->cfg.uinf * d / std::fabs(c->ycoord[0] - ymin);
+      else if (c->cfg.flow_select == 1 && !(i + 1 <= c->cfg.kwall || nyp - (i + 1) <= c->cfg.kwall))
+        initu[i] = c->cfg.re * c->dPdx * (0.5 * d * d - 0.5 * yl * d);
+    }
+  }
+  // sources ordered by (y, linear in-plane address), with per-type constants
+  std::vector<double> par;   // [6] per source
   for (int i = 0; i < nyp; ++i)
     for (int k = 0; k < mx; ++k)
       for (int j = 0; j < mz; ++j) {
         const int t = at(i, j, k);
-        if (t == 1 || t == 4) {
-          srcid[(size_t)i * mz * mx + (size_t)k * mz + j] = (int)src.size();
-          src.push_back(make_int4(i, j, k, t));
+        if (t == 0) continue;
+        double q[6] = {0, 0, 0, 0, 0, 0};
+        if (t == 6) {  // bfugain, bfgain at k - bfhead + 1, initu, initv, initw at (i, j, k)
+          const int b = (k + 1) - bfhead + 1;
+          if (b < 1 || b > 1200) return fail(KMM_E_CONFIG, "imatrix 6 cell outside the buffer gains (x index < 2)");
+          q[0] = bfugain[b - 1]; q[1] = bfgain[b - 1]; q[2] = initu[i];
+        } else if (t == 7) {  // vdes(i), Uinf, initw
+          q[0] = vdes(i + 1); q[1] = c->cfg.uinf;
+        } else if (t == 8) {  // wdes at x index k
+          const int k1 = k + 1;
+          q[0] = (k1 <= bfwidth * 3 + 1) ? (0.3 * c->cfg.uinf) / slope * (k1 - (bfwidth * 3 + 1 - slope))
+                                         : 0.3 * c->cfg.uinf;
         }
+        src.push_back(make_int4(i, j, k, t));
+        par.insert(par.end(), q, q + 6);
       }
   struct Ent { int y, z, x, s; double w; };
   std::vector<Ent> ents;
   for (size_t s = 0; s < src.size(); ++s) {
     const int i = src[s].x, j = src[s].y, k = src[s].z;
+    if (src[s].w == 6 || src[s].w == 7) {
+      // buffer and suction cells add their force at the cell itself, after
+      // its product is assigned, and spread nothing
+      ents.push_back({i, j, k, (int)s, 1.0});
+      continue;
+    }
     if (i + 1 > kmax) continue;
     const long ls = (long)k * mz + j;
     for (int ii = -2; ii <= 2; ++ii) {
@@ -619,6 +670,7 @@ extern "C" int kmm_set_geometry(kmm_ctx* c, const int32_t* im) {
   c->nsrc = (int)src.size();
   c->src_h = src;
   if ((r = up((void**)&c->src, src.data(), src.size() * sizeof(int4)))) return r;
+  if ((r = up((void**)&c->srcp, par.data(), par.size() * sizeof(double)))) return r;
   if ((r = dalloc(c, (void**)&c->fint, 3 * src.size() * sizeof(double)))) return r;
   if ((r = dalloc(c, (void**)&c->drag, 3 * src.size() * sizeof(double)))) return r;
   if ((r = up((void**)&c->trow, trow.data(), trow.size() * sizeof(int)))) return r;

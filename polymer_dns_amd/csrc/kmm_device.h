@@ -1,6 +1,6 @@
 // Device-side primitives shared by the pencil (y) and plane (x/z) kernels of
 // the KMM channel step: complex FP64 arithmetic, wave64 scans/reductions, and
-// an LDS Stockham FFT for lengths 2^a 3^b.  Written for gfx950 (CDNA4):
+// an LDS Stockham FFT for lengths 2^a 3^b 5^c.  Written for gfx950 (CDNA4):
 // 64-wide wavefronts, one wavefront owns one FFT line / one Chebyshev pencil.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -187,21 +187,30 @@ __device__ __forceinline__ double2 prefix_affine(double b, double2 a, int lane) 
 }
 
 // ------------------------------------------------------------------ FFT --
-// Radix plan for N = 2^a 3^b: radix-3 stages first, then radix-4, one
-// radix-2 if a is odd.
+// Radix plan for N = 2^a 3^b 5^c (the radices of its stages, in order).
 struct FFTPlan {
   int n;
   int nst;
   int r[16];
 };
 // Radix plan for one wavefront: each stage runs ceil((N/R)/64) butterfly
-// rounds, so the plan minimises the total number of rounds (then the number
-// of stages) over factorisations into radices 8, 6, 4, 3, 2.
+// rounds.  Cost model 0 minimises the total number of rounds (then the
+// number of stages); model 1 minimises an estimate of the instructions a
+// lane issues: per round of a radix-R stage, the butterfly's adds and
+// multiplies, R-1 twiddle multiplies (none in the first stage), R LDS loads
+// and stores and the index arithmetic, plus a fixed cost per stage for its
+// wave barrier and LDS latency.  Radices 8, 6, 5, 4, 3, 2.
 struct PlanSearch {
   int best_cost, best_nst, cur_nst;
   int best[16], cur[16];
 };
-__host__ __device__ constexpr void plan_dfs(PlanSearch& s, int n, int m, int cost) {
+__host__ __device__ constexpr int radix_cost(int r, bool first) {
+  // butterfly arithmetic (real ops) + twiddles (6 per complex multiply) +
+  // 2R LDS ops + 10 index operations
+  const int bf = r == 2 ? 4 : r == 3 ? 12 : r == 4 ? 16 : r == 5 ? 34 : r == 6 ? 40 : 60;
+  return bf + (first ? 0 : 6 * (r - 1)) + 2 * r + 10;
+}
+__host__ __device__ constexpr void plan_dfs(PlanSearch& s, int n, int m, int cost, int model) {
   if (m == 1) {
     if (cost < s.best_cost || (cost == s.best_cost && s.cur_nst < s.best_nst)) {
       s.best_cost = cost;
@@ -211,19 +220,20 @@ __host__ __device__ constexpr void plan_dfs(PlanSearch& s, int n, int m, int cos
     return;
   }
   if (s.cur_nst >= 16 || cost >= s.best_cost) return;
-  const int rad[5] = {8, 6, 4, 3, 2};
-  for (int i = 0; i < 5; ++i) {
+  const int rad[6] = {8, 6, 5, 4, 3, 2};
+  for (int i = 0; i < 6; ++i) {
     const int r = rad[i];
     if (m % r) continue;
-    const int nb = n / r;
+    const int rounds = (n / r + 63) / 64;
+    const int c = model == 0 ? rounds : rounds * radix_cost(r, s.cur_nst == 0) + 24;
     s.cur[s.cur_nst++] = r;
-    plan_dfs(s, n, m / r, cost + (nb + 63) / 64);
+    plan_dfs(s, n, m / r, cost + c, model);
     --s.cur_nst;
   }
 }
-__host__ __device__ constexpr FFTPlan make_plan(int n) {
+__host__ __device__ constexpr FFTPlan make_plan(int n, int model = 0) {
   PlanSearch s{1 << 30, 99, 0, {0}, {0}};
-  plan_dfs(s, n, n, 0);
+  plan_dfs(s, n, n, 0, model);
   FFTPlan p{n, s.best_nst, {0}};
   for (int i = 0; i < s.best_nst; ++i) p.r[i] = s.best[i];
   return p;
@@ -267,6 +277,29 @@ __device__ __forceinline__ void dft4(V& a, V& b, V& c, V& d) {
   c = csub(t0, t2);
   b = cadd(t1, it3);
   d = csub(t1, it3);
+}
+// 5-point DFT from its definition, with c1 = cos(2 pi/5), c2 = cos(4 pi/5),
+// s1 = sin(2 pi/5), s2 = sin(4 pi/5): with t1 = x1 + x4, t2 = x2 + x3,
+// t3 = x1 - x4, t4 = x2 - x3, X_1,4 = x0 + c1 t1 + c2 t2 -/+ i (s1 t3 + s2 t4)
+// and X_2,3 = x0 + c2 t1 + c1 t2 -/+ i (s2 t3 - s1 t4) for the forward sign
+template <int DIR, typename V>
+__device__ __forceinline__ void dft5(V& x0, V& x1, V& x2, V& x3, V& x4) {
+  using T = typename Scal<V>::T;
+  const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+  const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
+  const V t1 = cadd(x1, x4), t2 = cadd(x2, x3), t3 = csub(x1, x4), t4 = csub(x2, x3);
+  const V a1 = mkv<V>(x0.x + c1 * t1.x + c2 * t2.x, x0.y + c1 * t1.y + c2 * t2.y);
+  const V a2 = mkv<V>(x0.x + c2 * t1.x + c1 * t2.x, x0.y + c2 * t1.y + c1 * t2.y);
+  const V b1 = mkv<V>(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+  const V b2 = mkv<V>(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+  // i b = (-b.y, b.x); forward: X1 = a1 - i b1
+  const T sg = (DIR < 0) ? (T)1 : (T)-1;
+  const V ib1 = mkv<V>(-sg * b1.y, sg * b1.x), ib2 = mkv<V>(-sg * b2.y, sg * b2.x);
+  x0 = cadd(x0, cadd(t1, t2));
+  x1 = csub(a1, ib1);
+  x4 = cadd(a1, ib1);
+  x2 = csub(a2, ib2);
+  x3 = cadd(a2, ib2);
 }
 // 6-point DFT, natural order: two 3-point DFTs of the even/odd samples
 template <int DIR, typename V>
@@ -316,9 +349,9 @@ template <int N> struct FFTBuf { static constexpr int len = N + N / 16 + 1; };
 // One-wavefront Stockham FFT of length N held in LDS buffer x (scratch y of
 // the same padded size FFTBuf<N>::len, indexed through fpad).  Unnormalised; DIR=-1 is FFTW_FORWARD (exp(-i..)),
 // DIR=+1 is FFTW_BACKWARD.  Returns the buffer holding the result.
-template <int N, int DIR, typename V>
+template <int N, int DIR, typename V, int MODEL = 0>
 __device__ V* wave_fft(V* x, V* y, const V* __restrict__ tw, int lane) {
-  constexpr FFTPlan P = make_plan(N);
+  constexpr FFTPlan P = make_plan(N, MODEL);
   int Ns = 1;
 #pragma unroll
   for (int s = 0; s < P.nst; ++s) {
@@ -350,6 +383,17 @@ __device__ V* wave_fft(V* x, V* y, const V* __restrict__ tw, int lane) {
         dft6<DIR>(v[0], v[1], v[2], v[3], v[4], v[5]);
 #pragma unroll
         for (int m = 0; m < 6; ++m) y[fpad(idxD + m * Ns)] = v[m];
+      } else if (R == 5) {
+        V v[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) v[m] = x[fpad(j + m * nb)];
+        if (Ns > 1) {
+#pragma unroll
+          for (int m = 1; m < 5; ++m) v[m] = cmul(v[m], twid<DIR>(tw, m * k * tstride));
+        }
+        dft5<DIR>(v[0], v[1], v[2], v[3], v[4]);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) y[fpad(idxD + m * Ns)] = v[m];
       } else if (R == 4) {
         V v0 = x[fpad(j)], v1 = x[fpad(j + nb)], v2 = x[fpad(j + 2 * nb)], v3 = x[fpad(j + 3 * nb)];
         if (Ns > 1) {

@@ -111,6 +111,8 @@ struct PlaneArgs {
   // immersed boundary (Goldstein feedback, solvers.f90:2996-3108)
   int nsrc;
   const int4* src;        // (y, z, x, imatrix type) per solid cell
+  const double* srcp;     // [nsrc][6] per-type constants (buffer gains, initial
+                          // profile, suction and wall velocities)
   double* fint;           // [3][nsrc] feedback integrals
   double* drag;           // [3][nsrc]
   double gain, ugain;

@@ -110,6 +110,11 @@ __device__ __forceinline__ void wall_derivs(const PF& f, double dyde, int lane, 
 }
 
 // ----------------------------------------------- Chebyshev transforms --
+// FFT plan cost model of the pencils' length-2NY transforms (make_plan): 1,
+// fewest vector instructions per lane, since one transform per wave leaves
+// every stage at a single round
+constexpr int kChebPlan = 1;
+
 // Inverse (solvers.f90:2241-2546): values at y_k = cos(pi k/NY) of
 // sum_n a_n T_n via a length-2NY FFT of the even extension of c_n/2 * a_n,
 // computed in the transfer precision V and stored to dst[y][pencil].
@@ -127,7 +132,7 @@ __device__ void cheb_to_phys_store(const PF& f, V* buf, V* scr, const V* tw, int
   }
   if (lane == 0) buf[fpad(NY)] = to_v<V>(f.e);
   wave_sync();
-  V* r = wave_fft<M, -1>(buf, scr, tw, lane);
+  V* r = wave_fft<M, -1, V, kChebPlan>(buf, scr, tw, lane);
   if (lane < L) {
     dst[(size_t)(2 * lane) * P + p] = r[fpad(2 * lane)];
     dst[(size_t)(2 * lane + 1) * P + p] = r[fpad(2 * lane + 1)];
@@ -166,7 +171,7 @@ __device__ PF phys_to_cheb(const PhysPencil<V>& q, V* buf, V* scr, const V* tw, 
   }
   if (lane == 0) buf[fpad(NY)] = q.e;
   wave_sync();
-  V* r = wave_fft<M, -1>(buf, scr, tw, lane);
+  V* r = wave_fft<M, -1, V, kChebPlan>(buf, scr, tw, lane);
   const double s = 1.0 / NY;
   PF o;
   if (lane < L) {
@@ -192,7 +197,7 @@ __device__ PF phys_to_cheb_load(const V* __restrict__ src, size_t P, size_t p, V
   }
   if (lane == 0) buf[fpad(NY)] = src[(size_t)NY * P + p];
   wave_sync();
-  V* r = wave_fft<M, -1>(buf, scr, tw, lane);
+  V* r = wave_fft<M, -1, V, kChebPlan>(buf, scr, tw, lane);
   const double s = 1.0 / NY;
   PF o;
   if (lane < L) {
@@ -506,6 +511,7 @@ __device__ __forceinline__ PF smooth_apply(const PF& f, const double* __restrict
     case 32: { constexpr int NY = 32; __VA_ARGS__; } break; \
     case 48: { constexpr int NY = 48; __VA_ARGS__; } break; \
     case 64: { constexpr int NY = 64; __VA_ARGS__; } break; \
+    case 80: { constexpr int NY = 80; __VA_ARGS__; } break; \
     case 96: { constexpr int NY = 96; __VA_ARGS__; } break; \
     case 128: { constexpr int NY = 128; __VA_ARGS__; } break; \
     default: return hipErrorInvalidValue; \

@@ -119,7 +119,12 @@ __global__ __launch_bounds__(256) void k_ibdrag(PlaneArgs A) {
   const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (s >= A.nsrc) return;
   const int4 c = A.src[s];
-  const bool active = (c.w == 1) || (c.w == 4 && A.it <= A.perturbtime);
+  const int t = c.w;
+  const bool active = (t == 1) || (t == 4 && A.it <= A.perturbtime) || t == 3 || t == 6 || t == 7 || t == 8;
+  if (t == 11) {  // constant x-force between the wall and the suction layer
+    if (lane < 3) A.drag[lane * A.nsrc + s] = (lane == 0) ? 1.0 : 0.0;
+    return;
+  }
   if (!active) {
     if (lane < 3) A.drag[lane * A.nsrc + s] = 0.0;
     return;
@@ -135,18 +140,45 @@ __global__ __launch_bounds__(256) void k_ibdrag(PlaneArgs A) {
       if (k == 0) {
         acc += v.x;
       } else {
-        const double2 t = to_d(static_cast<const V*>(A.twX)[(int)(((long long)k * c.z) % MX)]);  // exp(-2 pi i m/MX)
-        acc += 2.0 * (v.x * t.x + v.y * t.y);
+        const double2 tw = to_d(static_cast<const V*>(A.twX)[(int)(((long long)k * c.z) % MX)]);  // exp(-2 pi i m/MX)
+        acc += 2.0 * (v.x * tw.x + v.y * tw.y);
       }
     }
     u3[d] = wave_sum(acc);
   }
   if (lane < 3) {
     const double u = (lane == 0) ? u3[0] : (lane == 1 ? u3[1] : u3[2]);
+    const double* q = A.srcp + (size_t)6 * s;
     double* fi = A.fint + (size_t)lane * A.nsrc + s;
-    const double fnew = *fi + u;
-    *fi = fnew;
-    A.drag[lane * A.nsrc + s] = -A.ugain * u - A.gain * fnew;
+    double* dr = A.drag + (size_t)lane * A.nsrc + s;
+    if (t == 1 || t == 4) {  // solid surface (solvers.f90:2996-3002)
+      const double fnew = *fi + u;
+      *fi = fnew;
+      *dr = -A.ugain * u - A.gain * fnew;
+    } else if (t == 3) {  // spanwise-damping texture (:3004-3006)
+      if (lane == 2) {
+        const double fnew = *fi + u;
+        *fi = fnew;
+        *dr = -A.ugain * u - A.gain * fnew;
+      } else {
+        *dr = 0.0;
+      }
+    } else if (t == 8) {  // spanwise moving wall at wdes (:3008-3021)
+      const double e = (lane == 2) ? u - q[0] : u;
+      const double fnew = *fi + e;
+      *fi = fnew;
+      *dr = -A.ugain * e - A.gain * fnew;
+    } else if (t == 6) {  // buffer zone, force added at the cell (:3023-3031)
+      const double e = u - (lane == 0 ? q[2] : 0.0);
+      const double fnew = *fi + e;
+      *fi = fnew;
+      *dr = -q[0] * e - q[1] * fnew;
+    } else {  // 7, suction region (:3033-3039): v to vdes(i), u to Uinf, w to initw
+      const double e = u - (lane == 0 ? q[1] : (lane == 1 ? q[0] : 0.0));
+      const double fnew = *fi + e;
+      *fi = fnew;
+      *dr = -A.gain * fnew - A.ugain * e;
+    }
   }
 }
 

@@ -295,3 +295,24 @@ def new_run_state(u, v, w, xl, zl):
     w0 = ws[0, 0, :].real.copy()
     vs[0, 0, :] = 0.0
     return dict(v=vs, omy=omy, u0=u0, w0=w0)
+
+
+def ensemble_members(seed, n=16, sizes=(64, 96, 128), nys=(48, 64, 80, 96), rtau=(120.0, 240.0)):
+    """Seeded ensemble of independent channels of uneven size: per member
+    (nx, ny, nz, Re_tau, state seed) with nx = nz drawn from ``sizes``, ny
+    from ``nys`` and Re_tau uniform in ``rtau`` (SplitMix64 of ``seed``).
+    The first members cover the smallest (64 x 49 x 64) and the largest
+    (128 x 97 x 128) grid so that every ensemble spans the whole range."""
+    rng = SplitMix64(seed)
+    out = []
+    for m in range(n):
+        if m == 0:
+            nx, ny = sizes[0], nys[0]
+        elif m == 1:
+            nx, ny = sizes[-1], nys[-1]
+        else:
+            nx = sizes[int(rng.uniform() * len(sizes))]
+            ny = nys[int(rng.uniform() * len(nys))]
+        rt = rtau[0] + (rtau[1] - rtau[0]) * rng.uniform()
+        out.append((nx, ny, nx, rt, int(rng.next_u64() & 0x7FFFFFFF)))
+    return out

@@ -79,6 +79,7 @@ class KmmConfig(C.Structure):
         ("cmf", C.c_int), ("ubulk", C.c_double), ("particle_swirl", C.c_int),
         ("sigmax", C.c_double), ("sigmay", C.c_double), ("sigmaz", C.c_double),
         ("gravity", C.c_double),
+        ("bftail", C.c_int), ("xstart", C.c_double),
     ]
 
 
@@ -217,7 +218,7 @@ def make_config(nx, ny, nz, **kw) -> KmmConfig:
                     particle_flag=0, ratio=0.0, ap=0.0, c_mu=1.0, cd_switch=0,
                     polymer=0, itarget=0, ipeter=1, ipolyflag=1, zlmax=30.0, tpoly=0.1, alpha_poly=0.0,
                     diffpoly=0.5, c11z=1.0, c22z=1.0, c33z=1.0, qbeta=0.9, cmf=0, ubulk=0.0, particle_swirl=1,
-                    sigmax=0.0, sigmay=0.0, sigmaz=0.0, gravity=0.0)
+                    sigmax=0.0, sigmay=0.0, sigmaz=0.0, gravity=0.0, bftail=0, xstart=0.0)
     defaults.update(kw)
     cfg.nx, cfg.ny, cfg.nz = nx, ny, nz
     for k, v in defaults.items():

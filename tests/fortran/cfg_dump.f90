@@ -25,7 +25,8 @@ program cfg_dump
   call r('diffpoly', c%diffpoly); call r('c11z', c%c11z); call r('c22z', c%c22z); call r('c33z', c%c33z)
   call r('qbeta', c%qbeta); call i('cmf', c%cmf); call r('ubulk', c%ubulk)
   call i('particle_swirl', c%particle_swirl); call r('sigmax', c%sigmax); call r('sigmay', c%sigmay)
-  call r('sigmaz', c%sigmaz); call r('gravity', c%gravity)
+  call r('sigmaz', c%sigmaz); call r('gravity', c%gravity); call i('bftail', c%bftail)
+  call r('xstart', c%xstart)
 contains
   subroutine i(name, v)
     character(len=*), intent(in) :: name

@@ -21,6 +21,7 @@ def cfg_for(p, **kw):
                            deltaT=p.deltaT, perturbtime=p.perturbtime, **kw)
 
 
+
 def log(*a):
     print(*a, flush=True)
 
@@ -269,9 +270,11 @@ MIXED_TOL = 1e-4   # FP32 transforms/products, FP64 state and tau solves
 
 
 @pytest.mark.parametrize("grid,precision", [((64, 48, 64), 0), ((64, 48, 64), 1), ((96, 96, 96), 0),
-                                            ((96, 96, 96), 1), ((128, 96, 128), 1)])
+                                            ((96, 96, 96), 1), ((128, 96, 128), 1), ((64, 80, 64), 0),
+                                            ((96, 80, 96), 1), ((128, 80, 128), 0)])
 def test_ensemble_member_sizes_match_oracle(grid, precision):
-    """The pencil templates the ensemble members use (ny = 48 and 96; the
+    """The pencil templates the ensemble members use (ny = 48, 80 and 96; 80
+    runs the Chebyshev transforms as 160-point FFTs with a radix-5 stage; the
     members' planes 96 x 96, 144 x 144, 192 x 192) at a seeded Re_tau in
     [120, 240]: start-up and 3 steps against the oracle, FP64 to 1e-10 and
     mixed within MIXED_TOL of the FP64 oracle."""
@@ -420,3 +423,31 @@ def test_cfl_number_and_stop_match_oracle():
             cfl, it = ch.last_cfl()
         log(f"  dt scale {scale}: stopped at step {it} with cfl {cfl:.12f} (oracle {want:.12f})")
         assert it == fail_it and abs(cfl - want) <= 1e-10 * want
+
+
+def test_ib_all_cell_types_steps():
+    """Every Goldstein cell type of solvers.f90:2996-3043 in one channel:
+    solid 1, spanwise damping 3, perturbation 4 (expiring at perturbtime),
+    buffer zone 6, suction 7, spanwise moving wall 8, constant x-force 11,
+    with forcepre's buffer gains, slope and suction profile (BFTAIL 20,
+    Uinf 1, XSTART 60): start-up and 5 steps against the oracle, the
+    Goldstein integrals included."""
+    p = params(64, 32, 32, gain=20.0, ugain=100.0, dt=0.002, perturbtime=2, bftail=20, uinf=1.0, xstart=60.0)
+    im = np.zeros((p.mx, p.mz, p.nyp), dtype=np.int32)
+    im[20:23, :, 1:4] = 1
+    im[26:28, 4:12, 1:3] = 3
+    im[31:33, 10:20, 1:3] = 4
+    im[5:8, 6:14, 2:5] = 6
+    im[40:43, 20:26, 1:3] = 7
+    im[44:46, 20:26, 25:27] = 7     # vdes is indexed by the y index: nonzero past BFTAIL
+    im[50:53, 8:14, 1:4] = 8
+    im[60, 30:40, 1:3] = 11
+    log("case: IB cell types 1 3 4 6 7 8 11, 64x33x32")
+    sol, ost, ch = run_pair(p, 0, imatrix=im, kw=dict(bftail=20, uinf=1.0, xstart=60.0))
+    for it in range(5):
+        sol.step(ost)
+        ch.step(1)
+        compare(ost, ch, FIELDS_CHECK, TOL, f"step{it + 1}")
+    for k in ("fxintg", "fyintg", "fzintg"):
+        assert rel(ch.get(k), ost[k]) < 1e-10, k
+    ch.close()

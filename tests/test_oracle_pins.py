@@ -279,28 +279,96 @@ def test_scalar_mean_conserved_without_source():
 
 
 # -------------------------------------------------- IB spreading semantics
-def literal_reference_plane(up, vp, wp, imat, vw_assign, gain, ugain, fxi, fyi, fzi, kmax, i1):
+def literal_forcepre(p):
+    """forcepre's forcing constants replayed literally (main_dns.f90:1490-
+    1546), 1-based: bfgain(i), bfugain(i), slopelength, vdes(i), and the
+    channel start initu(i) (init_flow.f90:82-139)."""
+    bfhead = 2
+    bftail = p.bftail
+    bfwidth = bftail - bfhead
+    bfwh = bfwidth // 2
+    bfcntr = bfwh + 1
+    bfgain = [0.0] * 1201
+    bfugain = [0.0] * 1201
+    for i in range(1, bfwidth + 2):
+        ib = i - bfcntr
+        if abs(ib) <= bfwh // 3:
+            bfgain[i] = 20.0 * math.exp(-60.0 * (float(ib) / float(bfwh)) ** 2)
+        bfugain[i] = 20.0 * math.exp(-6.0 * (float(ib) / float(bfwh)) ** 2)
+    slopelength = bfwidth / 2.0
+    delxm = p.xl / float(p.mx - 1)
+    xsuction = delxm * (bftail - 1)
+    vdes = [0.0] * (p.mx + 1)
+    for i in range(1, p.mx + 1):
+        x = delxm * (i - 1)
+        if x <= xsuction:
+            vdes[i] = 0.0
+        else:
+            vdes[i] = (-0.8604 * p.uinf) / math.sqrt(p.re * p.uinf * (x + p.xstart - xsuction))
+    initu = [0.0] * (p.nyp + 1)
+    for i in range(1, p.nyp + 1):
+        d = abs(p.ycoord[i - 1] - p.ycoord[p.nyp - 1])
+        initu[i] = p.re * p.dPdx * (0.5 * d * d - p.yl / 2.0 * d)
+    return bfhead, bfwidth, bfgain, bfugain, slopelength, vdes, initu
+
+
+def literal_reference_plane(up, vp, wp, imat, vw_assign, gain, ugain, fxi, fyi, fzi, kmax, i1, p=None, it=1):
     """The reference's loop (solvers.f90:2891-3108) replayed literally on
     a 1-based (mz, mx) column-major plane: visit k (x) outer, j (z) inner,
-    assign the product, then spread with unwrapped z addressing."""
+    assign the product, apply the cell's branch of the if-chain (types 1/4,
+    3, 8, 6, 7, 11), then spread with unwrapped z addressing."""
     mx, mz = up.shape
     flat = [np.zeros(mz * mx) for _ in range(3)]   # linear address (k-1)*mz + j - 1
     fs = O.fspread_matrix()
+    if p is not None:
+        bfhead, bfwidth, bfgain, bfugain, slopelength, vdes, initu = literal_forcepre(p)
     for k in range(1, mx + 1):
         for j in range(1, mz + 1):
             L = (k - 1) * mz + (j - 1)
             for c in range(3):
                 flat[c][L] = vw_assign[c][k - 1, j - 1]
+            u, v, w = up[k - 1, j - 1], vp[k - 1, j - 1], wp[k - 1, j - 1]
+            t = imat[k - 1, j - 1]
             drag = [0.0, 0.0, 0.0]
-            if imat[k - 1, j - 1] == 1:
-                fxi[k - 1, j - 1] += up[k - 1, j - 1]
-                fyi[k - 1, j - 1] += vp[k - 1, j - 1]
-                fzi[k - 1, j - 1] += wp[k - 1, j - 1]
-                drag = [-ugain * up[k - 1, j - 1] - gain * fxi[k - 1, j - 1],
-                        -ugain * vp[k - 1, j - 1] - gain * fyi[k - 1, j - 1],
-                        -ugain * wp[k - 1, j - 1] - gain * fzi[k - 1, j - 1]]
-            # non-solid points spread exact zeros (a no-op; skipped here so the
-            # replay never addresses past the plane edge)
+            if t == 1 or (t == 4 and p is not None and it <= p.perturbtime):
+                fxi[k - 1, j - 1] += u
+                fyi[k - 1, j - 1] += v
+                fzi[k - 1, j - 1] += w
+                drag = [-ugain * u - gain * fxi[k - 1, j - 1],
+                        -ugain * v - gain * fyi[k - 1, j - 1],
+                        -ugain * w - gain * fzi[k - 1, j - 1]]
+            elif t == 3:
+                fzi[k - 1, j - 1] += w
+                drag[2] = -ugain * w - gain * fzi[k - 1, j - 1]
+            elif t == 8:
+                if k <= bfwidth * 3 + 1:
+                    wdes = (0.3 * p.uinf) / slopelength * (k - (bfwidth * 3 + 1 - slopelength))
+                else:
+                    wdes = 0.3 * p.uinf
+                fxi[k - 1, j - 1] += u
+                fyi[k - 1, j - 1] += v
+                fzi[k - 1, j - 1] += w - wdes
+                drag = [-ugain * u - gain * fxi[k - 1, j - 1], -ugain * v - gain * fyi[k - 1, j - 1],
+                        -ugain * (w - wdes) - gain * fzi[k - 1, j - 1]]
+            elif t == 6:
+                b = k - bfhead + 1
+                fxi[k - 1, j - 1] += u - initu[i1]
+                fyi[k - 1, j - 1] += v
+                fzi[k - 1, j - 1] += w
+                flat[0][L] += -bfugain[b] * (u - initu[i1]) - bfgain[b] * fxi[k - 1, j - 1]
+                flat[1][L] += -bfugain[b] * v - bfgain[b] * fyi[k - 1, j - 1]
+                flat[2][L] += -bfugain[b] * w - bfgain[b] * fzi[k - 1, j - 1]
+            elif t == 7:
+                fyi[k - 1, j - 1] += v - vdes[i1]
+                flat[1][L] += (-gain * fyi[k - 1, j - 1]) + (-ugain * (v - vdes[i1]))
+                fxi[k - 1, j - 1] += u - p.uinf
+                flat[0][L] += (-gain * fxi[k - 1, j - 1]) + (-ugain * (u - p.uinf))
+                fzi[k - 1, j - 1] += w
+                flat[2][L] += (-gain * fzi[k - 1, j - 1]) + (-ugain * w)
+            elif t == 11:
+                drag[0] = 1.0
+            # points with no drag spread exact zeros (a no-op; skipped here so
+            # the replay never addresses past the plane edge)
             if i1 <= kmax and any(d != 0.0 for d in drag):
                 for ii in range(-2, 3):
                     ipii = k + ii
@@ -314,6 +382,38 @@ def literal_reference_plane(up, vp, wp, imat, vw_assign, gain, ugain, fxi, fyi, 
                         for c in range(3):
                             flat[c][T] += seg * drag[c]
     return [f.reshape(mx, mz) for f in flat]
+
+
+def test_ib_all_cell_types_match_literal_reference_loop():
+    """Cell types 1, 3, 4 (active and expired), 6, 7, 8 and 11 in one plane
+    (BFTAIL 20, Uinf 1, XSTART 60) against the literal replay of the
+    reference's if-chain and spreading, forcing constants included."""
+    rng = np.random.default_rng(19)
+    p = params(32, 16, 16, gain=3.0, ugain=7.0, bftail=20, uinf=1.0, xstart=60.0, perturbtime=2)
+    mx, mz = p.mx, p.mz
+    up, vp, wp = (rng.standard_normal((mx, mz)) for _ in range(3))
+    imat = np.zeros((mx, mz), dtype=np.int32)
+    imat[5:8, 3:7] = 1
+    imat[9:11, 4:9] = 3
+    imat[12:14, 2:5] = 4
+    imat[15, 6:10] = 6
+    imat[17:19, 8:12] = 6
+    imat[24:27, 3:6] = 7
+    imat[30:33, 9:12] = 8
+    imat[36, 4:8] = 11
+    assign = [rng.standard_normal((mx, mz)) for _ in range(3)]
+    for it in (1, 3):    # type 4 active, then past perturbtime
+        fx1, fy1, fz1 = (rng.standard_normal((mx, mz)) for _ in range(3))
+        fx2, fy2, fz2 = fx1.copy(), fy1.copy(), fz1.copy()
+        ref = literal_reference_plane(up, vp, wp, imat, assign, 3.0, 7.0, fx1, fy1, fz1, kmax=40, i1=4,
+                                      p=p, it=it)
+        vw = [a.copy() for a in assign]
+        O.apply_ib_plane(3, up, vp, wp, vw, None, fx2, fy2, fz2, imat, it, p, 3.0, 7.0, 40)
+        for a, b in zip(vw, ref):
+            assert np.abs(a - b).max() < 1e-12
+        for a, b in ((fx1, fx2), (fy1, fy2), (fz1, fz2)):
+            assert np.abs(a - b).max() < 1e-13
+    assert O.kmaxsurf(imat[:, :, None]) == 14   # types 1, 3, 4 only (main_dns.f90:1526-1535)
 
 
 def test_ib_spreading_matches_literal_reference_loop():
