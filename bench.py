@@ -69,7 +69,9 @@ def workload_state():
 # seeded 1e-3 noise, 100 000 tracers
 PNX, PNY, PNZ = 192, 128, 192
 WI_TAU, QBETA, ZLMAX, SC = 50.0, 0.9, 30.0, 0.5
-AUDIT_EVERY, AUDIT_WINDOW, AUDIT_GATE = 25, 3, 0.1  # closure audits in the learned-closure line
+AUDIT_EVERY, AUDIT_WINDOW, AUDIT_GATE = 50, 3, 0.02  # closure audits: 3 FENE-P steps every 50, 2 % gate
+RETRAIN_STEPS, RETRAIN_BATCH, RETRAIN_LR = 200, 1024, 3e-4  # Adam steps after each passed audit window
+PRE_STEPS = 8    # FENE-P steps before the warm-up (the learned line draws its training pairs there)
 NPART = 100_000
 AP = 1e-3   # tracer radius: keeps the wall clamp of tracer_update inside the grid
 
@@ -99,7 +101,7 @@ def polymer_mode(kmm, precision, steps, warmup, learned=False):
         # training pairs drawn by the GPU from its own FENE-P steps, GPU training,
         # then the network replaces the conformation transport
         ch.set_poly_capture(True)
-        ch.step(2)
+        ch.step(PRE_STEPS - 6)
         Xs, Ts = [], []
         for s in range(6):
             ch.step(1)
@@ -115,17 +117,23 @@ def polymer_mode(kmm, precision, steps, warmup, learned=False):
                       [round(float(losses[:10].mean()), 4), round(float(losses[-10:].mean()), 4)],
                       "samples": int(X.shape[0])})
         ch.set_closure(net)
-        # audit windows of full FENE-P transport (3 steps after every 25
-        # closure steps) run inside the timed region; a residual above 10 %
-        # falls back to FENE-P
+        # audit windows of full FENE-P transport (3 steps after every 50
+        # closure steps) run inside the timed region; a residual above 2 %
+        # falls back to FENE-P; each passed window retrains the network on
+        # pairs drawn from its FENE-P steps
         ch.set_closure_audit(AUDIT_EVERY, AUDIT_WINDOW, AUDIT_GATE)
+        ch.set_closure_retrain(RETRAIN_STEPS, RETRAIN_BATCH, RETRAIN_LR, seed=SEED + 7)
+    else:
+        ch.step(PRE_STEPS)
     ch.step(warmup)
     ch.set_timing(2)
     t0 = time.perf_counter()
     ch.step(steps)
     t = time.perf_counter() - t0
+    stats = ch.statistics()     # profiles after PRE_STEPS + warmup + steps steps
     if learned:
-        extra["audit"] = dict(ch.closure_status(), every=AUDIT_EVERY, window=AUDIT_WINDOW, gate=AUDIT_GATE)
+        extra["audit"] = dict(ch.closure_status(), every=AUDIT_EVERY, window=AUDIT_WINDOW, gate=AUDIT_GATE,
+                              retrain=[RETRAIN_STEPS, RETRAIN_BATCH, RETRAIN_LR])
     ch.set_timing(True)
     ch.step(steps)
     ktimes, _ = ch.kernel_times()
@@ -136,6 +144,8 @@ def polymer_mode(kmm, precision, steps, warmup, learned=False):
     out = {"value": round(G * steps / t / 1e6, 2), "ms_per_step": round(1000.0 * t / steps, 4),
            "kernel_ms": {k: round(v, 4) for k, v in ktimes.items()}}
     out.update(extra)
+    out["_stats"] = stats
+    out["at_step"] = PRE_STEPS + warmup + steps
     return out
 
 
@@ -205,21 +215,36 @@ def measured_traffic(mode, kernel, tune):
 
 
 def cpu_baseline(nsteps=20):
+    """The oracle's FP64 step (numpy; its x/z/y transforms on scipy.fft with
+    one worker per core, up to the 16 cores a GPU's share of the host
+    holds; the elementwise passes and transposes stay on one thread) timed on
+    a bounded number of steps of the same workload."""
     from threadpoolctl import threadpool_limits
     from oracle import kmm_oracle as O
+    cores = max(1, min(16, os.cpu_count() or 1))
+    O.WORKERS = cores
     p = O.Params(nx=NX, ny=NY, nz=NZ, xl=XL, zl=ZL, re=RE, dt=DT, R_tau=RTAU, gain=GAIN, ugain=UGAIN,
                  scalar=True, diff=PR, scl_flag=6, deltaT=SRC)
     st = workload_state()
     sol = O.Solver(p, imatrix=rib(NX, NY, NZ))
     sol.startup(st)
-    with threadpool_limits(1):   # one core: numpy's BLAS pools would otherwise spread
+    with threadpool_limits(1):   # BLAS pools stay single-threaded (the transforms use the workers)
         t0 = time.perf_counter()
         for _ in range(nsteps):
             sol.step(st)
         t = time.perf_counter() - t0
+    O.WORKERS = 1
     G = NX * (NY + 1) * NZ
-    return dict(value=G * nsteps / t / 1e6, unit="Mgridpt-steps/s", cores=1, kind="port",
-                sample=f"{nsteps} FP64 steps of the same workload (numpy restatement, single thread)")
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    return dict(value=G * nsteps / t / 1e6, unit="Mgridpt-steps/s", cores=cores, kind="port",
+                nproc=os.cpu_count(), cpu_model=model,
+                sample=f"{nsteps} FP64 steps of the same workload (numpy restatement; scipy.fft transforms on "
+                       f"{cores} workers, elementwise passes on one thread)")
 
 
 def run_mode(kmm, precision, steps, warmup):
@@ -253,8 +278,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--poly-steps", type=int, default=56,
-                    help="steps of the viscoelastic workload (0: skip); 56 = two closure/audit periods")
+    ap.add_argument("--poly-steps", type=int, default=200,
+                    help="steps of the viscoelastic workload (0: skip); 200 = four closure/audit periods")
     ap.add_argument("--ensemble-steps", type=int, default=100,
                     help="steps of the 16-member uneven ensemble (0: skip)")
     args = ap.parse_args()
@@ -296,6 +321,10 @@ def main():
         pf = polymer_mode(kmm, 0, args.poly_steps, 5)
         pm = polymer_mode(kmm, 1, args.poly_steps, 5)
         pl = polymer_mode(kmm, 1, args.poly_steps, 5, learned=True)
+        a, b = pl.pop("_stats"), pm.pop("_stats")
+        pf.pop("_stats")
+        pl["profile_error_vs_fenep_mixed"] = {k: round(float(np.abs(a[k] - b[k]).max() / np.abs(b[k]).max()), 6)
+                                              for k in ("umean", "urms", "rss", "pss")}
         out["polymer"] = {"workload": f"viscoelastic channel {PNX}x{PNY + 1}x{PNZ}, FENE-P Wi_tau {WI_TAU:g}, "
                                       f"beta {QBETA}, L^2 {ZLMAX ** 2:g}, Sc {SC}, {NPART} tracers, "
                                       "conformation started at the mean-shear equilibrium",

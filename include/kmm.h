@@ -369,6 +369,11 @@ int kmm_get_poly_stress(kmm_ctx* ctx, int comp, double* out);
  * it the run stays on FENE-P (fallback) until kmm_set_closure is called again.
  * every = 0 turns audits off.  Needs the captures (kmm_set_poly_capture). */
 int kmm_set_closure_audit(kmm_ctx* ctx, int every, int window, double gate);
+/* Retraining of the closure at the end of every passed audit window: `steps`
+ * Adam steps of `batch` pairs, learning rate lr, drawn (seeded) from the
+ * window's FENE-P evaluations; steps = 0 turns it off.  No reference
+ * counterpart. */
+int kmm_set_closure_retrain(kmm_ctx* ctx, int steps, int batch, float lr, unsigned long long seed);
 /* active: 1 while the network carries the stress; audits: completed windows;
  * last_residual: the last window's residual (-1 before the first);
  * fallback_it: the first FENE-P step after a failed audit, -1 if none */

@@ -50,6 +50,10 @@ import math
 import numpy as np
 import scipy.fft as sfft
 
+# threads of the x/z/y transforms (scipy.fft workers); the CPU baseline sets
+# it to the host's core count, the tests leave it at 1
+WORKERS = 1
+
 
 # --------------------------------------------------------------------------- #
 #  Parameters (dns.config semantics, FFTW_DNS/code/src/dns/main_dns.f90:829-1171)
@@ -631,15 +635,15 @@ def to_physical(a, p: Params):
     x, all unnormalised."""
     fac = p.c / 2.0
     ay = a * fac
-    ayr = sfft.dct(ay.real, type=1, axis=-1)
-    ayi = sfft.dct(ay.imag, type=1, axis=-1)
+    ayr = sfft.dct(ay.real, type=1, axis=-1, workers=WORKERS)
+    ayi = sfft.dct(ay.imag, type=1, axis=-1, workers=WORKERS)
     ay = ayr + 1j * ayi                                  # (nxh, nz, nyp), physical y
     pad = np.zeros((p.nyp, p.mz, p.mx // 2 + 1), dtype=np.complex128)
     t = np.transpose(ay, (2, 1, 0))                      # (nyp, nz, nxh)
     pad[:, :p.nzh, :p.nxh] = t[:, :p.nzh, :]
     pad[:, p.mz - p.nzh:, :p.nxh] = t[:, p.nzh:, :]
-    zt = np.fft.ifft(pad, axis=1) * p.mz                 # BACKWARD along z
-    phys = np.fft.irfft(zt, n=p.mx, axis=2) * p.mx       # c2r along x
+    zt = sfft.ifft(pad, axis=1, workers=WORKERS) * p.mz                 # BACKWARD along z
+    phys = sfft.irfft(zt, n=p.mx, axis=2, workers=WORKERS) * p.mx       # c2r along x
     return np.ascontiguousarray(np.transpose(phys, (2, 1, 0)))   # (mx, mz, nyp)
 
 
@@ -648,15 +652,15 @@ def to_spectral(ph, p: Params):
     FORWARD c2c along z, keep nxh x nz modes, / (mx mz); DCT-I, / ny,
     halve n=0 and n=ny."""
     t = np.transpose(ph, (2, 1, 0))                      # (nyp, mz, mx)
-    xt = np.fft.rfft(t, axis=2)
-    zt = np.fft.fft(xt, axis=1)
+    xt = sfft.rfft(t, axis=2, workers=WORKERS)
+    zt = sfft.fft(xt, axis=1, workers=WORKERS)
     sp = np.empty((p.nyp, p.nz, p.nxh), dtype=np.complex128)
     sp[:, :p.nzh, :] = zt[:, :p.nzh, :p.nxh]
     sp[:, p.nzh:, :] = zt[:, p.mz - p.nzh:, :p.nxh]
     sp /= float(p.mx * p.mz)
     sp = np.transpose(sp, (2, 1, 0))                     # (nxh, nz, nyp)
-    r = sfft.dct(sp.real, type=1, axis=-1)
-    i = sfft.dct(sp.imag, type=1, axis=-1)
+    r = sfft.dct(sp.real, type=1, axis=-1, workers=WORKERS)
+    i = sfft.dct(sp.imag, type=1, axis=-1, workers=WORKERS)
     out = (r + 1j * i) / float(p.ny)
     out[..., 0] /= 2.0
     out[..., -1] /= 2.0

@@ -142,6 +142,12 @@ struct kmm_ctx {
   void* conf_nlp[6] = {};                         // its x-z transforms (physical y)
   double* audit_part = nullptr;                   // residual partial sums
   double* audit_acc = nullptr;                    // [2] residual sums of the running window
+  // retraining at the end of each passed audit window on pairs drawn from the
+  // window's FENE-P evaluations (kmm_set_closure_retrain)
+  int retrain_steps = 0, retrain_batch = 0;
+  float retrain_lr = 0.0f;
+  unsigned long long retrain_seed = 0;
+  float *rX = nullptr, *rT = nullptr;
   // per-routine slab solves (kmm_solve/kmm_penta): one kx slab and its tables,
   // allocated at the first call and kept until kmm_destroy
   void* slab = nullptr;
@@ -151,6 +157,8 @@ struct kmm_ctx {
 };
 
 static void drop_graph(kmm_ctx* c);
+int kmm_mlp_train_device(kmm_mlp* n, int npts, const float* dX, const float* dT, int steps, int batch, float lr,
+                         hipStream_t s);  // kmm_mlp_api.cpp
 
 static int dalloc(kmm_ctx* c, void** p, size_t bytes) {
   KMM_HIP(hipMalloc(p, bytes > 0 ? bytes : 16));
@@ -1270,6 +1278,27 @@ extern "C" int kmm_startup(kmm_ctx* c) {
   return r ? r : check_particles(c);
 }
 
+// Adam steps of the closure on training pairs drawn from the captures of
+// the window's last evaluations (FENE-P stresses at it, it-1; it-2 from the
+// window's opening step), queued on the channel's stream.
+static int retrain_closure(kmm_ctx* c) {
+  const int n = c->retrain_steps * c->retrain_batch;
+  kmm::MlpFieldIO io;
+  io.P = (size_t)c->nyp * c->mz * c->mx;
+  io.plane_pts = c->mz * c->mx;
+  io.half = 0.5 * c->cfg.yl;
+  io.ycoord = c->ycoord_d;
+  io.g = c->capg;
+  io.h1 = c->hist[hslot(c->it - 1)];
+  io.h2 = c->hist[hslot(c->it - 2)];
+  io.out = nullptr;
+  hipError_t e = kmm::launch_mlp_sample(io, c->hist[hslot(c->it)], n, c->retrain_seed + (unsigned long long)c->it,
+                                        c->rX, c->rT, c->stream);
+  if (e != hipSuccess) return fail(KMM_E_HIP, std::string("closure retraining samples: ") + hipGetErrorString(e));
+  return kmm_mlp_train_device(c->closure, n, c->rX, c->rT, c->retrain_steps, c->retrain_batch, c->retrain_lr,
+                              c->stream);
+}
+
 // Launch one time step on the library stream (no synchronisation).  s is the
 // step's slot in the CFL buffer; ev (or nullptr) receives per-kernel events;
 // stage_ev records the first-step stage events of the untimed path.
@@ -1341,6 +1370,7 @@ static int enqueue_step(kmm_ctx* c, int s, hipEvent_t* ev, bool stage_ev) {
       c->audit_w = -1;
       if (c->audit_last <= c->audit_gate) {
         c->closure_it0 = c->it + 1;  // the closure resumes from the FENE-P stress history
+        if (c->retrain_steps > 0 && (r = retrain_closure(c))) return r;
       } else {
         c->closure_active = false;   // fallback: the transport carries on
         c->fallback_it = c->it + 1;
@@ -2461,6 +2491,23 @@ extern "C" int kmm_set_closure_audit(kmm_ctx* c, int every, int window, double g
   c->audit_window = window;
   c->audit_gate = gate;
   drop_graph(c);
+  return KMM_OK;
+}
+
+extern "C" int kmm_set_closure_retrain(kmm_ctx* c, int steps, int batch, float lr, unsigned long long seed) {
+  if (!c || steps < 0 || (steps > 0 && (batch <= 0 || !(lr > 0.0f))))
+    return fail(KMM_E_ARG, "retrain: steps >= 0; batch > 0 and lr > 0 when steps > 0");
+  KMM_HIP(hipSetDevice(c->device));
+  const size_t n = (size_t)steps * batch;
+  if (steps > 0 && n > (size_t)(c->retrain_steps) * c->retrain_batch) {
+    int r;
+    if ((r = dalloc(c, (void**)&c->rX, n * 22 * sizeof(float)))) return r;
+    if ((r = dalloc(c, (void**)&c->rT, n * 6 * sizeof(float)))) return r;
+  }
+  c->retrain_steps = steps;
+  c->retrain_batch = batch;
+  c->retrain_lr = lr;
+  c->retrain_seed = seed;
   return KMM_OK;
 }
 

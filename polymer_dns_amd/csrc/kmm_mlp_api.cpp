@@ -216,6 +216,30 @@ extern "C" int kmm_mlp_train(kmm_mlp* n, int npts, const float* X, const float* 
   return KMM_OK;
 }
 
+// Adam steps on samples already on the device (dX [npts][22], dT [npts][6]),
+// queued on stream s without host synchronisation: step k uses the k-th
+// consecutive block of `batch` samples (cycling), so the samples must come
+// in random order (kmm_poly_samples' sampler draws them so).  Used by the
+// channel to retrain the closure on pairs drawn inside audit windows.
+int kmm_mlp_train_device(kmm_mlp* n, int npts, const float* dX, const float* dT, int steps, int batch, float lr,
+                         hipStream_t s) {
+  if (!n || npts <= 0 || !dX || !dT || steps <= 0 || batch <= 0 || batch > npts || !(lr > 0.0f))
+    return mfail(KMM_E_ARG, "bad argument");
+  const int nb = kmm::mlp_fb_blocks(batch);
+  int r;
+  if ((r = grow(&n->gpart, &n->capG, (size_t)nb * n->np))) return r;
+  if (n->lpart) hipFree(n->lpart);
+  MHIP(hipMalloc((void**)&n->lpart, (size_t)nb * sizeof(float)));
+  const int nblocks = npts / batch;
+  for (int k = 0; k < steps; ++k) {
+    const size_t off = (size_t)(k % nblocks) * batch;
+    MHIP(kmm::launch_mlp_grad(batch, dX + off * 22, dT + off * 6, n->par, n->norm, 1, n->gpart, n->lpart, n->g,
+                              n->loss, s));
+    MHIP(kmm::launch_mlp_adam(n->par, n->m, n->v, n->g, lr, ++n->adam_t, n->wb, s));
+  }
+  return KMM_OK;
+}
+
 bool kmm_mlp_view(kmm_mlp* m, const void** wb, const float** par, const float** norm, int* device) {
   if (!m) return false;
   *wb = m->wb;

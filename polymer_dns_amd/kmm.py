@@ -52,6 +52,7 @@ EXPORTS = [
     "kmm_flowfield", "kmm_write_flowfield",
     "kmm_phirhs", "kmm_nonlin", "kmm_veloc", "kmm_vort", "kmm_gradscl", "kmm_derivscji", "kmm_polyforce",
     "kmm_subforce", "kmm_polyNL", "kmm_norm", "kmm_uzero",
+    "kmm_set_closure_retrain",
 ]
 
 
@@ -167,6 +168,7 @@ def load_library(path: str | None = None):
     lib.kmm_set_closure.argtypes = [C.c_void_p, C.c_void_p]
     lib.kmm_get_poly_stress.argtypes = [C.c_void_p, C.c_int, dp]
     lib.kmm_set_closure_audit.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double]
+    lib.kmm_set_closure_retrain.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_ulonglong]
     lib.kmm_closure_status.argtypes = [C.c_void_p, ip, ip, dp, ip]
     lib.kmm_poly_conf_from_stress.argtypes = [C.c_void_p]
     lib.kmm_autotune.argtypes = [C.c_void_p, C.c_int, dp]
@@ -605,6 +607,11 @@ class Channel:
         the network's residual over a window gates the closure (above `gate`
         the run falls back to FENE-P)."""
         _check(self.lib, self.lib.kmm_set_closure_audit(self.h, int(every), int(window), float(gate)))
+
+    def set_closure_retrain(self, steps, batch=1024, lr=3e-4, seed=0):
+        """Adam steps on pairs drawn from each passed audit window's FENE-P
+        evaluations before the closure resumes (0: off)."""
+        _check(self.lib, self.lib.kmm_set_closure_retrain(self.h, int(steps), int(batch), float(lr), int(seed)))
 
     def closure_status(self):
         a, n, f = C.c_int(), C.c_int(), C.c_int()
