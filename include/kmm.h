@@ -200,6 +200,16 @@ int kmm_cderiv(kmm_ctx* ctx, const double* f, double* df);
 int kmm_solve(kmm_ctx* ctx, double* u, double x, double g, double dyde, const double* bctop,
               const double* bcbot, int ib);
 
+/* kmm_solve without allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, hosts that forbid hipMalloc).  Same arguments,
+ * kernels and results as kmm_solve, with every device buffer taken from a
+ * fixed array each GPU holds until the process ends (about 5.3 MB): slabs
+ * up to nz <= 1024 and nyp*nz <= 131072 complex values; larger slabs return
+ * KMM_E_ARG.  Calls that overlap on one GPU share the array and race. */
+int kmm_solve_noalloc(kmm_ctx* ctx, double* u, double x, double g, double dyde, const double* bctop,
+                      const double* bcbot, int ib);
+
 /* penta(u,x,g,dyde,ib,at,bt,gtop,ab,bb,gbot,a,wrkc), solvers.f90:680 */
 int kmm_penta(kmm_ctx* ctx, double* u, double x, double g, double dyde, int ib, double at, double bt,
               const double* gtop, double ab, double bb, const double* gbot);

@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -2051,6 +2052,82 @@ extern "C" int kmm_uzero(kmm_ctx* c, double* u0, const double* h1, const double*
   if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
   KMM_HIP(hipMemcpyAsync(u0, a.m3, c->nyp * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   KMM_HIP(hipStreamSynchronize(c->stream));
+  return KMM_OK;
+}
+
+// ---------------------------------------------- slab solve, no allocation --
+// kmm_solve_noalloc runs kmm_solve's kernels with every device buffer carved
+// from `scratch`, a fixed array each GPU holds until the process ends: two
+// slab copies (reference column-major order and the kernels' [y][kz] order),
+// the tau table and the per-kz constants.  Nothing is allocated on the device
+// or the heap; the host stages only through stack arrays.  Every call fully
+// writes what it reads back, so no part of the array needs clearing; calls
+// that overlap on one GPU share the array and race.
+namespace {
+constexpr size_t kNaSlab = 131072;  // max nyp * nz complex elements of one slab
+constexpr int kNaNz = 1024;         // max nz
+constexpr size_t kNaBytes = 2 * kNaSlab * sizeof(double2) + (size_t)kNaNz * (65 + 2) * sizeof(double2) +
+                            (size_t)kNaNz * (sizeof(double) + sizeof(int));
+constexpr int kNaDevices = 64;
+std::mutex g_na_mu;
+void* g_na_ptr[kNaDevices] = {};    // device address of `scratch`, per device
+}  // namespace
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+// out[c * R + r] = in[r * C + c]: between the reference's column-major slab
+// (R = nz rows of nyp) and the kernels' [y][kz] order (R = nyp rows of nz)
+__global__ void k_slab_transpose(const double2* __restrict__ in, double2* __restrict__ out, int R, int C) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= R * C) return;
+  const int r = t / C, col = t - r * C;
+  out[(size_t)col * R + r] = in[t];
+}
+
+extern "C" int kmm_solve_noalloc(kmm_ctx* c, double* u, double x, double g, double dyde, const double* bctop,
+                                 const double* bcbot, int ib) {
+  if (!c || !u || !bctop || !bcbot) return fail(KMM_E_ARG, "null argument");
+  const int nz = c->nz, nyp = c->nyp;
+  if (nz > kNaNz || (size_t)nyp * nz > kNaSlab)
+    return fail(KMM_E_ARG, "kmm_solve_noalloc: slab larger than its fixed scratch (nz <= 1024, nyp*nz <= 131072)");
+  if (c->device < 0 || c->device >= kNaDevices) return fail(KMM_E_ARG, "kmm_solve_noalloc: device index >= 64");
+  KMM_HIP(hipSetDevice(c->device));
+  unsigned char* base;
+  {
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[c->device]) {
+      void* p = nullptr;
+      KMM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      g_na_ptr[c->device] = p;
+    }
+    base = static_cast<unsigned char*>(g_na_ptr[c->device]);
+  }
+  double2* dcol = reinterpret_cast<double2*>(base);           // reference order
+  double2* du = dcol + kNaSlab;                               // [y][kz]
+  double2* dtab = du + kNaSlab;
+  double2* dbt = dtab + (size_t)kNaNz * 65;
+  double2* dbb = dbt + kNaNz;
+  double* dwn = reinterpret_cast<double*>(dbb + kNaNz);
+  int* dkz = reinterpret_cast<int*>(dwn + kNaNz);
+  double wn[kNaNz];
+  int kz[kNaNz];
+  const bool anyk = slab_wn(c, x, g, ib, wn, kz);
+  const size_t nu = (size_t)nyp * nz;
+  const int nb = (int)((nu + 255) / 256);
+  KMM_HIP(hipMemcpyAsync(dcol, u, nu * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  KMM_HIP(hipMemcpyAsync(dbt, bctop, nz * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  KMM_HIP(hipMemcpyAsync(dbb, bcbot, nz * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  KMM_HIP(hipMemcpyAsync(dwn, wn, nz * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  KMM_HIP(hipMemcpyAsync(dkz, kz, nz * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  k_slab_transpose<<<nb, 256, 0, c->stream>>>(dcol, du, nz, nyp);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = slab_launch(c, du, dtab, dbt, dbb, dwn, dkz, anyk, dyde, ib, false, 0, 0, 0, 0);
+  if (e == hipSuccess) {
+    k_slab_transpose<<<nb, 256, 0, c->stream>>>(du, dcol, nyp, nz);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(u, dcol, nu * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
   return KMM_OK;
 }
 

@@ -51,7 +51,7 @@ EXPORTS = [
     "kmm_autotune", "kmm_set_tuning", "kmm_get_tuning", "kmm_statistics", "kmm_write_outputs",
     "kmm_flowfield", "kmm_write_flowfield",
     "kmm_phirhs", "kmm_nonlin", "kmm_veloc", "kmm_vort", "kmm_gradscl", "kmm_derivscji", "kmm_polyforce",
-    "kmm_subforce", "kmm_polyNL", "kmm_norm", "kmm_uzero",
+    "kmm_subforce", "kmm_polyNL", "kmm_norm", "kmm_uzero", "kmm_solve_noalloc",
     "kmm_set_closure_retrain",
 ]
 
@@ -122,6 +122,7 @@ def load_library(path: str | None = None):
         getattr(lib, n).argtypes = [C.c_void_p, C.c_char_p]
     lib.kmm_cderiv.argtypes = [C.c_void_p, dp, dp]
     lib.kmm_solve.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, dp, dp, C.c_int]
+    lib.kmm_solve_noalloc.argtypes = lib.kmm_solve.argtypes
     lib.kmm_penta.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_int,
                               C.c_double, C.c_double, dp, C.c_double, C.c_double, dp]
     pp = C.POINTER(dp)
