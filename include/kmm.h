@@ -227,6 +227,19 @@ int kmm_vcw3dp(kmm_ctx* ctx, const double* u, const double* v, const double* w, 
                const double* scalar, const double* sclx, const double* scly, const double* sclz, double* scn,
                int it);
 
+/* vcw3dp with the POLYMER argument list (solvers.f90:1726-1740): as
+ * kmm_vcw3dp, plus cij[9] = c11 c12 c13 c21 c22 c23 c31 c32 c33 and dc[27]
+ * (derivscji's dummy order, see kmm_derivscji) in; cn[6], strn[6], qp[6]
+ * (components 11 12 13 22 23 33: c_ij^n, str_ij^n, qp_ij) out, for it >=
+ * src_start - 1 as in the reference (otherwise they are left as they are).
+ * uij (u11 ... u33) is required: u12, u22, u32 enter the products.  C is
+ * taken symmetric (c21 = c12, ...), as the step carries it. */
+int kmm_vcw3dp_poly(kmm_ctx* ctx, const double* u, const double* v, const double* w, const double* omx,
+                    const double* omy, double* omz, double* fn, double* gn, const double* const* uij,
+                    const double* scalar, const double* sclx, const double* scly, const double* sclz, double* scn,
+                    const double* const* cij, const double* const* dc, double* const* cn, double* const* strn,
+                    double* const* qp, int it);
+
 /* Spectral helper procedures on whole fields (complex (nyp,nz,nxh) each,
  * the reference's layout), with the reference's argument meaning.  Work
  * arrays of the reference (wrkc, wrk1) are internal. */
@@ -347,6 +360,9 @@ int kmm_mlp_infer(kmm_mlp* mlp, int n, const float* X, float* Y);
  * gradient; mode 0: FP32, mode 1: bf16 rounding as in inference */
 int kmm_mlp_grad(kmm_mlp* mlp, int n, const float* X, const float* T, int mode, float* grad, float* loss);
 /* Adam (lr, beta 0.9/0.999) on seeded mini-batches; losses[steps] optional */
+/* hidden-layer activation: 0 ReLU (default), 1 GELU as x sigma(1.702 x)
+ * (applied in FP32 before the bf16 rounding of the next layer's input) */
+int kmm_mlp_set_activation(kmm_mlp* mlp, int act);
 int kmm_mlp_train(kmm_mlp* mlp, int n, const float* X, const float* T, int steps, int batch, float lr,
                   unsigned long long seed, float* losses);
 

@@ -1,13 +1,13 @@
 """CPU restatement (numpy) of the learned polymer-stress closure: a 22-input,
-4 x 128 ReLU, 6-output multilayer perceptron (no reference counterpart; the
+4 x 128 ReLU or GELU, 6-output multilayer perceptron (no reference counterpart; the
 network is new here).
 
 TEST INFRASTRUCTURE ONLY: only tests/ may import this module.
 
 Conventions shared with polymer_dns_amd/csrc/kmm_mlp.hip:
   * parameters: W[l] (n_out x n_in, row-major), b[l] (n_out); layer sizes
-    22 -> 128 -> 128 -> 128 -> 128 -> 6; ReLU after the four hidden layers,
-    none after the last;
+    22 -> 128 -> 128 -> 128 -> 128 -> 6; ReLU (or GELU, module ACT) after
+    the four hidden layers, none after the last;
   * input and output affine normalisation: x_n = (x - mu_x)/sd_x,
     y = y_n sd_y + mu_y;
   * "bf16" mode rounds weights and the input of every layer to bfloat16
@@ -38,14 +38,28 @@ def bf16(x):
     return np.where(np.isfinite(a), out, a)
 
 
-def act(x):
-    """ReLU."""
+GELU_A = 1.702   # GELU in the sigmoid form x sigma(1.702 x) (Hendrycks & Gimpel 2016, section 2)
+ACT = "relu"     # hidden-layer activation of the functions below: "relu" or "gelu"
+
+
+def act(x, kind=None):
+    """ReLU, or GELU as x sigma(1.702 x)."""
+    kind = kind or ACT
+    if kind == "gelu":
+        x = np.asarray(x)
+        return x / (1.0 + np.exp(-GELU_A * x))
     return np.maximum(x, 0.0)
 
 
-def act_grad(x):
-    """d act / dx (0 at the kink, as the GPU training kernel takes it)."""
-    return (x > 0).astype(np.asarray(x).dtype)
+def act_grad(x, kind=None):
+    """d act / dx (ReLU: 0 at the kink, as the GPU training kernel takes it;
+    GELU: sigma + 1.702 x sigma (1 - sigma))."""
+    kind = kind or ACT
+    x = np.asarray(x)
+    if kind == "gelu":
+        s = 1.0 / (1.0 + np.exp(-GELU_A * x))
+        return s + GELU_A * x * s * (1.0 - s)
+    return (x > 0).astype(x.dtype)
 
 
 def init_params(seed):

@@ -160,6 +160,8 @@ struct kmm_ctx {
 static void drop_graph(kmm_ctx* c);
 int kmm_mlp_train_device(kmm_mlp* n, int npts, const float* dX, const float* dT, int steps, int batch, float lr,
                          hipStream_t s);  // kmm_mlp_api.cpp
+int kmm_mlp_activation(const kmm_mlp* m);  // kmm_mlp_api.cpp
+int kmm_mlp_last_loss(kmm_mlp* m, float* loss, hipStream_t s);  // kmm_mlp_api.cpp
 
 static int dalloc(kmm_ctx* c, void** p, size_t bytes) {
   KMM_HIP(hipMalloc(p, bytes > 0 ? bytes : 16));
@@ -895,7 +897,7 @@ static int run_closure_plane(kmm_ctx* c, int it, bool swirl, double* mrf) {
   io.h1 = c->hist[hslot(it - 1)];
   io.h2 = c->hist[hslot(it - 2)];
   io.out = c->hist[hslot(it)];
-  if ((e = kmm::launch_mlp_field(io, wb, par, nrm, c->stream)) != hipSuccess)
+  if ((e = kmm::launch_mlp_field(io, wb, par, nrm, kmm_mlp_activation(c->closure), c->stream)) != hipSuccess)
     return fail(KMM_E_HIP, std::string("closure network: ") + hipGetErrorString(e));
   sa.caps = c->hist[hslot(it)];
   for (int q = 0; q < 12; ++q) sa.out[q] = c->NLP[q];
@@ -971,7 +973,7 @@ static int audit_compare(kmm_ctx* c, int it) {
   io.h1 = c->hist[hslot(it - 1)];
   io.h2 = c->hist[hslot(it - 2)];
   io.out = c->audit_pred;
-  hipError_t e = kmm::launch_mlp_field(io, wb, par, nrm, c->stream);
+  hipError_t e = kmm::launch_mlp_field(io, wb, par, nrm, kmm_mlp_activation(c->closure), c->stream);
   if (e == hipSuccess)
     e = kmm::launch_audit_residual(c->audit_pred, c->hist[hslot(it)], 6 * io.P, c->audit_part, c->audit_acc,
                                    c->stream);
@@ -1367,11 +1369,22 @@ static int enqueue_step(kmm_ctx* c, int s, hipEvent_t* ev, bool stage_ev) {
       KMM_HIP(hipMemcpyAsync(acc, c->audit_acc, sizeof(acc), hipMemcpyDeviceToHost, c->stream));
       KMM_HIP(hipStreamSynchronize(c->stream));
       c->audit_last = acc[1] > 0.0 ? std::sqrt(acc[0] / acc[1]) : (acc[0] > 0.0 ? INFINITY : 0.0);
+      // a non-finite state or prediction fails the gate
+      if (!(std::isfinite(acc[0]) && std::isfinite(acc[1]))) c->audit_last = INFINITY;
       c->audits += 1;
       c->audit_w = -1;
       if (c->audit_last <= c->audit_gate) {
         c->closure_it0 = c->it + 1;  // the closure resumes from the FENE-P stress history
-        if (c->retrain_steps > 0 && (r = retrain_closure(c))) return r;
+        if (c->retrain_steps > 0) {
+          if ((r = retrain_closure(c))) return r;
+          // a diverged retraining (non-finite loss) ends the closure: FENE-P carries on
+          float loss = 0.0f;
+          if ((r = kmm_mlp_last_loss(c->closure, &loss, c->stream))) return r;
+          if (!std::isfinite(loss)) {
+            c->closure_active = false;
+            c->fallback_it = c->it + 1;
+          }
+        }
       } else {
         c->closure_active = false;   // fallback: the transport carries on
         c->fallback_it = c->it + 1;
@@ -2131,12 +2144,20 @@ extern "C" int kmm_solve_noalloc(kmm_ctx* c, double* u, double x, double g, doub
   return KMM_OK;
 }
 
-extern "C" int kmm_vcw3dp(kmm_ctx* c, const double* u, const double* v, const double* w, const double* omx,
-                          const double* omy, double* omz, double* fn, double* gn, const double* const* uij,
-                          const double* scalar, const double* sclx, const double* scly, const double* sclz,
-                          double* scn, int it) {
-  (void)uij;     // the velocity gradients feed calcswirl/part_track, which kmm_step runs itself
-  (void)scalar;  // the scalar enters only through its gradient
+// vcw3dp's body for both argument lists: NS (+ scalar) inputs, and with
+// poly set the conformation C (six independent components), its y
+// derivatives and the velocity's y derivatives for the FENE-P products.
+struct Vcw3dpPoly {
+  const double* c[6];    // C11 C12 C13 C22 C23 C33 (spectral)
+  const double* dyc[6];  // their y derivatives (dc112, dc122, dc132, dc222, dc232, dc332)
+  const double* dyu[3];  // du/dy, dv/dy, dw/dy (u12, u22, u32)
+  double* cn[6];         // outputs: c_n, str_n, qp of the six components
+  double* strn[6];
+  double* qp[6];
+};
+static int vcw3dp_body(kmm_ctx* c, const double* u, const double* v, const double* w, const double* omx,
+                       const double* omy, double* omz, double* fn, double* gn, const double* sclx,
+                       const double* scly, const double* sclz, double* scn, int it, const Vcw3dpPoly* poly) {
   double* omz_out = omz;
   if (!c || !u || !v || !w || !omx || !omy || !omz || !gn || !fn) return fail(KMM_E_ARG, "null argument");
   const bool sc = c->cfg.scalar != 0;
@@ -2145,25 +2166,83 @@ extern "C" int kmm_vcw3dp(kmm_ctx* c, const double* u, const double* v, const do
   const double* ins[9] = {u, v, w, omx, omy, omz, sclx, scly, sclz};
   const int nin = sc ? 9 : 6;
   // spectral inputs -> NL buffers (scratch) -> Chebyshev-to-physical -> T
-  for (int f = 0; f < nin; ++f) {
-    int r = upload_field(c, static_cast<double2*>(c->ZF), ins[f]);
+  auto to_transfer = [&](const double* src, void* dst) -> int {
+    int r = upload_field(c, static_cast<double2*>(c->ZF), src);
     if (r) return r;
-    hipError_t e = kmm::launch_cheb(c->ny, c->P, c->twM, c->ZF, c->T[f], 1, c->cfg.precision, c->stream);
-    if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
+    hipError_t e = kmm::launch_cheb(c->ny, c->P, c->twM, c->ZF, dst, 1, c->cfg.precision, c->stream);
+    return e == hipSuccess ? KMM_OK : fail(KMM_E_HIP, hipGetErrorString(e));
+  };
+  int r;
+  for (int f = 0; f < nin; ++f)
+    if ((r = to_transfer(ins[f], c->T[f]))) return r;
+  // the FENE-P products are evaluated for it >= src_start - 1 (solvers.f90:2901)
+  const bool pterms = poly && it >= c->cfg.src_start - 1;
+  if (pterms) {
+    for (int q = 0; q < 6; ++q) {
+      if ((r = to_transfer(poly->c[q], c->TP[q]))) return r;
+      if ((r = to_transfer(poly->dyc[q], c->TP[6 + q]))) return r;
+    }
+    for (int q = 0; q < 3; ++q)
+      if ((r = to_transfer(poly->dyu[q], c->TPD[q]))) return r;
   }
   KMM_HIP(hipMemsetAsync(c->cfl_d, 0, sizeof(unsigned long long), c->stream));
-  int r = run_plane(c, it, c->cfl_d);
+  r = run_plane(c, it, c->cfl_d, nullptr, false, pterms, false);
   if (r) return r;
+  auto from_nl = [&](const void* src, double* dst) -> int {
+    hipError_t e = kmm::launch_cheb(c->ny, c->P, c->twM, src, c->ZF, 0, c->cfg.precision, c->stream);
+    if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
+    return download_field(c, static_cast<double2*>(c->ZF), dst);
+  };
   double* outs[4] = {gn, fn, omz_out, scn};
   const int nout = sc ? 4 : 3;
-  for (int o = 0; o < nout; ++o) {
-    hipError_t e = kmm::launch_cheb(c->ny, c->P, c->twM, c->NL[o], c->ZF, 0, c->cfg.precision, c->stream);
-    if (e != hipSuccess) return fail(KMM_E_HIP, hipGetErrorString(e));
-    r = download_field(c, static_cast<double2*>(c->ZF), outs[o]);
-    if (r) return r;
+  for (int o = 0; o < nout; ++o)
+    if ((r = from_nl(c->NL[o], outs[o]))) return r;
+  if (pterms) {
+    // qp = (1 - beta)/(re beta tpoly) str with beta = qbeta (itarget 0)
+    const double zb = (1.0 - c->cfg.qbeta) / (c->cfg.re * c->cfg.qbeta * c->cfg.tpoly);
+    for (int q = 0; q < 6; ++q) {
+      if ((r = from_nl(c->NLP[q], poly->cn[q]))) return r;
+      if ((r = from_nl(c->NLP[6 + q], poly->strn[q]))) return r;
+      for (size_t e = 0; e < 2 * c->S; ++e) poly->qp[q][e] = zb * poly->strn[q][e];
+    }
   }
   c->nl_valid = false;
   return check_cfl(c, 1, it);
+}
+
+extern "C" int kmm_vcw3dp(kmm_ctx* c, const double* u, const double* v, const double* w, const double* omx,
+                          const double* omy, double* omz, double* fn, double* gn, const double* const* uij,
+                          const double* scalar, const double* sclx, const double* scly, const double* sclz,
+                          double* scn, int it) {
+  (void)uij;     // the velocity gradients feed calcswirl/part_track, which kmm_step runs itself
+  (void)scalar;  // the scalar enters only through its gradient
+  return vcw3dp_body(c, u, v, w, omx, omy, omz, fn, gn, sclx, scly, sclz, scn, it, nullptr);
+}
+
+extern "C" int kmm_vcw3dp_poly(kmm_ctx* c, const double* u, const double* v, const double* w, const double* omx,
+                               const double* omy, double* omz, double* fn, double* gn, const double* const* uij,
+                               const double* scalar, const double* sclx, const double* scly, const double* sclz,
+                               double* scn, const double* const* cij, const double* const* dc, double* const* cn,
+                               double* const* strn, double* const* qp, int it) {
+  (void)scalar;
+  if (!c || !uij || !cij || !dc || !cn || !strn || !qp) return fail(KMM_E_ARG, "null argument");
+  if (!c->cfg.polymer) return fail(KMM_E_CONFIG, "the POLYMER argument list needs polymer = 1");
+  if (c->cfg.itarget != 0) return fail(KMM_E_CONFIG, "only itarget 0 is built");
+  Vcw3dpPoly pl;
+  // independent components 11 12 13 22 23 33 of the nine (c21 = c12, ...)
+  static const int ci[6] = {0, 1, 2, 4, 5, 8};
+  for (int q = 0; q < 6; ++q) {
+    const int i = ci[q] / 3, j = ci[q] % 3;   // C_(i+1)(j+1)
+    pl.c[q] = cij[ci[q]];
+    pl.dyc[q] = dc[9 * j + 3 * i + 1];       // dc_ij2 in derivscji's dummy order
+    pl.cn[q] = cn[q];
+    pl.strn[q] = strn[q];
+    pl.qp[q] = qp[q];
+    if (!pl.c[q] || !pl.dyc[q] || !pl.cn[q] || !pl.strn[q] || !pl.qp[q]) return fail(KMM_E_ARG, "null argument");
+  }
+  pl.dyu[0] = uij[1]; pl.dyu[1] = uij[4]; pl.dyu[2] = uij[7];   // u12, u22, u32
+  if (!pl.dyu[0] || !pl.dyu[1] || !pl.dyu[2]) return fail(KMM_E_ARG, "u12, u22, u32 required");
+  return vcw3dp_body(c, u, v, w, omx, omy, omz, fn, gn, sclx, scly, sclz, scn, it, &pl);
 }
 
 // --------------------------------------------------------- particles ----

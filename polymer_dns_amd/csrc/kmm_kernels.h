@@ -174,7 +174,8 @@ struct MlpFieldIO {
   const float* h2;
   float* out;
 };
-hipError_t launch_mlp_field(const MlpFieldIO& io, const void* wb, const float* par, const float* norm,
+// act: 0 ReLU, 1 GELU (sigmoid form)
+hipError_t launch_mlp_field(const MlpFieldIO& io, const void* wb, const float* par, const float* norm, int act,
                             hipStream_t s);
 // training samples from captured fields: for i < n, point p_i = hash(seed, i)
 // mod P; X[i] = features of p_i, T[i] = t - 2 h1 + h2 at p_i
@@ -277,8 +278,9 @@ int mlp_param_count();
 int mlp_wb_count();
 int mlp_fb_blocks(int n);
 hipError_t launch_mlp_infer(int n, const float* X, float* Y, const void* wb, const float* par, const float* norm,
-                            hipStream_t s);
+                            int act, hipStream_t s);
 hipError_t launch_mlp_pack(const float* par, void* wb, hipStream_t s);
+// mode bit 0: bf16 rounding as in inference; bit 1: GELU (else ReLU)
 hipError_t launch_mlp_grad(int n, const float* X, const float* T, const float* par, const float* norm, int mode,
                            float* gpart, float* lpart, float* g, float* loss, hipStream_t s);
 hipError_t launch_mlp_adam(float* par, float* m, float* v, const float* g, float lr, int t, void* wb,

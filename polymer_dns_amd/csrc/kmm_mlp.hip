@@ -69,6 +69,21 @@ constexpr int kBias = 4 * kH + 16;
 // the activation's instruction count sets a large part of inference time.
 __device__ __forceinline__ float act_f(float x) { return fmaxf(x, 0.0f); }
 __device__ __forceinline__ float act_df(float x) { return x > 0.0f ? 1.0f : 0.0f; }
+// GELU in the sigmoid form x sigma(1.702 x) (Hendrycks and Gimpel, "Gaussian
+// Error Linear Units", 2016, section 2; within 0.02 of x Phi(x)): one
+// exponential, one reciprocal, three multiply-adds per activation
+constexpr float kGeluA = 1.702f;
+__device__ __forceinline__ float gelu_sig(float x) {
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-kGeluA * x));
+}
+__device__ __forceinline__ float gelu_f(float x) { return x * gelu_sig(x); }
+__device__ __forceinline__ float gelu_df(float x) {
+  const float sg = gelu_sig(x);
+  return sg + kGeluA * x * sg * (1.0f - sg);
+}
+// activation a (0 ReLU, 1 GELU) and its derivative
+__device__ __forceinline__ float act_a(int a, float x) { return a ? gelu_f(x) : act_f(x); }
+__device__ __forceinline__ float act_da(int a, float x) { return a ? gelu_df(x) : act_df(x); }
 // round to bfloat16 (nearest even) and back
 __device__ __forceinline__ float bf16r(float x) {
   unsigned u = __float_as_uint(x);
@@ -131,10 +146,17 @@ __device__ __forceinline__ float field_feature(const MlpFieldIO& io, size_t p, i
   return io.h1[(size_t)(k - 16) * io.P + p] - io.h2[(size_t)(k - 16) * io.P + p];
 }
 
-// ReLU of two FP32 values, rounded to a packed bf16 pair
-__device__ __forceinline__ unsigned relu_pk(float a, float b) {
-  const bf16x2 p = __builtin_convertvector(f32x2{a, b}, bf16x2);  // one v_cvt_pk_bf16_f32
-  return __builtin_bit_cast(unsigned, __builtin_elementwise_max(__builtin_bit_cast(i16x2, p), i16x2{0, 0}));
+// activation of two FP32 values, rounded to a packed bf16 pair: ReLU after
+// the rounding (ACT 0), GELU before it (ACT 1)
+template <int ACT>
+__device__ __forceinline__ unsigned act_pk(float a, float b) {
+  if constexpr (ACT == 1) {
+    const bf16x2 p = __builtin_convertvector(f32x2{gelu_f(a), gelu_f(b)}, bf16x2);
+    return __builtin_bit_cast(unsigned, p);
+  } else {
+    const bf16x2 p = __builtin_convertvector(f32x2{a, b}, bf16x2);  // one v_cvt_pk_bf16_f32
+    return __builtin_bit_cast(unsigned, __builtin_elementwise_max(__builtin_bit_cast(i16x2, p), i16x2{0, 0}));
+  }
 }
 
 __device__ __forceinline__ u32x4 pack8(const float (&v)[8]) {
@@ -152,7 +174,7 @@ __device__ __forceinline__ u32x4 pack8(const float (&v)[8]) {
 // Software pipelined: tile nt's products are interleaved with the activation
 // of tile nt-1 (matrix and vector pipes busy together) and with the weight
 // reads of tile nt+1.
-template <int KS>
+template <int KS, int ACT>
 __device__ __forceinline__ void mlp_layer(const u32x4 (&in)[KS][kT], u32x4 (&out)[4][kT], const __bf16* W, int wr,
                                           const float* bias, int r16, int kq) {
   const __bf16* wl = W + r16 * wr + kq * 8;  // this lane's A row in tile 0
@@ -184,8 +206,8 @@ __device__ __forceinline__ void mlp_layer(const u32x4 (&in)[KS][kT], u32x4 (&out
       const int m = nt - 1;
 #pragma unroll
       for (int t = 0; t < kT; ++t) {
-        out[m >> 1][t][2 * (m & 1)] = relu_pk(acc[m & 1][t][0], acc[m & 1][t][1]);
-        out[m >> 1][t][2 * (m & 1) + 1] = relu_pk(acc[m & 1][t][2], acc[m & 1][t][3]);
+        out[m >> 1][t][2 * (m & 1)] = act_pk<ACT>(acc[m & 1][t][0], acc[m & 1][t][1]);
+        out[m >> 1][t][2 * (m & 1) + 1] = act_pk<ACT>(acc[m & 1][t][2], acc[m & 1][t][3]);
       }
     }
     if (nt > 0 && nt < 8) {
@@ -204,7 +226,7 @@ __device__ __forceinline__ void mlp_layer(const u32x4 (&in)[KS][kT], u32x4 (&out
 
 // FIELD = false: X[n][22] -> Y[n][6]; FIELD = true: io fields -> io.out[6][P].
 // Persistent: wavefront w takes passes w, w + (all wavefronts), ...
-template <bool FIELD>
+template <bool FIELD, int ACT>
 __global__ __launch_bounds__(kInferThreads) void k_mlp_infer(int n, const float* __restrict__ X,
                                                              float* __restrict__ Y, MlpFieldIO io,
                                                              const __bf16* __restrict__ wb,
@@ -304,10 +326,10 @@ __global__ __launch_bounds__(kInferThreads) void k_mlp_infer(int n, const float*
       x0[0][t] = pack8(v);
     }
     u32x4 a1[4][kT], a2[4][kT];
-    mlp_layer<1>(x0, a1, sw + wb_off(0), wrow(0), sb, r16, kq);
-    mlp_layer<4>(a1, a2, sw + wb_off(1), wrow(1), sb + 128, r16, kq);
-    mlp_layer<4>(a2, a1, sw + wb_off(2), wrow(2), sb + 256, r16, kq);
-    mlp_layer<4>(a1, a2, sw + wb_off(3), wrow(3), sb + 384, r16, kq);
+    mlp_layer<1, ACT>(x0, a1, sw + wb_off(0), wrow(0), sb, r16, kq);
+    mlp_layer<4, ACT>(a1, a2, sw + wb_off(1), wrow(1), sb + 128, r16, kq);
+    mlp_layer<4, ACT>(a2, a1, sw + wb_off(2), wrow(2), sb + 256, r16, kq);
+    mlp_layer<4, ACT>(a1, a2, sw + wb_off(3), wrow(3), sb + 384, r16, kq);
     // output layer: one neuron tile (rows 0-5 used)
     f32x4 acc[kT];
     {
@@ -355,37 +377,43 @@ static int device_cus() {
   return cus;
 }
 
-template <bool FIELD>
-static hipError_t mlp_infer_launch(int n, const float* X, float* Y, const MlpFieldIO& io, const void* wb,
-                                   const float* par, const float* norm, hipStream_t s) {
+template <bool FIELD, int ACT>
+static hipError_t mlp_infer_launch_a(int n, const float* X, float* Y, const MlpFieldIO& io, const void* wb,
+                                     const float* par, const float* norm, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const int passes = (n + kChunk - 1) / kChunk;
   static const int cus = device_cus();
   const int blocks = std::min((passes + kInferThreads / 64 - 1) / (kInferThreads / 64), cus);
   static bool attr = false;  // > 64 KB of dynamic LDS (160 KB per CU on gfx950)
   if (!attr) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_mlp_infer<FIELD>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)mlp_infer_lds());
+    hipError_t e = hipFuncSetAttribute((const void*)k_mlp_infer<FIELD, ACT>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)mlp_infer_lds());
     if (e != hipSuccess) return e;
     attr = true;
   }
-  hipLaunchKernelGGL((k_mlp_infer<FIELD>), dim3(blocks), dim3(kInferThreads), mlp_infer_lds(), s, n, X, Y, io,
+  hipLaunchKernelGGL((k_mlp_infer<FIELD, ACT>), dim3(blocks), dim3(kInferThreads), mlp_infer_lds(), s, n, X, Y, io,
                      static_cast<const __bf16*>(wb), par, norm);
   return hipGetLastError();
 }
-
-hipError_t launch_mlp_infer(int n, const float* X, float* Y, const void* wb, const float* par, const float* norm,
-                            hipStream_t s) {
-  MlpFieldIO io;
-  std::memset(&io, 0, sizeof(io));
-  return mlp_infer_launch<false>(n, X, Y, io, wb, par, norm, s);
+template <bool FIELD>
+static hipError_t mlp_infer_launch(int n, const float* X, float* Y, const MlpFieldIO& io, const void* wb,
+                                   const float* par, const float* norm, int act, hipStream_t s) {
+  return act ? mlp_infer_launch_a<FIELD, 1>(n, X, Y, io, wb, par, norm, s)
+             : mlp_infer_launch_a<FIELD, 0>(n, X, Y, io, wb, par, norm, s);
 }
 
-hipError_t launch_mlp_field(const MlpFieldIO& io, const void* wb, const float* par, const float* norm,
+hipError_t launch_mlp_infer(int n, const float* X, float* Y, const void* wb, const float* par, const float* norm,
+                            int act, hipStream_t s) {
+  MlpFieldIO io;
+  std::memset(&io, 0, sizeof(io));
+  return mlp_infer_launch<false>(n, X, Y, io, wb, par, norm, act, s);
+}
+
+hipError_t launch_mlp_field(const MlpFieldIO& io, const void* wb, const float* par, const float* norm, int act,
                             hipStream_t s) {
   // 32-bit point indices inside the kernel
   if (io.P >= (size_t)0x7fffffff || io.plane_pts <= 0) return hipErrorInvalidValue;
-  return mlp_infer_launch<true>((int)io.P, nullptr, nullptr, io, wb, par, norm, s);
+  return mlp_infer_launch<true>((int)io.P, nullptr, nullptr, io, wb, par, norm, act, s);
 }
 
 // SplitMix64 of (seed, i): point index of sample i
@@ -435,14 +463,15 @@ __global__ __launch_bounds__(256) void k_mlp_fb(int n, const float* __restrict__
   const float* sd_x = norm + kIn;
   const float* mu_y = norm + 2 * kIn;
   const float* sd_y = norm + 2 * kIn + kOut;
-  const bool rb = mode == 1;
+  const bool rb = (mode & 1) != 0;   // bf16 rounding where inference rounds
+  const int act = (mode >> 1) & 1;    // 0 ReLU, 1 GELU
   auto Wv = [&](int l, int j, int k) -> float {
     const float w = par[w_off(l) + j * lay_in(l) + k];
     return rb ? bf16r(w) : w;
   };
   // layer input value a_l[s][k] (layer l's input): a0 or act(z_{l-1}), bf16-rounded in mode 1
   auto Ain = [&](int l, int s, int k) -> float {
-    const float v = (l == 0) ? a0[s][k] : act_f(z[l - 1][s][k]);
+    const float v = (l == 0) ? a0[s][k] : act_a(act, z[l - 1][s][k]);
     return rb ? bf16r(v) : v;
   };
   for (int e = tid; e < kBS * kIn; e += 256) {
@@ -505,7 +534,7 @@ __global__ __launch_bounds__(256) void k_mlp_fb(int n, const float* __restrict__
       const int s = e / K, k = e % K;
       float acc = 0.0f;
       for (int j = 0; j < N; ++j) acc += d[cur][s][j] * Wv(l, j, k);
-      d[cur ^ 1][s][k] = acc * act_df(z[l - 1][s][k]);
+      d[cur ^ 1][s][k] = acc * act_da(act, z[l - 1][s][k]);
     }
     __syncthreads();
     cur ^= 1;

@@ -23,6 +23,7 @@ struct kmm_mlp {
   float* norm = nullptr;            // mu_x[22], sd_x[22], mu_y[6], sd_y[6]
   void* wb = nullptr;               // padded bf16 weights
   int adam_t = 0;
+  int act = 0;                      // hidden-layer activation: 0 ReLU, 1 GELU
   // scratch (grown on demand)
   float *X = nullptr, *T = nullptr, *Y = nullptr, *gpart = nullptr, *lpart = nullptr, *loss = nullptr;
   size_t capX = 0, capT = 0, capY = 0, capG = 0;
@@ -145,7 +146,7 @@ extern "C" int kmm_mlp_infer(kmm_mlp* n, int npts, const float* X, float* Y) {
   int r;
   if ((r = grow(&n->X, &n->capX, (size_t)npts * 22)) || (r = grow(&n->Y, &n->capY, (size_t)npts * 6))) return r;
   MHIP(hipMemcpyAsync(n->X, X, (size_t)npts * 22 * sizeof(float), hipMemcpyHostToDevice, n->stream));
-  MHIP(kmm::launch_mlp_infer(npts, n->X, n->Y, n->wb, n->par, n->norm, n->stream));
+  MHIP(kmm::launch_mlp_infer(npts, n->X, n->Y, n->wb, n->par, n->norm, n->act, n->stream));
   MHIP(hipMemcpyAsync(Y, n->Y, (size_t)npts * 6 * sizeof(float), hipMemcpyDeviceToHost, n->stream));
   MHIP(hipStreamSynchronize(n->stream));
   return KMM_OK;
@@ -166,7 +167,8 @@ extern "C" int kmm_mlp_grad(kmm_mlp* n, int npts, const float* X, const float* T
   MHIP(hipMalloc((void**)&n->lpart, (size_t)nb * sizeof(float)));
   MHIP(hipMemcpyAsync(n->X, X, (size_t)npts * 22 * sizeof(float), hipMemcpyHostToDevice, n->stream));
   MHIP(hipMemcpyAsync(n->T, T, (size_t)npts * 6 * sizeof(float), hipMemcpyHostToDevice, n->stream));
-  MHIP(kmm::launch_mlp_grad(npts, n->X, n->T, n->par, n->norm, mode, n->gpart, n->lpart, n->g, n->loss, n->stream));
+  MHIP(kmm::launch_mlp_grad(npts, n->X, n->T, n->par, n->norm, mode | (n->act << 1), n->gpart, n->lpart, n->g,
+                            n->loss, n->stream));
   MHIP(hipMemcpyAsync(grad, n->g, (size_t)n->np * sizeof(float), hipMemcpyDeviceToHost, n->stream));
   MHIP(hipMemcpyAsync(loss, n->loss, sizeof(float), hipMemcpyDeviceToHost, n->stream));
   MHIP(hipStreamSynchronize(n->stream));
@@ -207,7 +209,8 @@ extern "C" int kmm_mlp_train(kmm_mlp* n, int npts, const float* X, const float* 
     pos += batch;
     MHIP(hipMemcpyAsync(n->X, bx.data(), bx.size() * sizeof(float), hipMemcpyHostToDevice, n->stream));
     MHIP(hipMemcpyAsync(n->T, bt.data(), bt.size() * sizeof(float), hipMemcpyHostToDevice, n->stream));
-    MHIP(kmm::launch_mlp_grad(batch, n->X, n->T, n->par, n->norm, 1, n->gpart, n->lpart, n->g, n->loss, n->stream));
+    MHIP(kmm::launch_mlp_grad(batch, n->X, n->T, n->par, n->norm, 1 | (n->act << 1), n->gpart, n->lpart, n->g,
+                              n->loss, n->stream));
     MHIP(kmm::launch_mlp_adam(n->par, n->m, n->v, n->g, lr, ++n->adam_t, n->wb, n->stream));
     if (losses) MHIP(hipMemcpyAsync(losses + s, n->loss, sizeof(float), hipMemcpyDeviceToHost, n->stream));
     // the host buffers are reused by the next batch
@@ -233,10 +236,26 @@ int kmm_mlp_train_device(kmm_mlp* n, int npts, const float* dX, const float* dT,
   const int nblocks = npts / batch;
   for (int k = 0; k < steps; ++k) {
     const size_t off = (size_t)(k % nblocks) * batch;
-    MHIP(kmm::launch_mlp_grad(batch, dX + off * 22, dT + off * 6, n->par, n->norm, 1, n->gpart, n->lpart, n->g,
+    MHIP(kmm::launch_mlp_grad(batch, dX + off * 22, dT + off * 6, n->par, n->norm, 1 | (n->act << 1), n->gpart,
+                              n->lpart, n->g,
                               n->loss, s));
     MHIP(kmm::launch_mlp_adam(n->par, n->m, n->v, n->g, lr, ++n->adam_t, n->wb, s));
   }
+  return KMM_OK;
+}
+
+extern "C" int kmm_mlp_set_activation(kmm_mlp* n, int act) {
+  if (!n || (act != 0 && act != 1)) return mfail(KMM_E_ARG, "activation: 0 (ReLU) or 1 (GELU)");
+  n->act = act;
+  return KMM_OK;
+}
+
+int kmm_mlp_activation(const kmm_mlp* m) { return m ? m->act : 0; }
+
+// loss of the last Adam step queued on stream s (waits for it)
+int kmm_mlp_last_loss(kmm_mlp* m, float* loss, hipStream_t s) {
+  MHIP(hipMemcpyAsync(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost, s));
+  MHIP(hipStreamSynchronize(s));
   return KMM_OK;
 }
 

@@ -20,6 +20,12 @@
 
 namespace kmm {
 
+// FFT plan cost model of the x and z transforms (make_plan): 1, fewest
+// vector instructions per lane.  Measured against fewest rounds at 128^3
+// FP64: k_zinv -11 %, k_zfwd -9 %, k_xplane -3 % (more LDS passes, fewer
+// idle lanes in the radix-8 stages).
+constexpr int kPlanePlan = 1;
+
 // one wave's line of nz spectral values, NJ per lane
 template <int NJ, typename V>
 __device__ __forceinline__ void zinv_line_load(V (&pre)[NJ], const V* __restrict__ src, int nz, int lane) {
@@ -102,7 +108,7 @@ __global__ __launch_bounds__(512) void k_zinv(PlaneArgs A) {
     wave_sync();
     // every wave's result sits at the same offset (buf or scr) of its own
     // pair of buffers: the block stores the group's z rows from there
-    const int roff = (int)(wave_fft<MZ, +1>(buf, scr, tw, lane) - buf);
+    const int roff = (int)(wave_fft<MZ, +1, V, kPlanePlan>(buf, scr, tw, lane) - buf);
     __syncthreads();
     V* dst = static_cast<V*>(A.Z) + ((size_t)f * A.nyp + y) * MZ * A.nxh;
     for (int e = threadIdx.x; e < MZ * 8; e += 512) {
@@ -245,7 +251,7 @@ void k_xplane(PlaneArgs A) {
     }
     if (f + 1 < NFIN) rows(f + 1);
     wave_sync();
-    V* r = wave_fft<MX, +1>(buf, scr, tw, lane);
+    V* r = wave_fft<MX, +1, V, kPlanePlan>(buf, scr, tw, lane);
 #pragma unroll
     for (int m = 0; m < XPL; ++m) {
       const int x = lane + 64 * m;
@@ -345,7 +351,7 @@ void k_xplane(PlaneArgs A) {
       if (x < MX) buf[fpad(x)] = ph[o][m];
     }
     wave_sync();
-    V* r = wave_fft<MX, -1>(buf, scr, tw, lane);
+    V* r = wave_fft<MX, -1, V, kPlanePlan>(buf, scr, tw, lane);
     V* zo = static_cast<V*>(A.ZF) + ((size_t)o * A.nyp + y) * A.mz * nxh;
     for (int k = lane; k < nxh; k += 64) {
       const V zk = r[fpad(k)], zm = r[fpad((MX - k) % MX)];
@@ -430,7 +436,7 @@ __global__ __launch_bounds__(512) void k_zfwd(PlaneArgs A) {
       const V* sp = src(it + gridDim.x, k0);
       zfwd_tile_load<MZ, TL, V, T>(prx, pry, sp, A.nxh, k0);
     }
-    V* r = wave_fft<MZ, -1>(buf, scr, tw, lane);
+    V* r = wave_fft<MZ, -1, V, kPlanePlan>(buf, scr, tw, lane);
     V* o2 = (r == buf) ? scr : buf;
     for (int j = lane; j < nz; j += 64) {
       const int jj = (j < nzh) ? j : (MZ - nz + j);
@@ -508,7 +514,7 @@ __device__ __forceinline__ void xrow_pair_fwd(const typename Scal<V>::T (&a)[(MX
     if (x < MX) buf[fpad(x)] = mkv<V>(a[m], b[m]);
   }
   wave_sync();
-  V* r = wave_fft<MX, -1>(buf, scr, tw, lane);
+  V* r = wave_fft<MX, -1, V, kPlanePlan>(buf, scr, tw, lane);
   for (int k = lane; k < nxh; k += 64) {
     const V zk = r[fpad(k)], zm = r[fpad((MX - k) % MX)];
     oa[k] = mkv<V>((T)0.5 * (zk.x + zm.x), (T)0.5 * (zk.y - zm.y));
@@ -561,7 +567,7 @@ __device__ __forceinline__ void pair_inv(const PairIn<MX, V>& r, bool dxa, bool 
     }
   }
   wave_sync();
-  V* res = wave_fft<MX, +1>(buf, scr, tw, lane);
+  V* res = wave_fft<MX, +1, V, kPlanePlan>(buf, scr, tw, lane);
 #pragma unroll
   for (int m = 0; m < XPL; ++m) {
     const int x = lane + 64 * m;

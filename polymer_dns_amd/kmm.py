@@ -52,7 +52,7 @@ EXPORTS = [
     "kmm_flowfield", "kmm_write_flowfield",
     "kmm_phirhs", "kmm_nonlin", "kmm_veloc", "kmm_vort", "kmm_gradscl", "kmm_derivscji", "kmm_polyforce",
     "kmm_subforce", "kmm_polyNL", "kmm_norm", "kmm_uzero", "kmm_solve_noalloc",
-    "kmm_set_closure_retrain",
+    "kmm_set_closure_retrain", "kmm_vcw3dp_poly", "kmm_mlp_set_activation",
 ]
 
 
@@ -127,6 +127,7 @@ def load_library(path: str | None = None):
                               C.c_double, C.c_double, dp, C.c_double, C.c_double, dp]
     pp = C.POINTER(dp)
     lib.kmm_vcw3dp.argtypes = [C.c_void_p] + [dp] * 8 + [pp] + [dp] * 5 + [C.c_int]
+    lib.kmm_vcw3dp_poly.argtypes = [C.c_void_p] + [dp] * 8 + [pp] + [dp] * 5 + [pp] * 5 + [C.c_int]
     lib.kmm_phirhs.argtypes = [C.c_void_p, dp, dp, dp, dp, dp, C.c_double]
     lib.kmm_nonlin.argtypes = [C.c_void_p, dp, dp, dp]
     lib.kmm_veloc.argtypes = [C.c_void_p] + [dp] * 6
@@ -169,6 +170,7 @@ def load_library(path: str | None = None):
     lib.kmm_set_closure.argtypes = [C.c_void_p, C.c_void_p]
     lib.kmm_get_poly_stress.argtypes = [C.c_void_p, C.c_int, dp]
     lib.kmm_set_closure_audit.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double]
+    lib.kmm_mlp_set_activation.argtypes = [C.c_void_p, C.c_int]
     lib.kmm_set_closure_retrain.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_ulonglong]
     lib.kmm_closure_status.argtypes = [C.c_void_p, ip, ip, dp, ip]
     lib.kmm_poly_conf_from_stress.argtypes = [C.c_void_p]
@@ -465,6 +467,24 @@ class Channel:
         outs = [gn, fn, oz] + ([scn] if sc else [])
         return outs  # gn, fn, omz (, scn)
 
+    def vcw3dp_poly(self, u, v, w, omx, omy, omz, uij, cij, dc, it=1):
+        """vcw3dp with the POLYMER argument list: uij 9 gradients (u11..u33),
+        cij 9 components (c11..c33), dc 27 derivatives in derivscji's dummy
+        order.  Returns (gn, fn, omz, {ij: c_n}, {ij: str_n}, {ij: qp})."""
+        Q = ("11", "12", "13", "22", "23", "33")
+        ins = [self._c(a) for a in (u, v, w, omx, omy)]
+        oz = self._c(omz).copy()
+        fn, gn = np.empty_like(oz), np.empty_like(oz)
+        U = [self._c(a) for a in uij]
+        Cs = [self._c(a) for a in cij]
+        D = [self._c(a) for a in dc]
+        outs = {k: [np.empty_like(oz) for _ in Q] for k in ("cn", "str", "qp")}
+        arr = lambda lst: (C.POINTER(C.c_double) * len(lst))(*[_dp(a) for a in lst])
+        _check(self.lib, self.lib.kmm_vcw3dp_poly(self.h, *[_dp(a) for a in ins], _dp(oz), _dp(fn), _dp(gn), arr(U),
+                                                  None, None, None, None, None, arr(Cs), arr(D), arr(outs["cn"]),
+                                                  arr(outs["str"]), arr(outs["qp"]), int(it)))
+        return (gn, fn, oz) + tuple(dict(zip(Q, outs[k])) for k in ("cn", "str", "qp"))
+
     # ---- Lagrangian tracers (part_track.f90) ----
     def set_particles(self, x, y, z):
         """xpart, ypart, zpart; velocities are taken from the fluid at the
@@ -696,6 +716,10 @@ class Mlp:
             b.append(p[o:o + no])
             o += no
         return W, b
+
+    def set_activation(self, act):
+        """Hidden-layer activation: "relu" (default) or "gelu" (x sigma(1.702 x))."""
+        _check(self.lib, self.lib.kmm_mlp_set_activation(self.h, {"relu": 0, "gelu": 1}[act]))
 
     def set_norm(self, mu_x, sd_x, mu_y, sd_y):
         a = [np.ascontiguousarray(x, dtype=np.float32) for x in (mu_x, sd_x, mu_y, sd_y)]

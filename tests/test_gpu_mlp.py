@@ -88,3 +88,36 @@ def test_training_reduces_loss():
     assert np.isfinite(losses).all()
     assert l1 < 0.1 * l0, (l0, l1)
     net.close()
+
+
+def test_gelu_gradients_and_inference_match_oracle():
+    """The GELU network (x sigma(1.702 x)): FP32 gradients vs the oracle's
+    FP64 gradients, bf16 gradients and matrix-core inference vs the oracle's
+    bf16 mode."""
+    net, nm = setup_net(seed=16)
+    net.set_activation("gelu")
+    rng = np.random.default_rng(12)
+    X = rng.standard_normal((600, 22)).astype(np.float32)
+    T = rng.standard_normal((600, 6)).astype(np.float32)
+    W, b = net.unpack()
+    M.ACT = "gelu"
+    try:
+        loss, g = net.grad(X, T, mode=0)
+        l_ref, gW, gb = M.loss_and_grads([w.astype(np.float64) for w in W], [x.astype(np.float64) for x in b],
+                                         X.astype(np.float64), T.astype(np.float64),
+                                         *[x.astype(np.float64) for x in nm], mode="fp64")
+        gr = flat(gW, gb)
+        assert abs(loss - l_ref) < 1e-5 * abs(l_ref)
+        assert np.abs(g - gr).max() < 1e-4 * np.abs(gr).max()
+        loss, g = net.grad(X, T, mode=1)
+        l_ref, gW, gb = M.loss_and_grads(W, b, X, T, *nm, mode="bf16")
+        gr = flat(gW, gb)
+        assert abs(loss - l_ref) < 1e-3 * abs(l_ref)
+        assert np.linalg.norm(g - gr) < 2e-2 * np.linalg.norm(gr)
+        Y = net.infer(X)
+        Yr = M.forward(W, b, X, *nm, mode="bf16")
+        err = np.abs(Y - Yr) / np.abs(Yr).max()
+        assert np.median(err) < 1e-3 and err.max() < 3e-2
+    finally:
+        M.ACT = "relu"
+    net.close()

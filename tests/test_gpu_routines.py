@@ -106,3 +106,42 @@ def test_uzero(setup):
     for flag in (1, 3):
         got = ch.uzero(u0, h1, h1l, flag)
         assert rel(got, O.uzero(u0, h1, h1l, flag, p)) < 1e-11, flag
+
+
+def test_vcw3dp_polymer_argument_list():
+    """vcw3dp with the POLYMER argument list (solvers.f90:1726-1740, FENE-P
+    products :2901-2983) on a seeded state and conformation, against the
+    oracle's vcw3dp: u x omega and c_n, str_n, qp of every component."""
+    from tests._cases import poly_state
+    p = params(32, 32, 32, dt=0.005, polymer=True, tpoly=0.5, zlmax=30.0, qbeta=0.9, src_start=0)
+    cfg = kmm.make_config(p.nx, p.ny, p.nz, xl=p.xl, zl=p.zl, re=p.re, dt=p.dt, r_tau=p.R_tau, polymer=1,
+                          tpoly=p.tpoly, zlmax=p.zlmax, qbeta=p.qbeta, crstrt=1, src_start=0)
+    st = seeded(p, 44, amp=0.3)
+    C = poly_state(p, 45, amp=0.05)
+    C = {ij: C["c" + ij] for ij in O.POLY}
+    u, v, w = O.veloc(st["v"], st["omy"], st["u0"], st["w0"], p)
+    omx, omz = O.vort(u, v, w, p)
+    for a in (u, v, w, omx, omz):
+        O.norm(a)
+    gv = O.gradv(u, v, w, p)
+    dC = O.derivscji(C, p)
+    fields = dict(u=u, v=v, w=w, omx=omx, omy=st["omy"], omz=omz, C=C, dC=dC, gradv=gv)
+    ost = dict(fxintg=np.zeros((p.mx, p.mz, p.nyp)), fyintg=np.zeros((p.mx, p.mz, p.nyp)),
+               fzintg=np.zeros((p.mx, p.mz, p.nyp)))
+    gn, fn, oz, _, phys = O.vcw3dp(ost, p, 1, fields)
+    uij = [gv[f"u{i}{k}"] for i in (1, 2, 3) for k in (1, 2, 3)]
+    full = {(i, j): C[O._CIDX[(i, j)]] for i in range(3) for j in range(3)}
+    cij = [full[(i, j)] for i in range(3) for j in range(3)]
+    dc = [None] * 27
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                dc[9 * j + 3 * i + k] = dC[O._CIDX[(i, j)]][k]
+    with kmm.Channel(cfg) as ch:
+        g2, f2, z2, cn, sn, qp = ch.vcw3dp_poly(u, v, w, omx, st["omy"], omz, uij, cij, dc)
+    for a, b in ((g2, gn), (f2, fn), (z2, oz)):
+        assert rel(a, O.norm(b.copy())) < 1e-12
+    for ij in O.POLY:
+        for got, key in ((cn, "cn"), (sn, "str"), (qp, "qp")):
+            want = O.norm(phys["poly"][key][ij].copy())
+            assert rel(got[ij], want) < 1e-11, (key, ij, rel(got[ij], want))

@@ -59,3 +59,39 @@ def test_gradients_match_finite_differences():
         fd = (M.loss_and_grads(W, bp, X, T, mx, sx, my, sy, "fp64")[0]
               - M.loss_and_grads(W, bm, X, T, mx, sx, my, sy, "fp64")[0]) / (2 * h)
         assert abs(fd - gb[l][i]) < 1e-7 * max(1.0, abs(fd))
+
+
+def test_gelu_sigmoid_form_against_the_exact_gelu_and_its_derivative():
+    """GELU as x sigma(1.702 x): within 0.021 of x Phi(x) (the exact GELU,
+    Phi the standard normal CDF), 0 at 0, -> x for large x, -> 0 for large
+    -x; the derivative against central differences; the gradients of the
+    whole network against finite differences with GELU."""
+    x = np.linspace(-8, 8, 4001)
+    phi = 0.5 * (1.0 + np.array([math.erf(v / math.sqrt(2.0)) for v in x]))
+    g = M.act(x, "gelu")
+    assert np.abs(g - x * phi).max() < 0.021
+    assert M.act(np.array([0.0]), "gelu")[0] == 0.0
+    assert abs(M.act(np.array([30.0]), "gelu")[0] - 30.0) < 1e-12
+    assert abs(M.act(np.array([-30.0]), "gelu")[0]) < 1e-12
+    h = 1e-6
+    fd = (M.act(x + h, "gelu") - M.act(x - h, "gelu")) / (2 * h)
+    assert np.abs(fd - M.act_grad(x, "gelu")).max() < 1e-8
+    rng = np.random.default_rng(4)
+    W, b = M.init_params(8)
+    W = [w.astype(np.float64) for w in W]
+    b = [bb.astype(np.float64) + 0.01 * rng.standard_normal(bb.shape) for bb in b]
+    X = rng.standard_normal((5, 22))
+    T = rng.standard_normal((5, 6))
+    nm = (np.zeros(22), np.ones(22), np.zeros(6), np.ones(6))
+    M.ACT = "gelu"
+    try:
+        loss, gW, gb = M.loss_and_grads(W, b, X, T, *nm, mode="fp64")
+        for l, (i, j) in [(0, (3, 5)), (2, (1, 2)), (4, (5, 17))]:
+            Wp = [w.copy() for w in W]
+            Wm = [w.copy() for w in W]
+            Wp[l][i, j] += h
+            Wm[l][i, j] -= h
+            fd = (M.loss_and_grads(Wp, b, X, T, *nm, "fp64")[0] - M.loss_and_grads(Wm, b, X, T, *nm, "fp64")[0]) / (2 * h)
+            assert abs(fd - gW[l][i, j]) < 1e-7 * max(1.0, abs(fd)), (l, i, j)
+    finally:
+        M.ACT = "relu"
