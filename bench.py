@@ -76,10 +76,17 @@ ACTIVATION = "gelu"   # the closure network's hidden activation (x sigma(1.702 x
 PRE_STEPS = 8    # FENE-P steps before the warm-up (the learned line draws its training pairs there)
 NPART = 100_000
 AP = 1e-3   # tracer radius: keeps the wall clamp of tracer_update inside the grid
+# the two viscoelastic workloads: Re_tau 180 at 192 x 129 x 192 with 100 000
+# tracers, and Re_tau 395 (Re_bulk 6875 in half-height units) at
+# 256 x 193 x 256 with the same FENE-P parameters
+CASE_180 = dict(nx=PNX, ny=PNY, nz=PNZ, re=RE, rtau=RTAU, dt=DT, npart=NPART, seed=SEED)
+CASE_395 = dict(nx=256, ny=192, nz=256, re=6875.0, rtau=395.0, dt=0.001, npart=0, seed=SEED + 20)
 
 
-def polymer_mode(kmm, precision, steps, warmup, learned=False, audit_every=AUDIT_EVERY):
+def polymer_mode(kmm, precision, steps, warmup, learned=False, audit_every=AUDIT_EVERY, case=CASE_180):
     from polymer_dns_amd import ic
+    PNX, PNY, PNZ, RE, RTAU, DT = (case[k] for k in ("nx", "ny", "nz", "re", "rtau", "dt"))
+    NPART, SEED = case["npart"], case["seed"]
     utau = RTAU / RE                    # friction velocity in bulk units (half height 1)
     tpoly = WI_TAU / (RE * utau * utau)   # Wi_tau = lambda u_tau^2 / nu
     cfg = kmm.make_config(PNX, PNY, PNZ, xl=XL, zl=ZL, re=RE, dt=DT, r_tau=RTAU, polymer=1, zlmax=ZLMAX,
@@ -96,8 +103,10 @@ def polymer_mode(kmm, precision, steps, warmup, learned=False, audit_every=AUDIT
         mean = ic.shear_equilibrium(st["u0"], tpoly, ZLMAX, ipeter=1)
         for k, v in ic.conformation_state(PNX, PNY, PNZ, SEED + 3, amp=1e-3, mean=mean).items():
             ch.put(k, v)
-        rng = np.random.default_rng(SEED + 4)
-        ch.set_particles(rng.uniform(0, XL, NPART), rng.uniform(-0.99, 0.99, NPART), rng.uniform(0, ZL, NPART))
+        if NPART:
+            rng = np.random.default_rng(SEED + 4)
+            ch.set_particles(rng.uniform(0, XL, NPART), rng.uniform(-0.99, 0.99, NPART),
+                             rng.uniform(0, ZL, NPART))
         ch.startup()
         extra = {"autotune": ch.autotune(3)}
         if learned:
@@ -156,6 +165,42 @@ def polymer_mode(kmm, precision, steps, warmup, learned=False, audit_every=AUDIT
     out.update(extra)
     out["_stats"] = stats
     out["at_step"] = PRE_STEPS + warmup + steps
+    return out
+
+
+def polymer_lines(kmm, steps, case, dense):
+    """FENE-P in FP64 and mixed precision, and the learned closure (mixed)
+    at the specified audit cadence (and, with dense, every AUDIT_DENSE
+    steps), on one viscoelastic workload."""
+    pf = polymer_mode(kmm, 0, steps, 5, case=case)
+    pm = polymer_mode(kmm, 1, steps, 5, case=case)
+    b = pm.pop("_stats")
+    pf.pop("_stats")
+    learned = {}
+    lines = [("mixed_learned", AUDIT_EVERY)] + ([("mixed_learned_dense", AUDIT_DENSE)] if dense else [])
+    for name, every in lines:
+        try:
+            pl = polymer_mode(kmm, 1, steps, 5, learned=True, audit_every=every, case=case)
+            a = pl.pop("_stats")
+            pl["profile_error_vs_fenep_mixed"] = {
+                k: round(float(np.abs(a[k] - b[k]).max() / np.abs(b[k]).max()), 6)
+                for k in ("umean", "urms", "rss", "pss")}
+        except kmm.KmmError as e:   # a learned run that leaves the solver's range is reported, not fatal
+            pl = {"error": str(e), "value": None, "ms_per_step": None, "audit_every": every}
+        learned[name] = pl
+    nx, ny, nz = case["nx"], case["ny"], case["nz"]
+
+    def speedup(ref, line):
+        return round(ref["ms_per_step"] / line["ms_per_step"], 3) if line.get("ms_per_step") else None
+    out = {"workload": f"viscoelastic channel {nx}x{ny + 1}x{nz}, Re_tau {case['rtau']:g}, FENE-P Wi_tau "
+                       f"{WI_TAU:g}, beta {QBETA}, L^2 {ZLMAX ** 2:g}, Sc {SC}, {case['npart']} tracers, "
+                       "conformation started at the mean-shear equilibrium",
+           "unit": "Mgridpt-steps/s", "fp64": pf, "mixed": pm, **learned,
+           "speedup_mixed_vs_fp64": speedup(pf, pm),
+           "speedup_learned_vs_fenep_fp64": speedup(pf, learned["mixed_learned"]),
+           "speedup_learned_vs_fenep_mixed": speedup(pm, learned["mixed_learned"])}
+    if dense:
+        out["speedup_learned_dense_vs_fenep_fp64"] = speedup(pf, learned["mixed_learned_dense"])
     return out
 
 
@@ -328,34 +373,8 @@ def main():
                   "speedup_vs_fp64": round(t / tm, 3), "autotune": tunem},
     }
     if args.poly_steps > 0:
-        pf = polymer_mode(kmm, 0, args.poly_steps, 5)
-        pm = polymer_mode(kmm, 1, args.poly_steps, 5)
-        b = pm.pop("_stats")
-        pf.pop("_stats")
-        learned = {}
-        for name, every in (("mixed_learned", AUDIT_EVERY), ("mixed_learned_dense", AUDIT_DENSE)):
-            try:
-                pl = polymer_mode(kmm, 1, args.poly_steps, 5, learned=True, audit_every=every)
-                a = pl.pop("_stats")
-                pl["profile_error_vs_fenep_mixed"] = {
-                    k: round(float(np.abs(a[k] - b[k]).max() / np.abs(b[k]).max()), 6)
-                    for k in ("umean", "urms", "rss", "pss")}
-            except kmm.KmmError as e:   # a learned run that leaves the solver's range is reported, not fatal
-                pl = {"error": str(e), "value": None, "ms_per_step": None, "audit_every": every}
-            learned[name] = pl
-        pl = learned["mixed_learned"]
-        out["polymer"] = {"workload": f"viscoelastic channel {PNX}x{PNY + 1}x{PNZ}, FENE-P Wi_tau {WI_TAU:g}, "
-                                      f"beta {QBETA}, L^2 {ZLMAX ** 2:g}, Sc {SC}, {NPART} tracers, "
-                                      "conformation started at the mean-shear equilibrium",
-                          "unit": "Mgridpt-steps/s", "fp64": pf, "mixed": pm, **learned,
-                          "speedup_mixed_vs_fp64": round(pf["ms_per_step"] / pm["ms_per_step"], 3),
-                          "speedup_learned_vs_fenep_fp64": (round(pf["ms_per_step"] / pl["ms_per_step"], 3)
-                                                            if pl.get("ms_per_step") else None),
-                          "speedup_learned_vs_fenep_mixed": (round(pm["ms_per_step"] / pl["ms_per_step"], 3)
-                                                             if pl.get("ms_per_step") else None),
-                          "speedup_learned_dense_vs_fenep_fp64": (
-                              round(pf["ms_per_step"] / learned["mixed_learned_dense"]["ms_per_step"], 3)
-                              if learned["mixed_learned_dense"].get("ms_per_step") else None)}
+        out["polymer"] = polymer_lines(kmm, args.poly_steps, CASE_180, dense=True)
+        out["polymer_re395"] = polymer_lines(kmm, args.poly_steps, CASE_395, dense=False)
     if args.ensemble_steps > 0:
         out["ensemble"] = ensemble_mode(kmm, args.ensemble_steps, 10)
     if not args.no_cpu:

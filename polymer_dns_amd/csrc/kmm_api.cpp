@@ -379,8 +379,8 @@ extern "C" int kmm_create(const kmm_config* cfg, int device, kmm_ctx** out) {
   const int nx = cfg->nx, ny = cfg->ny, nz = cfg->nz;
   if (nx <= 0 || ny <= 0 || nz <= 0 || nx % 2 || ny % 2 || nz % 2)
     return fail(KMM_E_CONFIG, "grid sizes must be positive and even");
-  if (!(ny == 32 || ny == 48 || ny == 64 || ny == 80 || ny == 96 || ny == 128))
-    return fail(KMM_E_CONFIG, "ny must be one of 32, 48, 64, 80, 96, 128");
+  if (!(ny == 32 || ny == 48 || ny == 64 || ny == 80 || ny == 96 || ny == 128 || ny == 192))
+    return fail(KMM_E_CONFIG, "ny must be one of 32, 48, 64, 80, 96, 128, 192");
   if ((nx / 2) % 8 != 0) return fail(KMM_E_CONFIG, "nx/2 must be a multiple of 8");
   if (!fft_len_ok(3 * nx / 2) || !fft_len_ok(3 * nz / 2)) return fail(KMM_E_CONFIG, "3/2 grid not 2^a 3^b");
   if (cfg->theta > 0.99) return fail(KMM_E_CONFIG, "THETA > 0.99 is not supported");
@@ -439,7 +439,7 @@ extern "C" int kmm_create(const kmm_config* cfg, int device, kmm_ctx** out) {
     const size_t zb = (size_t)c->nyp * c->mz * c->nxh * c->vsz;
     if ((r = dalloc(c, (void**)&c->Z, 9 * zb))) goto fail_;
     if ((r = dalloc(c, (void**)&c->ZF, 4 * zb))) goto fail_;
-    const size_t tb = (size_t)c->P * 65 * sizeof(double2);
+    const size_t tb = (size_t)c->P * kmm::tab_width(ny) * sizeof(double2);
     for (auto p : {&c->tabH, &c->tabP, &c->tabS}) if ((r = dalloc(c, (void**)p, tb))) goto fail_;
     const size_t gb = (size_t)c->P * (ny / 2 + 1) * sizeof(double2);
     for (auto p : {&c->G1, &c->G2}) if ((r = dalloc(c, (void**)p, gb))) goto fail_;
@@ -1845,9 +1845,9 @@ static int slab_solve(kmm_ctx* c, double* u, double x, double g, double dyde, co
   std::vector<double> wn(nz);
   std::vector<int> kz(nz, 0);
   const bool anyk = slab_wn(c, x, g, ib, wn.data(), kz.data());
-  // slab layout: u [nyp][nz], 1/d table [nz][65], bctop, bcbot [nz] (double2),
-  // wn [nz] (double), zero-mode flags [nz] (int)
-  const size_t nu = (size_t)nyp * nz, ntab = (size_t)nz * 65;
+  // slab layout: u [nyp][nz], 1/d table [nz][tab_width], bctop, bcbot [nz]
+  // (double2), wn [nz] (double), zero-mode flags [nz] (int)
+  const size_t nu = (size_t)nyp * nz, ntab = (size_t)nz * kmm::tab_width(c->ny);
   if (!c->slab) {
     int r = dalloc(c, &c->slab, (nu + ntab + 2 * (size_t)nz) * sizeof(double2) + (size_t)nz * (sizeof(double) + sizeof(int)));
     if (r) return r;
@@ -2079,7 +2079,8 @@ extern "C" int kmm_uzero(kmm_ctx* c, double* u0, const double* h1, const double*
 namespace {
 constexpr size_t kNaSlab = 131072;  // max nyp * nz complex elements of one slab
 constexpr int kNaNz = 1024;         // max nz
-constexpr size_t kNaBytes = 2 * kNaSlab * sizeof(double2) + (size_t)kNaNz * (65 + 2) * sizeof(double2) +
+constexpr int kNaTab = kmm::tab_width(192);  // widest 1/d table (ny = 192)
+constexpr size_t kNaBytes = 2 * kNaSlab * sizeof(double2) + (size_t)kNaNz * (kNaTab + 2) * sizeof(double2) +
                             (size_t)kNaNz * (sizeof(double) + sizeof(int));
 constexpr int kNaDevices = 64;
 std::mutex g_na_mu;
@@ -2117,7 +2118,7 @@ extern "C" int kmm_solve_noalloc(kmm_ctx* c, double* u, double x, double g, doub
   double2* dcol = reinterpret_cast<double2*>(base);           // reference order
   double2* du = dcol + kNaSlab;                               // [y][kz]
   double2* dtab = du + kNaSlab;
-  double2* dbt = dtab + (size_t)kNaNz * 65;
+  double2* dbt = dtab + (size_t)kNaNz * kNaTab;
   double2* dbb = dbt + kNaNz;
   double* dwn = reinterpret_cast<double*>(dbb + kNaNz);
   int* dkz = reinterpret_cast<int*>(dwn + kNaNz);

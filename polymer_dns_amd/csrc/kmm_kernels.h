@@ -6,6 +6,12 @@
 
 namespace kmm {
 
+// Per-pencil table of the 1/d reciprocals of eqsolve/pntsolve
+// (solvers.f90:500-553, 931-935): entry g (x: even, y: odd) for the
+// coefficient pairs g = 0..ny/2-1, and the even extra row's 1/d in .x of the
+// last entry; 65 entries for ny <= 128.
+__host__ __device__ constexpr int tab_width(int ny) { return ny / 2 <= 64 ? 65 : ny / 2 + 1; }
+
 // Pencil-stage arguments.  Field buffers are [y][kx][kz] complex128; a
 // "pencil" p = kx*nz + kz; P = nxh*nz pencils per y-plane.
 struct PencilArgs {
@@ -22,7 +28,7 @@ struct PencilArgs {
   const double* smz;   // [nz] z smoothing factors
   const double* smy;   // [nyp] y smoothing factors
   const void* twM;     // FFT twiddles for length 2*ny (transfer precision)
-  const double2* tabH; // [P][65] 1/d tables, wn = k2 + gH
+  const double2* tabH; // [P][tab_width(ny)] 1/d tables, wn = k2 + gH
   const double2* tabP; // wn = k2
   const double2* tabS; // wn = k2 + gS
   const double2* G1;   // [P][ny/2+1] Green's functions (real, lane-pair packed)

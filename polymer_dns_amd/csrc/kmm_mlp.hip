@@ -1,6 +1,7 @@
-// Learned polymer-stress closure: a 22 -> 4 x 128 (ReLU) -> 6 multilayer
+// Learned polymer-stress closure: a 22 -> 4 x 128 -> 6 multilayer
 // perceptron with bfloat16 weights and FP32 accumulation (no reference
-// counterpart).
+// counterpart); hidden activation ReLU or GELU in its sigmoid form
+// x sigma(1.702 x), chosen per network.
 //
 //   k_mlp_infer   inference on the matrix cores: v_mfma_f32_16x16x32_bf16
 //                 with the weights as the A operand (staged once per block in

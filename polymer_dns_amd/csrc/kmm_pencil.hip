@@ -34,13 +34,14 @@ __global__ void k_tau_tables(int P, int nz, double alpha, double beta, double dy
     const double kxw = kx * alpha, kzw = kzs * beta;
     wn = kxw * kxw + kzw * kzw + g;
   }
-  double2* t = tab + (size_t)p * 65;
-  for (int i = 0; i < 65; ++i) t[i] = make_double2(0.0, 0.0);
+  constexpr int TW = tab_width(NY);
+  double2* t = tab + (size_t)p * TW;
+  for (int i = 0; i < TW; ++i) t[i] = make_double2(0.0, 0.0);
   // even: m = L
   double a1, a2, a3, b1, b2, b3;
   coef_even<NY>(L, wn, dyde, a1, a2, a3);
   double d = a2;
-  t[64].x = 1.0 / d;
+  t[TW - 1].x = 1.0 / d;
   double dn = d;  // d_{i+1}
   for (int i = L - 1; i >= 1; --i) {
     coef_even<NY>(i, wn, dyde, a1, a2, a3);
@@ -74,6 +75,7 @@ __global__ __launch_bounds__(PW * 64) void k_green(PencilArgs A, double pb1, dou
                                                    double2* __restrict__ G1, double2* __restrict__ G2,
                                                    double4* __restrict__ gk) {
   constexpr int L = NY / 2;
+  using PF = PFN<NY>;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int p = blockIdx.x * PW + wv;
   if (p >= A.P) return;
@@ -81,17 +83,21 @@ __global__ __launch_bounds__(PW * 64) void k_green(PencilArgs A, double pb1, dou
   int kxi, kzi;
   pencil_k(A, p, kx, kz, kxi, kzi);
   const double k2 = kx * kx + kz * kz;
-  const double2* tH = A.tabH + (size_t)p * 65;
-  const double2* tP = A.tabP + (size_t)p * 65;
+  const double2* tH = A.tabH + (size_t)p * tab_width(NY);
+  const double2* tP = A.tabP + (size_t)p * tab_width(NY);
   double2 gt[2], gb[2];
   for (int w = 0; w < 2; ++w) {
     const double pb = w ? pb2 : pb1, pt = w ? pt2 : pt1;
-    PF z = pf_zero();
+    PF z = pf_zero<NY>();
     PF phi = tau_solve<NY>(z, k2 + A.gH, A.dyde, tH, make_double2(pt, 0.0), make_double2(pb, 0.0), 0, lane);
     PF g = tau_solve<NY>(phi, k2, A.dyde, tP, czero(), czero(), 0, lane);
     g = pf_real(g);
     double2* G = (w ? G2 : G1) + (size_t)p * (L + 1);
-    if (lane < L) G[lane] = make_double2(g.a.x, g.b.x);
+#pragma unroll
+    for (int q = 0; q < PLay<NY>::S; ++q) {
+      const int gi = q * WAVE + lane;
+      if (gi < L) G[gi] = make_double2(g.a[q].x, g.b[q].x);
+    }
     if (lane == 0) G[L] = make_double2(g.e.x, 0.0);
     wall_derivs<NY>(g, A.dyde, lane, gt[w], gb[w]);
   }
@@ -105,20 +111,17 @@ __global__ __launch_bounds__(PW * 64) void k_green(PencilArgs A, double pb1, dou
 
 // ----------------------------------------------------------- the step --
 template <int NY>
-__device__ __forceinline__ PF green_load(const double2* __restrict__ G, int p, int lane) {
+__device__ __forceinline__ PFN<NY> green_load(const double2* __restrict__ G, int p, int lane) {
   constexpr int L = NY / 2;
   const double2* g = G + (size_t)p * (L + 1);
-  PF f = pf_zero();
-  if (lane < L) { double2 v = g[lane]; f.a.x = v.x; f.b.x = v.y; }
+  PFN<NY> f = pf_zero<NY>();
+#pragma unroll
+  for (int q = 0; q < PLay<NY>::S; ++q) {
+    const int gi = q * WAVE + lane;
+    if (gi < L) { double2 v = g[gi]; f.a[q].x = v.x; f.b[q].x = v.y; }
+  }
   f.e.x = g[L].x;
   return f;
-}
-
-// complex pencil times a real pencil coefficientwise
-__device__ __forceinline__ PF pf_cmul_real(double2 c, const PF& g) {
-  PF r;
-  r.a = cscale(c, g.a.x); r.b = cscale(c, g.b.x); r.e = cscale(c, g.e.x);
-  return r;
 }
 
 
@@ -129,7 +132,7 @@ __device__ __forceinline__ PF pf_cmul_real(double2 c, const PF& g) {
 // budget, 2 waves/SIMD at NY = 128; 1: at least 3 waves/SIMD, <= 168 VGPRs)
 template <int NY, bool SCALAR, int MODE, typename V, int TV = 0>
 __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(TV ? 3 : 1))) void k_pencil(PencilArgs A) {
-  constexpr int L = NY / 2;
+  using PF = PFN<NY>;
   extern __shared__ __align__(16) unsigned char smraw[];
   V* smem = reinterpret_cast<V*>(smraw);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -150,7 +153,7 @@ __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(TV ? 3 
   const int nT = SCALAR ? 9 : 6;
 
   if (nyq) {  // norm zeroes the z-Nyquist row (helpers.f90:149-156)
-    PF z = pf_zero();
+    PF z = pf_zero<NY>();
     if (MODE == 0) {
       pf_store<NY>(A.V, P, p, lane, z); pf_store<NY>(A.OMY, P, p, lane, z);
       pf_store<NY>(A.FNM1, P, p, lane, z); pf_store<NY>(A.GNM1, P, p, lane, z);
@@ -164,8 +167,7 @@ __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(TV ? 3 
       for (int t = 0; t < nT + 3; ++t) {
         V* d = (t < nT) ? TT[t] : static_cast<V*>(A.TPD[t - nT]);
         if (!d) continue;
-        if (lane < L) { d[(size_t)(2 * lane) * P + p] = vzero<V>(); d[(size_t)(2 * lane + 1) * P + p] = vzero<V>(); }
-        if (lane == 0) d[(size_t)NY * P + p] = vzero<V>();
+        pf_store_zero<NY, V>(d, P, p, lane);
       }
     return;
   }
@@ -177,12 +179,12 @@ __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(TV ? 3 
     if (MODE == 2 || A.first_nl) {
       const double sz = A.smooth ? A.smz[kzi] : 1.0;
       // each pencil's load is issued one transform ahead
-      PhysPencil<V> q1 = phys_load<NY, V>(static_cast<const V*>(A.NL1), P, p, lane);
-      PhysPencil<V> q3 = phys_load<NY, V>(static_cast<const V*>(A.NL3), P, p, lane);
+      PhysN<NY, V> q1 = phys_load<NY, V>(static_cast<const V*>(A.NL1), P, p, lane);
+      PhysN<NY, V> q3 = phys_load<NY, V>(static_cast<const V*>(A.NL3), P, p, lane);
       PF h1 = phys_to_cheb<NY, V>(q1, buf, scr, twl, lane);
-      PhysPencil<V> q2 = phys_load<NY, V>(static_cast<const V*>(A.NL2), P, p, lane);
+      PhysN<NY, V> q2 = phys_load<NY, V>(static_cast<const V*>(A.NL2), P, p, lane);
       PF h3 = phys_to_cheb<NY, V>(q3, buf, scr, twl, lane);
-      PhysPencil<V> qs;
+      PhysN<NY, V> qs;
       if (SCALAR) qs = phys_load<NY, V>(static_cast<const V*>(A.NLS), P, p, lane);
       PF h2 = phys_to_cheb<NY, V>(q2, buf, scr, twl, lane);
       if (A.TF[0]) {  // polymer force, subforce (helpers.f90:564-606), before smoothing
@@ -242,7 +244,7 @@ __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(TV ? 3 
       pf_store<NY>(A.SCNM1, P, p, lane, sn);
       auto tt = [dy](int n) -> double { return dy * (double)n * (double)n; };
       auto tb = [dy](int n) -> double { double v = dy * (double)n * (double)n; return (n & 1) ? -v : v; };
-      sc = tau_penta<NY>(rhs, k2 + A.gS, dy, A.tabS + (size_t)p * 65, tt, tb, czero(), czero(), lane);
+      sc = tau_penta<NY>(rhs, k2 + A.gS, dy, A.tabS + (size_t)p * tab_width(NY), tt, tb, czero(), czero(), lane);
       if (is00) sc = pf_real(sc);
       pf_store<NY>(A.TH, P, p, lane, sc);
     }
@@ -262,8 +264,8 @@ __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(TV ? 3 
       d2 = cderiv<NY>(d1, dy, lane);
       PF rhs = phirhs<NY>(phi, d2, fn, fnm1, k2, A.re, 1.0, A.dt, th);
       pf_store<NY>(A.FNM1, P, p, lane, fn);
-      phi = tau_solve<NY>(rhs, k2 + A.gH, dy, A.tabH + (size_t)p * 65, czero(), czero(), 0, lane);
-      PF vp = tau_solve<NY>(phi, k2, dy, A.tabP + (size_t)p * 65, czero(), czero(), 0, lane);
+      phi = tau_solve<NY>(rhs, k2 + A.gH, dy, A.tabH + (size_t)p * tab_width(NY), czero(), czero(), 0, lane);
+      PF vp = tau_solve<NY>(phi, k2, dy, A.tabP + (size_t)p * tab_width(NY), czero(), czero(), 0, lane);
       double2 vt, vb;
       wall_derivs<NY>(vp, dy, lane, vt, vb);
       const double4 g = A.gk[p];
@@ -280,18 +282,18 @@ __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(TV ? 3 
       const double atu = A.atu, btu = A.btu, abu = A.abu, bbu = A.bbu;
       auto tt = [=](int n) -> double { return atu + dy * btu * (double)n * (double)n; };
       auto tb = [=](int n) -> double { double t = abu + dy * bbu * (double)n * (double)n; return (n & 1) ? -t : t; };
-      omy = tau_penta<NY>(orhs, k2 + A.gH, dy, A.tabH + (size_t)p * 65, tt, tb, czero(), czero(), lane);
+      omy = tau_penta<NY>(orhs, k2 + A.gH, dy, A.tabH + (size_t)p * tab_width(NY), tt, tb, czero(), czero(), lane);
       pf_store<NY>(A.V, P, p, lane, v);
       pf_store<NY>(A.OMY, P, p, lane, omy);
     } else {
       // (0,0): v = 0 (veloc), omega_y = 0, fnm1 = fn = 0, gnm1 = gn = 0,
       // and the mean-flow solves (uzero, solvers.f90:1065-1176).
-      v = pf_zero();
-      omy = pf_zero();
+      v = pf_zero<NY>();
+      omy = pf_zero<NY>();
       pf_store<NY>(A.V, P, p, lane, v);
       pf_store<NY>(A.OMY, P, p, lane, omy);
-      pf_store<NY>(A.FNM1, P, p, lane, pf_zero());
-      pf_store<NY>(A.GNM1, P, p, lane, pf_zero());
+      pf_store<NY>(A.FNM1, P, p, lane, pf_zero<NY>());
+      pf_store<NY>(A.GNM1, P, p, lane, pf_zero<NY>());
       for (int w = 0; w < 2; ++w) {
         double* um = w ? A.w0 : A.u0;
         double* hl = w ? A.h3nm1 : A.h1nm1;
@@ -312,15 +314,15 @@ __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(TV ? 3 
         const double gt = w ? A.gtw : A.gtu, gb = w ? A.gbw : A.gbu;
         auto tt = [=](int n) -> double { return at + dy * bt * (double)n * (double)n; };
         auto tb = [=](int n) -> double { double t = ab - dy * bb * (double)n * (double)n; return (n & 1) ? -t : t; };
-        PF un = tau_penta<NY>(r, A.gH, dy, A.tabH + (size_t)p * 65, tt, tb, make_double2(gt, 0.0),
+        PF un = tau_penta<NY>(r, A.gH, dy, A.tabH + (size_t)p * tab_width(NY), tt, tb, make_double2(gt, 0.0),
                               make_double2(gb, 0.0), lane);
         un = pf_real(un);
         if (w == 0 && A.cmf) {
           // constant mass flux: u0 += f phi, phi = response to a unit forcing
           // held over the step (rhs -re/theta, homogeneous wall data)
-          PF unit = pf_zero();
-          if (lane == 0) unit.a = make_double2(-A.re / th, 0.0);
-          PF phi = tau_penta<NY>(unit, A.gH, dy, A.tabH + (size_t)p * 65, tt, tb, czero(), czero(), lane);
+          PF unit = pf_zero<NY>();
+          if (lane == 0) unit.a[0] = make_double2(-A.re / th, 0.0);
+          PF phi = tau_penta<NY>(unit, A.gH, dy, A.tabH + (size_t)p * tab_width(NY), tt, tb, czero(), czero(), lane);
           phi = pf_real(phi);
           const double bu = pf_bulk<NY>(un, lane), bp = pf_bulk<NY>(phi, lane);
           const double fc = (A.ubulk - bu) / bp;
@@ -337,7 +339,7 @@ __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(TV ? 3 
     v = pf_load<NY>(A.V, P, p, lane);
     omy = pf_load<NY>(A.OMY, P, p, lane);
     if (is00) {
-      v = pf_zero(); omy = pf_zero();
+      v = pf_zero<NY>(); omy = pf_zero<NY>();
       pf_store<NY>(A.V, P, p, lane, v);
       pf_store<NY>(A.OMY, P, p, lane, omy);
       u0p = pf_load_real<NY>(A.u0, lane);
@@ -384,6 +386,7 @@ __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(TV ? 3 
 template <int NY>
 __global__ __launch_bounds__(PW * 64) void k_cderiv(int P, double dyde, const double2* __restrict__ in,
                                                     double2* __restrict__ out) {
+  using PF = PFN<NY>;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int p = blockIdx.x * PW + wv;
   if (p >= P) return;
@@ -397,6 +400,7 @@ template <int NY>
 __global__ __launch_bounds__(PW * 64) void k_gradv(int P, int nz, int nzh, double alpha, double beta, double dyde,
                                                    const double2* __restrict__ u, const double2* __restrict__ v,
                                                    const double2* __restrict__ w, double2* const* __restrict__ out) {
+  using PF = PFN<NY>;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int p = blockIdx.x * PW + wv;
   if (p >= P) return;
@@ -422,6 +426,7 @@ __global__ __launch_bounds__(PW * 64) void k_gradv(int P, int nz, int nzh, doubl
 // transformed to physical y; one wave per pencil
 template <int NY, typename V>
 __global__ __launch_bounds__(PW * 64) void k_mr_lap(MrLapArgs A) {
+  using PF = PFN<NY>;
   extern __shared__ __align__(16) unsigned char smraw[];
   V* smem = reinterpret_cast<V*>(smraw);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -437,7 +442,7 @@ __global__ __launch_bounds__(PW * 64) void k_mr_lap(MrLapArgs A) {
   const double k2 = kx * kx + kz * kz;
   const double dy = A.dyde;
   const bool is00 = (kxi == 0 && kzi == 0);
-  PF f[3] = {pf_zero(), pf_zero(), pf_zero()};
+  PF f[3] = {pf_zero<NY>(), pf_zero<NY>(), pf_zero<NY>()};
   if (kzi != A.nzh) {  // norm zeroes the z-Nyquist row
     if (is00) {
       f[0] = pf_load_real<NY>(A.u0, lane);
@@ -464,6 +469,7 @@ __global__ __launch_bounds__(PW * 64) void k_mr_lap(MrLapArgs A) {
 template <int NY, typename V>
 __global__ __launch_bounds__(PW * 64) void k_cheb(int P, const V* __restrict__ tw, const void* in, void* out,
                                                   int inverse) {
+  using PF = PFN<NY>;
   extern __shared__ __align__(16) unsigned char smraw[];
   V* smem = reinterpret_cast<V*>(smraw);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -485,12 +491,13 @@ __global__ __launch_bounds__(PW * 64) void k_solve(int P, double dyde, const dou
                                                    const double* __restrict__ wn, double2* __restrict__ u,
                                                    const double2* __restrict__ bct, const double2* __restrict__ bcb,
                                                    int ib, const int* __restrict__ kzero) {
+  using PF = PFN<NY>;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int p = blockIdx.x * PW + wv;
   if (p >= P) return;
   PF f = pf_load<NY>(u, P, p, lane);
-  PF r = tau_solve<NY>(f, wn[p], dyde, tab + (size_t)p * 65, bct[p], bcb[p], ib, lane);
-  if (kzero && kzero[p]) r = pf_zero();
+  PF r = tau_solve<NY>(f, wn[p], dyde, tab + (size_t)p * tab_width(NY), bct[p], bcb[p], ib, lane);
+  if (kzero && kzero[p]) r = pf_zero<NY>();
   pf_store<NY>(u, P, p, lane, r);
 }
 
@@ -500,14 +507,15 @@ __global__ __launch_bounds__(PW * 64) void k_penta(int P, double dyde, const dou
                                                    double at, double bt, const double2* __restrict__ gtop,
                                                    double ab, double bb, const double2* __restrict__ gbot,
                                                    double bsign, const int* __restrict__ kzero) {
+  using PF = PFN<NY>;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int p = blockIdx.x * PW + wv;
   if (p >= P) return;
   PF f = pf_load<NY>(u, P, p, lane);
   auto tt = [=](int n) -> double { return at + dyde * bt * (double)n * (double)n; };
   auto tb = [=](int n) -> double { double t = ab + bsign * dyde * bb * (double)n * (double)n; return (n & 1) ? -t : t; };
-  PF r = tau_penta<NY>(f, wn[p], dyde, tab + (size_t)p * 65, tt, tb, gtop[p], gbot[p], lane);
-  if (kzero && kzero[p]) r = pf_zero();
+  PF r = tau_penta<NY>(f, wn[p], dyde, tab + (size_t)p * tab_width(NY), tt, tb, gtop[p], gbot[p], lane);
+  if (kzero && kzero[p]) r = pf_zero<NY>();
   pf_store<NY>(u, P, p, lane, r);
 }
 

@@ -1066,6 +1066,7 @@ static hipError_t plane_dispatch(const PlaneArgs& a, hipStream_t s, bool inv, hi
   KMM_PLANE_CASE(192, 192)
   KMM_PLANE_CASE(288, 288)
   KMM_PLANE_CASE(384, 192)
+  KMM_PLANE_CASE(384, 384)
 #undef KMM_PLANE_CASE
   return hipErrorInvalidValue;
 }
@@ -1083,6 +1084,7 @@ hipError_t launch_poly_plane(const PolyPlaneArgs& a, hipStream_t s) {
   KMM_POLY_CASE(192, 192)
   KMM_POLY_CASE(288, 288)
   KMM_POLY_CASE(384, 192)
+  KMM_POLY_CASE(384, 384)
 #undef KMM_POLY_CASE
   return hipErrorInvalidValue;
 }
@@ -1100,6 +1102,7 @@ hipError_t launch_swirl_plane(const PolyPlaneArgs& a, bool zstage, hipStream_t s
   KMM_SWIRL_CASE(192, 192)
   KMM_SWIRL_CASE(288, 288)
   KMM_SWIRL_CASE(384, 192)
+  KMM_SWIRL_CASE(384, 384)
 #undef KMM_SWIRL_CASE
   return hipErrorInvalidValue;
 }
@@ -1117,6 +1120,7 @@ hipError_t launch_mr_lap_plane(const MrLapArgs& a, hipStream_t s) {
   KMM_MRL_CASE(192, 192)
   KMM_MRL_CASE(288, 288)
   KMM_MRL_CASE(384, 192)
+  KMM_MRL_CASE(384, 384)
 #undef KMM_MRL_CASE
   return hipErrorInvalidValue;
 }
@@ -1133,6 +1137,7 @@ hipError_t launch_closure_fwd(const PolyPlaneArgs& a, hipStream_t s) {
   KMM_CLO_CASE(192, 192)
   KMM_CLO_CASE(288, 288)
   KMM_CLO_CASE(384, 192)
+  KMM_CLO_CASE(384, 384)
 #undef KMM_CLO_CASE
   return hipErrorInvalidValue;
 }

@@ -30,7 +30,7 @@ namespace kmm {
 
 template <int NY, int MODE, typename V>
 __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
-  constexpr int L = NY / 2;
+  using PF = PFN<NY>;
   extern __shared__ __align__(16) unsigned char smraw[];
   V* smem = reinterpret_cast<V*>(smraw);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
   if (MODE == 3) {
 #pragma unroll 1
     for (int c = 0; c < 6; ++c) {
-      PF C = pf_zero();
+      PF C = pf_zero<NY>();
       if (kzi != A.nzh) {
         C = phys_to_cheb_load<NY, V>(static_cast<const V*>(A.NLP[c]), P, p, buf, scr, twl, lane);
         if (is00) C = pf_real(C);
@@ -61,12 +61,12 @@ __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
   }
   if (MODE == 4) {
     // each component's load is issued before the previous one's transform
-    PF t1 = pf_zero(), t2 = pf_zero(), t3 = pf_zero();
+    PF t1 = pf_zero<NY>(), t2 = pf_zero<NY>(), t3 = pf_zero<NY>();
     if (kzi != A.nzh) {
-      PhysPencil<V> q = phys_load<NY, V>(static_cast<const V*>(A.NLP[6]), P, p, lane);
+      PhysN<NY, V> q = phys_load<NY, V>(static_cast<const V*>(A.NLP[6]), P, p, lane);
 #pragma unroll 1
       for (int c = 0; c < 6; ++c) {
-        const PhysPencil<V> cur = q;
+        const PhysN<NY, V> cur = q;
         if (c < 5) q = phys_load<NY, V>(static_cast<const V*>(A.NLP[7 + c]), P, p, lane);
         PF sn = phys_to_cheb<NY, V>(cur, buf, scr, twl, lane);
         if (is00) sn = pf_real(sn);
@@ -88,7 +88,7 @@ __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
     return;
   }
   if (kzi == A.nzh) {  // norm zeroes the z-Nyquist row (helpers.f90:149-156)
-    const PF z = pf_zero();
+    const PF z = pf_zero<NY>();
     for (int c = 0; c < 6; ++c) {
       if (MODE == 0 || MODE == 2) {
         if (A.partB || MODE == 2) {
@@ -100,9 +100,7 @@ __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
       if (MODE == 0 && (A.init || A.solve)) pf_store<NY>(A.C[c], P, p, lane, z);
       if (MODE != 2 && A.evaluate)
         for (int h = 0; h < 2; ++h) {
-          V* d = static_cast<V*>(A.TP[c + 6 * h]);
-          if (lane < L) { d[(size_t)(2 * lane) * P + p] = vzero<V>(); d[(size_t)(2 * lane + 1) * P + p] = vzero<V>(); }
-          if (lane == 0) d[(size_t)NY * P + p] = vzero<V>();
+          pf_store_zero<NY, V>(static_cast<V*>(A.TP[c + 6 * h]), P, p, lane);
         }
     }
     if (A.force && (MODE == 2 || A.partB))
@@ -112,7 +110,7 @@ __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
 
   const double sz = A.smooth ? A.smz[kzi] : 1.0;
   const double ti = 1.0 / A.tpoly;
-  PF t1 = pf_zero(), t2 = pf_zero(), t3 = pf_zero();
+  PF t1 = pf_zero<NY>(), t2 = pf_zero<NY>(), t3 = pf_zero<NY>();
 #pragma unroll 1
   for (int c = 0; c < 6; ++c) {
     // component c: 11, 12, 13, 22, 23, 33
@@ -120,7 +118,7 @@ __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
     PF cnm1_copy, snm1_copy;
     const bool haveB = (MODE == 2) || (MODE == 0 && A.partB);
     if (haveB) {
-      cn = A.closure ? pf_zero() : phys_to_cheb_load<NY, V>(static_cast<const V*>(A.NLP[c]), P, p, buf, scr, twl, lane);
+      cn = A.closure ? pf_zero<NY>() : phys_to_cheb_load<NY, V>(static_cast<const V*>(A.NLP[c]), P, p, buf, scr, twl, lane);
       sn = phys_to_cheb_load<NY, V>(static_cast<const V*>(A.NLP[6 + c]), P, p, buf, scr, twl, lane);
       if (is00) { cn = pf_real(cn); sn = pf_real(sn); }
       if (A.force) {
@@ -155,10 +153,10 @@ __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
     PF C;
     if (MODE == 0) {
       if (A.init) {
-        C = pf_zero();
+        C = pf_zero<NY>();
         if (is00 && (c == 0 || c == 3 || c == 5)) {
           const double v0 = A.c0[c == 0 ? 0 : (c == 3 ? 1 : 2)];
-          if (lane == 0) C.a = make_double2(v0, 0.0);
+          if (lane == 0) C.a[0] = make_double2(v0, 0.0);
         }
       } else {
         C = pf_load<NY>(A.C[c], P, p, lane);
@@ -180,7 +178,7 @@ __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
         // Neumann rows at both walls (at = ab = 0, bt = bb = 1, zero data)
         auto tt = [dy](int n) -> double { return dy * (double)n * (double)n; };
         auto tb = [dy](int n) -> double { double v = dy * (double)n * (double)n; return (n & 1) ? -v : v; };
-        C = tau_penta<NY>(rhs, k2 + A.gC, dy, A.tabC + (size_t)p * 65, tt, tb, czero(), czero(), lane);
+        C = tau_penta<NY>(rhs, k2 + A.gC, dy, A.tabC + (size_t)p * tab_width(NY), tt, tb, czero(), czero(), lane);
         if (is00) C = pf_real(C);
       }
       if (A.init || A.solve) pf_store<NY>(A.C[c], P, p, lane, C);

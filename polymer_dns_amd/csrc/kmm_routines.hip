@@ -12,6 +12,7 @@ namespace kmm {
 
 template <int NY>
 __global__ __launch_bounds__(PW * 64) void k_routine(RoutineArgs A) {
+  using PF = PFN<NY>;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int p = blockIdx.x * PW + wv;
   if (p >= A.P) return;
@@ -44,7 +45,7 @@ __global__ __launch_bounds__(PW * 64) void k_routine(RoutineArgs A) {
       if (kxi == 0 && kzi == 0) {
         u = pf_load_real<NY>(A.m0, lane);
         w = pf_load_real<NY>(A.m1, lane);
-        v = pf_zero();
+        v = pf_zero<NY>();
       } else {
         const PF dv = cderiv<NY>(v, dy, lane);
         const double ik2 = 1.0 / k2;
@@ -88,19 +89,23 @@ __global__ __launch_bounds__(PW * 64) void k_routine(RoutineArgs A) {
     }
     case RT_NORM: {  // helpers.f90:91-158
       if (kzi == A.nzh) {
-        st(0, pf_zero());
+        st(0, pf_zero<NY>());
       } else if (kxi == 0 && kzi == 0) {
         st(0, pf_real(ld(0)));
       } else if (kxi == 0 && kzi < A.nzh) {
         // Hermitian pair kz, -kz of the kx = 0 plane: a(j) = (a(j) + conj a(-j))/2, a(-j) = conj a(j)
         const PF a = ld(0);
         const PF b = pf_load<NY>(A.in[0], P, (size_t)(A.nz - kzi), lane);
-        PF h = a;
-        h.a = cscale(cadd(a.a, cconj(b.a)), 0.5);
-        h.b = cscale(cadd(a.b, cconj(b.b)), 0.5);
+        PF h = a, c;
+#pragma unroll
+        for (int q = 0; q < PLay<NY>::S; ++q) {
+          h.a[q] = cscale(cadd(a.a[q], cconj(b.a[q])), 0.5);
+          h.b[q] = cscale(cadd(a.b[q], cconj(b.b[q])), 0.5);
+          c.a[q] = cconj(h.a[q]);
+          c.b[q] = cconj(h.b[q]);
+        }
         h.e = cscale(cadd(a.e, cconj(b.e)), 0.5);
-        PF c = h;
-        c.a = cconj(h.a); c.b = cconj(h.b); c.e = cconj(h.e);
+        c.e = cconj(h.e);
         st(0, h);
         pf_store<NY>(A.out[0], P, (size_t)(A.nz - kzi), lane, c);
       } else if (kxi != 0) {
@@ -118,6 +123,7 @@ __global__ __launch_bounds__(PW * 64) void k_routine(RoutineArgs A) {
 // Chebyshev profiles), result in m3.
 template <int NY>
 __global__ __launch_bounds__(64) void k_uzero(RoutineArgs A, const double2* __restrict__ tab) {
+  using PF = PFN<NY>;
   const int lane = threadIdx.x;
   const double th = A.theta, dy = A.dyde;
   const PF u = pf_load_real<NY>(A.m0, lane);
