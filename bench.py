@@ -269,26 +269,28 @@ def measured_traffic(mode, kernel, tune):
     return table.get(mode, {}).get(kernel, {}).get(str(var))
 
 
-def cpu_baseline(nsteps=20):
-    """The oracle's FP64 step (numpy; its x/z/y transforms on scipy.fft with
-    one worker per core, up to the 16 cores a GPU's share of the host
-    holds; the elementwise passes and transposes stay on one thread) timed on
-    a bounded number of steps of the same workload."""
-    from threadpoolctl import threadpool_limits
+def cpu_baseline(target_s=15.0, max_steps=200):
+    """The compiled FP64 restatement of the step (oracle/kmm_cpu.c, C +
+    OpenMP, on the cores a GPU's share of the host holds, at most 16) timed
+    on a bounded number of steps of the same workload: start-up by the numpy
+    oracle (untimed), 2 warm-up steps, then as many steps as fill about
+    target_s seconds (at least 3)."""
+    from oracle import kmm_cpu
     from oracle import kmm_oracle as O
     cores = max(1, min(16, os.cpu_count() or 1))
-    O.WORKERS = cores
     p = O.Params(nx=NX, ny=NY, nz=NZ, xl=XL, zl=ZL, re=RE, dt=DT, R_tau=RTAU, gain=GAIN, ugain=UGAIN,
                  scalar=True, diff=PR, scl_flag=6, deltaT=SRC)
     st = workload_state()
     sol = O.Solver(p, imatrix=rib(NX, NY, NZ))
     sol.startup(st)
-    with threadpool_limits(1):   # BLAS pools stay single-threaded (the transforms use the workers)
-        t0 = time.perf_counter()
-        for _ in range(nsteps):
-            sol.step(st)
-        t = time.perf_counter() - t0
-    O.WORKERS = 1
+    kmm_cpu.steps(sol, st, 2, threads=cores)
+    t0 = time.perf_counter()
+    kmm_cpu.steps(sol, st, 1, threads=cores)
+    one = time.perf_counter() - t0
+    nsteps = int(max(3, min(max_steps, target_s / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    kmm_cpu.steps(sol, st, nsteps, threads=cores)
+    t = time.perf_counter() - t0
     G = NX * (NY + 1) * NZ
     model = ""
     try:
@@ -297,9 +299,9 @@ def cpu_baseline(nsteps=20):
     except OSError:
         pass
     return dict(value=G * nsteps / t / 1e6, unit="Mgridpt-steps/s", cores=cores, kind="port",
-                nproc=os.cpu_count(), cpu_model=model,
-                sample=f"{nsteps} FP64 steps of the same workload (numpy restatement; scipy.fft transforms on "
-                       f"{cores} workers, elementwise passes on one thread)")
+                nproc=os.cpu_count(), cpu_model=model, ms_per_step=round(1000.0 * t / nsteps, 2),
+                sample=f"{nsteps} FP64 steps of the same workload after 3 untimed ones (compiled C + OpenMP "
+                       f"restatement oracle/kmm_cpu.c, {cores} threads, own mixed-radix FFTs)")
 
 
 def run_mode(kmm, precision, steps, warmup):
