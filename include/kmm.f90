@@ -142,6 +142,25 @@ module kmm_c
     integer(c_int) function kmm_uzero(ctx, u0, h1, h1l, uflag) bind(C)
       import; type(c_ptr), value :: ctx; real(c_double) :: u0(*), h1(*), h1l(*); integer(c_int), value :: uflag
     end function
+    ! vcw3dp: uij is the address of 9 c_ptr (u11 .. u33) or c_null_ptr; the
+    ! scalar arguments may be c_null_ptr (no SCALAR list)
+    integer(c_int) function kmm_vcw3dp(ctx, u, v, w, omx, omy, omz, fn, gn, uij, scalar, sclx, scly, sclz, &
+                                       scn, it) bind(C)
+      import; type(c_ptr), value :: ctx
+      complex(c_double_complex) :: u(*), v(*), w(*), omx(*), omy(*), omz(*), fn(*), gn(*)
+      type(c_ptr), value :: uij, scalar, sclx, scly, sclz, scn
+      integer(c_int), value :: it
+    end function
+    ! vcw3dp with the POLYMER list: cij(9), dc(27) in, cn(6), strn(6), qp(6) out
+    integer(c_int) function kmm_vcw3dp_poly(ctx, u, v, w, omx, omy, omz, fn, gn, uij, scalar, sclx, scly, sclz, &
+                                            scn, cij, dc, cn, strn, qp, it) bind(C)
+      import; type(c_ptr), value :: ctx
+      complex(c_double_complex) :: u(*), v(*), w(*), omx(*), omy(*), omz(*), fn(*), gn(*)
+      type(c_ptr) :: uij(9)
+      type(c_ptr), value :: scalar, sclx, scly, sclz, scn
+      type(c_ptr) :: cij(9), dc(27), cn(6), strn(6), qp(6)
+      integer(c_int), value :: it
+    end function
   end interface
 
 contains

@@ -157,11 +157,17 @@ int kmm_calcswirl(kmm_ctx* ctx, const double* u11, const double* u21, const doub
                   double* swirl);
 
 /* Ensemble of independent channels (any grid sizes, one context each, same
- * device): advances every member by nsteps.  Each round queues one chunk of
- * steps (a graph replay where possible) on every member's stream before it
- * waits for any of them, so the members run concurrently.  Returns the
- * first error; its message names the member.  No reference counterpart. */
+ * device): advances every member by nsteps.  Members of one shape (grid,
+ * precision, scalar flag, occupancy variants) whose steps are plain channel
+ * steps (no polymer, particles, IB cells, scalar source window, closure or
+ * timing) run together: one launch per stage and step for the whole group,
+ * each channel a slice of the grid with its own argument block.  The other
+ * members queue one chunk of steps (a graph replay where possible) on their
+ * own streams, so groups and members run concurrently.  Returns the first
+ * error; its message names the member or group.  No reference counterpart. */
 int kmm_ensemble_step(kmm_ctx** ctxs, int n, int nsteps);
+/* Number of this channel's steps that ran in a group's shared launches. */
+int kmm_batched_steps(kmm_ctx* ctx, long long* n);
 
 /* Capture `steps` consecutive time steps into a HIP graph and replay it
  * whenever kmm_step runs steps whose launches are all alike (NL terms
@@ -178,7 +184,11 @@ int kmm_set_dpdx(kmm_ctx* ctx, double dpdx);
 /* Largest CFL number of the last step run and the step counter it. */
 int kmm_last_cfl(kmm_ctx* ctx, double* cflmax, int* it);
 
-/* outputs/last-restart (main_dns.f90:776-781) / setup/restart (1744-1747) */
+/* outputs/last-restart (main_dns.f90:776-781) / setup/restart (1744-1747).
+ * With irstrt = 2 in the configuration, kmm_read_restart reads a restart
+ * written at half the streamwise resolution (nx/2; main_dns.f90:1748-1809)
+ * into the low kx modes and the first mx/2 columns of the IB integrals, the
+ * rest zero. */
 int kmm_write_restart(kmm_ctx* ctx, const char* path);
 int kmm_read_restart(kmm_ctx* ctx, const char* path);
 /* c-restart (main_dns.f90:782-793, 1910-1916).  SCALAR only: the three

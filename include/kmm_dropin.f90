@@ -12,13 +12,17 @@
 ! Reference interfaces (FFTW_DNS/code/src/dns):
 !   derivs:  cderiv derivatives.f90:258, gradscl :90, derivscji :148
 !   solvers: solve solvers.f90:174, penta :680, phirhs :555, nonlin :1497,
-!            veloc :1576, vort :1663, uzero :1065
+!            veloc :1576, vort :1663, uzero :1065, vcw3dp :1726 (the NS,
+!            SCALAR and SCALAR+POLYMER argument lists under one generic name;
+!            the step index the reference reads from its COMMON block is
+!            kmm_it, which the host sets before the call)
 !   helpers: norm helpers.f90:91, polyforce :498, subforce :564, polyNL :610
 module kmm_dropin_state
   use iso_c_binding
   use kmm_c
   implicit none
   type(c_ptr), save :: kmm_ctx_handle = c_null_ptr
+  integer, save :: kmm_it = 0
 contains
   subroutine kmm_attach(ctx)
     type(c_ptr), intent(in) :: ctx
@@ -68,6 +72,9 @@ end module derivs
 module solvers
   use kmm_dropin_state
   implicit none
+  interface vcw3dp
+    module procedure vcw3dp_ns, vcw3dp_scalar, vcw3dp_scalar_polymer
+  end interface
 contains
   subroutine solve(u, x, g, dyde, bctop, bcbot, ib, a, wrkc)
     complex(8) :: u(*), bctop(*), bcbot(*), wrkc(*)
@@ -103,6 +110,64 @@ contains
     real(8) :: u0(*), h1(*), h1l(*)
     integer :: uflag
     call kmm_check(kmm_uzero(kmm_ctx_handle, u0, h1, h1l, int(uflag, c_int)), 'uzero')
+  end subroutine
+  ! vcw3dp, NS list (Lu, Lv, Lw feed the inertial particles the step tracks itself)
+  subroutine vcw3dp_ns(u, v, w, omx, omy, omz, fn, gn, u11, u12, u13, u21, u22, u23, u31, u32, u33, Lu, Lv, Lw)
+    complex(8) :: u(*), v(*), w(*), omx(*), omy(*), omz(*), fn(*), gn(*), Lu(*), Lv(*), Lw(*)
+    complex(8), target :: u11(*), u12(*), u13(*), u21(*), u22(*), u23(*), u31(*), u32(*), u33(*)
+    type(c_ptr), target :: g(9)
+    g = [c_loc(u11), c_loc(u12), c_loc(u13), c_loc(u21), c_loc(u22), c_loc(u23), c_loc(u31), c_loc(u32), &
+         c_loc(u33)]
+    call kmm_check(kmm_vcw3dp(kmm_ctx_handle, u, v, w, omx, omy, omz, fn, gn, c_loc(g), c_null_ptr, c_null_ptr, &
+                              c_null_ptr, c_null_ptr, c_null_ptr, int(kmm_it, c_int)), 'vcw3dp')
+  end subroutine
+  ! vcw3dp, SCALAR list
+  subroutine vcw3dp_scalar(u, v, w, omx, omy, omz, fn, gn, u11, u12, u13, u21, u22, u23, u31, u32, u33, &
+                           scalar, sclx, scly, sclz, scn, Lu, Lv, Lw)
+    complex(8) :: u(*), v(*), w(*), omx(*), omy(*), omz(*), fn(*), gn(*), Lu(*), Lv(*), Lw(*)
+    complex(8), target :: u11(*), u12(*), u13(*), u21(*), u22(*), u23(*), u31(*), u32(*), u33(*)
+    complex(8), target :: scalar(*), sclx(*), scly(*), sclz(*), scn(*)
+    type(c_ptr), target :: g(9)
+    g = [c_loc(u11), c_loc(u12), c_loc(u13), c_loc(u21), c_loc(u22), c_loc(u23), c_loc(u31), c_loc(u32), &
+         c_loc(u33)]
+    call kmm_check(kmm_vcw3dp(kmm_ctx_handle, u, v, w, omx, omy, omz, fn, gn, c_loc(g), c_loc(scalar), &
+                              c_loc(sclx), c_loc(scly), c_loc(sclz), c_loc(scn), int(kmm_it, c_int)), 'vcw3dp')
+  end subroutine
+  ! vcw3dp, SCALAR + POLYMER list (the reference's polymer build)
+  subroutine vcw3dp_scalar_polymer(u, v, w, omx, omy, omz, fn, gn, u11, u12, u13, u21, u22, u23, u31, u32, u33, &
+                                   scalar, sclx, scly, sclz, scn, &
+                                   c11, c12, c13, c21, c22, c23, c31, c32, c33, &
+                                   dc111, dc112, dc113, dc211, dc212, dc213, dc311, dc312, dc313, &
+                                   dc121, dc122, dc123, dc221, dc222, dc223, dc321, dc322, dc323, &
+                                   dc131, dc132, dc133, dc231, dc232, dc233, dc331, dc332, dc333, &
+                                   c11n, c12n, c13n, c22n, c23n, c33n, &
+                                   str11n, str12n, str13n, str22n, str23n, str33n, &
+                                   qp11, qp12, qp13, qp22, qp23, qp33, Lu, Lv, Lw)
+    complex(8) :: u(*), v(*), w(*), omx(*), omy(*), omz(*), fn(*), gn(*), Lu(*), Lv(*), Lw(*)
+    complex(8), target :: u11(*), u12(*), u13(*), u21(*), u22(*), u23(*), u31(*), u32(*), u33(*)
+    complex(8), target :: scalar(*), sclx(*), scly(*), sclz(*), scn(*)
+    complex(8), target :: c11(*), c12(*), c13(*), c21(*), c22(*), c23(*), c31(*), c32(*), c33(*)
+    complex(8), target :: dc111(*), dc112(*), dc113(*), dc211(*), dc212(*), dc213(*), dc311(*), dc312(*), dc313(*)
+    complex(8), target :: dc121(*), dc122(*), dc123(*), dc221(*), dc222(*), dc223(*), dc321(*), dc322(*), dc323(*)
+    complex(8), target :: dc131(*), dc132(*), dc133(*), dc231(*), dc232(*), dc233(*), dc331(*), dc332(*), dc333(*)
+    complex(8), target :: c11n(*), c12n(*), c13n(*), c22n(*), c23n(*), c33n(*)
+    complex(8), target :: str11n(*), str12n(*), str13n(*), str22n(*), str23n(*), str33n(*)
+    complex(8), target :: qp11(*), qp12(*), qp13(*), qp22(*), qp23(*), qp33(*)
+    type(c_ptr) :: g(9), c(9), d(27), cn(6), sn(6), q(6)
+    g = [c_loc(u11), c_loc(u12), c_loc(u13), c_loc(u21), c_loc(u22), c_loc(u23), c_loc(u31), c_loc(u32), &
+         c_loc(u33)]
+    c = [c_loc(c11), c_loc(c12), c_loc(c13), c_loc(c21), c_loc(c22), c_loc(c23), c_loc(c31), c_loc(c32), &
+         c_loc(c33)]
+    d = [c_loc(dc111), c_loc(dc112), c_loc(dc113), c_loc(dc211), c_loc(dc212), c_loc(dc213), c_loc(dc311), &
+         c_loc(dc312), c_loc(dc313), c_loc(dc121), c_loc(dc122), c_loc(dc123), c_loc(dc221), c_loc(dc222), &
+         c_loc(dc223), c_loc(dc321), c_loc(dc322), c_loc(dc323), c_loc(dc131), c_loc(dc132), c_loc(dc133), &
+         c_loc(dc231), c_loc(dc232), c_loc(dc233), c_loc(dc331), c_loc(dc332), c_loc(dc333)]
+    cn = [c_loc(c11n), c_loc(c12n), c_loc(c13n), c_loc(c22n), c_loc(c23n), c_loc(c33n)]
+    sn = [c_loc(str11n), c_loc(str12n), c_loc(str13n), c_loc(str22n), c_loc(str23n), c_loc(str33n)]
+    q = [c_loc(qp11), c_loc(qp12), c_loc(qp13), c_loc(qp22), c_loc(qp23), c_loc(qp33)]
+    call kmm_check(kmm_vcw3dp_poly(kmm_ctx_handle, u, v, w, omx, omy, omz, fn, gn, g, c_loc(scalar), &
+                                   c_loc(sclx), c_loc(scly), c_loc(sclz), c_loc(scn), c, d, cn, sn, q, &
+                                   int(kmm_it, c_int)), 'vcw3dp')
   end subroutine
 end module solvers
 

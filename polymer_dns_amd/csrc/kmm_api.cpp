@@ -155,6 +155,11 @@ struct kmm_ctx {
   // per-routine entries (kmm_phirhs ...): 9 FP64 spectral fields and four
   // mean profiles, allocated at the first call and kept until kmm_destroy
   double2* rt = nullptr;
+  // batched ensemble launches (kmm_ensemble_step): the argument blocks of
+  // the steps of one chunk for every channel of the group this context leads
+  void* batch_args = nullptr;
+  size_t batch_cap = 0;
+  long long batched_steps = 0;  // steps this channel ran in shared launches
 };
 
 static void drop_graph(kmm_ctx* c);
@@ -1554,6 +1559,69 @@ extern "C" int kmm_step(kmm_ctx* c, int nsteps) {
   return KMM_OK;
 }
 
+// A channel whose steady step is just the pencil kernel and the plane stage
+// (no polymer, particles, IB cells, scalar source window, closure, timing
+// or per-step bookkeeping): such channels of one size can share launches.
+static bool batchable(const kmm_ctx* c) {
+  return !c->cfg.polymer && c->npart == 0 && c->nsrc == 0 && !c->scsrc && !c->closure && !c->poly_capture &&
+         !c->timing && !inertial(c) && c->nl_valid;
+}
+static bool same_shape(const kmm_ctx* a, const kmm_ctx* b) {
+  return a->nx == b->nx && a->ny == b->ny && a->nz == b->nz && a->cfg.precision == b->cfg.precision &&
+         a->cfg.scalar == b->cfg.scalar && a->tune_pencil == b->tune_pencil && a->tune_xplane == b->tune_xplane;
+}
+
+// n steps of the channels g (same shape, all batchable) with one launch per
+// stage and step on the first channel's stream; each channel keeps its own
+// CFL slots, checked after the chunk.
+static int batch_chunk(const std::vector<kmm_ctx*>& g, int n) {
+  kmm_ctx* c0 = g[0];
+  const int M = (int)g.size();
+  std::vector<PencilArgs> pa((size_t)n * M);
+  std::vector<PlaneArgs> la((size_t)n * M);
+  std::vector<int> it0(M);
+  for (int m = 0; m < M; ++m) {
+    kmm_ctx* c = g[m];
+    // the channel's earlier work on its own stream ends before the group's
+    KMM_HIP(hipMemsetAsync(c->cfl_d, 0, n * sizeof(unsigned long long), c->stream));
+    KMM_HIP(hipStreamSynchronize(c->stream));
+    it0[m] = c->it + 1;
+    for (int s = 0; s < n; ++s) {
+      c->it += 1;
+      pa[(size_t)s * M + m] = pencil_args(c);
+      la[(size_t)s * M + m] = plane_args(c, c->it, c->cfl_d + s);
+    }
+  }
+  const size_t pb = pa.size() * sizeof(PencilArgs), lb = la.size() * sizeof(PlaneArgs);
+  if (c0->batch_cap < pb + lb) {
+    int r = dalloc(c0, &c0->batch_args, pb + lb);
+    if (r) return r;
+    c0->batch_cap = pb + lb;
+  }
+  PencilArgs* dpa = static_cast<PencilArgs*>(c0->batch_args);
+  PlaneArgs* dla = reinterpret_cast<PlaneArgs*>(static_cast<char*>(c0->batch_args) + pb);
+  KMM_HIP(hipMemcpyAsync(dpa, pa.data(), pb, hipMemcpyHostToDevice, c0->stream));
+  KMM_HIP(hipMemcpyAsync(dla, la.data(), lb, hipMemcpyHostToDevice, c0->stream));
+  for (int s = 0; s < n; ++s) {
+    hipError_t e = kmm::launch_pencil_step_batch(c0->ny, pa[(size_t)s * M], dpa + (size_t)s * M, M, c0->stream);
+    if (e == hipSuccess) e = kmm::launch_plane_batch(la[(size_t)s * M], dla + (size_t)s * M, M, c0->stream);
+    if (e != hipSuccess) return fail(KMM_E_HIP, std::string("batched ensemble step: ") + hipGetErrorString(e));
+  }
+  KMM_HIP(hipStreamSynchronize(c0->stream));  // also keeps pa/la alive until their copies are done
+  for (int m = 0; m < M; ++m) {
+    g[m]->batched_steps += n;
+    int r = check_cfl(g[m], n, it0[m]);
+    if (r) return r;
+  }
+  return KMM_OK;
+}
+
+extern "C" int kmm_batched_steps(kmm_ctx* c, long long* n) {
+  if (!c || !n) return fail(KMM_E_ARG, "null argument");
+  *n = c->batched_steps;
+  return KMM_OK;
+}
+
 extern "C" int kmm_ensemble_step(kmm_ctx** ctxs, int n, int nsteps) {
   if (!ctxs || n <= 0 || nsteps < 0) return fail(KMM_E_ARG, "bad argument");
   for (int m = 0; m < n; ++m) {
@@ -1566,13 +1634,40 @@ extern "C" int kmm_ensemble_step(kmm_ctx** ctxs, int n, int nsteps) {
   std::vector<StepChunk> ch(n);
   for (;;) {
     bool any = false;
-    // queue one chunk on every member's stream, then wait for them in turn
+    // groups of batchable members of one shape, each with at least two
+    // members: one chunk of their steps in shared launches
+    std::vector<int> grouped(n, 0);
+    std::vector<std::vector<int>> groups;
+    for (int m = 0; m < n; ++m) {
+      if (grouped[m] || done[m] >= nsteps || !batchable(ctxs[m])) continue;
+      std::vector<int> gi{m};
+      for (int q = m + 1; q < n; ++q)
+        if (!grouped[q] && done[q] == done[m] && batchable(ctxs[q]) && same_shape(ctxs[m], ctxs[q])) gi.push_back(q);
+      if (gi.size() < 2) continue;
+      for (int q : gi) grouped[q] = 1;
+      groups.push_back(gi);
+    }
+    // the other members: one chunk queued on every member's stream
     for (int m = 0; m < n; ++m) {
       ch[m].n = 0;
-      if (done[m] >= nsteps) continue;
+      if (done[m] >= nsteps || grouped[m]) continue;
       any = true;
       int r = step_chunk_enqueue(ctxs[m], nsteps - done[m], ch[m]);
       if (r) return r;
+    }
+    for (const auto& gi : groups) {
+      any = true;
+      const int k = std::min(nsteps - done[gi[0]], kEagerChunk);
+      std::vector<kmm_ctx*> g;
+      for (int q : gi) g.push_back(ctxs[q]);
+      int r = batch_chunk(g, k);
+      if (r) {
+        char msg[64];
+        std::snprintf(msg, sizeof msg, " (ensemble group led by member %d)", gi[0]);
+        g_last_error += msg;
+        return r;
+      }
+      for (int q : gi) done[q] += k;
     }
     if (!any) break;
     for (int m = 0; m < n; ++m) {
@@ -1670,16 +1765,27 @@ extern "C" int kmm_write_restart(kmm_ctx* c, const char* path) {
 
 extern "C" int kmm_read_restart(kmm_ctx* c, const char* path) {
   if (!c || !path) return fail(KMM_E_ARG, "null argument");
-  const size_t fs = c->S * 2, ms = c->nyp;
-  std::vector<double> buf(6 * fs + 6 * ms);
+  // irstrt = 2 (main_dns.f90:1748-1809): the file holds the channel at half
+  // the streamwise resolution (nxh/2 modes, mx/2 columns of the IB
+  // integrals, the other sizes equal); it fills the low kx modes and the
+  // first x columns, the rest starts at zero.  In the reference layouts
+  // (nyp, nz, nxh) and (nyp, mz, mx), x slowest, that is a prefix of each array.
+  const bool half = c->cfg.irstrt == 2;
+  if (half && (c->nxh % 2 || c->mx % 2)) return fail(KMM_E_CONFIG, "irstrt = 2 needs nx/2 and 3 nx/2 even");
+  const size_t fs = c->S * 2, ms = c->nyp, fsf = half ? fs / 2 : fs;
+  const size_t psz = c->fint_full[0].size(), psf = half ? psz / 2 : psz;
+  std::vector<double> rec(6 * fsf + 6 * ms);
   FILE* f = fopen(path, "rb");
   if (!f) return fail(KMM_E_IO, std::string("cannot open ") + path);
-  std::vector<double> p0(c->fint_full[0].size()), p1(p0.size()), p2(p0.size());
-  const size_t pb = p0.size() * sizeof(double);
-  bool ok = read_record(f, {{buf.data(), buf.size() * sizeof(double)}}) &&
+  std::vector<double> p0(psz, 0.0), p1(psz, 0.0), p2(psz, 0.0);
+  const size_t pb = psf * sizeof(double);
+  bool ok = read_record(f, {{rec.data(), rec.size() * sizeof(double)}}) &&
             read_record(f, {{p0.data(), pb}, {p1.data(), pb}, {p2.data(), pb}});
   fclose(f);
   if (!ok) return fail(KMM_E_IO, std::string("restart file ") + path + " has the wrong size or record markers");
+  std::vector<double> buf(6 * fs + 6 * ms, 0.0);
+  for (int q = 0; q < 6; ++q) std::copy(rec.begin() + q * fsf, rec.begin() + (q + 1) * fsf, buf.begin() + q * fs);
+  std::copy(rec.begin() + 6 * fsf, rec.end(), buf.begin() + 6 * fs);
   c->nl_valid = false;
   const int ids[6] = {KMM_V, KMM_FN, KMM_FNM1, KMM_OMY, KMM_GN, KMM_GNM1};
   for (int q = 0; q < 6; ++q) {

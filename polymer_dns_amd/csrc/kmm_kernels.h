@@ -307,6 +307,12 @@ hipError_t launch_plane(const PlaneArgs& a, hipStream_t s);
 hipError_t launch_plane_fwd_only(const PlaneArgs& a, hipStream_t s);
 // ev[0..3] recorded after k_zinv, k_ibdrag, k_xplane, k_zfwd
 hipError_t launch_plane_timed(const PlaneArgs& a, hipStream_t s, hipEvent_t* ev);
+// one step of nmem channels of the same size in one launch per stage: dA
+// (device) holds each channel's arguments; a0 is a host copy of the first
+// (sizes, precision, scalar flag and occupancy variants).  Channels without
+// IB cells (k_ibdrag is not part of the batched plane stage).
+hipError_t launch_pencil_step_batch(int ny, const PencilArgs& a0, const PencilArgs* dA, int nmem, hipStream_t s);
+hipError_t launch_plane_batch(const PlaneArgs& a0, const PlaneArgs* dA, int nmem, hipStream_t s);
 
 // per-routine (parity) kernels
 hipError_t launch_cderiv(int ny, int P, double dyde, const double2* in, double2* out, hipStream_t s);

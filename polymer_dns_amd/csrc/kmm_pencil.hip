@@ -130,13 +130,15 @@ __device__ __forceinline__ PFN<NY> green_load(const double2* __restrict__ G, int
 // SCN, h1n, h3n).
 // TV: occupancy variant picked by the autotuner (0: the compiler's register
 // budget, 2 waves/SIMD at NY = 128; 1: at least 3 waves/SIMD, <= 168 VGPRs)
-template <int NY, bool SCALAR, int MODE, typename V, int TV = 0>
-__global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(TV ? 3 : 1))) void k_pencil(PencilArgs A) {
+// The body of one block of pencils (blk: the block's index along its
+// channel's pencils).
+template <int NY, bool SCALAR, int MODE, typename V>
+__device__ __forceinline__ void pencil_body(const PencilArgs& A, int blk) {
   using PF = PFN<NY>;
   extern __shared__ __align__(16) unsigned char smraw[];
   V* smem = reinterpret_cast<V*>(smraw);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int p = blockIdx.x * PW + wv;
+  const int p = blk * PW + wv;
   const V* twl = lds_twiddles<NY, V>(smem, static_cast<const V*>(A.twM));
   if (p >= A.P) return;
   V* buf = wave_lds<NY, V>(smem, wv);
@@ -382,6 +384,19 @@ __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(TV ? 3 
   }
 }
 
+template <int NY, bool SCALAR, int MODE, typename V, int TV = 0>
+__global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(TV ? 3 : 1))) void k_pencil(PencilArgs A) {
+  pencil_body<NY, SCALAR, MODE, V>(A, blockIdx.x);
+}
+
+// The step kernel of an ensemble of channels of one size in one launch:
+// channel blockIdx.y takes its arguments from AA[blockIdx.y].
+template <int NY, bool SCALAR, typename V, int TV>
+__global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(TV ? 3 : 1)))
+void k_pencil_batch(const PencilArgs* __restrict__ AA) {
+  pencil_body<NY, SCALAR, 0, V>(AA[blockIdx.y], blockIdx.x);
+}
+
 // -------------------------------------------------- parity kernels --
 template <int NY>
 __global__ __launch_bounds__(PW * 64) void k_cderiv(int P, double dyde, const double2* __restrict__ in,
@@ -558,6 +573,27 @@ static hipError_t launch_pencil_mode(int ny, const PencilArgs& a, hipStream_t s)
   return a.mixed ? launch_pencil_mode_v<MODE, float2>(ny, a, s) : launch_pencil_mode_v<MODE, double2>(ny, a, s);
 }
 hipError_t launch_pencil_step(int ny, const PencilArgs& a, hipStream_t s) { return launch_pencil_mode<0>(ny, a, s); }
+
+template <typename V, int TV>
+static hipError_t launch_pencil_batch_v(int ny, const PencilArgs& a0, const PencilArgs* dA, int nmem,
+                                        hipStream_t s) {
+  if (a0.scalar) {
+    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_pencil_batch<NY, true, V, TV>), dim3(nblk(a0.P), nmem),
+                                           dim3(PW * 64), (lds_bytes<NY, V>()), s, dA));
+  } else {
+    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_pencil_batch<NY, false, V, TV>), dim3(nblk(a0.P), nmem),
+                                           dim3(PW * 64), (lds_bytes<NY, V>()), s, dA));
+  }
+  return hipGetLastError();
+}
+hipError_t launch_pencil_step_batch(int ny, const PencilArgs& a0, const PencilArgs* dA, int nmem, hipStream_t s) {
+  if (nmem < 1) return hipErrorInvalidValue;
+  if (a0.mixed)
+    return a0.tune == 1 ? launch_pencil_batch_v<float2, 1>(ny, a0, dA, nmem, s)
+                        : launch_pencil_batch_v<float2, 0>(ny, a0, dA, nmem, s);
+  return a0.tune == 1 ? launch_pencil_batch_v<double2, 1>(ny, a0, dA, nmem, s)
+                      : launch_pencil_batch_v<double2, 0>(ny, a0, dA, nmem, s);
+}
 hipError_t launch_pencil_startup(int ny, const PencilArgs& a, hipStream_t s) { return launch_pencil_mode<1>(ny, a, s); }
 hipError_t launch_pencil_finish(int ny, const PencilArgs& a, hipStream_t s) { return launch_pencil_mode<2>(ny, a, s); }
 

@@ -12,6 +12,7 @@
 //   k_zfwd   : forward c2c along z, keep nz modes, 1/(mx mz), and the +-kz
 //              projection of ``norm`` (helpers.f90:91-158).
 // Intermediate layout Z[f][y][z][kx] keeps the x lines contiguous.
+#include <algorithm>
 #include <cstring>
 #include <utility>
 
@@ -52,7 +53,7 @@ struct ZLds {
 // (y, group) the last round of blocks is a small fraction of the work, where a
 // grid of one block per (y, group) would leave a nearly empty third round.
 template <int MZ, int NFIN, typename V>
-__global__ __launch_bounds__(512) void k_zinv(PlaneArgs A) {
+__device__ __forceinline__ void zinv_body(const PlaneArgs& A, int bid) {
   extern __shared__ __align__(16) unsigned char smraw[];
   V* sm = reinterpret_cast<V*>(smraw);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -71,7 +72,7 @@ __global__ __launch_bounds__(512) void k_zinv(PlaneArgs A) {
     __syncthreads();
     tw = twl;
   }
-  int it = blockIdx.x;
+  int it = bid;
   if (it >= nitems) return;
   auto line = [&](int i) -> const V* {
     const int f = i / ngroups, g = i - f * ngroups;
@@ -117,6 +118,16 @@ __global__ __launch_bounds__(512) void k_zinv(PlaneArgs A) {
     }
     __syncthreads();
   }
+}
+
+template <int MZ, int NFIN, typename V>
+__global__ __launch_bounds__(512) void k_zinv(PlaneArgs A) {
+  zinv_body<MZ, NFIN, V>(A, blockIdx.x);
+}
+// channel blockIdx.y of an ensemble of one size (arguments AA[blockIdx.y])
+template <int MZ, int NFIN, typename V>
+__global__ __launch_bounds__(512) void k_zinv_batch(const PlaneArgs* __restrict__ AA) {
+  zinv_body<MZ, NFIN, V>(AA[blockIdx.y], blockIdx.x);
 }
 
 template <int MX, typename V>
@@ -204,17 +215,16 @@ __device__ __forceinline__ void xrow_pair_load(T (&ax)[NK], T (&ay)[NK], T (&bx)
 // TV: occupancy variant picked by the autotuner (0: the compiler's register
 // budget; 1: one more wave per SIMD than FP64/FP32 need at MX = 192, i.e.
 // >= 3 (FP64) or >= 5 (FP32) waves/SIMD)
-template <int MX, int NFIN, int NFOUT, int WPB, typename V, int TV = 0>
-__global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(TV ? (sizeof(V) == 16 ? 3 : 5) : 1)))
-void k_xplane(PlaneArgs A) {
+template <int MX, int NFIN, int NFOUT, int WPB, typename V>
+__device__ __forceinline__ void xplane_body(const PlaneArgs& A, int bid) {
   using T = typename Scal<V>::T;
   constexpr int XPL = (MX + 63) / 64;  // x positions per lane: x = lane + 64 m
   extern __shared__ __align__(16) unsigned char smraw[];
   V* sm = reinterpret_cast<V*>(smraw);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int nzb = A.mz / (2 * WPB);
-  const int y = blockIdx.x / nzb;
-  const int zA = (blockIdx.x % nzb) * 2 * WPB + 2 * wv, zB = zA + 1;
+  const int y = bid / nzb;
+  const int zA = (bid % nzb) * 2 * WPB + 2 * wv, zB = zA + 1;
   constexpr int BL = FFTBuf<MX>::len;
   V* tw = sm;                              // [MX] twiddles
   V* buf = sm + MX + (size_t)wv * 2 * BL;
@@ -364,6 +374,17 @@ void k_xplane(PlaneArgs A) {
   }
 }
 
+template <int MX, int NFIN, int NFOUT, int WPB, typename V, int TV = 0>
+__global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(TV ? (sizeof(V) == 16 ? 3 : 5) : 1)))
+void k_xplane(PlaneArgs A) {
+  xplane_body<MX, NFIN, NFOUT, WPB, V>(A, blockIdx.x);
+}
+template <int MX, int NFIN, int NFOUT, int WPB, typename V, int TV = 0>
+__global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(TV ? (sizeof(V) == 16 ? 3 : 5) : 1)))
+void k_xplane_batch(const PlaneArgs* __restrict__ AA) {
+  xplane_body<MX, NFIN, NFOUT, WPB, V>(AA[blockIdx.y], blockIdx.x);
+}
+
 // one block's [MZ][8] tile of a z-line group, TL values per thread (real
 // and imaginary parts kept as scalars so that they stay in registers)
 template <int MZ, int TL, typename V, typename T>
@@ -380,7 +401,7 @@ __device__ __forceinline__ void zfwd_tile_load(T (&px)[TL], T (&py)[TL], const V
 
 // Work items (output field, y, group of 8 kx lines) walked like k_zinv's.
 template <int MZ, int NFOUT, typename V>
-__global__ __launch_bounds__(512) void k_zfwd(PlaneArgs A) {
+__device__ __forceinline__ void zfwd_body(const PlaneArgs& A, int bid) {
   using T = typename Scal<V>::T;
   extern __shared__ __align__(16) unsigned char smraw[];
   V* sm = reinterpret_cast<V*>(smraw);
@@ -400,7 +421,7 @@ __global__ __launch_bounds__(512) void k_zfwd(PlaneArgs A) {
     __syncthreads();
     tw = twl;
   }
-  int it = blockIdx.x;
+  int it = bid;
   if (it >= nitems) return;
   const T scale = (T)(1.0 / ((double)A.mx * (double)MZ));
   const int nz = A.nz, nzh = A.nzh;
@@ -454,6 +475,15 @@ __global__ __launch_bounds__(512) void k_zfwd(PlaneArgs A) {
     }
     __syncthreads();
   }
+}
+
+template <int MZ, int NFOUT, typename V>
+__global__ __launch_bounds__(512) void k_zfwd(PlaneArgs A) {
+  zfwd_body<MZ, NFOUT, V>(A, blockIdx.x);
+}
+template <int MZ, int NFOUT, typename V>
+__global__ __launch_bounds__(512) void k_zfwd_batch(const PlaneArgs* __restrict__ AA) {
+  zfwd_body<MZ, NFOUT, V>(AA[blockIdx.y], blockIdx.x);
 }
 
 // Grid of the item-walking z kernels: the blocks the device holds at once
@@ -1058,6 +1088,65 @@ static hipError_t plane_dispatch(const PlaneArgs& a, hipStream_t s, bool inv, hi
   if (a.nxh % 8 != 0) return hipErrorInvalidValue;
 #define KMM_PLANE_CASE(X, Zz) \
   if (a.mx == X && a.mz == Zz) return plane_dispatch_f<X, Zz>(a, s, inv, ev);
+  KMM_PLANE_CASE(48, 48)
+  KMM_PLANE_CASE(96, 48)
+  KMM_PLANE_CASE(96, 96)
+  KMM_PLANE_CASE(144, 144)
+  KMM_PLANE_CASE(192, 96)
+  KMM_PLANE_CASE(192, 192)
+  KMM_PLANE_CASE(288, 288)
+  KMM_PLANE_CASE(384, 192)
+  KMM_PLANE_CASE(384, 384)
+#undef KMM_PLANE_CASE
+  return hipErrorInvalidValue;
+}
+
+// The plane stage of nmem channels of one size: each kernel's grid gains a
+// y dimension over the channels; the item-walking z kernels share the
+// blocks the device holds at once among the channels.
+template <int MX, int MZ, int NFIN, int NFOUT, typename V>
+static hipError_t plane_batch_impl(const PlaneArgs& a, const PlaneArgs* dA, int nmem, hipStream_t s) {
+  const size_t zlds = ZLds<MZ, V>::bytes;
+  {
+    const int nitems = NFIN * a.nyp * (a.nxh / 8);
+    static ZGridCache cache;
+    const int cap = zitems_grid(cache, k_zinv_batch<MZ, NFIN, V>, zlds, 1 << 30);
+    const int grid = std::min(nitems, std::max(1, (cap + nmem - 1) / nmem));
+    hipLaunchKernelGGL((k_zinv_batch<MZ, NFIN, V>), dim3(grid, nmem), dim3(512), zlds, s, dA);
+  }
+  constexpr int WPB = 4;
+  const size_t xlds = (size_t)(MX + WPB * 2 * FFTBuf<MX>::len) * sizeof(V);
+  const dim3 gx(a.nyp * (a.mz / (2 * WPB)), nmem);
+  if (a.tune == 1)
+    hipLaunchKernelGGL((k_xplane_batch<MX, NFIN, NFOUT, WPB, V, 1>), gx, dim3(WPB * 64), xlds, s, dA);
+  else
+    hipLaunchKernelGGL((k_xplane_batch<MX, NFIN, NFOUT, WPB, V, 0>), gx, dim3(WPB * 64), xlds, s, dA);
+  {
+    const int nitems = NFOUT * a.nyp * (a.nxh / 8);
+    static ZGridCache cache;
+    const int cap = zitems_grid(cache, k_zfwd_batch<MZ, NFOUT, V>, zlds, 1 << 30);
+    const int grid = std::min(nitems, std::max(1, (cap + nmem - 1) / nmem));
+    hipLaunchKernelGGL((k_zfwd_batch<MZ, NFOUT, V>), dim3(grid, nmem), dim3(512), zlds, s, dA);
+  }
+  return hipGetLastError();
+}
+
+template <int MX, int MZ>
+static hipError_t plane_batch_f(const PlaneArgs& a, const PlaneArgs* dA, int nmem, hipStream_t s) {
+  if (a.mixed) {
+    if (a.nfin == 6 && a.nfout == 3) return plane_batch_impl<MX, MZ, 6, 3, float2>(a, dA, nmem, s);
+    if (a.nfin == 9 && a.nfout == 4) return plane_batch_impl<MX, MZ, 9, 4, float2>(a, dA, nmem, s);
+  } else {
+    if (a.nfin == 6 && a.nfout == 3) return plane_batch_impl<MX, MZ, 6, 3, double2>(a, dA, nmem, s);
+    if (a.nfin == 9 && a.nfout == 4) return plane_batch_impl<MX, MZ, 9, 4, double2>(a, dA, nmem, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_plane_batch(const PlaneArgs& a, const PlaneArgs* dA, int nmem, hipStream_t s) {
+  if (a.nxh % 8 != 0 || a.nsrc > 0 || nmem < 1) return hipErrorInvalidValue;
+#define KMM_PLANE_CASE(X, Zz) \
+  if (a.mx == X && a.mz == Zz) return plane_batch_f<X, Zz>(a, dA, nmem, s);
   KMM_PLANE_CASE(48, 48)
   KMM_PLANE_CASE(96, 48)
   KMM_PLANE_CASE(96, 96)

@@ -43,7 +43,7 @@ EXPORTS = [
     "kmm_read_particles", "kmm_set_particles", "kmm_particle_count", "kmm_get_particles", "kmm_write_particles",
     "kmm_fluid_interp", "kmm_get_physical", "kmm_get_dpdx", "kmm_set_dpdx",
     "kmm_gradv", "kmm_calcswirl", "kmm_get_particle_swirl", "kmm_set_graph",
-    "kmm_ensemble_step",
+    "kmm_ensemble_step", "kmm_batched_steps",
     "kmm_mlp_param_count", "kmm_mlp_create", "kmm_mlp_destroy", "kmm_mlp_set_params", "kmm_mlp_get_params",
     "kmm_mlp_set_norm", "kmm_mlp_infer", "kmm_mlp_grad", "kmm_mlp_train",
     "kmm_set_poly_capture", "kmm_poly_samples", "kmm_set_closure", "kmm_get_poly_stress",
@@ -155,6 +155,7 @@ def load_library(path: str | None = None):
     lib.kmm_get_particle_swirl.argtypes = [C.c_void_p, C.c_int, dp]
     lib.kmm_set_graph.argtypes = [C.c_void_p, C.c_int]
     lib.kmm_ensemble_step.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int]
+    lib.kmm_batched_steps.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
     fp = C.POINTER(C.c_float)
     lib.kmm_mlp_param_count.argtypes = []
     lib.kmm_mlp_create.argtypes = [C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]
@@ -321,6 +322,13 @@ class Channel:
         kernel only; False / 0: off."""
         mode = on if isinstance(on, int) and not isinstance(on, bool) else int(bool(on))
         _check(self.lib, self.lib.kmm_set_timing(self.h, mode))
+
+    def batched_steps(self):
+        """Steps of this channel that ran in an ensemble group's shared
+        launches (kmm_batched_steps)."""
+        n = C.c_longlong()
+        _check(self.lib, self.lib.kmm_batched_steps(self.h, C.byref(n)))
+        return n.value
 
     def kernel_times(self):
         """Average ms per step of each kernel group over the last step() call."""
@@ -655,8 +663,9 @@ class Channel:
 
 
 def ensemble_step(channels, nsteps):
-    """Advance independent channels together (kmm_ensemble_step): each round
-    queues work on every member's stream before waiting for any of them."""
+    """Advance independent channels together (kmm_ensemble_step): members of
+    one shape with plain channel steps share one launch per stage; the
+    others queue work on their own streams before any is waited for."""
     lib = load_library()
     arr = (C.c_void_p * len(channels))(*[ch.h for ch in channels])
     _check(lib, lib.kmm_ensemble_step(arr, len(channels), int(nsteps)))

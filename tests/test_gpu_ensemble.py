@@ -1,8 +1,9 @@
 """The 16-member ensemble of independent channels of uneven size (nx = nz
 in {64, 96, 128}, ny in {48, 64, 80, 96}, Re_tau uniform in [120, 240], all
 seeded; mixed precision; steady steps replayed as HIP graphs) advanced
-together by kmm_ensemble_step: every member equals the same channel run
-alone, bit for bit, and the smallest (64 x 49 x 64) and the largest
+together by kmm_ensemble_step (members of one size in shared launches, one
+per stage and step): every member equals the same channel run alone, bit
+for bit, and the smallest (64 x 49 x 64) and the largest
 (128 x 97 x 128) member stay within 1e-4 of the FP64 oracle."""
 import math
 
@@ -51,6 +52,10 @@ def test_sixteen_member_ensemble():
     kmm.ensemble_step(chans, 12)
     for ch in alone:
         ch.step(12)
+    # members sharing a shape ran in their group's shared launches
+    shapes = [m[:3] for m in mem]
+    for ch, sh in zip(chans, shapes):
+        assert ch.batched_steps() == (12 if shapes.count(sh) > 1 else 0), sh
     for a, b in zip(chans, alone):
         for k in ("v", "omy", "fn", "gn", "u0"):
             assert np.array_equal(a.get(k), b.get(k)), k
