@@ -79,8 +79,16 @@ AP = 1e-3   # tracer radius: keeps the wall clamp of tracer_update inside the gr
 # the two viscoelastic workloads: Re_tau 180 at 192 x 129 x 192 with 100 000
 # tracers, and Re_tau 395 (Re_bulk 6875 in half-height units) at
 # 256 x 193 x 256 with the same FENE-P parameters
-CASE_180 = dict(nx=PNX, ny=PNY, nz=PNZ, re=RE, rtau=RTAU, dt=DT, npart=NPART, seed=SEED)
-CASE_395 = dict(nx=256, ny=192, nz=256, re=6875.0, rtau=395.0, dt=0.001, npart=0, seed=SEED + 20)
+CASE_180 = dict(nx=PNX, ny=PNY, nz=PNZ, re=RE, rtau=RTAU, dt=DT, npart=NPART, seed=SEED, steps=None)
+# Re_tau 395: dt 5e-4 and 100 timed steps (its FENE-P start at Wi_tau 50
+# with the smaller diffusivity 1/(Sc Re) leaves CFL < 1 for about 0.14 time
+# units at this grid; the line runs 0.11)
+CASE_395 = dict(nx=256, ny=192, nz=256, re=6875.0, rtau=395.0, dt=5e-4, npart=0, seed=SEED + 20, steps=100)
+
+
+def progress(msg):
+    """A progress line on stderr (the JSON result is the one stdout line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def polymer_mode(kmm, precision, steps, warmup, learned=False, audit_every=AUDIT_EVERY, case=CASE_180):
@@ -172,8 +180,18 @@ def polymer_lines(kmm, steps, case, dense):
     """FENE-P in FP64 and mixed precision, and the learned closure (mixed)
     at the specified audit cadence (and, with dense, every AUDIT_DENSE
     steps), on one viscoelastic workload."""
-    pf = polymer_mode(kmm, 0, steps, 5, case=case)
-    pm = polymer_mode(kmm, 1, steps, 5, case=case)
+    steps = min(steps, case["steps"]) if case.get("steps") else steps
+    tag = f"{case['nx']}x{case['ny'] + 1}x{case['nz']}"
+
+    def line(precision):
+        try:
+            return polymer_mode(kmm, precision, steps, 5, case=case)
+        except kmm.KmmError as e:   # a run that leaves the solver's range is reported, not fatal
+            return {"error": str(e), "value": None, "ms_per_step": None, "_stats": None}
+    pf = line(0)
+    progress(f"{tag} FENE-P FP64: {pf['ms_per_step']} ms/step {pf.get('error') or ''}")
+    pm = line(1)
+    progress(f"{tag} FENE-P mixed: {pm['ms_per_step']} ms/step {pm.get('error') or ''}")
     b = pm.pop("_stats")
     pf.pop("_stats")
     learned = {}
@@ -182,17 +200,21 @@ def polymer_lines(kmm, steps, case, dense):
         try:
             pl = polymer_mode(kmm, 1, steps, 5, learned=True, audit_every=every, case=case)
             a = pl.pop("_stats")
-            pl["profile_error_vs_fenep_mixed"] = {
-                k: round(float(np.abs(a[k] - b[k]).max() / np.abs(b[k]).max()), 6)
-                for k in ("umean", "urms", "rss", "pss")}
+            if b is not None:
+                pl["profile_error_vs_fenep_mixed"] = {
+                    k: round(float(np.abs(a[k] - b[k]).max() / np.abs(b[k]).max()), 6)
+                    for k in ("umean", "urms", "rss", "pss")}
         except kmm.KmmError as e:   # a learned run that leaves the solver's range is reported, not fatal
             pl = {"error": str(e), "value": None, "ms_per_step": None, "audit_every": every}
         learned[name] = pl
+        progress(f"{tag} {name}: {pl.get('ms_per_step')} ms/step {pl.get('error') or ''}")
     nx, ny, nz = case["nx"], case["ny"], case["nz"]
 
-    def speedup(ref, line):
-        return round(ref["ms_per_step"] / line["ms_per_step"], 3) if line.get("ms_per_step") else None
-    out = {"workload": f"viscoelastic channel {nx}x{ny + 1}x{nz}, Re_tau {case['rtau']:g}, FENE-P Wi_tau "
+    def speedup(ref, ln):
+        ok = ref.get("ms_per_step") and ln.get("ms_per_step")
+        return round(ref["ms_per_step"] / ln["ms_per_step"], 3) if ok else None
+    out = {"workload": f"viscoelastic channel {nx}x{ny + 1}x{nz}, Re_tau {case['rtau']:g}, dt {case['dt']:g}, "
+                       f"{steps} timed steps, FENE-P Wi_tau "
                        f"{WI_TAU:g}, beta {QBETA}, L^2 {ZLMAX ** 2:g}, Sc {SC}, {case['npart']} tracers, "
                        "conformation started at the mean-shear equilibrium",
            "unit": "Mgridpt-steps/s", "fp64": pf, "mixed": pm, **learned,
@@ -346,7 +368,9 @@ def main():
     from polymer_dns_amd import kmm
     G = NX * (NY + 1) * NZ
     t, ktimes, cfl, it, tune = run_mode(kmm, 0, args.steps, args.warmup)
+    progress(f"headline FP64: {1000.0 * t / args.steps:.4f} ms/step")
     tm, ktm, _, _, tunem = run_mode(kmm, 1, args.steps, args.warmup)
+    progress(f"headline mixed: {1000.0 * tm / args.steps:.4f} ms/step")
     ms = 1000.0 * t / args.steps
     value = G * args.steps / t / 1e6
     kb = kernel_bytes(G)
@@ -376,9 +400,12 @@ def main():
     }
     if args.poly_steps > 0:
         out["polymer"] = polymer_lines(kmm, args.poly_steps, CASE_180, dense=True)
+        progress("viscoelastic Re_tau 180 lines done")
         out["polymer_re395"] = polymer_lines(kmm, args.poly_steps, CASE_395, dense=False)
+        progress("viscoelastic Re_tau 395 lines done")
     if args.ensemble_steps > 0:
         out["ensemble"] = ensemble_mode(kmm, args.ensemble_steps, 10)
+        progress("ensemble done")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

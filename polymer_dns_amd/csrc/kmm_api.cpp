@@ -159,6 +159,7 @@ struct kmm_ctx {
   // the steps of one chunk for every channel of the group this context leads
   void* batch_args = nullptr;
   size_t batch_cap = 0;
+  std::vector<char> batch_host;
   long long batched_steps = 0;  // steps this channel ran in shared launches
 };
 
@@ -1572,27 +1573,35 @@ static bool same_shape(const kmm_ctx* a, const kmm_ctx* b) {
 }
 
 // n steps of the channels g (same shape, all batchable) with one launch per
-// stage and step on the first channel's stream; each channel keeps its own
-// CFL slots, checked after the chunk.
-static int batch_chunk(const std::vector<kmm_ctx*>& g, int n) {
-  kmm_ctx* c0 = g[0];
-  const int M = (int)g.size();
-  std::vector<PencilArgs> pa((size_t)n * M);
-  std::vector<PlaneArgs> la((size_t)n * M);
-  std::vector<int> it0(M);
+// stage and step on the first channel's stream, queued without waiting
+// (batch_finish waits); each channel keeps its own CFL slots.  The argument
+// blocks are staged in the first channel's host buffer, which lives until
+// its next chunk.
+struct BatchChunk {
+  std::vector<kmm_ctx*> g;
+  int n = 0;
+  std::vector<int> it0;
+};
+static int batch_enqueue(BatchChunk& b) {
+  kmm_ctx* c0 = b.g[0];
+  const int M = (int)b.g.size(), n = b.n;
+  const size_t pb = (size_t)n * M * sizeof(PencilArgs), lb = (size_t)n * M * sizeof(PlaneArgs);
+  c0->batch_host.resize(pb + lb);
+  PencilArgs* pa = reinterpret_cast<PencilArgs*>(c0->batch_host.data());
+  PlaneArgs* la = reinterpret_cast<PlaneArgs*>(c0->batch_host.data() + pb);
+  b.it0.assign(M, 0);
   for (int m = 0; m < M; ++m) {
-    kmm_ctx* c = g[m];
+    kmm_ctx* c = b.g[m];
     // the channel's earlier work on its own stream ends before the group's
     KMM_HIP(hipMemsetAsync(c->cfl_d, 0, n * sizeof(unsigned long long), c->stream));
     KMM_HIP(hipStreamSynchronize(c->stream));
-    it0[m] = c->it + 1;
+    b.it0[m] = c->it + 1;
     for (int s = 0; s < n; ++s) {
       c->it += 1;
       pa[(size_t)s * M + m] = pencil_args(c);
       la[(size_t)s * M + m] = plane_args(c, c->it, c->cfl_d + s);
     }
   }
-  const size_t pb = pa.size() * sizeof(PencilArgs), lb = la.size() * sizeof(PlaneArgs);
   if (c0->batch_cap < pb + lb) {
     int r = dalloc(c0, &c0->batch_args, pb + lb);
     if (r) return r;
@@ -1600,17 +1609,19 @@ static int batch_chunk(const std::vector<kmm_ctx*>& g, int n) {
   }
   PencilArgs* dpa = static_cast<PencilArgs*>(c0->batch_args);
   PlaneArgs* dla = reinterpret_cast<PlaneArgs*>(static_cast<char*>(c0->batch_args) + pb);
-  KMM_HIP(hipMemcpyAsync(dpa, pa.data(), pb, hipMemcpyHostToDevice, c0->stream));
-  KMM_HIP(hipMemcpyAsync(dla, la.data(), lb, hipMemcpyHostToDevice, c0->stream));
+  KMM_HIP(hipMemcpyAsync(c0->batch_args, c0->batch_host.data(), pb + lb, hipMemcpyHostToDevice, c0->stream));
   for (int s = 0; s < n; ++s) {
     hipError_t e = kmm::launch_pencil_step_batch(c0->ny, pa[(size_t)s * M], dpa + (size_t)s * M, M, c0->stream);
     if (e == hipSuccess) e = kmm::launch_plane_batch(la[(size_t)s * M], dla + (size_t)s * M, M, c0->stream);
     if (e != hipSuccess) return fail(KMM_E_HIP, std::string("batched ensemble step: ") + hipGetErrorString(e));
   }
-  KMM_HIP(hipStreamSynchronize(c0->stream));  // also keeps pa/la alive until their copies are done
-  for (int m = 0; m < M; ++m) {
-    g[m]->batched_steps += n;
-    int r = check_cfl(g[m], n, it0[m]);
+  return KMM_OK;
+}
+static int batch_finish(const BatchChunk& b) {
+  KMM_HIP(hipStreamSynchronize(b.g[0]->stream));
+  for (size_t m = 0; m < b.g.size(); ++m) {
+    b.g[m]->batched_steps += b.n;
+    int r = check_cfl(b.g[m], b.n, b.it0[m]);
     if (r) return r;
   }
   return KMM_OK;
@@ -1655,21 +1666,25 @@ extern "C" int kmm_ensemble_step(kmm_ctx** ctxs, int n, int nsteps) {
       int r = step_chunk_enqueue(ctxs[m], nsteps - done[m], ch[m]);
       if (r) return r;
     }
-    for (const auto& gi : groups) {
+    std::vector<BatchChunk> bc(groups.size());
+    for (size_t q = 0; q < groups.size(); ++q) {
       any = true;
-      const int k = std::min(nsteps - done[gi[0]], kEagerChunk);
-      std::vector<kmm_ctx*> g;
-      for (int q : gi) g.push_back(ctxs[q]);
-      int r = batch_chunk(g, k);
+      bc[q].n = std::min(nsteps - done[groups[q][0]], kEagerChunk);
+      for (int m : groups[q]) bc[q].g.push_back(ctxs[m]);
+      int r = batch_enqueue(bc[q]);
+      if (r) return r;
+    }
+    if (!any) break;
+    for (size_t q = 0; q < groups.size(); ++q) {
+      int r = batch_finish(bc[q]);
       if (r) {
         char msg[64];
-        std::snprintf(msg, sizeof msg, " (ensemble group led by member %d)", gi[0]);
+        std::snprintf(msg, sizeof msg, " (ensemble group led by member %d)", groups[q][0]);
         g_last_error += msg;
         return r;
       }
-      for (int q : gi) done[q] += k;
+      for (int m : groups[q]) done[m] += bc[q].n;
     }
-    if (!any) break;
     for (int m = 0; m < n; ++m) {
       if (!ch[m].n) continue;
       int r = step_chunk_finish(ctxs[m], ch[m]);
