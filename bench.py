@@ -229,15 +229,10 @@ def polymer_lines(kmm, steps, case, dense):
 ENS_SEED = 20261202 + 4
 
 
-def ensemble_mode(kmm, steps, warmup):
-    """The 16-member ensemble of uneven channels (nx = nz in {64, 96, 128},
-    ny in {48, 64, 80, 96}, Re_tau in [120, 240], seeded perturbed laminar
-    starts), mixed precision, advanced together by kmm_ensemble_step with
-    steady steps replayed as graphs; value = grid points of all members x
-    steps / wall seconds."""
+def _ensemble_run(kmm, members, steps, warmup, batch):
     from polymer_dns_amd import ic
     chans, G = [], 0
-    for nx, ny, nz, rt, seed in ic.ensemble_members(ENS_SEED):
+    for nx, ny, nz, rt, seed in members:
         cfg = kmm.make_config(nx, ny, nz, xl=XL, zl=ZL, re=RE, dt=DT, r_tau=rt, precision=1)
         ch = kmm.Channel(cfg)
         st = ic.seeded_state(nx, ny, nz, XL, ZL, RE, rt, seed, amp=0.05, kmax=8)
@@ -247,18 +242,44 @@ def ensemble_mode(kmm, steps, warmup):
         ch.set_graph(10)
         chans.append(ch)
         G += nx * (ny + 1) * nz
-    kmm.ensemble_step(chans, warmup)
+    kmm.ensemble_step(chans, warmup, batch=batch)
     t0 = time.perf_counter()
-    kmm.ensemble_step(chans, steps)
+    kmm.ensemble_step(chans, steps, batch=batch)
     t = time.perf_counter() - t0
-    members = sorted({(c.nx, c.nyp, c.nz) for c in chans})
+    shared = sum(1 for ch in chans if ch.batched_steps() > 0)
     for ch in chans:
         ch.close()
+    return G, t, shared
+
+
+def ensemble_mode(kmm, steps, warmup):
+    """The 16-member ensemble of uneven channels (nx = nz in {64, 96, 128},
+    ny in {48, 64, 80, 96}, Re_tau in [120, 240], seeded perturbed laminar
+    starts), mixed precision, advanced together by kmm_ensemble_step: members
+    of one size in shared launches (one per stage and step), the others on
+    their own streams with steady steps replayed as graphs; value = grid
+    points of all members x steps / wall seconds.  Also the same set with
+    every member on its own stream, and 16 channels of one size (64x49x64,
+    Re_tau as the set's) both ways."""
+    from polymer_dns_amd import ic
+    members = ic.ensemble_members(ENS_SEED)
+    G, t, shared = _ensemble_run(kmm, members, steps, warmup, True)
+    _, t1, _ = _ensemble_run(kmm, members, steps, warmup, False)
+    small = [(64, 48, 64, m[3], m[4]) for m in members]
+    Gs, ts, shs = _ensemble_run(kmm, small, steps, warmup, True)
+    _, ts1, _ = _ensemble_run(kmm, small, steps, warmup, False)
+    sizes = sorted({(m[0], m[1] + 1, m[2]) for m in members})
     return {"workload": "16 independent channels of uneven size, nx = nz in {64, 96, 128}, "
-                        "ny + 1 in {49, 65, 81, 97}, Re_tau in [120, 240], mixed precision, graphs of 10 steps",
-            "sizes": [list(m) for m in members], "gridpts": G, "steps": steps,
+                        "ny + 1 in {49, 65, 81, 97}, Re_tau in [120, 240], mixed precision",
+            "sizes": [list(m) for m in sizes], "gridpts": G, "steps": steps,
             "value": round(G * steps / t / 1e6, 2), "unit": "Mgridpt-steps/s (all members)",
-            "ms_per_step": round(1000.0 * t / steps, 4)}
+            "ms_per_step": round(1000.0 * t / steps, 4), "members_in_shared_launches": shared,
+            "own_streams": {"value": round(G * steps / t1 / 1e6, 2), "ms_per_step": round(1000.0 * t1 / steps, 4)},
+            "uniform_64x49x64": {"members": 16, "shared_launches": {"value": round(Gs * steps / ts / 1e6, 2),
+                                                                  "ms_per_step": round(1000.0 * ts / steps, 4),
+                                                                  "members_in_shared_launches": shs},
+                                 "own_streams": {"value": round(Gs * steps / ts1 / 1e6, 2),
+                                                 "ms_per_step": round(1000.0 * ts1 / steps, 4)}}}
 
 
 def kernel_bytes(G):

@@ -166,6 +166,10 @@ int kmm_calcswirl(kmm_ctx* ctx, const double* u11, const double* u21, const doub
  * own streams, so groups and members run concurrently.  Returns the first
  * error; its message names the member or group.  No reference counterpart. */
 int kmm_ensemble_step(kmm_ctx** ctxs, int n, int nsteps);
+/* kmm_ensemble_step with flags: KMM_ENSEMBLE_NO_BATCH runs every member on
+ * its own stream (no shared launches). */
+#define KMM_ENSEMBLE_NO_BATCH 1
+int kmm_ensemble_step_ex(kmm_ctx** ctxs, int n, int nsteps, int flags);
 /* Number of this channel's steps that ran in a group's shared launches. */
 int kmm_batched_steps(kmm_ctx* ctx, long long* n);
 

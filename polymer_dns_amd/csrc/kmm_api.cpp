@@ -1633,8 +1633,14 @@ extern "C" int kmm_batched_steps(kmm_ctx* c, long long* n) {
   return KMM_OK;
 }
 
+extern "C" int kmm_ensemble_step_ex(kmm_ctx** ctxs, int n, int nsteps, int flags);
 extern "C" int kmm_ensemble_step(kmm_ctx** ctxs, int n, int nsteps) {
-  if (!ctxs || n <= 0 || nsteps < 0) return fail(KMM_E_ARG, "bad argument");
+  return kmm_ensemble_step_ex(ctxs, n, nsteps, 0);
+}
+
+extern "C" int kmm_ensemble_step_ex(kmm_ctx** ctxs, int n, int nsteps, int flags) {
+  if (!ctxs || n <= 0 || nsteps < 0 || (flags & ~KMM_ENSEMBLE_NO_BATCH)) return fail(KMM_E_ARG, "bad argument");
+  const bool allow_batch = !(flags & KMM_ENSEMBLE_NO_BATCH);
   for (int m = 0; m < n; ++m) {
     if (!ctxs[m]) return fail(KMM_E_ARG, "null member");
     if (m && ctxs[m]->device != ctxs[0]->device) return fail(KMM_E_ARG, "ensemble members must share a device");
@@ -1650,7 +1656,7 @@ extern "C" int kmm_ensemble_step(kmm_ctx** ctxs, int n, int nsteps) {
     std::vector<int> grouped(n, 0);
     std::vector<std::vector<int>> groups;
     for (int m = 0; m < n; ++m) {
-      if (grouped[m] || done[m] >= nsteps || !batchable(ctxs[m])) continue;
+      if (!allow_batch || grouped[m] || done[m] >= nsteps || !batchable(ctxs[m])) continue;
       std::vector<int> gi{m};
       for (int q = m + 1; q < n; ++q)
         if (!grouped[q] && done[q] == done[m] && batchable(ctxs[q]) && same_shape(ctxs[m], ctxs[q])) gi.push_back(q);

@@ -43,7 +43,7 @@ EXPORTS = [
     "kmm_read_particles", "kmm_set_particles", "kmm_particle_count", "kmm_get_particles", "kmm_write_particles",
     "kmm_fluid_interp", "kmm_get_physical", "kmm_get_dpdx", "kmm_set_dpdx",
     "kmm_gradv", "kmm_calcswirl", "kmm_get_particle_swirl", "kmm_set_graph",
-    "kmm_ensemble_step", "kmm_batched_steps",
+    "kmm_ensemble_step", "kmm_ensemble_step_ex", "kmm_batched_steps",
     "kmm_mlp_param_count", "kmm_mlp_create", "kmm_mlp_destroy", "kmm_mlp_set_params", "kmm_mlp_get_params",
     "kmm_mlp_set_norm", "kmm_mlp_infer", "kmm_mlp_grad", "kmm_mlp_train",
     "kmm_set_poly_capture", "kmm_poly_samples", "kmm_set_closure", "kmm_get_poly_stress",
@@ -156,6 +156,7 @@ def load_library(path: str | None = None):
     lib.kmm_set_graph.argtypes = [C.c_void_p, C.c_int]
     lib.kmm_ensemble_step.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int]
     lib.kmm_batched_steps.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
+    lib.kmm_ensemble_step_ex.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int]
     fp = C.POINTER(C.c_float)
     lib.kmm_mlp_param_count.argtypes = []
     lib.kmm_mlp_create.argtypes = [C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]
@@ -662,13 +663,14 @@ class Channel:
         _check(self.lib, self.lib.kmm_set_graph(self.h, int(steps)))
 
 
-def ensemble_step(channels, nsteps):
+def ensemble_step(channels, nsteps, batch=True):
     """Advance independent channels together (kmm_ensemble_step): members of
     one shape with plain channel steps share one launch per stage; the
-    others queue work on their own streams before any is waited for."""
+    others queue work on their own streams before any is waited for.
+    batch=False: every member on its own stream."""
     lib = load_library()
     arr = (C.c_void_p * len(channels))(*[ch.h for ch in channels])
-    _check(lib, lib.kmm_ensemble_step(arr, len(channels), int(nsteps)))
+    _check(lib, lib.kmm_ensemble_step_ex(arr, len(channels), int(nsteps), 0 if batch else 1))
 
 
 def _fp(a):

@@ -68,3 +68,24 @@ def test_sixteen_member_ensemble():
             assert e < 1e-4, k
     for ch in chans + alone:
         ch.close()
+
+
+def test_shared_launches_equal_own_streams():
+    """Four channels of one size (64x49x64, their own Re_tau and seeds):
+    kmm_ensemble_step with shared launches and with every member on its own
+    stream give the same states bit for bit; only the members of the first
+    run count their steps as shared."""
+    mem = [(64, 48, 64, 120.0 + 30.0 * i, 100 + i) for i in range(4)]
+    a = [member(*m)[2] for m in mem]
+    b = [member(*m)[2] for m in mem]
+    for ch in a + b:
+        ch.step(1)
+    kmm.ensemble_step(a, 9, batch=True)
+    kmm.ensemble_step(b, 9, batch=False)
+    assert [ch.batched_steps() for ch in a] == [9] * 4
+    assert [ch.batched_steps() for ch in b] == [0] * 4
+    for x, y in zip(a, b):
+        for k in ("v", "omy", "fn", "gn", "u0", "w0"):
+            assert np.array_equal(x.get(k), y.get(k)), k
+    for ch in a + b:
+        ch.close()
