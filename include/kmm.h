@@ -218,7 +218,7 @@ int kmm_solve(kmm_ctx* ctx, double* u, double x, double g, double dyde, const do
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, hosts that forbid hipMalloc).  Same arguments,
  * kernels and results as kmm_solve, with every device buffer taken from a
- * fixed array each GPU holds until the process ends (about 5.3 MB): slabs
+ * fixed array each GPU holds until the process ends (about 5.8 MB): slabs
  * up to nz <= 1024 and nyp*nz <= 131072 complex values; larger slabs return
  * KMM_E_ARG.  Calls that overlap on one GPU share the array and race. */
 int kmm_solve_noalloc(kmm_ctx* ctx, double* u, double x, double g, double dyde, const double* bctop,
