@@ -133,7 +133,7 @@ module kmm_c
     integer(c_int) function kmm_subforce(ctx, gn, fn, omz, t1, t2, t3) bind(C)
       import; type(c_ptr), value :: ctx; complex(c_double_complex) :: gn(*), fn(*), omz(*), t1(*), t2(*), t3(*)
     end function
-    integer(c_int) function kmm_polyNL(ctx, cNL, cn, cnm1, strn, strnm1) bind(C)
+    integer(c_int) function kmm_polyNL(ctx, cNL, cn, cnm1, strn, strnm1) bind(C, name="kmm_polyNL")
       import; type(c_ptr), value :: ctx; type(c_ptr) :: cNL(6), cn(6), cnm1(6), strn(6), strnm1(6)
     end function
     integer(c_int) function kmm_norm(ctx, a) bind(C)

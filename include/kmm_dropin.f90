@@ -2,8 +2,8 @@
 ! solvers, helpers) whose procedures keep the reference's names and dummy
 ! argument lists and run on the GPU through the C ABI of kmm.h.
 !
-! A reference host keeps its calls unchanged after `call kmm_attach(ctx)`
-! with a context made by kmm_create (module kmm_c).  Arrays are the
+! A reference host keeps its calls unchanged after `call kmm_attach(ctx, ny,
+! nz)` with a context made by kmm_create (module kmm_c).  Arrays are the
 ! reference's: spectral fields complex(8) (nyp,nz,nxh), profiles real(8)
 ! (nyp), one kx slab complex(8) (nyp,nz) for solve/penta.  The reference's
 ! work arrays (a, wrkc, wrk1) are accepted and not used.  A failing call
@@ -23,10 +23,16 @@ module kmm_dropin_state
   implicit none
   type(c_ptr), save :: kmm_ctx_handle = c_null_ptr
   integer, save :: kmm_it = 0
+  ! leading extents (nyp, nz) of the (nyp,nz,nxh) fields that vcw3dp's
+  ! rank-3 dummies take (a generic name resolves by rank)
+  integer, save :: kmm_nyp = 1, kmm_nz = 1
 contains
-  subroutine kmm_attach(ctx)
+  subroutine kmm_attach(ctx, ny, nz)
     type(c_ptr), intent(in) :: ctx
+    integer, intent(in), optional :: ny, nz
     kmm_ctx_handle = ctx
+    if (present(ny)) kmm_nyp = ny + 1
+    if (present(nz)) kmm_nz = nz
   end subroutine
   subroutine kmm_check(rc, where)
     integer(c_int), intent(in) :: rc
@@ -113,8 +119,13 @@ contains
   end subroutine
   ! vcw3dp, NS list (Lu, Lv, Lw feed the inertial particles the step tracks itself)
   subroutine vcw3dp_ns(u, v, w, omx, omy, omz, fn, gn, u11, u12, u13, u21, u22, u23, u31, u32, u33, Lu, Lv, Lw)
-    complex(8) :: u(*), v(*), w(*), omx(*), omy(*), omz(*), fn(*), gn(*), Lu(*), Lv(*), Lw(*)
-    complex(8), target :: u11(*), u12(*), u13(*), u21(*), u22(*), u23(*), u31(*), u32(*), u33(*)
+    complex(8) :: u(kmm_nyp, kmm_nz, *), v(kmm_nyp, kmm_nz, *), w(kmm_nyp, kmm_nz, *)
+    complex(8) :: omx(kmm_nyp, kmm_nz, *), omy(kmm_nyp, kmm_nz, *), omz(kmm_nyp, kmm_nz, *)
+    complex(8) :: fn(kmm_nyp, kmm_nz, *), gn(kmm_nyp, kmm_nz, *), Lu(kmm_nyp, kmm_nz, *)
+    complex(8) :: Lv(kmm_nyp, kmm_nz, *), Lw(kmm_nyp, kmm_nz, *)
+    complex(8), target :: u11(kmm_nyp, kmm_nz, *), u12(kmm_nyp, kmm_nz, *), u13(kmm_nyp, kmm_nz, *)
+    complex(8), target :: u21(kmm_nyp, kmm_nz, *), u22(kmm_nyp, kmm_nz, *), u23(kmm_nyp, kmm_nz, *)
+    complex(8), target :: u31(kmm_nyp, kmm_nz, *), u32(kmm_nyp, kmm_nz, *), u33(kmm_nyp, kmm_nz, *)
     type(c_ptr), target :: g(9)
     g = [c_loc(u11), c_loc(u12), c_loc(u13), c_loc(u21), c_loc(u22), c_loc(u23), c_loc(u31), c_loc(u32), &
          c_loc(u33)]
@@ -124,9 +135,15 @@ contains
   ! vcw3dp, SCALAR list
   subroutine vcw3dp_scalar(u, v, w, omx, omy, omz, fn, gn, u11, u12, u13, u21, u22, u23, u31, u32, u33, &
                            scalar, sclx, scly, sclz, scn, Lu, Lv, Lw)
-    complex(8) :: u(*), v(*), w(*), omx(*), omy(*), omz(*), fn(*), gn(*), Lu(*), Lv(*), Lw(*)
-    complex(8), target :: u11(*), u12(*), u13(*), u21(*), u22(*), u23(*), u31(*), u32(*), u33(*)
-    complex(8), target :: scalar(*), sclx(*), scly(*), sclz(*), scn(*)
+    complex(8) :: u(kmm_nyp, kmm_nz, *), v(kmm_nyp, kmm_nz, *), w(kmm_nyp, kmm_nz, *)
+    complex(8) :: omx(kmm_nyp, kmm_nz, *), omy(kmm_nyp, kmm_nz, *), omz(kmm_nyp, kmm_nz, *)
+    complex(8) :: fn(kmm_nyp, kmm_nz, *), gn(kmm_nyp, kmm_nz, *), Lu(kmm_nyp, kmm_nz, *)
+    complex(8) :: Lv(kmm_nyp, kmm_nz, *), Lw(kmm_nyp, kmm_nz, *)
+    complex(8), target :: u11(kmm_nyp, kmm_nz, *), u12(kmm_nyp, kmm_nz, *), u13(kmm_nyp, kmm_nz, *)
+    complex(8), target :: u21(kmm_nyp, kmm_nz, *), u22(kmm_nyp, kmm_nz, *), u23(kmm_nyp, kmm_nz, *)
+    complex(8), target :: u31(kmm_nyp, kmm_nz, *), u32(kmm_nyp, kmm_nz, *), u33(kmm_nyp, kmm_nz, *)
+    complex(8), target :: scalar(kmm_nyp, kmm_nz, *), sclx(kmm_nyp, kmm_nz, *), scly(kmm_nyp, kmm_nz, *)
+    complex(8), target :: sclz(kmm_nyp, kmm_nz, *), scn(kmm_nyp, kmm_nz, *)
     type(c_ptr), target :: g(9)
     g = [c_loc(u11), c_loc(u12), c_loc(u13), c_loc(u21), c_loc(u22), c_loc(u23), c_loc(u31), c_loc(u32), &
          c_loc(u33)]
@@ -143,16 +160,33 @@ contains
                                    c11n, c12n, c13n, c22n, c23n, c33n, &
                                    str11n, str12n, str13n, str22n, str23n, str33n, &
                                    qp11, qp12, qp13, qp22, qp23, qp33, Lu, Lv, Lw)
-    complex(8) :: u(*), v(*), w(*), omx(*), omy(*), omz(*), fn(*), gn(*), Lu(*), Lv(*), Lw(*)
-    complex(8), target :: u11(*), u12(*), u13(*), u21(*), u22(*), u23(*), u31(*), u32(*), u33(*)
-    complex(8), target :: scalar(*), sclx(*), scly(*), sclz(*), scn(*)
-    complex(8), target :: c11(*), c12(*), c13(*), c21(*), c22(*), c23(*), c31(*), c32(*), c33(*)
-    complex(8), target :: dc111(*), dc112(*), dc113(*), dc211(*), dc212(*), dc213(*), dc311(*), dc312(*), dc313(*)
-    complex(8), target :: dc121(*), dc122(*), dc123(*), dc221(*), dc222(*), dc223(*), dc321(*), dc322(*), dc323(*)
-    complex(8), target :: dc131(*), dc132(*), dc133(*), dc231(*), dc232(*), dc233(*), dc331(*), dc332(*), dc333(*)
-    complex(8), target :: c11n(*), c12n(*), c13n(*), c22n(*), c23n(*), c33n(*)
-    complex(8), target :: str11n(*), str12n(*), str13n(*), str22n(*), str23n(*), str33n(*)
-    complex(8), target :: qp11(*), qp12(*), qp13(*), qp22(*), qp23(*), qp33(*)
+    complex(8) :: u(kmm_nyp, kmm_nz, *), v(kmm_nyp, kmm_nz, *), w(kmm_nyp, kmm_nz, *)
+    complex(8) :: omx(kmm_nyp, kmm_nz, *), omy(kmm_nyp, kmm_nz, *), omz(kmm_nyp, kmm_nz, *)
+    complex(8) :: fn(kmm_nyp, kmm_nz, *), gn(kmm_nyp, kmm_nz, *), Lu(kmm_nyp, kmm_nz, *)
+    complex(8) :: Lv(kmm_nyp, kmm_nz, *), Lw(kmm_nyp, kmm_nz, *)
+    complex(8), target :: u11(kmm_nyp, kmm_nz, *), u12(kmm_nyp, kmm_nz, *), u13(kmm_nyp, kmm_nz, *)
+    complex(8), target :: u21(kmm_nyp, kmm_nz, *), u22(kmm_nyp, kmm_nz, *), u23(kmm_nyp, kmm_nz, *)
+    complex(8), target :: u31(kmm_nyp, kmm_nz, *), u32(kmm_nyp, kmm_nz, *), u33(kmm_nyp, kmm_nz, *)
+    complex(8), target :: scalar(kmm_nyp, kmm_nz, *), sclx(kmm_nyp, kmm_nz, *), scly(kmm_nyp, kmm_nz, *)
+    complex(8), target :: sclz(kmm_nyp, kmm_nz, *), scn(kmm_nyp, kmm_nz, *)
+    complex(8), target :: c11(kmm_nyp, kmm_nz, *), c12(kmm_nyp, kmm_nz, *), c13(kmm_nyp, kmm_nz, *)
+    complex(8), target :: c21(kmm_nyp, kmm_nz, *), c22(kmm_nyp, kmm_nz, *), c23(kmm_nyp, kmm_nz, *)
+    complex(8), target :: c31(kmm_nyp, kmm_nz, *), c32(kmm_nyp, kmm_nz, *), c33(kmm_nyp, kmm_nz, *)
+    complex(8), target :: dc111(kmm_nyp, kmm_nz, *), dc112(kmm_nyp, kmm_nz, *), dc113(kmm_nyp, kmm_nz, *)
+    complex(8), target :: dc211(kmm_nyp, kmm_nz, *), dc212(kmm_nyp, kmm_nz, *), dc213(kmm_nyp, kmm_nz, *)
+    complex(8), target :: dc311(kmm_nyp, kmm_nz, *), dc312(kmm_nyp, kmm_nz, *), dc313(kmm_nyp, kmm_nz, *)
+    complex(8), target :: dc121(kmm_nyp, kmm_nz, *), dc122(kmm_nyp, kmm_nz, *), dc123(kmm_nyp, kmm_nz, *)
+    complex(8), target :: dc221(kmm_nyp, kmm_nz, *), dc222(kmm_nyp, kmm_nz, *), dc223(kmm_nyp, kmm_nz, *)
+    complex(8), target :: dc321(kmm_nyp, kmm_nz, *), dc322(kmm_nyp, kmm_nz, *), dc323(kmm_nyp, kmm_nz, *)
+    complex(8), target :: dc131(kmm_nyp, kmm_nz, *), dc132(kmm_nyp, kmm_nz, *), dc133(kmm_nyp, kmm_nz, *)
+    complex(8), target :: dc231(kmm_nyp, kmm_nz, *), dc232(kmm_nyp, kmm_nz, *), dc233(kmm_nyp, kmm_nz, *)
+    complex(8), target :: dc331(kmm_nyp, kmm_nz, *), dc332(kmm_nyp, kmm_nz, *), dc333(kmm_nyp, kmm_nz, *)
+    complex(8), target :: c11n(kmm_nyp, kmm_nz, *), c12n(kmm_nyp, kmm_nz, *), c13n(kmm_nyp, kmm_nz, *)
+    complex(8), target :: c22n(kmm_nyp, kmm_nz, *), c23n(kmm_nyp, kmm_nz, *), c33n(kmm_nyp, kmm_nz, *)
+    complex(8), target :: str11n(kmm_nyp, kmm_nz, *), str12n(kmm_nyp, kmm_nz, *), str13n(kmm_nyp, kmm_nz, *)
+    complex(8), target :: str22n(kmm_nyp, kmm_nz, *), str23n(kmm_nyp, kmm_nz, *), str33n(kmm_nyp, kmm_nz, *)
+    complex(8), target :: qp11(kmm_nyp, kmm_nz, *), qp12(kmm_nyp, kmm_nz, *), qp13(kmm_nyp, kmm_nz, *)
+    complex(8), target :: qp22(kmm_nyp, kmm_nz, *), qp23(kmm_nyp, kmm_nz, *), qp33(kmm_nyp, kmm_nz, *)
     type(c_ptr) :: g(9), c(9), d(27), cn(6), sn(6), q(6)
     g = [c_loc(u11), c_loc(u12), c_loc(u13), c_loc(u21), c_loc(u22), c_loc(u23), c_loc(u31), c_loc(u32), &
          c_loc(u33)]

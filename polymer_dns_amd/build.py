@@ -39,20 +39,24 @@ FORTRAN_BIN = os.path.join(ROOT, "tests", "fortran", "bin")
 
 
 def build_fortran(verbose: bool = False) -> str | None:
-    """The Fortran host program of the drop-in test (tests/fortran/dns_kmm.f90
-    over include/kmm.f90), linked to libkmm.so by a path relative to the
-    binary.  Skipped (None) when the image has no flang."""
+    """The Fortran host programs of the drop-in tests (tests/fortran/dns_kmm.f90
+    over include/kmm.f90, tests/fortran/dropin_routines.f90 over the drop-in
+    modules), linked to libkmm.so by a path relative to the binary.  Skipped
+    (None) when the image has no flang."""
     if not os.path.exists(FLANG):
         if verbose:
             print("flang not found: Fortran host program not built")
         return None
     os.makedirs(FORTRAN_BIN, exist_ok=True)
     out = os.path.join(FORTRAN_BIN, "dns_kmm")
+    rpath = "-Wl,-rpath,$ORIGIN/../../../polymer_dns_amd"
     steps = [
         [FLANG, "-c", os.path.join(ROOT, "include", "kmm.f90"), "-o", "kmm.o"],
         [FLANG, "-c", os.path.join(ROOT, "include", "kmm_dropin.f90"), "-o", "kmm_dropin.o"],
         [FLANG, "-I.", os.path.join(ROOT, "tests", "fortran", "dns_kmm.f90"), "kmm.o", "-L" + HERE, "-lkmm",
-         "-Wl,-rpath,$ORIGIN/../../../polymer_dns_amd", "-o", out],
+         rpath, "-o", out],
+        [FLANG, "-I.", os.path.join(ROOT, "tests", "fortran", "dropin_routines.f90"), "kmm.o", "kmm_dropin.o",
+         "-L" + HERE, "-lkmm", rpath, "-o", os.path.join(FORTRAN_BIN, "dropin_routines")],
     ]
     for cmd in steps:
         r = subprocess.run(cmd, cwd=FORTRAN_BIN, capture_output=True, text=True)
