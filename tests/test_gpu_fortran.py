@@ -52,3 +52,38 @@ def test_fortran_host_restart_run_matches_python(tmp_path):
     a, b = open(rf, "rb").read(), open(rp, "rb").read()
     assert len(a) == len(b) and a == b
     assert a != open(r0, "rb").read()   # the run advanced the state
+
+
+DROPIN = os.path.join(ROOT, "tests", "fortran", "bin", "dropin_routines")
+
+
+def test_fortran_dropin_modules_match_c_abi(tmp_path):
+    """The drop-in modules' procedures under the reference's names, called
+    from Fortran with the reference's rank-3 arrays (tests/fortran/
+    dropin_routines.f90: derivs::cderiv, solvers::vort, solvers::vcw3dp with
+    the NS argument list, helpers::norm), give the same fields bit for bit
+    as the C ABI called through ctypes."""
+    from oracle import kmm_oracle as O
+    assert os.path.exists(DROPIN), "tests/fortran/bin/dropin_routines is built by build()"
+    cfgp = str(tmp_path / "dns.config")
+    write_config(cfgp, 32, 32, 32, 1, 0.002)
+    cfg = kmm.read_config(cfgp)
+    p = params(32, 32, 32, dt=cfg.dt, re=cfg.re, R_tau=cfg.r_tau, xl=cfg.xl, zl=cfg.zl)
+    st = seeded(p, 62, amp=0.2)
+    u, v, w = O.veloc(st["v"], st["omy"], st["u0"], st["w0"], p)
+    for a in (u, v, w):
+        O.norm(a)
+    omy = st["omy"]
+    fin, fout = tmp_path / "fields", tmp_path / "results"
+    np.concatenate([np.ascontiguousarray(a, dtype=np.complex128).ravel() for a in (u, v, w, omy)]).tofile(fin)
+    print("case: Fortran drop-in modules, 32x33x32", flush=True)
+    r = subprocess.run([DROPIN, cfgp, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.fromfile(fout, dtype=np.complex128).reshape(6, *u.shape)
+    with kmm.Channel(cfg) as ch:
+        du = ch.cderiv(u)
+        omx, omz = ch.vort(u, v, w)
+        gn, fn, oz = ch.vcw3dp(u, v, w, omx, omy, omz)
+        want = [du, omx, omz, ch.norm(fn), gn, oz]
+    for name, a, b in zip(("cderiv", "omx", "omz", "fn", "gn", "omz out"), got, want):
+        assert np.array_equal(a, b), name
