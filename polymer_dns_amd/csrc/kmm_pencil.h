@@ -298,8 +298,7 @@ constexpr int kChebPlan = 1;
 // sum_n a_n T_n via a length-2NY FFT of the even extension of c_n/2 * a_n,
 // computed in the transfer precision V and stored to dst[y][pencil].
 template <int NY, typename V>
-__device__ void cheb_to_phys_store(const PFN<NY>& f, V* buf, V* scr, const V* tw, int lane, V* __restrict__ dst,
-                                   size_t P, size_t p) {
+__device__ __forceinline__ V* cheb_to_phys(const PFN<NY>& f, V* buf, V* scr, const V* tw, int lane) {
   constexpr int L = PLay<NY>::L, S = PLay<NY>::S, M = 2 * NY;
 #pragma unroll
   for (int s = 0; s < S; ++s) {
@@ -315,7 +314,13 @@ __device__ void cheb_to_phys_store(const PFN<NY>& f, V* buf, V* scr, const V* tw
   }
   if (lane == 0) buf[fpad(NY)] = to_v<V>(f.e);
   wave_sync();
-  V* r = wave_fft<M, -1, V, kChebPlan>(buf, scr, tw, lane);
+  return wave_fft<M, -1, V, kChebPlan>(buf, scr, tw, lane);
+}
+template <int NY, typename V>
+__device__ void cheb_to_phys_store(const PFN<NY>& f, V* buf, V* scr, const V* tw, int lane, V* __restrict__ dst,
+                                   size_t P, size_t p) {
+  constexpr int L = PLay<NY>::L, S = PLay<NY>::S;
+  const V* r = cheb_to_phys<NY, V>(f, buf, scr, tw, lane);
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     const int g = s * WAVE + lane;
@@ -624,19 +629,31 @@ __device__ __forceinline__ void tab_load(const double2* __restrict__ tab, int la
   invdx = tab[tab_width(NY) - 1].x;
 }
 
+// a pencil's 1/d table entries held in registers (loaded ahead of the solve)
+template <int S>
+struct TabV {
+  double te[S], to[S], invdx;
+};
+template <int NY>
+using TabN = TabV<PLay<NY>::S>;
+template <int NY>
+__device__ __forceinline__ TabN<NY> tab_get(const double2* __restrict__ tab, int lane) {
+  TabN<NY> t;
+  tab_load<NY>(tab, lane, t.te, t.to, t.invdx);
+  return t;
+}
+
 // solve (solvers.f90:174-283): (dyde^2 D^2 - wn) u = f, Dirichlet rows
 // (ib=0) or Neumann rows (ib!=0, wn already guarded by the caller).
-// tab: the pencil's 1/d table (tab_width entries).
+// tb: the pencil's 1/d table (tab_width entries).
 template <int NY>
-__device__ PFN<NY> tau_solve(const PFN<NY>& f, double wn, double dyde, const double2* __restrict__ tab,
-                             double2 bctop, double2 bcbot, int ib, int lane) {
+__device__ PFN<NY> tau_solve(const PFN<NY>& f, double wn, double dyde, const TabN<NY>& tb, double2 bctop,
+                             double2 bcbot, int ib, int lane) {
   constexpr int S = PLay<NY>::S;
   double2 we[S], wo[S], wex;
   tau_rhs<NY>(f, lane, we, wo, wex);
-  double te[S], to[S], invdx;
-  tab_load<NY>(tab, lane, te, to, invdx);
-  const TauPar<S> E = tau_eliminate<NY>(true, wn, dyde, te, invdx, we, wex, lane);
-  const TauPar<S> O = tau_eliminate<NY>(false, wn, dyde, to, 0.0, wo, czero(), lane);
+  const TauPar<S> E = tau_eliminate<NY>(true, wn, dyde, tb.te, tb.invdx, we, wex, lane);
+  const TauPar<S> O = tau_eliminate<NY>(false, wn, dyde, tb.to, 0.0, wo, czero(), lane);
   double2 bce, bco;
   if (ib == 0) { bce = cscale(cadd(bctop, bcbot), 0.5); bco = cscale(csub(bctop, bcbot), 0.5); }
   else { bce = cscale(csub(bctop, bcbot), 0.5); bco = cscale(cadd(bctop, bcbot), 0.5); }
@@ -660,19 +677,23 @@ __device__ PFN<NY> tau_solve(const PFN<NY>& f, double wn, double dyde, const dou
   u.e = xx;
   return u;
 }
+template <int NY>
+__device__ __forceinline__ PFN<NY> tau_solve(const PFN<NY>& f, double wn, double dyde,
+                                             const double2* __restrict__ tab, double2 bctop, double2 bcbot, int ib,
+                                             int lane) {
+  return tau_solve<NY>(f, wn, dyde, tab_get<NY>(tab, lane), bctop, bcbot, ib, lane);
+}
 
 // penta (solvers.f90:680-1061) with Robin rows sum tt(n) u_n = gtop,
 // sum tb(n) u_n = gbot; tt/tb are closed-form functions of n.
 template <int NY, typename TT, typename TB>
-__device__ PFN<NY> tau_penta(const PFN<NY>& f, double wn, double dyde, const double2* __restrict__ tab,
-                             TT tt, TB tb, double2 gtop, double2 gbot, int lane) {
+__device__ PFN<NY> tau_penta(const PFN<NY>& f, double wn, double dyde, const TabN<NY>& tab, TT tt, TB tb,
+                             double2 gtop, double2 gbot, int lane) {
   constexpr int S = PLay<NY>::S;
   double2 we[S], wo[S], wex;
   tau_rhs<NY>(f, lane, we, wo, wex);
-  double te[S], to[S], invdx;
-  tab_load<NY>(tab, lane, te, to, invdx);
-  const TauPar<S> E = tau_eliminate<NY>(true, wn, dyde, te, invdx, we, wex, lane);
-  const TauPar<S> O = tau_eliminate<NY>(false, wn, dyde, to, 0.0, wo, czero(), lane);
+  const TauPar<S> E = tau_eliminate<NY>(true, wn, dyde, tab.te, tab.invdx, we, wex, lane);
+  const TauPar<S> O = tau_eliminate<NY>(false, wn, dyde, tab.to, 0.0, wo, czero(), lane);
   double f2e, f1e, f2o, f1o;
   double2 s2e = tau_bcsum<NY>(E, tb, lane, f2e);   // js = 2 (bottom row) first
   double2 s1e = tau_bcsum<NY>(E, tt, lane, f1e);
@@ -689,6 +710,12 @@ __device__ PFN<NY> tau_penta(const PFN<NY>& f, double wn, double dyde, const dou
   tau_forward<NY>(O, u1, lane, u.b, dummy);
   u.e = xx;
   return u;
+}
+template <int NY, typename TT, typename TB>
+__device__ __forceinline__ PFN<NY> tau_penta(const PFN<NY>& f, double wn, double dyde,
+                                             const double2* __restrict__ tab, TT tt, TB tb, double2 gtop,
+                                             double2 gbot, int lane) {
+  return tau_penta<NY>(f, wn, dyde, tab_get<NY>(tab, lane), tt, tb, gtop, gbot, lane);
 }
 
 // phirhs (solvers.f90:555-676) for theta <= 0.99:
@@ -733,6 +760,151 @@ __device__ __forceinline__ V* wave_lds(V* smem, int wv) {
   // twiddles [2NY], then two padded FFT buffers of length 2NY per wave
   return smem + 2 * NY + (size_t)wv * 2 * FFTBuf<2 * NY>::len;
 }
+// ------------------------------------------------ block-staged tiles --
+// The PWS consecutive pencils p0 .. p0+PWS-1 of a block, rows y = 0..NY of
+// a field laid out [y][pencil], staged through an LDS tile [NY+1][PWS+1]
+// (one pad element per row): the block's global accesses then cover whole
+// rows of PWS pencils (64 bytes per row for FP64), and each wave takes its
+// own pencil's column from the tile.  Thread t moves elements
+// e = t + k PWS*64 (row e / PWS, pencil e % PWS).  Four pencils per block
+// keep the staged step kernel within half the CU's LDS, so that two blocks
+// share a CU and one computes while the other waits at a block barrier.
+constexpr int PWS = 4;
+constexpr int TLD = PWS + 1;
+template <int NY>
+struct TileG {
+  static constexpr int ROWS = NY + 1, NEL = ROWS * PWS, LEN = ROWS * TLD;
+  static constexpr int PER = (NEL + PWS * WAVE - 1) / (PWS * WAVE);
+};
+template <int NY, typename E>
+__device__ __forceinline__ void tile_fetch(const E* __restrict__ F, size_t P, int p0, E (&r)[TileG<NY>::PER]) {
+  using G = TileG<NY>;
+#pragma unroll
+  for (int k = 0; k < G::PER; ++k) {
+    const int e = (int)threadIdx.x + k * PWS * WAVE;
+    const int y = e / PWS, w = e % PWS;
+    r[k] = (e < G::NEL && (size_t)(p0 + w) < P) ? F[(size_t)y * P + p0 + w] : vzero<E>();
+  }
+}
+template <int NY, typename E>
+__device__ __forceinline__ void tile_put(E* t, const E (&r)[TileG<NY>::PER]) {
+  using G = TileG<NY>;
+#pragma unroll
+  for (int k = 0; k < G::PER; ++k) {
+    const int e = (int)threadIdx.x + k * PWS * WAVE;
+    if (e < G::NEL) t[(e / PWS) * TLD + e % PWS] = r[k];
+  }
+}
+template <int NY, typename E>
+__device__ __forceinline__ void tile_store(const E* t, E* __restrict__ F, size_t P, int p0) {
+  using G = TileG<NY>;
+#pragma unroll
+  for (int k = 0; k < G::PER; ++k) {
+    const int e = (int)threadIdx.x + k * PWS * WAVE;
+    const int y = e / PWS, w = e % PWS;
+    if (e < G::NEL && (size_t)(p0 + w) < P) F[(size_t)y * P + p0 + w] = t[y * TLD + w];
+  }
+}
+// a wave's pencil (column wv) of an FP64 tile in the pair layout, and back
+template <int NY>
+__device__ __forceinline__ PFN<NY> tile_col(const double2* t, int wv, int lane) {
+  constexpr int L = PLay<NY>::L, S = PLay<NY>::S;
+  PFN<NY> f;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int g = s * WAVE + lane;
+    if (g < L) {
+      f.a[s] = t[(2 * g) * TLD + wv];
+      f.b[s] = t[(2 * g + 1) * TLD + wv];
+    } else {
+      f.a[s] = f.b[s] = czero();
+    }
+  }
+  f.e = t[NY * TLD + wv];
+  return f;
+}
+template <int NY>
+__device__ __forceinline__ void tile_col_put(double2* t, int wv, int lane, const PFN<NY>& f) {
+  constexpr int L = PLay<NY>::L, S = PLay<NY>::S;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int g = s * WAVE + lane;
+    if (g < L) {
+      t[(2 * g) * TLD + wv] = f.a[s];
+      t[(2 * g + 1) * TLD + wv] = f.b[s];
+    }
+  }
+  if (lane == 0) t[NY * TLD + wv] = f.e;
+}
+// physical values of column wv of a tile in the transfer precision
+template <int NY, typename V>
+__device__ __forceinline__ PhysN<NY, V> tile_phys(const V* t, int wv, int lane) {
+  constexpr int L = PLay<NY>::L, S = PLay<NY>::S;
+  PhysN<NY, V> q;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int g0 = s * WAVE + lane;
+    const int g = g0 < L ? g0 : L - 1;
+    q.a[s] = t[(2 * g) * TLD + wv];
+    q.b[s] = t[(2 * g + 1) * TLD + wv];
+  }
+  q.e = t[NY * TLD + wv];
+  return q;
+}
+// inverse transform of f into column wv of a transfer tile (zeros: zero column)
+template <int NY, typename V>
+__device__ __forceinline__ void cheb_to_phys_tile(const PFN<NY>& f, V* buf, V* scr, const V* tw, int lane, V* t,
+                                                  int wv) {
+  constexpr int L = PLay<NY>::L, S = PLay<NY>::S;
+  const V* r = cheb_to_phys<NY, V>(f, buf, scr, tw, lane);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int g = s * WAVE + lane;
+    if (g < L) {
+      t[(2 * g) * TLD + wv] = r[fpad(2 * g)];
+      t[(2 * g + 1) * TLD + wv] = r[fpad(2 * g + 1)];
+    }
+  }
+  if (lane == 0) t[NY * TLD + wv] = r[fpad(NY)];
+  wave_sync();
+}
+template <int NY, typename V>
+__device__ __forceinline__ void tile_col_zero(V* t, int wv, int lane) {
+  for (int y = lane; y <= NY; y += WAVE) t[y * TLD + wv] = vzero<V>();
+}
+
+// Pencil group of block b of nb: blocks are dispatched round-robin over the
+// 8 XCDs (block b on XCD b % 8), each with its own L2; groups 4k .. 4k+3
+// share the cache lines of every [y][pencil] row (64 bytes per group for
+// FP64, 32 for FP32), so they go to blocks b, b+8, b+16, b+24 of one XCD.
+// A bijection of 0..nb-1 (blocks past the last multiple of 32 keep their
+// index).
+__device__ __forceinline__ int xcd_group(int b, int nb) {
+  if (b >= nb / 32 * 32) return b;
+  const int x = b % 8, s = b / 8;
+  return (s / 4) * 32 + x * 4 + s % 4;
+}
+
+// LDS plan of the staged step kernel (bytes): twiddles, then a region used
+// as [FFT buffers | nonlinear-term tiles] while the nonlinear terms are
+// transformed, as [state tiles] (FP64) through the solves, and as [FFT
+// buffers | transfer tiles] for the inverse transforms.
+template <int NY, typename V, bool SCALAR>
+struct StepLds {
+  static constexpr size_t CAP = 80 * 1024;  // two blocks per CU
+  static constexpr size_t TWB = (size_t)2 * NY * sizeof(V);
+  static constexpr size_t FFT = (size_t)PWS * 2 * FFTBuf<2 * NY>::len * sizeof(V);
+  static constexpr size_t TV = (size_t)TileG<NY>::LEN * sizeof(V);
+  static constexpr size_t TD = (size_t)TileG<NY>::LEN * sizeof(double2);
+  static constexpr int NNL = SCALAR ? 4 : 3, NST = SCALAR ? 6 : 4, NOUT = SCALAR ? 9 : 6;
+  static constexpr int fit = (CAP > TWB + FFT) ? (int)((CAP - TWB - FFT) / TV) : 0;
+  static constexpr int NTS = fit < NOUT ? fit : NOUT;  // transfer tiles per flush
+  static constexpr size_t A1 = FFT + NNL * TV, A2 = NST * TD, A3 = FFT + (size_t)NTS * TV;
+  static constexpr size_t REGION = A1 > A2 ? (A1 > A3 ? A1 : A3) : (A2 > A3 ? A2 : A3);
+  static constexpr bool ok = NTS >= 3 && TWB + REGION <= CAP && PLay<NY>::S == 1;
+  static constexpr size_t bytes = TWB + REGION;
+};
+
 // block-wide copy of the length-2NY twiddles into LDS (before any early exit)
 template <int NY, typename V>
 __device__ __forceinline__ const V* lds_twiddles(V* smem, const V* __restrict__ tw) {
@@ -784,5 +956,20 @@ __device__ __forceinline__ PFN<NY> smooth_apply(const PFN<NY>& f, const double* 
 static inline int nblk(int P) { return (P + PW - 1) / PW; }
 template <int NY, typename V>
 static inline size_t lds_bytes() { return (size_t)(PW * 2 * FFTBuf<2 * NY>::len + 2 * NY) * sizeof(V); }
+// the step kernel's launch: the staged plan (PWS pencils per block) where
+// it fits, else PW pencils per block with their FFT buffers
+template <int NY, typename V, bool SCALAR, int MODE>
+constexpr bool step_tiled() { return MODE == 0 && StepLds<NY, V, SCALAR>::ok; }
+template <int NY, typename V, bool SCALAR, int MODE>
+static inline size_t step_lds_bytes() {
+  if constexpr (step_tiled<NY, V, SCALAR, MODE>()) return StepLds<NY, V, SCALAR>::bytes;
+  else return lds_bytes<NY, V>();
+}
+template <int NY, typename V, bool SCALAR, int MODE>
+static inline int step_blocks(int P) {
+  return step_tiled<NY, V, SCALAR, MODE>() ? (P + PWS - 1) / PWS : nblk(P);
+}
+template <int NY, typename V, bool SCALAR, int MODE>
+static inline int step_threads() { return (step_tiled<NY, V, SCALAR, MODE>() ? PWS : PW) * 64; }
 
 }  // namespace kmm

@@ -384,9 +384,284 @@ __device__ __forceinline__ void pencil_body(const PencilArgs& A, int blk) {
   }
 }
 
+// The full step (MODE 0) with every [y][pencil] field of the block staged
+// through LDS tiles (coalesced rows of PW pencils): the nonlinear terms
+// and the six state fields are read a block at a time, the state is
+// updated in place in its tiles, and the transfer fields leave through
+// tiles after their inverse transforms.  Same operations in the same order
+// per pencil as pencil_body, so the results are identical; every wave of
+// the block, the z-Nyquist pencil's included, passes every block barrier.
+template <int NY, bool SCALAR, typename V>
+__device__ __forceinline__ void pencil_step_tiled(const PencilArgs& A, int blk) {
+  using PF = PFN<NY>;
+  using G = TileG<NY>;
+  using LP = StepLds<NY, V, SCALAR>;
+  constexpr int NNL = LP::NNL, NST = LP::NST, NTS = LP::NTS;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  V* smem = reinterpret_cast<V*>(smraw);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int p0 = blk * PWS, p = p0 + wv;
+  const V* twl = lds_twiddles<NY, V>(smem, static_cast<const V*>(A.twM));
+  V* buf = wave_lds<NY, V>(smem, wv);
+  V* scr = buf + FFTBuf<2 * NY>::len;
+  unsigned char* region = smraw + LP::TWB;
+  V* vtile = reinterpret_cast<V*>(region + LP::FFT);     // NL / transfer tiles after the FFT buffers
+  double2* stile = reinterpret_cast<double2*>(region);  // state tiles over the whole region
+  V* const* TT = reinterpret_cast<V* const*>(A.T);
+  const size_t P = (size_t)A.P;
+  const bool live = (size_t)p < P;
+  double kx, kz;
+  int kxi, kzi;
+  pencil_k(A, live ? p : 0, kx, kz, kxi, kzi);
+  const double k2 = kx * kx + kz * kz;
+  const bool is00 = live && kxi == 0 && kzi == 0;
+  const bool nyq = kzi == A.nzh;
+  const bool act = live && !nyq;  // computes; the z-Nyquist pencil is zeroed (helpers.f90:149-156)
+  const double dy = A.dyde;
+  const double th = A.theta;
+  // state tiles: 0 V, 1 FNM1, 2 OMY, 3 GNM1, 4 TH, 5 SCNM1
+  double2* const SF[6] = {A.V, A.FNM1, A.OMY, A.GNM1, A.TH, A.SCNM1};
+  double2 rs[NST][G::PER];
+
+  // ---------------- part B: nonlinear terms -> fn, gn, scn (spectral) --
+  PF fn, gn, sn;
+  PF h1m, h3m;  // mean-mode profiles (only used by the (0,0) pencil)
+  if (A.first_nl) {
+    {
+      V rn[NNL][G::PER];
+      const V* NLF[4] = {static_cast<const V*>(A.NL1), static_cast<const V*>(A.NL2),
+                         static_cast<const V*>(A.NL3), static_cast<const V*>(A.NLS)};
+#pragma unroll
+      for (int f = 0; f < NNL; ++f) tile_fetch<NY, V>(NLF[f], P, p0, rn[f]);
+#pragma unroll
+      for (int f = 0; f < NST; ++f) tile_fetch<NY, double2>(SF[f], P, p0, rs[f]);
+#pragma unroll
+      for (int f = 0; f < NNL; ++f) tile_put<NY, V>(vtile + (size_t)f * G::LEN, rn[f]);
+    }
+    __syncthreads();
+    if (act) {
+      const double sz = A.smooth ? A.smz[kzi] : 1.0;
+      PF h1 = phys_to_cheb<NY, V>(tile_phys<NY, V>(vtile, wv, lane), buf, scr, twl, lane);
+      PF h3 = phys_to_cheb<NY, V>(tile_phys<NY, V>(vtile + 2 * G::LEN, wv, lane), buf, scr, twl, lane);
+      PF h2 = phys_to_cheb<NY, V>(tile_phys<NY, V>(vtile + G::LEN, wv, lane), buf, scr, twl, lane);
+      if (A.TF[0]) {  // polymer force, subforce (helpers.f90:564-606), before smoothing
+        h1 = pf_add(h1, pf_load<NY>(A.TF[0], P, p, lane));
+        h2 = pf_add(h2, pf_load<NY>(A.TF[1], P, p, lane));
+        h3 = pf_add(h3, pf_load<NY>(A.TF[2], P, p, lane));
+      }
+      if (A.smooth) {
+        h1 = smooth_apply<NY>(h1, A.smy, sz, lane);
+        h2 = smooth_apply<NY>(h2, A.smy, sz, lane);
+        h3 = smooth_apply<NY>(h3, A.smy, sz, lane);
+      }
+      if (is00) { h1 = pf_real(h1); h2 = pf_real(h2); h3 = pf_real(h3); }
+      h1m = h1;
+      h3m = h3;
+      // nonlin (solvers.f90:1497-1572)
+      PF x = pf_add(pf_imul(h3, kz), pf_imul(h1, kx));
+      PF dx = cderiv<NY>(x, dy, lane);
+      fn = pf_sub(pf_scale(h2, -k2), dx);
+      gn = pf_sub(pf_imul(h1, kz), pf_imul(h3, kx));
+      if (SCALAR) {
+        sn = phys_to_cheb<NY, V>(tile_phys<NY, V>(vtile + 3 * G::LEN, wv, lane), buf, scr, twl, lane);
+        if (A.smooth) sn = smooth_apply<NY>(sn, A.smy, sz, lane);
+        if (is00) sn = pf_real(sn);
+      }
+    }
+    __syncthreads();  // the tiles and FFT buffers become the state tiles
+  } else {
+#pragma unroll
+    for (int f = 0; f < NST; ++f) tile_fetch<NY, double2>(SF[f], P, p0, rs[f]);
+    if (act) {
+      fn = pf_load<NY>(A.FN, P, p, lane);
+      gn = pf_load<NY>(A.GN, P, p, lane);
+      if (SCALAR) sn = pf_load<NY>(A.SCN, P, p, lane);
+      if (is00) { h1m = pf_load_real<NY>(A.h1n, lane); h3m = pf_load_real<NY>(A.h3n, lane); }
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < NST; ++f) tile_put<NY, double2>(stile + (size_t)f * G::LEN, rs[f]);
+  __syncthreads();
+
+  // ---------------- solves, in place in the state tiles ----------------
+  double2* tV = stile;
+  double2* tFNM1 = stile + G::LEN;
+  double2* tOMY = stile + 2 * G::LEN;
+  double2* tGNM1 = stile + 3 * G::LEN;
+  double2* tTH = stile + 4 * G::LEN;
+  double2* tSCNM1 = stile + 5 * G::LEN;
+  PF sc, v, omy;
+  PF u0p, w0p;
+  if (act) {
+    // scalar (main_dns.f90:324-358)
+    if (SCALAR) {
+      sc = tile_col<NY>(tTH, wv, lane);
+      const PF scnm1 = tile_col<NY>(tSCNM1, wv, lane);
+      PF d1 = cderiv<NY>(sc, dy, lane);
+      PF d2 = cderiv<NY>(d1, dy, lane);
+      PF rhs = phirhs<NY>(sc, d2, sn, scnm1, k2, A.re, A.diff, A.dt, th);
+      tile_col_put<NY>(tSCNM1, wv, lane, sn);
+      auto tt = [dy](int n) -> double { return dy * (double)n * (double)n; };
+      auto tb = [dy](int n) -> double { double v = dy * (double)n * (double)n; return (n & 1) ? -v : v; };
+      sc = tau_penta<NY>(rhs, k2 + A.gS, dy, A.tabS + (size_t)p * tab_width(NY), tt, tb, czero(), czero(), lane);
+      if (is00) sc = pf_real(sc);
+      tile_col_put<NY>(tTH, wv, lane, sc);
+    }
+    // v and omega_y (main_dns.f90:360-502)
+    if (!is00) {
+      v = tile_col<NY>(tV, wv, lane);
+      const PF fnm1 = tile_col<NY>(tFNM1, wv, lane);
+      PF d1 = cderiv<NY>(v, dy, lane);
+      PF d2 = cderiv<NY>(d1, dy, lane);
+      PF phi = pf_sub(d2, pf_scale(v, k2));
+      d1 = cderiv<NY>(phi, dy, lane);
+      d2 = cderiv<NY>(d1, dy, lane);
+      PF rhs = phirhs<NY>(phi, d2, fn, fnm1, k2, A.re, 1.0, A.dt, th);
+      tile_col_put<NY>(tFNM1, wv, lane, fn);
+      phi = tau_solve<NY>(rhs, k2 + A.gH, dy, A.tabH + (size_t)p * tab_width(NY), czero(), czero(), 0, lane);
+      PF vp = tau_solve<NY>(phi, k2, dy, A.tabP + (size_t)p * tab_width(NY), czero(), czero(), 0, lane);
+      double2 vt, vb;
+      wall_derivs<NY>(vp, dy, lane, vt, vb);
+      const double4 g = A.gk[p];
+      double2 c1 = cadd(cscale(vb, g.x), cscale(vt, g.y));
+      double2 c2 = cadd(cscale(vb, g.z), cscale(vt, g.w));
+      v = pf_add(vp, pf_add(pf_cmul_real(c1, green_load<NY>(A.G1, p, lane)),
+                            pf_cmul_real(c2, green_load<NY>(A.G2, p, lane))));
+      omy = tile_col<NY>(tOMY, wv, lane);
+      const PF gnm1 = tile_col<NY>(tGNM1, wv, lane);
+      PF e1 = cderiv<NY>(omy, dy, lane);
+      PF e2 = cderiv<NY>(e1, dy, lane);
+      PF orhs = phirhs<NY>(omy, e2, gn, gnm1, k2, A.re, 1.0, A.dt, th);
+      tile_col_put<NY>(tGNM1, wv, lane, gn);
+      const double atu = A.atu, btu = A.btu, abu = A.abu, bbu = A.bbu;
+      auto tt = [=](int n) -> double { return atu + dy * btu * (double)n * (double)n; };
+      auto tb = [=](int n) -> double { double t = abu + dy * bbu * (double)n * (double)n; return (n & 1) ? -t : t; };
+      omy = tau_penta<NY>(orhs, k2 + A.gH, dy, A.tabH + (size_t)p * tab_width(NY), tt, tb, czero(), czero(), lane);
+      tile_col_put<NY>(tV, wv, lane, v);
+      tile_col_put<NY>(tOMY, wv, lane, omy);
+    } else {
+      // (0,0): v = 0 (veloc), omega_y = 0, fnm1 = fn = 0, gnm1 = gn = 0,
+      // and the mean-flow solves (uzero, solvers.f90:1065-1176).
+      v = pf_zero<NY>();
+      omy = pf_zero<NY>();
+      tile_col_put<NY>(tV, wv, lane, v);
+      tile_col_put<NY>(tOMY, wv, lane, omy);
+      tile_col_put<NY>(tFNM1, wv, lane, pf_zero<NY>());
+      tile_col_put<NY>(tGNM1, wv, lane, pf_zero<NY>());
+      for (int w = 0; w < 2; ++w) {
+        double* um = w ? A.w0 : A.u0;
+        double* hl = w ? A.h3nm1 : A.h1nm1;
+        double* hn = w ? A.h3n : A.h1n;
+        const PF& hcur = w ? h3m : h1m;
+        PF u = pf_load_real<NY>(um, lane);
+        PF hlp = pf_load_real<NY>(hl, lane);
+        PF d1 = cderiv<NY>(u, dy, lane);
+        PF d2 = cderiv<NY>(d1, dy, lane);
+        PF r = pf_scale(u, -A.re / (A.dt * th));
+        r = pf_axpy(-3.0 * A.re / (2.0 * th), hcur, r);
+        r = pf_axpy(A.re / (2.0 * th), hlp, r);
+        r = pf_axpy(-(1.0 - th) / th, d2, r);
+        const double at = w ? A.atw : A.atu, bt = A.btu, ab = w ? A.abw : A.abu, bb = w ? A.bbw : A.bbu;
+        const double gt = w ? A.gtw : A.gtu, gb = w ? A.gbw : A.gbu;
+        auto tt = [=](int n) -> double { return at + dy * bt * (double)n * (double)n; };
+        auto tb = [=](int n) -> double { double t = ab - dy * bb * (double)n * (double)n; return (n & 1) ? -t : t; };
+        PF un = tau_penta<NY>(r, A.gH, dy, A.tabH + (size_t)p * tab_width(NY), tt, tb, make_double2(gt, 0.0),
+                              make_double2(gb, 0.0), lane);
+        un = pf_real(un);
+        if (w == 0 && A.cmf) {
+          PF unit = pf_zero<NY>();
+          if (lane == 0) unit.a[0] = make_double2(-A.re / th, 0.0);
+          PF phi = tau_penta<NY>(unit, A.gH, dy, A.tabH + (size_t)p * tab_width(NY), tt, tb, czero(), czero(), lane);
+          phi = pf_real(phi);
+          const double bu = pf_bulk<NY>(un, lane), bp = pf_bulk<NY>(phi, lane);
+          const double fc = (A.ubulk - bu) / bp;
+          un = pf_axpy(fc, phi, un);
+          if (lane == 0) *A.dpdx = *A.dpdx - fc;
+        }
+        pf_store_real<NY>(um, lane, un);
+        pf_store_real<NY>(hl, lane, hcur);
+        pf_store_real<NY>(hn, lane, hcur);
+        if (w) w0p = un; else u0p = un;
+      }
+    }
+  } else if (live) {
+#pragma unroll
+    for (int f = 0; f < NST; ++f) tile_col_put<NY>(stile + (size_t)f * G::LEN, wv, lane, pf_zero<NY>());
+  }
+  __syncthreads();
+#pragma unroll
+  for (int f = 0; f < NST; ++f) tile_store<NY, double2>(stile + (size_t)f * G::LEN, SF[f], P, p0);
+  __syncthreads();  // the region becomes FFT buffers and transfer tiles
+
+  // ---------------- veloc / vort / gradscl (solvers.f90:1576-1722) ------
+  PF u, w;
+  if (act) {
+    if (is00) {
+      u = u0p;
+      w = w0p;
+    } else {
+      PF dv = cderiv<NY>(v, dy, lane);
+      const double ik2 = 1.0 / k2;
+      u = pf_scale(pf_sub(pf_imul(dv, kx), pf_imul(omy, kz)), ik2);
+      w = pf_scale(pf_add(pf_imul(dv, kz), pf_imul(omy, kx)), ik2);
+    }
+  }
+  // inverse transforms into transfer tiles, NTS at a time; out(j) is the
+  // j-th transfer field, dst(j) its destination
+  auto flush = [&](int j0, int n, V* const* dst) {
+    __syncthreads();
+    for (int j = 0; j < n; ++j) tile_store<NY, V>(vtile + (size_t)j * G::LEN, dst[j0 + j], P, p0);
+    __syncthreads();
+  };
+  if (A.TPD[0]) {  // D u, D v, D w for the velocity-gradient tensor (gradv, derivatives.f90:6-86)
+    V* const D[3] = {static_cast<V*>(A.TPD[0]), static_cast<V*>(A.TPD[1]), static_cast<V*>(A.TPD[2])};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      V* t = vtile + (size_t)j * G::LEN;
+      if (act) {
+        const PF d = cderiv<NY>(j == 0 ? u : (j == 1 ? v : w), dy, lane);
+        cheb_to_phys_tile<NY, V>(d, buf, scr, twl, lane, t, wv);
+      } else if (live) {
+        tile_col_zero<NY, V>(t, wv, lane);
+      }
+    }
+    flush(0, 3, D);
+  }
+  PF du, dw, ox, oz;
+  if (act) {
+    du = cderiv<NY>(u, dy, lane);
+    dw = cderiv<NY>(w, dy, lane);
+    ox = pf_sub(dw, pf_imul(v, kz));
+    oz = pf_sub(pf_imul(v, kx), du);
+  }
+  constexpr int NOUT = LP::NOUT;
+#pragma unroll
+  for (int j = 0; j < NOUT; ++j) {
+    V* t = vtile + (size_t)(j % NTS) * G::LEN;
+    if (act) {
+      PF o;
+      if (j == 0) o = u;
+      else if (j == 1) o = v;
+      else if (j == 2) o = w;
+      else if (j == 3) o = ox;
+      else if (j == 4) o = omy;
+      else if (j == 5) o = oz;
+      else if (j == 6) o = pf_imul(sc, kx);
+      else if (j == 7) o = cderiv<NY>(sc, dy, lane);
+      else o = pf_imul(sc, kz);
+      cheb_to_phys_tile<NY, V>(o, buf, scr, twl, lane, t, wv);
+    } else if (live) {
+      tile_col_zero<NY, V>(t, wv, lane);
+    }
+    if (j % NTS == NTS - 1 || j == NOUT - 1) flush(j - j % NTS, j % NTS + 1, TT);
+  }
+}
+
 template <int NY, bool SCALAR, int MODE, typename V, int TV = 0>
 __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(TV ? 3 : 1))) void k_pencil(PencilArgs A) {
-  pencil_body<NY, SCALAR, MODE, V>(A, blockIdx.x);
+  if constexpr (MODE == 0 && StepLds<NY, V, SCALAR>::ok)
+    pencil_step_tiled<NY, SCALAR, V>(A, xcd_group(blockIdx.x, gridDim.x));
+  else pencil_body<NY, SCALAR, MODE, V>(A, blockIdx.x);
 }
 
 // The step kernel of an ensemble of channels of one size in one launch:
@@ -394,7 +669,9 @@ __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(TV ? 3 
 template <int NY, bool SCALAR, typename V, int TV>
 __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(TV ? 3 : 1)))
 void k_pencil_batch(const PencilArgs* __restrict__ AA) {
-  pencil_body<NY, SCALAR, 0, V>(AA[blockIdx.y], blockIdx.x);
+  if constexpr (StepLds<NY, V, SCALAR>::ok)
+    pencil_step_tiled<NY, SCALAR, V>(AA[blockIdx.y], xcd_group(blockIdx.x, gridDim.x));
+  else pencil_body<NY, SCALAR, 0, V>(AA[blockIdx.y], blockIdx.x);
 }
 
 // -------------------------------------------------- parity kernels --
@@ -554,11 +831,15 @@ hipError_t launch_green(int ny, const PencilArgs& a, double pb1, double pt1, dou
 template <int MODE, typename V, int TV>
 static hipError_t launch_pencil_tv(int ny, const PencilArgs& a, hipStream_t s) {
   if (a.scalar) {
-    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_pencil<NY, true, MODE, V, TV>), dim3(nblk(a.P)), dim3(PW * 64),
-                                           (lds_bytes<NY, V>()), s, a));
+    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_pencil<NY, true, MODE, V, TV>),
+                                           dim3(step_blocks<NY, V, true, MODE>(a.P)),
+                                           dim3(step_threads<NY, V, true, MODE>()),
+                                           (step_lds_bytes<NY, V, true, MODE>()), s, a));
   } else {
-    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_pencil<NY, false, MODE, V, TV>), dim3(nblk(a.P)), dim3(PW * 64),
-                                           (lds_bytes<NY, V>()), s, a));
+    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_pencil<NY, false, MODE, V, TV>),
+                                           dim3(step_blocks<NY, V, false, MODE>(a.P)),
+                                           dim3(step_threads<NY, V, false, MODE>()),
+                                           (step_lds_bytes<NY, V, false, MODE>()), s, a));
   }
   return hipGetLastError();
 }
@@ -578,11 +859,15 @@ template <typename V, int TV>
 static hipError_t launch_pencil_batch_v(int ny, const PencilArgs& a0, const PencilArgs* dA, int nmem,
                                         hipStream_t s) {
   if (a0.scalar) {
-    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_pencil_batch<NY, true, V, TV>), dim3(nblk(a0.P), nmem),
-                                           dim3(PW * 64), (lds_bytes<NY, V>()), s, dA));
+    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_pencil_batch<NY, true, V, TV>),
+                                           dim3(step_blocks<NY, V, true, 0>(a0.P), nmem),
+                                           dim3(step_threads<NY, V, true, 0>()),
+                                           (step_lds_bytes<NY, V, true, 0>()), s, dA));
   } else {
-    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_pencil_batch<NY, false, V, TV>), dim3(nblk(a0.P), nmem),
-                                           dim3(PW * 64), (lds_bytes<NY, V>()), s, dA));
+    KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_pencil_batch<NY, false, V, TV>),
+                                           dim3(step_blocks<NY, V, false, 0>(a0.P), nmem),
+                                           dim3(step_threads<NY, V, false, 0>()),
+                                           (step_lds_bytes<NY, V, false, 0>()), s, dA));
   }
   return hipGetLastError();
 }
