@@ -425,6 +425,88 @@ __device__ V* wave_fft(V* x, V* y, const V* __restrict__ tw, int lane) {
   return x;
 }
 
+// radix-R DFT of v[0..R-1] in place (natural order)
+template <int R, int DIR, typename V>
+__device__ __forceinline__ void dft_r(V (&v)[8]) {
+  if constexpr (R == 8) dft8<DIR>(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]);
+  else if constexpr (R == 6) dft6<DIR>(v[0], v[1], v[2], v[3], v[4], v[5]);
+  else if constexpr (R == 5) dft5<DIR>(v[0], v[1], v[2], v[3], v[4]);
+  else if constexpr (R == 4) dft4<DIR>(v[0], v[1], v[2], v[3]);
+  else if constexpr (R == 3) dft3<DIR>(v[0], v[1], v[2]);
+  else dft2<DIR>(v[0], v[1]);
+}
+__host__ __device__ constexpr FFTPlan reverse_plan(FFTPlan p) {
+  FFTPlan q{p.n, p.nst, {0}};
+  for (int i = 0; i < p.nst; ++i) q.r[i] = p.r[p.nst - 1 - i];
+  return q;
+}
+// One stage s of the Stockham FFT of plan PL (the radix is a compile-time
+// constant), with its input taken from registers (FIRST: lane j holds
+// x[j + m N/R], one round) or its output left in registers (LAST: lane j
+// keeps y[j + m N/R], one round).  TWC: the twiddles w^2 .. w^(R-1) of a
+// butterfly are powers of the one read from the table (fewer LDS loads).
+template <int N, int DIR, typename V, int R, int NS, bool FIRST, bool LAST, bool TWC>
+__device__ __forceinline__ void fft_stage(V* x, V* y, const V* __restrict__ tw, int lane, const V (&in)[8],
+                                          V (&out)[8]) {
+  constexpr int nb = N / R;
+  constexpr int tstride = N / (NS * R);
+  static_assert(!(FIRST || LAST) || nb == WAVE, "register stages need N / R == 64");
+  for (int j = lane; j < nb; j += WAVE) {
+    const int k = j % NS;
+    const int idxD = (j / NS) * NS * R + k;
+    V v[8];
+#pragma unroll
+    for (int m = 0; m < R; ++m) v[m] = FIRST ? in[m] : x[fpad(j + m * nb)];
+    if constexpr (NS > 1) {
+      if constexpr (TWC) {
+        const V w1 = twid<DIR>(tw, k * tstride);
+        V wm = w1;
+#pragma unroll
+        for (int m = 1; m < R; ++m) {
+          v[m] = cmul(v[m], wm);
+          if (m + 1 < R) wm = cmul(wm, w1);
+        }
+      } else {
+#pragma unroll
+        for (int m = 1; m < R; ++m) v[m] = cmul(v[m], twid<DIR>(tw, m * k * tstride));
+      }
+    }
+    dft_r<R, DIR, V>(v);
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      if (LAST) out[m] = v[m];
+      else y[fpad(idxD + m * NS)] = v[m];
+    }
+  }
+}
+template <int N, int DIR, typename V, int MODEL, bool REV, bool FIRST, bool LAST, bool TWC, int S, int NS>
+__device__ __forceinline__ V* fft_stages(V* x, V* y, const V* __restrict__ tw, int lane, const V (&in)[8],
+                                         V (&out)[8]) {
+  constexpr FFTPlan P = REV ? reverse_plan(make_plan(N, MODEL)) : make_plan(N, MODEL);
+  if constexpr (S == P.nst) {
+    return x;
+  } else {
+    constexpr int R = P.r[S];
+    constexpr bool last = LAST && S == P.nst - 1;
+    fft_stage<N, DIR, V, R, NS, FIRST && S == 0, last, TWC>(x, y, tw, lane, in, out);
+    if constexpr (last) return y;
+    wave_sync();
+    return fft_stages<N, DIR, V, MODEL, REV, FIRST, LAST, TWC, S + 1, NS * R>(y, x, tw, lane, in, out);
+  }
+}
+// The one-wavefront FFT of wave_fft with the plan optionally reversed, the
+// first stage optionally fed from registers and the last stage's outputs
+// optionally kept in registers (then nothing is written by it and the
+// returned buffer is not the result).
+template <int N, int DIR, typename V, int MODEL, bool REV, bool FIRST, bool LAST, bool TWC = false>
+__device__ __forceinline__ V* wave_fft_ex(V* x, V* y, const V* __restrict__ tw, int lane, const V (&in)[8],
+                                          V (&out)[8]) {
+  return fft_stages<N, DIR, V, MODEL, REV, FIRST, LAST, TWC, 0, 1>(x, y, tw, lane, in, out);
+}
+// first / last radix of a plan
+template <int N, int MODEL>
+constexpr int plan_last_radix() { return make_plan(N, MODEL).r[make_plan(N, MODEL).nst - 1]; }
+
 // Swirl strength (calcswirl + eig, helpers.f90:197-319, 360-404): the
 // reference evaluates the three eigenvalues of the velocity-gradient tensor
 // with Cardano's closed form in complex arithmetic and returns

@@ -243,9 +243,20 @@ __device__ __forceinline__ void xplane_body(const PlaneArgs& A, int bid) {
     xrow_pair_load<NK, V, T>(pax, pay, pbx, pby, zr + (size_t)zA * nxh, zr + (size_t)zB * nxh, nxh, lane);
   };
   rows(0);
+  // MX = 64 R with R the plan's last radix: the last stage of an inverse
+  // transform leaves lane j the outputs x = j + 64 m, which can stay in
+  // registers, and a forward transform can run the reversed plan, whose
+  // first stage reads them there.  Measured at MX = 192: FP32 gains from the
+  // inverse side (the forward side costs registers), FP64 from the forward
+  // side (the inverse side takes it from 3 to 2 waves per SIMD).
+  constexpr bool XOK = MX % 64 == 0 && plan_last_radix<MX, kPlanePlan>() == MX / 64;
+  constexpr bool XINV = XOK && sizeof(V) == 8, XFWD = XOK && sizeof(V) == 16;
+  const V none[8] = {};
+  V regs[8];
 #pragma unroll
   for (int f = 0; f < NFIN; ++f) {
-    for (int x = lane; x < MX; x += 64) buf[fpad(x)] = vzero<V>();
+    // zeros at the dealiasing band [nxh, MX - nxh]; the packed rows fill the rest
+    for (int x = nxh + lane; x <= MX - nxh; x += 64) buf[fpad(x)] = vzero<V>();
     wave_sync();
 #pragma unroll
     for (int q = 0; q < NK; ++q) {
@@ -261,11 +272,17 @@ __device__ __forceinline__ void xplane_body(const PlaneArgs& A, int bid) {
     }
     if (f + 1 < NFIN) rows(f + 1);
     wave_sync();
-    V* r = wave_fft<MX, +1, V, kPlanePlan>(buf, scr, tw, lane);
+    if constexpr (XINV) {
+      wave_fft_ex<MX, +1, V, kPlanePlan, false, false, true>(buf, scr, tw, lane, none, regs);
 #pragma unroll
-    for (int m = 0; m < XPL; ++m) {
-      const int x = lane + 64 * m;
-      ph[f][m] = (x < MX) ? r[fpad(x)] : vzero<V>();
+      for (int m = 0; m < XPL; ++m) ph[f][m] = regs[m];
+    } else {
+      V* r = wave_fft<MX, +1, V, kPlanePlan>(buf, scr, tw, lane);
+#pragma unroll
+      for (int m = 0; m < XPL; ++m) {
+        const int x = lane + 64 * m;
+        ph[f][m] = (x < MX) ? r[fpad(x)] : vzero<V>();
+      }
     }
     wave_sync();
   }
@@ -355,13 +372,20 @@ __device__ __forceinline__ void xplane_body(const PlaneArgs& A, int bid) {
   // forward x transforms (r2c) of the outputs, two rows per transform
 #pragma unroll
   for (int o = 0; o < NFOUT; ++o) {
+    V* r;
+    if constexpr (XFWD) {
 #pragma unroll
-    for (int m = 0; m < XPL; ++m) {
-      const int x = lane + 64 * m;
-      if (x < MX) buf[fpad(x)] = ph[o][m];
+      for (int m = 0; m < XPL; ++m) regs[m] = ph[o][m];
+      r = wave_fft_ex<MX, -1, V, kPlanePlan, true, true, false>(buf, scr, tw, lane, regs, regs);
+    } else {
+#pragma unroll
+      for (int m = 0; m < XPL; ++m) {
+        const int x = lane + 64 * m;
+        if (x < MX) buf[fpad(x)] = ph[o][m];
+      }
+      wave_sync();
+      r = wave_fft<MX, -1, V, kPlanePlan>(buf, scr, tw, lane);
     }
-    wave_sync();
-    V* r = wave_fft<MX, -1, V, kPlanePlan>(buf, scr, tw, lane);
     V* zo = static_cast<V*>(A.ZF) + ((size_t)o * A.nyp + y) * A.mz * nxh;
     for (int k = lane; k < nxh; k += 64) {
       const V zk = r[fpad(k)], zm = r[fpad((MX - k) % MX)];
