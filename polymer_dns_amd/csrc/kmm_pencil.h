@@ -885,6 +885,17 @@ __device__ __forceinline__ int xcd_group(int b, int nb) {
   return (s / 4) * 32 + x * 4 + s % 4;
 }
 
+// A wave's FFT can run in place (one buffer) when every stage of its plan
+// is one round (N / R <= 64): a stage's LDS reads of all lanes then precede
+// its writes, which depend on them.
+template <int N, int MODEL>
+constexpr bool fft_in_place() {
+  constexpr FFTPlan P = make_plan(N, MODEL);
+  for (int s = 0; s < P.nst; ++s)
+    if (N / P.r[s] > WAVE) return false;
+  return true;
+}
+
 // LDS plan of the staged step kernel (bytes): twiddles, then a region used
 // as [FFT buffers | nonlinear-term tiles] while the nonlinear terms are
 // transformed, as [state tiles] (FP64) through the solves, and as [FFT
@@ -893,7 +904,7 @@ template <int NY, typename V, bool SCALAR>
 struct StepLds {
   static constexpr size_t CAP = 80 * 1024;  // two blocks per CU
   static constexpr size_t TWB = (size_t)2 * NY * sizeof(V);
-  static constexpr size_t FFT = (size_t)PWS * 2 * FFTBuf<2 * NY>::len * sizeof(V);
+  static constexpr size_t FFT = (size_t)PWS * FFTBuf<2 * NY>::len * sizeof(V);  // one in-place buffer per wave
   static constexpr size_t TV = (size_t)TileG<NY>::LEN * sizeof(V);
   static constexpr size_t TD = (size_t)TileG<NY>::LEN * sizeof(double2);
   static constexpr int NNL = SCALAR ? 4 : 3, NST = SCALAR ? 6 : 4, NOUT = SCALAR ? 9 : 6;
@@ -901,7 +912,8 @@ struct StepLds {
   static constexpr int NTS = fit < NOUT ? fit : NOUT;  // transfer tiles per flush
   static constexpr size_t A1 = FFT + NNL * TV, A2 = NST * TD, A3 = FFT + (size_t)NTS * TV;
   static constexpr size_t REGION = A1 > A2 ? (A1 > A3 ? A1 : A3) : (A2 > A3 ? A2 : A3);
-  static constexpr bool ok = NTS >= 3 && TWB + REGION <= CAP && PLay<NY>::S == 1;
+  static constexpr bool ok = NTS >= 3 && TWB + REGION <= CAP && PLay<NY>::S == 1 &&
+                             fft_in_place<2 * NY, kChebPlan>();
   static constexpr size_t bytes = TWB + REGION;
 };
 

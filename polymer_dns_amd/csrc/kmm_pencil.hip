@@ -402,8 +402,8 @@ __device__ __forceinline__ void pencil_step_tiled(const PencilArgs& A, int blk) 
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int p0 = blk * PWS, p = p0 + wv;
   const V* twl = lds_twiddles<NY, V>(smem, static_cast<const V*>(A.twM));
-  V* buf = wave_lds<NY, V>(smem, wv);
-  V* scr = buf + FFTBuf<2 * NY>::len;
+  V* buf = smem + 2 * NY + (size_t)wv * FFTBuf<2 * NY>::len;
+  V* scr = buf;  // transforms in place
   unsigned char* region = smraw + LP::TWB;
   V* vtile = reinterpret_cast<V*>(region + LP::FFT);     // NL / transfer tiles after the FFT buffers
   double2* stile = reinterpret_cast<double2*>(region);  // state tiles over the whole region

@@ -251,6 +251,9 @@ __device__ __forceinline__ void xplane_body(const PlaneArgs& A, int bid) {
   // side (the inverse side takes it from 3 to 2 waves per SIMD).
   constexpr bool XOK = MX % 64 == 0 && plan_last_radix<MX, kPlanePlan>() == MX / 64;
   constexpr bool XINV = XOK && sizeof(V) == 8, XFWD = XOK && sizeof(V) == 16;
+  // FP64: twiddles as powers of one table value per butterfly (fewer LDS
+  // loads in this LDS-bound kernel, -3 % at MX = 192; no gain in FP32)
+  constexpr bool XTW = sizeof(V) == 16;
   const V none[8] = {};
   V regs[8];
 #pragma unroll
@@ -273,11 +276,11 @@ __device__ __forceinline__ void xplane_body(const PlaneArgs& A, int bid) {
     if (f + 1 < NFIN) rows(f + 1);
     wave_sync();
     if constexpr (XINV) {
-      wave_fft_ex<MX, +1, V, kPlanePlan, false, false, true>(buf, scr, tw, lane, none, regs);
+      wave_fft_ex<MX, +1, V, kPlanePlan, false, false, true, XTW>(buf, scr, tw, lane, none, regs);
 #pragma unroll
       for (int m = 0; m < XPL; ++m) ph[f][m] = regs[m];
     } else {
-      V* r = wave_fft<MX, +1, V, kPlanePlan>(buf, scr, tw, lane);
+      V* r = wave_fft_ex<MX, +1, V, kPlanePlan, false, false, false, XTW>(buf, scr, tw, lane, none, regs);
 #pragma unroll
       for (int m = 0; m < XPL; ++m) {
         const int x = lane + 64 * m;
@@ -376,7 +379,7 @@ __device__ __forceinline__ void xplane_body(const PlaneArgs& A, int bid) {
     if constexpr (XFWD) {
 #pragma unroll
       for (int m = 0; m < XPL; ++m) regs[m] = ph[o][m];
-      r = wave_fft_ex<MX, -1, V, kPlanePlan, true, true, false>(buf, scr, tw, lane, regs, regs);
+      r = wave_fft_ex<MX, -1, V, kPlanePlan, true, true, false, XTW>(buf, scr, tw, lane, regs, regs);
     } else {
 #pragma unroll
       for (int m = 0; m < XPL; ++m) {
@@ -384,7 +387,7 @@ __device__ __forceinline__ void xplane_body(const PlaneArgs& A, int bid) {
         if (x < MX) buf[fpad(x)] = ph[o][m];
       }
       wave_sync();
-      r = wave_fft<MX, -1, V, kPlanePlan>(buf, scr, tw, lane);
+      r = wave_fft_ex<MX, -1, V, kPlanePlan, false, false, false, XTW>(buf, scr, tw, lane, none, regs);
     }
     V* zo = static_cast<V*>(A.ZF) + ((size_t)o * A.nyp + y) * A.mz * nxh;
     for (int k = lane; k < nxh; k += 64) {
