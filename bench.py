@@ -72,7 +72,7 @@ WI_TAU, QBETA, ZLMAX, SC = 50.0, 0.9, 30.0, 0.5
 AUDIT_EVERY, AUDIT_WINDOW, AUDIT_GATE = 50, 3, 0.02  # closure audits: 3 FENE-P steps every 50, 2 % gate
 AUDIT_DENSE = 10   # the second learned line audits every 10 steps (same window and gate)
 RETRAIN_STEPS, RETRAIN_BATCH, RETRAIN_LR = 0, 1024, 3e-4  # retraining after passed audit windows (off)
-ACTIVATION = "gelu"   # the closure network's hidden activation (x sigma(1.702 x))
+ACTIVATION = "gelu_poly"   # the closure network's hidden activation: GELU with the polynomial sigmoid
 PRE_STEPS = 8    # FENE-P steps before the warm-up (the learned line draws its training pairs there)
 NPART = 100_000
 AP = 1e-3   # tracer radius: keeps the wall clamp of tracer_update inside the grid

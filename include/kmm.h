@@ -374,8 +374,11 @@ int kmm_mlp_infer(kmm_mlp* mlp, int n, const float* X, float* Y);
  * gradient; mode 0: FP32, mode 1: bf16 rounding as in inference */
 int kmm_mlp_grad(kmm_mlp* mlp, int n, const float* X, const float* T, int mode, float* grad, float* loss);
 /* Adam (lr, beta 0.9/0.999) on seeded mini-batches; losses[steps] optional */
-/* hidden-layer activation: 0 ReLU (default), 1 GELU as x sigma(1.702 x)
- * (applied in FP32 before the bf16 rounding of the next layer's input) */
+/* hidden-layer activation: 0 ReLU (default), 1 GELU as x sigma(1.702 x),
+ * 2 GELU as x s(t) with t = clamp(1/2 + x/5.28, 0, 1) and the quintic
+ * smoothstep s(t) = t^3 (t (6 t - 15) + 10) (no transcendental; within
+ * 0.0195 of the exact GELU); applied in FP32 before the bf16 rounding of the
+ * next layer's input */
 int kmm_mlp_set_activation(kmm_mlp* mlp, int act);
 int kmm_mlp_train(kmm_mlp* mlp, int n, const float* X, const float* T, int steps, int batch, float lr,
                   unsigned long long seed, float* losses);

@@ -39,26 +39,40 @@ def bf16(x):
 
 
 GELU_A = 1.702   # GELU in the sigmoid form x sigma(1.702 x) (Hendrycks & Gimpel 2016, section 2)
-ACT = "relu"     # hidden-layer activation of the functions below: "relu" or "gelu"
+GELU_C = 2.64    # GELU with the polynomial sigmoid: t = clamp(1/2 + x/(2 c), 0, 1)
+ACT = "relu"     # hidden-layer activation of the functions below: "relu", "gelu" or "gelu_poly"
+
+
+def _poly_t(x):
+    return np.clip(0.5 + x / (2.0 * GELU_C), 0.0, 1.0)
 
 
 def act(x, kind=None):
-    """ReLU, or GELU as x sigma(1.702 x)."""
+    """ReLU; GELU as x sigma(1.702 x); or GELU as x s(t) with the quintic
+    smoothstep s(t) = t^3 (t (6 t - 15) + 10) of t = clamp(1/2 + x/(2 c), 0, 1)."""
     kind = kind or ACT
     if kind == "gelu":
         x = np.asarray(x)
         return x / (1.0 + np.exp(-GELU_A * x))
+    if kind == "gelu_poly":
+        x = np.asarray(x)
+        t = _poly_t(x)
+        return x * (t ** 3 * (t * (6.0 * t - 15.0) + 10.0))
     return np.maximum(x, 0.0)
 
 
 def act_grad(x, kind=None):
     """d act / dx (ReLU: 0 at the kink, as the GPU training kernel takes it;
-    GELU: sigma + 1.702 x sigma (1 - sigma))."""
+    GELU: sigma + 1.702 x sigma (1 - sigma); polynomial GELU:
+    s(t) + x s'(t) / (2 c), s'(t) = 30 t^2 (t - 1)^2)."""
     kind = kind or ACT
     x = np.asarray(x)
     if kind == "gelu":
         s = 1.0 / (1.0 + np.exp(-GELU_A * x))
         return s + GELU_A * x * s * (1.0 - s)
+    if kind == "gelu_poly":
+        t = _poly_t(x)
+        return t ** 3 * (t * (6.0 * t - 15.0) + 10.0) + x / (2.0 * GELU_C) * 30.0 * (t * (t - 1.0)) ** 2
     return (x > 0).astype(x.dtype)
 
 

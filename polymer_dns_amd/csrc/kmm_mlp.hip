@@ -1,6 +1,7 @@
 // Learned polymer-stress closure: a 22 -> 4 x 128 -> 6 multilayer
 // perceptron with bfloat16 weights and FP32 accumulation (no reference
-// counterpart); hidden activation ReLU or GELU in its sigmoid form
+// counterpart); hidden activation ReLU, or GELU with a sigmoid or a
+// polynomial-sigmoid factor
 // x sigma(1.702 x), chosen per network.
 //
 //   k_mlp_infer   inference on the matrix cores: v_mfma_f32_16x16x32_bf16
@@ -82,9 +83,28 @@ __device__ __forceinline__ float gelu_df(float x) {
   const float sg = gelu_sig(x);
   return sg + kGeluA * x * sg * (1.0f - sg);
 }
-// activation a (0 ReLU, 1 GELU) and its derivative
-__device__ __forceinline__ float act_a(int a, float x) { return a ? gelu_f(x) : act_f(x); }
-__device__ __forceinline__ float act_da(int a, float x) { return a ? gelu_df(x) : act_df(x); }
+// GELU with a polynomial sigmoid (no exponential, no reciprocal: only
+// full-rate vector operations): x s(t), t = clamp(1/2 + x/(2c), 0, 1),
+// s(t) = t^3 (t (6 t - 15) + 10) (the quintic smoothstep, s' = 30 t^2 (t-1)^2
+// vanishes at both clamps), c = 2.64: within 0.0195 of x Phi(x) (the sigmoid
+// form above: 0.0204)
+constexpr float kGpK = 1.0f / (2.0f * 2.64f);
+__device__ __forceinline__ float gelup_t(float x) { return fminf(fmaxf(fmaf(x, kGpK, 0.5f), 0.0f), 1.0f); }
+__device__ __forceinline__ float gelup_f(float x) {
+  const float t = gelup_t(x);
+  return x * (t * t * t * fmaf(t, fmaf(t, 6.0f, -15.0f), 10.0f));
+}
+__device__ __forceinline__ float gelup_df(float x) {
+  const float t = gelup_t(x), u = t * (t - 1.0f);
+  return t * t * t * fmaf(t, fmaf(t, 6.0f, -15.0f), 10.0f) + x * kGpK * 30.0f * u * u;
+}
+// activation a (0 ReLU, 1 GELU, 2 GELU with the polynomial sigmoid) and its derivative
+__device__ __forceinline__ float act_a(int a, float x) {
+  return a == 2 ? gelup_f(x) : (a ? gelu_f(x) : act_f(x));
+}
+__device__ __forceinline__ float act_da(int a, float x) {
+  return a == 2 ? gelup_df(x) : (a ? gelu_df(x) : act_df(x));
+}
 // round to bfloat16 (nearest even) and back
 __device__ __forceinline__ float bf16r(float x) {
   unsigned u = __float_as_uint(x);
@@ -148,11 +168,14 @@ __device__ __forceinline__ float field_feature(const MlpFieldIO& io, size_t p, i
 }
 
 // activation of two FP32 values, rounded to a packed bf16 pair: ReLU after
-// the rounding (ACT 0), GELU before it (ACT 1)
+// the rounding (ACT 0), GELU before it (ACT 1, 2)
 template <int ACT>
 __device__ __forceinline__ unsigned act_pk(float a, float b) {
   if constexpr (ACT == 1) {
     const bf16x2 p = __builtin_convertvector(f32x2{gelu_f(a), gelu_f(b)}, bf16x2);
+    return __builtin_bit_cast(unsigned, p);
+  } else if constexpr (ACT == 2) {
+    const bf16x2 p = __builtin_convertvector(f32x2{gelup_f(a), gelup_f(b)}, bf16x2);
     return __builtin_bit_cast(unsigned, p);
   } else {
     const bf16x2 p = __builtin_convertvector(f32x2{a, b}, bf16x2);  // one v_cvt_pk_bf16_f32
@@ -399,6 +422,7 @@ static hipError_t mlp_infer_launch_a(int n, const float* X, float* Y, const MlpF
 template <bool FIELD>
 static hipError_t mlp_infer_launch(int n, const float* X, float* Y, const MlpFieldIO& io, const void* wb,
                                    const float* par, const float* norm, int act, hipStream_t s) {
+  if (act == 2) return mlp_infer_launch_a<FIELD, 2>(n, X, Y, io, wb, par, norm, s);
   return act ? mlp_infer_launch_a<FIELD, 1>(n, X, Y, io, wb, par, norm, s)
              : mlp_infer_launch_a<FIELD, 0>(n, X, Y, io, wb, par, norm, s);
 }
@@ -465,7 +489,7 @@ __global__ __launch_bounds__(256) void k_mlp_fb(int n, const float* __restrict__
   const float* mu_y = norm + 2 * kIn;
   const float* sd_y = norm + 2 * kIn + kOut;
   const bool rb = (mode & 1) != 0;   // bf16 rounding where inference rounds
-  const int act = (mode >> 1) & 1;    // 0 ReLU, 1 GELU
+  const int act = (mode >> 1) & 3;    // 0 ReLU, 1 GELU, 2 GELU (polynomial sigmoid)
   auto Wv = [&](int l, int j, int k) -> float {
     const float w = par[w_off(l) + j * lay_in(l) + k];
     return rb ? bf16r(w) : w;

@@ -23,7 +23,7 @@ struct kmm_mlp {
   float* norm = nullptr;            // mu_x[22], sd_x[22], mu_y[6], sd_y[6]
   void* wb = nullptr;               // padded bf16 weights
   int adam_t = 0;
-  int act = 0;                      // hidden-layer activation: 0 ReLU, 1 GELU
+  int act = 0;                      // hidden-layer activation: 0 ReLU, 1 GELU, 2 GELU (polynomial sigmoid)
   // scratch (grown on demand)
   float *X = nullptr, *T = nullptr, *Y = nullptr, *gpart = nullptr, *lpart = nullptr, *loss = nullptr;
   size_t capX = 0, capT = 0, capY = 0, capG = 0;
@@ -245,7 +245,8 @@ int kmm_mlp_train_device(kmm_mlp* n, int npts, const float* dX, const float* dT,
 }
 
 extern "C" int kmm_mlp_set_activation(kmm_mlp* n, int act) {
-  if (!n || (act != 0 && act != 1)) return mfail(KMM_E_ARG, "activation: 0 (ReLU) or 1 (GELU)");
+  if (!n || act < 0 || act > 2)
+    return mfail(KMM_E_ARG, "activation: 0 (ReLU), 1 (GELU, sigmoid form) or 2 (GELU, polynomial sigmoid)");
   n->act = act;
   return KMM_OK;
 }

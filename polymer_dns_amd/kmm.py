@@ -729,8 +729,10 @@ class Mlp:
         return W, b
 
     def set_activation(self, act):
-        """Hidden-layer activation: "relu" (default) or "gelu" (x sigma(1.702 x))."""
-        _check(self.lib, self.lib.kmm_mlp_set_activation(self.h, {"relu": 0, "gelu": 1}[act]))
+        """Hidden-layer activation: "relu" (default), "gelu" (x sigma(1.702 x)) or
+        "gelu_poly" (x s(t), t = clamp(1/2 + x/5.28, 0, 1), s the quintic
+        smoothstep: no transcendental)."""
+        _check(self.lib, self.lib.kmm_mlp_set_activation(self.h, {"relu": 0, "gelu": 1, "gelu_poly": 2}[act]))
 
     def set_norm(self, mu_x, sd_x, mu_y, sd_y):
         a = [np.ascontiguousarray(x, dtype=np.float32) for x in (mu_x, sd_x, mu_y, sd_y)]

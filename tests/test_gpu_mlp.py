@@ -90,17 +90,18 @@ def test_training_reduces_loss():
     net.close()
 
 
-def test_gelu_gradients_and_inference_match_oracle():
-    """The GELU network (x sigma(1.702 x)): FP32 gradients vs the oracle's
-    FP64 gradients, bf16 gradients and matrix-core inference vs the oracle's
-    bf16 mode."""
+@pytest.mark.parametrize("kind", ["gelu", "gelu_poly"])
+def test_gelu_gradients_and_inference_match_oracle(kind):
+    """The GELU networks (x sigma(1.702 x); x s(t) with the polynomial
+    sigmoid): FP32 gradients vs the oracle's FP64 gradients, bf16 gradients
+    and matrix-core inference vs the oracle's bf16 mode."""
     net, nm = setup_net(seed=16)
-    net.set_activation("gelu")
+    net.set_activation(kind)
     rng = np.random.default_rng(12)
     X = rng.standard_normal((600, 22)).astype(np.float32)
     T = rng.standard_normal((600, 6)).astype(np.float32)
     W, b = net.unpack()
-    M.ACT = "gelu"
+    M.ACT = kind
     try:
         loss, g = net.grad(X, T, mode=0)
         l_ref, gW, gb = M.loss_and_grads([w.astype(np.float64) for w in W], [x.astype(np.float64) for x in b],

@@ -95,3 +95,44 @@ def test_gelu_sigmoid_form_against_the_exact_gelu_and_its_derivative():
             assert abs(fd - gW[l][i, j]) < 1e-7 * max(1.0, abs(fd)), (l, i, j)
     finally:
         M.ACT = "relu"
+
+
+def test_gelu_polynomial_sigmoid_against_the_exact_gelu_and_its_derivative():
+    """GELU as x s(t), t = clamp(1/2 + x/5.28, 0, 1), s the quintic smoothstep:
+    within 0.0195 of x Phi(x) (closer than the sigmoid form), exactly x for
+    x >= 2.64 and exactly 0 for x <= -2.64, 0 and slope 1/2 at 0 (as x Phi(x));
+    the derivative against central differences (continuous at the clamps,
+    where s' = 0); the whole network's gradients against finite differences."""
+    x = np.linspace(-8, 8, 4001)
+    phi = 0.5 * (1.0 + np.array([math.erf(v / math.sqrt(2.0)) for v in x]))
+    g = M.act(x, "gelu_poly")
+    assert np.abs(g - x * phi).max() < 0.0195
+    assert np.abs(g - x * phi).max() < np.abs(M.act(x, "gelu") - x * phi).max()
+    big = x[x >= 2.64]
+    assert np.array_equal(M.act(big, "gelu_poly"), big)
+    assert not np.any(M.act(x[x <= -2.64], "gelu_poly"))
+    assert M.act(np.array([0.0]), "gelu_poly")[0] == 0.0
+    assert abs(M.act_grad(np.array([0.0]), "gelu_poly")[0] - 0.5) < 1e-15
+    h = 1e-6
+    xs = x[np.abs(np.abs(x) - 2.64) > 1e-3]
+    fd = (M.act(xs + h, "gelu_poly") - M.act(xs - h, "gelu_poly")) / (2 * h)
+    assert np.abs(fd - M.act_grad(xs, "gelu_poly")).max() < 1e-7
+    rng = np.random.default_rng(5)
+    W, b = M.init_params(9)
+    W = [w.astype(np.float64) for w in W]
+    b = [bb.astype(np.float64) + 0.01 * rng.standard_normal(bb.shape) for bb in b]
+    X = rng.standard_normal((5, 22))
+    T = rng.standard_normal((5, 6))
+    nm = (np.zeros(22), np.ones(22), np.zeros(6), np.ones(6))
+    M.ACT = "gelu_poly"
+    try:
+        loss, gW, gb = M.loss_and_grads(W, b, X, T, *nm, mode="fp64")
+        for l, (i, j) in [(0, (3, 5)), (2, (1, 2)), (4, (5, 17))]:
+            Wp = [w.copy() for w in W]
+            Wm = [w.copy() for w in W]
+            Wp[l][i, j] += h
+            Wm[l][i, j] -= h
+            fd = (M.loss_and_grads(Wp, b, X, T, *nm, "fp64")[0] - M.loss_and_grads(Wm, b, X, T, *nm, "fp64")[0]) / (2 * h)
+            assert abs(fd - gW[l][i, j]) < 1e-7 * max(1.0, abs(fd)), (l, i, j)
+    finally:
+        M.ACT = "relu"
