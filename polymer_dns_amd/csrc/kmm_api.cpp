@@ -301,6 +301,22 @@ static int download_field(kmm_ctx* c, const double2* d, double* ref) {
   return KMM_OK;
 }
 
+// fields stored pencil-major on the device, [kx][kz][y]: the reference's
+// own order of a spectral field, so they move as plain copies
+static bool pencil_major(const kmm_ctx* c, int f) {
+  return c->cfg.polymer && f >= KMM_C11 && f <= KMM_STR33NM1;
+}
+static int upload_pm(kmm_ctx* c, double2* d, const double* ref) {
+  KMM_HIP(hipMemcpyAsync(d, ref, c->S * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  KMM_HIP(hipStreamSynchronize(c->stream));
+  return KMM_OK;
+}
+static int download_pm(kmm_ctx* c, const double2* d, double* ref) {
+  KMM_HIP(hipMemcpyAsync(ref, d, c->S * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+  KMM_HIP(hipStreamSynchronize(c->stream));
+  return KMM_OK;
+}
+
 static std::vector<double2> twiddles(int n) {
   std::vector<double2> t(n);
   const long double pi = 3.141592653589793238462643383279502884L;
@@ -785,7 +801,7 @@ extern "C" int kmm_put_field(kmm_ctx* c, int f, const double* h) {
     if (r) return r;
     c->nl_valid = false;
   }
-  if (double2* d = spectral_ptr(c, f)) return upload_field(c, d, h);
+  if (double2* d = spectral_ptr(c, f)) return pencil_major(c, f) ? upload_pm(c, d, h) : upload_field(c, d, h);
   if (double* m = mean_ptr(c, f)) {
     KMM_HIP(hipMemcpyAsync(m, h, c->nyp * sizeof(double), hipMemcpyHostToDevice, c->stream));
     KMM_HIP(hipStreamSynchronize(c->stream));
@@ -805,7 +821,7 @@ extern "C" int kmm_get_field(kmm_ctx* c, int f, double* h) {
     int r = materialize(c);
     if (r) return r;
   }
-  if (double2* d = spectral_ptr(c, f)) return download_field(c, d, h);
+  if (double2* d = spectral_ptr(c, f)) return pencil_major(c, f) ? download_pm(c, d, h) : download_field(c, d, h);
   if (double* m = mean_ptr(c, f)) {
     KMM_HIP(hipMemcpyAsync(h, m, c->nyp * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     KMM_HIP(hipStreamSynchronize(c->stream));

@@ -203,6 +203,40 @@ __device__ __forceinline__ void pf_store(double2* __restrict__ F, size_t P, size
   }
   if (lane == 0) F[(size_t)NY * P + p] = f.e;
 }
+// the same for a field laid out pencil-major, [pencil][y] (the reference's
+// own order of a spectral field): a pencil is NY+1 contiguous values
+template <int NY>
+__device__ __forceinline__ PFN<NY> pf_load_pm(const double2* __restrict__ F, size_t p, int lane) {
+  constexpr int L = PLay<NY>::L, S = PLay<NY>::S;
+  const double2* f0 = F + p * (size_t)(NY + 1);
+  PFN<NY> f;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int g = s * WAVE + lane;
+    if (g < L) {
+      f.a[s] = f0[2 * g];
+      f.b[s] = f0[2 * g + 1];
+    } else {
+      f.a[s] = f.b[s] = czero();
+    }
+  }
+  f.e = f0[NY];
+  return f;
+}
+template <int NY>
+__device__ __forceinline__ void pf_store_pm(double2* __restrict__ F, size_t p, int lane, const PFN<NY>& f) {
+  constexpr int L = PLay<NY>::L, S = PLay<NY>::S;
+  double2* f0 = F + p * (size_t)(NY + 1);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int g = s * WAVE + lane;
+    if (g < L) {
+      f0[2 * g] = f.a[s];
+      f0[2 * g + 1] = f.b[s];
+    }
+  }
+  if (lane == 0) f0[NY] = f.e;
+}
 // zero rows 0..NY of pencil p of a field in the transfer precision V
 template <int NY, typename V>
 __device__ __forceinline__ void pf_store_zero(V* __restrict__ F, size_t P, size_t p, int lane) {

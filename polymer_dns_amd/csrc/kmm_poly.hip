@@ -18,6 +18,12 @@
 //           y-derivatives, derivatives.f90:148-254; the x and z derivatives
 //           are taken in the plane stage).
 //
+// The conformation tensor and its histories (C, CN, CNM1, STRN, STRNM1) are
+// stored pencil-major, [pencil][y] (the reference's order of a spectral
+// field): a wave reads and writes its pencil as contiguous memory.  The
+// fields shared with the plane stage (the physical-y NLP and TP fields, the
+// force TF) keep the [y][pencil] layout of the transforms.
+//
 // MODE 0: step, MODE 1: startup (evaluate only), MODE 2: finish (part B only),
 // MODE 3: C from the physical-y fields in NLP[0..5] (a conformation tensor
 // rebuilt on the 3/2 grid and forward transformed in x and z), MODE 4: part B
@@ -55,7 +61,7 @@ __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
         C = phys_to_cheb_load<NY, V>(static_cast<const V*>(A.NLP[c]), P, p, buf, scr, twl, lane);
         if (is00) C = pf_real(C);
       }
-      pf_store<NY>(A.C[c], P, p, lane, C);
+      pf_store_pm<NY>(A.C[c], p, lane, C);
     }
     return;
   }
@@ -92,12 +98,12 @@ __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
     for (int c = 0; c < 6; ++c) {
       if (MODE == 0 || MODE == 2) {
         if (A.partB || MODE == 2) {
-          if (!A.closure) pf_store<NY>(A.CN[c], P, p, lane, z);
-          if (!A.closure || MODE == 2) pf_store<NY>(A.STRN[c], P, p, lane, z);
+          if (!A.closure) pf_store_pm<NY>(A.CN[c], p, lane, z);
+          if (!A.closure || MODE == 2) pf_store_pm<NY>(A.STRN[c], p, lane, z);
         }
-        if (A.copy || A.solve) { pf_store<NY>(A.CNM1[c], P, p, lane, z); pf_store<NY>(A.STRNM1[c], P, p, lane, z); }
+        if (A.copy || A.solve) { pf_store_pm<NY>(A.CNM1[c], p, lane, z); pf_store_pm<NY>(A.STRNM1[c], p, lane, z); }
       }
-      if (MODE == 0 && (A.init || A.solve)) pf_store<NY>(A.C[c], P, p, lane, z);
+      if (MODE == 0 && (A.init || A.solve)) pf_store_pm<NY>(A.C[c], p, lane, z);
       if (MODE != 2 && A.evaluate)
         for (int h = 0; h < 2; ++h) {
           pf_store_zero<NY, V>(static_cast<V*>(A.TP[c + 6 * h]), P, p, lane);
@@ -136,8 +142,8 @@ __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
       if (A.copy) {
         cnm1_copy = cn;
         snm1_copy = sn;
-        pf_store<NY>(A.CNM1[c], P, p, lane, cn);
-        pf_store<NY>(A.STRNM1[c], P, p, lane, sn);
+        pf_store_pm<NY>(A.CNM1[c], p, lane, cn);
+        pf_store_pm<NY>(A.STRNM1[c], p, lane, sn);
       }
       // a closure stress feeds only the force: in a step its smoothed copy
       // in STRN has no reader (an explicit finish still stores it)
@@ -146,8 +152,8 @@ __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
         cn = smooth_apply<NY>(cn, A.smy, sz, lane);
         sn = smooth_apply<NY>(sn, A.smy, sz, lane);
       }
-      if (!A.closure) pf_store<NY>(A.CN[c], P, p, lane, cn);
-      if (keep) pf_store<NY>(A.STRN[c], P, p, lane, sn);
+      if (!A.closure) pf_store_pm<NY>(A.CN[c], p, lane, cn);
+      if (keep) pf_store_pm<NY>(A.STRN[c], p, lane, sn);
       if (MODE == 2) continue;
     }
     PF C;
@@ -159,19 +165,19 @@ __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
           if (lane == 0) C.a[0] = make_double2(v0, 0.0);
         }
       } else {
-        C = pf_load<NY>(A.C[c], P, p, lane);
+        C = pf_load_pm<NY>(A.C[c], p, lane);
       }
       if (A.solve) {
         if (!haveB) {
-          cn = pf_load<NY>(A.CN[c], P, p, lane);
-          sn = pf_load<NY>(A.STRN[c], P, p, lane);
+          cn = pf_load_pm<NY>(A.CN[c], p, lane);
+          sn = pf_load_pm<NY>(A.STRN[c], p, lane);
         }
-        const PF cm = A.copy && haveB ? cnm1_copy : pf_load<NY>(A.CNM1[c], P, p, lane);
-        const PF sm = A.copy && haveB ? snm1_copy : pf_load<NY>(A.STRNM1[c], P, p, lane);
+        const PF cm = A.copy && haveB ? cnm1_copy : pf_load_pm<NY>(A.CNM1[c], p, lane);
+        const PF sm = A.copy && haveB ? snm1_copy : pf_load_pm<NY>(A.STRNM1[c], p, lane);
         // cNL = 3 (c_n - str_n/lambda) - (c_nm1 - str_nm1/lambda)
         const PF nl = pf_sub(pf_scale(pf_axpy(-ti, sn, cn), 3.0), pf_axpy(-ti, sm, cm));
-        pf_store<NY>(A.CNM1[c], P, p, lane, cn);
-        pf_store<NY>(A.STRNM1[c], P, p, lane, sn);
+        pf_store_pm<NY>(A.CNM1[c], p, lane, cn);
+        pf_store_pm<NY>(A.STRNM1[c], p, lane, sn);
         const PF d1 = cderiv<NY>(C, dy, lane);
         const PF d2 = cderiv<NY>(d1, dy, lane);
         const PF rhs = phirhs<NY>(C, d2, nl, pf_scale(nl, 2.0), k2, A.re, A.diffpoly, A.dt, A.theta);
@@ -181,9 +187,9 @@ __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
         C = tau_penta<NY>(rhs, k2 + A.gC, dy, A.tabC + (size_t)p * tab_width(NY), tt, tb, czero(), czero(), lane);
         if (is00) C = pf_real(C);
       }
-      if (A.init || A.solve) pf_store<NY>(A.C[c], P, p, lane, C);
+      if (A.init || A.solve) pf_store_pm<NY>(A.C[c], p, lane, C);
     } else {
-      C = pf_load<NY>(A.C[c], P, p, lane);
+      C = pf_load_pm<NY>(A.C[c], p, lane);
     }
     if (A.evaluate) {
       cheb_to_phys_store<NY, V>(C, buf, scr, twl, lane, static_cast<V*>(A.TP[c]), P, p);
