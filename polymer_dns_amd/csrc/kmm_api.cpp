@@ -301,10 +301,19 @@ static int download_field(kmm_ctx* c, const double2* d, double* ref) {
   return KMM_OK;
 }
 
-// fields stored pencil-major on the device, [kx][kz][y]: the reference's
-// own order of a spectral field, so they move as plain copies
+// fields stored pencil-major on the device, [kx][kz][y]: the state of the
+// step (v, omega_y, theta and their nonlinear-term histories, the FENE-P
+// conformation and its histories), read and written only by the pencil
+// kernels; the reference's own order of a spectral field, so they move as
+// plain copies
 static bool pencil_major(const kmm_ctx* c, int f) {
-  return c->cfg.polymer && f >= KMM_C11 && f <= KMM_STR33NM1;
+  switch (f) {
+    case KMM_V: case KMM_FN: case KMM_FNM1: case KMM_OMY: case KMM_GN: case KMM_GNM1:
+    case KMM_SCALAR: case KMM_SCN: case KMM_SCNM1:
+      return true;
+    default:
+      return c->cfg.polymer && f >= KMM_C11 && f <= KMM_STR33NM1;
+  }
 }
 static int upload_pm(kmm_ctx* c, double2* d, const double* ref) {
   KMM_HIP(hipMemcpyAsync(d, ref, c->S * sizeof(double2), hipMemcpyHostToDevice, c->stream));
@@ -2969,8 +2978,8 @@ extern "C" int kmm_statistics(kmm_ctx* c, double* prof, double* scal) {
   std::vector<double> u0(nyp), th(nyp, 0.0);
   KMM_HIP(hipMemcpyAsync(u0.data(), c->u0, nyp * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   if (g.scalar)
-    KMM_HIP(hipMemcpy2DAsync(th.data(), sizeof(double), c->TH, (size_t)c->P * sizeof(double2), sizeof(double), nyp,
-                             hipMemcpyDeviceToHost, c->stream));
+    KMM_HIP(hipMemcpy2DAsync(th.data(), sizeof(double), c->TH, sizeof(double2), sizeof(double), nyp,
+                             hipMemcpyDeviceToHost, c->stream));  // pencil 0, pencil-major
   // swirl strength: the tracers' field when this evaluation made one, else
   // computed here from the same z intermediates
   std::vector<double> swy(nyp, 0.0);
@@ -3103,7 +3112,7 @@ extern "C" int kmm_flowfield(kmm_ctx* c, double* out) {
     e = kmm::launch_swirl_plane(sa, true, c->stream);
   }
   if (e == hipSuccess && c->cfg.scalar) {
-    e = kmm::launch_cheb(c->ny, c->P, c->twM, c->TH, lt, 1, c->cfg.precision, c->stream);
+    e = kmm::launch_cheb(c->ny, c->P, c->twM, c->TH, lt, 2, c->cfg.precision, c->stream);  // pencil-major input
     for (int f = 0; f < 3; ++f) la.LT[f] = lt;
     la.out = d + 7 * P;
     if (e == hipSuccess) e = kmm::launch_mr_lap_plane(la, c->stream);

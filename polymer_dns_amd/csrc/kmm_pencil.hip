@@ -157,13 +157,13 @@ __device__ __forceinline__ void pencil_body(const PencilArgs& A, int blk) {
   if (nyq) {  // norm zeroes the z-Nyquist row (helpers.f90:149-156)
     PF z = pf_zero<NY>();
     if (MODE == 0) {
-      pf_store<NY>(A.V, P, p, lane, z); pf_store<NY>(A.OMY, P, p, lane, z);
-      pf_store<NY>(A.FNM1, P, p, lane, z); pf_store<NY>(A.GNM1, P, p, lane, z);
-      if (SCALAR) { pf_store<NY>(A.TH, P, p, lane, z); pf_store<NY>(A.SCNM1, P, p, lane, z); }
+      pf_store_pm<NY>(A.V, p, lane, z); pf_store_pm<NY>(A.OMY, p, lane, z);
+      pf_store_pm<NY>(A.FNM1, p, lane, z); pf_store_pm<NY>(A.GNM1, p, lane, z);
+      if (SCALAR) { pf_store_pm<NY>(A.TH, p, lane, z); pf_store_pm<NY>(A.SCNM1, p, lane, z); }
     }
     if (MODE == 2) {
-      pf_store<NY>(A.FN, P, p, lane, z); pf_store<NY>(A.GN, P, p, lane, z);
-      if (SCALAR) pf_store<NY>(A.SCN, P, p, lane, z);
+      pf_store_pm<NY>(A.FN, p, lane, z); pf_store_pm<NY>(A.GN, p, lane, z);
+      if (SCALAR) pf_store_pm<NY>(A.SCN, p, lane, z);
     }
     if (MODE != 2)
       for (int t = 0; t < nT + 3; ++t) {
@@ -220,15 +220,15 @@ __device__ __forceinline__ void pencil_body(const PencilArgs& A, int blk) {
         }
       }
       if (MODE == 2) {
-        pf_store<NY>(A.FN, P, p, lane, fn);
-        pf_store<NY>(A.GN, P, p, lane, gn);
-        if (SCALAR) pf_store<NY>(A.SCN, P, p, lane, sn);
+        pf_store_pm<NY>(A.FN, p, lane, fn);
+        pf_store_pm<NY>(A.GN, p, lane, gn);
+        if (SCALAR) pf_store_pm<NY>(A.SCN, p, lane, sn);
         return;
       }
     } else {
-      fn = pf_load<NY>(A.FN, P, p, lane);
-      gn = pf_load<NY>(A.GN, P, p, lane);
-      if (SCALAR) sn = pf_load<NY>(A.SCN, P, p, lane);
+      fn = pf_load_pm<NY>(A.FN, p, lane);
+      gn = pf_load_pm<NY>(A.GN, p, lane);
+      if (SCALAR) sn = pf_load_pm<NY>(A.SCN, p, lane);
       if (is00) { h1m = pf_load_real<NY>(A.h1n, lane); h3m = pf_load_real<NY>(A.h3n, lane); }
     }
   }
@@ -237,18 +237,18 @@ __device__ __forceinline__ void pencil_body(const PencilArgs& A, int blk) {
   // ---------------- scalar (main_dns.f90:324-358) ----------------------
   PF sc;
   if (SCALAR) {
-    sc = pf_load<NY>(A.TH, P, p, lane);
+    sc = pf_load_pm<NY>(A.TH, p, lane);
     if (MODE == 0) {
-      PF scnm1 = pf_load<NY>(A.SCNM1, P, p, lane);
+      PF scnm1 = pf_load_pm<NY>(A.SCNM1, p, lane);
       PF d1 = cderiv<NY>(sc, dy, lane);
       PF d2 = cderiv<NY>(d1, dy, lane);
       PF rhs = phirhs<NY>(sc, d2, sn, scnm1, k2, A.re, A.diff, A.dt, th);
-      pf_store<NY>(A.SCNM1, P, p, lane, sn);
+      pf_store_pm<NY>(A.SCNM1, p, lane, sn);
       auto tt = [dy](int n) -> double { return dy * (double)n * (double)n; };
       auto tb = [dy](int n) -> double { double v = dy * (double)n * (double)n; return (n & 1) ? -v : v; };
       sc = tau_penta<NY>(rhs, k2 + A.gS, dy, A.tabS + (size_t)p * tab_width(NY), tt, tb, czero(), czero(), lane);
       if (is00) sc = pf_real(sc);
-      pf_store<NY>(A.TH, P, p, lane, sc);
+      pf_store_pm<NY>(A.TH, p, lane, sc);
     }
   }
 
@@ -257,15 +257,15 @@ __device__ __forceinline__ void pencil_body(const PencilArgs& A, int blk) {
   PF u0p, w0p;
   if (MODE == 0) {
     if (!is00) {
-      v = pf_load<NY>(A.V, P, p, lane);
-      PF fnm1 = pf_load<NY>(A.FNM1, P, p, lane);
+      v = pf_load_pm<NY>(A.V, p, lane);
+      PF fnm1 = pf_load_pm<NY>(A.FNM1, p, lane);
       PF d1 = cderiv<NY>(v, dy, lane);
       PF d2 = cderiv<NY>(d1, dy, lane);
       PF phi = pf_sub(d2, pf_scale(v, k2));
       d1 = cderiv<NY>(phi, dy, lane);
       d2 = cderiv<NY>(d1, dy, lane);
       PF rhs = phirhs<NY>(phi, d2, fn, fnm1, k2, A.re, 1.0, A.dt, th);
-      pf_store<NY>(A.FNM1, P, p, lane, fn);
+      pf_store_pm<NY>(A.FNM1, p, lane, fn);
       phi = tau_solve<NY>(rhs, k2 + A.gH, dy, A.tabH + (size_t)p * tab_width(NY), czero(), czero(), 0, lane);
       PF vp = tau_solve<NY>(phi, k2, dy, A.tabP + (size_t)p * tab_width(NY), czero(), czero(), 0, lane);
       double2 vt, vb;
@@ -275,27 +275,27 @@ __device__ __forceinline__ void pencil_body(const PencilArgs& A, int blk) {
       double2 c2 = cadd(cscale(vb, g.z), cscale(vt, g.w));
       v = pf_add(vp, pf_add(pf_cmul_real(c1, green_load<NY>(A.G1, p, lane)),
                             pf_cmul_real(c2, green_load<NY>(A.G2, p, lane))));
-      omy = pf_load<NY>(A.OMY, P, p, lane);
-      PF gnm1 = pf_load<NY>(A.GNM1, P, p, lane);
+      omy = pf_load_pm<NY>(A.OMY, p, lane);
+      PF gnm1 = pf_load_pm<NY>(A.GNM1, p, lane);
       PF e1 = cderiv<NY>(omy, dy, lane);
       PF e2 = cderiv<NY>(e1, dy, lane);
       PF orhs = phirhs<NY>(omy, e2, gn, gnm1, k2, A.re, 1.0, A.dt, th);
-      pf_store<NY>(A.GNM1, P, p, lane, gn);
+      pf_store_pm<NY>(A.GNM1, p, lane, gn);
       const double atu = A.atu, btu = A.btu, abu = A.abu, bbu = A.bbu;
       auto tt = [=](int n) -> double { return atu + dy * btu * (double)n * (double)n; };
       auto tb = [=](int n) -> double { double t = abu + dy * bbu * (double)n * (double)n; return (n & 1) ? -t : t; };
       omy = tau_penta<NY>(orhs, k2 + A.gH, dy, A.tabH + (size_t)p * tab_width(NY), tt, tb, czero(), czero(), lane);
-      pf_store<NY>(A.V, P, p, lane, v);
-      pf_store<NY>(A.OMY, P, p, lane, omy);
+      pf_store_pm<NY>(A.V, p, lane, v);
+      pf_store_pm<NY>(A.OMY, p, lane, omy);
     } else {
       // (0,0): v = 0 (veloc), omega_y = 0, fnm1 = fn = 0, gnm1 = gn = 0,
       // and the mean-flow solves (uzero, solvers.f90:1065-1176).
       v = pf_zero<NY>();
       omy = pf_zero<NY>();
-      pf_store<NY>(A.V, P, p, lane, v);
-      pf_store<NY>(A.OMY, P, p, lane, omy);
-      pf_store<NY>(A.FNM1, P, p, lane, pf_zero<NY>());
-      pf_store<NY>(A.GNM1, P, p, lane, pf_zero<NY>());
+      pf_store_pm<NY>(A.V, p, lane, v);
+      pf_store_pm<NY>(A.OMY, p, lane, omy);
+      pf_store_pm<NY>(A.FNM1, p, lane, pf_zero<NY>());
+      pf_store_pm<NY>(A.GNM1, p, lane, pf_zero<NY>());
       for (int w = 0; w < 2; ++w) {
         double* um = w ? A.w0 : A.u0;
         double* hl = w ? A.h3nm1 : A.h1nm1;
@@ -338,12 +338,12 @@ __device__ __forceinline__ void pencil_body(const PencilArgs& A, int blk) {
       }
     }
   } else {  // MODE 1 (startup): current state only
-    v = pf_load<NY>(A.V, P, p, lane);
-    omy = pf_load<NY>(A.OMY, P, p, lane);
+    v = pf_load_pm<NY>(A.V, p, lane);
+    omy = pf_load_pm<NY>(A.OMY, p, lane);
     if (is00) {
       v = pf_zero<NY>(); omy = pf_zero<NY>();
-      pf_store<NY>(A.V, P, p, lane, v);
-      pf_store<NY>(A.OMY, P, p, lane, omy);
+      pf_store_pm<NY>(A.V, p, lane, v);
+      pf_store_pm<NY>(A.OMY, p, lane, omy);
       u0p = pf_load_real<NY>(A.u0, lane);
       w0p = pf_load_real<NY>(A.w0, lane);
     }
@@ -384,19 +384,19 @@ __device__ __forceinline__ void pencil_body(const PencilArgs& A, int blk) {
   }
 }
 
-// The full step (MODE 0) with every [y][pencil] field of the block staged
-// through LDS tiles (coalesced rows of PW pencils): the nonlinear terms
-// and the six state fields are read a block at a time, the state is
-// updated in place in its tiles, and the transfer fields leave through
-// tiles after their inverse transforms.  Same operations in the same order
-// per pencil as pencil_body, so the results are identical; every wave of
-// the block, the z-Nyquist pencil's included, passes every block barrier.
+// The full step (MODE 0) with the [y][pencil] fields of the block (the
+// nonlinear terms in, the transfer fields out) staged through LDS tiles
+// (coalesced rows of PWS pencils); the state, stored pencil-major, is read
+// and written by each wave as contiguous memory.  Same operations in the
+// same order per pencil as pencil_body, so the results are identical; every
+// wave of the block, the z-Nyquist pencil's included, passes every block
+// barrier.
 template <int NY, bool SCALAR, typename V>
 __device__ __forceinline__ void pencil_step_tiled(const PencilArgs& A, int blk) {
   using PF = PFN<NY>;
   using G = TileG<NY>;
   using LP = StepLds<NY, V, SCALAR>;
-  constexpr int NNL = LP::NNL, NST = LP::NST, NTS = LP::NTS;
+  constexpr int NNL = LP::NNL, NTS = LP::NTS;
   extern __shared__ __align__(16) unsigned char smraw[];
   V* smem = reinterpret_cast<V*>(smraw);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -406,7 +406,6 @@ __device__ __forceinline__ void pencil_step_tiled(const PencilArgs& A, int blk) 
   V* scr = buf;  // transforms in place
   unsigned char* region = smraw + LP::TWB;
   V* vtile = reinterpret_cast<V*>(region + LP::FFT);     // NL / transfer tiles after the FFT buffers
-  double2* stile = reinterpret_cast<double2*>(region);  // state tiles over the whole region
   V* const* TT = reinterpret_cast<V* const*>(A.T);
   const size_t P = (size_t)A.P;
   const bool live = (size_t)p < P;
@@ -419,9 +418,22 @@ __device__ __forceinline__ void pencil_step_tiled(const PencilArgs& A, int blk) 
   const bool act = live && !nyq;  // computes; the z-Nyquist pencil is zeroed (helpers.f90:149-156)
   const double dy = A.dyde;
   const double th = A.theta;
-  // state tiles: 0 V, 1 FNM1, 2 OMY, 3 GNM1, 4 TH, 5 SCNM1
-  double2* const SF[6] = {A.V, A.FNM1, A.OMY, A.GNM1, A.TH, A.SCNM1};
-  double2 rs[NST][G::PER];
+  // the state is stored pencil-major: each wave loads its own pencils
+  // (contiguous), issued ahead of the nonlinear-term transforms
+  PF sc_in, scnm1_in, v_in, fnm1_in, omy_in, gnm1_in;
+  auto load_state = [&]() {
+    if (!act) return;
+    if (SCALAR) {
+      sc_in = pf_load_pm<NY>(A.TH, p, lane);
+      scnm1_in = pf_load_pm<NY>(A.SCNM1, p, lane);
+    }
+    if (!is00) {
+      v_in = pf_load_pm<NY>(A.V, p, lane);
+      fnm1_in = pf_load_pm<NY>(A.FNM1, p, lane);
+      omy_in = pf_load_pm<NY>(A.OMY, p, lane);
+      gnm1_in = pf_load_pm<NY>(A.GNM1, p, lane);
+    }
+  };
 
   // ---------------- part B: nonlinear terms -> fn, gn, scn (spectral) --
   PF fn, gn, sn;
@@ -433,8 +445,7 @@ __device__ __forceinline__ void pencil_step_tiled(const PencilArgs& A, int blk) 
                          static_cast<const V*>(A.NL3), static_cast<const V*>(A.NLS)};
 #pragma unroll
       for (int f = 0; f < NNL; ++f) tile_fetch<NY, V>(NLF[f], P, p0, rn[f]);
-#pragma unroll
-      for (int f = 0; f < NST; ++f) tile_fetch<NY, double2>(SF[f], P, p0, rs[f]);
+      load_state();
 #pragma unroll
       for (int f = 0; f < NNL; ++f) tile_put<NY, V>(vtile + (size_t)f * G::LEN, rn[f]);
     }
@@ -468,56 +479,46 @@ __device__ __forceinline__ void pencil_step_tiled(const PencilArgs& A, int blk) 
         if (is00) sn = pf_real(sn);
       }
     }
-    __syncthreads();  // the tiles and FFT buffers become the state tiles
+    __syncthreads();  // the NL tiles become transfer tiles
   } else {
-#pragma unroll
-    for (int f = 0; f < NST; ++f) tile_fetch<NY, double2>(SF[f], P, p0, rs[f]);
+    load_state();
     if (act) {
-      fn = pf_load<NY>(A.FN, P, p, lane);
-      gn = pf_load<NY>(A.GN, P, p, lane);
-      if (SCALAR) sn = pf_load<NY>(A.SCN, P, p, lane);
+      fn = pf_load_pm<NY>(A.FN, p, lane);
+      gn = pf_load_pm<NY>(A.GN, p, lane);
+      if (SCALAR) sn = pf_load_pm<NY>(A.SCN, p, lane);
       if (is00) { h1m = pf_load_real<NY>(A.h1n, lane); h3m = pf_load_real<NY>(A.h3n, lane); }
     }
   }
-#pragma unroll
-  for (int f = 0; f < NST; ++f) tile_put<NY, double2>(stile + (size_t)f * G::LEN, rs[f]);
-  __syncthreads();
 
-  // ---------------- solves, in place in the state tiles ----------------
-  double2* tV = stile;
-  double2* tFNM1 = stile + G::LEN;
-  double2* tOMY = stile + 2 * G::LEN;
-  double2* tGNM1 = stile + 3 * G::LEN;
-  double2* tTH = stile + 4 * G::LEN;
-  double2* tSCNM1 = stile + 5 * G::LEN;
+  // ---------------- solves ----------------------------------------------
   PF sc, v, omy;
   PF u0p, w0p;
   if (act) {
     // scalar (main_dns.f90:324-358)
     if (SCALAR) {
-      sc = tile_col<NY>(tTH, wv, lane);
-      const PF scnm1 = tile_col<NY>(tSCNM1, wv, lane);
+      sc = sc_in;
+      const PF& scnm1 = scnm1_in;
       PF d1 = cderiv<NY>(sc, dy, lane);
       PF d2 = cderiv<NY>(d1, dy, lane);
       PF rhs = phirhs<NY>(sc, d2, sn, scnm1, k2, A.re, A.diff, A.dt, th);
-      tile_col_put<NY>(tSCNM1, wv, lane, sn);
+      pf_store_pm<NY>(A.SCNM1, p, lane, sn);
       auto tt = [dy](int n) -> double { return dy * (double)n * (double)n; };
       auto tb = [dy](int n) -> double { double v = dy * (double)n * (double)n; return (n & 1) ? -v : v; };
       sc = tau_penta<NY>(rhs, k2 + A.gS, dy, A.tabS + (size_t)p * tab_width(NY), tt, tb, czero(), czero(), lane);
       if (is00) sc = pf_real(sc);
-      tile_col_put<NY>(tTH, wv, lane, sc);
+      pf_store_pm<NY>(A.TH, p, lane, sc);
     }
     // v and omega_y (main_dns.f90:360-502)
     if (!is00) {
-      v = tile_col<NY>(tV, wv, lane);
-      const PF fnm1 = tile_col<NY>(tFNM1, wv, lane);
+      v = v_in;
+      const PF& fnm1 = fnm1_in;
       PF d1 = cderiv<NY>(v, dy, lane);
       PF d2 = cderiv<NY>(d1, dy, lane);
       PF phi = pf_sub(d2, pf_scale(v, k2));
       d1 = cderiv<NY>(phi, dy, lane);
       d2 = cderiv<NY>(d1, dy, lane);
       PF rhs = phirhs<NY>(phi, d2, fn, fnm1, k2, A.re, 1.0, A.dt, th);
-      tile_col_put<NY>(tFNM1, wv, lane, fn);
+      pf_store_pm<NY>(A.FNM1, p, lane, fn);
       phi = tau_solve<NY>(rhs, k2 + A.gH, dy, A.tabH + (size_t)p * tab_width(NY), czero(), czero(), 0, lane);
       PF vp = tau_solve<NY>(phi, k2, dy, A.tabP + (size_t)p * tab_width(NY), czero(), czero(), 0, lane);
       double2 vt, vb;
@@ -527,27 +528,27 @@ __device__ __forceinline__ void pencil_step_tiled(const PencilArgs& A, int blk) 
       double2 c2 = cadd(cscale(vb, g.z), cscale(vt, g.w));
       v = pf_add(vp, pf_add(pf_cmul_real(c1, green_load<NY>(A.G1, p, lane)),
                             pf_cmul_real(c2, green_load<NY>(A.G2, p, lane))));
-      omy = tile_col<NY>(tOMY, wv, lane);
-      const PF gnm1 = tile_col<NY>(tGNM1, wv, lane);
+      omy = omy_in;
+      const PF& gnm1 = gnm1_in;
       PF e1 = cderiv<NY>(omy, dy, lane);
       PF e2 = cderiv<NY>(e1, dy, lane);
       PF orhs = phirhs<NY>(omy, e2, gn, gnm1, k2, A.re, 1.0, A.dt, th);
-      tile_col_put<NY>(tGNM1, wv, lane, gn);
+      pf_store_pm<NY>(A.GNM1, p, lane, gn);
       const double atu = A.atu, btu = A.btu, abu = A.abu, bbu = A.bbu;
       auto tt = [=](int n) -> double { return atu + dy * btu * (double)n * (double)n; };
       auto tb = [=](int n) -> double { double t = abu + dy * bbu * (double)n * (double)n; return (n & 1) ? -t : t; };
       omy = tau_penta<NY>(orhs, k2 + A.gH, dy, A.tabH + (size_t)p * tab_width(NY), tt, tb, czero(), czero(), lane);
-      tile_col_put<NY>(tV, wv, lane, v);
-      tile_col_put<NY>(tOMY, wv, lane, omy);
+      pf_store_pm<NY>(A.V, p, lane, v);
+      pf_store_pm<NY>(A.OMY, p, lane, omy);
     } else {
       // (0,0): v = 0 (veloc), omega_y = 0, fnm1 = fn = 0, gnm1 = gn = 0,
       // and the mean-flow solves (uzero, solvers.f90:1065-1176).
       v = pf_zero<NY>();
       omy = pf_zero<NY>();
-      tile_col_put<NY>(tV, wv, lane, v);
-      tile_col_put<NY>(tOMY, wv, lane, omy);
-      tile_col_put<NY>(tFNM1, wv, lane, pf_zero<NY>());
-      tile_col_put<NY>(tGNM1, wv, lane, pf_zero<NY>());
+      pf_store_pm<NY>(A.V, p, lane, v);
+      pf_store_pm<NY>(A.OMY, p, lane, omy);
+      pf_store_pm<NY>(A.FNM1, p, lane, pf_zero<NY>());
+      pf_store_pm<NY>(A.GNM1, p, lane, pf_zero<NY>());
       for (int w = 0; w < 2; ++w) {
         double* um = w ? A.w0 : A.u0;
         double* hl = w ? A.h3nm1 : A.h1nm1;
@@ -584,14 +585,17 @@ __device__ __forceinline__ void pencil_step_tiled(const PencilArgs& A, int blk) 
         if (w) w0p = un; else u0p = un;
       }
     }
-  } else if (live) {
-#pragma unroll
-    for (int f = 0; f < NST; ++f) tile_col_put<NY>(stile + (size_t)f * G::LEN, wv, lane, pf_zero<NY>());
+  } else if (live) {  // the z-Nyquist pencil: zero state
+    const PF z = pf_zero<NY>();
+    pf_store_pm<NY>(A.V, p, lane, z);
+    pf_store_pm<NY>(A.OMY, p, lane, z);
+    pf_store_pm<NY>(A.FNM1, p, lane, z);
+    pf_store_pm<NY>(A.GNM1, p, lane, z);
+    if (SCALAR) {
+      pf_store_pm<NY>(A.TH, p, lane, z);
+      pf_store_pm<NY>(A.SCNM1, p, lane, z);
+    }
   }
-  __syncthreads();
-#pragma unroll
-  for (int f = 0; f < NST; ++f) tile_store<NY, double2>(stile + (size_t)f * G::LEN, SF[f], P, p0);
-  __syncthreads();  // the region becomes FFT buffers and transfer tiles
 
   // ---------------- veloc / vort / gradscl (solvers.f90:1576-1722) ------
   PF u, w;
@@ -740,8 +744,8 @@ __global__ __launch_bounds__(PW * 64) void k_mr_lap(MrLapArgs A) {
       f[0] = pf_load_real<NY>(A.u0, lane);
       f[2] = pf_load_real<NY>(A.w0, lane);
     } else {
-      const PF v = pf_load<NY>(A.V, P, p, lane);
-      const PF omy = pf_load<NY>(A.OMY, P, p, lane);
+      const PF v = pf_load_pm<NY>(A.V, p, lane);
+      const PF omy = pf_load_pm<NY>(A.OMY, p, lane);
       const PF dv = cderiv<NY>(v, dy, lane);
       const double ik2 = 1.0 / k2;
       f[0] = pf_scale(pf_sub(pf_imul(dv, kx), pf_imul(omy, kz)), ik2);
@@ -769,8 +773,9 @@ __global__ __launch_bounds__(PW * 64) void k_cheb(int P, const V* __restrict__ t
   const V* twl = lds_twiddles<NY, V>(smem, tw);
   if (p >= P) return;
   V* buf = wave_lds<NY, V>(smem, wv);
-  if (inverse) {
-    PF f = pf_load<NY>(static_cast<const double2*>(in), P, p, lane);
+  if (inverse) {  // 2: the spectral input is stored pencil-major
+    PF f = inverse == 2 ? pf_load_pm<NY>(static_cast<const double2*>(in), p, lane)
+                        : pf_load<NY>(static_cast<const double2*>(in), P, p, lane);
     cheb_to_phys_store<NY, V>(f, buf, buf + FFTBuf<2 * NY>::len, twl, lane, static_cast<V*>(out), P, p);
   } else {
     PF r = phys_to_cheb_load<NY, V>(static_cast<const V*>(in), P, p, buf, buf + FFTBuf<2 * NY>::len, twl, lane);
