@@ -931,19 +931,20 @@ constexpr bool fft_in_place() {
 }
 
 // LDS plan of the staged step kernel (bytes): twiddles, then a region used
-// as [FFT buffers | nonlinear-term tiles] while the nonlinear terms are
-// transformed and as [FFT buffers | transfer tiles] for the inverse
-// transforms.
+// as [FFT buffers | two nonlinear-term tiles] while the nonlinear terms are
+// transformed (two rounds of two) and as [FFT buffers | transfer tiles] for
+// the inverse transforms; at most 54 KB, so that three blocks of four waves
+// share a CU (three waves per SIMD at the step kernel's 168 registers).
 template <int NY, typename V, bool SCALAR>
 struct StepLds {
-  static constexpr size_t CAP = 80 * 1024;  // two blocks per CU
+  static constexpr size_t CAP = 54 * 1024;  // three blocks per CU
   static constexpr size_t TWB = (size_t)2 * NY * sizeof(V);
   static constexpr size_t FFT = (size_t)PWS * FFTBuf<2 * NY>::len * sizeof(V);  // one in-place buffer per wave
   static constexpr size_t TV = (size_t)TileG<NY>::LEN * sizeof(V);
-  static constexpr int NNL = SCALAR ? 4 : 3, NOUT = SCALAR ? 9 : 6;
+  static constexpr int NOUT = SCALAR ? 9 : 6;
   static constexpr int fit = (CAP > TWB + FFT) ? (int)((CAP - TWB - FFT) / TV) : 0;
   static constexpr int NTS = fit < NOUT ? fit : NOUT;  // transfer tiles per flush
-  static constexpr size_t A1 = FFT + NNL * TV, A3 = FFT + (size_t)NTS * TV;
+  static constexpr size_t A1 = FFT + 2 * TV, A3 = FFT + (size_t)NTS * TV;
   static constexpr size_t REGION = A1 > A3 ? A1 : A3;
   static constexpr bool ok = NTS >= 3 && TWB + REGION <= CAP && PLay<NY>::S == 1 &&
                              fft_in_place<2 * NY, kChebPlan>();

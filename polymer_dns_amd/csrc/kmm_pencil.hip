@@ -396,7 +396,7 @@ __device__ __forceinline__ void pencil_step_tiled(const PencilArgs& A, int blk) 
   using PF = PFN<NY>;
   using G = TileG<NY>;
   using LP = StepLds<NY, V, SCALAR>;
-  constexpr int NNL = LP::NNL, NTS = LP::NTS;
+  constexpr int NTS = LP::NTS;
   extern __shared__ __align__(16) unsigned char smraw[];
   V* smem = reinterpret_cast<V*>(smraw);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -418,43 +418,33 @@ __device__ __forceinline__ void pencil_step_tiled(const PencilArgs& A, int blk) 
   const bool act = live && !nyq;  // computes; the z-Nyquist pencil is zeroed (helpers.f90:149-156)
   const double dy = A.dyde;
   const double th = A.theta;
-  // the state is stored pencil-major: each wave loads its own pencils
-  // (contiguous), issued ahead of the nonlinear-term transforms
-  PF sc_in, scnm1_in, v_in, fnm1_in, omy_in, gnm1_in;
-  auto load_state = [&]() {
-    if (!act) return;
-    if (SCALAR) {
-      sc_in = pf_load_pm<NY>(A.TH, p, lane);
-      scnm1_in = pf_load_pm<NY>(A.SCNM1, p, lane);
-    }
-    if (!is00) {
-      v_in = pf_load_pm<NY>(A.V, p, lane);
-      fnm1_in = pf_load_pm<NY>(A.FNM1, p, lane);
-      omy_in = pf_load_pm<NY>(A.OMY, p, lane);
-      gnm1_in = pf_load_pm<NY>(A.GNM1, p, lane);
-    }
-  };
-
   // ---------------- part B: nonlinear terms -> fn, gn, scn (spectral) --
   PF fn, gn, sn;
   PF h1m, h3m;  // mean-mode profiles (only used by the (0,0) pencil)
   if (A.first_nl) {
+    V r2[2][G::PER];
     {
-      V rn[NNL][G::PER];
-      const V* NLF[4] = {static_cast<const V*>(A.NL1), static_cast<const V*>(A.NL2),
-                         static_cast<const V*>(A.NL3), static_cast<const V*>(A.NLS)};
-#pragma unroll
-      for (int f = 0; f < NNL; ++f) tile_fetch<NY, V>(NLF[f], P, p0, rn[f]);
-      load_state();
-#pragma unroll
-      for (int f = 0; f < NNL; ++f) tile_put<NY, V>(vtile + (size_t)f * G::LEN, rn[f]);
+      V rn[2][G::PER];
+      tile_fetch<NY, V>(static_cast<const V*>(A.NL1), P, p0, rn[0]);
+      tile_fetch<NY, V>(static_cast<const V*>(A.NL3), P, p0, rn[1]);
+      tile_fetch<NY, V>(static_cast<const V*>(A.NL2), P, p0, r2[0]);
+      if (SCALAR) tile_fetch<NY, V>(static_cast<const V*>(A.NLS), P, p0, r2[1]);
+      tile_put<NY, V>(vtile, rn[0]);
+      tile_put<NY, V>(vtile + G::LEN, rn[1]);
     }
+    __syncthreads();
+    PF h1, h3, h2;
+    if (act) {
+      h1 = phys_to_cheb<NY, V>(tile_phys<NY, V>(vtile, wv, lane), buf, scr, twl, lane);
+      h3 = phys_to_cheb<NY, V>(tile_phys<NY, V>(vtile + G::LEN, wv, lane), buf, scr, twl, lane);
+    }
+    __syncthreads();
+    tile_put<NY, V>(vtile, r2[0]);
+    if (SCALAR) tile_put<NY, V>(vtile + G::LEN, r2[1]);
     __syncthreads();
     if (act) {
       const double sz = A.smooth ? A.smz[kzi] : 1.0;
-      PF h1 = phys_to_cheb<NY, V>(tile_phys<NY, V>(vtile, wv, lane), buf, scr, twl, lane);
-      PF h3 = phys_to_cheb<NY, V>(tile_phys<NY, V>(vtile + 2 * G::LEN, wv, lane), buf, scr, twl, lane);
-      PF h2 = phys_to_cheb<NY, V>(tile_phys<NY, V>(vtile + G::LEN, wv, lane), buf, scr, twl, lane);
+      h2 = phys_to_cheb<NY, V>(tile_phys<NY, V>(vtile, wv, lane), buf, scr, twl, lane);
       if (A.TF[0]) {  // polymer force, subforce (helpers.f90:564-606), before smoothing
         h1 = pf_add(h1, pf_load<NY>(A.TF[0], P, p, lane));
         h2 = pf_add(h2, pf_load<NY>(A.TF[1], P, p, lane));
@@ -474,14 +464,13 @@ __device__ __forceinline__ void pencil_step_tiled(const PencilArgs& A, int blk) 
       fn = pf_sub(pf_scale(h2, -k2), dx);
       gn = pf_sub(pf_imul(h1, kz), pf_imul(h3, kx));
       if (SCALAR) {
-        sn = phys_to_cheb<NY, V>(tile_phys<NY, V>(vtile + 3 * G::LEN, wv, lane), buf, scr, twl, lane);
+        sn = phys_to_cheb<NY, V>(tile_phys<NY, V>(vtile + G::LEN, wv, lane), buf, scr, twl, lane);
         if (A.smooth) sn = smooth_apply<NY>(sn, A.smy, sz, lane);
         if (is00) sn = pf_real(sn);
       }
     }
     __syncthreads();  // the NL tiles become transfer tiles
   } else {
-    load_state();
     if (act) {
       fn = pf_load_pm<NY>(A.FN, p, lane);
       gn = pf_load_pm<NY>(A.GN, p, lane);
@@ -491,13 +480,16 @@ __device__ __forceinline__ void pencil_step_tiled(const PencilArgs& A, int blk) 
   }
 
   // ---------------- solves ----------------------------------------------
+  // The state is stored pencil-major: each wave loads its pencils
+  // (contiguous) where the solves use them, which keeps the kernel within
+  // the registers of three waves per SIMD.
   PF sc, v, omy;
   PF u0p, w0p;
   if (act) {
     // scalar (main_dns.f90:324-358)
     if (SCALAR) {
-      sc = sc_in;
-      const PF& scnm1 = scnm1_in;
+      sc = pf_load_pm<NY>(A.TH, p, lane);
+      const PF scnm1 = pf_load_pm<NY>(A.SCNM1, p, lane);
       PF d1 = cderiv<NY>(sc, dy, lane);
       PF d2 = cderiv<NY>(d1, dy, lane);
       PF rhs = phirhs<NY>(sc, d2, sn, scnm1, k2, A.re, A.diff, A.dt, th);
@@ -510,8 +502,8 @@ __device__ __forceinline__ void pencil_step_tiled(const PencilArgs& A, int blk) 
     }
     // v and omega_y (main_dns.f90:360-502)
     if (!is00) {
-      v = v_in;
-      const PF& fnm1 = fnm1_in;
+      v = pf_load_pm<NY>(A.V, p, lane);
+      const PF fnm1 = pf_load_pm<NY>(A.FNM1, p, lane);
       PF d1 = cderiv<NY>(v, dy, lane);
       PF d2 = cderiv<NY>(d1, dy, lane);
       PF phi = pf_sub(d2, pf_scale(v, k2));
@@ -528,8 +520,8 @@ __device__ __forceinline__ void pencil_step_tiled(const PencilArgs& A, int blk) 
       double2 c2 = cadd(cscale(vb, g.z), cscale(vt, g.w));
       v = pf_add(vp, pf_add(pf_cmul_real(c1, green_load<NY>(A.G1, p, lane)),
                             pf_cmul_real(c2, green_load<NY>(A.G2, p, lane))));
-      omy = omy_in;
-      const PF& gnm1 = gnm1_in;
+      omy = pf_load_pm<NY>(A.OMY, p, lane);
+      const PF gnm1 = pf_load_pm<NY>(A.GNM1, p, lane);
       PF e1 = cderiv<NY>(omy, dy, lane);
       PF e2 = cderiv<NY>(e1, dy, lane);
       PF orhs = phirhs<NY>(omy, e2, gn, gnm1, k2, A.re, 1.0, A.dt, th);
@@ -631,13 +623,9 @@ __device__ __forceinline__ void pencil_step_tiled(const PencilArgs& A, int blk) 
     }
     flush(0, 3, D);
   }
-  PF du, dw, ox, oz;
-  if (act) {
-    du = cderiv<NY>(u, dy, lane);
-    dw = cderiv<NY>(w, dy, lane);
-    ox = pf_sub(dw, pf_imul(v, kz));
-    oz = pf_sub(pf_imul(v, kx), du);
-  }
+  // each transfer field is formed just before its transform, and
+  // D u, D w right after u, w leave: at most five pencil fields stay live
+  PF du, dw;
   constexpr int NOUT = LP::NOUT;
 #pragma unroll
   for (int j = 0; j < NOUT; ++j) {
@@ -647,13 +635,15 @@ __device__ __forceinline__ void pencil_step_tiled(const PencilArgs& A, int blk) 
       if (j == 0) o = u;
       else if (j == 1) o = v;
       else if (j == 2) o = w;
-      else if (j == 3) o = ox;
+      else if (j == 3) o = pf_sub(dw, pf_imul(v, kz));  // omega_x
       else if (j == 4) o = omy;
-      else if (j == 5) o = oz;
+      else if (j == 5) o = pf_sub(pf_imul(v, kx), du);  // omega_z
       else if (j == 6) o = pf_imul(sc, kx);
       else if (j == 7) o = cderiv<NY>(sc, dy, lane);
       else o = pf_imul(sc, kz);
       cheb_to_phys_tile<NY, V>(o, buf, scr, twl, lane, t, wv);
+      if (j == 0) du = cderiv<NY>(u, dy, lane);
+      if (j == 2) dw = cderiv<NY>(w, dy, lane);
     } else if (live) {
       tile_col_zero<NY, V>(t, wv, lane);
     }
