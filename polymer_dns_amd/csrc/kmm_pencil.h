@@ -937,7 +937,8 @@ constexpr bool fft_in_place() {
 // share a CU (three waves per SIMD at the step kernel's 168 registers).
 template <int NY, typename V, bool SCALAR>
 struct StepLds {
-  static constexpr size_t CAP = 54 * 1024;  // three blocks per CU
+  // three blocks per CU up to ny = 128, two at ny = 192
+  static constexpr size_t CAP = (NY <= 128 ? 54 : 80) * 1024;
   static constexpr size_t TWB = (size_t)2 * NY * sizeof(V);
   static constexpr size_t FFT = (size_t)PWS * FFTBuf<2 * NY>::len * sizeof(V);  // one in-place buffer per wave
   static constexpr size_t TV = (size_t)TileG<NY>::LEN * sizeof(V);
@@ -946,8 +947,7 @@ struct StepLds {
   static constexpr int NTS = fit < NOUT ? fit : NOUT;  // transfer tiles per flush
   static constexpr size_t A1 = FFT + 2 * TV, A3 = FFT + (size_t)NTS * TV;
   static constexpr size_t REGION = A1 > A3 ? A1 : A3;
-  static constexpr bool ok = NTS >= 3 && TWB + REGION <= CAP && PLay<NY>::S == 1 &&
-                             fft_in_place<2 * NY, kChebPlan>();
+  static constexpr bool ok = NTS >= 3 && TWB + REGION <= CAP && fft_in_place<2 * NY, kChebPlan>();
   static constexpr size_t bytes = TWB + REGION;
 };
 
