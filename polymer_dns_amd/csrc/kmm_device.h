@@ -239,6 +239,17 @@ __host__ __device__ constexpr FFTPlan make_plan(int n, int model = 0) {
   return p;
 }
 
+// A wave's FFT can run in place (one buffer) when every stage of its plan
+// is one round (N / R <= 64): a stage's LDS reads of all lanes then precede
+// its writes, which depend on them.
+template <int N, int MODEL>
+constexpr bool fft_in_place() {
+  constexpr FFTPlan P = make_plan(N, MODEL);
+  for (int s = 0; s < P.nst; ++s)
+    if (N / P.r[s] > WAVE) return false;
+  return true;
+}
+
 // twiddle w^k with w = exp(-2 pi i / N) from a table of N entries; DIR=+1
 // (FFTW_BACKWARD) uses the conjugate.
 template <int DIR, typename V>

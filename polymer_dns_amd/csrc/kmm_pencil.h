@@ -919,17 +919,6 @@ __device__ __forceinline__ int xcd_group(int b, int nb) {
   return (s / 4) * 32 + x * 4 + s % 4;
 }
 
-// A wave's FFT can run in place (one buffer) when every stage of its plan
-// is one round (N / R <= 64): a stage's LDS reads of all lanes then precede
-// its writes, which depend on them.
-template <int N, int MODEL>
-constexpr bool fft_in_place() {
-  constexpr FFTPlan P = make_plan(N, MODEL);
-  for (int s = 0; s < P.nst; ++s)
-    if (N / P.r[s] > WAVE) return false;
-  return true;
-}
-
 // LDS plan of the staged step kernel (bytes): twiddles, then a region used
 // as [FFT buffers | two nonlinear-term tiles] while the nonlinear terms are
 // transformed (two rounds of two) and as [FFT buffers | transfer tiles] for

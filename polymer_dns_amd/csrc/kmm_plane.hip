@@ -37,15 +37,15 @@ __device__ __forceinline__ void zinv_line_load(V (&pre)[NJ], const V* __restrict
   }
 }
 
-// LDS of the z kernels: per wave an FFT buffer and its scratch (the waves'
-// results are read in place by the block's store, and k_zfwd's block load
-// writes straight into them), then the MZ twiddles.  FP64 lines longer than
-// 192 read their twiddles from global memory instead, so that two blocks fit
-// a CU (16 x 307 x 16 B = 78.6 KB at MZ = 288).
+// LDS of the z kernels: per wave one FFT buffer (the transforms run in
+// place: every stage of the z plans is one round of butterflies, so a
+// stage's LDS reads precede its writes; the waves' results are read in
+// place by the block's store, and k_zfwd's block load writes straight into
+// them), then the MZ twiddles: 43.9 KB for FP64 lines of 288, 29.3 KB at 192.
 template <int MZ, typename V>
 struct ZLds {
-  static constexpr bool tw_global = sizeof(V) == 16 && MZ > 192;
-  static constexpr size_t bytes = (size_t)(16 * FFTBuf<MZ>::len + (tw_global ? 0 : MZ)) * sizeof(V);
+  static_assert(fft_in_place<MZ, kPlanePlan>(), "z transforms run in place");
+  static constexpr size_t bytes = (size_t)(8 * FFTBuf<MZ>::len + MZ) * sizeof(V);
 };
 
 // Work items are (field, y, group of 8 kx lines), walked by a grid sized to
@@ -61,17 +61,12 @@ __device__ __forceinline__ void zinv_body(const PlaneArgs& A, int bid) {
   const int ngroups = A.nyp * ngx;
   const int nitems = NFIN * ngroups;
   constexpr int BL = FFTBuf<MZ>::len;
-  V* buf = sm + wv * 2 * BL;
-  V* scr = buf + BL;
-  const V* tw;
-  if constexpr (ZLds<MZ, V>::tw_global) {
-    tw = static_cast<const V*>(A.twZ);
-  } else {
-    V* twl = sm + 16 * BL;
-    for (int e = threadIdx.x; e < MZ; e += 512) twl[e] = static_cast<const V*>(A.twZ)[e];
-    __syncthreads();
-    tw = twl;
-  }
+  V* buf = sm + wv * BL;
+  V* scr = buf;  // in place
+  V* twl = sm + 8 * BL;
+  for (int e = threadIdx.x; e < MZ; e += 512) twl[e] = static_cast<const V*>(A.twZ)[e];
+  __syncthreads();
+  const V* tw = twl;
   int it = bid;
   if (it >= nitems) return;
   auto line = [&](int i) -> const V* {
@@ -107,14 +102,14 @@ __device__ __forceinline__ void zinv_body(const PlaneArgs& A, int bid) {
     }
     if (it + (int)gridDim.x < nitems) zinv_line_load<NJ, V>(pre, line(it + gridDim.x), A.nz, lane);
     wave_sync();
-    // every wave's result sits at the same offset (buf or scr) of its own
-    // pair of buffers: the block stores the group's z rows from there
-    const int roff = (int)(wave_fft<MZ, +1, V, kPlanePlan>(buf, scr, tw, lane) - buf);
+    // every wave's result sits in its own buffer: the block stores the
+    // group's z rows from there
+    wave_fft<MZ, +1, V, kPlanePlan>(buf, scr, tw, lane);
     __syncthreads();
     V* dst = static_cast<V*>(A.Z) + ((size_t)f * A.nyp + y) * MZ * A.nxh;
     for (int e = threadIdx.x; e < MZ * 8; e += 512) {
       const int z = e >> 3, c = e & 7;
-      dst[(size_t)z * A.nxh + kx0 + c] = sm[c * 2 * BL + roff + fpad(z)];
+      dst[(size_t)z * A.nxh + kx0 + c] = sm[c * BL + fpad(z)];
     }
     __syncthreads();
   }
@@ -437,17 +432,12 @@ __device__ __forceinline__ void zfwd_body(const PlaneArgs& A, int bid) {
   const int ngroups = A.nyp * ngx;
   const int nitems = NFOUT * ngroups;
   constexpr int BL = FFTBuf<MZ>::len;
-  V* buf = sm + wv * 2 * BL;
-  V* scr = buf + BL;
-  const V* tw;
-  if constexpr (ZLds<MZ, V>::tw_global) {
-    tw = static_cast<const V*>(A.twZ);
-  } else {
-    V* twl = sm + 16 * BL;
-    for (int e = threadIdx.x; e < MZ; e += 512) twl[e] = static_cast<const V*>(A.twZ)[e];
-    __syncthreads();
-    tw = twl;
-  }
+  V* buf = sm + wv * BL;
+  V* scr = buf;  // in place
+  V* twl = sm + 8 * BL;
+  for (int e = threadIdx.x; e < MZ; e += 512) twl[e] = static_cast<const V*>(A.twZ)[e];
+  __syncthreads();
+  const V* tw = twl;
   int it = bid;
   if (it >= nitems) return;
   const T scale = (T)(1.0 / ((double)A.mx * (double)MZ));
@@ -476,7 +466,7 @@ __device__ __forceinline__ void zfwd_body(const PlaneArgs& A, int bid) {
 #pragma unroll
     for (int q = 0; q < TL; ++q) {
       const int e = threadIdx.x + 512 * q;
-      if (e < MZ * 8) sm[(e & 7) * 2 * BL + fpad(e >> 3)] = mkv<V>(prx[q], pry[q]);
+      if (e < MZ * 8) sm[(e & 7) * BL + fpad(e >> 3)] = mkv<V>(prx[q], pry[q]);
     }
     __syncthreads();
     if (it + (int)gridDim.x < nitems) {
@@ -484,19 +474,18 @@ __device__ __forceinline__ void zfwd_body(const PlaneArgs& A, int bid) {
       const V* sp = src(it + gridDim.x, k0);
       zfwd_tile_load<MZ, TL, V, T>(prx, pry, sp, A.nxh, k0);
     }
-    V* r = wave_fft<MZ, -1, V, kPlanePlan>(buf, scr, tw, lane);
-    V* o2 = (r == buf) ? scr : buf;
-    for (int j = lane; j < nz; j += 64) {
+    const V* r = wave_fft<MZ, -1, V, kPlanePlan>(buf, scr, tw, lane);
+    // kept mode j of the line, scaled (zero at the z Nyquist mode)
+    auto mode = [&](int j) -> V {
       const int jj = (j < nzh) ? j : (MZ - nz + j);
-      o2[j] = (j == nzh) ? vzero<V>() : cscale(r[fpad(jj)], scale);
-    }
-    wave_sync();
+      return (j == nzh) ? vzero<V>() : cscale(r[fpad(jj)], scale);
+    };
     V* dst = static_cast<V*>(A.out[o]) + ((size_t)y * A.nxh + kx) * nz;
     for (int j = lane; j < nz; j += 64) {
-      V v = o2[j];
+      V v = mode(j);
       if (kx == 0) {
         if (j == 0) v.y = 0;
-        else if (j != nzh) v = cscale(cadd(o2[j], cconj(o2[nz - j])), (T)0.5);
+        else if (j != nzh) v = cscale(cadd(v, cconj(mode(nz - j))), (T)0.5);
       }
       dst[j] = v;
     }
