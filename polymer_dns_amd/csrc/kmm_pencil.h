@@ -839,37 +839,6 @@ __device__ __forceinline__ void tile_store(const E* t, E* __restrict__ F, size_t
     if (e < G::NEL && (size_t)(p0 + w) < P) F[(size_t)y * P + p0 + w] = t[y * TLD + w];
   }
 }
-// a wave's pencil (column wv) of an FP64 tile in the pair layout, and back
-template <int NY>
-__device__ __forceinline__ PFN<NY> tile_col(const double2* t, int wv, int lane) {
-  constexpr int L = PLay<NY>::L, S = PLay<NY>::S;
-  PFN<NY> f;
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int g = s * WAVE + lane;
-    if (g < L) {
-      f.a[s] = t[(2 * g) * TLD + wv];
-      f.b[s] = t[(2 * g + 1) * TLD + wv];
-    } else {
-      f.a[s] = f.b[s] = czero();
-    }
-  }
-  f.e = t[NY * TLD + wv];
-  return f;
-}
-template <int NY>
-__device__ __forceinline__ void tile_col_put(double2* t, int wv, int lane, const PFN<NY>& f) {
-  constexpr int L = PLay<NY>::L, S = PLay<NY>::S;
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int g = s * WAVE + lane;
-    if (g < L) {
-      t[(2 * g) * TLD + wv] = f.a[s];
-      t[(2 * g + 1) * TLD + wv] = f.b[s];
-    }
-  }
-  if (lane == 0) t[NY * TLD + wv] = f.e;
-}
 // physical values of column wv of a tile in the transfer precision
 template <int NY, typename V>
 __device__ __forceinline__ PhysN<NY, V> tile_phys(const V* t, int wv, int lane) {
