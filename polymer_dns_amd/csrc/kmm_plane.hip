@@ -560,7 +560,10 @@ __device__ __forceinline__ void xrow_pair_fwd(const typename Scal<V>::T (&a)[(MX
     if (x < MX) buf[fpad(x)] = mkv<V>(a[m], b[m]);
   }
   wave_sync();
-  V* r = wave_fft<MX, -1, V, kPlanePlan>(buf, scr, tw, lane);
+  // FP64: twiddles as powers of one table value per butterfly (fewer LDS loads)
+  const V none[8] = {};
+  V unused[8];
+  V* r = wave_fft_ex<MX, -1, V, kPlanePlan, false, false, false, sizeof(V) == 16>(buf, scr, tw, lane, none, unused);
   for (int k = lane; k < nxh; k += 64) {
     const V zk = r[fpad(k)], zm = r[fpad((MX - k) % MX)];
     oa[k] = mkv<V>((T)0.5 * (zk.x + zm.x), (T)0.5 * (zk.y - zm.y));
@@ -594,7 +597,8 @@ __device__ __forceinline__ void pair_inv(const PairIn<MX, V>& r, bool dxa, bool 
                                          typename Scal<V>::T (&ob)[(MX + 63) / 64]) {
   using T = typename Scal<V>::T;
   constexpr int XPL = (MX + 63) / 64;
-  for (int x = lane; x < MX; x += 64) buf[fpad(x)] = vzero<V>();
+  // zeros at the dealiasing band [nxh, MX - nxh]; the packed spectra fill the rest
+  for (int x = nxh + lane; x <= MX - nxh; x += 64) buf[fpad(x)] = vzero<V>();
   wave_sync();
 #pragma unroll
   for (int j = 0; j < PairIn<MX, V>::KPL; ++j) {
@@ -613,7 +617,9 @@ __device__ __forceinline__ void pair_inv(const PairIn<MX, V>& r, bool dxa, bool 
     }
   }
   wave_sync();
-  V* res = wave_fft<MX, +1, V, kPlanePlan>(buf, scr, tw, lane);
+  const V none[8] = {};
+  V unused[8];
+  V* res = wave_fft_ex<MX, +1, V, kPlanePlan, false, false, false, sizeof(V) == 16>(buf, scr, tw, lane, none, unused);
 #pragma unroll
   for (int m = 0; m < XPL; ++m) {
     const int x = lane + 64 * m;
