@@ -83,7 +83,8 @@ __device__ __forceinline__ void zinv_body(const PlaneArgs& A, int bid) {
     const int f = it / ngroups, g = it - f * ngroups;
     const int y = g / ngx, kx0 = (g - y * ngx) * 8;
     const int kx = kx0 + wv;
-    for (int z = lane; z < MZ; z += 64) buf[fpad(z)] = vzero<V>();
+    // zeros at the padding band [nzh, MZ - nzh); the line's modes fill the rest
+    for (int z = A.nzh + lane; z < MZ - A.nzh; z += 64) buf[fpad(z)] = vzero<V>();
     wave_sync();
     const bool dz = (A.dzmask >> f) & 1u;  // d/dz = multiplication by i kz
 #pragma unroll
