@@ -959,7 +959,7 @@ __device__ __forceinline__ PFN<NY> smooth_apply(const PFN<NY>& f, const double* 
 
 static inline int nblk(int P) { return (P + PW - 1) / PW; }
 template <int NY, typename V>
-static inline size_t lds_bytes() { return (size_t)(PW * 2 * FFTBuf<2 * NY>::len + 2 * NY) * sizeof(V); }
+static inline size_t lds_bytes(int waves = PW) { return (size_t)(waves * 2 * FFTBuf<2 * NY>::len + 2 * NY) * sizeof(V); }
 // the step kernel's launch: the staged plan (PWS pencils per block) where
 // it fits, else PW pencils per block with their FFT buffers
 template <int NY, typename V, bool SCALAR, int MODE>

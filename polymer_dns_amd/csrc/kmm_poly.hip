@@ -34,13 +34,19 @@
 
 namespace kmm {
 
+// Pencils (wavefronts) per workgroup. At ny = 192 the two-segment solve needs
+// more than 256 registers a lane: a 4-wave workgroup lets the compiler hold
+// them in the unified VGPR/AGPR file at one wave per SIMD instead of spilling
+// about 300 bytes a lane to scratch at two (4.5 % faster in FP64).
+template <int NY> constexpr int poly_pw() { return NY > 128 ? 4 : PW; }
+
 template <int NY, int MODE, typename V>
-__global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
+__global__ __launch_bounds__(poly_pw<NY>() * 64) void k_poly_pencil(PolyArgs A) {
   using PF = PFN<NY>;
   extern __shared__ __align__(16) unsigned char smraw[];
   V* smem = reinterpret_cast<V*>(smraw);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int p = blockIdx.x * PW + wv;
+  const int p = blockIdx.x * poly_pw<NY>() + wv;
   const V* twl = lds_twiddles<NY, V>(smem, static_cast<const V*>(A.twM));
   if (p >= A.P) return;
   V* buf = wave_lds<NY, V>(smem, wv);
@@ -206,8 +212,9 @@ __global__ __launch_bounds__(PW * 64) void k_poly_pencil(PolyArgs A) {
 
 template <int MODE, typename V>
 static hipError_t launch_poly_v(int ny, const PolyArgs& a, hipStream_t s) {
-  KMM_NY_DISPATCH(ny, hipLaunchKernelGGL((k_poly_pencil<NY, MODE, V>), dim3(nblk(a.P)), dim3(PW * 64),
-                                         (lds_bytes<NY, V>()), s, a));
+  KMM_NY_DISPATCH(ny, constexpr int W = poly_pw<NY>();
+                  hipLaunchKernelGGL((k_poly_pencil<NY, MODE, V>), dim3((a.P + W - 1) / W), dim3(W * 64),
+                                     (lds_bytes<NY, V>(W)), s, a));
   return hipGetLastError();
 }
 
