@@ -782,9 +782,11 @@ struct PolyRow {
 };
 
 // Rows up to MX = 288 run two waves per SIMD (256 registers: FP64 spills 3
-// values at MX = 288, FP32 none); FP64 rows of MX = 384 keep one.
+// values at MX = 288, FP32 none). At MX = 384, FP64 also runs two (92 bytes a
+// lane spilled: 22 % faster than one wave per SIMD holding 280 registers) and
+// FP32 three (36 bytes a lane spilled; 7 % faster than two).
 template <int MX, typename V>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX <= 288 ? 2 : 1)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX <= 288 ? 2 : (sizeof(V) == 8 ? 3 : 2))))
 void k_xplane_poly(PolyPlaneArgs A) {
   using T = typename Scal<V>::T;
   constexpr int WPB = 4;
