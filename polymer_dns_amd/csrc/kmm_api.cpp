@@ -881,6 +881,12 @@ static kmm::PolyArgs poly_args(kmm_ctx* c) {
   return a;
 }
 
+// From this ny up, a FENE-P step runs its polymer pencil work as two passes
+// (part B, then C init / solve / evaluate) instead of one fused pass.
+#ifndef KMM_POLY_SPLIT_MIN_NY
+#define KMM_POLY_SPLIT_MIN_NY 192
+#endif
+
 static int run_poly_pencil(kmm_ctx* c, int mode, const kmm::PolyArgs& a) {
   hipError_t e = kmm::launch_poly_pencil(c->ny, mode, a, c->cfg.precision == 1, c->stream);
   if (e != hipSuccess) return fail(KMM_E_HIP, std::string("polymer pencil: ") + hipGetErrorString(e));
@@ -1376,7 +1382,16 @@ static int enqueue_step(kmm_ctx* c, int s, hipEvent_t* ev, bool stage_ev) {
     if (!clo) { pa.init = pf.init; pa.solve = pf.solve; pa.evaluate = pf.evaluate; }
     // a closure step finishing a closure evaluation only forms the force
     const bool force_only = clo && pa.partB && pa.closure && pa.force && !pa.copy;
-    int r = run_poly_pencil(c, force_only ? 4 : 0, pa);
+    int r;
+    if (!clo && !pa.closure && c->ny >= KMM_POLY_SPLIT_MIN_NY) {
+      // part B, then the step without it: each pass holds fewer registers
+      if (pa.partB && (r = run_poly_pencil(c, 2, pa))) return r;
+      kmm::PolyArgs pb = pa;
+      pb.partB = 0;
+      r = run_poly_pencil(c, 5, pb);
+    } else {
+      r = run_poly_pencil(c, force_only ? 4 : 0, pa);
+    }
     if (r) return r;
     if (pa.force) for (int q = 0; q < 3; ++q) a.TF[q] = c->TF[q];
   }

@@ -28,7 +28,10 @@
 // MODE 3: C from the physical-y fields in NLP[0..5] (a conformation tensor
 // rebuilt on the 3/2 grid and forward transformed in x and z), MODE 4: part B
 // of a learned-closure stress (the force only: a closure stress is neither
-// smoothed nor stored, and there is no conformation to transport).
+// smoothed nor stored, and there is no conformation to transport), MODE 5:
+// the step after a separate MODE 2 pass (C init, solve and evaluate, reading
+// the smoothed c_n and str_n that pass stored; at ny = 192 the two passes
+// hold fewer registers than the fused step).
 #include "kmm_kernels.h"
 #include "kmm_pencil.h"
 
@@ -43,6 +46,7 @@ template <int NY> constexpr int poly_pw() { return NY > 128 ? 4 : PW; }
 template <int NY, int MODE, typename V>
 __global__ __launch_bounds__(poly_pw<NY>() * 64) void k_poly_pencil(PolyArgs A) {
   using PF = PFN<NY>;
+  constexpr bool M0 = (MODE == 0 || MODE == 5);  // mode 5: mode 0 without part B
   extern __shared__ __align__(16) unsigned char smraw[];
   V* smem = reinterpret_cast<V*>(smraw);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -102,14 +106,14 @@ __global__ __launch_bounds__(poly_pw<NY>() * 64) void k_poly_pencil(PolyArgs A) 
   if (kzi == A.nzh) {  // norm zeroes the z-Nyquist row (helpers.f90:149-156)
     const PF z = pf_zero<NY>();
     for (int c = 0; c < 6; ++c) {
-      if (MODE == 0 || MODE == 2) {
+      if (M0 || MODE == 2) {
         if (A.partB || MODE == 2) {
           if (!A.closure) pf_store_pm<NY>(A.CN[c], p, lane, z);
           if (!A.closure || MODE == 2) pf_store_pm<NY>(A.STRN[c], p, lane, z);
         }
         if (A.copy || A.solve) { pf_store_pm<NY>(A.CNM1[c], p, lane, z); pf_store_pm<NY>(A.STRNM1[c], p, lane, z); }
       }
-      if (MODE == 0 && (A.init || A.solve)) pf_store_pm<NY>(A.C[c], p, lane, z);
+      if (M0 && (A.init || A.solve)) pf_store_pm<NY>(A.C[c], p, lane, z);
       if (MODE != 2 && A.evaluate)
         for (int h = 0; h < 2; ++h) {
           pf_store_zero<NY, V>(static_cast<V*>(A.TP[c + 6 * h]), P, p, lane);
@@ -163,7 +167,7 @@ __global__ __launch_bounds__(poly_pw<NY>() * 64) void k_poly_pencil(PolyArgs A) 
       if (MODE == 2) continue;
     }
     PF C;
-    if (MODE == 0) {
+    if (M0) {
       if (A.init) {
         C = pf_zero<NY>();
         if (is00 && (c == 0 || c == 3 || c == 5)) {
@@ -225,6 +229,7 @@ hipError_t launch_poly_pencil(int ny, int mode, const PolyArgs& a, bool mixed, h
     case 2: return mixed ? launch_poly_v<2, float2>(ny, a, s) : launch_poly_v<2, double2>(ny, a, s);
     case 3: return mixed ? launch_poly_v<3, float2>(ny, a, s) : launch_poly_v<3, double2>(ny, a, s);
     case 4: return mixed ? launch_poly_v<4, float2>(ny, a, s) : launch_poly_v<4, double2>(ny, a, s);
+    case 5: return mixed ? launch_poly_v<5, float2>(ny, a, s) : launch_poly_v<5, double2>(ny, a, s);
     default: return hipErrorInvalidValue;
   }
 }
